@@ -1,0 +1,512 @@
+// C ABI of the dfp projection library: plans, entry points, profiling, errors.
+//
+// Plan set-up restates the reference's operator construction
+// (initialization.f90:416-434 kxx/kzz, :506-535 Dyy) so that the library
+// needs only the grid; the spectral solve keeps the x modes kx = 0..nxp/2 of
+// the real-to-complex transform, as the reference's GPU build does
+// (poisson_gpu.f90:92-97).
+#include <cmath>
+#include <cstdarg>
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "dfp_internal.h"
+
+using namespace dfp;
+
+namespace {
+
+int fail(dfp_plan* p, int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+int fail(dfp_plan* p, int code, const char* fmt, ...) {
+    if (p) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        p->last_error = buf;
+    }
+    return code;
+}
+
+#define HIPCHK(p, expr)                                                                                 \
+    do {                                                                                                \
+        hipError_t e_ = (expr);                                                                         \
+        if (e_ != hipSuccess) return fail((p), DFP_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));      \
+    } while (0)
+
+// radices 4, 2, 3 for n = 2^a 3^b; false if another prime divides n
+bool factor(int n, FftDesc& d) {
+    d = FftDesc{};
+    d.n = n;
+    int m = n, nst = 0;
+    while (m % 4 == 0) { if (nst >= kMaxStages) return false; d.rad[nst++] = 4; m /= 4; }
+    while (m % 2 == 0) { if (nst >= kMaxStages) return false; d.rad[nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { if (nst >= kMaxStages) return false; d.rad[nst++] = 3; m /= 3; }
+    d.nst = nst;
+    return m == 1;
+}
+
+// twiddles exp(-2 pi i t / n) from the exact index, rounded from long double
+double2 expi(long t, long n) {
+    long tt = t % n;
+    if (tt < 0) tt += n;
+    const long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)tt / (long double)n;
+    return make_double2((double)cosl(a), (double)(-sinl(a)));
+}
+
+void stage_twiddles(FftDesc& d, std::vector<double2>& tw) {
+    int ncur = d.n;
+    for (int st = 0; st < d.nst; ++st) {
+        const int r = d.rad[st], m = ncur / r;
+        d.twoff[st] = (int)tw.size();
+        for (int p = 0; p < m; ++p)
+            for (int u = 1; u < r; ++u) tw.push_back(expi((long)p * u, ncur));
+        ncur = m;
+    }
+}
+
+// initialization.f90:420-425 (and :428-433 for z)
+void wavenumbers(int n, double d, std::vector<double>& k) {
+    const double pi = 4.0 * std::atan(1.0);
+    k.assign(n, 0.0);
+    const int half = (int)std::ceil((double)(float)n / 2.0);
+    for (int i = 0; i <= half && i <= n - 1; ++i) k[i] = 2.0 * (std::cos(2.0 * pi * (double)i / (double)n) - 1.0) / std::pow(d, 2.0);
+    for (int i = half + 1; i <= n - 1; ++i)
+        k[i] = 2.0 * (std::cos(2.0 * pi * (double)(-n + i) / (double)n) - 1.0) / std::pow(d, 2.0);
+}
+
+// initialization.f90:506-535, rows j = 2..nyg-1 stored at j-2
+void dyy(const std::vector<double>& y, const std::vector<double>& yg, std::vector<double>& dl, std::vector<double>& dd,
+         std::vector<double>& du) {
+    const int nyg = (int)yg.size(), nym = nyg - 2;
+    dl.assign(nym > 1 ? nym - 1 : 1, 0.0);
+    du.assign(nym > 1 ? nym - 1 : 1, 0.0);
+    dd.assign(nym, 0.0);
+    auto Y = [&](int j) { return y[j - 1]; };
+    auto G = [&](int j) { return yg[j - 1]; };
+    for (int j = 2; j <= nyg - 1; ++j) {
+        const double a = 1.0 / (Y(j) - Y(j - 1)) / (G(j + 1) - G(j));
+        const double b = 1.0 / (Y(j) - Y(j - 1)) * (-1.0 / (G(j + 1) - G(j)) - 1.0 / (G(j) - G(j - 1)));
+        const double c = 1.0 / (Y(j) - Y(j - 1)) / (G(j) - G(j - 1));
+        const int jj = j - 2;
+        if (j == 2 && j == nyg - 1) {
+            dd[jj] = b + c + a;
+        } else if (j == 2) {
+            dd[jj] = b + c;
+            du[jj] = a;
+        } else if (j == nyg - 1) {
+            dd[jj] = a + b;
+            dl[jj - 1] = c;
+        } else {
+            du[jj] = a;
+            dl[jj - 1] = c;
+            dd[jj] = b;
+        }
+    }
+}
+
+bool finite_all(const double* a, int n) {
+    for (int i = 0; i < n; ++i)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+
+void free_plan_memory(dfp_plan* p) {
+    (void)hipFree(p->d_tables);
+    (void)hipFree(p->d_tw);
+    (void)hipFree(p->d_spec);
+    (void)hipFree(p->d_piv);
+    (void)hipFree(p->d_red);
+    (void)hipFree(p->d_U);
+    (void)hipFree(p->d_V);
+    (void)hipFree(p->d_W);
+    (void)hipFree(p->d_rhs);
+    (void)hipFree(p->d_P);
+    for (auto& pe : p->pending) {
+        (void)hipEventDestroy(pe.second.first);
+        (void)hipEventDestroy(pe.second.second);
+    }
+    for (auto ev : p->event_pool) (void)hipEventDestroy(ev);
+}
+
+int ensure_staging(dfp_plan* p) {
+    if (p->d_U) return DFP_OK;
+    HIPCHK(p, hipMalloc(&p->d_U, sizeof(double) * p->sz_u));
+    HIPCHK(p, hipMalloc(&p->d_V, sizeof(double) * p->sz_v));
+    HIPCHK(p, hipMalloc(&p->d_W, sizeof(double) * p->sz_w));
+    HIPCHK(p, hipMalloc(&p->d_rhs, sizeof(double) * p->sz_rhs));
+    HIPCHK(p, hipMalloc(&p->d_P, sizeof(double) * p->sz_p));
+    HIPCHK(p, hipMemset(p->d_rhs, 0, sizeof(double) * p->sz_rhs));
+    return DFP_OK;
+}
+
+int h2d(dfp_plan* p, double* dst, const double* src, int64_t n) {
+    HIPCHK(p, hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyHostToDevice, p->stream));
+    return DFP_OK;
+}
+
+int d2h(dfp_plan* p, double* dst, const double* src, int64_t n) {
+    HIPCHK(p, hipMemcpyAsync(dst, src, sizeof(double) * n, hipMemcpyDeviceToHost, p->stream));
+    return DFP_OK;
+}
+
+int finish(dfp_plan* p, int flags) {
+    HIPCHK(p, hipGetLastError());
+    if (!(flags & DFP_ASYNC) || !(flags & DFP_FIELDS_ON_DEVICE)) HIPCHK(p, hipStreamSynchronize(p->stream));
+    return DFP_OK;
+}
+
+int check_flags(dfp_plan* p, int flags) {
+    if (flags & ~(DFP_FIELDS_ON_DEVICE | DFP_SAVE_P | DFP_ASYNC)) return fail(p, DFP_E_ARG, "unknown flag bits 0x%x", flags);
+    HIPCHK(p, hipSetDevice(p->device));
+    return DFP_OK;
+}
+
+// the spectral chain on device arrays: rhs (read unless from_fields) -> phi in rhs (+P)
+int spectral_solve(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs, double* P, double dtc,
+                   int save_p, int from_fields) {
+    HIPCHK(p, launch_div_rfft_x(p, U, V, W, rhs, from_fields ? 0 : 1));
+    HIPCHK(p, launch_fft_z(p, -1));
+    HIPCHK(p, launch_thomas_y(p));
+    HIPCHK(p, launch_fft_z(p, +1));
+    HIPCHK(p, launch_irfft_x(p, rhs, P, dtc, save_p));
+    return DFP_OK;
+}
+
+}  // namespace
+
+namespace dfp {
+
+int prof_begin(dfp_plan* p, const char* name) {
+    if (!p->prof) return -1;
+    int slot = -1;
+    for (size_t s = 0; s < p->slots.size(); ++s)
+        if (p->slots[s].name == name) slot = (int)s;
+    if (slot < 0) {
+        p->slots.push_back(ProfSlot{name, 0, 0.0});
+        slot = (int)p->slots.size() - 1;
+    }
+    hipEvent_t a, b;
+    if (p->event_pool.size() >= 2) {
+        a = p->event_pool.back(); p->event_pool.pop_back();
+        b = p->event_pool.back(); p->event_pool.pop_back();
+    } else {
+        if (hipEventCreate(&a) != hipSuccess) return -1;
+        if (hipEventCreate(&b) != hipSuccess) { (void)hipEventDestroy(a); return -1; }
+    }
+    (void)hipEventRecord(a, p->stream);
+    p->pending.push_back({slot, {a, b}});
+    return (int)p->pending.size() - 1;
+}
+
+void prof_end(dfp_plan* p, int idx) {
+    if (idx < 0 || idx >= (int)p->pending.size()) return;
+    (void)hipEventRecord(p->pending[idx].second.second, p->stream);
+}
+
+}  // namespace dfp
+
+static int prof_collect(dfp_plan* p) {
+    for (auto& pe : p->pending) {
+        HIPCHK(p, hipEventSynchronize(pe.second.second));
+        float ms = 0.f;
+        HIPCHK(p, hipEventElapsedTime(&ms, pe.second.first, pe.second.second));
+        p->slots[pe.first].launches += 1;
+        p->slots[pe.first].ms += ms;
+        p->event_pool.push_back(pe.second.first);
+        p->event_pool.push_back(pe.second.second);
+    }
+    p->pending.clear();
+    return DFP_OK;
+}
+
+extern "C" {
+
+int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out) {
+    if (!out) return DFP_E_ARG;
+    *out = nullptr;
+    if (!g || !g->y || !g->z || !g->yg || !g->zg) return DFP_E_ARG;
+    if (g->x_bc_type != 0 || g->y_bc_type != 1 || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
+    if (precision != DFP_FP64) return DFP_E_UNSUPPORTED;
+    if (g->nxp < 2 || (g->nxp & 1) || g->nym < 1 || g->nzp < 1) return DFP_E_UNSUPPORTED;
+    if (g->nxp > 8192 || g->nzp > 4096 || g->nym > 65535) return DFP_E_UNSUPPORTED;
+    if (!(g->dx > 0.0) || !(g->dz > 0.0)) return DFP_E_ARG;
+    dfp_plan* p = new (std::nothrow) dfp_plan();
+    if (!p) return DFP_E_NOMEM;
+    auto bail = [&](int code) {
+        free_plan_memory(p);
+        delete p;
+        return code;
+    };
+    p->device = device;
+    p->precision = precision;
+    if (!factor(g->nxp / 2, p->fx) || !factor(g->nzp, p->fz)) return bail(DFP_E_UNSUPPORTED);
+    if (hipSetDevice(device) != hipSuccess) return bail(DFP_E_HIP);
+    Geo& G = p->geo;
+    G.nxp = g->nxp; G.nym = g->nym; G.nzp = g->nzp; G.nxh = g->nxp / 2 + 1;
+    G.nx = g->nxp + 2; G.ny = g->nym + 1; G.nz = g->nzp + 2;
+    G.nxg = g->nxp + 3; G.nyg = g->nym + 2; G.nzg = g->nzp + 3;
+    p->dx = g->dx; p->dz = g->dz;
+    G.inv_dx = 1.0 / g->dx;
+    if (g->x) p->hx.assign(g->x, g->x + G.nx);
+    p->hy.assign(g->y, g->y + G.ny);
+    p->hz.assign(g->z, g->z + G.nz);
+    p->hyg.assign(g->yg, g->yg + G.nyg);
+    p->hzg.assign(g->zg, g->zg + G.nzg);
+    if ((!p->hx.empty() && !finite_all(p->hx.data(), G.nx)) || !finite_all(p->hy.data(), G.ny) ||
+        !finite_all(p->hz.data(), G.nz) || !finite_all(p->hyg.data(), G.nyg) || !finite_all(p->hzg.data(), G.nzg))
+        return bail(DFP_E_NONFINITE);
+
+    // 1-D tables
+    std::vector<double> inv_dyf(G.nym), inv_dzf(G.nzp + 1), inv_dyg(G.nym > 1 ? G.nym - 1 : 1, 0.0), inv_dzg(G.nzp);
+    for (int j = 2; j <= G.nyg - 1; ++j) inv_dyf[j - 2] = 1.0 / (p->hy[j - 1] - p->hy[j - 2]);
+    for (int k = 2; k <= G.nzg - 1; ++k) inv_dzf[k - 2] = 1.0 / (p->hz[k - 1] - p->hz[k - 2]);
+    for (int j = 2; j <= G.ny - 1; ++j) inv_dyg[j - 2] = 1.0 / (p->hyg[j] - p->hyg[j - 1]);
+    for (int k = 2; k <= G.nz - 1; ++k) inv_dzg[k - 2] = 1.0 / (p->hzg[k] - p->hzg[k - 1]);
+    wavenumbers(G.nxp, g->dx, p->hkxx);
+    wavenumbers(G.nzp, g->dz, p->hkzz);
+    dyy(p->hy, p->hyg, p->hdl, p->hdd, p->hdu);
+
+    std::vector<double> tab;
+    auto put = [&](const std::vector<double>& v) {
+        size_t off = tab.size();
+        tab.insert(tab.end(), v.begin(), v.end());
+        tab.resize((tab.size() + 7) & ~size_t(7), 0.0);
+        return off;
+    };
+    std::vector<double> kxx_half(p->hkxx.begin(), p->hkxx.begin() + G.nxh);
+    const size_t o_dyf = put(inv_dyf), o_dzf = put(inv_dzf), o_dyg = put(inv_dyg), o_dzg = put(inv_dzg);
+    const size_t o_kxx = put(kxx_half), o_kzz = put(p->hkzz);
+    const size_t o_dl = put(p->hdl), o_dd = put(p->hdd), o_du = put(p->hdu);
+    const size_t o_x = p->hx.empty() ? (size_t)-1 : put(p->hx);
+    const size_t o_y = put(p->hy), o_z = put(p->hz), o_yg = put(p->hyg);
+    if (hipMalloc(&p->d_tables, sizeof(double) * tab.size()) != hipSuccess) return bail(DFP_E_NOMEM);
+    if (hipMemcpy(p->d_tables, tab.data(), sizeof(double) * tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(DFP_E_HIP);
+    double* T = p->d_tables;
+    G.inv_dyf = T + o_dyf; G.inv_dzf = T + o_dzf; G.inv_dyg = T + o_dyg; G.inv_dzg = T + o_dzg;
+    G.x = o_x == (size_t)-1 ? nullptr : T + o_x;
+    G.y = T + o_y; G.z = T + o_z; G.yg = T + o_yg;
+    p->ops.kxx = T + o_kxx; p->ops.kzz = T + o_kzz;
+    p->ops.dl = T + o_dl; p->ops.dd = T + o_dd; p->ops.du = T + o_du;
+
+    // twiddles: x stages, z stages, R2C post twiddles
+    std::vector<double2> tw;
+    stage_twiddles(p->fx, tw);
+    const size_t o_tz = tw.size();
+    stage_twiddles(p->fz, tw);
+    const size_t o_r2c = tw.size();
+    for (int k = 0; k <= G.nxp / 2; ++k) tw.push_back(expi(k, G.nxp));
+    for (int st = 0; st < p->fz.nst; ++st) p->fz.twoff[st] -= (int)o_tz;
+    if (hipMalloc(&p->d_tw, sizeof(double2) * tw.size()) != hipSuccess) return bail(DFP_E_NOMEM);
+    if (hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
+        return bail(DFP_E_HIP);
+    p->tw_x = p->d_tw;
+    p->tw_z = p->d_tw + o_tz;
+    p->tw_r2c = p->d_tw + o_r2c;
+
+    const size_t H = (size_t)G.nxh * G.nym * G.nzp;
+    if (hipMalloc(&p->d_spec, sizeof(double2) * H) != hipSuccess) return bail(DFP_E_NOMEM);
+    if (hipMalloc(&p->d_piv, sizeof(double) * H) != hipSuccess) return bail(DFP_E_NOMEM);
+    p->red_len = (int)((long)ceil_div(G.nxg - 2, 256) * (G.nyg - 2) * (G.nzg - 2));
+    if (hipMalloc(&p->d_red, sizeof(double) * (p->red_len + 8)) != hipSuccess) return bail(DFP_E_NOMEM);
+
+    p->sz_u = (int64_t)G.nx * G.nyg * G.nzg;
+    p->sz_v = (int64_t)G.nxg * G.ny * G.nzg;
+    p->sz_w = (int64_t)G.nxg * G.nyg * G.nz;
+    p->sz_p = (int64_t)G.nxg * G.nyg * G.nzg;
+    p->sz_rhs = (int64_t)(G.nxg - 1) * G.nym * (G.nzg - 1);
+
+    // launch shapes: about 1024 complex points per block in the LDS transforms
+    // (two LDS buffers of 16 KiB each at that size)
+    const int M = G.nxp / 2;
+    p->tk_x = std::max(1, std::min(64, 1024 / std::max(M, 1)));
+    p->rb_z = std::max(1, std::min(64, 1024 / std::max(G.nzp, 1)));
+    *out = p;
+    return DFP_OK;
+}
+
+int dfp_plan_destroy(dfp_plan* plan) {
+    if (!plan) return DFP_E_ARG;
+    (void)hipSetDevice(plan->device);
+    (void)hipStreamSynchronize(plan->stream);
+    free_plan_memory(plan);
+    delete plan;
+    return DFP_OK;
+}
+
+int dfp_set_stream(dfp_plan* p, void* s) {
+    if (!p) return DFP_E_ARG;
+    p->stream = reinterpret_cast<hipStream_t>(s);
+    return DFP_OK;
+}
+
+int dfp_field_sizes(const dfp_plan* p, int64_t sizes[5]) {
+    if (!p || !sizes) return DFP_E_ARG;
+    sizes[0] = p->sz_u; sizes[1] = p->sz_v; sizes[2] = p->sz_w; sizes[3] = p->sz_p; sizes[4] = p->sz_rhs;
+    return DFP_OK;
+}
+
+int dfp_project(dfp_plan* p, double* U, double* V, double* W, double* rhs_p, double* P, double dt, double c,
+                int flags) {
+    if (!p) return DFP_E_ARG;
+    int rc = check_flags(p, flags);
+    if (rc) return rc;
+    const int save_p = (flags & DFP_SAVE_P) ? 1 : 0;
+    if (!U || !V || !W) return fail(p, DFP_E_ARG, "U, V and W are required");
+    if (save_p && (!P || !(dt * c != 0.0) || !std::isfinite(dt * c)))
+        return fail(p, DFP_E_ARG, "DFP_SAVE_P needs P and a finite nonzero dt*c");
+    if (flags & DFP_FIELDS_ON_DEVICE) {
+        if (!rhs_p) return fail(p, DFP_E_ARG, "rhs_p is required with DFP_FIELDS_ON_DEVICE");
+        rc = spectral_solve(p, U, V, W, rhs_p, P, dt * c, save_p, 1);
+        if (rc) return rc;
+        HIPCHK(p, launch_correct(p, U, V, W, rhs_p));
+        return finish(p, flags);
+    }
+    if ((rc = ensure_staging(p))) return rc;
+    if ((rc = h2d(p, p->d_U, U, p->sz_u)) || (rc = h2d(p, p->d_V, V, p->sz_v)) || (rc = h2d(p, p->d_W, W, p->sz_w)))
+        return rc;
+    if (rhs_p && (rc = h2d(p, p->d_rhs, rhs_p, p->sz_rhs))) return rc;
+    rc = spectral_solve(p, p->d_U, p->d_V, p->d_W, p->d_rhs, p->d_P, dt * c, save_p, 1);
+    if (rc) return rc;
+    HIPCHK(p, launch_correct(p, p->d_U, p->d_V, p->d_W, p->d_rhs));
+    if ((rc = d2h(p, U, p->d_U, p->sz_u)) || (rc = d2h(p, V, p->d_V, p->sz_v)) || (rc = d2h(p, W, p->d_W, p->sz_w)))
+        return rc;
+    if (rhs_p && (rc = d2h(p, rhs_p, p->d_rhs, p->sz_rhs))) return rc;
+    if (save_p && (rc = d2h(p, P, p->d_P, p->sz_p))) return rc;
+    return finish(p, flags);
+}
+
+int dfp_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs_p, int flags) {
+    if (!p) return DFP_E_ARG;
+    int rc = check_flags(p, flags);
+    if (rc) return rc;
+    if (!U || !V || !W || !rhs_p) return fail(p, DFP_E_ARG, "U, V, W and rhs_p are required");
+    if (flags & DFP_FIELDS_ON_DEVICE) {
+        HIPCHK(p, launch_divergence(p, U, V, W, rhs_p));
+        return finish(p, flags);
+    }
+    if ((rc = ensure_staging(p))) return rc;
+    if ((rc = h2d(p, p->d_U, U, p->sz_u)) || (rc = h2d(p, p->d_V, V, p->sz_v)) || (rc = h2d(p, p->d_W, W, p->sz_w)) ||
+        (rc = h2d(p, p->d_rhs, rhs_p, p->sz_rhs)))
+        return rc;
+    HIPCHK(p, launch_divergence(p, p->d_U, p->d_V, p->d_W, p->d_rhs));
+    if ((rc = d2h(p, rhs_p, p->d_rhs, p->sz_rhs))) return rc;
+    return finish(p, flags);
+}
+
+int dfp_solve(dfp_plan* p, double* rhs_p, double* P, double dt, double c, int flags) {
+    if (!p) return DFP_E_ARG;
+    int rc = check_flags(p, flags);
+    if (rc) return rc;
+    const int save_p = (flags & DFP_SAVE_P) ? 1 : 0;
+    if (!rhs_p) return fail(p, DFP_E_ARG, "rhs_p is required");
+    if (save_p && (!P || !(dt * c != 0.0) || !std::isfinite(dt * c)))
+        return fail(p, DFP_E_ARG, "DFP_SAVE_P needs P and a finite nonzero dt*c");
+    if (flags & DFP_FIELDS_ON_DEVICE) {
+        rc = spectral_solve(p, nullptr, nullptr, nullptr, rhs_p, P, dt * c, save_p, 0);
+        if (rc) return rc;
+        return finish(p, flags);
+    }
+    if ((rc = ensure_staging(p))) return rc;
+    if ((rc = h2d(p, p->d_rhs, rhs_p, p->sz_rhs))) return rc;
+    rc = spectral_solve(p, nullptr, nullptr, nullptr, p->d_rhs, p->d_P, dt * c, save_p, 0);
+    if (rc) return rc;
+    if ((rc = d2h(p, rhs_p, p->d_rhs, p->sz_rhs))) return rc;
+    if (save_p && (rc = d2h(p, P, p->d_P, p->sz_p))) return rc;
+    return finish(p, flags);
+}
+
+int dfp_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs_p, int flags) {
+    if (!p) return DFP_E_ARG;
+    int rc = check_flags(p, flags);
+    if (rc) return rc;
+    if (!U || !V || !W || !rhs_p) return fail(p, DFP_E_ARG, "U, V, W and rhs_p are required");
+    if (flags & DFP_FIELDS_ON_DEVICE) {
+        HIPCHK(p, launch_correct(p, U, V, W, rhs_p));
+        return finish(p, flags);
+    }
+    if ((rc = ensure_staging(p))) return rc;
+    if ((rc = h2d(p, p->d_U, U, p->sz_u)) || (rc = h2d(p, p->d_V, V, p->sz_v)) || (rc = h2d(p, p->d_W, W, p->sz_w)) ||
+        (rc = h2d(p, p->d_rhs, rhs_p, p->sz_rhs)))
+        return rc;
+    HIPCHK(p, launch_correct(p, p->d_U, p->d_V, p->d_W, p->d_rhs));
+    if ((rc = d2h(p, U, p->d_U, p->sz_u)) || (rc = d2h(p, V, p->d_V, p->sz_v)) || (rc = d2h(p, W, p->d_W, p->sz_w)))
+        return rc;
+    return finish(p, flags);
+}
+
+int dfp_max_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* out, int flags) {
+    if (!p) return DFP_E_ARG;
+    int rc = check_flags(p, flags & ~DFP_ASYNC);
+    if (rc) return rc;
+    if (!U || !V || !W || !out) return fail(p, DFP_E_ARG, "U, V, W and out are required");
+    const double *dU = U, *dV = V, *dW = W;
+    if (!(flags & DFP_FIELDS_ON_DEVICE)) {
+        if ((rc = ensure_staging(p))) return rc;
+        if ((rc = h2d(p, p->d_U, U, p->sz_u)) || (rc = h2d(p, p->d_V, V, p->sz_v)) || (rc = h2d(p, p->d_W, W, p->sz_w)))
+            return rc;
+        dU = p->d_U; dV = p->d_V; dW = p->d_W;
+    }
+    double* d_out = p->d_red + p->red_len;
+    HIPCHK(p, launch_max_div(p, dU, dV, dW, d_out));
+    HIPCHK(p, hipMemcpyAsync(out, d_out, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(p, hipStreamSynchronize(p->stream));
+    return DFP_OK;
+}
+
+int dfp_synth_fields(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma) {
+    if (!p || !U || !V || !W) return DFP_E_ARG;
+    HIPCHK(p, hipSetDevice(p->device));
+    HIPCHK(p, launch_synth(p, U, V, W, seed, sigma));
+    HIPCHK(p, hipStreamSynchronize(p->stream));
+    return DFP_OK;
+}
+
+int dfp_profile_enable(dfp_plan* p, int enable) {
+    if (!p) return DFP_E_ARG;
+    if (!enable) {
+        int rc = prof_collect(p);
+        if (rc) return rc;
+    }
+    p->prof = enable != 0;
+    return DFP_OK;
+}
+
+int dfp_profile_reset(dfp_plan* p) {
+    if (!p) return DFP_E_ARG;
+    int rc = prof_collect(p);
+    if (rc) return rc;
+    p->slots.clear();
+    return DFP_OK;
+}
+
+int dfp_profile_read(dfp_plan* p, int index, const char** name, int64_t* launches, double* ms) {
+    if (!p) return DFP_E_ARG;
+    int rc = prof_collect(p);
+    if (rc) return rc;
+    if (index < 0 || index >= (int)p->slots.size()) return DFP_E_ARG;
+    if (name) *name = p->slots[index].name.c_str();
+    if (launches) *launches = p->slots[index].launches;
+    if (ms) *ms = p->slots[index].ms;
+    return DFP_OK;
+}
+
+const char* dfp_strerror(int code) {
+    switch (code) {
+        case DFP_OK: return "success";
+        case DFP_E_ARG: return "invalid argument";
+        case DFP_E_UNSUPPORTED: return "unsupported configuration";
+        case DFP_E_HIP: return "HIP runtime error";
+        case DFP_E_NOMEM: return "device allocation failed";
+        case DFP_E_NONFINITE: return "non-finite value";
+        default: return "unknown error";
+    }
+}
+
+const char* dfp_last_error(const dfp_plan* p) { return p ? p->last_error.c_str() : ""; }
+
+}  // extern "C"

@@ -1,0 +1,119 @@
+// Internal declarations shared by the dfp HIP translation units.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/dfp.h"
+
+namespace dfp {
+
+constexpr int kMaxStages = 16;
+
+// Stockham plan of one FFT length n = 2^a 3^b on the device.
+// Stage st has radix rad[st]; its twiddles exp(-2 pi i p u / ncur) for
+// p = 0..m-1, u = 1..r-1 sit at tw[twoff[st] + p*(r-1) + (u-1)], where
+// ncur = n / (rad[0]*...*rad[st-1]) and m = ncur / r.
+struct FftDesc {
+    int n;
+    int nst;
+    int rad[kMaxStages];
+    int twoff[kMaxStages];
+};
+
+// Geometry and 1-D metric tables of one case, as the kernels read it.
+// Index conventions (reference indices are 1-based):
+//   inv_dyf[j-2] = 1/(y(j)-y(j-1)),    j = 2..nyg-1   (nym entries)
+//   inv_dzf[k-2] = 1/(z(k)-z(k-1)),    k = 2..nzg-1   (nzp+1 entries)
+//   inv_dyg[j-2] = 1/(yg(j+1)-yg(j)),  j = 2..ny-1    (nym-1 entries)
+//   inv_dzg[k-2] = 1/(zg(k+1)-zg(k)),  k = 2..nz-1    (nzp entries)
+struct Geo {
+    int nxp, nym, nzp, nxh;
+    int nx, ny, nz, nxg, nyg, nzg;
+    double inv_dx;
+    const double* inv_dyf;
+    const double* inv_dzf;
+    const double* inv_dyg;
+    const double* inv_dzg;
+    const double* x;   // x(1:nx)
+    const double* y;   // y(1:ny)
+    const double* z;   // z(1:nz)
+    const double* yg;  // yg(1:nyg)
+};
+
+// y-operator of the spectral solve: Dyy(j+1,j), Dyy(j,j), Dyy(j,j+1) and the
+// modified wavenumbers of the kept modes.
+struct Ops {
+    const double* dl;   // nym-1
+    const double* dd;   // nym
+    const double* du;   // nym-1
+    const double* kxx;  // nxh (kx = 0..nxp/2)
+    const double* kzz;  // nzp
+};
+
+// Launch-time record of one kernel for the profiler.
+struct ProfSlot {
+    std::string name;
+    int64_t launches = 0;
+    double ms = 0.0;
+};
+
+}  // namespace dfp
+
+struct dfp_plan {
+    int device = 0;
+    int precision = DFP_FP64;
+    dfp::Geo geo{};
+    dfp::Ops ops{};
+    double dx = 0, dz = 0;
+    std::vector<double> hx, hy, hz, hyg, hzg, hkxx, hkzz, hdl, hdd, hdu;
+    // device tables
+    double* d_tables = nullptr;  // one allocation holding every 1-D table
+    double2* d_tw = nullptr;     // FFT twiddles: x stages, z stages, R2C post twiddles
+    dfp::FftDesc fx{}, fz{};     // x: length nxp/2 (half-length complex), z: length nzp
+    const double2* tw_x = nullptr;
+    const double2* tw_z = nullptr;
+    const double2* tw_r2c = nullptr;  // exp(-2 pi i k / nxp), k = 0..nxp/2
+    // device work arrays
+    double2* d_spec = nullptr;   // (nxp/2+1) x nym x nzp complex, layout [kx][j][kz]
+    double* d_piv = nullptr;     // Thomas pivots, same layout as d_spec (real)
+    double* d_red = nullptr;     // reduction partials
+    int red_len = 0;
+    // staging for host-resident fields
+    double *d_U = nullptr, *d_V = nullptr, *d_W = nullptr, *d_rhs = nullptr, *d_P = nullptr;
+    int64_t sz_u = 0, sz_v = 0, sz_w = 0, sz_p = 0, sz_rhs = 0;
+    hipStream_t stream = nullptr;
+    // profiling
+    bool prof = false;
+    std::vector<dfp::ProfSlot> slots;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> pending;
+    std::vector<hipEvent_t> event_pool;
+    std::string last_error;
+    // launch shapes
+    int tk_x = 8;   // lines per block of the x-transform kernels
+    int rb_z = 4;   // rows per block of the z-transform kernels
+};
+
+namespace dfp {
+
+// launchers (defined next to their kernels)
+hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W,
+                             const double* rhs, int from_rhs);
+hipError_t launch_fft_z(dfp_plan* p, int sign);
+hipError_t launch_thomas_y(dfp_plan* p);
+hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p);
+hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs);
+hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs);
+hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);
+hipError_t launch_synth(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma);
+
+// profiler hooks used by the launchers
+int prof_begin(dfp_plan* p, const char* name);
+void prof_end(dfp_plan* p, int slot);
+
+inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+}  // namespace dfp

@@ -1,0 +1,214 @@
+// Physical-space stencils of the projection (FP64, HBM-bound):
+//   k_divergence   rhs_p = div(U,V,W)            projection.f90:37-63
+//   k_correct      U,V,W -= grad(rhs_p)          projection.f90:551-604
+//   k_max_div      max |div(U,V,W)|              projection.f90:607-639
+//   k_synth        seeded synthetic channel fields (benchmark/test inputs)
+// Threads run along x (the contiguous index of every field); blockIdx.y/z
+// are the y and z cell indices, so every row access is coalesced.  The
+// stencil arithmetic is kept operation-for-operation as the reference writes
+// it (no FMA contraction), so these kernels reproduce the CPU path bit for bit.
+#include "dfp_internal.h"
+
+namespace dfp {
+
+// rhs_p(2:nxg,2:nyg-1,2:nzg) element of reference cell (i,j,k)
+__device__ __forceinline__ long ridx(const Geo& g, int i, int j, int k) {
+    return (long)(i - 2) + (long)(j - 2) * (g.nxg - 1) + (long)(k - 2) * (long)(g.nxg - 1) * g.nym;
+}
+
+__global__ void __launch_bounds__(256) k_divergence(const double* __restrict__ U, const double* __restrict__ V,
+                                                    const double* __restrict__ W, double* __restrict__ rhs, Geo g) {
+#pragma clang fp contract(off)
+    const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;  // 2..nxg-1
+    const int j = 2 + blockIdx.y;                             // 2..nyg-1
+    const int k = 2 + blockIdx.z;                             // 2..nzg-1
+    if (i > g.nxg - 1) return;
+    const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg;
+    const long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.ny;
+    const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.nyg;
+    const double du = (U[iu] - U[iu - 1]) * g.inv_dx;
+    const double dv = (V[iv] - V[iv - g.nxg]) * g.inv_dyf[j - 2];
+    const double dw = (W[iw] - W[iw - (long)g.nxg * g.nyg]) * g.inv_dzf[k - 2];
+    rhs[ridx(g, i, j, k)] = du + dv + dw;
+}
+
+hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs) {
+    const Geo& g = p->geo;
+    dim3 grid(ceil_div(g.nxg - 2, 256), g.nyg - 2, g.nzg - 2);
+    const int slot = prof_begin(p, "divergence");
+    hipLaunchKernelGGL(k_divergence, grid, dim3(256), 0, p->stream, U, V, W, rhs, g);
+    prof_end(p, slot);
+    return hipGetLastError();
+}
+
+// U(i,j,k) -= (phi(i+1)-phi(i))/dx          i=2..nx-1,  j=2..nyg-1, k=2..nzg-1
+// V(i,j,k) -= (phi(j+1)-phi(j))/dyg(j)      i=2..nxg-1, j=2..ny-1,  k=2..nzg-1
+// W(i,j,k) -= (phi(k+1)-phi(k))/dzg(k)      i=2..nxg-1, j=2..nyg-1, k=2..nz-1
+__global__ void __launch_bounds__(256) k_correct(double* __restrict__ U, double* __restrict__ V,
+                                                 double* __restrict__ W, const double* __restrict__ rhs, Geo g) {
+#pragma clang fp contract(off)
+    const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y;
+    const int k = 2 + blockIdx.z;
+    if (i > g.nxg - 1) return;
+    const long r = ridx(g, i, j, k);
+    const double phi = rhs[r];
+    if (i <= g.nx - 1) {
+        const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg;
+        U[iu] = U[iu] - (rhs[r + 1] - phi) * g.inv_dx;
+    }
+    if (j <= g.ny - 1) {
+        const long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.ny;
+        V[iv] = V[iv] - (rhs[r + (g.nxg - 1)] - phi) * g.inv_dyg[j - 2];
+    }
+    if (k <= g.nz - 1) {
+        const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.nyg;
+        W[iw] = W[iw] - (rhs[r + (long)(g.nxg - 1) * g.nym] - phi) * g.inv_dzg[k - 2];
+    }
+}
+
+hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs) {
+    const Geo& g = p->geo;
+    dim3 grid(ceil_div(g.nxg - 2, 256), g.nyg - 2, g.nzg - 2);
+    const int slot = prof_begin(p, "correct");
+    hipLaunchKernelGGL(k_correct, grid, dim3(256), 0, p->stream, U, V, W, rhs, g);
+    prof_end(p, slot);
+    return hipGetLastError();
+}
+
+// Block-level max of |div| over one (j,k) row segment; partials[block] holds it.
+__global__ void __launch_bounds__(256) k_max_div(const double* __restrict__ U, const double* __restrict__ V,
+                                                 const double* __restrict__ W, double* __restrict__ partials, Geo g) {
+    __shared__ double red[256];
+    const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + blockIdx.y;
+    const int k = 2 + blockIdx.z;
+    double v = 0.0;
+    if (i <= g.nxg - 1) {
+        const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg;
+        const long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.ny;
+        const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.nyg;
+        const double dxi = g.x ? (g.x[i - 1] - g.x[i - 2]) : 1.0 / g.inv_dx;
+        v = fabs((U[iu] - U[iu - 1]) / dxi + (V[iv] - V[iv - g.nxg]) / (g.y[j - 1] - g.y[j - 2]) +
+                 (W[iw] - W[iw - (long)g.nxg * g.nyg]) / (g.z[k - 1] - g.z[k - 2]));
+    }
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
+        partials[b] = red[0];
+    }
+}
+
+__global__ void __launch_bounds__(256) k_reduce_max(const double* __restrict__ partials, long n,
+                                                    double* __restrict__ out) {
+    __shared__ double red[256];
+    double v = 0.0;
+    for (long e = threadIdx.x; e < n; e += blockDim.x) v = fmax(v, partials[e]);
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = red[0];
+}
+
+hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev) {
+    const Geo& g = p->geo;
+    dim3 grid(ceil_div(g.nxg - 2, 256), g.nyg - 2, g.nzg - 2);
+    const long nb = (long)grid.x * grid.y * grid.z;
+    if (nb > p->red_len) return hipErrorInvalidValue;
+    const int slot = prof_begin(p, "max_div");
+    hipLaunchKernelGGL(k_max_div, grid, dim3(256), 0, p->stream, U, V, W, p->d_red, g);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, p->stream, p->d_red, nb, out_dev);
+    prof_end(p, slot);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// synthetic fields: SplitMix64 counter hash -> Box-Muller normal deviates
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ unsigned long long mix64(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ double gauss(unsigned long long key, unsigned long long e) {
+    const unsigned long long G = 0x9E3779B97F4A7C15ull;
+    const unsigned long long a = mix64(key + (2ull * e + 1ull) * G);
+    const unsigned long long b = mix64(key + (2ull * e + 2ull) * G);
+    const double u1 = (double)((a >> 11) + 1ull) * 0x1.0p-53;  // (0,1]
+    const double u2 = (double)(b >> 11) * 0x1.0p-53;           // [0,1)
+    return sqrt(-2.0 * log(u1)) * cos(6.283185307179586476925286766559 * u2);
+}
+
+// periodic source index of a cell-centred ghost (1-based): 1 -> n-2, n-1 -> 2, n -> 3
+__device__ __forceinline__ int per_cell(int i, int n) { return i == 1 ? n - 2 : (i == n - 1 ? 2 : (i == n ? 3 : i)); }
+// periodic source index of a face ghost: 1 -> n-1, n -> 2
+__device__ __forceinline__ int per_face(int i, int n) { return i == 1 ? n - 1 : (i == n ? 2 : i); }
+
+// field f: 0 = U, 1 = V, 2 = W.  One thread per array element (ghosts included).
+__global__ void __launch_bounds__(256) k_synth(double* __restrict__ F, int f, Geo g, unsigned long long key,
+                                               double sigma, double yc0, double h) {
+    const int n1 = (f == 0) ? g.nx : g.nxg;
+    const int n2 = (f == 1) ? g.ny : g.nyg;
+    const int n3 = (f == 2) ? g.nz : g.nzg;
+    const int i = 1 + blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 1 + blockIdx.y;
+    const int k = 1 + blockIdx.z;
+    if (i > n1) return;
+    const int is = (f == 0) ? per_face(i, n1) : per_cell(i, n1);
+    const int ks = (f == 2) ? per_face(k, n3) : per_cell(k, n3);
+    double sgn = 1.0;
+    int js = j;
+    if (f == 1) {
+        if (j == 1 || j == n2) {
+            F[(i - 1) + (long)(j - 1) * n1 + (long)(k - 1) * n1 * n2] = 0.0;
+            return;
+        }
+    } else if (j == 1) {
+        js = 2; sgn = -1.0;
+    } else if (j == n2) {
+        js = n2 - 1; sgn = -1.0;
+    }
+    const unsigned long long e = (unsigned long long)(is - 1) + (unsigned long long)(js - 1) * n1 +
+                                 (unsigned long long)(ks - 1) * n1 * n2;
+    double v = sigma * gauss(key, e);
+    if (f == 0) {
+        const double yc = (g.yg[js - 1] - yc0) / h;
+        v = (1.0 - yc * yc) + v;
+    }
+    F[(i - 1) + (long)(j - 1) * n1 + (long)(k - 1) * n1 * n2] = sgn * v;
+}
+
+static unsigned long long mix64_host(unsigned long long z) {
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+
+hipError_t launch_synth(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma) {
+    const Geo& g = p->geo;
+    const double yc0 = 0.5 * (p->hy.front() + p->hy.back());
+    const double h = 0.5 * (p->hy.back() - p->hy.front());
+    double* F[3] = {U, V, W};
+    const int slot = prof_begin(p, "synth");
+    for (int f = 0; f < 3; ++f) {
+        const int n1 = (f == 0) ? g.nx : g.nxg;
+        const int n2 = (f == 1) ? g.ny : g.nyg;
+        const int n3 = (f == 2) ? g.nz : g.nzg;
+        const unsigned long long key = mix64_host(seed + (unsigned long long)(f + 1) * 0x9E3779B97F4A7C15ull);
+        hipLaunchKernelGGL(k_synth, dim3(ceil_div(n1, 256), n2, n3), dim3(256), 0, p->stream, F[f], f, g, key, sigma,
+                           yc0, h);
+    }
+    prof_end(p, slot);
+    return hipGetLastError();
+}
+
+}  // namespace dfp

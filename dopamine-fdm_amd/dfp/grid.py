@@ -1,0 +1,99 @@
+"""Channel grid and metrics, built as the reference builds them.
+
+x and z faces are uniform (src/genGridandIC.f90:22-41), y faces follow the
+symmetric tanh map of grid_type = 2 (:54-61) or are uniform (grid_type = 1,
+:48-52); centres carry one mirrored ghost per side
+(src/initialization.f90:249-272) and the FFT spacings are dx = dxmin,
+dz = dzmin (:279-281, :326-327).  The library receives these arrays through
+``dfp_grid``; a Fortran caller passes its own ``global`` arrays instead.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class ChannelGrid:
+    nxp: int
+    nym: int
+    nzp: int
+    Lx_i: float   # namelist (face-to-face) lengths, duplicate periodic cell included
+    Ly_i: float
+    Lz_i: float
+    alpha: float
+    grid_type: int = 2
+    x: np.ndarray = field(init=False)
+    y: np.ndarray = field(init=False)
+    z: np.ndarray = field(init=False)
+    xg: np.ndarray = field(init=False)
+    yg: np.ndarray = field(init=False)
+    zg: np.ndarray = field(init=False)
+    dx: float = field(init=False)
+    dz: float = field(init=False)
+
+    def __post_init__(self):
+        if self.nxp < 2 or self.nxp % 2 or self.nym < 1 or self.nzp < 1:
+            raise ValueError("need nxp even >= 2, nym >= 1, nzp >= 1")
+        nx, ny, nz = self.nx, self.ny, self.nz
+        self.x = np.array([self.Lx_i * float(i) / float(nx - 1) for i in range(nx)])
+        self.z = np.array([self.Lz_i * float(i) / float(nz - 1) for i in range(nz)])
+        if self.grid_type == 1:
+            self.y = np.array([self.Ly_i * float(i) / float(ny - 1) for i in range(ny)])
+        elif self.grid_type == 2:
+            th = math.tanh(0.5 * self.alpha)
+            self.y = np.array([self.Ly_i * 0.5 * (1.0 + math.tanh(self.alpha * (float(i) / float(ny - 1) - 0.5)) / th)
+                               for i in range(ny)])
+        else:
+            raise ValueError("grid_type must be 1 (uniform) or 2 (tanh)")
+        self.xg = _centres(self.x)
+        self.yg = _centres(self.y)
+        self.zg = _centres(self.z)
+        self.dx = float(np.min(np.diff(self.xg)))
+        self.dz = float(np.min(np.diff(self.zg)))
+
+    nx = property(lambda s: s.nxp + 2)
+    ny = property(lambda s: s.nym + 1)
+    nz = property(lambda s: s.nzp + 2)
+    nxg = property(lambda s: s.nxp + 3)
+    nyg = property(lambda s: s.nym + 2)
+    nzg = property(lambda s: s.nzp + 3)
+    shape_u = property(lambda s: (s.nx, s.nyg, s.nzg))
+    shape_v = property(lambda s: (s.nxg, s.ny, s.nzg))
+    shape_w = property(lambda s: (s.nxg, s.nyg, s.nz))
+    shape_p = property(lambda s: (s.nxg, s.nyg, s.nzg))
+    shape_rhs = property(lambda s: (s.nxg - 1, s.nyg - 2, s.nzg - 1))
+
+    @property
+    def n_cells(self):
+        return self.nxp * self.nym * self.nzp
+
+    @property
+    def n_modes(self):
+        return (self.nxp // 2 + 1) * self.nym * self.nzp
+
+    @property
+    def min_spacing(self):
+        return min(self.dx, self.dz, float(np.min(np.diff(self.y))))
+
+
+def _centres(f: np.ndarray) -> np.ndarray:
+    n = f.size
+    g = np.empty(n + 1)
+    g[1:n] = 0.5 * (f[:-1] + f[1:])
+    g[0] = g[1] - 2.0 * (g[1] - f[0])
+    g[n] = g[n - 1] + 2.0 * (f[n - 1] - g[n - 1])
+    return g
+
+
+def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * math.pi, Lz=math.pi, Ly=2.0, grid_type=2) -> ChannelGrid:
+    """Channel whose periodic lengths are Lx and Lz (the reference's period is
+    namelist Lx minus one cell, src/initialization.f90:332)."""
+    return ChannelGrid(nxp, nym, nzp, Lx * (nxp + 1) / nxp, Ly, Lz * (nzp + 1) / nzp, alpha, grid_type)
+
+
+def namelist_grid(nx, ny, nz, Lx, Ly, Lz, alpha, grid_type=2) -> ChannelGrid:
+    """Grid from the reference's namelist values (face counts, lengths)."""
+    return ChannelGrid(nx - 2, ny - 1, nz - 2, Lx, Ly, Lz, alpha, grid_type)

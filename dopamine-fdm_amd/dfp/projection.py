@@ -1,0 +1,244 @@
+"""Host-side mirror of the reference's ``projection`` module over libdfp.
+
+``Plan`` wraps one ``dfp_plan`` (one grid on one device).  ``Projection``
+keeps the reference's subroutine names (src/projection.f90) and works on a
+state object holding the ``global`` module arrays, so a test reads like the
+reference's own call sequence:
+
+    st = ChannelState(grid);  st.U[...] = ...;  st.rk_step = 3
+    proj = Projection(st)
+    proj.compute_projection_step()          # projection.f90:21-34
+    proj.check_divergence()                 # projection.f90:607-639
+
+Arrays may be numpy arrays (host; copied to the device and back each call)
+or torch CUDA tensors (device-resident; no copies).  Errors raise ``DfpError``
+(the reference aborts through gpu_abort, poisson_gpu.f90:51-56).
+"""
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib
+from ._lib import DFP_ASYNC, DFP_FIELDS_ON_DEVICE, DFP_FP64, DFP_SAVE_P, DfpGrid, check, lib
+from .grid import ChannelGrid
+from .hipmem import DeviceArray
+
+# src/initialization.f90:654-660 (diagonal of the RK3 table)
+RK_COEF_DIAG = (8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0)
+
+
+def _is_torch(a) -> bool:
+    return type(a).__module__.startswith("torch")
+
+
+def _ptr(a, shape, name):
+    """Address of a Fortran-ordered float64 array, plus whether it is on the device."""
+    if a is None:
+        return None, None
+    if isinstance(a, DeviceArray):
+        if a.shape != tuple(shape):
+            raise ValueError(f"{name}: expected device array of shape {shape}, got {a.shape}")
+        return a.ptr, True
+    if _is_torch(a):
+        import torch
+
+        if a.dtype != torch.float64:
+            raise ValueError(f"{name}: float64 required")
+        # a Fortran array of shape (n1,n2,n3) is a C-contiguous (n3,n2,n1) tensor,
+        # or its transposed view
+        if tuple(a.shape) == tuple(reversed(shape)) and a.is_contiguous():
+            pass
+        elif tuple(a.shape) == tuple(shape) and a.permute(2, 1, 0).is_contiguous():
+            pass
+        else:
+            raise ValueError(f"{name}: expected Fortran layout {shape}")
+        return a.data_ptr(), bool(a.is_cuda)
+    if not isinstance(a, np.ndarray) or a.dtype != np.float64:
+        raise ValueError(f"{name}: float64 numpy array or torch tensor required")
+    if a.shape != tuple(shape) or not a.flags.f_contiguous:
+        raise ValueError(f"{name}: expected a Fortran-ordered array of shape {shape}, got {a.shape}")
+    if not a.flags.writeable:
+        raise ValueError(f"{name}: array is read-only")
+    return a.ctypes.data, False
+
+
+class Plan:
+    """One ``dfp_plan``: operators, twiddles and device scratch of one grid."""
+
+    def __init__(self, g: ChannelGrid, device: int = 0, precision: int = DFP_FP64):
+        self.g = g
+        self._arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in (g.x, g.y, g.z, g.yg, g.zg)]
+        dp = ctypes.POINTER(ctypes.c_double)
+        x, y, z, yg, zg = (a.ctypes.data_as(dp) for a in self._arrays)
+        self._grid = DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, x, y, z, yg, zg, 0, 1, 0)
+        h = ctypes.c_void_p()
+        check(lib().dfp_plan_create(ctypes.byref(self._grid), precision, device, ctypes.byref(h)), "dfp_plan_create")
+        self._h = h
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if self._h:
+            lib().dfp_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- helpers -------------------------------------------------------------
+    def _flags(self, *devs, save_p=False, asynchronous=False):
+        devs = [d for d in devs if d is not None]
+        if any(devs) and not all(devs):
+            raise ValueError("mixing host and device arrays in one call")
+        f = DFP_FIELDS_ON_DEVICE if (devs and devs[0]) else 0
+        if save_p:
+            f |= DFP_SAVE_P
+        if asynchronous and f & DFP_FIELDS_ON_DEVICE:
+            f |= DFP_ASYNC
+        return f
+
+    def set_stream(self, stream_handle: int | None):
+        check(lib().dfp_set_stream(self._h, stream_handle), "dfp_set_stream", self._h)
+
+    # --- entry points -----------------------------------------------------------
+    def project(self, U, V, W, rhs_p, P=None, dt=0.0, c=0.0, save_p=False, asynchronous=False):
+        g = self.g
+        pu, du = _ptr(U, g.shape_u, "U")
+        pv, dv = _ptr(V, g.shape_v, "V")
+        pw, dw = _ptr(W, g.shape_w, "W")
+        pr, dr = _ptr(rhs_p, g.shape_rhs, "rhs_p")
+        pp, dpp = _ptr(P, g.shape_p, "P")
+        f = self._flags(du, dv, dw, dr, dpp, save_p=save_p, asynchronous=asynchronous)
+        check(lib().dfp_project(self._h, pu, pv, pw, pr, pp, float(dt), float(c), f), "dfp_project", self._h)
+
+    def divergence(self, U, V, W, rhs_p):
+        g = self.g
+        pu, du = _ptr(U, g.shape_u, "U")
+        pv, dv = _ptr(V, g.shape_v, "V")
+        pw, dw = _ptr(W, g.shape_w, "W")
+        pr, dr = _ptr(rhs_p, g.shape_rhs, "rhs_p")
+        f = self._flags(du, dv, dw, dr)
+        check(lib().dfp_divergence(self._h, pu, pv, pw, pr, f), "dfp_divergence", self._h)
+
+    def solve(self, rhs_p, P=None, dt=0.0, c=0.0, save_p=False):
+        g = self.g
+        pr, dr = _ptr(rhs_p, g.shape_rhs, "rhs_p")
+        pp, dpp = _ptr(P, g.shape_p, "P")
+        f = self._flags(dr, dpp, save_p=save_p)
+        check(lib().dfp_solve(self._h, pr, pp, float(dt), float(c), f), "dfp_solve", self._h)
+
+    def correct(self, U, V, W, rhs_p):
+        g = self.g
+        pu, du = _ptr(U, g.shape_u, "U")
+        pv, dv = _ptr(V, g.shape_v, "V")
+        pw, dw = _ptr(W, g.shape_w, "W")
+        pr, dr = _ptr(rhs_p, g.shape_rhs, "rhs_p")
+        f = self._flags(du, dv, dw, dr)
+        check(lib().dfp_correct(self._h, pu, pv, pw, pr, f), "dfp_correct", self._h)
+
+    def max_divergence(self, U, V, W) -> float:
+        g = self.g
+        pu, du = _ptr(U, g.shape_u, "U")
+        pv, dv = _ptr(V, g.shape_v, "V")
+        pw, dw = _ptr(W, g.shape_w, "W")
+        f = self._flags(du, dv, dw)
+        out = ctypes.c_double()
+        check(lib().dfp_max_divergence(self._h, pu, pv, pw, ctypes.byref(out), f), "dfp_max_divergence", self._h)
+        return out.value
+
+    def synth(self, U, V, W, seed: int, sigma: float):
+        """Device-side synthetic fields (torch CUDA tensors in Fortran layout)."""
+        g = self.g
+        pu, du = _ptr(U, g.shape_u, "U")
+        pv, dv = _ptr(V, g.shape_v, "V")
+        pw, dw = _ptr(W, g.shape_w, "W")
+        if not (du and dv and dw):
+            raise ValueError("synth needs device arrays")
+        check(lib().dfp_synth_fields(self._h, pu, pv, pw, ctypes.c_uint64(seed & (2 ** 64 - 1)), float(sigma)),
+              "dfp_synth_fields", self._h)
+
+    # --- stage timing ---------------------------------------------------------------
+    def profile(self, enable: bool):
+        check(lib().dfp_profile_enable(self._h, 1 if enable else 0), "dfp_profile_enable", self._h)
+
+    def profile_reset(self):
+        check(lib().dfp_profile_reset(self._h), "dfp_profile_reset", self._h)
+
+    def profile_read(self) -> dict:
+        out = {}
+        i = 0
+        while True:
+            name = ctypes.c_char_p()
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            rc = lib().dfp_profile_read(self._h, i, ctypes.byref(name), ctypes.byref(n), ctypes.byref(ms))
+            if rc != 0:
+                break
+            out[name.value.decode()] = (int(n.value), float(ms.value))
+            i += 1
+        return out
+
+
+@dataclass
+class ChannelState:
+    """The ``global``-module data the projection reads and writes
+    (global.f90:139-146, :177-178, :293-294): fields, dt and the RK stage."""
+
+    g: ChannelGrid
+    U: np.ndarray = field(init=False)
+    V: np.ndarray = field(init=False)
+    W: np.ndarray = field(init=False)
+    P: np.ndarray = field(init=False)
+    rhs_p: np.ndarray = field(init=False)
+    dt: float = 1e-3
+    rk_step: int = 1
+
+    def __post_init__(self):
+        g = self.g
+        self.U = np.zeros(g.shape_u, order="F")
+        self.V = np.zeros(g.shape_v, order="F")
+        self.W = np.zeros(g.shape_w, order="F")
+        self.P = np.zeros(g.shape_p, order="F")
+        self.rhs_p = np.zeros(g.shape_rhs, order="F")
+
+
+class Projection:
+    """The reference module's subroutines (src/projection.f90) over a plan."""
+
+    def __init__(self, state: ChannelState, device: int = 0):
+        self.st = state
+        self.plan = Plan(state.g, device)
+
+    def _save_p(self, skip_p_save):
+        # projection.f90:109-110, :362
+        return self.st.rk_step == 3 and not bool(skip_p_save)
+
+    def compute_projection_step(self):
+        """projection.f90:21-34 (the fused device path)."""
+        st = self.st
+        # P = phi/(dt*rk_coef(3,3)) at the last stage only (projection.f90:362-377)
+        self.plan.project(st.U, st.V, st.W, st.rhs_p, st.P, st.dt, RK_COEF_DIAG[2], save_p=self._save_p(False))
+
+    def compute_pseudo_pressure_rhs(self):
+        st = self.st
+        self.plan.divergence(st.U, st.V, st.W, st.rhs_p)
+
+    def solve_poisson_equation(self, skip_p_save=None):
+        st = self.st
+        self.plan.solve(st.rhs_p, st.P, st.dt, RK_COEF_DIAG[2], save_p=self._save_p(skip_p_save))
+
+    def project_velocity(self):
+        st = self.st
+        self.plan.correct(st.U, st.V, st.W, st.rhs_p)
+
+    def check_divergence(self) -> float:
+        st = self.st
+        return self.plan.max_divergence(st.U, st.V, st.W)

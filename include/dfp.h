@@ -1,0 +1,128 @@
+/*
+ * dfp -- MI355X-native pressure projection for dopamine-fdm (C ABI).
+ *
+ * Replaces the Poisson/projection path of the reference's Fortran module
+ * `projection` (src/projection.f90) and its GPU solver (src/poisson_gpu.f90)
+ * for the channel case: x periodic, y walls, z periodic
+ * (x_bc_type = 0, y_bc_type = 1, z_bc_type = 0), one rank.
+ *
+ * All arrays are the reference's Fortran arrays, passed by address
+ * (column-major, x fastest), with these extents (initialization.f90:204-220):
+ *     U(nx,nyg,nzg)   V(nxg,ny,nzg)   W(nxg,nyg,nz)   P(nxg,nyg,nzg)
+ *     rhs_p(2:nxg, 2:nyg-1, 2:nzg)   ->  (nxg-1) x (nyg-2) x (nzg-1)
+ * where nxp, nym, nzp are the Fourier/cell counts and
+ *     nx = nxp+2, ny = nym+1, nz = nzp+2, nxg = nxp+3, nyg = nym+2, nzg = nzp+3.
+ *
+ * Calls are synchronous unless DFP_ASYNC is given.  A plan is not
+ * thread-safe: use one plan per host thread.  Every function returns 0 on
+ * success or a negative DFP_E* code; dfp_strerror() names it, and
+ * dfp_last_error() gives the plan's last detailed message.
+ */
+#ifndef DFP_H
+#define DFP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dfp_plan dfp_plan;
+
+/* Grid of one case.  Mirrors the `global` module data the reference's
+ * projection reads (global.f90:130-178, initialization.f90:243-345). */
+typedef struct dfp_grid {
+    int32_t nxp, nym, nzp;  /* Fourier/cell grid: namelist nx-2, ny-1, nz-2       */
+    double dx, dz;          /* FFT spacings: dxmin, dzmin (initialization.f90:326) */
+    const double *x;        /* x faces  x(1:nx)   (max-divergence only; may be NULL: */
+                            /*          then x(i)-x(i-1) is taken as dx)            */
+    const double *y;        /* y faces  y(1:ny)                                     */
+    const double *z;        /* z faces  z(1:nz)                                     */
+    const double *yg;       /* y centres with ghosts yg(1:nyg)                      */
+    const double *zg;       /* z centres with ghosts zg(1:nzg)                      */
+    int32_t x_bc_type, y_bc_type, z_bc_type; /* must be 0, 1, 0 in this build       */
+} dfp_grid;
+
+/* precision of the spectral solve */
+enum { DFP_FP64 = 0, DFP_FP32_REFINE = 1 };
+
+/* flags */
+enum {
+    DFP_FIELDS_ON_DEVICE = 1, /* array arguments are device pointers (resident mode)  */
+    DFP_SAVE_P = 2,           /* also write P = phi/(dt*c) with its ghosts (rk_step 3) */
+    DFP_ASYNC = 4             /* do not synchronise the stream before returning       */
+};
+
+/* error codes */
+enum {
+    DFP_OK = 0,
+    DFP_E_ARG = -1,         /* bad argument (null pointer, wrong size, bad flag)      */
+    DFP_E_UNSUPPORTED = -2, /* boundary conditions or FFT lengths outside this build  */
+    DFP_E_HIP = -3,         /* HIP runtime error                                      */
+    DFP_E_NOMEM = -4,       /* device allocation failed                               */
+    DFP_E_NONFINITE = -5    /* NaN or Inf found where a finite value was required     */
+};
+
+/* Build a plan: operators (kxx, kzz, Dyy as in initialization.f90:416-539),
+ * FFT twiddles and device scratch for one grid on one device.
+ * Replaces the reference's set-up in initialization.f90:322-539 and
+ * gpu_poisson_init / gpu_gtsv_init (poisson_gpu.f90:60-135, :388-428). */
+int dfp_plan_create(const dfp_grid *g, int precision, int device, dfp_plan **out);
+int dfp_plan_destroy(dfp_plan *plan);
+
+/* Whole projection: divergence -> Poisson solve -> velocity correction.
+ * Replaces compute_projection_step (projection.f90:21-34).  On return U, V,
+ * W hold the corrected velocity (projection.f90:551-604), rhs_p holds the
+ * pseudo-pressure phi with the periodic duplicates filled
+ * (projection.f90:441-500) and, with DFP_SAVE_P, P = phi/(dt*c) with its
+ * ghosts (projection.f90:361-380, :388-438). */
+int dfp_project(dfp_plan *plan, double *U, double *V, double *W, double *rhs_p, double *P,
+                double dt, double c, int flags);
+
+/* rhs_p = div(U,V,W) on i=2..nxg-1, j=2..nyg-1, k=2..nzg-1.
+ * Replaces compute_pseudo_pressure_rhs (projection.f90:37-63). */
+int dfp_divergence(dfp_plan *plan, const double *U, const double *V, const double *W,
+                   double *rhs_p, int flags);
+
+/* Solve the pressure Poisson equation in place on rhs_p.
+ * Replaces solve_poisson_equation (projection.f90:100-382); P may be NULL,
+ * and is written only when DFP_SAVE_P is given (the reference's
+ * skip_p_save=.True. corresponds to leaving DFP_SAVE_P out). */
+int dfp_solve(dfp_plan *plan, double *rhs_p, double *P, double dt, double c, int flags);
+
+/* U,V,W -= grad(rhs_p).  Replaces project_velocity (projection.f90:551-604). */
+int dfp_correct(dfp_plan *plan, double *U, double *V, double *W, const double *rhs_p, int flags);
+
+/* max |div u| over the interior cells, with the face spacings of
+ * check_divergence (projection.f90:607-639, no immersed boundary). */
+int dfp_max_divergence(dfp_plan *plan, const double *U, const double *V, const double *W,
+                       double *out, int flags);
+
+/* Launch stream for the plan's kernels (a hipStream_t; NULL = default stream). */
+int dfp_set_stream(dfp_plan *plan, void *hip_stream);
+
+/* Stage timing, the analogue of the reference's profiler tags
+ * (profiler.f90:14-16).  While enabled, every kernel launch is bracketed by
+ * HIP events on the plan's stream; dfp_profile_read returns, per kernel
+ * name, the number of launches and the summed milliseconds. */
+int dfp_profile_enable(dfp_plan *plan, int enable);
+int dfp_profile_read(dfp_plan *plan, int index, const char **name, int64_t *launches, double *ms);
+int dfp_profile_reset(dfp_plan *plan);
+
+/* Sizes of the arrays a plan expects, in doubles: U, V, W, P, rhs_p. */
+int dfp_field_sizes(const dfp_plan *plan, int64_t sizes[5]);
+
+/* Synthetic channel fields for tests and benchmarks (no counterpart in the
+ * reference, whose initial conditions use unseeded noise): U = 1-(y/h)^2 +
+ * sigma*N(0,1), V, W = sigma*N(0,1), V = 0 on the walls, periodic x/z ghosts
+ * and no-slip y ghosts as boundary_conditions.f90 leaves them.  Normal
+ * deviates: SplitMix64 counter hash -> Box-Muller.  Device pointers only. */
+int dfp_synth_fields(dfp_plan *plan, double *U, double *V, double *W, uint64_t seed, double sigma);
+
+const char *dfp_strerror(int code);
+const char *dfp_last_error(const dfp_plan *plan);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* DFP_H */

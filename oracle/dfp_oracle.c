@@ -1,0 +1,574 @@
+/*
+ * C restatement of dopamine-fdm's channel pressure projection (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: the product never links or calls this code.  It
+ * is the checker the GPU parity tests compare against and the host-CPU
+ * baseline that bench.py times.
+ *
+ * Reference lines restated (src/...):
+ *   grid .............. genGridandIC.f90:22-61, initialization.f90:243-281, :326-345
+ *   kxx, kzz .......... initialization.f90:416-434
+ *   Dyy ............... initialization.f90:506-535
+ *   divergence ........ projection.f90:37-63
+ *   solve (CPU) ....... projection.f90:100-382: copy :121, dcmplx + x C2C FFT
+ *                       :124-125, z C2C FFT :134-136, Zgtsv per mode :277-297
+ *                       with the (0,0) pin :292, inverse z :308-310, inverse x
+ *                       + Real()/(nxp*nzp) :325-327, copy back :342,
+ *                       periodic duplicates :441-500, P save :361-380,
+ *                       finish_pressure_ghosts :388-438 with
+ *                       boundary_conditions.f90:277-289, :591-602
+ *   correction ........ projection.f90:551-604
+ *   max divergence .... projection.f90:606-639
+ * Third-party algorithms restated from their published definitions:
+ *   LAPACK zgtsv (Gaussian elimination with partial pivoting on a
+ *   tridiagonal matrix; LAPACK version unpinned by the reference,
+ *   CMakeLists.txt:50-51) and FFTW's unnormalised complex DFT
+ *   (FFTW_FORWARD = e^{-2 pi i jk/n}); the DFT is computed with this file's
+ *   own mixed-radix Stockham FFT.
+ * At one rank the 2decomp&fft transposes are data copies that do not change
+ * values; they are left out.
+ *
+ * Pinned against the independent numpy restatement (oracle/ref_numpy.py) and
+ * the golden fixtures in tests/golden by tests/test_oracle_c.py.
+ */
+#include "dfp_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { double re, im; } cplx;
+
+static inline cplx cadd(cplx a, cplx b) { cplx r = {a.re + b.re, a.im + b.im}; return r; }
+static inline cplx csub(cplx a, cplx b) { cplx r = {a.re - b.re, a.im - b.im}; return r; }
+static inline cplx cmul(cplx a, cplx b) {
+    cplx r = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return r;
+}
+
+/* -------------------------------------------------------------------------
+ * mixed-radix Stockham FFT (decimation in frequency, natural-order output)
+ * ------------------------------------------------------------------------- */
+#define MAXST 40
+typedef struct fplan {
+    int n, nst;
+    int rad[MAXST];
+    cplx *tw[MAXST];   /* tw[st][p*(r-1)+(u-1)] = exp(-2 pi i p u / ncur) */
+    cplx *root[MAXST]; /* generic radix: exp(-2 pi i q / r), q = 0..r-1 */
+} fplan;
+
+static cplx expi_index(long t, long n, int sign) {
+    /* exp(sign * 2 pi i t / n) from the exact index, in long double */
+    long tt = t % n;
+    if (tt < 0) tt += n;
+    long double a = 2.0L * 3.14159265358979323846264338327950288L * (long double)tt / (long double)n;
+    cplx r = {(double)cosl(a), (double)(sign * sinl(a))};
+    return r;
+}
+
+static void fplan_init(fplan *f, int n) {
+    memset(f, 0, sizeof(*f));
+    f->n = n;
+    int m = n, nst = 0;
+    while (m % 4 == 0) { f->rad[nst++] = 4; m /= 4; }
+    while (m % 2 == 0 && m > 1) { f->rad[nst++] = 2; m /= 2; }
+    while (m % 3 == 0) { f->rad[nst++] = 3; m /= 3; }
+    while (m % 5 == 0) { f->rad[nst++] = 5; m /= 5; }
+    for (int p = 7; m > 1; p += 2) {
+        while (m % p == 0) { f->rad[nst++] = p; m /= p; }
+    }
+    f->nst = nst;
+    int ncur = n;
+    for (int st = 0; st < nst; st++) {
+        int r = f->rad[st], mm = ncur / r;
+        f->tw[st] = (cplx *)malloc(sizeof(cplx) * (size_t)(mm * (r - 1) + 1));
+        for (int p = 0; p < mm; p++)
+            for (int u = 1; u < r; u++) f->tw[st][p * (r - 1) + (u - 1)] = expi_index((long)p * u, ncur, -1);
+        if (r != 2 && r != 3 && r != 4) {
+            f->root[st] = (cplx *)malloc(sizeof(cplx) * (size_t)r);
+            for (int q = 0; q < r; q++) f->root[st][q] = expi_index(q, r, -1);
+        }
+        ncur = mm;
+    }
+}
+
+static void fplan_free(fplan *f) {
+    for (int st = 0; st < f->nst; st++) { free(f->tw[st]); free(f->root[st]); }
+}
+
+/* One Stockham pass over "vectors" of V complex values per element.
+ * src element e occupies src[e*V .. e*V+V-1]. */
+static void stockham(const fplan *f, cplx *x, cplx *y, int V, int sign) {
+    int ncur = f->n, s = 1;
+    cplx *src = x, *dst = y;
+    cplx a[64], b[64];
+    for (int st = 0; st < f->nst; st++) {
+        const int r = f->rad[st], m = ncur / r;
+        const cplx *tw = f->tw[st];
+        for (int p = 0; p < m; p++) {
+            for (int q = 0; q < s; q++) {
+                for (int v = 0; v < V; v++) {
+                    for (int t = 0; t < r; t++) a[t] = src[((size_t)q + (size_t)s * (p + (size_t)t * m)) * V + v];
+                    if (r == 2) {
+                        b[0] = cadd(a[0], a[1]);
+                        b[1] = csub(a[0], a[1]);
+                    } else if (r == 4) {
+                        cplx s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+                        cplx s13 = cadd(a[1], a[3]), d13 = csub(a[1], a[3]);
+                        /* forward: omega = -i ; backward: omega = +i */
+                        cplx jd13 = {-sign * d13.im, sign * d13.re}; /* sign*i*d13 */
+                        b[0] = cadd(s02, s13);
+                        b[2] = csub(s02, s13);
+                        b[1] = cadd(d02, jd13);
+                        b[3] = csub(d02, jd13);
+                    } else if (r == 3) {
+                        const double s3 = 0.86602540378443864676372317075294;
+                        cplx t1 = cadd(a[1], a[2]);
+                        cplx t2 = {a[0].re - 0.5 * t1.re, a[0].im - 0.5 * t1.im};
+                        cplx d = csub(a[1], a[2]);
+                        cplx jd = {-sign * s3 * d.im, sign * s3 * d.re}; /* sign*i*(sqrt3/2)*d */
+                        b[0] = cadd(a[0], t1);
+                        b[1] = cadd(t2, jd);
+                        b[2] = csub(t2, jd);
+                    } else {
+                        const cplx *rt = f->root[st];
+                        for (int u = 0; u < r; u++) {
+                            cplx acc = {0.0, 0.0};
+                            for (int t = 0; t < r; t++) {
+                                cplx w = rt[(t * u) % r];
+                                if (sign > 0) w.im = -w.im;
+                                acc = cadd(acc, cmul(a[t], w));
+                            }
+                            b[u] = acc;
+                        }
+                    }
+                    dst[((size_t)q + (size_t)s * ((size_t)r * p)) * V + v] = b[0];
+                    for (int u = 1; u < r; u++) {
+                        cplx w = tw[p * (r - 1) + (u - 1)];
+                        if (sign > 0) w.im = -w.im;
+                        dst[((size_t)q + (size_t)s * ((size_t)r * p + u)) * V + v] = cmul(b[u], w);
+                    }
+                }
+            }
+        }
+        cplx *t = src; src = dst; dst = t;
+        s *= r;
+        ncur = m;
+    }
+    if (src != x) memcpy(x, src, sizeof(cplx) * (size_t)f->n * V);
+}
+
+#define VB 16 /* lines transformed together when the batch index is contiguous */
+
+/* FFTW plan_many semantics on interleaved complex data */
+static void fft_many(const fplan *f, cplx *data, long howmany, long stride, long dist, int sign, int nthreads) {
+    const int n = f->n;
+    if (dist == 1 && stride >= VB) {
+        long nblk = (howmany + VB - 1) / VB;
+#pragma omp parallel num_threads(nthreads)
+        {
+            cplx *x = (cplx *)malloc(sizeof(cplx) * (size_t)n * VB);
+            cplx *y = (cplx *)malloc(sizeof(cplx) * (size_t)n * VB);
+#pragma omp for schedule(static)
+            for (long blk = 0; blk < nblk; blk++) {
+                long b0 = blk * VB;
+                int V = (int)((howmany - b0) < VB ? (howmany - b0) : VB);
+                for (int e = 0; e < n; e++)
+                    memcpy(&x[(size_t)e * V], &data[b0 + e * stride], sizeof(cplx) * (size_t)V);
+                stockham(f, x, y, V, sign);
+                for (int e = 0; e < n; e++)
+                    memcpy(&data[b0 + e * stride], &x[(size_t)e * V], sizeof(cplx) * (size_t)V);
+            }
+            free(x);
+            free(y);
+        }
+    } else {
+#pragma omp parallel num_threads(nthreads)
+        {
+            cplx *x = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+            cplx *y = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+#pragma omp for schedule(static)
+            for (long b = 0; b < howmany; b++) {
+                for (int e = 0; e < n; e++) x[e] = data[b * dist + e * stride];
+                stockham(f, x, y, 1, sign);
+                for (int e = 0; e < n; e++) data[b * dist + e * stride] = x[e];
+            }
+            free(x);
+            free(y);
+        }
+    }
+}
+
+int ora_fft_many(int n, int howmany, double *data, long stride, long dist, int sign) {
+    if (n <= 0 || howmany < 0) return -1;
+    fplan f;
+    fplan_init(&f, n);
+    fft_many(&f, (cplx *)data, howmany, stride, dist, sign, 1);
+    fplan_free(&f);
+    return 0;
+}
+
+/* -------------------------------------------------------------------------
+ * LAPACK zgtsv with real-valued DL, D, DU (the reference's Dyy-derived
+ * coefficients are real; their complex storage has zero imaginary parts, so
+ * CABS1 reduces to |x| and every coefficient update stays real).
+ * ------------------------------------------------------------------------- */
+static int zgtsv_real_coef(int n, double *dl, double *d, double *du, cplx *b) {
+    if (n == 0) return 0;
+    for (int k = 0; k < n - 1; k++) {
+        if (dl[k] == 0.0) {
+            if (d[k] == 0.0) return k + 1;
+        } else if (fabs(d[k]) >= fabs(dl[k])) {
+            /* no row interchange */
+            double mult = dl[k] / d[k];
+            d[k + 1] = d[k + 1] - mult * du[k];
+            b[k + 1].re = b[k + 1].re - mult * b[k].re;
+            b[k + 1].im = b[k + 1].im - mult * b[k].im;
+            if (k < n - 2) dl[k] = 0.0;
+        } else {
+            /* interchange rows k and k+1 */
+            double mult = d[k] / dl[k];
+            d[k] = dl[k];
+            double temp = d[k + 1];
+            d[k + 1] = du[k] - mult * temp;
+            if (k < n - 2) {
+                dl[k] = du[k + 1];
+                du[k + 1] = -mult * dl[k];
+            }
+            du[k] = temp;
+            cplx tb = b[k];
+            b[k] = b[k + 1];
+            b[k + 1].re = tb.re - mult * b[k + 1].re;
+            b[k + 1].im = tb.im - mult * b[k + 1].im;
+        }
+    }
+    if (d[n - 1] == 0.0) return n;
+    b[n - 1].re = b[n - 1].re / d[n - 1];
+    b[n - 1].im = b[n - 1].im / d[n - 1];
+    if (n > 1) {
+        b[n - 2].re = (b[n - 2].re - du[n - 2] * b[n - 1].re) / d[n - 2];
+        b[n - 2].im = (b[n - 2].im - du[n - 2] * b[n - 1].im) / d[n - 2];
+    }
+    for (int k = n - 3; k >= 0; k--) {
+        b[k].re = (b[k].re - du[k] * b[k + 1].re - dl[k] * b[k + 2].re) / d[k];
+        b[k].im = (b[k].im - du[k] * b[k + 1].im - dl[k] * b[k + 2].im) / d[k];
+    }
+    return 0;
+}
+
+int ora_zgtsv(int n, double *dl, double *d, double *du, double *b) {
+    return zgtsv_real_coef(n, dl, d, du, (cplx *)b);
+}
+
+/* -------------------------------------------------------------------------
+ * grid (genGridandIC.f90:22-61, initialization.f90:243-281, :326-327)
+ * ------------------------------------------------------------------------- */
+static void centres_with_ghosts(int n, const double *f, double *g) {
+    /* initialization.f90:249-272 */
+    for (int i = 0; i < n - 1; i++) g[i + 1] = 0.5 * (f[i] + f[i + 1]);
+    g[0] = g[1] - 2.0 * (g[1] - f[0]);
+    g[n] = g[n - 1] + 2.0 * (f[n - 1] - g[n - 1]);
+}
+
+void ora_channel_grid(int nxp, int nym, int nzp, double Lx_i, double Ly_i, double Lz_i,
+                      double alpha, int grid_type, double *x, double *y, double *z,
+                      double *xg, double *yg, double *zg, double *dx, double *dz) {
+    int nx = nxp + 2, ny = nym + 1, nz = nzp + 2;
+    for (int i = 1; i <= nx; i++) x[i - 1] = Lx_i * (double)(i - 1) / (double)(nx - 1);
+    for (int i = 1; i <= nz; i++) z[i - 1] = Lz_i * (double)(i - 1) / (double)(nz - 1);
+    if (grid_type == 1) {
+        for (int i = 1; i <= ny; i++) y[i - 1] = Ly_i * (double)(i - 1) / (double)(ny - 1);
+    } else {
+        double th = tanh(0.5 * alpha);
+        for (int i = 1; i <= ny; i++) {
+            double eta = (double)(i - 1) / (double)(ny - 1);
+            y[i - 1] = Ly_i * 0.5 * (1.0 + tanh(alpha * (eta - 0.5)) / th);
+        }
+    }
+    centres_with_ghosts(nx, x, xg);
+    centres_with_ghosts(ny, y, yg);
+    centres_with_ghosts(nz, z, zg);
+    double mx = xg[1] - xg[0], mz = zg[1] - zg[0];
+    for (int i = 1; i < nx; i++) if (xg[i + 1] - xg[i] < mx) mx = xg[i + 1] - xg[i];
+    for (int i = 1; i < nz; i++) if (zg[i + 1] - zg[i] < mz) mz = zg[i + 1] - zg[i];
+    *dx = mx;
+    *dz = mz;
+}
+
+/* -------------------------------------------------------------------------
+ * plan
+ * ------------------------------------------------------------------------- */
+struct ora_plan {
+    int nxp, nym, nzp;
+    int nx, ny, nz, nxg, nyg, nzg;
+    double dx, dz;
+    double *x, *y, *z, *yg, *zg;
+    double *kxx, *kzz;
+    double *dyy_l, *dyy_d, *dyy_u; /* Dyy(j+1,j), Dyy(j,j), Dyy(j,j+1) */
+    fplan fx, fz;
+    cplx *work; /* poisson_y_c at np = 1: nxp*nym*nzp */
+    int nthreads;
+};
+
+static double *dup_array(const double *a, int n) {
+    double *r = (double *)malloc(sizeof(double) * (size_t)n);
+    memcpy(r, a, sizeof(double) * (size_t)n);
+    return r;
+}
+
+static void wavenumbers(int n, double d, double *k) {
+    /* initialization.f90:420-425 / :428-433 */
+    const double pi = 4.0 * atan(1.0);
+    int half = (int)ceil((double)(float)n / 2.0);
+    for (int i = 0; i < n; i++) k[i] = 0.0;
+    for (int i = 0; i <= half && i <= n - 1; i++) k[i] = 2.0 * (cos(2.0 * pi * (double)i / (double)n) - 1.0) / pow(d, 2.0);
+    for (int i = half + 1; i <= n - 1; i++) k[i] = 2.0 * (cos(2.0 * pi * (double)(-n + i) / (double)n) - 1.0) / pow(d, 2.0);
+}
+
+ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
+    if (g->nxp < 1 || g->nym < 1 || g->nzp < 1) return NULL;
+    ora_plan *p = (ora_plan *)calloc(1, sizeof(ora_plan));
+    p->nxp = g->nxp; p->nym = g->nym; p->nzp = g->nzp;
+    p->nx = g->nxp + 2; p->ny = g->nym + 1; p->nz = g->nzp + 2;
+    p->nxg = g->nxp + 3; p->nyg = g->nym + 2; p->nzg = g->nzp + 3;
+    p->dx = g->dx; p->dz = g->dz;
+    p->x = dup_array(g->x, p->nx);
+    p->y = dup_array(g->y, p->ny);
+    p->z = dup_array(g->z, p->nz);
+    p->yg = dup_array(g->yg, p->nyg);
+    p->zg = dup_array(g->zg, p->nzg);
+    p->kxx = (double *)malloc(sizeof(double) * (size_t)p->nxp);
+    p->kzz = (double *)malloc(sizeof(double) * (size_t)p->nzp);
+    wavenumbers(p->nxp, p->dx, p->kxx);
+    wavenumbers(p->nzp, p->dz, p->kzz);
+    /* Dyy, initialization.f90:506-535; index jj = j-2 for reference row j */
+    int nym = p->nym, nyg = p->nyg;
+    p->dyy_l = (double *)calloc((size_t)(nym > 1 ? nym - 1 : 1), sizeof(double));
+    p->dyy_u = (double *)calloc((size_t)(nym > 1 ? nym - 1 : 1), sizeof(double));
+    p->dyy_d = (double *)calloc((size_t)nym, sizeof(double));
+    const double *y = p->y, *yg = p->yg;
+#define YF(j) y[(j) - 1]
+#define YG(j) yg[(j) - 1]
+    for (int j = 2; j <= nyg - 1; j++) {
+        double a = 1.0 / (YF(j) - YF(j - 1)) / (YG(j + 1) - YG(j));
+        double b = 1.0 / (YF(j) - YF(j - 1)) * (-1.0 / (YG(j + 1) - YG(j)) - 1.0 / (YG(j) - YG(j - 1)));
+        double c = 1.0 / (YF(j) - YF(j - 1)) / (YG(j) - YG(j - 1));
+        int jj = j - 2;
+        if (j == 2 && j == nyg - 1) {
+            p->dyy_d[jj] = b + c + a; /* one-cell column: both wall rows (not used by any config) */
+        } else if (j == 2) {
+            p->dyy_d[jj] = b + c;
+            p->dyy_u[jj] = a;
+        } else if (j == nyg - 1) {
+            p->dyy_d[jj] = a + b;
+            p->dyy_l[jj - 1] = c;
+        } else {
+            p->dyy_u[jj] = a;
+            p->dyy_l[jj - 1] = c;
+            p->dyy_d[jj] = b;
+        }
+    }
+#undef YF
+#undef YG
+    fplan_init(&p->fx, p->nxp);
+    fplan_init(&p->fz, p->nzp);
+    p->work = (cplx *)malloc(sizeof(cplx) * (size_t)p->nxp * p->nym * p->nzp);
+#ifdef _OPENMP
+    p->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads();
+#else
+    p->nthreads = 1;
+#endif
+    return p;
+}
+
+void ora_plan_destroy(ora_plan *p) {
+    if (!p) return;
+    free(p->x); free(p->y); free(p->z); free(p->yg); free(p->zg);
+    free(p->kxx); free(p->kzz);
+    free(p->dyy_l); free(p->dyy_d); free(p->dyy_u);
+    fplan_free(&p->fx); fplan_free(&p->fz);
+    free(p->work);
+    free(p);
+}
+
+int ora_threads(const ora_plan *p) { return p->nthreads; }
+
+void ora_operators(const ora_plan *p, double *kxx, double *kzz, double *dl, double *d, double *du) {
+    if (kxx) memcpy(kxx, p->kxx, sizeof(double) * (size_t)p->nxp);
+    if (kzz) memcpy(kzz, p->kzz, sizeof(double) * (size_t)p->nzp);
+    if (d) memcpy(d, p->dyy_d, sizeof(double) * (size_t)p->nym);
+    if (p->nym > 1) {
+        if (dl) memcpy(dl, p->dyy_l, sizeof(double) * (size_t)(p->nym - 1));
+        if (du) memcpy(du, p->dyy_u, sizeof(double) * (size_t)(p->nym - 1));
+    }
+}
+
+/* Fortran-order element addresses (1-based reference indices) */
+#define IDX3(a, n1, n2, i, j, k) ((a)[((size_t)((k) - 1) * (size_t)(n2) + (size_t)((j) - 1)) * (size_t)(n1) + (size_t)((i) - 1)])
+#define U_(i, j, k) IDX3(U, p->nx, p->nyg, i, j, k)
+#define V_(i, j, k) IDX3(V, p->nxg, p->ny, i, j, k)
+#define W_(i, j, k) IDX3(W, p->nxg, p->nyg, i, j, k)
+#define P_(i, j, k) IDX3(P, p->nxg, p->nyg, i, j, k)
+/* rhs_p(2:nxg, 2:nyg-1, 2:nzg) */
+#define R_(i, j, k) IDX3(rhs_p, p->nxg - 1, p->nyg - 2, (i) - 1, (j) - 1, (k) - 1)
+
+void ora_divergence(const ora_plan *p, const double *U, const double *V, const double *W, double *rhs_p) {
+    /* projection.f90:42-61 */
+    const double inv_dx_p = 1.0 / p->dx;
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+    for (int k = 2; k <= p->nzg - 1; k++) {
+        for (int j = 2; j <= p->nyg - 1; j++) {
+            double inv_dzk = 1.0 / (p->z[k - 1] - p->z[k - 2]);
+            double inv_dyj = 1.0 / (p->y[j - 1] - p->y[j - 2]);
+            for (int i = 2; i <= p->nxg - 1; i++) {
+                R_(i, j, k) = (U_(i, j, k) - U_(i - 1, j, k)) * inv_dx_p +
+                              (V_(i, j, k) - V_(i, j - 1, k)) * inv_dyj +
+                              (W_(i, j, k) - W_(i, j, k - 1)) * inv_dzk;
+            }
+        }
+    }
+}
+
+static void apply_periodic_xz_pressure(const ora_plan *p, double *rhs_p) {
+    /* projection.f90:453-456 and :479-482 (single rank) */
+    for (int k = 2; k <= p->nzg; k++)
+        for (int j = 2; j <= p->nyg - 1; j++) R_(p->nxg - 1, j, k) = R_(2, j, k);
+    for (int j = 2; j <= p->nyg - 1; j++)
+        for (int i = 2; i <= p->nxg - 1; i++) R_(i, j, p->nzp + 2) = R_(i, j, 2);
+}
+
+static void finish_pressure_ghosts(const ora_plan *p, double *P) {
+    /* projection.f90:396-410, boundary_conditions.f90:283-285 and :597-599 (id = 4) */
+    int nxg = p->nxg, nyg = p->nyg, nzg = p->nzg;
+    for (int k = 1; k <= nzg; k++)
+        for (int i = 1; i <= nxg; i++) {
+            P_(i, 1, k) = P_(i, 2, k);
+            P_(i, nyg, k) = P_(i, nyg - 1, k);
+        }
+    for (int k = 1; k <= nzg; k++)
+        for (int j = 1; j <= nyg; j++) {
+            P_(1, j, k) = P_(nxg - 2, j, k);
+            P_(nxg - 1, j, k) = P_(2, j, k);
+            P_(nxg, j, k) = P_(3, j, k);
+        }
+    for (int j = 1; j <= nyg; j++)
+        for (int i = 1; i <= nxg; i++) {
+            P_(i, j, 1) = P_(i, j, nzg - 2);
+            P_(i, j, nzg - 1) = P_(i, j, 2);
+            P_(i, j, nzg) = P_(i, j, 3);
+        }
+}
+
+int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
+    const int nxp = p->nxp, nym = p->nym, nzp = p->nzp;
+    cplx *wk = p->work;
+    /* :121 poisson_y_r = rhs_p(2:nxp+1, 2:nym+1, 2:nzp+1); :124 dcmplx */
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+    for (int k = 0; k < nzp; k++)
+        for (int j = 0; j < nym; j++)
+            for (int i = 0; i < nxp; i++) {
+                cplx v = {R_(i + 2, j + 2, k + 2), 0.0};
+                wk[((size_t)k * nym + j) * nxp + i] = v;
+            }
+    /* :125 C2C forward along x, batch nym*nzp, contiguous lines */
+    fft_many(&p->fx, wk, (long)nym * nzp, 1, nxp, -1, p->nthreads);
+    /* :134-136 C2C forward along z, stride nxp*nym, batch nxp*nym */
+    fft_many(&p->fz, wk, (long)nxp * nym, (long)nxp * nym, 1, -1, p->nthreads);
+    /* :277-297 one Zgtsv per (i,k) mode */
+    int info_all = 0;
+#pragma omp parallel num_threads(p->nthreads)
+    {
+        double *DL = (double *)malloc(sizeof(double) * (size_t)(nym > 1 ? nym - 1 : 1));
+        double *DU = (double *)malloc(sizeof(double) * (size_t)(nym > 1 ? nym - 1 : 1));
+        double *D = (double *)malloc(sizeof(double) * (size_t)nym);
+        cplx *b = (cplx *)malloc(sizeof(cplx) * (size_t)nym);
+#pragma omp for schedule(static)
+        for (int k = 0; k < nzp; k++) {
+            for (int i = 0; i < nxp; i++) {
+                double wavenum_sum = p->kxx[i] + p->kzz[k];
+                for (int j = 0; j < nym - 1; j++) {
+                    DL[j] = p->dyy_l[j];
+                    DU[j] = p->dyy_u[j];
+                }
+                for (int j = 0; j < nym; j++) D[j] = p->dyy_d[j] + wavenum_sum;
+                if (i == 0 && k == 0) D[0] = 3.0 / 2.0 * D[0];
+                cplx *line = wk + (size_t)k * nym * nxp + i;
+                for (int j = 0; j < nym; j++) b[j] = line[(size_t)j * nxp];
+                int info = zgtsv_real_coef(nym, DL, D, DU, b);
+                if (info != 0) {
+#pragma omp atomic write
+                    info_all = info;
+                }
+                for (int j = 0; j < nym; j++) line[(size_t)j * nxp] = b[j];
+            }
+        }
+        free(DL); free(DU); free(D); free(b);
+    }
+    /* :308-310 inverse z; :325 inverse x */
+    fft_many(&p->fz, wk, (long)nxp * nym, (long)nxp * nym, 1, +1, p->nthreads);
+    fft_many(&p->fx, wk, (long)nym * nzp, 1, nxp, +1, p->nthreads);
+    /* :327 Real()/(nxp*nzp); :342 back into rhs_p */
+    const double norm = (double)((long)nxp * nzp);
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+    for (int k = 0; k < nzp; k++)
+        for (int j = 0; j < nym; j++)
+            for (int i = 0; i < nxp; i++) R_(i + 2, j + 2, k + 2) = wk[((size_t)k * nym + j) * nxp + i].re / norm;
+    apply_periodic_xz_pressure(p, rhs_p);
+    /* :357-358 single rank: no interior halo neighbours */
+    if (P) {
+        /* :377 */
+        const double den = dt * c;
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+        for (int k = 2; k <= p->nzg - 1; k++)
+            for (int j = 2; j <= p->nyg - 1; j++)
+                for (int i = 2; i <= p->nxg - 1; i++) P_(i, j, k) = R_(i, j, k) / den;
+        finish_pressure_ghosts(p, P);
+    }
+    return info_all ? -100 - info_all : 0;
+}
+
+void ora_correct(const ora_plan *p, double *U, double *V, double *W, const double *rhs_p) {
+    /* projection.f90:558-601, one rank (the seam face nx is the domain edge) */
+    const double inv_dx = 1.0 / p->dx;
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+    for (int k = 2; k <= p->nzg - 1; k++)
+        for (int j = 2; j <= p->nyg - 1; j++)
+            for (int i = 2; i <= p->nx - 1; i++) U_(i, j, k) = U_(i, j, k) - (R_(i + 1, j, k) - R_(i, j, k)) * inv_dx;
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+    for (int k = 2; k <= p->nzg - 1; k++)
+        for (int j = 2; j <= p->ny - 1; j++) {
+            double inv_dygj = 1.0 / (p->yg[j] - p->yg[j - 1]);
+            for (int i = 2; i <= p->nxg - 1; i++) V_(i, j, k) = V_(i, j, k) - (R_(i, j + 1, k) - R_(i, j, k)) * inv_dygj;
+        }
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+    for (int k = 2; k <= p->nz - 1; k++) {
+        double inv_dzgk = 1.0 / (p->zg[k] - p->zg[k - 1]);
+        for (int j = 2; j <= p->nyg - 1; j++)
+            for (int i = 2; i <= p->nxg - 1; i++) W_(i, j, k) = W_(i, j, k) - (R_(i, j, k + 1) - R_(i, j, k)) * inv_dzgk;
+    }
+}
+
+int ora_project(ora_plan *p, double *U, double *V, double *W, double *rhs_p, double *P, double dt, double c) {
+    ora_divergence(p, U, V, W, rhs_p);
+    int rc = ora_solve(p, rhs_p, P, dt, c);
+    ora_correct(p, U, V, W, rhs_p);
+    return rc;
+}
+
+double ora_max_divergence(const ora_plan *p, const double *U, const double *V, const double *W) {
+    double div = 0.0;
+#pragma omp parallel for schedule(static) reduction(max : div) num_threads(p->nthreads)
+    for (int k = 2; k <= p->nzg - 1; k++)
+        for (int j = 2; j <= p->nyg - 1; j++)
+            for (int i = 2; i <= p->nxg - 1; i++) {
+                double d = fabs((U_(i, j, k) - U_(i - 1, j, k)) / (p->x[i - 1] - p->x[i - 2]) +
+                                (V_(i, j, k) - V_(i, j - 1, k)) / (p->y[j - 1] - p->y[j - 2]) +
+                                (W_(i, j, k) - W_(i, j, k - 1)) / (p->z[k - 1] - p->z[k - 2]));
+                if (d > div) div = d;
+            }
+    return div;
+}

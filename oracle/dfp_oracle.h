@@ -1,0 +1,77 @@
+/*
+ * C restatement of dopamine-fdm's channel pressure projection (the oracle).
+ *
+ * TEST INFRASTRUCTURE ONLY: the product never links or calls this library.
+ * tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use it as
+ * the checker and as the host-CPU baseline.
+ *
+ * It restates the reference's CPU (FFTW + LAPACK) branch of the projection
+ * for the channel case (x periodic, y walls, z periodic) on one rank; every
+ * function cites the reference lines it follows in dfp_oracle.c.
+ *
+ * Arrays are the reference's Fortran arrays by address (column-major, x
+ * fastest):
+ *   U(nx,nyg,nzg)  V(nxg,ny,nzg)  W(nxg,nyg,nz)  P(nxg,nyg,nzg)
+ *   rhs_p(2:nxg, 2:nyg-1, 2:nzg)  -> extents (nxg-1, nyg-2, nzg-1)
+ * with nx=nxp+2, ny=nym+1, nz=nzp+2, nxg=nxp+3, nyg=nym+2, nzg=nzp+3.
+ */
+#ifndef DFP_ORACLE_H
+#define DFP_ORACLE_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_grid {
+    int nxp, nym, nzp;
+    double dx, dz;        /* dxmin, dzmin (initialization.f90:279-281, 326-327) */
+    const double *x;      /* faces  x(1:nx)   */
+    const double *y;      /* faces  y(1:ny)   */
+    const double *z;      /* faces  z(1:nz)   */
+    const double *yg;     /* centres yg(1:nyg) */
+    const double *zg;     /* centres zg(1:nzg) */
+} ora_grid;
+
+typedef struct ora_plan ora_plan;
+
+/* Grid of genGridandIC.f90:22-61 + initialization.f90:243-281.  Output
+ * arrays: x[nx], y[ny], z[nz], xg[nxg], yg[nyg], zg[nzg]; lengths are the
+ * namelist (face-to-face) lengths. */
+void ora_channel_grid(int nxp, int nym, int nzp, double Lx_i, double Ly_i, double Lz_i,
+                      double alpha, int grid_type, double *x, double *y, double *z,
+                      double *xg, double *yg, double *zg, double *dx, double *dz);
+
+ora_plan *ora_plan_create(const ora_grid *g, int nthreads);
+void ora_plan_destroy(ora_plan *p);
+
+/* operators: kxx[nxp], kzz[nzp]; Dyy tridiagonal as lower[nym-1], diag[nym], upper[nym-1] */
+void ora_operators(const ora_plan *p, double *kxx, double *kzz, double *dl, double *d, double *du);
+
+/* projection.f90:37-63 */
+void ora_divergence(const ora_plan *p, const double *U, const double *V, const double *W, double *rhs_p);
+/* projection.f90:100-382 CPU branch; P may be NULL (no pressure save) */
+int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c);
+/* projection.f90:551-604 */
+void ora_correct(const ora_plan *p, double *U, double *V, double *W, const double *rhs_p);
+/* projection.f90:21-34 */
+int ora_project(ora_plan *p, double *U, double *V, double *W, double *rhs_p, double *P,
+                double dt, double c);
+/* projection.f90:607-639 (no IBM) */
+double ora_max_divergence(const ora_plan *p, const double *U, const double *V, const double *W);
+
+/* LAPACK zgtsv on one system, complex stored as interleaved (re,im) doubles;
+ * dl, d, du are real here (they are real-valued in the reference) and are
+ * overwritten like zgtsv's; returns info. */
+int ora_zgtsv(int n, double *dl, double *d, double *du, double *b);
+
+/* batched unnormalised complex FFT, sign -1 forward / +1 backward, FFTW
+ * plan_many semantics (interleaved complex doubles) */
+int ora_fft_many(int n, int howmany, double *data, long stride, long dist, int sign);
+
+/* threads the plan's loops use (OpenMP); returns the count in effect */
+int ora_threads(const ora_plan *p);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

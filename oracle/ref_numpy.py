@@ -1,0 +1,494 @@
+"""Numpy restatement of dopamine-fdm's channel pressure projection (the oracle).
+
+TEST INFRASTRUCTURE ONLY.  Nothing in the product imports this module: only
+``tests/``, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+``bench.py`` may use it, and only as the checker.
+
+It restates, operation by operation, the reference's CPU (FFTW + LAPACK)
+branch of the projection for the channel case (x periodic, y walls, z
+periodic: ``x_bc_type=0, y_bc_type=1, z_bc_type=0``) on one rank:
+
+* grid and metrics ............ src/genGridandIC.f90:22-61,
+                                src/initialization.f90:243-281, :326-345
+* modified wavenumbers ........ src/initialization.f90:416-434
+* wall-normal operator Dyy .... src/initialization.f90:506-535
+* divergence (rhs_p) .......... src/projection.f90:37-63
+* Poisson solve, CPU branch ... src/projection.f90:100-382 (``#else`` parts):
+    x C2C FFT of the real data :121-125, z C2C FFT :132-136,
+    per-(kx,kz) Zgtsv with the (0,0) pin :275-297,
+    inverse z :306-310, inverse x + Real()/(nxp*nzp) :322-342
+* periodic duplicate fill ..... src/projection.f90:441-500 (single rank)
+* pressure save + ghosts ...... src/projection.f90:361-380, :388-438,
+                                src/boundary_conditions.f90:263-289, :580-602
+* velocity correction ......... src/projection.f90:551-604 (single rank)
+* max divergence .............. src/projection.f90:606-639 (no IBM)
+
+LAPACK ``zgtsv`` (unpinned LAPACK version, CMakeLists.txt:50-51; the routine's
+algorithm is unchanged since LAPACK 3.0) is restated from its published
+algorithm: Gaussian elimination with partial pivoting on the tridiagonal,
+then back substitution with the second superdiagonal created by row
+interchanges.  FFTW3 (``fftw_plan_many_dft``, FFTW_FORWARD = e^{-i}) is
+represented by ``numpy.fft`` (pocketfft) with the same unnormalised
+conventions; FFT roundoff differs from FFTW's at the 1e-16 level.
+
+Array conventions: every 3-D array is a numpy array in Fortran order whose
+element ``[i-1, j-1, k-1]`` is the reference's ``A(i,j,k)`` (1-based).  The
+reference's ``rhs_p(2:nxg, 2:nyg-1, 2:nzg)`` is stored with element
+``[i-2, j-2, k-2]``.
+
+How each part is pinned (tests/test_oracle_numpy.py): the grid and Dyy by
+their closed forms and by symmetry of W*Dyy and zero row sums; kxx/kzz by the
+eigenvalue identity of the periodic 3-point Laplacian; the FFT stages by a
+naive O(n^2) DFT; ``zgtsv`` by dense ``numpy.linalg.solve`` and by a
+case forced through the row-interchange branch; the whole solve by a dense
+solve of the assembled 7-point operator with the pin on small grids; the
+projection by a manufactured solution u* = curl_h(A) + grad_h(phi_true) and
+by idempotence.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# global.f90:21  pi = 4d0*datan(1d0)
+PI = 4.0 * math.atan(1.0)
+
+# initialization.f90:654-660 (diagonal of the low-storage RK3 table)
+RK_COEF_DIAG = (8.0 / 15.0, 5.0 / 12.0, 3.0 / 4.0)
+
+
+# ---------------------------------------------------------------------------
+# grid
+# ---------------------------------------------------------------------------
+@dataclass
+class ChannelGrid:
+    """Grid and metrics of the channel case, built as the reference builds them.
+
+    ``nxp, nym, nzp`` are the Fourier/cell counts; the reference namelist has
+    ``nx = nxp+2, ny = nym+1, nz = nzp+2`` face points
+    (src/initialization.f90:170-191, :331-345).  ``Lx_i, Ly_i, Lz_i`` are the
+    namelist lengths (face-to-face, including the duplicate periodic cell).
+    """
+
+    nxp: int
+    nym: int
+    nzp: int
+    Lx_i: float
+    Ly_i: float
+    Lz_i: float
+    alpha: float
+    grid_type: int = 2
+    x: np.ndarray = field(init=False)
+    y: np.ndarray = field(init=False)
+    z: np.ndarray = field(init=False)
+    xg: np.ndarray = field(init=False)
+    yg: np.ndarray = field(init=False)
+    zg: np.ndarray = field(init=False)
+    dx: float = field(init=False)
+    dz: float = field(init=False)
+
+    def __post_init__(self):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        # genGridandIC.f90:23-25, :38-40  x(i) = Lx*(i-1)/(nx-1)
+        self.x = np.array([self.Lx_i * float(i - 1) / float(nx - 1) for i in range(1, nx + 1)])
+        self.z = np.array([self.Lz_i * float(i - 1) / float(nz - 1) for i in range(1, nz + 1)])
+        if self.grid_type == 1:
+            # genGridandIC.f90:48-52
+            self.y = np.array([self.Ly_i * float(i - 1) / float(ny - 1) for i in range(1, ny + 1)])
+        elif self.grid_type == 2:
+            # genGridandIC.f90:54-61 symmetric tanh
+            ys = []
+            t_half = math.tanh(0.5 * self.alpha)
+            for i in range(1, ny + 1):
+                eta = float(i - 1) / float(ny - 1)
+                ys.append(self.Ly_i * 0.5 * (1.0 + math.tanh(self.alpha * (eta - 0.5)) / t_half))
+            self.y = np.array(ys)
+        else:
+            raise ValueError("grid_type must be 1 or 2")
+        self.xg = _centres_with_ghosts(self.x)
+        self.yg = _centres_with_ghosts(self.y)
+        self.zg = _centres_with_ghosts(self.z)
+        # initialization.f90:279-281 and :326-327  dx = dxmin, dz = dzmin
+        self.dx = float(np.min(self.xg[1:] - self.xg[:-1]))
+        self.dz = float(np.min(self.zg[1:] - self.zg[:-1]))
+
+    # face counts and ghosted cell counts (initialization.f90:170-191)
+    @property
+    def nx(self):
+        return self.nxp + 2
+
+    @property
+    def ny(self):
+        return self.nym + 1
+
+    @property
+    def nz(self):
+        return self.nzp + 2
+
+    @property
+    def nxg(self):
+        return self.nxp + 3
+
+    @property
+    def nyg(self):
+        return self.nym + 2
+
+    @property
+    def nzg(self):
+        return self.nzp + 3
+
+    # array shapes of the field contract (initialization.f90:204-220)
+    @property
+    def shape_u(self):
+        return (self.nx, self.nyg, self.nzg)
+
+    @property
+    def shape_v(self):
+        return (self.nxg, self.ny, self.nzg)
+
+    @property
+    def shape_w(self):
+        return (self.nxg, self.nyg, self.nz)
+
+    @property
+    def shape_p(self):
+        return (self.nxg, self.nyg, self.nzg)
+
+    @property
+    def shape_rhs(self):
+        # rhs_p(2:nxg, 2:nyg-1, 2:nzg)
+        return (self.nxg - 1, self.nyg - 2, self.nzg - 1)
+
+
+def _centres_with_ghosts(f: np.ndarray) -> np.ndarray:
+    """initialization.f90:249-272: centres plus one mirrored ghost centre per side."""
+    n = f.size
+    fm = np.array([0.5 * (f[i] + f[i + 1]) for i in range(n - 1)])
+    g = np.empty(n + 1)
+    g[1:n] = fm
+    g[0] = fm[0] - 2.0 * (fm[0] - f[0])
+    g[n] = fm[n - 2] + 2.0 * (f[n - 1] - fm[n - 2])
+    return g
+
+
+def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * PI, Lz=PI, Ly=2.0, grid_type=2):
+    """Channel grid whose periodic lengths are ``Lx`` and ``Lz``.
+
+    The reference's periodic period is ``Lx_i - dx`` (initialization.f90:332),
+    so the namelist length is ``Lx_i = Lx*(nxp+1)/nxp``.
+    """
+    return ChannelGrid(nxp, nym, nzp, Lx * (nxp + 1) / nxp, Ly, Lz * (nzp + 1) / nzp, alpha, grid_type)
+
+
+def namelist_grid(nx, ny, nz, Lx, Ly, Lz, alpha, grid_type=2):
+    """Grid from the reference's own namelist values (face counts and lengths)."""
+    return ChannelGrid(nx - 2, ny - 1, nz - 2, Lx, Ly, Lz, alpha, grid_type)
+
+
+# ---------------------------------------------------------------------------
+# operators
+# ---------------------------------------------------------------------------
+def wavenumbers(n: int, d: float) -> np.ndarray:
+    """initialization.f90:420-425 (kxx) and :428-433 (kzz), periodic case."""
+    k = np.zeros(n)
+    half = math.ceil(float(n) / 2.0)
+    for i in range(0, half + 1):
+        if i <= n - 1:
+            k[i] = 2.0 * (math.cos(2.0 * PI * float(i) / float(n)) - 1.0) / d ** 2.0
+    for i in range(half + 1, n):
+        k[i] = 2.0 * (math.cos(2.0 * PI * float(-n + i) / float(n)) - 1.0) / d ** 2.0
+    return k
+
+
+def dyy_matrix(y: np.ndarray, yg: np.ndarray) -> np.ndarray:
+    """initialization.f90:506-535.  Returns Dyy(2:nyg-1, 2:nyg-1) as a dense
+    (nym, nym) real array: element [j-2, jj-2] is Dyy(j, jj)."""
+    nyg = yg.size
+    nym = nyg - 2
+    D = np.zeros((nym, nym))
+
+    def abc(j):  # j is the 1-based reference index
+        a = 1.0 / (y[j - 1] - y[j - 2]) / (yg[j] - yg[j - 1])
+        b = 1.0 / (y[j - 1] - y[j - 2]) * (-1.0 / (yg[j] - yg[j - 1]) - 1.0 / (yg[j - 1] - yg[j - 2]))
+        c = 1.0 / (y[j - 1] - y[j - 2]) / (yg[j - 1] - yg[j - 2])
+        return a, b, c
+
+    for j in range(3, nyg - 1):
+        a, b, c = abc(j)
+        D[j - 2, j - 1] = a
+        D[j - 2, j - 3] = c
+        D[j - 2, j - 2] = b
+    j = 2
+    a, b, c = abc(j)
+    D[0, 0] = b + c
+    if nym > 1:
+        D[0, 1] = a
+    j = nyg - 1
+    a, b, c = abc(j)
+    D[nym - 1, nym - 1] = a + b
+    if nym > 1:
+        D[nym - 1, nym - 2] = c
+    return D
+
+
+# ---------------------------------------------------------------------------
+# LAPACK zgtsv (published algorithm), vectorised over independent systems
+# ---------------------------------------------------------------------------
+def zgtsv(dl, d, du, b):
+    """Solve batches of tridiagonal systems exactly as LAPACK ``zgtsv`` does.
+
+    ``dl, d, du`` have shape (M, n-1), (M, n), (M, n-1) and ``b`` (M, n); each
+    row is one system.  Gaussian elimination with partial pivoting (row
+    interchange when |D(k)| < |DL(k)| in the CABS1 = |re|+|im| norm), then
+    back substitution using the second superdiagonal held in DL.  Returns
+    (x, info) where info[m] is 0 or the 1-based index of a zero pivot.
+    Inputs are copied, not modified.
+    """
+    dl = np.array(dl, dtype=np.complex128, copy=True)
+    d = np.array(d, dtype=np.complex128, copy=True)
+    du = np.array(du, dtype=np.complex128, copy=True)
+    b = np.array(b, dtype=np.complex128, copy=True)
+    M, n = d.shape
+    info = np.zeros(M, dtype=np.int64)
+    if n == 0:
+        return b, info
+
+    def cabs1(z):
+        return np.abs(z.real) + np.abs(z.imag)
+
+    for k in range(n - 1):
+        dlk = dl[:, k].copy()
+        dk = d[:, k].copy()
+        zero_sub = dlk == 0
+        sing = zero_sub & (dk == 0) & (info == 0)
+        info[sing] = k + 1
+        nopiv = (~zero_sub) & (cabs1(dk) >= cabs1(dlk))
+        piv = (~zero_sub) & (~nopiv)
+        # -- no row interchange
+        if np.any(nopiv):
+            mult = np.where(nopiv, dlk / np.where(nopiv, dk, 1.0), 0.0)
+            d[:, k + 1] = np.where(nopiv, d[:, k + 1] - mult * du[:, k], d[:, k + 1])
+            b[:, k + 1] = np.where(nopiv, b[:, k + 1] - mult * b[:, k], b[:, k + 1])
+            if k < n - 2:
+                dl[:, k] = np.where(nopiv, 0.0, dl[:, k])
+        # -- interchange rows k and k+1
+        if np.any(piv):
+            mult = np.where(piv, dk / np.where(piv, dlk, 1.0), 0.0)
+            new_dk = dlk
+            temp = d[:, k + 1].copy()
+            new_dk1 = du[:, k] - mult * temp
+            if k < n - 2:
+                new_dlk = du[:, k + 1]
+                new_duk1 = -mult * new_dlk
+                dl[:, k] = np.where(piv, new_dlk, dl[:, k])
+                du[:, k + 1] = np.where(piv, new_duk1, du[:, k + 1])
+            du[:, k] = np.where(piv, temp, du[:, k])
+            d[:, k] = np.where(piv, new_dk, d[:, k])
+            d[:, k + 1] = np.where(piv, new_dk1, d[:, k + 1])
+            bk = b[:, k].copy()
+            bk1 = b[:, k + 1].copy()
+            b[:, k] = np.where(piv, bk1, bk)
+            b[:, k + 1] = np.where(piv, bk - mult * bk1, bk1)
+    sing = (d[:, n - 1] == 0) & (info == 0)
+    info[sing] = n
+    ok = info == 0
+    dsafe = np.where(d == 0, 1.0, d)
+    b[:, n - 1] = np.where(ok, b[:, n - 1] / dsafe[:, n - 1], b[:, n - 1])
+    if n > 1:
+        b[:, n - 2] = np.where(ok, (b[:, n - 2] - du[:, n - 2] * b[:, n - 1]) / dsafe[:, n - 2], b[:, n - 2])
+    for k in range(n - 3, -1, -1):
+        b[:, k] = np.where(ok, (b[:, k] - du[:, k] * b[:, k + 1] - dl[:, k] * b[:, k + 2]) / dsafe[:, k], b[:, k])
+    return b, info
+
+
+# ---------------------------------------------------------------------------
+# the projection
+# ---------------------------------------------------------------------------
+class Projection:
+    """The reference's ``projection`` module (channel case, one rank) on numpy arrays."""
+
+    def __init__(self, g: ChannelGrid):
+        self.g = g
+        self.kxx = wavenumbers(g.nxp, g.dx)
+        self.kzz = wavenumbers(g.nzp, g.dz)
+        self.Dyy = dyy_matrix(g.y, g.yg)
+
+    # -- projection.f90:37-63 ------------------------------------------------
+    def compute_pseudo_pressure_rhs(self, U, V, W, rhs_p):
+        g = self.g
+        inv_dx = 1.0 / g.dx
+        # k = 2..nzg-1, j = 2..nyg-1, i = 2..nxg-1 (1-based)
+        i = np.arange(2, g.nxg)
+        j = np.arange(2, g.nyg)
+        k = np.arange(2, g.nzg)
+        inv_dz = 1.0 / (g.z[k - 1] - g.z[k - 2])
+        inv_dy = 1.0 / (g.y[j - 1] - g.y[j - 2])
+        I, J, K = np.ix_(i - 1, j - 1, k - 1)
+        du = (U[I, J, K] - U[I - 1, J, K]) * inv_dx
+        dv = (V[I, J, K] - V[I, J - 1, K]) * inv_dy[None, :, None]
+        dw = (W[I, J, K] - W[I, J, K - 1]) * inv_dz[None, None, :]
+        rhs_p[i[0] - 2:i[-1] - 1, j[0] - 2:j[-1] - 1, k[0] - 2:k[-1] - 1] = du + dv + dw
+
+    # -- projection.f90:100-382, CPU branch ----------------------------------
+    def solve_poisson_equation(self, rhs_p, P=None, dt=None, rk_coef33=None):
+        g = self.g
+        nxp, nym, nzp = g.nxp, g.nym, g.nzp
+        # :121 copy of the interior (decomp_poisson%ysz = nxp, nym, nzp at np=1)
+        ry = np.array(rhs_p[0:nxp, 0:nym, 0:nzp], dtype=np.float64)
+        # :124-125 dcmplx + complex FFT along x (FFTW_FORWARD, unnormalised)
+        c = np.fft.fft(ry.astype(np.complex128), axis=0)
+        # :134-136 complex FFT along z
+        c = np.fft.fft(c, axis=2)
+        # :277-297 one Zgtsv per (i,k) mode
+        lam = self.kxx[:, None] + self.kzz[None, :]          # wavenum_sum, (nxp, nzp)
+        diag = np.diag(self.Dyy)                              # Dyy(j,j)
+        dl1 = np.array([self.Dyy[jj + 1, jj] for jj in range(nym - 1)])  # Dyy(j+1,j)
+        du1 = np.array([self.Dyy[jj, jj + 1] for jj in range(nym - 1)])  # Dyy(j,j+1)
+        M = nxp * nzp
+        D = diag[None, :] + lam.reshape(M, order="F")[:, None]  # (M, nym), mode m = i + nxp*k
+        D = D.copy()
+        D[0, 0] = 3.0 / 2.0 * D[0, 0]                           # :292 pin of the (0,0) mode
+        B = np.transpose(c, (0, 2, 1)).reshape(M, nym, order="F")  # B[i + nxp*k, j]
+        DL = np.broadcast_to(dl1, (M, nym - 1))
+        DU = np.broadcast_to(du1, (M, nym - 1))
+        X, info = zgtsv(DL, D, DU, B)
+        if np.any(info != 0):
+            raise FloatingPointError("zgtsv reported a zero pivot")
+        c = np.transpose(X.reshape(nxp, nzp, nym, order="F"), (0, 2, 1))
+        # :308-310 inverse z, :325-327 inverse x, Real()/(nxp*nzp)
+        c = np.fft.ifft(c, axis=2) * nzp
+        c = np.fft.ifft(c, axis=0) * nxp
+        phi = c.real / float(nxp * nzp)
+        # :342
+        rhs_p[0:nxp, 0:nym, 0:nzp] = phi
+        self.apply_periodic_xz_pressure(rhs_p)
+        # :357-358 single rank: no interior halo neighbours, nothing to exchange
+        if P is not None:
+            # :377 P = rhs_p/(dt*rk_coef(3,3)) on 2:nxg-1, 2:nyg-1, 2:nzg-1
+            P[1:g.nxg - 1, 1:g.nyg - 1, 1:g.nzg - 1] = rhs_p[0:g.nxg - 2, :, 0:g.nzg - 2] / (dt * rk_coef33)
+            self.finish_pressure_ghosts(P)
+
+    # -- projection.f90:441-500 single rank ---------------------------------
+    def apply_periodic_xz_pressure(self, rhs_p):
+        g = self.g
+        # :455 rhs_p(nxg-1,:,:) = rhs_p(2,:,:)
+        rhs_p[g.nxg - 1 - 2, :, :] = rhs_p[0, :, :]
+        # :481 rhs_p(2:nxg-1,:,nzp+2) = rhs_p(2:nxg-1,:,2)   (local nzp = nzm-1 = nzp_global)
+        rhs_p[0:g.nxg - 2, :, g.nzp + 2 - 2] = rhs_p[0:g.nxg - 2, :, 0]
+
+    # -- projection.f90:388-438 with boundary_conditions.f90:277-289, :591-602 (id=4)
+    def finish_pressure_ghosts(self, Pf):
+        g = self.g
+        Pf[:, 0, :] = Pf[:, 1, :]
+        Pf[:, g.nyg - 1, :] = Pf[:, g.nyg - 2, :]
+        # update_ghost_interior_planes(_x): no neighbours on one rank
+        Pf[0, :, :] = Pf[g.nxg - 3, :, :]
+        Pf[g.nxg - 2, :, :] = Pf[1, :, :]
+        Pf[g.nxg - 1, :, :] = Pf[2, :, :]
+        Pf[:, :, 0] = Pf[:, :, g.nzg - 3]
+        Pf[:, :, g.nzg - 2] = Pf[:, :, 1]
+        Pf[:, :, g.nzg - 1] = Pf[:, :, 2]
+
+    # -- projection.f90:551-604 single rank ---------------------------------
+    def project_velocity(self, U, V, W, rhs_p):
+        g = self.g
+        inv_dx = 1.0 / g.dx
+        nx, ny, nz, nxg, nyg, nzg = g.nx, g.ny, g.nz, g.nxg, g.nyg, g.nzg
+
+        def R(i0, i1, j0, j1, k0, k1):  # rhs_p(i0:i1, j0:j1, k0:k1), 1-based inclusive
+            return rhs_p[i0 - 2:i1 - 1, j0 - 2:j1 - 1, k0 - 2:k1 - 1]
+
+        # U, i = 2..nx-1 (the seam face nx is skipped on the last x row)
+        U[1:nx - 1, 1:nyg - 1, 1:nzg - 1] -= (R(3, nx, 2, nyg - 1, 2, nzg - 1) - R(2, nx - 1, 2, nyg - 1, 2, nzg - 1)) * inv_dx
+        # V, j = 2..ny-1, inv_dygj = 1/(yg(j+1)-yg(j))
+        jj = np.arange(2, ny)
+        inv_dyg = 1.0 / (g.yg[jj] - g.yg[jj - 1])
+        V[1:nxg - 1, 1:ny - 1, 1:nzg - 1] -= (R(2, nxg - 1, 3, ny, 2, nzg - 1) - R(2, nxg - 1, 2, ny - 1, 2, nzg - 1)) * inv_dyg[None, :, None]
+        # W, k = 2..nz-1, inv_dzgk = 1/(zg(k+1)-zg(k))
+        kk = np.arange(2, nz)
+        inv_dzg = 1.0 / (g.zg[kk] - g.zg[kk - 1])
+        W[1:nxg - 1, 1:nyg - 1, 1:nz - 1] -= (R(2, nxg - 1, 2, nyg - 1, 3, nz) - R(2, nxg - 1, 2, nyg - 1, 2, nz - 1)) * inv_dzg[None, None, :]
+
+    # -- projection.f90:21-34 -------------------------------------------------
+    def compute_projection_step(self, U, V, W, rhs_p, P=None, dt=None, rk_coef33=None):
+        self.compute_pseudo_pressure_rhs(U, V, W, rhs_p)
+        self.solve_poisson_equation(rhs_p, P, dt, rk_coef33)
+        self.project_velocity(U, V, W, rhs_p)
+
+    # -- projection.f90:607-639, ibm_input_mode = 0 ---------------------------
+    def check_divergence(self, U, V, W):
+        g = self.g
+        i = np.arange(2, g.nxg)
+        j = np.arange(2, g.nyg)
+        k = np.arange(2, g.nzg)
+        I, J, K = np.ix_(i - 1, j - 1, k - 1)
+        dxi = (g.x[i - 1] - g.x[i - 2])[:, None, None]
+        dyj = (g.y[j - 1] - g.y[j - 2])[None, :, None]
+        dzk = (g.z[k - 1] - g.z[k - 2])[None, None, :]
+        div = (U[I, J, K] - U[I - 1, J, K]) / dxi + (V[I, J, K] - V[I, J - 1, K]) / dyj + (W[I, J, K] - W[I, J, K - 1]) / dzk
+        return float(np.max(np.abs(div))) if div.size else 0.0
+
+
+def new_fields(g: ChannelGrid):
+    """Zeroed U, V, W, P (Fortran order) and rhs_p, shaped as initialization.f90:204-220."""
+    U = np.zeros(g.shape_u, order="F")
+    V = np.zeros(g.shape_v, order="F")
+    W = np.zeros(g.shape_w, order="F")
+    P = np.zeros(g.shape_p, order="F")
+    rhs = np.zeros(g.shape_rhs, order="F")
+    return U, V, W, P, rhs
+
+
+# ---------------------------------------------------------------------------
+# independent checks used by the tests
+# ---------------------------------------------------------------------------
+def dense_pinned_operator(g: ChannelGrid):
+    """The pinned discrete Poisson operator that the spectral solve inverts,
+    assembled densely in physical space on the nxp*nym*nzp periodic cells.
+
+    L = periodic 3-point second differences in x (spacing dx) and z (spacing
+    dz), plus Dyy in y; the (0,0)-mode pin (projection.f90:292) multiplies
+    the first row of the mode's Dyy by 3/2, which in physical space adds
+    0.5*Dyy(2,2)*mean_xz(phi(:,2,:)) to every cell of the first interior row.
+    Cell (i,j,k) (0-based, periodic) has index i + nxp*(j + nym*k).
+    """
+    nxp, nym, nzp = g.nxp, g.nym, g.nzp
+    n = nxp * nym * nzp
+    A = np.zeros((n, n))
+    Dyy = dyy_matrix(g.y, g.yg)
+    cx = 1.0 / g.dx ** 2
+    cz = 1.0 / g.dz ** 2
+
+    def idx(i, j, k):
+        return (i % nxp) + nxp * (j + nym * (k % nzp))
+
+    for k in range(nzp):
+        for j in range(nym):
+            for i in range(nxp):
+                r = idx(i, j, k)
+                A[r, idx(i + 1, j, k)] += cx
+                A[r, idx(i - 1, j, k)] += cx
+                A[r, r] -= 2.0 * cx
+                A[r, idx(i, j, k + 1)] += cz
+                A[r, idx(i, j, k - 1)] += cz
+                A[r, r] -= 2.0 * cz
+                for jj in range(max(0, j - 1), min(nym, j + 2)):
+                    A[r, idx(i, jj, k)] += Dyy[j, jj]
+    pin = 0.5 * Dyy[0, 0] / float(nxp * nzp)
+    rows = [idx(i, 0, k) for k in range(nzp) for i in range(nxp)]
+    for r in rows:
+        for c_ in rows:
+            A[r, c_] += pin
+    return A
+
+
+def naive_dft(a: np.ndarray, axis: int, sign: int = -1) -> np.ndarray:
+    """O(n^2) DFT with exact-angle twiddles (used to pin the FFT stages)."""
+    a = np.moveaxis(np.asarray(a, dtype=np.complex128), axis, -1)
+    n = a.shape[-1]
+    jk = np.outer(np.arange(n), np.arange(n)) % n
+    w = np.exp(sign * 2j * np.pi * jk / n)
+    out = a @ w.T
+    return np.moveaxis(out, -1, axis)

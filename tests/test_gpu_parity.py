@@ -1,0 +1,139 @@
+"""GPU parity: the HIP projection against the C oracle on identical inputs.
+
+Bars (written here, as the tests apply them):
+  * divergence, correction, max-divergence: bit-identical (same operations)
+  * phi (rhs_p), P and the corrected U, V, W: relative L-inf <= 1e-12
+  * post-projection max|div u| <= 1e-12 * max|u| / min spacing (after the
+    periodic velocity ghosts are refreshed, as the reference's monitor sees them)
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import dfp
+from oracle import ref_c
+from _cases import (SMALL_CASES, chan_les_small_grid, div_bound, fields, make_grid, oracle_project, rel_linf)
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+DT = 1e-3
+C3 = 3.0 / 4.0
+
+
+def _run_both(g, seed, sigma, save_p=True):
+    U, V, W, P, rhs = fields(g, seed, sigma)
+    o = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    oracle_project(g, *o, DT, C3, save_p=save_p)
+    plan = dfp.Plan(g)
+    d = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    plan.project(d[0], d[1], d[2], d[4], d[3] if save_p else None, DT, C3, save_p=save_p)
+    plan.close()
+    return o, d
+
+
+@pytest.mark.parametrize("name,kw,sigma", SMALL_CASES, ids=[c[0] for c in SMALL_CASES])
+def test_project_small_cases(name, kw, sigma):
+    g = make_grid(**kw)
+    print(f"case {name}", flush=True)
+    o, d = _run_both(g, 1000 + len(name), sigma)
+    for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+        err = rel_linf(a, b)
+        assert err <= TOL, f"{name} {lab}: rel-Linf {err:.3e}"
+
+
+def test_project_chan_les_small_geometry():
+    g = chan_les_small_grid()
+    o, d = _run_both(g, 7, 0.1)
+    for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+        assert rel_linf(a, b) <= TOL, lab
+
+
+def test_stages_bit_identical_and_solve():
+    g = make_grid(24, 16, 12, 1.5)
+    U, V, W, P, rhs = fields(g, 11, 0.1)
+    op = ref_c.CProjection(g)
+    plan = dfp.Plan(g)
+    # divergence (projection.f90:37-63)
+    r_o = rhs.copy(order="F"); r_d = rhs.copy(order="F")
+    op.compute_pseudo_pressure_rhs(U, V, W, r_o)
+    plan.divergence(U, V, W, r_d)
+    assert np.array_equal(r_o, r_d)
+    # solve (projection.f90:100-382) with P
+    P_o = P.copy(order="F"); P_d = P.copy(order="F")
+    op.solve_poisson_equation(r_o, P_o, DT, C3)
+    plan.solve(r_d, P_d, DT, C3, save_p=True)
+    assert rel_linf(r_d, r_o) <= TOL
+    assert rel_linf(P_d, P_o) <= TOL
+    # correction from the same phi (projection.f90:551-604)
+    Uo, Vo, Wo = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
+    Ud, Vd, Wd = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
+    op.project_velocity(Uo, Vo, Wo, r_o)
+    plan.correct(Ud, Vd, Wd, r_o)
+    assert np.array_equal(Uo, Ud) and np.array_equal(Vo, Vd) and np.array_equal(Wo, Wd)
+    # max divergence (projection.f90:607-639)
+    assert plan.max_divergence(U, V, W) == op.check_divergence(U, V, W)
+    op.close(); plan.close()
+
+
+def test_skip_p_save_leaves_P():
+    g = make_grid(16, 8, 8, 1.5)
+    U, V, W, P, rhs = fields(g, 5, 0.1)
+    P[:] = 123.0
+    plan = dfp.Plan(g)
+    plan.project(U, V, W, rhs, P, DT, C3, save_p=False)
+    assert np.all(P == 123.0)
+
+
+def test_divergence_invariant_config1():
+    # 64^3, stretching 1.5, sigma 0.1 (the plain single-projection case)
+    g = make_grid(64, 64, 64, 1.5)
+    print("case config1 64^3", flush=True)
+    o, d = _run_both(g, 2024, 0.1)
+    for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+        assert rel_linf(a, b) <= TOL, lab
+    U, V, W = d[0], d[1], d[2]
+    dfp.refresh_velocity_ghosts(g, U, V, W)
+    plan = dfp.Plan(g)
+    assert plan.max_divergence(U, V, W) <= div_bound(g, U, V, W)
+
+
+def test_device_resident_equals_host():
+    g = make_grid(32, 24, 16, 2.0)
+    U, V, W, P, rhs = fields(g, 99, 0.15)
+    plan = dfp.Plan(g)
+    h = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    plan.project(h[0], h[1], h[2], h[4], h[3], DT, C3, save_p=True)
+    dev = [dfp.DeviceArray.from_host(a) for a in (U, V, W, P, rhs)]
+    plan.project(dev[0], dev[1], dev[2], dev[4], dev[3], DT, C3, save_p=True)
+    for a, b in zip(h, dev):
+        assert np.array_equal(a, b.download())
+
+
+def test_device_synth_matches_host_generator():
+    g = make_grid(16, 12, 8, 1.5)
+    plan = dfp.Plan(g)
+    dev = [dfp.DeviceArray(s) for s in (g.shape_u, g.shape_v, g.shape_w)]
+    plan.synth(dev[0], dev[1], dev[2], 31337, 0.15)
+    host = dfp.synth_fields(g, 31337, 0.15)
+    for a, b in zip(host, dev):
+        np.testing.assert_allclose(b.download(), a, rtol=1e-13, atol=1e-14)
+
+
+@pytest.mark.parametrize("seed", [1, 2])
+def test_config2_full_size_parity(seed):
+    # 256x192x256, stretching 2.0, sigma 0.15: full-size parity with the C oracle
+    g = make_grid(256, 192, 256, 2.0)
+    print(f"case config2 256x192x256 seed {seed}", flush=True)
+    o, d = _run_both(g, seed, 0.15)
+    for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+        err = rel_linf(a, b)
+        print(f"  {lab} rel-Linf {err:.3e}", flush=True)
+        assert err <= TOL, lab
+    U, V, W = d[0], d[1], d[2]
+    dfp.refresh_velocity_ghosts(g, U, V, W)
+    plan = dfp.Plan(g)
+    md = plan.max_divergence(U, V, W)
+    print(f"  max div {md:.3e} bound {div_bound(g, U, V, W):.3e}", flush=True)
+    assert md <= div_bound(g, U, V, W)
