@@ -29,13 +29,13 @@ HBM_PEAK_GBS = 8000.0
 
 
 def algorithmic_bytes(N: int, H: int) -> dict:
-    """Per-launch algorithmic bytes of each kernel: the five-stage model
-    B = 80 N + 128 H (FP64 I/O, complex FP64 spectrum), split by kernel."""
+    """Per-launch algorithmic bytes of each kernel: the data it must read and
+    write (FP64 fields, complex FP64 spectrum of H = (nxp/2+1)*nym*nzp modes).
+    The whole projection is measured against B_model = 80 N + 128 H."""
     return {
         "div_rfft_x": 24 * N + 16 * H,   # read u*,v*,w*; write the x half-spectrum
-        "fft_z_fwd": 32 * H,             # read + write the spectrum
-        "thomas_y": 32 * H,
-        "fft_z_inv": 32 * H,
+        "zsolve_fwd": 32 * H,            # read the spectrum; write the eliminated rows
+        "zsolve_bwd": 32 * H,            # read them; write the solved spectrum
         "irfft_x": 16 * H + 8 * N,       # read the spectrum; write phi
         "correct": 48 * N,               # read u*,v*,w*; write u,v,w
     }
@@ -77,7 +77,18 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-CPU baseline")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="only run one device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0:
+        from dfp import hipmem
+
+        n = args.copy_probe_mib * (1 << 20) // 8
+        a, b = hipmem.DeviceArray((n,), zero=True), hipmem.DeviceArray((n,), zero=True)
+        b.copy_from(a)
+        hipmem.synchronize()
+        print(json.dumps({"copy_probe_bytes": 8 * n}))
+        return 0
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the projection's multi-GPU sharding is a design, not a build)", file=sys.stderr)

@@ -12,7 +12,7 @@
 #include <cstring>
 #include <new>
 
-#include "dfp_internal.h"
+#include "fft_lds.h"
 
 using namespace dfp;
 
@@ -118,7 +118,7 @@ void free_plan_memory(dfp_plan* p) {
     (void)hipFree(p->d_tables);
     (void)hipFree(p->d_tw);
     (void)hipFree(p->d_spec);
-    (void)hipFree(p->d_piv);
+    (void)hipFree(p->d_lu);
     (void)hipFree(p->d_red);
     (void)hipFree(p->d_U);
     (void)hipFree(p->d_V);
@@ -169,9 +169,8 @@ int check_flags(dfp_plan* p, int flags) {
 int spectral_solve(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs, double* P, double dtc,
                    int save_p, int from_fields) {
     HIPCHK(p, launch_div_rfft_x(p, U, V, W, rhs, from_fields ? 0 : 1));
-    HIPCHK(p, launch_fft_z(p, -1));
-    HIPCHK(p, launch_thomas_y(p));
-    HIPCHK(p, launch_fft_z(p, +1));
+    HIPCHK(p, launch_zsolve(p, -1));
+    HIPCHK(p, launch_zsolve(p, +1));
     HIPCHK(p, launch_irfft_x(p, rhs, P, dtc, save_p));
     return DFP_OK;
 }
@@ -232,7 +231,9 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     if (g->x_bc_type != 0 || g->y_bc_type != 1 || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
     if (precision != DFP_FP64) return DFP_E_UNSUPPORTED;
     if (g->nxp < 2 || (g->nxp & 1) || g->nym < 1 || g->nzp < 1) return DFP_E_UNSUPPORTED;
-    if (g->nxp > 8192 || g->nzp > 4096 || g->nym > 65535) return DFP_E_UNSUPPORTED;
+    // an x block holds zr lines of nxp/2 complex points in 32 KiB of LDS; a z
+    // block holds one mode per thread
+    if (g->nxp > 1024 || g->nzp > 1024 || g->nym > 65535) return DFP_E_UNSUPPORTED;
     if (!(g->dx > 0.0) || !(g->dz > 0.0)) return DFP_E_ARG;
     dfp_plan* p = new (std::nothrow) dfp_plan();
     if (!p) return DFP_E_NOMEM;
@@ -243,7 +244,14 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     };
     p->device = device;
     p->precision = precision;
-    if (!factor(g->nxp / 2, p->fx) || !factor(g->nzp, p->fz)) return bail(DFP_E_UNSUPPORTED);
+    // z split: the x kernels apply the first radix-zr stage of the z FFT, the
+    // z kernels transform sub-rows of nzp/zr points, one mode per thread
+    int zr = 1;
+    if (g->nzp % 4 == 0 && g->nzp / 4 >= 8) zr = 4;
+    else if (g->nzp % 2 == 0 && g->nzp / 2 >= 4) zr = 2;
+    else if (g->nzp % 3 == 0 && g->nzp / 3 >= 4) zr = 3;
+    p->zr = zr;
+    if (!factor(g->nxp / 2, p->fx) || !factor(g->nzp / zr, p->fz)) return bail(DFP_E_UNSUPPORTED);
     if (hipSetDevice(device) != hipSuccess) return bail(DFP_E_HIP);
     Geo& G = p->geo;
     G.nxp = g->nxp; G.nym = g->nym; G.nzp = g->nzp; G.nxh = g->nxp / 2 + 1;
@@ -300,6 +308,10 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     stage_twiddles(p->fz, tw);
     const size_t o_r2c = tw.size();
     for (int k = 0; k <= G.nxp / 2; ++k) tw.push_back(expi(k, G.nxp));
+    const size_t o_zs = tw.size();
+    for (int k = 0; k < G.nzp / zr; ++k)
+        for (int u = 1; u < zr; ++u) tw.push_back(expi((long)k * u, G.nzp));
+    tw.push_back(make_double2(1.0, 0.0));
     for (int st = 0; st < p->fz.nst; ++st) p->fz.twoff[st] -= (int)o_tz;
     if (hipMalloc(&p->d_tw, sizeof(double2) * tw.size()) != hipSuccess) return bail(DFP_E_NOMEM);
     if (hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
@@ -307,11 +319,14 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     p->tw_x = p->d_tw;
     p->tw_z = p->d_tw + o_tz;
     p->tw_r2c = p->d_tw + o_r2c;
+    p->tw_zs = p->d_tw + o_zs;
 
     const size_t H = (size_t)G.nxh * G.nym * G.nzp;
     if (hipMalloc(&p->d_spec, sizeof(double2) * H) != hipSuccess) return bail(DFP_E_NOMEM);
-    if (hipMalloc(&p->d_piv, sizeof(double) * H) != hipSuccess) return bail(DFP_E_NOMEM);
-    p->red_len = (int)((long)ceil_div(G.nxg - 2, 256) * (G.nyg - 2) * (G.nzg - 2));
+    const size_t nlu = (size_t)G.nxh * G.nym * (G.nzp / 2 + 1);
+    if (hipMalloc(&p->d_lu, sizeof(double) * nlu) != hipSuccess) return bail(DFP_E_NOMEM);
+    if (launch_lu_table(p) != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) return bail(DFP_E_HIP);
+    p->red_len = (int)((long)ceil_div((long)(G.nxg - 2) * G.nym, 256) * (G.nzg - 2));
     if (hipMalloc(&p->d_red, sizeof(double) * (p->red_len + 8)) != hipSuccess) return bail(DFP_E_NOMEM);
 
     p->sz_u = (int64_t)G.nx * G.nyg * G.nzg;
@@ -320,11 +335,18 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     p->sz_p = (int64_t)G.nxg * G.nyg * G.nzg;
     p->sz_rhs = (int64_t)(G.nxg - 1) * G.nym * (G.nzg - 1);
 
-    // launch shapes: about 1024 complex points per block in the LDS transforms
-    // (two LDS buffers of 16 KiB each at that size)
-    const int M = G.nxp / 2;
-    p->tk_x = std::max(1, std::min(64, 1024 / std::max(M, 1)));
-    p->rb_z = std::max(1, std::min(64, 1024 / std::max(G.nzp, 1)));
+    // launch shapes: x blocks hold about 2048 complex points (zr lines per
+    // column), z blocks as many rows as fit their registers and LDS
+    int lds_attr = 0;
+    if (hipDeviceGetAttribute(&lds_attr, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds_attr > 0)
+        p->lds_limit = (size_t)lds_attr;
+    const int M = G.nxp / 2, nz2 = G.nzp / zr;
+    p->tk_x = std::max(1, std::min(nz2, 2048 / (zr * M)));
+    if (sizeof(double2) * ((size_t)zr * p->tk_x * (M + 1) + twiddle_count(p->fx)) > p->lds_limit ||
+        (size_t)zr * p->tk_x * M > (size_t)inplace_capacity(p->fx, 512, 2))
+        return bail(DFP_E_UNSUPPORTED);
+    p->g_zs = zsolve_rows_per_group(p->fz, G.nym, p->lds_limit);
+    if (p->g_zs < 1) return bail(DFP_E_UNSUPPORTED);
     *out = p;
     return DFP_OK;
 }

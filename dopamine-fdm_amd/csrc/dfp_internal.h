@@ -73,13 +73,15 @@ struct dfp_plan {
     // device tables
     double* d_tables = nullptr;  // one allocation holding every 1-D table
     double2* d_tw = nullptr;     // FFT twiddles: x stages, z stages, R2C post twiddles
-    dfp::FftDesc fx{}, fz{};     // x: length nxp/2 (half-length complex), z: length nzp
+    dfp::FftDesc fx{}, fz{};     // x: length nxp/2 (half-length complex), z: length nzp/zr
+    int zr = 1;                  // radix of the z stage applied by the x kernels (z split)
     const double2* tw_x = nullptr;
     const double2* tw_z = nullptr;
     const double2* tw_r2c = nullptr;  // exp(-2 pi i k / nxp), k = 0..nxp/2
+    const double2* tw_zs = nullptr;   // exp(-2 pi i k' u / nzp), [k'*(zr-1) + u-1], k' < nzp/zr, 1 <= u < zr
     // device work arrays
-    double2* d_spec = nullptr;   // (nxp/2+1) x nym x nzp complex, layout [kx][j][kz]
-    double* d_piv = nullptr;     // Thomas pivots, same layout as d_spec (real)
+    double2* d_spec = nullptr;   // (nxp/2+1) x nym x nzp complex, layout [kx][u][j][m], kz = zr*m + u
+    double* d_lu = nullptr;      // inverse pivots of the y solve, [kx][j][min(kz,nzp-kz)]
     double* d_red = nullptr;     // reduction partials
     int red_len = 0;
     // staging for host-resident fields
@@ -93,8 +95,9 @@ struct dfp_plan {
     std::vector<hipEvent_t> event_pool;
     std::string last_error;
     // launch shapes
-    int tk_x = 8;   // lines per block of the x-transform kernels
-    int rb_z = 4;   // rows per block of the z-transform kernels
+    int tk_x = 8;   // z columns per block of the x-transform kernels (zr lines each)
+    int g_zs = 4;   // rows per group of the fused z-transform / y-solve kernels
+    size_t lds_limit = 65536;  // LDS bytes per block on this device
 };
 
 namespace dfp {
@@ -102,8 +105,9 @@ namespace dfp {
 // launchers (defined next to their kernels)
 hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W,
                              const double* rhs, int from_rhs);
-hipError_t launch_fft_z(dfp_plan* p, int sign);
-hipError_t launch_thomas_y(dfp_plan* p);
+hipError_t launch_lu_table(dfp_plan* p);
+hipError_t launch_zsolve(dfp_plan* p, int dir);
+int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit);  // -1: z FFT + forward elimination, +1: back substitution + inverse z FFT
 hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p);
 hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs);
 hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs);

@@ -3,8 +3,9 @@
 //   k_correct      U,V,W -= grad(rhs_p)          projection.f90:551-604
 //   k_max_div      max |div(U,V,W)|              projection.f90:607-639
 //   k_synth        seeded synthetic channel fields (benchmark/test inputs)
-// Threads run along x (the contiguous index of every field); blockIdx.y/z
-// are the y and z cell indices, so every row access is coalesced.  The
+// Threads run over the (x,y) plane of one z index, x fastest (the contiguous
+// index of every field), blockIdx.y is the z index; consecutive lanes read
+// consecutive addresses of one row.  The
 // stencil arithmetic is kept operation-for-operation as the reference writes
 // it (no FMA contraction), so these kernels reproduce the CPU path bit for bit.
 #include "dfp_internal.h"
@@ -19,10 +20,12 @@ __device__ __forceinline__ long ridx(const Geo& g, int i, int j, int k) {
 __global__ void __launch_bounds__(256) k_divergence(const double* __restrict__ U, const double* __restrict__ V,
                                                     const double* __restrict__ W, double* __restrict__ rhs, Geo g) {
 #pragma clang fp contract(off)
-    const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;  // 2..nxg-1
-    const int j = 2 + blockIdx.y;                             // 2..nyg-1
-    const int k = 2 + blockIdx.z;                             // 2..nzg-1
-    if (i > g.nxg - 1) return;
+    const int ni = g.nxg - 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * g.nym) return;
+    const int j = 2 + t / ni;                 // 2..nyg-1
+    const int i = 2 + t - (j - 2) * ni;       // 2..nxg-1
+    const int k = 2 + blockIdx.y;             // 2..nzg-1
     const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg;
     const long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.ny;
     const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.nyg;
@@ -34,7 +37,7 @@ __global__ void __launch_bounds__(256) k_divergence(const double* __restrict__ U
 
 hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs) {
     const Geo& g = p->geo;
-    dim3 grid(ceil_div(g.nxg - 2, 256), g.nyg - 2, g.nzg - 2);
+    dim3 grid(ceil_div((long)(g.nxg - 2) * g.nym, 256), g.nzg - 2);
     const int slot = prof_begin(p, "divergence");
     hipLaunchKernelGGL(k_divergence, grid, dim3(256), 0, p->stream, U, V, W, rhs, g);
     prof_end(p, slot);
@@ -47,10 +50,12 @@ hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, cons
 __global__ void __launch_bounds__(256) k_correct(double* __restrict__ U, double* __restrict__ V,
                                                  double* __restrict__ W, const double* __restrict__ rhs, Geo g) {
 #pragma clang fp contract(off)
-    const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 2 + blockIdx.y;
-    const int k = 2 + blockIdx.z;
-    if (i > g.nxg - 1) return;
+    const int ni = g.nxg - 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= ni * g.nym) return;
+    const int j = 2 + t / ni;
+    const int i = 2 + t - (j - 2) * ni;
+    const int k = 2 + blockIdx.y;
     const long r = ridx(g, i, j, k);
     const double phi = rhs[r];
     if (i <= g.nx - 1) {
@@ -69,7 +74,7 @@ __global__ void __launch_bounds__(256) k_correct(double* __restrict__ U, double*
 
 hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs) {
     const Geo& g = p->geo;
-    dim3 grid(ceil_div(g.nxg - 2, 256), g.nyg - 2, g.nzg - 2);
+    dim3 grid(ceil_div((long)(g.nxg - 2) * g.nym, 256), g.nzg - 2);
     const int slot = prof_begin(p, "correct");
     hipLaunchKernelGGL(k_correct, grid, dim3(256), 0, p->stream, U, V, W, rhs, g);
     prof_end(p, slot);
@@ -80,11 +85,13 @@ hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const do
 __global__ void __launch_bounds__(256) k_max_div(const double* __restrict__ U, const double* __restrict__ V,
                                                  const double* __restrict__ W, double* __restrict__ partials, Geo g) {
     __shared__ double red[256];
-    const int i = 2 + blockIdx.x * blockDim.x + threadIdx.x;
-    const int j = 2 + blockIdx.y;
-    const int k = 2 + blockIdx.z;
+    const int ni = g.nxg - 2;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int j = 2 + t / ni;
+    const int i = 2 + t - (j - 2) * ni;
+    const int k = 2 + blockIdx.y;
     double v = 0.0;
-    if (i <= g.nxg - 1) {
+    if (t < ni * g.nym) {
         const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg;
         const long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.ny;
         const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.nyg;
@@ -99,8 +106,7 @@ __global__ void __launch_bounds__(256) k_max_div(const double* __restrict__ U, c
         __syncthreads();
     }
     if (threadIdx.x == 0) {
-        const long b = blockIdx.x + (long)gridDim.x * (blockIdx.y + (long)gridDim.y * blockIdx.z);
-        partials[b] = red[0];
+        partials[blockIdx.x + (long)gridDim.x * blockIdx.y] = red[0];
     }
 }
 
@@ -120,8 +126,8 @@ __global__ void __launch_bounds__(256) k_reduce_max(const double* __restrict__ p
 
 hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev) {
     const Geo& g = p->geo;
-    dim3 grid(ceil_div(g.nxg - 2, 256), g.nyg - 2, g.nzg - 2);
-    const long nb = (long)grid.x * grid.y * grid.z;
+    dim3 grid(ceil_div((long)(g.nxg - 2) * g.nym, 256), g.nzg - 2);
+    const long nb = (long)grid.x * grid.y;
     if (nb > p->red_len) return hipErrorInvalidValue;
     const int slot = prof_begin(p, "max_div");
     hipLaunchKernelGGL(k_max_div, grid, dim3(256), 0, p->stream, U, V, W, p->d_red, g);

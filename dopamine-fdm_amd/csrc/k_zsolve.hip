@@ -1,0 +1,243 @@
+// Fused z-transform + wall-normal tridiagonal solve (FP64), one block per
+// (kx, u) sub-plane of the split spectrum (see k_spectral.hip: the x kernel
+// already applied the first radix-R z stage, so each sub-plane holds
+// independent rows of length nz2 = nzp/R whose DFTs are the modes kz = R*m+u).
+//
+// The y system of every mode (kx,kz) is the one of projection.f90:277-297:
+//   DL(j) = Dyy(j+1,j), DU(j) = Dyy(j,j+1), D(j) = Dyy(j,j) + (kxx + kzz),
+//   D(2) *= 3/2 for the (0,0) mode (:292).
+// Its LU factors depend only on kxx(kx) + kzz(kz), and kzz(kz) = kzz(nzp-kz)
+// exactly, so the plan holds the inverse pivots 1/D'(j) of the elimination
+// once per (kx, min(kz, nzp-kz)):  lu[(kx*nym + j)*nzs + kzs],
+// nzs = nzp/2 + 1.  The elimination follows zgtsv's no-interchange branch
+// (mult = DL/D', D' <- D - mult*DU); LAPACK interchanges rows only in a few
+// low modes, which moves the solution at the 1e-15 level.
+//
+// k_zsolve_fwd: rows j = 0..nym-1 in groups of G: FFT of the rows in LDS,
+//   then per mode b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1); b' overwrites them.
+// k_zsolve_bwd: rows j = nym-1..0: x(j) = (b'(j) - DU(j) x(j+1)) / D'(j), then
+//   the inverse FFT of the rows in LDS.
+// Both read and write the spectrum once (64 H bytes for the pair).  While a
+// group is transformed and swept, the next group's rows, pivots and
+// coefficients are already loading into registers.
+#include <algorithm>
+
+#include "fft_lds.h"
+
+namespace dfp {
+
+__global__ void __launch_bounds__(256) k_lu_table(double* __restrict__ lu, Ops o, int nxh, int nym, int nzp) {
+    const int nzs = nzp / 2 + 1;
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)nxh * nzs) return;
+    const int kx = (int)(t / nzs);
+    const int kzs = (int)(t - (long)kx * nzs);
+    const double lam = o.kxx[kx] + o.kzz[kzs];
+    double dp = o.dd[0] + lam;
+    if (kx == 0 && kzs == 0) dp = 3.0 / 2.0 * dp;
+    double* col = lu + (long)kx * nym * nzs + kzs;
+    col[0] = 1.0 / dp;
+    for (int j = 1; j < nym; ++j) {
+        const double mult = o.dl[j - 1] / dp;
+        dp = (o.dd[j] + lam) - mult * o.du[j - 1];
+        col[(long)j * nzs] = 1.0 / dp;
+    }
+}
+
+hipError_t launch_lu_table(dfp_plan* p) {
+    const Geo& g = p->geo;
+    const long n = (long)g.nxh * (g.nzp / 2 + 1);
+    hipLaunchKernelGGL(k_lu_table, dim3(ceil_div(n, 256)), dim3(256), 0, p->stream, p->d_lu, p->ops, g.nxh, g.nym,
+                       g.nzp);
+    return hipGetLastError();
+}
+
+constexpr int kZsThreads = 256;
+constexpr int kZsPf = 8;  // group points per thread held in flight
+
+// Shared layout of one group (G rows of length n):
+//   A[r*n + m]     the rows
+//   lus[r*n + m]   inverse pivot of mode m at row j0 + r
+//   dus[r]         DL(j-1) (forward) or DU(j) (backward) of row j = j0 + r
+// The prefetch registers are named variables (pr0..pr7, pl0..pl7, pd) so
+// they are never placed in scratch memory; loads are unconditional (indices
+// clamped into the group) so they stay in flight until staged.
+#define ZS_LOAD1(K)                                                               \
+    {                                                                             \
+        const int e = min((int)threadIdx.x + (K) * kZsThreads, lim_);             \
+        pr##K = src_[e];                                                          \
+        const int r = e / n;                                                      \
+        const int mm = e - r * n;                                                 \
+        const int kz = R * mm + u;                                                \
+        pl##K = lsrc_[(long)r * nzs + min(kz, nzp - kz)];                         \
+    }
+#define ZS_PREFETCH(J0, GQ, COEF, COFF)                                           \
+    do {                                                                          \
+        const double2* src_ = plane + (long)(J0) * n;                             \
+        const double* lsrc_ = lup + (long)(J0) * nzs;                             \
+        const int lim_ = (GQ) * n - 1;                                            \
+        ZS_LOAD1(0) ZS_LOAD1(1) ZS_LOAD1(2) ZS_LOAD1(3)                           \
+        ZS_LOAD1(4) ZS_LOAD1(5) ZS_LOAD1(6) ZS_LOAD1(7)                           \
+        const int jc = (J0) + min((int)threadIdx.x, (GQ) - 1) + (COFF);           \
+        pd = COEF[max(0, min(jc, nym - 2))];                                      \
+    } while (0)
+#define ZS_STORE1(K)                                                              \
+    {                                                                             \
+        const int e = (int)threadIdx.x + (K) * kZsThreads;                        \
+        if (e < np_) {                                                            \
+            A[e] = pr##K;                                                         \
+            lus[e] = pl##K;                                                       \
+        }                                                                         \
+    }
+#define ZS_STAGE(GR)                                                              \
+    do {                                                                          \
+        const int np_ = (GR) * n;                                                 \
+        ZS_STORE1(0) ZS_STORE1(1) ZS_STORE1(2) ZS_STORE1(3)                       \
+        ZS_STORE1(4) ZS_STORE1(5) ZS_STORE1(6) ZS_STORE1(7)                       \
+        if ((int)threadIdx.x < (GR)) dus[threadIdx.x] = pd;                       \
+    } while (0)
+#define ZS_REGS                                                                   \
+    double2 pr0, pr1, pr2, pr3, pr4, pr5, pr6, pr7;                               \
+    double pl0, pl1, pl2, pl3, pl4, pl5, pl6, pl7;                                \
+    double pd
+
+__global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(double2* __restrict__ spec, const double* __restrict__ lu,
+                                                           const double* __restrict__ dl, int nym, int nzp, int R,
+                                                           FftDesc fz, const double2* __restrict__ tw, int ntw,
+                                                           int G) {
+    extern __shared__ double2 smem[];
+    const int n = fz.n;
+    const int nzs = nzp / 2 + 1;
+    const int b = blockIdx.x;
+    const int kx = b / R;
+    const int u = b - kx * R;
+    double2* A = smem;
+    double2* twl = A + G * n;
+    double* lus = reinterpret_cast<double*>(twl + ntw);
+    double* dus = lus + G * n;
+    double2* plane = spec + (long)b * nym * n;
+    const double* lup = lu + (long)kx * nym * nzs;
+    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = tw[e];
+    ZS_REGS;
+    ZS_PREFETCH(0, min(G, nym), dl, -1);
+    double2 bp = make_double2(0.0, 0.0);
+    double ipp = 0.0;
+    const int m = threadIdx.x;  // mode of this thread (m < n)
+    for (int j0 = 0; j0 < nym; j0 += G) {
+        const int gr = min(G, nym - j0);
+        __syncthreads();
+        ZS_STAGE(gr);
+        __syncthreads();
+        if (j0 + G < nym) ZS_PREFETCH(j0 + G, min(G, nym - j0 - G), dl, -1);
+        lds_fft_inplace<-1, 2>(A, gr, n, fz, twl);
+        if (m < n) {
+            for (int r = 0; r < gr; ++r) {
+                double2 bb = A[r * n + m];
+                if (j0 + r > 0) {
+                    const double mult = dus[r] * ipp;  // DL(j-1) / D'(j-1)
+                    bb.x = bb.x - mult * bp.x;
+                    bb.y = bb.y - mult * bp.y;
+                }
+                A[r * n + m] = bb;
+                bp = bb;
+                ipp = lus[r * n + m];
+            }
+        }
+        __syncthreads();
+        double2* dst = plane + (long)j0 * n;
+        for (int e = threadIdx.x; e < gr * n; e += kZsThreads) dst[e] = A[e];
+    }
+}
+
+__global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(double2* __restrict__ spec, const double* __restrict__ lu,
+                                                           const double* __restrict__ du, int nym, int nzp, int R,
+                                                           FftDesc fz, const double2* __restrict__ tw, int ntw,
+                                                           int G) {
+    extern __shared__ double2 smem[];
+    const int n = fz.n;
+    const int nzs = nzp / 2 + 1;
+    const int b = blockIdx.x;
+    const int kx = b / R;
+    const int u = b - kx * R;
+    double2* A = smem;
+    double2* twl = A + G * n;
+    double* lus = reinterpret_cast<double*>(twl + ntw);
+    double* dus = lus + G * n;
+    double2* plane = spec + (long)b * nym * n;
+    const double* lup = lu + (long)kx * nym * nzs;
+    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = tw[e];
+    // groups from the top: [j0, j0+gr), j0 = max(0, nym - (gi+1)*G)
+    const int ngroups = (nym + G - 1) / G;
+    ZS_REGS;
+    {
+        const int j0q = max(0, nym - G);
+        ZS_PREFETCH(j0q, nym - j0q, du, 0);
+    }
+    double2 xn = make_double2(0.0, 0.0);
+    const int m = threadIdx.x;
+    for (int gi = 0; gi < ngroups; ++gi) {
+        const int j0 = max(0, nym - (gi + 1) * G);
+        const int gr = nym - gi * G - j0;
+        __syncthreads();
+        ZS_STAGE(gr);
+        __syncthreads();
+        if (gi + 1 < ngroups) {
+            const int j0q = max(0, nym - (gi + 2) * G);
+            ZS_PREFETCH(j0q, j0 - j0q, du, 0);
+        }
+        if (m < n) {
+            for (int r = gr - 1; r >= 0; --r) {
+                const int j = j0 + r;
+                const double2 bb = A[r * n + m];
+                const double ip = lus[r * n + m];
+                double2 x;
+                if (j == nym - 1) {
+                    x = make_double2(bb.x * ip, bb.y * ip);
+                } else {
+                    // x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
+                    x = make_double2((bb.x - dus[r] * xn.x) * ip, (bb.y - dus[r] * xn.y) * ip);
+                }
+                A[r * n + m] = x;
+                xn = x;
+            }
+        }
+        __syncthreads();
+        lds_fft_inplace<1, 2>(A, gr, n, fz, twl);
+        double2* dst = plane + (long)j0 * n;
+        for (int e = threadIdx.x; e < gr * n; e += kZsThreads) dst[e] = A[e];
+    }
+}
+
+hipError_t launch_zsolve(dfp_plan* p, int dir) {
+    const Geo& g = p->geo;
+    const int G = p->g_zs;
+    const int n = p->fz.n;
+    const int ntw = twiddle_count(p->fz);
+    const size_t lds = sizeof(double2) * ((size_t)G * n + ntw) + sizeof(double) * ((size_t)G * n + G);
+    const int slot = prof_begin(p, dir < 0 ? "zsolve_fwd" : "zsolve_bwd");
+    const dim3 grid(g.nxh * p->zr);
+    if (dir < 0)
+        hipLaunchKernelGGL(k_zsolve_fwd, grid, dim3(kZsThreads), lds, p->stream, p->d_spec, p->d_lu, p->ops.dl, g.nym,
+                           g.nzp, p->zr, p->fz, p->tw_z, ntw, G);
+    else
+        hipLaunchKernelGGL(k_zsolve_bwd, grid, dim3(kZsThreads), lds, p->stream, p->d_spec, p->d_lu, p->ops.du, g.nym,
+                           g.nzp, p->zr, p->fz, p->tw_z, ntw, G);
+    prof_end(p, slot);
+    return hipGetLastError();
+}
+
+// rows per group: the group fits the prefetch registers (kZsPf points per
+// thread), the in-place FFT capacity (2 butterflies per thread) and, with the
+// twiddles and pivots, the device's LDS per block; 0 if one row does not fit
+int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit) {
+    const int n = fz.n;
+    if (n > kZsThreads) return 0;  // one mode per thread
+    const size_t ntw = (size_t)twiddle_count(fz);
+    int G = std::min(kZsPf * kZsThreads, inplace_capacity(fz, kZsThreads, 2)) / n;
+    G = std::max(0, std::min(G, nym));
+    auto bytes = [&](int g) { return sizeof(double2) * ((size_t)g * n + ntw) + sizeof(double) * ((size_t)g * n + g); };
+    while (G > 0 && bytes(G) > lds_limit) --G;
+    return G;
+}
+
+}  // namespace dfp
