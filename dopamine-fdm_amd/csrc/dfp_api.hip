@@ -301,16 +301,18 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     p->ops.kxx = T + o_kxx; p->ops.kzz = T + o_kzz;
     p->ops.dl = T + o_dl; p->ops.dd = T + o_dd; p->ops.du = T + o_du;
 
-    // twiddles: x stages, z stages, R2C post twiddles
+    // twiddles, in the order the x kernels stage them into LDS with one
+    // copy: x stages, R2C/C2R packing (M+1), first z stage (nz2*(zr-1)); then
+    // the z-kernel stages
     std::vector<double2> tw;
     stage_twiddles(p->fx, tw);
-    const size_t o_tz = tw.size();
-    stage_twiddles(p->fz, tw);
     const size_t o_r2c = tw.size();
     for (int k = 0; k <= G.nxp / 2; ++k) tw.push_back(expi(k, G.nxp));
     const size_t o_zs = tw.size();
     for (int k = 0; k < G.nzp / zr; ++k)
         for (int u = 1; u < zr; ++u) tw.push_back(expi((long)k * u, G.nzp));
+    const size_t o_tz = tw.size();
+    stage_twiddles(p->fz, tw);
     tw.push_back(make_double2(1.0, 0.0));
     for (int st = 0; st < p->fz.nst; ++st) p->fz.twoff[st] -= (int)o_tz;
     if (hipMalloc(&p->d_tw, sizeof(double2) * tw.size()) != hipSuccess) return bail(DFP_E_NOMEM);
@@ -342,7 +344,8 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
         p->lds_limit = (size_t)lds_attr;
     const int M = G.nxp / 2, nz2 = G.nzp / zr;
     p->tk_x = std::max(1, std::min(nz2, 2048 / (zr * M)));
-    if (sizeof(double2) * ((size_t)zr * p->tk_x * (M + 1) + twiddle_count(p->fx)) > p->lds_limit ||
+    if (sizeof(double2) * ((size_t)zr * p->tk_x * (M + 1) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (zr - 1)) >
+            p->lds_limit ||
         (size_t)zr * p->tk_x * M > (size_t)inplace_capacity(p->fx, 512, 2))
         return bail(DFP_E_UNSUPPORTED);
     p->g_zs = zsolve_rows_per_group(p->fz, G.nym, p->lds_limit);

@@ -22,6 +22,9 @@
 // points z[n] = x[2n] + i x[2n+1] and transformed with a length-nxp/2
 // complex FFT; X[k] = E[k] + w^k O[k], E = (Z[k]+conj Z[M-k])/2,
 // O = (Z[k]-conj Z[M-k])/(2i), w = exp(-2 pi i/nxp) gives the nxp/2+1 modes.
+// The x FFT stage twiddles, the packing twiddles w^k and the first z stage
+// twiddles sit back to back in the plan's twiddle array (tw, rtw, tzs) and
+// are staged into LDS with one copy.
 #include "fft_lds.h"
 
 namespace dfp {
@@ -74,9 +77,12 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(const double* __restri
     const int nt = min(T, nz2 - k0);
     const int nl = R * nt;
     double2* A = smem;
-    double2* twl = smem + R * T * M;
+    double2* twl = smem + R * T * M;      // FFT stage twiddles
+    double2* rtl = twl + ntw;             // R2C unpack twiddles, M+1
+    double2* tzl = rtl + (M + 1);         // first z stage twiddles, nz2*(R-1)
     double* Ar = reinterpret_cast<double*>(A);
-    for (int e = threadIdx.x; e < ntw; e += kXThreads) twl[e] = tw[e];
+    // the three tables are contiguous in the plan's twiddle array
+    for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = tw[e];
 
     // reference indices of cell (i,j,k): (i+2, j+2, k+2)
     const long pu_j = (long)g.nx, pu_k = (long)g.nx * g.nyg;
@@ -139,7 +145,7 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(const double* __restri
                 const double2 E = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
                 const double2 D = make_double2(zk.x - zm.x, zk.y + zm.y);  // zk - conj(zm)
                 const double2 O = make_double2(0.5 * D.y, -0.5 * D.x);     // D / (2i)
-                a[t] = cadd(E, cmul(rtw[kx], O));
+                a[t] = cadd(E, cmul(rtl[kx], O));
             } else {
                 a[t] = make_double2(0.0, 0.0);
             }
@@ -150,7 +156,7 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(const double* __restri
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (u < R) {
-                const double2 y = u == 0 ? a[0] : cmul(a[u], tzs[kp * (R - 1) + (u - 1)]);
+                const double2 y = u == 0 ? a[0] : cmul(a[u], tzl[kp * (R - 1) + (u - 1)]);
                 out[(long)u * g.nym * nz2] = y;
             }
         }
@@ -163,7 +169,7 @@ hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, cons
     const int R = p->zr, T = p->tk_x, nz2 = g.nzp / R;
     dim3 grid(ceil_div(nz2, T), g.nym);
     const int ntw = twiddle_count(p->fx);
-    const size_t lds = sizeof(double2) * ((size_t)R * T * p->fx.n + ntw);
+    const size_t lds = sizeof(double2) * ((size_t)R * T * p->fx.n + ntw + p->fx.n + 1 + (size_t)nz2 * (R - 1));
     const int slot = prof_begin(p, from_rhs ? "rfft_x_rhs" : "div_rfft_x");
     if (from_rhs)
         hipLaunchKernelGGL(k_div_rfft_x<1>, grid, dim3(kXThreads), lds, p->stream, U, V, W, rhs, p->d_spec, g, p->fx,
@@ -200,8 +206,12 @@ __global__ void __launch_bounds__(kXThreads) k_irfft_x(const double2* __restrict
     const int nt = min(T, nz2 - k0);
     const int nl = R * nt;
     double2* A = smem;
-    double2* twl = smem + R * T * ld;
-    for (int e = threadIdx.x; e < ntw; e += kXThreads) twl[e] = tw[e];
+    double2* twl = smem + R * T * ld;     // FFT stage twiddles
+    double2* rtl = twl + ntw;             // C2R packing twiddles, M+1
+    double2* tzl = rtl + (M + 1);         // last z stage twiddles, nz2*(R-1)
+    // the three tables are contiguous in the plan's twiddle array
+    for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = tw[e];
+    __syncthreads();
 
     // gather the R sub-line values of each (kx, column), undo the first z stage
     const int npairs = (M + 1) * nt;
@@ -227,7 +237,7 @@ __global__ void __launch_bounds__(kXThreads) k_irfft_x(const double2* __restrict
             const int kp = k0 + c;
 #pragma unroll
             for (int u = 1; u < 4; ++u)
-                if (u < R) a[h][u] = cmulc(a[h][u], tzs[kp * (R - 1) + (u - 1)]);
+                if (u < R) a[h][u] = cmulc(a[h][u], tzl[kp * (R - 1) + (u - 1)]);
             dft_small<1>(a[h], R);
 #pragma unroll
             for (int t = 0; t < 4; ++t)
@@ -247,13 +257,13 @@ __global__ void __launch_bounds__(kXThreads) k_irfft_x(const double2* __restrict
         if (k < M) {
             const double2 S = make_double2(xk.x + xm.x, xk.y - xm.y);  // X[k] + conj X[M-k]
             const double2 D = make_double2(xk.x - xm.x, xk.y + xm.y);  // X[k] - conj X[M-k]
-            const double2 Tt = cmulc(D, rtw[k]);
+            const double2 Tt = cmulc(D, rtl[k]);
             X[k] = make_double2(S.x - Tt.y, S.y + Tt.x);
         }
         if (km != k && km < M) {
             const double2 S = make_double2(xm.x + xk.x, xm.y - xk.y);
             const double2 D = make_double2(xm.x - xk.x, xm.y + xk.y);
-            const double2 Tt = cmulc(D, rtw[km]);
+            const double2 Tt = cmulc(D, rtl[km]);
             X[km] = make_double2(S.x - Tt.y, S.y + Tt.x);
         }
     }
@@ -304,7 +314,7 @@ hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int s
     const int R = p->zr, T = p->tk_x, nz2 = g.nzp / R;
     dim3 grid(ceil_div(nz2, T), g.nym);
     const int ntw = twiddle_count(p->fx);
-    const size_t lds = sizeof(double2) * ((size_t)R * T * (p->fx.n + 1) + ntw);
+    const size_t lds = sizeof(double2) * ((size_t)R * T * (p->fx.n + 1) + ntw + p->fx.n + 1 + (size_t)nz2 * (R - 1));
     const double scale = 1.0 / ((double)g.nxp * (double)g.nzp);
     const int slot = prof_begin(p, "irfft_x");
     hipLaunchKernelGGL(k_irfft_x, grid, dim3(kXThreads), lds, p->stream, p->d_spec, rhs, P, g, p->fx, p->tw_x, ntw,
