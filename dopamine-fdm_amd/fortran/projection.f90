@@ -1,0 +1,137 @@
+!> Drop-in replacement of the reference's module projection (src/projection.f90) for the channel case
+!  (x periodic, y walls, z periodic) on one rank: the same subroutine names and arguments, backed by the
+!  dfp HIP library through its C ABI.  The fields are the global-module arrays (global.f90:139-146); they
+!  are copied to the device and back on each call.  A caller that keeps them on the device passes device
+!  addresses with DFP_FIELDS_ON_DEVICE instead (see dfp_project in include/dfp.h).
+Module projection
+
+  Use iso_c_binding
+  Use iso_fortran_env, Only : error_unit, Int32, Int64
+  Use global
+  Use dfp_capi
+
+  Implicit None
+
+  Private
+  Public :: compute_projection_step, compute_pseudo_pressure_rhs, solve_poisson_equation, project_velocity, &
+            apply_periodic_xz_pressure, update_ghost_interior_planes_pressure, &
+            update_ghost_interior_planes_pressure_x, finish_pressure_ghosts, check_divergence, projection_release
+
+  Type(c_ptr), Save :: plan = c_null_ptr
+  ! the grid the plan was built from (c_loc needs TARGET arrays)
+  Real(c_double), Allocatable, Target, Save :: gx(:), gy(:), gz(:), gyg(:), gzg(:)
+  ! stands in for P when no pressure is saved
+  Real(c_double), Save :: no_p(1) = 0d0
+
+Contains
+
+  !> Stops with the library's message (the reference aborts through gpu_abort, poisson_gpu.f90:51-56)
+  Subroutine dfp_check(rc, where)
+    Integer(c_int), Intent(In) :: rc
+    Character(*), Intent(In) :: where
+    If ( rc /= 0 ) Then
+       Write(error_unit,'(A)') 'ERROR: '//where//': '//c_string(dfp_strerror(rc))//': '//c_string(dfp_last_error(plan))
+       Error Stop 1
+    End If
+  End Subroutine dfp_check
+
+  !> Builds the plan on first use from the global grid (the reference builds its GPU plans lazily,
+  !  poisson_gpu.f90:234, :436)
+  Subroutine ensure_plan
+    Type(dfp_grid_t) :: g
+    If ( c_associated(plan) ) Return
+    If ( x_bc_type /= 0 .Or. y_bc_type /= 1 .Or. z_bc_type /= 0 ) Then
+       Write(error_unit,'(A)') 'ERROR: the dfp projection supports x periodic, y walls, z periodic only'
+       Error Stop 1
+    End If
+    gx = x;  gy = y;  gz = z;  gyg = yg;  gzg = zg
+    g%nxp = nx - 2;  g%nym = ny - 1;  g%nzp = nz - 2
+    g%dx = dx;  g%dz = dz
+    g%x = c_loc(gx);  g%y = c_loc(gy);  g%z = c_loc(gz);  g%yg = c_loc(gyg);  g%zg = c_loc(gzg)
+    g%x_bc_type = x_bc_type;  g%y_bc_type = y_bc_type;  g%z_bc_type = z_bc_type
+    Call dfp_check(dfp_plan_create(g, DFP_FP64, 0_c_int, plan), 'dfp_plan_create')
+  End Subroutine ensure_plan
+
+  !> Releases the plan (no counterpart in the reference; call at finalisation)
+  Subroutine projection_release
+    Integer(c_int) :: rc
+    If ( c_associated(plan) ) rc = dfp_plan_destroy(plan)
+    plan = c_null_ptr
+  End Subroutine projection_release
+
+  !> projection.f90:21-34
+  Subroutine compute_projection_step
+    Integer(c_int) :: flags
+    Call ensure_plan
+    flags = 0
+    If ( rk_step == 3 ) flags = DFP_SAVE_P
+    If ( rk_step == 3 ) Then
+       Call dfp_check(dfp_project(plan, U, V, W, rhs_p, P, dt, rk_coef(3,3), flags), 'dfp_project')
+    Else
+       Call dfp_check(dfp_project(plan, U, V, W, rhs_p, no_p, dt, rk_coef(3,3), flags), 'dfp_project')
+    End If
+  End Subroutine compute_projection_step
+
+  !> projection.f90:37-63
+  Subroutine compute_pseudo_pressure_rhs
+    Call ensure_plan
+    Call dfp_check(dfp_divergence(plan, U, V, W, rhs_p, 0_c_int), 'dfp_divergence')
+  End Subroutine compute_pseudo_pressure_rhs
+
+  !> projection.f90:100-382
+  Subroutine solve_poisson_equation(skip_p_save)
+    Logical, Intent(In), Optional :: skip_p_save
+    Logical :: save_p
+    save_p = .True.
+    If ( Present(skip_p_save) ) save_p = .Not. skip_p_save
+    Call ensure_plan
+    If ( rk_step == 3 .And. save_p ) Then
+       Call dfp_check(dfp_solve(plan, rhs_p, P, dt, rk_coef(3,3), DFP_SAVE_P), 'dfp_solve')
+    Else
+       Call dfp_check(dfp_solve(plan, rhs_p, no_p, dt, rk_coef(3,3), 0_c_int), 'dfp_solve')
+    End If
+  End Subroutine solve_poisson_equation
+
+  !> projection.f90:551-604
+  Subroutine project_velocity
+    Call ensure_plan
+    Call dfp_check(dfp_correct(plan, U, V, W, rhs_p, 0_c_int), 'dfp_correct')
+  End Subroutine project_velocity
+
+  !> projection.f90:441-500 on one rank: the periodic duplicate planes of rhs_p
+  Subroutine apply_periodic_xz_pressure
+    rhs_p ( nxg-1, :, : ) = rhs_p ( 2, :, : )
+    rhs_p ( 2:nxg-1, :, nz ) = rhs_p ( 2:nxg-1, :, 2 )
+  End Subroutine apply_periodic_xz_pressure
+
+  !> projection.f90:503-524: on one rank there is no z neighbour, nothing to exchange
+  Subroutine update_ghost_interior_planes_pressure
+  End Subroutine update_ghost_interior_planes_pressure
+
+  !> projection.f90:529-548: on one rank there is no x neighbour, nothing to exchange
+  Subroutine update_ghost_interior_planes_pressure_x
+  End Subroutine update_ghost_interior_planes_pressure_x
+
+  !> projection.f90:388-438 with boundary_conditions.f90:277-289, :591-602 (id = 4) on one rank
+  Subroutine finish_pressure_ghosts(Pf)
+    Real(Int64), Intent(InOut) :: Pf(nxg,nyg,nzg)
+    Pf(:,  1,:) = Pf(:,    2,:)
+    Pf(:,nyg,:) = Pf(:,nyg-1,:)
+    Pf(    1,:,:) = Pf(nxg-2,:,:)
+    Pf(nxg-1,:,:) = Pf(    2,:,:)
+    Pf(nxg  ,:,:) = Pf(    3,:,:)
+    Pf(:,:,    1) = Pf(:,:,nzg-2)
+    Pf(:,:,nzg-1) = Pf(:,:,    2)
+    Pf(:,:,nzg  ) = Pf(:,:,    3)
+  End Subroutine finish_pressure_ghosts
+
+  !> projection.f90:607-639 (no immersed boundary)
+  Subroutine check_divergence(max_divergence)
+    Real(Int64), Intent(Out) :: max_divergence
+    Real(c_double) :: m
+    Call ensure_plan
+    Call dfp_check(dfp_max_divergence(plan, U, V, W, m, 0_c_int), 'dfp_max_divergence')
+    max_divergence = m
+  End Subroutine check_divergence
+
+End Module projection

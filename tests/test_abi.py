@@ -1,0 +1,88 @@
+"""The C ABI library: it loads, exports every function include/dfp.h
+declares, and rejects what this build does not support before touching a
+device (no compute calls: these tests run without a GPU)."""
+from __future__ import annotations
+
+import ctypes
+import re
+import subprocess
+
+import numpy as np
+import pytest
+
+import dfp
+from dfp import _lib
+
+
+def _declared_functions():
+    txt = open(_lib.HEADER_PATH).read()
+    return sorted(set(re.findall(r"^\s*(?:int|const char \*|dfp_plan \*)\s*(dfp_\w+)\s*\(", txt, re.M)))
+
+
+def test_header_declares_the_boundary():
+    names = _declared_functions()
+    for n in ("dfp_plan_create", "dfp_plan_destroy", "dfp_project", "dfp_divergence", "dfp_solve",
+              "dfp_correct", "dfp_max_divergence", "dfp_set_stream", "dfp_strerror"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    L = dfp.lib()
+    out = subprocess.run(["nm", "-D", "--defined-only", _lib.LIB_PATH], capture_output=True, text=True, check=True).stdout
+    exported = set(re.findall(r" T (dfp_\w+)$", out, re.M))
+    for n in _declared_functions():
+        assert n in exported, n
+        assert hasattr(L, n)
+
+
+def test_strerror_codes():
+    for code in (0, -1, -2, -3, -4, -5, -99):
+        assert isinstance(dfp.lib().dfp_strerror(code), bytes)
+    assert dfp.lib().dfp_strerror(-2) == b"unsupported configuration"
+
+
+def _grid_struct(g, bc=(0, 1, 0)):
+    dp = ctypes.POINTER(ctypes.c_double)
+    keep = [np.ascontiguousarray(a) for a in (g.x, g.y, g.z, g.yg, g.zg)]
+    ptrs = [a.ctypes.data_as(dp) for a in keep]
+    return _lib.DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, *ptrs, *bc), keep
+
+
+@pytest.mark.parametrize("bc", [(1, 1, 0), (0, 0, 0), (0, 1, 1)])
+def test_other_boundary_conditions_are_rejected(bc):
+    g = dfp.channel_grid(16, 8, 16, 1.5)
+    s, keep = _grid_struct(g, bc)
+    h = ctypes.c_void_p()
+    assert dfp.lib().dfp_plan_create(ctypes.byref(s), 0, 0, ctypes.byref(h)) == -2
+    assert not h.value
+
+
+def test_unsupported_sizes_and_arguments_are_rejected():
+    L = dfp.lib()
+    h = ctypes.c_void_p()
+    # FFT length with a factor 5 (nxp/2 = 5)
+    g = dfp.channel_grid(10, 8, 16, 1.5)
+    s, keep = _grid_struct(g)
+    assert L.dfp_plan_create(ctypes.byref(s), 0, 0, ctypes.byref(h)) == -2
+    g = dfp.channel_grid(16, 8, 20, 1.5)   # nzp = 20 = 4*5
+    s, keep = _grid_struct(g)
+    assert L.dfp_plan_create(ctypes.byref(s), 0, 0, ctypes.byref(h)) == -2
+    # null grid / null output
+    assert L.dfp_plan_create(None, 0, 0, ctypes.byref(h)) == -1
+    assert L.dfp_plan_create(ctypes.byref(s), 0, 0, None) == -1
+    # a null plan
+    assert L.dfp_project(None, None, None, None, None, None, 0.0, 0.0, 0) == -1
+    assert L.dfp_plan_destroy(None) == -1
+
+
+def test_odd_nxp_rejected_by_host_grid_and_abi():
+    with pytest.raises(ValueError):
+        dfp.channel_grid(15, 8, 16, 1.5)
+
+
+def test_host_grid_matches_reference_rules():
+    g = dfp.namelist_grid(18, 17, 18, 3.0, 2.0, 1.5, 2.0)  # chan_les_small
+    assert (g.nxp, g.nym, g.nzp) == (16, 16, 16)
+    assert g.dx == pytest.approx(3.0 / 17, rel=1e-14)
+    assert g.shape_u == (18, 18, 19) and g.shape_v == (19, 17, 19) and g.shape_w == (19, 18, 18)
+    assert g.shape_rhs == (18, 16, 18)
