@@ -1,9 +1,12 @@
 #!/usr/bin/env python3
-"""Benchmark: plain FP64 pressure projection at channel scale 256x192x256.
+"""Benchmark: plain FP64 pressure projection at channel scale 256x192x256,
+beside the FP32+refine form at the same size and at 512x384x512.
 
 One step = one low-storage RK3 time step = three projections (substages 1-3;
 the third saves P = phi/(dt*c)), each on freshly seeded Gaussian-perturbed
-Poiseuille fields already resident in HBM.  Prints one JSON line.
+Poiseuille fields already resident in HBM.  Prints one JSON line; `value`
+is the FP64 rate, the "mixed" object holds the FP32+refine rates and their
+relative L-inf error against the FP64 GPU path per sweep count.
 
   python bench.py [--gpus 1] [--steps 100] [--warmup 5]
 
@@ -70,6 +73,93 @@ def cpu_baseline(g, budget_s: float = 20.0) -> dict:
             "sample": f"{n} projections at {g.nxp}x{g.nym}x{g.nzp} FP64 (C oracle, OpenMP, own FFT), {el:.1f} s"}
 
 
+def make_sets(plan, g, n_sets, seed0):
+    import dfp
+
+    sets = []
+    for s in range(n_sets):
+        f = [dfp.DeviceArray(sh) for sh in (g.shape_u, g.shape_v, g.shape_w)]
+        plan.synth(f[0], f[1], f[2], seed=seed0 + s, sigma=SIGMA)
+        sets.append(f)
+    return sets
+
+
+def time_steps(plan, sets, rhs, P, K, Wm, first=0):
+    """Wm untimed then K timed RK3 steps (3 projections each, P saved at the
+    third); returns the HIP-event milliseconds of the K steps."""
+    from dfp import hipmem
+
+    n_sets = len(sets)
+
+    def step(i):
+        for s in range(3):
+            f = sets[(3 * i + s) % n_sets]
+            plan.project(f[0], f[1], f[2], rhs, P, DT, RK_C[2], save_p=(s == 2), asynchronous=True)
+
+    for i in range(first, first + Wm):
+        step(i)
+    hipmem.synchronize()
+    e0, e1 = hipmem.Event(), hipmem.Event()
+    e0.record()
+    for i in range(first + Wm, first + Wm + K):
+        step(i)
+    e1.record()
+    e1.synchronize()
+    return e0.elapsed_ms(e1)
+
+
+def mixed_errors(g, sweeps_list, seed):
+    """Relative L-inf error of the FP32+refine projection against the FP64
+    GPU path on one seeded field set, per sweep count (max over phi, P, U, V, W)."""
+    import dfp
+    from dfp import hipmem
+
+    p64 = dfp.Plan(g)
+    base = [dfp.DeviceArray(sh) for sh in (g.shape_u, g.shape_v, g.shape_w)]
+    p64.synth(base[0], base[1], base[2], seed=seed, sigma=SIGMA)
+    host = [b.download() for b in base]
+    del base
+
+    def run(plan):
+        f = [dfp.DeviceArray.from_host(a) for a in host]
+        rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
+        P = dfp.DeviceArray(g.shape_p, zero=True)
+        plan.project(f[0], f[1], f[2], rhs, P, DT, RK_C[2], save_p=True)
+        hipmem.synchronize()
+        return [a.download() for a in (*f, P, rhs)]
+
+    ref = run(p64)
+    p64.close()
+    out = {}
+    for s in sweeps_list:
+        p32 = dfp.Plan(g, precision=dfp.DFP_FP32_REFINE, sweeps=s)
+        got = run(p32)
+        p32.close()
+        out[str(s)] = max(float(np.max(np.abs(a - b)) / np.max(np.abs(b))) for a, b in zip(got, ref))
+        del got
+    return out
+
+
+def bench_mixed(g, K, Wm, sweeps, n_sets):
+    """FP32+refine projections/s on resident fields (same loop as FP64)."""
+    import dfp
+    from dfp import hipmem
+
+    plan = dfp.Plan(g, precision=dfp.DFP_FP32_REFINE, sweeps=sweeps)
+    sets = make_sets(plan, g, n_sets, 0x5EED0000)
+    rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
+    P = dfp.DeviceArray(g.shape_p, zero=True)
+    hipmem.synchronize()
+    ms = time_steps(plan, sets, rhs, P, K, Wm)
+    plan.profile(True)
+    plan.profile_reset()
+    time_steps(plan, sets, rhs, P, max(1, K // 5), 0, first=Wm + K)
+    prof = plan.profile_read()
+    plan.profile(False)
+    plan.close()
+    return 3 * K / (ms / 1e3), {k: round(1e3 * m / max(n, 1), 2) for k, (n, m) in prof.items()}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -77,6 +167,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu", action="store_true", help="skip the host-CPU baseline")
     ap.add_argument("--profile-steps", type=int, default=20)
+    ap.add_argument("--no-mixed", action="store_true", help="skip the FP32+refine measurements")
+    ap.add_argument("--sweeps", type=int, default=2, help="refinement sweeps of the timed FP32+refine form")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -106,11 +198,7 @@ def main() -> int:
     set_bytes = 8 * (np.prod(g.shape_u) + np.prod(g.shape_v) + np.prod(g.shape_w))
     free, _ = hipmem.mem_info()
     n_sets = int(max(3, min(n_proj, (0.6 * free) // set_bytes, (120 << 30) // set_bytes)))
-    sets = []
-    for s in range(n_sets):
-        f = [dfp.DeviceArray(sh) for sh in (g.shape_u, g.shape_v, g.shape_w)]
-        plan.synth(f[0], f[1], f[2], seed=0x5EED0000 + s, sigma=SIGMA)
-        sets.append(f)
+    sets = make_sets(plan, g, n_sets, 0x5EED0000)
     rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
     P = dfp.DeviceArray(g.shape_p, zero=True)
     hipmem.synchronize()
@@ -120,18 +208,9 @@ def main() -> int:
             f = sets[(3 * i + s) % n_sets]
             plan.project(f[0], f[1], f[2], rhs, P, DT, RK_C[2], save_p=(s == 2), asynchronous=True)
 
-    for i in range(Wm):
-        step(i)
-    hipmem.synchronize()
-    e0, e1 = hipmem.Event(), hipmem.Event()
     t0 = time.perf_counter()
-    e0.record()
-    for i in range(Wm, Wm + K):
-        step(i)
-    e1.record()
-    e1.synchronize()
+    total_ms = time_steps(plan, sets, rhs, P, K, Wm)
     wall = time.perf_counter() - t0
-    total_ms = e0.elapsed_ms(e1)
     ms_per_step = total_ms / K
     value = 3 * K / (total_ms / 1e3)
 
@@ -178,6 +257,28 @@ def main() -> int:
                        "kernels_us": {k: round(v["avg_us"], 2) for k, v in kern.items()},
                        "wall_s": round(wall, 3)},
     }
+    if not args.no_mixed:
+        plan.close()
+        del sets, rhs, P
+        n_mix = max(3, min(n_sets, 24))
+        mixed = {"sweeps": args.sweeps}
+        v32, k32 = bench_mixed(g, K, Wm, args.sweeps, n_mix)
+        mixed["256x192x256"] = {"projections_per_s": round(v32, 2), "speedup_vs_fp64": round(v32 / value, 3),
+                                "kernels_us": k32, "rel_err_vs_fp64_by_sweeps": mixed_errors(g, (0, 1, 2, 3), 4242)}
+        gl = dfp.channel_grid(2 * NXP, 2 * NYM, 2 * NZP, ALPHA)
+        pl = dfp.Plan(gl)
+        sl = make_sets(pl, gl, 3, 0x5EED1000)
+        rl = dfp.DeviceArray(gl.shape_rhs, zero=True)
+        Pl = dfp.DeviceArray(gl.shape_p, zero=True)
+        kl = max(2, K // 10)
+        v64l = 3 * kl / (time_steps(pl, sl, rl, Pl, kl, 1) / 1e3)
+        pl.close()
+        del sl, rl, Pl
+        v32l, k32l = bench_mixed(gl, kl, 1, args.sweeps, 3)
+        mixed["512x384x512"] = {"fp64_projections_per_s": round(v64l, 2), "projections_per_s": round(v32l, 2),
+                                "speedup_vs_fp64": round(v32l / v64l, 3), "kernels_us": k32l,
+                                "rel_err_vs_fp64_by_sweeps": mixed_errors(gl, (1, 2, 3), 4343)}
+        out["mixed"] = mixed
     if rank == 0 and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g)
     print(json.dumps(out))

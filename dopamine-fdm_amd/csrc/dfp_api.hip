@@ -46,6 +46,13 @@ bool factor(int n, FftDesc& d) {
     while (m % 2 == 0) { if (nst >= kMaxStages) return false; d.rad[nst++] = 2; m /= 2; }
     while (m % 3 == 0) { if (nst >= kMaxStages) return false; d.rad[nst++] = 3; m /= 3; }
     d.nst = nst;
+    int ncur = n, stride = 1;
+    for (int st = 0; st < nst; ++st) {
+        d.nbdiv[st] = FastDiv((uint32_t)(ncur / d.rad[st] * stride));
+        d.sdiv[st] = FastDiv((uint32_t)stride);
+        stride *= d.rad[st];
+        ncur /= d.rad[st];
+    }
     return m == 1;
 }
 
@@ -117,6 +124,11 @@ bool finite_all(const double* a, int n) {
 void free_plan_memory(dfp_plan* p) {
     (void)hipFree(p->d_tables);
     (void)hipFree(p->d_tw);
+    (void)hipFree(p->d_tw32);
+    (void)hipFree(p->d_b);
+    (void)hipFree(p->d_rowpart);
+    (void)hipFree(p->d_part0);
+    (void)hipFree(p->d_pbar);
     (void)hipFree(p->d_spec);
     (void)hipFree(p->d_lu);
     (void)hipFree(p->d_red);
@@ -165,9 +177,42 @@ int check_flags(dfp_plan* p, int flags) {
     return DFP_OK;
 }
 
+// FP32 form: one FP32 spectral solve with the (0,0) mode in FP64, then
+// p->sweeps refinement sweeps on the FP64 residual b - L_pin phi
+int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs, double* P, double dtc,
+                int save_p, int from_fields) {
+    const int nparts = ceil_div(p->geo.nzp / p->zr, p->tk_x);
+    XArgs a{};
+    a.U = U; a.V = V; a.W = W; a.rhs = rhs;
+    a.b_out = p->d_b;
+    a.rowpart = p->d_rowpart;
+    HIPCHK(p, launch_rfft_x32(p, from_fields ? 0 : 1, a));
+    HIPCHK(p, launch_mode00(p, p->d_rowpart, nparts, p->d_pbar));
+    HIPCHK(p, launch_zsolve32(p, -1));
+    HIPCHK(p, launch_zsolve32(p, +1));
+    HIPCHK(p, launch_irfft_x32(p, rhs, 0, p->d_pbar, p->d_part0));
+    for (int s = 0; s < p->sweeps; ++s) {
+        XArgs r{};
+        r.rhs = rhs;
+        r.b = p->d_b;
+        r.rowpart = p->d_rowpart;
+        r.part0 = p->d_part0;
+        r.idx2 = 1.0 / (p->dx * p->dx);
+        r.idz2 = 1.0 / (p->dz * p->dz);
+        HIPCHK(p, launch_rfft_x32(p, 2, r));
+        HIPCHK(p, launch_mode00(p, p->d_rowpart, nparts, p->d_pbar));
+        HIPCHK(p, launch_zsolve32(p, -1));
+        HIPCHK(p, launch_zsolve32(p, +1));
+        HIPCHK(p, launch_irfft_x32(p, rhs, 1, p->d_pbar, p->d_part0));
+    }
+    if (save_p) HIPCHK(p, launch_save_p(p, rhs, P, dtc));
+    return DFP_OK;
+}
+
 // the spectral chain on device arrays: rhs (read unless from_fields) -> phi in rhs (+P)
 int spectral_solve(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs, double* P, double dtc,
                    int save_p, int from_fields) {
+    if (p->precision == DFP_FP32_REFINE) return mixed_solve(p, U, V, W, rhs, P, dtc, save_p, from_fields);
     HIPCHK(p, launch_div_rfft_x(p, U, V, W, rhs, from_fields ? 0 : 1));
     HIPCHK(p, launch_zsolve(p, -1));
     HIPCHK(p, launch_zsolve(p, +1));
@@ -229,7 +274,7 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     *out = nullptr;
     if (!g || !g->y || !g->z || !g->yg || !g->zg) return DFP_E_ARG;
     if (g->x_bc_type != 0 || g->y_bc_type != 1 || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
-    if (precision != DFP_FP64) return DFP_E_UNSUPPORTED;
+    if (precision != DFP_FP64 && precision != DFP_FP32_REFINE) return DFP_E_ARG;
     if (g->nxp < 2 || (g->nxp & 1) || g->nym < 1 || g->nzp < 1) return DFP_E_UNSUPPORTED;
     // an x block holds zr lines of nxp/2 complex points in 32 KiB of LDS; a z
     // block holds one mode per thread
@@ -322,6 +367,15 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     p->tw_z = p->d_tw + o_tz;
     p->tw_r2c = p->d_tw + o_r2c;
     p->tw_zs = p->d_tw + o_zs;
+    if (precision == DFP_FP32_REFINE) {
+        std::vector<float2> tw32(tw.size());
+        for (size_t e = 0; e < tw.size(); ++e) tw32[e] = make_float2((float)tw[e].x, (float)tw[e].y);
+        if (hipMalloc(&p->d_tw32, sizeof(float2) * tw32.size()) != hipSuccess) return bail(DFP_E_NOMEM);
+        if (hipMemcpy(p->d_tw32, tw32.data(), sizeof(float2) * tw32.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(DFP_E_HIP);
+        p->tw_x32 = p->d_tw32;
+        p->tw_z32 = p->d_tw32 + o_tz;
+    }
 
     const size_t H = (size_t)G.nxh * G.nym * G.nzp;
     if (hipMalloc(&p->d_spec, sizeof(double2) * H) != hipSuccess) return bail(DFP_E_NOMEM);
@@ -343,13 +397,25 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     if (hipDeviceGetAttribute(&lds_attr, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds_attr > 0)
         p->lds_limit = (size_t)lds_attr;
     const int M = G.nxp / 2, nz2 = G.nzp / zr;
-    p->tk_x = std::max(1, std::min(nz2, 2048 / (zr * M)));
-    if (sizeof(double2) * ((size_t)zr * p->tk_x * (M + 1) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (zr - 1)) >
+    // columns per x block: the largest divisor of nz2 within about 2048 points
+    p->tk_x = 1;
+    for (int t = std::max(1, std::min(nz2, 2048 / (zr * M))); t >= 1; --t)
+        if (nz2 % t == 0) { p->tk_x = t; break; }
+    if (sizeof(double2) * ((size_t)zr * p->tk_x * (M + 1) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (zr - 1)) +
+                64 * sizeof(double) >
             p->lds_limit ||
         (size_t)zr * p->tk_x * M > (size_t)inplace_capacity(p->fx, 512, 2))
         return bail(DFP_E_UNSUPPORTED);
     p->g_zs = zsolve_rows_per_group(p->fz, G.nym, p->lds_limit);
     if (p->g_zs < 1) return bail(DFP_E_UNSUPPORTED);
+    if (precision == DFP_FP32_REFINE) {
+        const size_t nparts = (size_t)ceil_div(nz2, p->tk_x);
+        if (hipMalloc(&p->d_b, sizeof(double) * p->sz_rhs) != hipSuccess ||
+            hipMalloc(&p->d_rowpart, sizeof(double) * nparts * G.nym) != hipSuccess ||
+            hipMalloc(&p->d_part0, sizeof(double) * nparts) != hipSuccess ||
+            hipMalloc(&p->d_pbar, sizeof(double) * G.nym) != hipSuccess)
+            return bail(DFP_E_NOMEM);
+    }
     *out = p;
     return DFP_OK;
 }
@@ -360,6 +426,13 @@ int dfp_plan_destroy(dfp_plan* plan) {
     (void)hipStreamSynchronize(plan->stream);
     free_plan_memory(plan);
     delete plan;
+    return DFP_OK;
+}
+
+int dfp_set_refine_sweeps(dfp_plan* p, int sweeps) {
+    if (!p) return DFP_E_ARG;
+    if (sweeps < 0 || sweeps > 8) return fail(p, DFP_E_ARG, "refinement sweeps must be 0..8, got %d", sweeps);
+    p->sweeps = sweeps;
     return DFP_OK;
 }
 

@@ -13,6 +13,26 @@ namespace dfp {
 
 constexpr int kMaxStages = 16;
 
+// Division by a runtime constant d >= 1 for 0 <= n < 2^31:
+//   q = (umulhi(n, mul) + n) >> shift, shift = ceil(log2 d),
+//   mul = floor(2^32 (2^shift - d) / d) + 1  (round-up multiply method).
+struct FastDiv {
+    uint32_t d = 1, mul = 1, shift = 0;
+    FastDiv() = default;
+    explicit FastDiv(uint32_t dd) : d(dd) {
+        shift = 0;
+        while ((1ull << shift) < dd) ++shift;
+        mul = (uint32_t)(((1ull << 32) * ((1ull << shift) - dd)) / dd + 1);
+    }
+    __host__ __device__ __forceinline__ uint32_t div(uint32_t n) const {
+#ifdef __HIP_DEVICE_COMPILE__
+        return (__umulhi(n, mul) + n) >> shift;
+#else
+        return (uint32_t)((((uint64_t)n * mul) >> 32) + n) >> shift;
+#endif
+    }
+};
+
 // Stockham plan of one FFT length n = 2^a 3^b on the device.
 // Stage st has radix rad[st]; its twiddles exp(-2 pi i p u / ncur) for
 // p = 0..m-1, u = 1..r-1 sit at tw[twoff[st] + p*(r-1) + (u-1)], where
@@ -22,6 +42,8 @@ struct FftDesc {
     int nst;
     int rad[kMaxStages];
     int twoff[kMaxStages];
+    FastDiv nbdiv[kMaxStages];  // butterflies per line of stage st: (n / rad[st])
+    FastDiv sdiv[kMaxStages];   // stride of stage st: product of the earlier radices
 };
 
 // Geometry and 1-D metric tables of one case, as the kernels read it.
@@ -54,6 +76,21 @@ struct Ops {
     const double* kzz;  // nzp
 };
 
+// Inputs of the forward x-transform kernel beyond the spectrum.
+//   mode 0: U, V, W (divergence); mode 1: rhs; mode 2: rhs = phi and b (the
+//   residual b - L_pin phi of the FP32 form).
+struct XArgs {
+    const double* U;
+    const double* V;
+    const double* W;
+    const double* rhs;
+    const double* b;
+    double* b_out;          // modes 0, 1: FP64 copy of the field (rhs_p layout), or null
+    double* rowpart;        // per-(row, block) sums of the field, or null
+    const double* part0;    // mode 2: per-block sums of phi's first row
+    double idx2, idz2;      // mode 2: 1/dx^2, 1/dz^2
+};
+
 // Launch-time record of one kernel for the profiler.
 struct ProfSlot {
     std::string name;
@@ -79,11 +116,20 @@ struct dfp_plan {
     const double2* tw_z = nullptr;
     const double2* tw_r2c = nullptr;  // exp(-2 pi i k / nxp), k = 0..nxp/2
     const double2* tw_zs = nullptr;   // exp(-2 pi i k' u / nzp), [k'*(zr-1) + u-1], k' < nzp/zr, 1 <= u < zr
+    float2* d_tw32 = nullptr;         // the same table rounded to FP32 (FP32 form)
+    const float2* tw_x32 = nullptr;
+    const float2* tw_z32 = nullptr;
     // device work arrays
     double2* d_spec = nullptr;   // (nxp/2+1) x nym x nzp complex, layout [kx][u][j][m], kz = zr*m + u
     double* d_lu = nullptr;      // inverse pivots of the y solve, [kx][j][min(kz,nzp-kz)]
     double* d_red = nullptr;     // reduction partials
     int red_len = 0;
+    // FP32 form: the FP64 right-hand side, partial sums and the (0,0) mode
+    int sweeps = 2;              // refinement sweeps after the first FP32 solve
+    double* d_b = nullptr;       // rhs_p layout
+    double* d_rowpart = nullptr; // nym x (x-kernel blocks per row)
+    double* d_part0 = nullptr;   // x-kernel blocks per row
+    double* d_pbar = nullptr;    // nym
     // staging for host-resident fields
     double *d_U = nullptr, *d_V = nullptr, *d_W = nullptr, *d_rhs = nullptr, *d_P = nullptr;
     int64_t sz_u = 0, sz_v = 0, sz_w = 0, sz_p = 0, sz_rhs = 0;
@@ -106,9 +152,16 @@ namespace dfp {
 hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W,
                              const double* rhs, int from_rhs);
 hipError_t launch_lu_table(dfp_plan* p);
+// dir -1: z FFT + forward elimination, +1: back substitution + inverse z FFT
 hipError_t launch_zsolve(dfp_plan* p, int dir);
-int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit);  // -1: z FFT + forward elimination, +1: back substitution + inverse z FFT
+int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit);
 hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p);
+// FP32 form
+hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a);
+hipError_t launch_zsolve32(dfp_plan* p, int dir);
+hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0);
+hipError_t launch_mode00(dfp_plan* p, const double* rowpart, int nparts, double* x);
+hipError_t launch_save_p(dfp_plan* p, const double* rhs, double* P, double dtc);
 hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs);
 hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs);
 hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);

@@ -14,78 +14,103 @@
 
 namespace dfp {
 
-__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
-__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
-__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
-    return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-// a * conj(b)
-__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {
-    return make_double2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
-}
+// complex arithmetic on double2 (FP64 path) and float2 (FP32 spectral path)
+template <typename C> struct CplxT;
+template <> struct CplxT<double2> {
+    using real = double;
+    static __device__ __forceinline__ double2 make(double x, double y) { return make_double2(x, y); }
+};
+template <> struct CplxT<float2> {
+    using real = float;
+    static __device__ __forceinline__ float2 make(float x, float y) { return make_float2(x, y); }
+};
 
-template <int SIGN>
-__device__ __forceinline__ double2 twiddle(double2 a, double2 w) {
+template <typename C>
+__device__ __forceinline__ C cadd(C a, C b) { return CplxT<C>::make(a.x + b.x, a.y + b.y); }
+template <typename C>
+__device__ __forceinline__ C csub(C a, C b) { return CplxT<C>::make(a.x - b.x, a.y - b.y); }
+template <typename C>
+__device__ __forceinline__ C cmul(C a, C b) { return CplxT<C>::make(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+// a * conj(b)
+template <typename C>
+__device__ __forceinline__ C cmulc(C a, C b) { return CplxT<C>::make(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y); }
+
+template <int SIGN, typename C>
+__device__ __forceinline__ C twiddle(C a, C w) {
     return SIGN < 0 ? cmul(a, w) : cmulc(a, w);
 }
 
 // SIGN * i * a
-template <int SIGN>
-__device__ __forceinline__ double2 mul_si(double2 a) {
-    return SIGN < 0 ? make_double2(a.y, -a.x) : make_double2(-a.y, a.x);
+template <int SIGN, typename C>
+__device__ __forceinline__ C mul_si(C a) {
+    return SIGN < 0 ? CplxT<C>::make(a.y, -a.x) : CplxT<C>::make(-a.y, a.x);
+}
+
+// Sum over the block of one value per thread in a fixed order (butterflies
+// within each 64-lane wave, then the waves in order); every thread of the
+// block must call it and receives the total.  wsum holds NT/64 doubles.
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* wsum) {
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+    if ((threadIdx.x & 63) == 0) wsum[threadIdx.x >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    for (int q = 0; q < NT / 64; ++q) s += wsum[q];
+    __syncthreads();
+    return s;
 }
 
 // Transforms nl lines in LDS; line l, element e at buf[l*ld + e].  bufA holds
 // the input, bufB is scratch of the same size; the result ends in one of the
 // two and is returned.  Every thread of the block must call this; it ends
 // with a barrier.
-template <int SIGN>
-__device__ double2* lds_fft_lines(double2* bufA, double2* bufB, int nl, int ld, const FftDesc& d,
-                                  const double2* __restrict__ tw) {
-    double2* src = bufA;
-    double2* dst = bufB;
+template <int SIGN, typename C>
+__device__ C* lds_fft_lines(C* bufA, C* bufB, int nl, int ld, const FftDesc& d, const C* __restrict__ tw) {
+    using R = typename CplxT<C>::real;
+    C* src = bufA;
+    C* dst = bufB;
     int ncur = d.n, s = 1;
     for (int st = 0; st < d.nst; ++st) {
         const int r = d.rad[st];
         const int m = ncur / r;
         const int nb = m * s;  // butterflies per line
-        const double2* __restrict__ tws = tw + d.twoff[st];
+        const C* __restrict__ tws = tw + d.twoff[st];
         for (int b = threadIdx.x; b < nb * nl; b += blockDim.x) {
-            const int l = b / nb;
+            const int l = (int)d.nbdiv[st].div((uint32_t)b);
             const int bi = b - l * nb;
-            const int p = bi / s;
+            const int p = (int)d.sdiv[st].div((uint32_t)bi);
             const int q = bi - p * s;
-            const double2* sl = src + l * ld + q + s * p;
-            double2* dl = dst + l * ld + q + s * r * p;
+            const C* sl = src + l * ld + q + s * p;
+            C* dl = dst + l * ld + q + s * r * p;
             if (r == 4) {
-                const double2 a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m], a3 = sl[3 * s * m];
-                const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
-                const double2 s13 = cadd(a1, a3), d13 = csub(a1, a3);
-                const double2 jd13 = mul_si<SIGN>(d13);
+                const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m], a3 = sl[3 * s * m];
+                const C s02 = cadd(a0, a2), d02 = csub(a0, a2);
+                const C s13 = cadd(a1, a3), d13 = csub(a1, a3);
+                const C jd13 = mul_si<SIGN>(d13);
                 dl[0] = cadd(s02, s13);
-                const double2* w = tws + p * 3;
+                const C* w = tws + p * 3;
                 dl[s] = twiddle<SIGN>(cadd(d02, jd13), w[0]);
                 dl[2 * s] = twiddle<SIGN>(csub(s02, s13), w[1]);
                 dl[3 * s] = twiddle<SIGN>(csub(d02, jd13), w[2]);
             } else if (r == 2) {
-                const double2 a0 = sl[0], a1 = sl[s * m];
+                const C a0 = sl[0], a1 = sl[s * m];
                 dl[0] = cadd(a0, a1);
                 dl[s] = twiddle<SIGN>(csub(a0, a1), tws[p]);
             } else {  // r == 3
-                const double s3 = 0.86602540378443864676372317075294;
-                const double2 a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m];
-                const double2 t1 = cadd(a1, a2);
-                const double2 t2 = make_double2(a0.x - 0.5 * t1.x, a0.y - 0.5 * t1.y);
-                const double2 dd = csub(a1, a2);
-                const double2 jd = mul_si<SIGN>(make_double2(s3 * dd.x, s3 * dd.y));
-                const double2* w = tws + p * 2;
+                const R s3 = (R)0.86602540378443864676372317075294;
+                const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m];
+                const C t1 = cadd(a1, a2);
+                const C t2 = CplxT<C>::make(a0.x - (R)0.5 * t1.x, a0.y - (R)0.5 * t1.y);
+                const C dd = csub(a1, a2);
+                const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
+                const C* w = tws + p * 2;
                 dl[0] = cadd(a0, t1);
                 dl[s] = twiddle<SIGN>(cadd(t2, jd), w[0]);
                 dl[2 * s] = twiddle<SIGN>(csub(t2, jd), w[1]);
             }
         }
         __syncthreads();
-        double2* t = src;
+        C* t = src;
         src = dst;
         dst = t;
         s *= r;
@@ -102,50 +127,51 @@ namespace dfp {
 // in registers between a read phase and a write phase, so one LDS buffer
 // suffices.  Requires nl * n / r <= KMAX * blockDim.x for every stage radix r.
 // Ends with a barrier.
-template <int SIGN, int KMAX>
-__device__ void lds_fft_inplace(double2* buf, int nl, int ld, const FftDesc& d, const double2* tw) {
+template <int SIGN, int KMAX, typename C>
+__device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const C* tw) {
+    using R = typename CplxT<C>::real;
     int ncur = d.n, s = 1;
     for (int st = 0; st < d.nst; ++st) {
         const int r = d.rad[st];
         const int m = ncur / r;
         const int nb = m * s;
         const int tot = nb * nl;
-        const double2* tws = tw + d.twoff[st];
-        double2 v[KMAX][4];
+        const C* tws = tw + d.twoff[st];
+        C v[KMAX][4];
         int dsto[KMAX];
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
             const int b = threadIdx.x + k * blockDim.x;
             dsto[k] = -1;
             if (b < tot) {
-                const int l = b / nb;
+                const int l = (int)d.nbdiv[st].div((uint32_t)b);
                 const int bi = b - l * nb;
-                const int p = bi / s;
+                const int p = (int)d.sdiv[st].div((uint32_t)bi);
                 const int q = bi - p * s;
-                const double2* sl = buf + l * ld + q + s * p;
+                const C* sl = buf + l * ld + q + s * p;
                 dsto[k] = l * ld + q + s * r * p;
                 if (r == 4) {
-                    const double2 a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m], a3 = sl[3 * s * m];
-                    const double2 s02 = cadd(a0, a2), d02 = csub(a0, a2);
-                    const double2 s13 = cadd(a1, a3), d13 = csub(a1, a3);
-                    const double2 jd13 = mul_si<SIGN>(d13);
-                    const double2* w = tws + p * 3;
+                    const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m], a3 = sl[3 * s * m];
+                    const C s02 = cadd(a0, a2), d02 = csub(a0, a2);
+                    const C s13 = cadd(a1, a3), d13 = csub(a1, a3);
+                    const C jd13 = mul_si<SIGN>(d13);
+                    const C* w = tws + p * 3;
                     v[k][0] = cadd(s02, s13);
                     v[k][1] = twiddle<SIGN>(cadd(d02, jd13), w[0]);
                     v[k][2] = twiddle<SIGN>(csub(s02, s13), w[1]);
                     v[k][3] = twiddle<SIGN>(csub(d02, jd13), w[2]);
                 } else if (r == 2) {
-                    const double2 a0 = sl[0], a1 = sl[s * m];
+                    const C a0 = sl[0], a1 = sl[s * m];
                     v[k][0] = cadd(a0, a1);
                     v[k][1] = twiddle<SIGN>(csub(a0, a1), tws[p]);
                 } else {
-                    const double s3 = 0.86602540378443864676372317075294;
-                    const double2 a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m];
-                    const double2 t1 = cadd(a1, a2);
-                    const double2 t2 = make_double2(a0.x - 0.5 * t1.x, a0.y - 0.5 * t1.y);
-                    const double2 dd = csub(a1, a2);
-                    const double2 jd = mul_si<SIGN>(make_double2(s3 * dd.x, s3 * dd.y));
-                    const double2* w = tws + p * 2;
+                    const R s3 = (R)0.86602540378443864676372317075294;
+                    const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m];
+                    const C t1 = cadd(a1, a2);
+                    const C t2 = CplxT<C>::make(a0.x - (R)0.5 * t1.x, a0.y - (R)0.5 * t1.y);
+                    const C dd = csub(a1, a2);
+                    const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
+                    const C* w = tws + p * 2;
                     v[k][0] = cadd(a0, t1);
                     v[k][1] = twiddle<SIGN>(cadd(t2, jd), w[0]);
                     v[k][2] = twiddle<SIGN>(csub(t2, jd), w[1]);
@@ -156,7 +182,7 @@ __device__ void lds_fft_inplace(double2* buf, int nl, int ld, const FftDesc& d, 
 #pragma unroll
         for (int k = 0; k < KMAX; ++k) {
             if (dsto[k] >= 0) {
-                double2* dl = buf + dsto[k];
+                C* dl = buf + dsto[k];
                 dl[0] = v[k][0];
                 dl[s] = v[k][1];
                 if (r >= 3) dl[2 * s] = v[k][2];

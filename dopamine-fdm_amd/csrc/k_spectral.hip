@@ -23,8 +23,8 @@
 // complex FFT; X[k] = E[k] + w^k O[k], E = (Z[k]+conj Z[M-k])/2,
 // O = (Z[k]-conj Z[M-k])/(2i), w = exp(-2 pi i/nxp) gives the nxp/2+1 modes.
 // The x FFT stage twiddles, the packing twiddles w^k and the first z stage
-// twiddles sit back to back in the plan's twiddle array (tw, rtw, tzs) and
-// are staged into LDS with one copy.
+// twiddles sit back to back in the plan's twiddle array and are staged into
+// LDS with one copy (the FP32 form has its own rounded copy of the table).
 #include "fft_lds.h"
 
 namespace dfp {
@@ -32,25 +32,26 @@ namespace dfp {
 constexpr int kXThreads = 512;
 
 // small DFT of R <= 4 points held in a[0..R-1], in place, sign SIGN
-template <int SIGN>
-__device__ __forceinline__ void dft_small(double2 (&a)[4], int R) {
+template <int SIGN, typename C>
+__device__ __forceinline__ void dft_small(C (&a)[4], int R) {
+    using Rl = typename CplxT<C>::real;
     if (R == 2) {
-        const double2 s = cadd(a[0], a[1]), d = csub(a[0], a[1]);
+        const C s = cadd(a[0], a[1]), d = csub(a[0], a[1]);
         a[0] = s;
         a[1] = d;
     } else if (R == 3) {
-        const double s3 = 0.86602540378443864676372317075294;
-        const double2 t1 = cadd(a[1], a[2]);
-        const double2 t2 = make_double2(a[0].x - 0.5 * t1.x, a[0].y - 0.5 * t1.y);
-        const double2 dd = csub(a[1], a[2]);
-        const double2 jd = mul_si<SIGN>(make_double2(s3 * dd.x, s3 * dd.y));
+        const Rl s3 = (Rl)0.86602540378443864676372317075294;
+        const C t1 = cadd(a[1], a[2]);
+        const C t2 = CplxT<C>::make(a[0].x - (Rl)0.5 * t1.x, a[0].y - (Rl)0.5 * t1.y);
+        const C dd = csub(a[1], a[2]);
+        const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
         a[0] = cadd(a[0], t1);
         a[1] = cadd(t2, jd);
         a[2] = csub(t2, jd);
     } else if (R == 4) {
-        const double2 s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
-        const double2 s13 = cadd(a[1], a[3]), d13 = csub(a[1], a[3]);
-        const double2 jd13 = mul_si<SIGN>(d13);
+        const C s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+        const C s13 = cadd(a[1], a[3]), d13 = csub(a[1], a[3]);
+        const C jd13 = mul_si<SIGN>(d13);
         a[0] = cadd(s02, s13);
         a[1] = cadd(d02, jd13);
         a[2] = csub(s02, s13);
@@ -61,26 +62,40 @@ __device__ __forceinline__ void dft_small(double2 (&a)[4], int R) {
 // ---------------------------------------------------------------------------
 // divergence + x R2C + first z stage
 // ---------------------------------------------------------------------------
-template <int FROM_RHS>
-__global__ void __launch_bounds__(kXThreads) k_div_rfft_x(const double* __restrict__ U, const double* __restrict__ V,
-                                                          const double* __restrict__ W, const double* __restrict__ rhs,
-                                                          double2* __restrict__ spec, Geo g, FftDesc fd,
-                                                          const double2* __restrict__ tw, int ntw,
-                                                          const double2* __restrict__ rtw,
-                                                          const double2* __restrict__ tzs, int R, int T) {
-    extern __shared__ double2 smem[];
+// MODE 0: divergence of (U,V,W) (projection.f90:55-57); MODE 1: rhs_p as
+// given; MODE 2: the FP64 residual b - L_pin phi of the FP32 form, where
+// L_pin is the 7-point operator whose eigenvalues the spectral solve
+// inverts: 3-point stencils in x and z with spacings dx, dz (eigenvalues
+// kxx, kzz), Dyy in y, and the (0,0) pin of projection.f90:292, which in
+// physical space adds 0.5*Dyy(2,2)*mean_xz(phi(:,2,:)) on the first row.
+// C: double2 (FP64 spectrum) or float2 (FP32 spectrum; the field is still
+// formed in FP64 and rounded once).  a.b_out receives the FP64 field in the
+// rhs_p layout; a.rowpart the per-block sums of the field (FP64, fixed
+// order), rowpart[j*gridDim.x + blockIdx.x].
+template <int MODE, typename C>
+__global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XArgs a, C* __restrict__ spec, Geo g, Ops o, FftDesc fd,
+                                                          const C* __restrict__ tw, int ntw, int R, int T,
+                                                          FastDiv div_nxp, FastDiv div_t, int npart0) {
+    const double* __restrict__ U = a.U;
+    const double* __restrict__ V = a.V;
+    const double* __restrict__ W = a.W;
+    const double* __restrict__ rhs = a.rhs;
+    __shared__ double wsum[kXThreads / 64];
+    using Rl = typename CplxT<C>::real;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* smem = reinterpret_cast<C*>(smem_raw);
     const int M = fd.n;
     const int nxp = g.nxp;
     const int nz2 = g.nzp / R;
     const int j = blockIdx.y;
     const int k0 = blockIdx.x * T;
-    const int nt = min(T, nz2 - k0);
+    const int nt = T;  // T divides nz2
     const int nl = R * nt;
-    double2* A = smem;
-    double2* twl = smem + R * T * M;      // FFT stage twiddles
-    double2* rtl = twl + ntw;             // R2C unpack twiddles, M+1
-    double2* tzl = rtl + (M + 1);         // first z stage twiddles, nz2*(R-1)
-    double* Ar = reinterpret_cast<double*>(A);
+    C* A = smem;
+    C* twl = smem + R * T * M;      // FFT stage twiddles
+    C* rtl = twl + ntw;             // R2C unpack twiddles, M+1
+    C* tzl = rtl + (M + 1);         // first z stage twiddles, nz2*(R-1)
+    Rl* Ar = reinterpret_cast<Rl*>(A);
     // the three tables are contiguous in the plan's twiddle array
     for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = tw[e];
 
@@ -91,21 +106,45 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(const double* __restri
     const long pr_j = (long)(g.nxg - 1), pr_k = (long)(g.nxg - 1) * g.nym;
     const int total = nl * nxp;
     const double inv_dy = g.inv_dyf[j];
+    double mean0 = 0.0;
+    if (MODE == 2 && j == 0) {
+        for (int q = 0; q < npart0; ++q) mean0 += a.part0[q];
+        mean0 = mean0 / ((double)nxp * (double)g.nzp);
+    }
+    double rsum = 0.0;
     // four elements per thread per batch; every load is issued before any is used
     for (int base = 0; base < total; base += 4 * kXThreads) {
         double v[4];
         int dst[4];
+        long cell[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const int e0 = base + (int)threadIdx.x + q * kXThreads;
             const int e = min(e0, total - 1);
-            const int l = e / nxp;
+            const int l = (int)div_nxp.div((uint32_t)e);
             const int i = e - l * nxp;
-            const int t = l / nt;
+            const int t = (int)div_t.div((uint32_t)l);
             const int k = k0 + (l - t * nt) + t * nz2;
             dst[q] = e0 < total ? e : -1;
-            if (FROM_RHS) {
-                v[q] = rhs[i + j * pr_j + k * pr_k];
+            cell[q] = i + j * pr_j + k * pr_k;
+            if (MODE == 1) {
+                v[q] = rhs[cell[q]];
+            } else if (MODE == 2) {
+                // rhs holds phi; index nxp (x) and nzp (z) hold the periodic duplicates of 0
+                const double* ph = rhs + cell[q];
+                const double p0 = ph[0];
+                const double pxm = ph[i == 0 ? nxp - 1 : -1];
+                const double pxp = ph[1];
+                const double pzm = ph[k == 0 ? (long)(g.nzp - 1) * pr_k : -pr_k];
+                const double pzp = ph[pr_k];
+                const double pym = j > 0 ? ph[-pr_j] : 0.0;
+                const double pyp = j < g.nym - 1 ? ph[pr_j] : 0.0;
+                const double lo = j > 0 ? o.dl[j - 1] : 0.0;
+                const double up = j < g.nym - 1 ? o.du[j] : 0.0;
+                double lap = (pxp - 2.0 * p0 + pxm) * a.idx2 + (lo * pym + o.dd[j] * p0 + up * pyp) +
+                             (pzp - 2.0 * p0 + pzm) * a.idz2;
+                if (j == 0) lap += 0.5 * o.dd[0] * mean0;
+                v[q] = a.b[cell[q]] - lap;
             } else {
                 const long iu = (i + 1) + (j + 1) * pu_j + (k + 1) * pu_k;  // U(i+2,j+2,k+2)
                 const long iv = (i + 1) + (j + 1) * pv_j + (k + 1) * pv_k;  // V(i+2,j+2,k+2)
@@ -125,60 +164,88 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(const double* __restri
             }
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q)
-            if (dst[q] >= 0) Ar[dst[q]] = v[q];
+        for (int q = 0; q < 4; ++q) {
+            if (dst[q] >= 0) {
+                Ar[dst[q]] = (Rl)v[q];
+                if (MODE != 2 && a.b_out) a.b_out[cell[q]] = v[q];
+                rsum += v[q];
+            }
+        }
+    }
+    if (a.rowpart) {
+        const double s = block_sum<kXThreads>(rsum, wsum);
+        if (threadIdx.x == 0) a.rowpart[(long)j * gridDim.x + blockIdx.x] = s;
     }
     __syncthreads();
     lds_fft_inplace<-1, 2>(A, nl, M, fd, twl);
 
     // unpack the R2C modes of the R lines of each column, apply the first z stage
     for (int pidx = threadIdx.x; pidx < (M + 1) * nt; pidx += kXThreads) {
-        const int kx = pidx / nt;
+        const int kx = (int)div_t.div((uint32_t)pidx);
         const int c = pidx - kx * nt;
-        double2 a[4];
+        C a[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             if (t < R) {
-                const double2* Z = A + (t * nt + c) * M;
-                const double2 zk = Z[kx == M ? 0 : kx];
-                const double2 zm = Z[kx == 0 ? 0 : M - kx];
-                const double2 E = make_double2(0.5 * (zk.x + zm.x), 0.5 * (zk.y - zm.y));
-                const double2 D = make_double2(zk.x - zm.x, zk.y + zm.y);  // zk - conj(zm)
-                const double2 O = make_double2(0.5 * D.y, -0.5 * D.x);     // D / (2i)
+                const C* Z = A + (t * nt + c) * M;
+                const C zk = Z[kx == M ? 0 : kx];
+                const C zm = Z[kx == 0 ? 0 : M - kx];
+                const C E = CplxT<C>::make((Rl)0.5 * (zk.x + zm.x), (Rl)0.5 * (zk.y - zm.y));
+                const C D = CplxT<C>::make(zk.x - zm.x, zk.y + zm.y);  // zk - conj(zm)
+                const C O = CplxT<C>::make((Rl)0.5 * D.y, (Rl)-0.5 * D.x);  // D / (2i)
                 a[t] = cadd(E, cmul(rtl[kx], O));
             } else {
-                a[t] = make_double2(0.0, 0.0);
+                a[t] = CplxT<C>::make(0, 0);
             }
         }
         dft_small<-1>(a, R);
         const int kp = k0 + c;
-        double2* out = spec + ((long)kx * R * g.nym + j) * nz2 + kp;
+        C* out = spec + ((long)kx * R * g.nym + j) * nz2 + kp;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (u < R) {
-                const double2 y = u == 0 ? a[0] : cmul(a[u], tzl[kp * (R - 1) + (u - 1)]);
+                const C y = u == 0 ? a[0] : cmul(a[u], tzl[kp * (R - 1) + (u - 1)]);
                 out[(long)u * g.nym * nz2] = y;
             }
         }
     }
 }
 
-hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W, const double* rhs,
-                             int from_rhs) {
+template <typename C>
+static hipError_t launch_div_rfft_x_t(dfp_plan* p, int mode, const XArgs& a, C* spec, const C* tw) {
     const Geo& g = p->geo;
     const int R = p->zr, T = p->tk_x, nz2 = g.nzp / R;
     dim3 grid(ceil_div(nz2, T), g.nym);
     const int ntw = twiddle_count(p->fx);
-    const size_t lds = sizeof(double2) * ((size_t)R * T * p->fx.n + ntw + p->fx.n + 1 + (size_t)nz2 * (R - 1));
-    const int slot = prof_begin(p, from_rhs ? "rfft_x_rhs" : "div_rfft_x");
-    if (from_rhs)
-        hipLaunchKernelGGL(k_div_rfft_x<1>, grid, dim3(kXThreads), lds, p->stream, U, V, W, rhs, p->d_spec, g, p->fx,
-                           p->tw_x, ntw, p->tw_r2c, p->tw_zs, R, T);
+    const size_t lds = sizeof(C) * ((size_t)R * T * p->fx.n + ntw + p->fx.n + 1 + (size_t)nz2 * (R - 1));
+    const bool f32 = sizeof(C) == 8;
+    static const char* names[2][3] = {{"div_rfft_x", "rfft_x_rhs", "res_rfft_x"},
+                                      {"div_rfft_x32", "rfft_x_rhs32", "res_rfft_x32"}};
+    const int slot = prof_begin(p, names[f32 ? 1 : 0][mode]);
+    const FastDiv dn(g.nxp), dt(T);
+    const int np0 = grid.x;
+    if (mode == 0)
+        hipLaunchKernelGGL((k_div_rfft_x<0, C>), grid, dim3(kXThreads), lds, p->stream, a, spec, g, p->ops, p->fx, tw,
+                           ntw, R, T, dn, dt, np0);
+    else if (mode == 1)
+        hipLaunchKernelGGL((k_div_rfft_x<1, C>), grid, dim3(kXThreads), lds, p->stream, a, spec, g, p->ops, p->fx, tw,
+                           ntw, R, T, dn, dt, np0);
     else
-        hipLaunchKernelGGL(k_div_rfft_x<0>, grid, dim3(kXThreads), lds, p->stream, U, V, W, rhs, p->d_spec, g, p->fx,
-                           p->tw_x, ntw, p->tw_r2c, p->tw_zs, R, T);
+        hipLaunchKernelGGL((k_div_rfft_x<2, C>), grid, dim3(kXThreads), lds, p->stream, a, spec, g, p->ops, p->fx, tw,
+                           ntw, R, T, dn, dt, np0);
     prof_end(p, slot);
     return hipGetLastError();
+}
+
+hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W, const double* rhs,
+                             int from_rhs) {
+    XArgs a{};
+    a.U = U; a.V = V; a.W = W; a.rhs = rhs;
+    return launch_div_rfft_x_t<double2>(p, from_rhs ? 1 : 0, a, p->d_spec, p->tw_x);
+}
+
+hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a) {
+    return launch_div_rfft_x_t<float2>(p, mode, a, reinterpret_cast<float2*>(p->d_spec), p->tw_x32);
 }
 
 // ---------------------------------------------------------------------------
@@ -190,25 +257,34 @@ hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, cons
 // gives z = IFFT_M(Z) = x_unnormalised[2n] + i x_unnormalised[2n+1].
 // Lines are stored with stride M+1 (the nxp/2+1 modes, then Z in place).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kXThreads) k_irfft_x(const double2* __restrict__ spec, double* __restrict__ rhs,
+// C: spectrum precision.  ACCUM: phi(i,j,k) = phi_old + x*scale + pbar[j]
+// (the refinement sweeps of the FP32 form, pbar = the FP64 (0,0)-mode
+// correction of row j, may be null); otherwise phi = x*scale (+ pbar[j]).
+// part0, if given, receives the per-block sums of the new phi on the first
+// row (j = 0), part0[blockIdx.x] (FP64, fixed order).
+template <typename C, int ACCUM>
+__global__ void __launch_bounds__(kXThreads) k_irfft_x(const C* __restrict__ spec, double* __restrict__ rhs,
                                                        double* __restrict__ P, Geo g, FftDesc fd,
-                                                       const double2* __restrict__ tw, int ntw,
-                                                       const double2* __restrict__ rtw,
-                                                       const double2* __restrict__ tzs, int R, int T, double scale,
-                                                       double dtc, int save_p) {
-    extern __shared__ double2 smem[];
+                                                       const C* __restrict__ tw, int ntw, int R, int T, double scale,
+                                                       double dtc, int save_p, FastDiv div_nxp, FastDiv div_t,
+                                                       FastDiv div_half, const double* __restrict__ pbar,
+                                                       double* __restrict__ part0) {
+    __shared__ double wsum[kXThreads / 64];
+    using Rl = typename CplxT<C>::real;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* smem = reinterpret_cast<C*>(smem_raw);
     const int M = fd.n;
     const int ld = M + 1;
     const int nxp = g.nxp;
     const int nz2 = g.nzp / R;
     const int j = blockIdx.y;
     const int k0 = blockIdx.x * T;
-    const int nt = min(T, nz2 - k0);
+    const int nt = T;  // T divides nz2
     const int nl = R * nt;
-    double2* A = smem;
-    double2* twl = smem + R * T * ld;     // FFT stage twiddles
-    double2* rtl = twl + ntw;             // C2R packing twiddles, M+1
-    double2* tzl = rtl + (M + 1);         // last z stage twiddles, nz2*(R-1)
+    C* A = smem;
+    C* twl = smem + R * T * ld;     // FFT stage twiddles
+    C* rtl = twl + ntw;             // C2R packing twiddles, M+1
+    C* tzl = rtl + (M + 1);         // last z stage twiddles, nz2*(R-1)
     // the three tables are contiguous in the plan's twiddle array
     for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = tw[e];
     __syncthreads();
@@ -216,23 +292,23 @@ __global__ void __launch_bounds__(kXThreads) k_irfft_x(const double2* __restrict
     // gather the R sub-line values of each (kx, column), undo the first z stage
     const int npairs = (M + 1) * nt;
     for (int base = 0; base < npairs; base += 2 * kXThreads) {
-        double2 a[2][4];
+        C a[2][4];
         int pk[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int p0 = base + (int)threadIdx.x + h * kXThreads;
             const int pidx = min(p0, npairs - 1);
             pk[h] = p0 < npairs ? pidx : -1;
-            const int kx = pidx / nt;
+            const int kx = (int)div_t.div((uint32_t)pidx);
             const int c = pidx - kx * nt;
-            const double2* src = spec + ((long)kx * R * g.nym + j) * nz2 + k0 + c;
+            const C* src = spec + ((long)kx * R * g.nym + j) * nz2 + k0 + c;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) a[h][u] = u < R ? src[(long)u * g.nym * nz2] : make_double2(0.0, 0.0);
+            for (int u = 0; u < 4; ++u) a[h][u] = u < R ? src[(long)u * g.nym * nz2] : CplxT<C>::make(0, 0);
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (pk[h] < 0) continue;
-            const int kx = pk[h] / nt;
+            const int kx = (int)div_t.div((uint32_t)pk[h]);
             const int c = pk[h] - kx * nt;
             const int kp = k0 + c;
 #pragma unroll
@@ -248,23 +324,23 @@ __global__ void __launch_bounds__(kXThreads) k_irfft_x(const double2* __restrict
     // C2R packing in place: the thread of pair (k, M-k) writes Z[k] and Z[M-k]
     const int half = M / 2 + 1;
     for (int e = threadIdx.x; e < nl * half; e += kXThreads) {
-        const int l = e / half;
+        const int l = (int)div_half.div((uint32_t)e);
         const int k = e - l * half;
-        double2* X = A + l * ld;
+        C* X = A + l * ld;
         const int km = M - k;
-        const double2 xk = X[k];
-        const double2 xm = X[km];
+        const C xk = X[k];
+        const C xm = X[km];
         if (k < M) {
-            const double2 S = make_double2(xk.x + xm.x, xk.y - xm.y);  // X[k] + conj X[M-k]
-            const double2 D = make_double2(xk.x - xm.x, xk.y + xm.y);  // X[k] - conj X[M-k]
-            const double2 Tt = cmulc(D, rtl[k]);
-            X[k] = make_double2(S.x - Tt.y, S.y + Tt.x);
+            const C S = CplxT<C>::make(xk.x + xm.x, xk.y - xm.y);  // X[k] + conj X[M-k]
+            const C D = CplxT<C>::make(xk.x - xm.x, xk.y + xm.y);  // X[k] - conj X[M-k]
+            const C Tt = cmulc(D, rtl[k]);
+            X[k] = CplxT<C>::make(S.x - Tt.y, S.y + Tt.x);
         }
         if (km != k && km < M) {
-            const double2 S = make_double2(xm.x + xk.x, xm.y - xk.y);
-            const double2 D = make_double2(xm.x - xk.x, xm.y + xk.y);
-            const double2 Tt = cmulc(D, rtl[km]);
-            X[km] = make_double2(S.x - Tt.y, S.y + Tt.x);
+            const C S = CplxT<C>::make(xm.x + xk.x, xm.y - xk.y);
+            const C D = CplxT<C>::make(xm.x - xk.x, xm.y + xk.y);
+            const C Tt = cmulc(D, rtl[km]);
+            X[km] = CplxT<C>::make(S.x - Tt.y, S.y + Tt.x);
         }
     }
     __syncthreads();
@@ -272,12 +348,16 @@ __global__ void __launch_bounds__(kXThreads) k_irfft_x(const double2* __restrict
     const long pr_j = (long)(g.nxg - 1), pr_k = (long)(g.nxg - 1) * g.nym;
     const long pp_j = (long)g.nxg, pp_k = (long)g.nxg * g.nyg;
     double* rr = rhs + (long)j * pr_j;
+    double psum = 0.0;
     for (int e = threadIdx.x; e < nl * nxp; e += kXThreads) {
-        const int l = e / nxp;
+        const int l = (int)div_nxp.div((uint32_t)e);
         const int i = e - l * nxp;
-        const int t = l / nt;
+        const int t = (int)div_t.div((uint32_t)l);
         const int k = k0 + (l - t * nt) + t * nz2;
-        const double phi = reinterpret_cast<const double*>(A + l * ld)[i] * scale;
+        double phi = (double)reinterpret_cast<const Rl*>(A + l * ld)[i] * scale;
+        if (pbar) phi += pbar[j];
+        if (ACCUM) phi += rr[i + k * pr_k];
+        psum += phi;
         // rhs_p(i+2, j+2, k+2) and the duplicates (i = nxp -> nxg-1, k = nzp -> nzp+2)
         rr[i + k * pr_k] = phi;
         if (i == 0) rr[nxp + k * pr_k] = phi;
@@ -286,41 +366,62 @@ __global__ void __launch_bounds__(kXThreads) k_irfft_x(const double2* __restrict
             if (i == 0) rr[nxp + (long)g.nzp * pr_k] = phi;
         }
         if (save_p) {
+            // every copy of this cell in P (reference 1-based indices; 0 = none):
+            // x: i+2, 1 if i = nxp-1, nxg-1 if i = 0, nxg if i = 1
+            // y: j+2, 1 if j = 0, nyg if j = nym-1
+            // z: k+2, 1 if k = nzp-1, nzg-1 if k = 0, nzg if k = 1 (mod nzp)
             const double pv = phi / dtc;
-            // reference 1-based indices of every copy of this cell in P
-            int xs[4], ys[3], zs[4];
-            int nxs = 0, nys = 0, nzs = 0;
-            xs[nxs++] = i + 2;
-            if (i == nxp - 1) xs[nxs++] = 1;
-            if (i == 0) xs[nxs++] = g.nxg - 1;
-            if (i == 1) xs[nxs++] = g.nxg;
-            ys[nys++] = j + 2;
-            if (j == 0) ys[nys++] = 1;
-            if (j == g.nym - 1) ys[nys++] = g.nyg;
-            zs[nzs++] = k + 2;
-            if (k == g.nzp - 1) zs[nzs++] = 1;
-            if (k == 0) zs[nzs++] = g.nzg - 1;
-            if (k == 1 % g.nzp) zs[nzs++] = g.nzg;  // P(nzg) = P(3), the duplicate cell when nzp = 1
-            for (int c3 = 0; c3 < nzs; ++c3)
-                for (int b = 0; b < nys; ++b)
-                    for (int a = 0; a < nxs; ++a)
-                        P[(xs[a] - 1) + (long)(ys[b] - 1) * pp_j + (long)(zs[c3] - 1) * pp_k] = pv;
+            const int x1 = i == nxp - 1 ? 1 : 0, x2 = i == 0 ? g.nxg - 1 : 0, x3 = i == 1 ? g.nxg : 0;
+            const int y1 = j == 0 ? 1 : 0, y2 = j == g.nym - 1 ? g.nyg : 0;
+            const int z1 = k == g.nzp - 1 ? 1 : 0, z2 = k == 0 ? g.nzg - 1 : 0, z3 = k == 1 % g.nzp ? g.nzg : 0;
+#pragma unroll
+            for (int c3 = 0; c3 < 4; ++c3) {
+                const int zz = c3 == 0 ? k + 2 : (c3 == 1 ? z1 : (c3 == 2 ? z2 : z3));
+                if (zz == 0) continue;
+#pragma unroll
+                for (int b = 0; b < 3; ++b) {
+                    const int yy = b == 0 ? j + 2 : (b == 1 ? y1 : y2);
+                    if (yy == 0) continue;
+#pragma unroll
+                    for (int a = 0; a < 4; ++a) {
+                        const int xx = a == 0 ? i + 2 : (a == 1 ? x1 : (a == 2 ? x2 : x3));
+                        if (xx == 0) continue;
+                        P[(xx - 1) + (long)(yy - 1) * pp_j + (long)(zz - 1) * pp_k] = pv;
+                    }
+                }
+            }
         }
+    }
+    if (part0 && j == 0) {
+        const double sblk = block_sum<kXThreads>(psum, wsum);
+        if (threadIdx.x == 0) part0[blockIdx.x] = sblk;
     }
 }
 
-hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p) {
+template <typename C, int ACCUM>
+static hipError_t launch_irfft_x_t(dfp_plan* p, const C* spec, const C* tw, double* rhs, double* P, double dtc,
+                                   int save_p, const double* pbar, double* part0) {
     const Geo& g = p->geo;
     const int R = p->zr, T = p->tk_x, nz2 = g.nzp / R;
     dim3 grid(ceil_div(nz2, T), g.nym);
     const int ntw = twiddle_count(p->fx);
-    const size_t lds = sizeof(double2) * ((size_t)R * T * (p->fx.n + 1) + ntw + p->fx.n + 1 + (size_t)nz2 * (R - 1));
+    const size_t lds = sizeof(C) * ((size_t)R * T * (p->fx.n + 1) + ntw + p->fx.n + 1 + (size_t)nz2 * (R - 1));
     const double scale = 1.0 / ((double)g.nxp * (double)g.nzp);
-    const int slot = prof_begin(p, "irfft_x");
-    hipLaunchKernelGGL(k_irfft_x, grid, dim3(kXThreads), lds, p->stream, p->d_spec, rhs, P, g, p->fx, p->tw_x, ntw,
-                       p->tw_r2c, p->tw_zs, R, T, scale, dtc, save_p);
+    const int slot = prof_begin(p, sizeof(C) == 8 ? "irfft_x32" : "irfft_x");
+    hipLaunchKernelGGL((k_irfft_x<C, ACCUM>), grid, dim3(kXThreads), lds, p->stream, spec, rhs, P, g, p->fx, tw, ntw, R,
+                       T, scale, dtc, save_p, FastDiv(g.nxp), FastDiv(T), FastDiv(p->fx.n / 2 + 1), pbar, part0);
     prof_end(p, slot);
     return hipGetLastError();
+}
+
+hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p) {
+    return launch_irfft_x_t<double2, 0>(p, p->d_spec, p->tw_x, rhs, P, dtc, save_p, nullptr, nullptr);
+}
+
+hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0) {
+    const float2* spec = reinterpret_cast<const float2*>(p->d_spec);
+    if (accumulate) return launch_irfft_x_t<float2, 1>(p, spec, p->tw_x32, rhs, nullptr, 1.0, 0, pbar, part0);
+    return launch_irfft_x_t<float2, 0>(p, spec, p->tw_x32, rhs, nullptr, 1.0, 0, pbar, part0);
 }
 
 }  // namespace dfp

@@ -18,12 +18,13 @@ __device__ __forceinline__ long ridx(const Geo& g, int i, int j, int k) {
 }
 
 __global__ void __launch_bounds__(256) k_divergence(const double* __restrict__ U, const double* __restrict__ V,
-                                                    const double* __restrict__ W, double* __restrict__ rhs, Geo g) {
+                                                    const double* __restrict__ W, double* __restrict__ rhs, Geo g,
+                                                    FastDiv div_ni) {
 #pragma clang fp contract(off)
     const int ni = g.nxg - 2;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni * g.nym) return;
-    const int j = 2 + t / ni;                 // 2..nyg-1
+    const int j = 2 + (int)div_ni.div((uint32_t)t);                 // 2..nyg-1
     const int i = 2 + t - (j - 2) * ni;       // 2..nxg-1
     const int k = 2 + blockIdx.y;             // 2..nzg-1
     const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg;
@@ -39,7 +40,7 @@ hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, cons
     const Geo& g = p->geo;
     dim3 grid(ceil_div((long)(g.nxg - 2) * g.nym, 256), g.nzg - 2);
     const int slot = prof_begin(p, "divergence");
-    hipLaunchKernelGGL(k_divergence, grid, dim3(256), 0, p->stream, U, V, W, rhs, g);
+    hipLaunchKernelGGL(k_divergence, grid, dim3(256), 0, p->stream, U, V, W, rhs, g, FastDiv(g.nxg - 2));
     prof_end(p, slot);
     return hipGetLastError();
 }
@@ -47,36 +48,55 @@ hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, cons
 // U(i,j,k) -= (phi(i+1)-phi(i))/dx          i=2..nx-1,  j=2..nyg-1, k=2..nzg-1
 // V(i,j,k) -= (phi(j+1)-phi(j))/dyg(j)      i=2..nxg-1, j=2..ny-1,  k=2..nzg-1
 // W(i,j,k) -= (phi(k+1)-phi(k))/dzg(k)      i=2..nxg-1, j=2..nyg-1, k=2..nz-1
+// Each thread walks kCorrectK consecutive z planes of one (i,j) column, so
+// phi(k+1) read for W at one step is phi(k) of the next: phi is read from
+// memory once.
+constexpr int kCorrectK = 8;
+
 __global__ void __launch_bounds__(256) k_correct(double* __restrict__ U, double* __restrict__ V,
-                                                 double* __restrict__ W, const double* __restrict__ rhs, Geo g) {
+                                                 double* __restrict__ W, const double* __restrict__ rhs, Geo g,
+                                                 FastDiv div_ni) {
 #pragma clang fp contract(off)
     const int ni = g.nxg - 2;
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= ni * g.nym) return;
-    const int j = 2 + t / ni;
+    const int j = 2 + (int)div_ni.div((uint32_t)t);
     const int i = 2 + t - (j - 2) * ni;
-    const int k = 2 + blockIdx.y;
-    const long r = ridx(g, i, j, k);
-    const double phi = rhs[r];
-    if (i <= g.nx - 1) {
-        const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg;
-        U[iu] = U[iu] - (rhs[r + 1] - phi) * g.inv_dx;
-    }
-    if (j <= g.ny - 1) {
-        const long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.ny;
-        V[iv] = V[iv] - (rhs[r + (g.nxg - 1)] - phi) * g.inv_dyg[j - 2];
-    }
-    if (k <= g.nz - 1) {
-        const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.nyg;
-        W[iw] = W[iw] - (rhs[r + (long)(g.nxg - 1) * g.nym] - phi) * g.inv_dzg[k - 2];
+    const int k0 = 2 + blockIdx.y * kCorrectK;
+    const int k1 = min(k0 + kCorrectK - 1, g.nzg - 1);
+    const long pr_k = (long)(g.nxg - 1) * g.nym;
+    const long pu_k = (long)g.nx * g.nyg, pv_k = (long)g.nxg * g.ny, pw_k = (long)g.nxg * g.nyg;
+    const bool do_u = i <= g.nx - 1, do_v = j <= g.ny - 1;
+    const double idyg = do_v ? g.inv_dyg[j - 2] : 0.0;
+    long r = ridx(g, i, j, k0);
+    long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k0 - 1) * pu_k;
+    long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k0 - 1) * pv_k;
+    long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k0 - 1) * pw_k;
+    double phi = rhs[r];
+#pragma unroll
+    for (int q = 0; q < kCorrectK; ++q) {
+        const int k = k0 + q;
+        if (k > k1) break;
+        // phi(k+1) exists up to k = nzg-1 (rhs_p's last plane is nzg); only W uses it, for k <= nz-1
+        const bool do_w = k <= g.nz - 1;
+        const double phin = do_w ? rhs[r + pr_k] : 0.0;
+        if (do_u) U[iu] = U[iu] - (rhs[r + 1] - phi) * g.inv_dx;
+        if (do_v) V[iv] = V[iv] - (rhs[r + (g.nxg - 1)] - phi) * idyg;
+        if (do_w) W[iw] = W[iw] - (phin - phi) * g.inv_dzg[k - 2];
+        phi = do_w ? phin : phi;
+        r += pr_k;
+        iu += pu_k;
+        iv += pv_k;
+        iw += pw_k;
+        if (!do_w && k < k1) phi = rhs[r];
     }
 }
 
 hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs) {
     const Geo& g = p->geo;
-    dim3 grid(ceil_div((long)(g.nxg - 2) * g.nym, 256), g.nzg - 2);
+    dim3 grid(ceil_div((long)(g.nxg - 2) * g.nym, 256), ceil_div(g.nzg - 2, kCorrectK));
     const int slot = prof_begin(p, "correct");
-    hipLaunchKernelGGL(k_correct, grid, dim3(256), 0, p->stream, U, V, W, rhs, g);
+    hipLaunchKernelGGL(k_correct, grid, dim3(256), 0, p->stream, U, V, W, rhs, g, FastDiv(g.nxg - 2));
     prof_end(p, slot);
     return hipGetLastError();
 }

@@ -1,4 +1,4 @@
-// Fused z-transform + wall-normal tridiagonal solve (FP64), one block per
+// Fused z-transform + wall-normal tridiagonal solve, one block per
 // (kx, u) sub-plane of the split spectrum (see k_spectral.hip: the x kernel
 // already applied the first radix-R z stage, so each sub-plane holds
 // independent rows of length nz2 = nzp/R whose DFTs are the modes kz = R*m+u).
@@ -17,7 +17,9 @@
 //   then per mode b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1); b' overwrites them.
 // k_zsolve_bwd: rows j = nym-1..0: x(j) = (b'(j) - DU(j) x(j+1)) / D'(j), then
 //   the inverse FFT of the rows in LDS.
-// Both read and write the spectrum once (64 H bytes for the pair).  While a
+// Both read and write the spectrum once (64 H bytes for the pair in FP64,
+// 32 H in the FP32 form, whose z FFTs run in FP32 while the elimination
+// stays FP64 on values rounded to FP32 only where stored).  While a
 // group is transformed and swept, the next group's rows, pivots and
 // coefficients are already loading into registers.
 #include <algorithm>
@@ -66,14 +68,14 @@ constexpr int kZsPf = 8;  // group points per thread held in flight
     {                                                                             \
         const int e = min((int)threadIdx.x + (K) * kZsThreads, lim_);             \
         pr##K = src_[e];                                                          \
-        const int r = e / n;                                                      \
+        const int r = (int)div_n.div((uint32_t)e);                                \
         const int mm = e - r * n;                                                 \
         const int kz = R * mm + u;                                                \
         pl##K = lsrc_[(long)r * nzs + min(kz, nzp - kz)];                         \
     }
 #define ZS_PREFETCH(J0, GQ, COEF, COFF)                                           \
     do {                                                                          \
-        const double2* src_ = plane + (long)(J0) * n;                             \
+        const C* src_ = plane + (long)(J0) * n;                                   \
         const double* lsrc_ = lup + (long)(J0) * nzs;                             \
         const int lim_ = (GQ) * n - 1;                                            \
         ZS_LOAD1(0) ZS_LOAD1(1) ZS_LOAD1(2) ZS_LOAD1(3)                           \
@@ -97,31 +99,36 @@ constexpr int kZsPf = 8;  // group points per thread held in flight
         if ((int)threadIdx.x < (GR)) dus[threadIdx.x] = pd;                       \
     } while (0)
 #define ZS_REGS                                                                   \
-    double2 pr0, pr1, pr2, pr3, pr4, pr5, pr6, pr7;                               \
+    C pr0, pr1, pr2, pr3, pr4, pr5, pr6, pr7;                                     \
     double pl0, pl1, pl2, pl3, pl4, pl5, pl6, pl7;                                \
     double pd
 
-__global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(double2* __restrict__ spec, const double* __restrict__ lu,
+// C: spectrum precision (double2, or float2 for the FP32 form).  The
+// pivots, coefficients and the elimination itself are FP64 in both forms;
+// the FP32 form rounds only what it stores.
+template <typename C>
+__global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(C* __restrict__ spec, const double* __restrict__ lu,
                                                            const double* __restrict__ dl, int nym, int nzp, int R,
-                                                           FftDesc fz, const double2* __restrict__ tw, int ntw,
-                                                           int G) {
-    extern __shared__ double2 smem[];
+                                                           FftDesc fz, const C* __restrict__ tw, int ntw,
+                                                           int G, FastDiv div_n) {
+    using Rl = typename CplxT<C>::real;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* smem = reinterpret_cast<C*>(smem_raw);
     const int n = fz.n;
     const int nzs = nzp / 2 + 1;
     const int b = blockIdx.x;
     const int kx = b / R;
     const int u = b - kx * R;
-    double2* A = smem;
-    double2* twl = A + G * n;
+    C* A = smem;
+    C* twl = A + G * n;
     double* lus = reinterpret_cast<double*>(twl + ntw);
     double* dus = lus + G * n;
-    double2* plane = spec + (long)b * nym * n;
+    C* plane = spec + (long)b * nym * n;
     const double* lup = lu + (long)kx * nym * nzs;
     for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = tw[e];
     ZS_REGS;
     ZS_PREFETCH(0, min(G, nym), dl, -1);
-    double2 bp = make_double2(0.0, 0.0);
-    double ipp = 0.0;
+    double bpx = 0.0, bpy = 0.0, ipp = 0.0;
     const int m = threadIdx.x;  // mode of this thread (m < n)
     for (int j0 = 0; j0 < nym; j0 += G) {
         const int gr = min(G, nym - j0);
@@ -132,38 +139,45 @@ __global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(double2* __restrict__
         lds_fft_inplace<-1, 2>(A, gr, n, fz, twl);
         if (m < n) {
             for (int r = 0; r < gr; ++r) {
-                double2 bb = A[r * n + m];
+                const C a = A[r * n + m];
+                double bx = (double)a.x, by = (double)a.y;
                 if (j0 + r > 0) {
                     const double mult = dus[r] * ipp;  // DL(j-1) / D'(j-1)
-                    bb.x = bb.x - mult * bp.x;
-                    bb.y = bb.y - mult * bp.y;
+                    bx = bx - mult * bpx;
+                    by = by - mult * bpy;
                 }
-                A[r * n + m] = bb;
-                bp = bb;
+                A[r * n + m] = CplxT<C>::make((Rl)bx, (Rl)by);
+                bpx = bx;
+                bpy = by;
                 ipp = lus[r * n + m];
             }
         }
         __syncthreads();
-        double2* dst = plane + (long)j0 * n;
+        C* dst = plane + (long)j0 * n;
         for (int e = threadIdx.x; e < gr * n; e += kZsThreads) dst[e] = A[e];
     }
 }
 
-__global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(double2* __restrict__ spec, const double* __restrict__ lu,
+// zero00: the (0,0) mode is left out (set to zero); the FP32 form solves it
+// in FP64 separately
+template <typename C>
+__global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(C* __restrict__ spec, const double* __restrict__ lu,
                                                            const double* __restrict__ du, int nym, int nzp, int R,
-                                                           FftDesc fz, const double2* __restrict__ tw, int ntw,
-                                                           int G) {
-    extern __shared__ double2 smem[];
+                                                           FftDesc fz, const C* __restrict__ tw, int ntw,
+                                                           int G, FastDiv div_n, int zero00) {
+    using Rl = typename CplxT<C>::real;
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    C* smem = reinterpret_cast<C*>(smem_raw);
     const int n = fz.n;
     const int nzs = nzp / 2 + 1;
     const int b = blockIdx.x;
     const int kx = b / R;
     const int u = b - kx * R;
-    double2* A = smem;
-    double2* twl = A + G * n;
+    C* A = smem;
+    C* twl = A + G * n;
     double* lus = reinterpret_cast<double*>(twl + ntw);
     double* dus = lus + G * n;
-    double2* plane = spec + (long)b * nym * n;
+    C* plane = spec + (long)b * nym * n;
     const double* lup = lu + (long)kx * nym * nzs;
     for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = tw[e];
     // groups from the top: [j0, j0+gr), j0 = max(0, nym - (gi+1)*G)
@@ -173,8 +187,9 @@ __global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(double2* __restrict__
         const int j0q = max(0, nym - G);
         ZS_PREFETCH(j0q, nym - j0q, du, 0);
     }
-    double2 xn = make_double2(0.0, 0.0);
+    double xnx = 0.0, xny = 0.0;
     const int m = threadIdx.x;
+    const bool is00 = zero00 && kx == 0 && u == 0 && m == 0;
     for (int gi = 0; gi < ngroups; ++gi) {
         const int j0 = max(0, nym - (gi + 1) * G);
         const int gr = nym - gi * G - j0;
@@ -188,42 +203,56 @@ __global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(double2* __restrict__
         if (m < n) {
             for (int r = gr - 1; r >= 0; --r) {
                 const int j = j0 + r;
-                const double2 bb = A[r * n + m];
+                const C bb = A[r * n + m];
                 const double ip = lus[r * n + m];
-                double2 x;
+                double xx, xy;
                 if (j == nym - 1) {
-                    x = make_double2(bb.x * ip, bb.y * ip);
+                    xx = (double)bb.x * ip;
+                    xy = (double)bb.y * ip;
                 } else {
                     // x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
-                    x = make_double2((bb.x - dus[r] * xn.x) * ip, (bb.y - dus[r] * xn.y) * ip);
+                    xx = ((double)bb.x - dus[r] * xnx) * ip;
+                    xy = ((double)bb.y - dus[r] * xny) * ip;
                 }
-                A[r * n + m] = x;
-                xn = x;
+                if (is00) xx = xy = 0.0;
+                A[r * n + m] = CplxT<C>::make((Rl)xx, (Rl)xy);
+                xnx = xx;
+                xny = xy;
             }
         }
         __syncthreads();
         lds_fft_inplace<1, 2>(A, gr, n, fz, twl);
-        double2* dst = plane + (long)j0 * n;
+        C* dst = plane + (long)j0 * n;
         for (int e = threadIdx.x; e < gr * n; e += kZsThreads) dst[e] = A[e];
     }
 }
 
-hipError_t launch_zsolve(dfp_plan* p, int dir) {
+template <typename C>
+static hipError_t launch_zsolve_t(dfp_plan* p, int dir, C* spec, const C* tw, int zero00) {
     const Geo& g = p->geo;
     const int G = p->g_zs;
     const int n = p->fz.n;
     const int ntw = twiddle_count(p->fz);
-    const size_t lds = sizeof(double2) * ((size_t)G * n + ntw) + sizeof(double) * ((size_t)G * n + G);
-    const int slot = prof_begin(p, dir < 0 ? "zsolve_fwd" : "zsolve_bwd");
+    const size_t lds = sizeof(C) * ((size_t)G * n + ntw) + sizeof(double) * ((size_t)G * n + G);
+    const bool f32 = sizeof(C) == 8;
+    const int slot = prof_begin(p, dir < 0 ? (f32 ? "zsolve_fwd32" : "zsolve_fwd") : (f32 ? "zsolve_bwd32" : "zsolve_bwd"));
     const dim3 grid(g.nxh * p->zr);
     if (dir < 0)
-        hipLaunchKernelGGL(k_zsolve_fwd, grid, dim3(kZsThreads), lds, p->stream, p->d_spec, p->d_lu, p->ops.dl, g.nym,
-                           g.nzp, p->zr, p->fz, p->tw_z, ntw, G);
+        hipLaunchKernelGGL(k_zsolve_fwd<C>, grid, dim3(kZsThreads), lds, p->stream, spec, p->d_lu, p->ops.dl, g.nym, g.nzp,
+                           p->zr, p->fz, tw, ntw, G, FastDiv(n));
     else
-        hipLaunchKernelGGL(k_zsolve_bwd, grid, dim3(kZsThreads), lds, p->stream, p->d_spec, p->d_lu, p->ops.du, g.nym,
-                           g.nzp, p->zr, p->fz, p->tw_z, ntw, G);
+        hipLaunchKernelGGL(k_zsolve_bwd<C>, grid, dim3(kZsThreads), lds, p->stream, spec, p->d_lu, p->ops.du, g.nym, g.nzp,
+                           p->zr, p->fz, tw, ntw, G, FastDiv(n), zero00);
     prof_end(p, slot);
     return hipGetLastError();
+}
+
+hipError_t launch_zsolve(dfp_plan* p, int dir) {
+    return launch_zsolve_t<double2>(p, dir, p->d_spec, p->tw_z, 0);
+}
+
+hipError_t launch_zsolve32(dfp_plan* p, int dir) {
+    return launch_zsolve_t<float2>(p, dir, reinterpret_cast<float2*>(p->d_spec), p->tw_z32, 1);
 }
 
 // rows per group: the group fits the prefetch registers (kZsPf points per

@@ -58,6 +58,7 @@ def lib() -> ctypes.CDLL:
         "dfp_correct": ([vp, vp, vp, vp, vp, i32], i32),
         "dfp_max_divergence": ([vp, vp, vp, vp, ctypes.POINTER(d), i32], i32),
         "dfp_set_stream": ([vp, vp], i32),
+        "dfp_set_refine_sweeps": ([vp, i32], i32),
         "dfp_profile_enable": ([vp, i32], i32),
         "dfp_profile_read": ([vp, i32, ctypes.POINTER(ctypes.c_char_p), ctypes.POINTER(i64), ctypes.POINTER(d)], i32),
         "dfp_profile_reset": ([vp], i32),

@@ -68,8 +68,9 @@ def _ptr(a, shape, name):
 class Plan:
     """One ``dfp_plan``: operators, twiddles and device scratch of one grid."""
 
-    def __init__(self, g: ChannelGrid, device: int = 0, precision: int = DFP_FP64):
+    def __init__(self, g: ChannelGrid, device: int = 0, precision: int = DFP_FP64, sweeps: int | None = None):
         self.g = g
+        self.precision = precision
         self._arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in (g.x, g.y, g.z, g.yg, g.zg)]
         dp = ctypes.POINTER(ctypes.c_double)
         x, y, z, yg, zg = (a.ctypes.data_as(dp) for a in self._arrays)
@@ -77,6 +78,8 @@ class Plan:
         h = ctypes.c_void_p()
         check(lib().dfp_plan_create(ctypes.byref(self._grid), precision, device, ctypes.byref(h)), "dfp_plan_create")
         self._h = h
+        if sweeps is not None:
+            self.set_refine_sweeps(sweeps)
 
     @property
     def handle(self):
@@ -104,6 +107,10 @@ class Plan:
         if asynchronous and f & DFP_FIELDS_ON_DEVICE:
             f |= DFP_ASYNC
         return f
+
+    def set_refine_sweeps(self, sweeps: int):
+        """Refinement sweeps of an FP32+refine plan (0..8)."""
+        check(lib().dfp_set_refine_sweeps(self._h, int(sweeps)), "dfp_set_refine_sweeps", self._h)
 
     def set_stream(self, stream_handle: int | None):
         check(lib().dfp_set_stream(self._h, stream_handle), "dfp_set_stream", self._h)

@@ -43,7 +43,13 @@ typedef struct dfp_grid {
     int32_t x_bc_type, y_bc_type, z_bc_type; /* must be 0, 1, 0 in this build       */
 } dfp_grid;
 
-/* precision of the spectral solve */
+/* Precision of the spectral solve.
+ * DFP_FP64: the reference's algorithm in FP64 (projection.f90:100-382).
+ * DFP_FP32_REFINE: FP32 FFTs on an FP32-stored spectrum (the y elimination
+ *   runs in FP64 on the stored values), the kx=kz=0 mode, the mean-pressure
+ *   pin, the residual b - L_pin phi and the accumulation of phi in FP64, then
+ *   dfp_set_refine_sweeps() refinement sweeps (default 2).  No counterpart in
+ *   the reference, whose solve is FP64 only. */
 enum { DFP_FP64 = 0, DFP_FP32_REFINE = 1 };
 
 /* flags */
@@ -97,6 +103,10 @@ int dfp_correct(dfp_plan *plan, double *U, double *V, double *W, const double *r
  * check_divergence (projection.f90:607-639, no immersed boundary). */
 int dfp_max_divergence(dfp_plan *plan, const double *U, const double *V, const double *W,
                        double *out, int flags);
+
+/* Refinement sweeps of a DFP_FP32_REFINE plan, 0..8 (no effect on an FP64
+ * plan).  No counterpart in the reference. */
+int dfp_set_refine_sweeps(dfp_plan *plan, int sweeps);
 
 /* Launch stream for the plan's kernels (a hipStream_t; NULL = default stream). */
 int dfp_set_stream(dfp_plan *plan, void *hip_stream);

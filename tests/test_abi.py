@@ -86,3 +86,14 @@ def test_host_grid_matches_reference_rules():
     assert g.dx == pytest.approx(3.0 / 17, rel=1e-14)
     assert g.shape_u == (18, 18, 19) and g.shape_v == (19, 17, 19) and g.shape_w == (19, 18, 18)
     assert g.shape_rhs == (18, 16, 18)
+
+
+def test_precision_argument_is_checked():
+    L = dfp.lib()
+    g = dfp.channel_grid(16, 8, 16, 1.5)
+    s, keep = _grid_struct(g)
+    h = ctypes.c_void_p()
+    for bad in (-1, 2, 7):
+        assert L.dfp_plan_create(ctypes.byref(s), bad, 0, ctypes.byref(h)) == -1
+        assert not h.value
+    assert L.dfp_set_refine_sweeps(None, 2) == -1
