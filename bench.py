@@ -160,6 +160,102 @@ def bench_mixed(g, K, Wm, sweeps, n_sets):
     return 3 * K / (ms / 1e3), {k: round(1e3 * m / max(n, 1), 2) for k, (n, m) in prof.items()}
 
 
+def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
+    """48 uneven cases (nxp, nzp in {32..128}, nym in {32,64,96}): one batched
+    call (persistent launch per stage) against the same 48 projections issued
+    one plan at a time, in FP64 and in FP32+refine; tuned launch shapes."""
+    import dfp
+    from dfp import autotune, hipmem
+    from dfp.batch import Batch, Case
+    from dfp.workloads import batched_cases
+
+    specs = batched_cases(48)
+    table = autotune.load_table()
+    sizes = sorted({s.key for s in specs})
+    t0 = time.perf_counter()
+    autotune.tune_sizes(sizes, (dfp.DFP_FP64, dfp.DFP_FP32_REFINE), table)
+    tune_s = time.perf_counter() - t0
+    if save_tuning:
+        autotune.save_table(table, save_tuning)
+    out = {"cases": 48, "distinct_sizes": len(sizes), "tuning_s": round(tune_s, 1)}
+    for prec, name in ((dfp.DFP_FP64, "fp64"), (dfp.DFP_FP32_REFINE, "fp32_refine")):
+        plans, cases = [], []
+        for sp in specs:
+            g = dfp.channel_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
+            plan = autotune.tuned_plan(g, prec, table)
+            f = [dfp.DeviceArray(sh) for sh in (g.shape_u, g.shape_v, g.shape_w)]
+            plan.synth(*f, seed=sp.seed, sigma=sp.sigma)
+            rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
+            P = dfp.DeviceArray(g.shape_p, zero=True)
+            plans.append(plan)
+            cases.append(Case(plan, f[0], f[1], f[2], rhs, P, DT, RK_C[2], True))
+        b = Batch(max_cases=48, graphs=True)
+        occ = autotune.tune_batch(b, cases)
+        b.project(cases)
+        hipmem.synchronize()
+        e0, e1 = hipmem.Event(), hipmem.Event()
+        e0.record()
+        for _ in range(reps):
+            b.project(cases, asynchronous=True)
+        e1.record()
+        e1.synchronize()
+        batch_ms = e0.elapsed_ms(e1) / reps
+        # the same projections one plan at a time (async, one stream)
+        for cs in cases:
+            cs.plan.project(cs.U, cs.V, cs.W, cs.rhs, cs.P, DT, RK_C[2], save_p=True, asynchronous=True)
+        hipmem.synchronize()
+        e0.record()
+        for _ in range(reps):
+            for cs in cases:
+                cs.plan.project(cs.U, cs.V, cs.W, cs.rhs, cs.P, DT, RK_C[2], save_p=True, asynchronous=True)
+        e1.record()
+        e1.synchronize()
+        seq_ms = e0.elapsed_ms(e1) / reps
+        out[name] = {"batch_ms": round(batch_ms, 4), "projections_per_s": round(48 / (batch_ms / 1e3), 1),
+                     "sequential_ms": round(seq_ms, 4), "speedup_vs_sequential": round(seq_ms / batch_ms, 3),
+                     "occupancy": occ["occupancy"]}
+        b.close()
+        for pl in plans:
+            pl.close()
+    return out
+
+
+def bench_serving(n_requests: int = 600, rate: float = 200.0, n_ids: int = 64) -> dict:
+    """Poisson arrivals (mean rate req/s) of projection requests, sizes
+    log-uniform between 32^3 and 256x192x256 (2^a 3^b), half FP32+refine;
+    p50/p99 latency at that rate, and throughput with every request queued at
+    once against the same requests served one at a time by single plans."""
+    import dfp
+    from dfp import autotune, serving
+
+    pool = serving.make_case_pool(n_ids)
+    table = autotune.load_table()
+    trace = serving.poisson_requests(n_requests, rate, pool)
+    srv = serving.Server(hbm_budget=200e9, max_batch=32, n_streams=2, graphs=True, table=table)
+    reqs, wall = serving.run_trace(srv, trace, realtime=True)
+    live = {"requests": len(reqs), "rate_req_s": rate, "wall_s": round(wall, 3),
+            "throughput_req_s": round(len(reqs) / wall, 1), "batches": srv.launches,
+            "latency": serving.latency_stats(reqs), "resident_GB": round(srv.resident_bytes / 1e9, 2)}
+    # saturation: the same requests all queued at once (fields and plans already resident)
+    trace2 = serving.poisson_requests(n_requests, rate, pool)
+    reqs2, wall2 = serving.run_trace(srv, trace2, realtime=False)
+    sat = {"wall_s": round(wall2, 3), "throughput_req_s": round(len(reqs2) / wall2, 1), "batches": srv.launches}
+    # plain sequential: one request at a time, synchronous dfp_project on resident state
+    t0 = time.perf_counter()
+    for _, q in serving.poisson_requests(n_requests, rate, pool):
+        r = srv.resident[q.case_id]
+        plan = r.plan(q.precision)
+        plan.set_stream(None)
+        plan.synth(r.U, r.V, r.W, q.seed, q.sigma)
+        plan.project(r.U, r.V, r.W, r.rhs, r.P, serving.DT, serving.C3, save_p=True)
+    seq_wall = time.perf_counter() - t0
+    sat["sequential_wall_s"] = round(seq_wall, 3)
+    sat["sequential_req_s"] = round(n_requests / seq_wall, 1)
+    sat["speedup_vs_sequential"] = round(seq_wall / wall2, 3)
+    srv.close()
+    return {"poisson": live, "saturated": sat, "case_ids": n_ids}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,6 +265,9 @@ def main() -> int:
     ap.add_argument("--profile-steps", type=int, default=20)
     ap.add_argument("--no-mixed", action="store_true", help="skip the FP32+refine measurements")
     ap.add_argument("--sweeps", type=int, default=2, help="refinement sweeps of the timed FP32+refine form")
+    ap.add_argument("--no-batched", action="store_true", help="skip the 48-case batched measurement")
+    ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--save-tuning", default=None, help="write the autotuner table (with new entries) here")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
     args = ap.parse_args()
@@ -279,6 +378,10 @@ def main() -> int:
                                 "speedup_vs_fp64": round(v32l / v64l, 3), "kernels_us": k32l,
                                 "rel_err_vs_fp64_by_sweeps": mixed_errors(gl, (1, 2, 3), 4343)}
         out["mixed"] = mixed
+    if not args.no_batched:
+        out["batched"] = bench_batched(save_tuning=args.save_tuning)
+    if not args.no_serving:
+        out["serving"] = bench_serving()
     if rank == 0 and not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(g)
     print(json.dumps(out))

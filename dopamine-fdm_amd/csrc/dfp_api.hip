@@ -181,13 +181,12 @@ int check_flags(dfp_plan* p, int flags) {
 // p->sweeps refinement sweeps on the FP64 residual b - L_pin phi
 int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs, double* P, double dtc,
                 int save_p, int from_fields) {
-    const int nparts = ceil_div(p->geo.nzp / p->zr, p->tk_x);
     XArgs a{};
     a.U = U; a.V = V; a.W = W; a.rhs = rhs;
     a.b_out = p->d_b;
     a.rowpart = p->d_rowpart;
     HIPCHK(p, launch_rfft_x32(p, from_fields ? 0 : 1, a));
-    HIPCHK(p, launch_mode00(p, p->d_rowpart, nparts, p->d_pbar));
+    HIPCHK(p, launch_mode00(p));
     HIPCHK(p, launch_zsolve32(p, -1));
     HIPCHK(p, launch_zsolve32(p, +1));
     HIPCHK(p, launch_irfft_x32(p, rhs, 0, p->d_pbar, p->d_part0));
@@ -200,7 +199,7 @@ int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, 
         r.idx2 = 1.0 / (p->dx * p->dx);
         r.idz2 = 1.0 / (p->dz * p->dz);
         HIPCHK(p, launch_rfft_x32(p, 2, r));
-        HIPCHK(p, launch_mode00(p, p->d_rowpart, nparts, p->d_pbar));
+        HIPCHK(p, launch_mode00(p));
         HIPCHK(p, launch_zsolve32(p, -1));
         HIPCHK(p, launch_zsolve32(p, +1));
         HIPCHK(p, launch_irfft_x32(p, rhs, 1, p->d_pbar, p->d_part0));
@@ -270,6 +269,18 @@ static int prof_collect(dfp_plan* p) {
 extern "C" {
 
 int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out) {
+    return dfp_plan_create_tuned(g, precision, device, nullptr, out);
+}
+
+int dfp_plan_tuning(const dfp_plan* p, dfp_tuning* t) {
+    if (!p || !t) return DFP_E_ARG;
+    t->z_split = p->zr;
+    t->x_columns = p->tk_x;
+    t->z_rows = p->g_zs;
+    return DFP_OK;
+}
+
+int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const dfp_tuning* tune, dfp_plan** out) {
     if (!out) return DFP_E_ARG;
     *out = nullptr;
     if (!g || !g->y || !g->z || !g->yg || !g->zg) return DFP_E_ARG;
@@ -292,9 +303,16 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     // z split: the x kernels apply the first radix-zr stage of the z FFT, the
     // z kernels transform sub-rows of nzp/zr points, one mode per thread
     int zr = 1;
-    if (g->nzp % 4 == 0 && g->nzp / 4 >= 8) zr = 4;
-    else if (g->nzp % 2 == 0 && g->nzp / 2 >= 4) zr = 2;
-    else if (g->nzp % 3 == 0 && g->nzp / 3 >= 4) zr = 3;
+    if (tune && tune->z_split > 0) {
+        zr = tune->z_split;
+        if (zr > 4 || g->nzp % zr != 0) return bail(DFP_E_ARG);
+    } else if (g->nzp % 4 == 0 && g->nzp / 4 >= 8) {
+        zr = 4;
+    } else if (g->nzp % 2 == 0 && g->nzp / 2 >= 4) {
+        zr = 2;
+    } else if (g->nzp % 3 == 0 && g->nzp / 3 >= 4) {
+        zr = 3;
+    }
     p->zr = zr;
     if (!factor(g->nxp / 2, p->fx) || !factor(g->nzp / zr, p->fz)) return bail(DFP_E_UNSUPPORTED);
     if (hipSetDevice(device) != hipSuccess) return bail(DFP_E_HIP);
@@ -399,8 +417,13 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
     const int M = G.nxp / 2, nz2 = G.nzp / zr;
     // columns per x block: the largest divisor of nz2 within about 2048 points
     p->tk_x = 1;
-    for (int t = std::max(1, std::min(nz2, 2048 / (zr * M))); t >= 1; --t)
-        if (nz2 % t == 0) { p->tk_x = t; break; }
+    if (tune && tune->x_columns > 0) {
+        if (nz2 % tune->x_columns != 0) return bail(DFP_E_ARG);
+        p->tk_x = tune->x_columns;
+    } else {
+        for (int t = std::max(1, std::min(nz2, 2048 / (zr * M))); t >= 1; --t)
+            if (nz2 % t == 0) { p->tk_x = t; break; }
+    }
     if (sizeof(double2) * ((size_t)zr * p->tk_x * (M + 1) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (zr - 1)) +
                 64 * sizeof(double) >
             p->lds_limit ||
@@ -408,6 +431,10 @@ int dfp_plan_create(const dfp_grid* g, int precision, int device, dfp_plan** out
         return bail(DFP_E_UNSUPPORTED);
     p->g_zs = zsolve_rows_per_group(p->fz, G.nym, p->lds_limit);
     if (p->g_zs < 1) return bail(DFP_E_UNSUPPORTED);
+    if (tune && tune->z_rows > 0) {
+        if (tune->z_rows > p->g_zs) return bail(DFP_E_UNSUPPORTED);
+        p->g_zs = tune->z_rows;
+    }
     if (precision == DFP_FP32_REFINE) {
         const size_t nparts = (size_t)ceil_div(nz2, p->tk_x);
         if (hipMalloc(&p->d_b, sizeof(double) * p->sz_rhs) != hipSuccess ||
@@ -557,10 +584,15 @@ int dfp_max_divergence(dfp_plan* p, const double* U, const double* V, const doub
 }
 
 int dfp_synth_fields(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma) {
+    return dfp_synth_fields_ex(p, U, V, W, seed, sigma, 0);
+}
+
+int dfp_synth_fields_ex(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma, int flags) {
     if (!p || !U || !V || !W) return DFP_E_ARG;
+    if (flags & ~DFP_ASYNC) return fail(p, DFP_E_ARG, "unknown flag bits 0x%x", flags);
     HIPCHK(p, hipSetDevice(p->device));
     HIPCHK(p, launch_synth(p, U, V, W, seed, sigma));
-    HIPCHK(p, hipStreamSynchronize(p->stream));
+    if (!(flags & DFP_ASYNC)) HIPCHK(p, hipStreamSynchronize(p->stream));
     return DFP_OK;
 }
 

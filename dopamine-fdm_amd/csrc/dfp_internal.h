@@ -91,6 +91,66 @@ struct XArgs {
     double idx2, idz2;      // mode 2: 1/dx^2, 1/dz^2
 };
 
+// Per-case parameters of the x-direction kernels (k_spectral.hip).  C is
+// double2 or float2 (the spectrum's precision).
+template <typename C>
+struct XPar {
+    Geo g;
+    Ops o;
+    FftDesc fd;          // x FFT of length nxp/2
+    const C* tw;         // x stages | R2C packing (nxp/2+1) | first z stage (nz2*(R-1))
+    C* spec;
+    int ntw, R, T, nbx;  // stage-twiddle count, z split radix, columns per block, blocks per row
+    FastDiv div_nxp, div_t, div_half;
+    XArgs a;             // forward kernel inputs
+    double* rhs;         // inverse: phi out (and in, when accumulating)
+    double* P;
+    double scale, dtc;
+    int save_p;
+    const double* pbar;  // inverse: FP64 (0,0)-mode part added per row, or null
+    double* part0;       // inverse: per-block sums of phi's first row, or null
+};
+
+// Per-case parameters of the fused z-transform / y-solve kernels (k_zsolve.hip).
+template <typename C>
+struct ZPar {
+    C* spec;
+    const double* lu;
+    const double* dl;
+    const double* du;
+    FftDesc fz;          // z FFT of length nzp/R
+    const C* tw;
+    int nym, nzp, R, ntw, G, zero00, nblocks;
+    FastDiv div_n;
+};
+
+// Per-case parameters of the stencil kernels (k_stencil.hip, k_mixed.hip).
+struct CPar {
+    double* U;
+    double* V;
+    double* W;
+    const double* rhs;
+    Geo g;
+    FastDiv div_ni;
+    int nbx;             // blocks over the (x,y) plane
+};
+struct MPar {
+    const double* rowpart;
+    const double* lu;
+    Ops o;
+    double* x;
+    double count;
+    int nparts, nym, nzs;
+};
+struct SPar {
+    const double* rhs;
+    double* P;
+    Geo g;
+    double dtc;
+    FastDiv div_x, div_y;
+    int nblocks;
+};
+
 // Launch-time record of one kernel for the profiler.
 struct ProfSlot {
     std::string name;
@@ -160,12 +220,36 @@ hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int s
 hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a);
 hipError_t launch_zsolve32(dfp_plan* p, int dir);
 hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0);
-hipError_t launch_mode00(dfp_plan* p, const double* rowpart, int nparts, double* x);
+hipError_t launch_mode00(dfp_plan* p);  // rowpart -> pbar of the plan's FP32 buffers
+int correct_blocks_z(const Geo& g);
 hipError_t launch_save_p(dfp_plan* p, const double* rhs, double* P, double dtc);
 hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs);
 hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs);
 hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);
 hipError_t launch_synth(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma);
+
+// parameter records and batched (persistent) launches: cs, off (per-case
+// work offsets, off[0] = 0, off[*nc] = total items) and the case count *nc
+// are in device memory
+template <typename C> XPar<C> make_xpar(const dfp_plan* p);
+template <typename C> ZPar<C> make_zpar(const dfp_plan* p, int zero00);
+CPar make_cpar(const dfp_plan* p, double* U, double* V, double* W, const double* rhs);
+MPar make_mpar(const dfp_plan* p);
+SPar make_spar(const dfp_plan* p, const double* rhs, double* P, double dtc);
+size_t x_lds_bytes(const dfp_plan* p, int inv, int f32);
+size_t z_lds_bytes(const dfp_plan* p, int f32);
+template <typename C>
+hipError_t launch_xfwd_batch(hipStream_t s, int mode, const XPar<C>* cs, const int* off, const int* nc, int grid,
+                             size_t lds);
+template <typename C>
+hipError_t launch_xinv_batch(hipStream_t s, int accum, const XPar<C>* cs, const int* off, const int* nc, int grid,
+                             size_t lds);
+template <typename C>
+hipError_t launch_zsolve_batch(hipStream_t s, int dir, const ZPar<C>* cs, const int* off, const int* nc, int grid,
+                               size_t lds);
+hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* nc, int grid);
+hipError_t launch_mode00_batch(hipStream_t s, const MPar* cs, const int* nc, int grid);
+hipError_t launch_save_p_batch(hipStream_t s, const SPar* cs, const int* off, const int* nc, int grid);
 
 // profiler hooks used by the launchers
 int prof_begin(dfp_plan* p, const char* name);

@@ -1,5 +1,5 @@
-// Batched complex FFTs staged in LDS (FP64), Stockham decimation in
-// frequency with natural-order output, radix 2, 3 and 4.
+// Batched complex FFTs staged in LDS (FP64 or FP32), Stockham decimation in
+// frequency with natural-order output, radix 2, 3 and 4, in place.
 //
 // Stage st of a length-n transform with radix r, current length ncur and
 // stride s (= product of the radices already applied), m = ncur / r:
@@ -60,72 +60,9 @@ __device__ __forceinline__ double block_sum(double v, double* wsum) {
     return s;
 }
 
-// Transforms nl lines in LDS; line l, element e at buf[l*ld + e].  bufA holds
-// the input, bufB is scratch of the same size; the result ends in one of the
-// two and is returned.  Every thread of the block must call this; it ends
-// with a barrier.
-template <int SIGN, typename C>
-__device__ C* lds_fft_lines(C* bufA, C* bufB, int nl, int ld, const FftDesc& d, const C* __restrict__ tw) {
-    using R = typename CplxT<C>::real;
-    C* src = bufA;
-    C* dst = bufB;
-    int ncur = d.n, s = 1;
-    for (int st = 0; st < d.nst; ++st) {
-        const int r = d.rad[st];
-        const int m = ncur / r;
-        const int nb = m * s;  // butterflies per line
-        const C* __restrict__ tws = tw + d.twoff[st];
-        for (int b = threadIdx.x; b < nb * nl; b += blockDim.x) {
-            const int l = (int)d.nbdiv[st].div((uint32_t)b);
-            const int bi = b - l * nb;
-            const int p = (int)d.sdiv[st].div((uint32_t)bi);
-            const int q = bi - p * s;
-            const C* sl = src + l * ld + q + s * p;
-            C* dl = dst + l * ld + q + s * r * p;
-            if (r == 4) {
-                const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m], a3 = sl[3 * s * m];
-                const C s02 = cadd(a0, a2), d02 = csub(a0, a2);
-                const C s13 = cadd(a1, a3), d13 = csub(a1, a3);
-                const C jd13 = mul_si<SIGN>(d13);
-                dl[0] = cadd(s02, s13);
-                const C* w = tws + p * 3;
-                dl[s] = twiddle<SIGN>(cadd(d02, jd13), w[0]);
-                dl[2 * s] = twiddle<SIGN>(csub(s02, s13), w[1]);
-                dl[3 * s] = twiddle<SIGN>(csub(d02, jd13), w[2]);
-            } else if (r == 2) {
-                const C a0 = sl[0], a1 = sl[s * m];
-                dl[0] = cadd(a0, a1);
-                dl[s] = twiddle<SIGN>(csub(a0, a1), tws[p]);
-            } else {  // r == 3
-                const R s3 = (R)0.86602540378443864676372317075294;
-                const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m];
-                const C t1 = cadd(a1, a2);
-                const C t2 = CplxT<C>::make(a0.x - (R)0.5 * t1.x, a0.y - (R)0.5 * t1.y);
-                const C dd = csub(a1, a2);
-                const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
-                const C* w = tws + p * 2;
-                dl[0] = cadd(a0, t1);
-                dl[s] = twiddle<SIGN>(cadd(t2, jd), w[0]);
-                dl[2 * s] = twiddle<SIGN>(csub(t2, jd), w[1]);
-            }
-        }
-        __syncthreads();
-        C* t = src;
-        src = dst;
-        dst = t;
-        s *= r;
-        ncur = m;
-    }
-    return src;
-}
-
-}  // namespace dfp
-
-namespace dfp {
-
-// In-place variant: each thread holds the outputs of up to KMAX butterflies
-// in registers between a read phase and a write phase, so one LDS buffer
-// suffices.  Requires nl * n / r <= KMAX * blockDim.x for every stage radix r.
+// Transforms nl lines in LDS (line l, element e at buf[l*ld + e]) in place:
+// each thread holds the outputs of up to KMAX butterflies in registers
+// between a read phase and a write phase, so one LDS buffer suffices.  Requires nl * n / r <= KMAX * blockDim.x for every stage radix r.
 // Ends with a barrier.
 template <int SIGN, int KMAX, typename C>
 __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const C* tw) {

@@ -18,64 +18,129 @@ namespace dfp {
 // kx = kz = 0 (lu[j*nzs], pin of projection.f90:292 included) and the
 // elimination order of the spectral kernels.  One block; the solve itself is
 // one thread (nym sequential steps).
-__global__ void __launch_bounds__(256) k_mode00(const double* __restrict__ rowpart, int nparts, double count,
-                                                const double* __restrict__ lu, Ops o, int nym, int nzs,
-                                                double* __restrict__ x) {
+__device__ __forceinline__ void mode00_body(const MPar& m) {
+    const double* __restrict__ rowpart = m.rowpart;
+    const double* __restrict__ lu = m.lu;
+    double* __restrict__ x = m.x;
+    const int nym = m.nym, nparts = m.nparts;
+    const long nzs = m.nzs;
     for (int j = threadIdx.x; j < nym; j += blockDim.x) {
         double s = 0.0;
         for (int q = 0; q < nparts; ++q) s += rowpart[(long)j * nparts + q];
-        x[j] = s / count;
+        x[j] = s / m.count;
     }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    double yp = x[0];
-    for (int j = 1; j < nym; ++j) {
-        const double mult = o.dl[j - 1] * lu[(long)(j - 1) * nzs];
-        yp = x[j] - mult * yp;
-        x[j] = yp;
-    }
-    double xn = x[nym - 1] * lu[(long)(nym - 1) * nzs];
-    x[nym - 1] = xn;
-    for (int j = nym - 2; j >= 0; --j) {
-        xn = (x[j] - o.du[j] * xn) * lu[(long)j * nzs];
-        x[j] = xn;
+    if (threadIdx.x == 0) {
+        double yp = x[0];
+        for (int j = 1; j < nym; ++j) {
+            const double mult = m.o.dl[j - 1] * lu[(j - 1) * nzs];
+            yp = x[j] - mult * yp;
+            x[j] = yp;
+        }
+        double xn = x[nym - 1] * lu[(nym - 1) * nzs];
+        x[nym - 1] = xn;
+        for (int j = nym - 2; j >= 0; --j) {
+            xn = (x[j] - m.o.du[j] * xn) * lu[j * nzs];
+            x[j] = xn;
+        }
     }
 }
 
-hipError_t launch_mode00(dfp_plan* p, const double* rowpart, int nparts, double* x) {
+__global__ void __launch_bounds__(256) k_mode00(MPar m) { mode00_body(m); }
+
+// cases c = blockIdx.x, blockIdx.x + gridDim.x, ...
+__global__ void __launch_bounds__(256) kb_mode00(const MPar* __restrict__ cs, const int* __restrict__ pnc) {
+    const int nc = *pnc;
+    for (int c = blockIdx.x; c < nc; c += gridDim.x) {
+        mode00_body(cs[c]);
+        __syncthreads();
+    }
+}
+
+MPar make_mpar(const dfp_plan* p) {
+    MPar m{};
     const Geo& g = p->geo;
+    m.rowpart = p->d_rowpart;
+    m.lu = p->d_lu;
+    m.o = p->ops;
+    m.x = p->d_pbar;
+    m.count = (double)g.nxp * (double)g.nzp;
+    m.nparts = (g.nzp / p->zr) / p->tk_x;
+    m.nym = g.nym;
+    m.nzs = g.nzp / 2 + 1;
+    return m;
+}
+
+hipError_t launch_mode00(dfp_plan* p) {
     const int slot = prof_begin(p, "mode00");
-    hipLaunchKernelGGL(k_mode00, dim3(1), dim3(256), 0, p->stream, rowpart, nparts, (double)g.nxp * (double)g.nzp,
-                       p->d_lu, p->ops, g.nym, g.nzp / 2 + 1, x);
+    hipLaunchKernelGGL(k_mode00, dim3(1), dim3(256), 0, p->stream, make_mpar(p));
     prof_end(p, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_mode00_batch(hipStream_t s, const MPar* cs, const int* nc, int grid) {
+    hipLaunchKernelGGL(kb_mode00, dim3(grid), dim3(256), 0, s, cs, nc);
     return hipGetLastError();
 }
 
 // P = phi/(dt*c) with every ghost (projection.f90:361-380, :388-438 on one
 // rank): each P element copies the periodic / wall-clamped interior cell.
-__global__ void __launch_bounds__(256) k_save_p(const double* __restrict__ rhs, double* __restrict__ P, Geo g,
-                                                double dtc, FastDiv div_x, FastDiv div_y) {
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+__device__ __forceinline__ void save_p_body(const SPar& sp, int bx) {
+    const Geo& g = sp.g;
+    const long t = (long)bx * 256 + threadIdx.x;
     const long n = (long)g.nxg * g.nyg * g.nzg;
     if (t >= n) return;
-    const uint32_t tj = div_x.div((uint32_t)t);
+    const uint32_t tj = sp.div_x.div((uint32_t)t);
     const int i = (int)(t - (long)tj * g.nxg) + 1;  // reference 1-based indices
-    const uint32_t tk = div_y.div(tj);
+    const uint32_t tk = sp.div_y.div(tj);
     const int j = (int)(tj - tk * (uint32_t)g.nyg) + 1;
     const int k = (int)tk + 1;
     const int ii = (i - 2 + g.nxp) % g.nxp;
     const int jj = j < 2 ? 0 : (j > g.nym + 1 ? g.nym - 1 : j - 2);
     const int kk = (k - 2 + g.nzp) % g.nzp;
-    P[t] = rhs[ii + jj * (long)(g.nxg - 1) + kk * (long)(g.nxg - 1) * g.nym] / dtc;
+    sp.P[t] = sp.rhs[ii + jj * (long)(g.nxg - 1) + kk * (long)(g.nxg - 1) * g.nym] / sp.dtc;
+}
+
+__global__ void __launch_bounds__(256) k_save_p(SPar sp) { save_p_body(sp, blockIdx.x); }
+
+__global__ void __launch_bounds__(256) kb_save_p(const SPar* __restrict__ cs, const int* __restrict__ off,
+                                                 const int* __restrict__ pnc) {
+    const int nc = *pnc;
+    const int total = off[nc];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int lo = 0, hi = nc;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off[mid] <= w) lo = mid;
+            else hi = mid;
+        }
+        save_p_body(cs[lo], w - off[lo]);
+    }
+}
+
+SPar make_spar(const dfp_plan* p, const double* rhs, double* P, double dtc) {
+    SPar sp{};
+    const Geo& g = p->geo;
+    sp.rhs = rhs;
+    sp.P = P;
+    sp.g = g;
+    sp.dtc = dtc;
+    sp.div_x = FastDiv(g.nxg);
+    sp.div_y = FastDiv(g.nyg);
+    sp.nblocks = ceil_div((long)g.nxg * g.nyg * g.nzg, 256);
+    return sp;
 }
 
 hipError_t launch_save_p(dfp_plan* p, const double* rhs, double* P, double dtc) {
-    const Geo& g = p->geo;
-    const long n = (long)g.nxg * g.nyg * g.nzg;
+    const SPar sp = make_spar(p, rhs, P, dtc);
     const int slot = prof_begin(p, "save_p");
-    hipLaunchKernelGGL(k_save_p, dim3(ceil_div(n, 256)), dim3(256), 0, p->stream, rhs, P, g, dtc, FastDiv(g.nxg),
-                       FastDiv(g.nyg));
+    hipLaunchKernelGGL(k_save_p, dim3(sp.nblocks), dim3(256), 0, p->stream, sp);
     prof_end(p, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_save_p_batch(hipStream_t s, const SPar* cs, const int* off, const int* nc, int grid) {
+    hipLaunchKernelGGL(kb_save_p, dim3(grid), dim3(256), 0, s, cs, off, nc);
     return hipGetLastError();
 }
 

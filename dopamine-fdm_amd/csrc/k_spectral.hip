@@ -25,6 +25,11 @@
 // The x FFT stage twiddles, the packing twiddles w^k and the first z stage
 // twiddles sit back to back in the plan's twiddle array and are staged into
 // LDS with one copy (the FP32 form has its own rounded copy of the table).
+//
+// Every kernel is a block body over one (column block, y row) of one case;
+// k_* run it for one case (blockIdx = block coordinates), kb_* for a batch of
+// cases, each workgroup walking the work items w = blockIdx.x,
+// blockIdx.x + gridDim.x, ... of the concatenated per-case block lists.
 #include "fft_lds.h"
 
 namespace dfp {
@@ -71,33 +76,29 @@ __device__ __forceinline__ void dft_small(C (&a)[4], int R) {
 // C: double2 (FP64 spectrum) or float2 (FP32 spectrum; the field is still
 // formed in FP64 and rounded once).  a.b_out receives the FP64 field in the
 // rhs_p layout; a.rowpart the per-block sums of the field (FP64, fixed
-// order), rowpart[j*gridDim.x + blockIdx.x].
+// order), rowpart[j*nbx + bx].
 template <int MODE, typename C>
-__global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XArgs a, C* __restrict__ spec, Geo g, Ops o, FftDesc fd,
-                                                          const C* __restrict__ tw, int ntw, int R, int T,
-                                                          FastDiv div_nxp, FastDiv div_t, int npart0) {
-    const double* __restrict__ U = a.U;
-    const double* __restrict__ V = a.V;
-    const double* __restrict__ W = a.W;
-    const double* __restrict__ rhs = a.rhs;
-    __shared__ double wsum[kXThreads / 64];
+__device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsigned char* smem_raw) {
     using Rl = typename CplxT<C>::real;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double wsum[kXThreads / 64];
+    const Geo& g = q.g;
+    const XArgs& a = q.a;
     C* smem = reinterpret_cast<C*>(smem_raw);
-    const int M = fd.n;
+    const int M = q.fd.n;
     const int nxp = g.nxp;
+    const int R = q.R;
     const int nz2 = g.nzp / R;
-    const int j = blockIdx.y;
-    const int k0 = blockIdx.x * T;
-    const int nt = T;  // T divides nz2
+    const int k0 = bx * q.T;
+    const int nt = q.T;  // T divides nz2
     const int nl = R * nt;
+    const int ntw = q.ntw;
     C* A = smem;
-    C* twl = smem + R * T * M;      // FFT stage twiddles
+    C* twl = smem + nl * M;         // FFT stage twiddles
     C* rtl = twl + ntw;             // R2C unpack twiddles, M+1
     C* tzl = rtl + (M + 1);         // first z stage twiddles, nz2*(R-1)
     Rl* Ar = reinterpret_cast<Rl*>(A);
     // the three tables are contiguous in the plan's twiddle array
-    for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = tw[e];
+    for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = q.tw[e];
 
     // reference indices of cell (i,j,k): (i+2, j+2, k+2)
     const long pu_j = (long)g.nx, pu_k = (long)g.nx * g.nyg;
@@ -108,7 +109,7 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XArgs a, C* __restrict
     const double inv_dy = g.inv_dyf[j];
     double mean0 = 0.0;
     if (MODE == 2 && j == 0) {
-        for (int q = 0; q < npart0; ++q) mean0 += a.part0[q];
+        for (int t = 0; t < q.nbx; ++t) mean0 += a.part0[t];
         mean0 = mean0 / ((double)nxp * (double)g.nzp);
     }
     double rsum = 0.0;
@@ -118,20 +119,20 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XArgs a, C* __restrict
         int dst[4];
         long cell[4];
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const int e0 = base + (int)threadIdx.x + q * kXThreads;
+        for (int u = 0; u < 4; ++u) {
+            const int e0 = base + (int)threadIdx.x + u * kXThreads;
             const int e = min(e0, total - 1);
-            const int l = (int)div_nxp.div((uint32_t)e);
+            const int l = (int)q.div_nxp.div((uint32_t)e);
             const int i = e - l * nxp;
-            const int t = (int)div_t.div((uint32_t)l);
+            const int t = (int)q.div_t.div((uint32_t)l);
             const int k = k0 + (l - t * nt) + t * nz2;
-            dst[q] = e0 < total ? e : -1;
-            cell[q] = i + j * pr_j + k * pr_k;
+            dst[u] = e0 < total ? e : -1;
+            cell[u] = i + j * pr_j + k * pr_k;
             if (MODE == 1) {
-                v[q] = rhs[cell[q]];
+                v[u] = a.rhs[cell[u]];
             } else if (MODE == 2) {
                 // rhs holds phi; index nxp (x) and nzp (z) hold the periodic duplicates of 0
-                const double* ph = rhs + cell[q];
+                const double* ph = a.rhs + cell[u];
                 const double p0 = ph[0];
                 const double pxm = ph[i == 0 ? nxp - 1 : -1];
                 const double pxp = ph[1];
@@ -139,19 +140,19 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XArgs a, C* __restrict
                 const double pzp = ph[pr_k];
                 const double pym = j > 0 ? ph[-pr_j] : 0.0;
                 const double pyp = j < g.nym - 1 ? ph[pr_j] : 0.0;
-                const double lo = j > 0 ? o.dl[j - 1] : 0.0;
-                const double up = j < g.nym - 1 ? o.du[j] : 0.0;
-                double lap = (pxp - 2.0 * p0 + pxm) * a.idx2 + (lo * pym + o.dd[j] * p0 + up * pyp) +
+                const double lo = j > 0 ? q.o.dl[j - 1] : 0.0;
+                const double up = j < g.nym - 1 ? q.o.du[j] : 0.0;
+                double lap = (pxp - 2.0 * p0 + pxm) * a.idx2 + (lo * pym + q.o.dd[j] * p0 + up * pyp) +
                              (pzp - 2.0 * p0 + pzm) * a.idz2;
-                if (j == 0) lap += 0.5 * o.dd[0] * mean0;
-                v[q] = a.b[cell[q]] - lap;
+                if (j == 0) lap += 0.5 * q.o.dd[0] * mean0;
+                v[u] = a.b[cell[u]] - lap;
             } else {
                 const long iu = (i + 1) + (j + 1) * pu_j + (k + 1) * pu_k;  // U(i+2,j+2,k+2)
                 const long iv = (i + 1) + (j + 1) * pv_j + (k + 1) * pv_k;  // V(i+2,j+2,k+2)
                 const long iw = (i + 1) + (j + 1) * pw_j + (k + 1) * pw_k;  // W(i+2,j+2,k+2)
-                const double u1 = U[iu], u0 = U[iu - 1];
-                const double v1 = V[iv], v0 = V[iv - pv_j];
-                const double w1 = W[iw], w0 = W[iw - pw_k];
+                const double u1 = a.U[iu], u0 = a.U[iu - 1];
+                const double v1 = a.V[iv], v0 = a.V[iv - pv_j];
+                const double w1 = a.W[iw], w0 = a.W[iw - pw_k];
                 const double idz = g.inv_dzf[k];
                 {
 #pragma clang fp contract(off)
@@ -159,31 +160,31 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XArgs a, C* __restrict
                     const double du = (u1 - u0) * g.inv_dx;
                     const double dv = (v1 - v0) * inv_dy;
                     const double dw = (w1 - w0) * idz;
-                    v[q] = du + dv + dw;
+                    v[u] = du + dv + dw;
                 }
             }
         }
 #pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            if (dst[q] >= 0) {
-                Ar[dst[q]] = (Rl)v[q];
-                if (MODE != 2 && a.b_out) a.b_out[cell[q]] = v[q];
-                rsum += v[q];
+        for (int u = 0; u < 4; ++u) {
+            if (dst[u] >= 0) {
+                Ar[dst[u]] = (Rl)v[u];
+                if (MODE != 2 && a.b_out) a.b_out[cell[u]] = v[u];
+                rsum += v[u];
             }
         }
     }
     if (a.rowpart) {
-        const double s = block_sum<kXThreads>(rsum, wsum);
-        if (threadIdx.x == 0) a.rowpart[(long)j * gridDim.x + blockIdx.x] = s;
+        const double sb = block_sum<kXThreads>(rsum, wsum);
+        if (threadIdx.x == 0) a.rowpart[(long)j * q.nbx + bx] = sb;
     }
     __syncthreads();
-    lds_fft_inplace<-1, 2>(A, nl, M, fd, twl);
+    lds_fft_inplace<-1, 2>(A, nl, M, q.fd, twl);
 
     // unpack the R2C modes of the R lines of each column, apply the first z stage
     for (int pidx = threadIdx.x; pidx < (M + 1) * nt; pidx += kXThreads) {
-        const int kx = (int)div_t.div((uint32_t)pidx);
+        const int kx = (int)q.div_t.div((uint32_t)pidx);
         const int c = pidx - kx * nt;
-        C a[4];
+        C av[4];
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             if (t < R) {
@@ -193,59 +194,55 @@ __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XArgs a, C* __restrict
                 const C E = CplxT<C>::make((Rl)0.5 * (zk.x + zm.x), (Rl)0.5 * (zk.y - zm.y));
                 const C D = CplxT<C>::make(zk.x - zm.x, zk.y + zm.y);  // zk - conj(zm)
                 const C O = CplxT<C>::make((Rl)0.5 * D.y, (Rl)-0.5 * D.x);  // D / (2i)
-                a[t] = cadd(E, cmul(rtl[kx], O));
+                av[t] = cadd(E, cmul(rtl[kx], O));
             } else {
-                a[t] = CplxT<C>::make(0, 0);
+                av[t] = CplxT<C>::make(0, 0);
             }
         }
-        dft_small<-1>(a, R);
+        dft_small<-1>(av, R);
         const int kp = k0 + c;
-        C* out = spec + ((long)kx * R * g.nym + j) * nz2 + kp;
+        C* out = q.spec + ((long)kx * R * g.nym + j) * nz2 + kp;
 #pragma unroll
         for (int u = 0; u < 4; ++u) {
             if (u < R) {
-                const C y = u == 0 ? a[0] : cmul(a[u], tzl[kp * (R - 1) + (u - 1)]);
+                const C y = u == 0 ? av[0] : cmul(av[u], tzl[kp * (R - 1) + (u - 1)]);
                 out[(long)u * g.nym * nz2] = y;
             }
         }
     }
 }
 
-template <typename C>
-static hipError_t launch_div_rfft_x_t(dfp_plan* p, int mode, const XArgs& a, C* spec, const C* tw) {
-    const Geo& g = p->geo;
-    const int R = p->zr, T = p->tk_x, nz2 = g.nzp / R;
-    dim3 grid(ceil_div(nz2, T), g.nym);
-    const int ntw = twiddle_count(p->fx);
-    const size_t lds = sizeof(C) * ((size_t)R * T * p->fx.n + ntw + p->fx.n + 1 + (size_t)nz2 * (R - 1));
-    const bool f32 = sizeof(C) == 8;
-    static const char* names[2][3] = {{"div_rfft_x", "rfft_x_rhs", "res_rfft_x"},
-                                      {"div_rfft_x32", "rfft_x_rhs32", "res_rfft_x32"}};
-    const int slot = prof_begin(p, names[f32 ? 1 : 0][mode]);
-    const FastDiv dn(g.nxp), dt(T);
-    const int np0 = grid.x;
-    if (mode == 0)
-        hipLaunchKernelGGL((k_div_rfft_x<0, C>), grid, dim3(kXThreads), lds, p->stream, a, spec, g, p->ops, p->fx, tw,
-                           ntw, R, T, dn, dt, np0);
-    else if (mode == 1)
-        hipLaunchKernelGGL((k_div_rfft_x<1, C>), grid, dim3(kXThreads), lds, p->stream, a, spec, g, p->ops, p->fx, tw,
-                           ntw, R, T, dn, dt, np0);
-    else
-        hipLaunchKernelGGL((k_div_rfft_x<2, C>), grid, dim3(kXThreads), lds, p->stream, a, spec, g, p->ops, p->fx, tw,
-                           ntw, R, T, dn, dt, np0);
-    prof_end(p, slot);
-    return hipGetLastError();
+template <int MODE, typename C>
+__global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XPar<C> q) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    xfwd_body<MODE, C>(q, blockIdx.x, blockIdx.y, smem_raw);
 }
 
-hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W, const double* rhs,
-                             int from_rhs) {
-    XArgs a{};
-    a.U = U; a.V = V; a.W = W; a.rhs = rhs;
-    return launch_div_rfft_x_t<double2>(p, from_rhs ? 1 : 0, a, p->d_spec, p->tw_x);
+// case of work item w: the last c with off[c] <= w (off[0] = 0, off[nc] = total)
+__device__ __forceinline__ int find_case(const int* __restrict__ off, int nc, int w) {
+    int lo = 0, hi = nc;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
 }
 
-hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a) {
-    return launch_div_rfft_x_t<float2>(p, mode, a, reinterpret_cast<float2*>(p->d_spec), p->tw_x32);
+template <int MODE, typename C>
+__global__ void __launch_bounds__(kXThreads) kb_div_rfft_x(const XPar<C>* __restrict__ cs, const int* __restrict__ off,
+                                                           const int* __restrict__ pnc) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int nc = *pnc;
+    const int total = off[nc];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const int c = find_case(off, nc, w);
+        const XPar<C>& q = cs[c];
+        const int li = w - off[c];
+        const int j = li / q.nbx;
+        xfwd_body<MODE, C>(q, li - j * q.nbx, j, smem_raw);
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -260,71 +257,67 @@ hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a) {
 // C: spectrum precision.  ACCUM: phi(i,j,k) = phi_old + x*scale + pbar[j]
 // (the refinement sweeps of the FP32 form, pbar = the FP64 (0,0)-mode
 // correction of row j, may be null); otherwise phi = x*scale (+ pbar[j]).
-// part0, if given, receives the per-block sums of the new phi on the first
-// row (j = 0), part0[blockIdx.x] (FP64, fixed order).
+// q.part0, if given, receives the per-block sums of the new phi on the first
+// row (j = 0), part0[bx] (FP64, fixed order).
 template <typename C, int ACCUM>
-__global__ void __launch_bounds__(kXThreads) k_irfft_x(const C* __restrict__ spec, double* __restrict__ rhs,
-                                                       double* __restrict__ P, Geo g, FftDesc fd,
-                                                       const C* __restrict__ tw, int ntw, int R, int T, double scale,
-                                                       double dtc, int save_p, FastDiv div_nxp, FastDiv div_t,
-                                                       FastDiv div_half, const double* __restrict__ pbar,
-                                                       double* __restrict__ part0) {
-    __shared__ double wsum[kXThreads / 64];
+__device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsigned char* smem_raw) {
     using Rl = typename CplxT<C>::real;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
+    __shared__ double wsum[kXThreads / 64];
+    const Geo& g = q.g;
     C* smem = reinterpret_cast<C*>(smem_raw);
-    const int M = fd.n;
+    const int M = q.fd.n;
     const int ld = M + 1;
     const int nxp = g.nxp;
+    const int R = q.R;
     const int nz2 = g.nzp / R;
-    const int j = blockIdx.y;
-    const int k0 = blockIdx.x * T;
-    const int nt = T;  // T divides nz2
+    const int k0 = bx * q.T;
+    const int nt = q.T;  // T divides nz2
     const int nl = R * nt;
+    const int ntw = q.ntw;
     C* A = smem;
-    C* twl = smem + R * T * ld;     // FFT stage twiddles
+    C* twl = smem + nl * ld;        // FFT stage twiddles
     C* rtl = twl + ntw;             // C2R packing twiddles, M+1
     C* tzl = rtl + (M + 1);         // last z stage twiddles, nz2*(R-1)
     // the three tables are contiguous in the plan's twiddle array
-    for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = tw[e];
+    for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = q.tw[e];
     __syncthreads();
 
     // gather the R sub-line values of each (kx, column), undo the first z stage
     const int npairs = (M + 1) * nt;
     for (int base = 0; base < npairs; base += 2 * kXThreads) {
-        C a[2][4];
+        C av[2][4];
         int pk[2];
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             const int p0 = base + (int)threadIdx.x + h * kXThreads;
             const int pidx = min(p0, npairs - 1);
             pk[h] = p0 < npairs ? pidx : -1;
-            const int kx = (int)div_t.div((uint32_t)pidx);
+            const int kx = (int)q.div_t.div((uint32_t)pidx);
             const int c = pidx - kx * nt;
-            const C* src = spec + ((long)kx * R * g.nym + j) * nz2 + k0 + c;
+            const C* src = q.spec + ((long)kx * R * g.nym + j) * nz2 + k0 + c;
 #pragma unroll
-            for (int u = 0; u < 4; ++u) a[h][u] = u < R ? src[(long)u * g.nym * nz2] : CplxT<C>::make(0, 0);
+            for (int u = 0; u < 4; ++u) av[h][u] = u < R ? src[(long)u * g.nym * nz2] : CplxT<C>::make(0, 0);
         }
 #pragma unroll
         for (int h = 0; h < 2; ++h) {
             if (pk[h] < 0) continue;
-            const int kx = (int)div_t.div((uint32_t)pk[h]);
+            const int kx = (int)q.div_t.div((uint32_t)pk[h]);
             const int c = pk[h] - kx * nt;
             const int kp = k0 + c;
 #pragma unroll
             for (int u = 1; u < 4; ++u)
-                if (u < R) a[h][u] = cmulc(a[h][u], tzl[kp * (R - 1) + (u - 1)]);
-            dft_small<1>(a[h], R);
+                if (u < R) av[h][u] = cmulc(av[h][u], tzl[kp * (R - 1) + (u - 1)]);
+            dft_small<1>(av[h], R);
 #pragma unroll
             for (int t = 0; t < 4; ++t)
-                if (t < R) A[(t * nt + c) * ld + kx] = a[h][t];
+                if (t < R) A[(t * nt + c) * ld + kx] = av[h][t];
         }
     }
     __syncthreads();
     // C2R packing in place: the thread of pair (k, M-k) writes Z[k] and Z[M-k]
     const int half = M / 2 + 1;
     for (int e = threadIdx.x; e < nl * half; e += kXThreads) {
-        const int l = (int)div_half.div((uint32_t)e);
+        const int l = (int)q.div_half.div((uint32_t)e);
         const int k = e - l * half;
         C* X = A + l * ld;
         const int km = M - k;
@@ -344,18 +337,21 @@ __global__ void __launch_bounds__(kXThreads) k_irfft_x(const C* __restrict__ spe
         }
     }
     __syncthreads();
-    lds_fft_inplace<1, 2>(A, nl, ld, fd, twl);
+    lds_fft_inplace<1, 2>(A, nl, ld, q.fd, twl);
     const long pr_j = (long)(g.nxg - 1), pr_k = (long)(g.nxg - 1) * g.nym;
     const long pp_j = (long)g.nxg, pp_k = (long)g.nxg * g.nyg;
-    double* rr = rhs + (long)j * pr_j;
+    double* rr = q.rhs + (long)j * pr_j;
+    double* P = q.P;
+    const double scale = q.scale, dtc = q.dtc;
+    const int save_p = q.save_p;
     double psum = 0.0;
     for (int e = threadIdx.x; e < nl * nxp; e += kXThreads) {
-        const int l = (int)div_nxp.div((uint32_t)e);
+        const int l = (int)q.div_nxp.div((uint32_t)e);
         const int i = e - l * nxp;
-        const int t = (int)div_t.div((uint32_t)l);
+        const int t = (int)q.div_t.div((uint32_t)l);
         const int k = k0 + (l - t * nt) + t * nz2;
         double phi = (double)reinterpret_cast<const Rl*>(A + l * ld)[i] * scale;
-        if (pbar) phi += pbar[j];
+        if (q.pbar) phi += q.pbar[j];
         if (ACCUM) phi += rr[i + k * pr_k];
         psum += phi;
         // rhs_p(i+2, j+2, k+2) and the duplicates (i = nxp -> nxg-1, k = nzp -> nzp+2)
@@ -392,36 +388,151 @@ __global__ void __launch_bounds__(kXThreads) k_irfft_x(const C* __restrict__ spe
             }
         }
     }
-    if (part0 && j == 0) {
+    if (q.part0 && j == 0) {
         const double sblk = block_sum<kXThreads>(psum, wsum);
-        if (threadIdx.x == 0) part0[blockIdx.x] = sblk;
+        if (threadIdx.x == 0) q.part0[bx] = sblk;
     }
 }
 
 template <typename C, int ACCUM>
-static hipError_t launch_irfft_x_t(dfp_plan* p, const C* spec, const C* tw, double* rhs, double* P, double dtc,
-                                   int save_p, const double* pbar, double* part0) {
+__global__ void __launch_bounds__(kXThreads) k_irfft_x(XPar<C> q) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    xinv_body<C, ACCUM>(q, blockIdx.x, blockIdx.y, smem_raw);
+}
+
+template <typename C, int ACCUM>
+__global__ void __launch_bounds__(kXThreads) kb_irfft_x(const XPar<C>* __restrict__ cs, const int* __restrict__ off,
+                                                        const int* __restrict__ pnc) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int nc = *pnc;
+    const int total = off[nc];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const int c = find_case(off, nc, w);
+        const XPar<C>& q = cs[c];
+        const int li = w - off[c];
+        const int j = li / q.nbx;
+        xinv_body<C, ACCUM>(q, li - j * q.nbx, j, smem_raw);
+        __syncthreads();
+    }
+}
+
+// ---------------------------------------------------------------------------
+// host side: parameters, LDS sizes, launches
+// ---------------------------------------------------------------------------
+template <typename C>
+XPar<C> make_xpar(const dfp_plan* p) {
+    XPar<C> q{};
     const Geo& g = p->geo;
-    const int R = p->zr, T = p->tk_x, nz2 = g.nzp / R;
-    dim3 grid(ceil_div(nz2, T), g.nym);
-    const int ntw = twiddle_count(p->fx);
-    const size_t lds = sizeof(C) * ((size_t)R * T * (p->fx.n + 1) + ntw + p->fx.n + 1 + (size_t)nz2 * (R - 1));
-    const double scale = 1.0 / ((double)g.nxp * (double)g.nzp);
+    q.g = g;
+    q.o = p->ops;
+    q.fd = p->fx;
+    q.R = p->zr;
+    q.T = p->tk_x;
+    q.nbx = (g.nzp / p->zr) / p->tk_x;
+    q.ntw = twiddle_count(p->fx);
+    q.div_nxp = FastDiv(g.nxp);
+    q.div_t = FastDiv(p->tk_x);
+    q.div_half = FastDiv(p->fx.n / 2 + 1);
+    q.scale = 1.0 / ((double)g.nxp * (double)g.nzp);
+    q.dtc = 1.0;
+    if (sizeof(C) == sizeof(double2)) {
+        q.tw = reinterpret_cast<const C*>(p->tw_x);
+        q.spec = reinterpret_cast<C*>(p->d_spec);
+    } else {
+        q.tw = reinterpret_cast<const C*>(p->tw_x32);
+        q.spec = reinterpret_cast<C*>(p->d_spec);
+    }
+    return q;
+}
+template XPar<double2> make_xpar<double2>(const dfp_plan*);
+template XPar<float2> make_xpar<float2>(const dfp_plan*);
+
+// dynamic LDS of the forward (inv = 0) or inverse (inv = 1) x kernel of a plan
+size_t x_lds_bytes(const dfp_plan* p, int inv, int f32) {
+    const int R = p->zr, T = p->tk_x, nz2 = p->geo.nzp / R, M = p->fx.n;
+    const size_t cs = f32 ? sizeof(float2) : sizeof(double2);
+    return cs * ((size_t)R * T * (M + inv) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (R - 1));
+}
+
+static const char* xname(int mode, bool f32) {
+    static const char* names[2][3] = {{"div_rfft_x", "rfft_x_rhs", "res_rfft_x"},
+                                      {"div_rfft_x32", "rfft_x_rhs32", "res_rfft_x32"}};
+    return names[f32 ? 1 : 0][mode];
+}
+
+template <typename C>
+static hipError_t launch_xfwd(dfp_plan* p, int mode, const XPar<C>& q) {
+    const dim3 grid(q.nbx, q.g.nym);
+    const size_t lds = x_lds_bytes(p, 0, sizeof(C) == 8);
+    const int slot = prof_begin(p, xname(mode, sizeof(C) == 8));
+    if (mode == 0) hipLaunchKernelGGL((k_div_rfft_x<0, C>), grid, dim3(kXThreads), lds, p->stream, q);
+    else if (mode == 1) hipLaunchKernelGGL((k_div_rfft_x<1, C>), grid, dim3(kXThreads), lds, p->stream, q);
+    else hipLaunchKernelGGL((k_div_rfft_x<2, C>), grid, dim3(kXThreads), lds, p->stream, q);
+    prof_end(p, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W, const double* rhs,
+                             int from_rhs) {
+    XPar<double2> q = make_xpar<double2>(p);
+    q.a.U = U; q.a.V = V; q.a.W = W; q.a.rhs = rhs;
+    return launch_xfwd(p, from_rhs ? 1 : 0, q);
+}
+
+hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a) {
+    XPar<float2> q = make_xpar<float2>(p);
+    q.a = a;
+    return launch_xfwd(p, mode, q);
+}
+
+template <typename C, int ACCUM>
+static hipError_t launch_xinv(dfp_plan* p, const XPar<C>& q) {
+    const dim3 grid(q.nbx, q.g.nym);
+    const size_t lds = x_lds_bytes(p, 1, sizeof(C) == 8);
     const int slot = prof_begin(p, sizeof(C) == 8 ? "irfft_x32" : "irfft_x");
-    hipLaunchKernelGGL((k_irfft_x<C, ACCUM>), grid, dim3(kXThreads), lds, p->stream, spec, rhs, P, g, p->fx, tw, ntw, R,
-                       T, scale, dtc, save_p, FastDiv(g.nxp), FastDiv(T), FastDiv(p->fx.n / 2 + 1), pbar, part0);
+    hipLaunchKernelGGL((k_irfft_x<C, ACCUM>), grid, dim3(kXThreads), lds, p->stream, q);
     prof_end(p, slot);
     return hipGetLastError();
 }
 
 hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p) {
-    return launch_irfft_x_t<double2, 0>(p, p->d_spec, p->tw_x, rhs, P, dtc, save_p, nullptr, nullptr);
+    XPar<double2> q = make_xpar<double2>(p);
+    q.rhs = rhs; q.P = P; q.dtc = dtc; q.save_p = save_p;
+    return launch_xinv<double2, 0>(p, q);
 }
 
 hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0) {
-    const float2* spec = reinterpret_cast<const float2*>(p->d_spec);
-    if (accumulate) return launch_irfft_x_t<float2, 1>(p, spec, p->tw_x32, rhs, nullptr, 1.0, 0, pbar, part0);
-    return launch_irfft_x_t<float2, 0>(p, spec, p->tw_x32, rhs, nullptr, 1.0, 0, pbar, part0);
+    XPar<float2> q = make_xpar<float2>(p);
+    q.rhs = rhs; q.pbar = pbar; q.part0 = part0;
+    if (accumulate) return launch_xinv<float2, 1>(p, q);
+    return launch_xinv<float2, 0>(p, q);
 }
+
+// batched launches: cs, off and the case count nc in device memory, grid =
+// persistent workgroups
+template <typename C>
+hipError_t launch_xfwd_batch(hipStream_t s, int mode, const XPar<C>* cs, const int* off, const int* nc, int grid,
+                             size_t lds) {
+    if (mode == 0) hipLaunchKernelGGL((kb_div_rfft_x<0, C>), dim3(grid), dim3(kXThreads), lds, s, cs, off, nc);
+    else if (mode == 1) hipLaunchKernelGGL((kb_div_rfft_x<1, C>), dim3(grid), dim3(kXThreads), lds, s, cs, off, nc);
+    else hipLaunchKernelGGL((kb_div_rfft_x<2, C>), dim3(grid), dim3(kXThreads), lds, s, cs, off, nc);
+    return hipGetLastError();
+}
+template hipError_t launch_xfwd_batch<double2>(hipStream_t, int, const XPar<double2>*, const int*, const int*, int,
+                                               size_t);
+template hipError_t launch_xfwd_batch<float2>(hipStream_t, int, const XPar<float2>*, const int*, const int*, int,
+                                              size_t);
+
+template <typename C>
+hipError_t launch_xinv_batch(hipStream_t s, int accum, const XPar<C>* cs, const int* off, const int* nc, int grid,
+                             size_t lds) {
+    if (accum) hipLaunchKernelGGL((kb_irfft_x<C, 1>), dim3(grid), dim3(kXThreads), lds, s, cs, off, nc);
+    else hipLaunchKernelGGL((kb_irfft_x<C, 0>), dim3(grid), dim3(kXThreads), lds, s, cs, off, nc);
+    return hipGetLastError();
+}
+template hipError_t launch_xinv_batch<double2>(hipStream_t, int, const XPar<double2>*, const int*, const int*, int,
+                                               size_t);
+template hipError_t launch_xinv_batch<float2>(hipStream_t, int, const XPar<float2>*, const int*, const int*, int,
+                                              size_t);
 
 }  // namespace dfp

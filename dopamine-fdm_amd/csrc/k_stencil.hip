@@ -53,16 +53,19 @@ hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, cons
 // memory once.
 constexpr int kCorrectK = 8;
 
-__global__ void __launch_bounds__(256) k_correct(double* __restrict__ U, double* __restrict__ V,
-                                                 double* __restrict__ W, const double* __restrict__ rhs, Geo g,
-                                                 FastDiv div_ni) {
+__device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
 #pragma clang fp contract(off)
+    const Geo& g = c.g;
+    double* __restrict__ U = c.U;
+    double* __restrict__ V = c.V;
+    double* __restrict__ W = c.W;
+    const double* __restrict__ rhs = c.rhs;
     const int ni = g.nxg - 2;
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = bx * 256 + threadIdx.x;
     if (t >= ni * g.nym) return;
-    const int j = 2 + (int)div_ni.div((uint32_t)t);
+    const int j = 2 + (int)c.div_ni.div((uint32_t)t);
     const int i = 2 + t - (j - 2) * ni;
-    const int k0 = 2 + blockIdx.y * kCorrectK;
+    const int k0 = 2 + bz * kCorrectK;
     const int k1 = min(k0 + kCorrectK - 1, g.nzg - 1);
     const long pr_k = (long)(g.nxg - 1) * g.nym;
     const long pu_k = (long)g.nx * g.nyg, pv_k = (long)g.nxg * g.ny, pw_k = (long)g.nxg * g.nyg;
@@ -92,12 +95,48 @@ __global__ void __launch_bounds__(256) k_correct(double* __restrict__ U, double*
     }
 }
 
+__global__ void __launch_bounds__(256) k_correct(CPar c) { correct_body(c, blockIdx.x, blockIdx.y); }
+
+__global__ void __launch_bounds__(256) kb_correct(const CPar* __restrict__ cs, const int* __restrict__ off,
+                                                  const int* __restrict__ pnc) {
+    const int nc = *pnc;
+    const int total = off[nc];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        int lo = 0, hi = nc;
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (off[mid] <= w) lo = mid;
+            else hi = mid;
+        }
+        const CPar& c = cs[lo];
+        const int li = w - off[lo];
+        const int bz = li / c.nbx;
+        correct_body(c, li - bz * c.nbx, bz);
+    }
+}
+
+CPar make_cpar(const dfp_plan* p, double* U, double* V, double* W, const double* rhs) {
+    CPar c{};
+    c.U = U; c.V = V; c.W = W; c.rhs = rhs;
+    c.g = p->geo;
+    c.div_ni = FastDiv(p->geo.nxg - 2);
+    c.nbx = ceil_div((long)(p->geo.nxg - 2) * p->geo.nym, 256);
+    return c;
+}
+
+int correct_blocks_z(const Geo& g) { return ceil_div(g.nzg - 2, kCorrectK); }
+
 hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs) {
-    const Geo& g = p->geo;
-    dim3 grid(ceil_div((long)(g.nxg - 2) * g.nym, 256), ceil_div(g.nzg - 2, kCorrectK));
+    const CPar c = make_cpar(p, U, V, W, rhs);
+    dim3 grid(c.nbx, correct_blocks_z(p->geo));
     const int slot = prof_begin(p, "correct");
-    hipLaunchKernelGGL(k_correct, grid, dim3(256), 0, p->stream, U, V, W, rhs, g, FastDiv(g.nxg - 2));
+    hipLaunchKernelGGL(k_correct, grid, dim3(256), 0, p->stream, c);
     prof_end(p, slot);
+    return hipGetLastError();
+}
+
+hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* nc, int grid) {
+    hipLaunchKernelGGL(kb_correct, dim3(grid), dim3(256), 0, s, cs, off, nc);
     return hipGetLastError();
 }
 

@@ -105,27 +105,25 @@ constexpr int kZsPf = 8;  // group points per thread held in flight
 
 // C: spectrum precision (double2, or float2 for the FP32 form).  The
 // pivots, coefficients and the elimination itself are FP64 in both forms;
-// the FP32 form rounds only what it stores.
+// the FP32 form rounds only what it stores.  b = (kx, u) sub-plane index.
 template <typename C>
-__global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(C* __restrict__ spec, const double* __restrict__ lu,
-                                                           const double* __restrict__ dl, int nym, int nzp, int R,
-                                                           FftDesc fz, const C* __restrict__ tw, int ntw,
-                                                           int G, FastDiv div_n) {
+__device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
     using Rl = typename CplxT<C>::real;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
-    const int n = fz.n;
+    const int n = zp.fz.n;
+    const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G, ntw = zp.ntw;
+    const FastDiv div_n = zp.div_n;
+    const double* __restrict__ dl = zp.dl;
     const int nzs = nzp / 2 + 1;
-    const int b = blockIdx.x;
     const int kx = b / R;
     const int u = b - kx * R;
     C* A = smem;
     C* twl = A + G * n;
     double* lus = reinterpret_cast<double*>(twl + ntw);
     double* dus = lus + G * n;
-    C* plane = spec + (long)b * nym * n;
-    const double* lup = lu + (long)kx * nym * nzs;
-    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = tw[e];
+    C* plane = zp.spec + (long)b * nym * n;
+    const double* lup = zp.lu + (long)kx * nym * nzs;
+    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = zp.tw[e];
     ZS_REGS;
     ZS_PREFETCH(0, min(G, nym), dl, -1);
     double bpx = 0.0, bpy = 0.0, ipp = 0.0;
@@ -136,7 +134,7 @@ __global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(C* __restrict__ spec,
         ZS_STAGE(gr);
         __syncthreads();
         if (j0 + G < nym) ZS_PREFETCH(j0 + G, min(G, nym - j0 - G), dl, -1);
-        lds_fft_inplace<-1, 2>(A, gr, n, fz, twl);
+        lds_fft_inplace<-1, 2>(A, gr, n, zp.fz, twl);
         if (m < n) {
             for (int r = 0; r < gr; ++r) {
                 const C a = A[r * n + m];
@@ -161,25 +159,23 @@ __global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(C* __restrict__ spec,
 // zero00: the (0,0) mode is left out (set to zero); the FP32 form solves it
 // in FP64 separately
 template <typename C>
-__global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(C* __restrict__ spec, const double* __restrict__ lu,
-                                                           const double* __restrict__ du, int nym, int nzp, int R,
-                                                           FftDesc fz, const C* __restrict__ tw, int ntw,
-                                                           int G, FastDiv div_n, int zero00) {
+__device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
     using Rl = typename CplxT<C>::real;
-    extern __shared__ __align__(16) unsigned char smem_raw[];
     C* smem = reinterpret_cast<C*>(smem_raw);
-    const int n = fz.n;
+    const int n = zp.fz.n;
+    const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G, ntw = zp.ntw;
+    const FastDiv div_n = zp.div_n;
+    const double* __restrict__ du = zp.du;
     const int nzs = nzp / 2 + 1;
-    const int b = blockIdx.x;
     const int kx = b / R;
     const int u = b - kx * R;
     C* A = smem;
     C* twl = A + G * n;
     double* lus = reinterpret_cast<double*>(twl + ntw);
     double* dus = lus + G * n;
-    C* plane = spec + (long)b * nym * n;
-    const double* lup = lu + (long)kx * nym * nzs;
-    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = tw[e];
+    C* plane = zp.spec + (long)b * nym * n;
+    const double* lup = zp.lu + (long)kx * nym * nzs;
+    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = zp.tw[e];
     // groups from the top: [j0, j0+gr), j0 = max(0, nym - (gi+1)*G)
     const int ngroups = (nym + G - 1) / G;
     ZS_REGS;
@@ -189,7 +185,7 @@ __global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(C* __restrict__ spec,
     }
     double xnx = 0.0, xny = 0.0;
     const int m = threadIdx.x;
-    const bool is00 = zero00 && kx == 0 && u == 0 && m == 0;
+    const bool is00 = zp.zero00 && kx == 0 && u == 0 && m == 0;
     for (int gi = 0; gi < ngroups; ++gi) {
         const int j0 = max(0, nym - (gi + 1) * G);
         const int gr = nym - gi * G - j0;
@@ -221,39 +217,103 @@ __global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(C* __restrict__ spec,
             }
         }
         __syncthreads();
-        lds_fft_inplace<1, 2>(A, gr, n, fz, twl);
+        lds_fft_inplace<1, 2>(A, gr, n, zp.fz, twl);
         C* dst = plane + (long)j0 * n;
         for (int e = threadIdx.x; e < gr * n; e += kZsThreads) dst[e] = A[e];
     }
 }
 
 template <typename C>
-static hipError_t launch_zsolve_t(dfp_plan* p, int dir, C* spec, const C* tw, int zero00) {
+__global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(ZPar<C> zp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    zfwd_body<C>(zp, blockIdx.x, smem_raw);
+}
+
+template <typename C>
+__global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(ZPar<C> zp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    zbwd_body<C>(zp, blockIdx.x, smem_raw);
+}
+
+__device__ __forceinline__ int zfind_case(const int* __restrict__ off, int nc, int w) {
+    int lo = 0, hi = nc;
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) >> 1;
+        if (off[mid] <= w) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+template <typename C, int DIR>
+__global__ void __launch_bounds__(kZsThreads) kb_zsolve(const ZPar<C>* __restrict__ cs, const int* __restrict__ off,
+                                                        const int* __restrict__ pnc) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    const int nc = *pnc;
+    const int total = off[nc];
+    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+        const int c = zfind_case(off, nc, w);
+        if (DIR < 0) zfwd_body<C>(cs[c], w - off[c], smem_raw);
+        else zbwd_body<C>(cs[c], w - off[c], smem_raw);
+        __syncthreads();
+    }
+}
+
+template <typename C>
+ZPar<C> make_zpar(const dfp_plan* p, int f32_zero00) {
+    ZPar<C> z{};
     const Geo& g = p->geo;
-    const int G = p->g_zs;
-    const int n = p->fz.n;
-    const int ntw = twiddle_count(p->fz);
-    const size_t lds = sizeof(C) * ((size_t)G * n + ntw) + sizeof(double) * ((size_t)G * n + G);
+    z.spec = reinterpret_cast<C*>(p->d_spec);
+    z.lu = p->d_lu;
+    z.dl = p->ops.dl;
+    z.du = p->ops.du;
+    z.fz = p->fz;
+    z.tw = sizeof(C) == sizeof(double2) ? reinterpret_cast<const C*>(p->tw_z) : reinterpret_cast<const C*>(p->tw_z32);
+    z.nym = g.nym;
+    z.nzp = g.nzp;
+    z.R = p->zr;
+    z.ntw = twiddle_count(p->fz);
+    z.G = p->g_zs;
+    z.zero00 = f32_zero00;
+    z.nblocks = g.nxh * p->zr;
+    z.div_n = FastDiv(p->fz.n);
+    return z;
+}
+template ZPar<double2> make_zpar<double2>(const dfp_plan*, int);
+template ZPar<float2> make_zpar<float2>(const dfp_plan*, int);
+
+size_t z_lds_bytes(const dfp_plan* p, int f32) {
+    const int G = p->g_zs, n = p->fz.n, ntw = twiddle_count(p->fz);
+    return (f32 ? sizeof(float2) : sizeof(double2)) * ((size_t)G * n + ntw) + sizeof(double) * ((size_t)G * n + G);
+}
+
+template <typename C>
+static hipError_t launch_zsolve_t(dfp_plan* p, int dir, const ZPar<C>& zp) {
     const bool f32 = sizeof(C) == 8;
+    const size_t lds = z_lds_bytes(p, f32);
     const int slot = prof_begin(p, dir < 0 ? (f32 ? "zsolve_fwd32" : "zsolve_fwd") : (f32 ? "zsolve_bwd32" : "zsolve_bwd"));
-    const dim3 grid(g.nxh * p->zr);
-    if (dir < 0)
-        hipLaunchKernelGGL(k_zsolve_fwd<C>, grid, dim3(kZsThreads), lds, p->stream, spec, p->d_lu, p->ops.dl, g.nym, g.nzp,
-                           p->zr, p->fz, tw, ntw, G, FastDiv(n));
-    else
-        hipLaunchKernelGGL(k_zsolve_bwd<C>, grid, dim3(kZsThreads), lds, p->stream, spec, p->d_lu, p->ops.du, g.nym, g.nzp,
-                           p->zr, p->fz, tw, ntw, G, FastDiv(n), zero00);
+    const dim3 grid(zp.nblocks);
+    if (dir < 0) hipLaunchKernelGGL(k_zsolve_fwd<C>, grid, dim3(kZsThreads), lds, p->stream, zp);
+    else hipLaunchKernelGGL(k_zsolve_bwd<C>, grid, dim3(kZsThreads), lds, p->stream, zp);
     prof_end(p, slot);
     return hipGetLastError();
 }
 
-hipError_t launch_zsolve(dfp_plan* p, int dir) {
-    return launch_zsolve_t<double2>(p, dir, p->d_spec, p->tw_z, 0);
-}
+hipError_t launch_zsolve(dfp_plan* p, int dir) { return launch_zsolve_t<double2>(p, dir, make_zpar<double2>(p, 0)); }
 
-hipError_t launch_zsolve32(dfp_plan* p, int dir) {
-    return launch_zsolve_t<float2>(p, dir, reinterpret_cast<float2*>(p->d_spec), p->tw_z32, 1);
+hipError_t launch_zsolve32(dfp_plan* p, int dir) { return launch_zsolve_t<float2>(p, dir, make_zpar<float2>(p, 1)); }
+
+template <typename C>
+hipError_t launch_zsolve_batch(hipStream_t s, int dir, const ZPar<C>* cs, const int* off, const int* nc, int grid,
+                               size_t lds) {
+    if (dir < 0) hipLaunchKernelGGL((kb_zsolve<C, -1>), dim3(grid), dim3(kZsThreads), lds, s, cs, off, nc);
+    else hipLaunchKernelGGL((kb_zsolve<C, 1>), dim3(grid), dim3(kZsThreads), lds, s, cs, off, nc);
+    return hipGetLastError();
 }
+template hipError_t launch_zsolve_batch<double2>(hipStream_t, int, const ZPar<double2>*, const int*, const int*, int,
+                                                 size_t);
+template hipError_t launch_zsolve_batch<float2>(hipStream_t, int, const ZPar<float2>*, const int*, const int*, int,
+                                                size_t);
 
 // rows per group: the group fits the prefetch registers (kZsPf points per
 // thread), the in-place FFT capacity (2 butterflies per thread) and, with the

@@ -26,6 +26,10 @@ class DfpGrid(ctypes.Structure):
     ]
 
 
+class DfpTuning(ctypes.Structure):
+    _fields_ = [("z_split", ctypes.c_int32), ("x_columns", ctypes.c_int32), ("z_rows", ctypes.c_int32)]
+
+
 class DfpError(RuntimeError):
     """A nonzero return code of the C ABI (the analogue of the reference's gpu_abort)."""
 
@@ -52,6 +56,18 @@ def lib() -> ctypes.CDLL:
     sig = {
         "dfp_plan_create": ([ctypes.POINTER(DfpGrid), i32, i32, ctypes.POINTER(vp)], i32),
         "dfp_plan_destroy": ([vp], i32),
+        "dfp_plan_create_tuned": ([ctypes.POINTER(DfpGrid), i32, i32, ctypes.POINTER(DfpTuning), ctypes.POINTER(vp)],
+                                  i32),
+        "dfp_plan_tuning": ([vp, ctypes.POINTER(DfpTuning)], i32),
+        "dfp_batch_create": ([i32, i32, ctypes.POINTER(vp)], i32),
+        "dfp_batch_destroy": ([vp], i32),
+        "dfp_batch_set_stream": ([vp, vp], i32),
+        "dfp_batch_set_occupancy": ([vp, i32, i32, i32], i32),
+        "dfp_batch_use_graphs": ([vp, i32], i32),
+        "dfp_batch_project": ([vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                               ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(d), ctypes.POINTER(d),
+                               ctypes.POINTER(i32), i32], i32),
+        "dfp_batch_last_error": ([vp], ctypes.c_char_p),
         "dfp_project": ([vp, vp, vp, vp, vp, vp, d, d, i32], i32),
         "dfp_divergence": ([vp, vp, vp, vp, vp, i32], i32),
         "dfp_solve": ([vp, vp, vp, d, d, i32], i32),
@@ -64,6 +80,7 @@ def lib() -> ctypes.CDLL:
         "dfp_profile_reset": ([vp], i32),
         "dfp_field_sizes": ([vp, ctypes.POINTER(i64)], i32),
         "dfp_synth_fields": ([vp, vp, vp, vp, ctypes.c_uint64, d], i32),
+        "dfp_synth_fields_ex": ([vp, vp, vp, vp, ctypes.c_uint64, d, i32], i32),
         "dfp_strerror": ([i32], ctypes.c_char_p),
         "dfp_last_error": ([vp], ctypes.c_char_p),
     }

@@ -39,6 +39,10 @@ def hip():
         _hip.hipGetErrorString.restype = ctypes.c_char_p
         _hip.hipGetErrorString.argtypes = [ctypes.c_int]
         _hip.hipMemGetInfo.argtypes = [ctypes.POINTER(sz), ctypes.POINTER(sz)]
+        _hip.hipEventQuery.argtypes = [vp]
+        _hip.hipStreamCreateWithFlags.argtypes = [ctypes.POINTER(vp), ctypes.c_uint]
+        _hip.hipStreamDestroy.argtypes = [vp]
+        _hip.hipStreamSynchronize.argtypes = [vp]
     return _hip
 
 
@@ -125,6 +129,16 @@ class Event:
     def synchronize(self):
         _chk(hip().hipEventSynchronize(self.e), "hipEventSynchronize")
 
+    def done(self) -> bool:
+        """True once the work recorded before this event has completed."""
+        rc = hip().hipEventQuery(self.e)
+        if rc == 0:
+            return True
+        if rc == 600:  # hipErrorNotReady
+            return False
+        _chk(rc, "hipEventQuery")
+        return False
+
     def elapsed_ms(self, end: "Event") -> float:
         ms = ctypes.c_float()
         _chk(hip().hipEventElapsedTime(ctypes.byref(ms), self.e, end.e), "hipEventElapsedTime")
@@ -133,5 +147,28 @@ class Event:
     def __del__(self):
         try:
             hip().hipEventDestroy(self.e)
+        except Exception:
+            pass
+
+
+class Stream:
+    """A non-blocking HIP stream at the default priority."""
+
+    def __init__(self):
+        s = ctypes.c_void_p()
+        _chk(hip().hipStreamCreateWithFlags(ctypes.byref(s), 1), "hipStreamCreateWithFlags")
+        self.handle = s.value
+
+    def synchronize(self):
+        _chk(hip().hipStreamSynchronize(self.handle), "hipStreamSynchronize")
+
+    def close(self):
+        if self.handle:
+            hip().hipStreamDestroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
         except Exception:
             pass

@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import DFP_ASYNC, DFP_FIELDS_ON_DEVICE, DFP_FP64, DFP_SAVE_P, DfpGrid, check, lib
+from ._lib import DFP_ASYNC, DFP_FIELDS_ON_DEVICE, DFP_FP64, DFP_SAVE_P, DfpGrid, DfpTuning, check, lib
 from .grid import ChannelGrid
 from .hipmem import DeviceArray
 
@@ -68,16 +68,25 @@ def _ptr(a, shape, name):
 class Plan:
     """One ``dfp_plan``: operators, twiddles and device scratch of one grid."""
 
-    def __init__(self, g: ChannelGrid, device: int = 0, precision: int = DFP_FP64, sweeps: int | None = None):
+    def __init__(self, g: ChannelGrid, device: int = 0, precision: int = DFP_FP64, sweeps: int | None = None,
+                 tuning: tuple | None = None):
+        """tuning: (z_split, x_columns, z_rows) launch shapes, 0 = the library's choice."""
         self.g = g
         self.precision = precision
+        self.device = device
         self._arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in (g.x, g.y, g.z, g.yg, g.zg)]
         dp = ctypes.POINTER(ctypes.c_double)
         x, y, z, yg, zg = (a.ctypes.data_as(dp) for a in self._arrays)
         self._grid = DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, x, y, z, yg, zg, 0, 1, 0)
         h = ctypes.c_void_p()
-        check(lib().dfp_plan_create(ctypes.byref(self._grid), precision, device, ctypes.byref(h)), "dfp_plan_create")
+        if tuning is None:
+            check(lib().dfp_plan_create(ctypes.byref(self._grid), precision, device, ctypes.byref(h)), "dfp_plan_create")
+        else:
+            t = DfpTuning(*(int(v) for v in tuning))
+            check(lib().dfp_plan_create_tuned(ctypes.byref(self._grid), precision, device, ctypes.byref(t),
+                                              ctypes.byref(h)), "dfp_plan_create_tuned")
         self._h = h
+        self.sweeps = 2 if sweeps is None else int(sweeps)
         if sweeps is not None:
             self.set_refine_sweeps(sweeps)
 
@@ -111,6 +120,14 @@ class Plan:
     def set_refine_sweeps(self, sweeps: int):
         """Refinement sweeps of an FP32+refine plan (0..8)."""
         check(lib().dfp_set_refine_sweeps(self._h, int(sweeps)), "dfp_set_refine_sweeps", self._h)
+        self.sweeps = int(sweeps)
+
+    @property
+    def tuning(self) -> tuple:
+        """(z_split, x_columns, z_rows) the plan runs with."""
+        t = DfpTuning()
+        check(lib().dfp_plan_tuning(self._h, ctypes.byref(t)), "dfp_plan_tuning", self._h)
+        return (t.z_split, t.x_columns, t.z_rows)
 
     def set_stream(self, stream_handle: int | None):
         check(lib().dfp_set_stream(self._h, stream_handle), "dfp_set_stream", self._h)
@@ -161,16 +178,16 @@ class Plan:
         check(lib().dfp_max_divergence(self._h, pu, pv, pw, ctypes.byref(out), f), "dfp_max_divergence", self._h)
         return out.value
 
-    def synth(self, U, V, W, seed: int, sigma: float):
-        """Device-side synthetic fields (torch CUDA tensors in Fortran layout)."""
+    def synth(self, U, V, W, seed: int, sigma: float, asynchronous: bool = False):
+        """Device-side synthetic fields (device arrays in Fortran layout), on the plan's stream."""
         g = self.g
         pu, du = _ptr(U, g.shape_u, "U")
         pv, dv = _ptr(V, g.shape_v, "V")
         pw, dw = _ptr(W, g.shape_w, "W")
         if not (du and dv and dw):
             raise ValueError("synth needs device arrays")
-        check(lib().dfp_synth_fields(self._h, pu, pv, pw, ctypes.c_uint64(seed & (2 ** 64 - 1)), float(sigma)),
-              "dfp_synth_fields", self._h)
+        check(lib().dfp_synth_fields_ex(self._h, pu, pv, pw, ctypes.c_uint64(seed & (2 ** 64 - 1)), float(sigma),
+                                        DFP_ASYNC if asynchronous else 0), "dfp_synth_fields_ex", self._h)
 
     # --- stage timing ---------------------------------------------------------------
     def profile(self, enable: bool):

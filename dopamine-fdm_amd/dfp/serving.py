@@ -1,0 +1,307 @@
+"""Serving loop: continuous batching of projection requests on one GPU.
+
+No counterpart in the reference (a solver that projects one grid per run).
+A request names a case id (which fixes the grid and stretching), a seed for
+its input velocity u* and a precision (FP64, or FP32+refine at the library's
+sweep count).  The server
+
+  * keeps each case's fields (U, V, W, rhs_p, P) and plans resident in HBM,
+    so a repeated case id reuses them (the input u* is generated on the
+    device from the request's seed, no host transfer);
+  * admits a request only when its case fits an HBM budget, evicting the
+    least recently used idle cases first;
+  * batches the admitted requests into one ``dfp_batch_project`` call (one
+    persistent launch per stage, hipGraph replay) on one of a few streams,
+    so requests that arrive while others run are launched as soon as a
+    stream is free and overlap on the device;
+  * records arrival, launch and completion times per request.
+
+A case is in at most one in-flight batch (its plans own their scratch), so
+each request's result is the same alone or in the mix: the batch executor
+reproduces single-plan projections bit for bit.
+"""
+from __future__ import annotations
+
+import collections
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from ._lib import DFP_FP32_REFINE, DFP_FP64
+from .autotune import tuned_plan
+from .batch import Batch, Case
+from .grid import channel_grid
+from .hipmem import DeviceArray, Event, Stream, mem_info
+from .workloads import serving_size
+
+DT = 1e-3
+C3 = 0.75
+
+
+@dataclass
+class Request:
+    rid: int
+    case_id: int
+    shape: tuple            # (nxp, nym, nzp)
+    alpha: float
+    seed: int
+    precision: int = DFP_FP64
+    sigma: float = 0.1
+    save_p: bool = True
+    arrival: float = 0.0    # host seconds (time.perf_counter)
+    launched: float | None = None
+    done: float | None = None
+
+    @property
+    def latency(self):
+        return None if self.done is None else self.done - self.arrival
+
+
+class _Resident:
+    """Device state of one case id."""
+
+    def __init__(self, case_id, g, table, sweeps):
+        self.case_id = case_id
+        self.g = g
+        self.table = table
+        self.sweeps = sweeps
+        self.plans = {}
+        self.U = DeviceArray(g.shape_u)
+        self.V = DeviceArray(g.shape_v)
+        self.W = DeviceArray(g.shape_w)
+        self.rhs = DeviceArray(g.shape_rhs, zero=True)
+        self.P = DeviceArray(g.shape_p, zero=True)
+        self.busy = False
+        self.last_used = 0.0
+
+    def plan(self, precision):
+        p = self.plans.get(precision)
+        if p is None:
+            p = tuned_plan(self.g, precision, self.table,
+                           sweeps=self.sweeps if precision == DFP_FP32_REFINE else None)
+            self.plans[precision] = p
+        return p
+
+    def free(self):
+        for p in self.plans.values():
+            p.close()
+        self.plans.clear()
+        for a in (self.U, self.V, self.W, self.rhs, self.P):
+            a.free()
+
+
+def case_bytes(g, precisions=(DFP_FP64,)) -> int:
+    """HBM a resident case holds: its five fields and, per precision, the
+    plan's spectrum, pivot table and (FP32+refine) FP64 right-hand side."""
+    f = 8 * (np.prod(g.shape_u) + np.prod(g.shape_v) + np.prod(g.shape_w) + np.prod(g.shape_rhs) +
+             np.prod(g.shape_p))
+    H = (g.nxp // 2 + 1) * g.nym * g.nzp
+    lu = 8 * (g.nxp // 2 + 1) * g.nym * (g.nzp // 2 + 1)
+    extra = 0
+    for prec in set(precisions) | {DFP_FP64}:
+        extra += 16 * H + lu
+        if prec == DFP_FP32_REFINE:
+            extra += 8 * np.prod(g.shape_rhs)
+    return int(f + extra)
+
+
+class Server:
+    def __init__(self, device: int = 0, hbm_budget: float = 200e9, max_batch: int = 32, n_streams: int = 2,
+                 graphs: bool = True, table: dict | None = None, sweeps: int = 2, occupancy=None):
+        self.device = device
+        self.budget = float(hbm_budget)
+        self.max_batch = max_batch
+        self.table = table
+        self.sweeps = sweeps
+        self.resident: "collections.OrderedDict[int, _Resident]" = collections.OrderedDict()
+        self.resident_bytes = 0
+        self.pending: "collections.deque[Request]" = collections.deque()
+        self.finished: list[Request] = []
+        self.streams = [Stream() for _ in range(n_streams)]
+        self.batches = [Batch(device, max_cases=max_batch, graphs=graphs) for _ in range(n_streams)]
+        for b, s in zip(self.batches, self.streams):
+            b.set_stream(s.handle)
+            if occupancy:
+                b.set_occupancy(*occupancy)
+        self.inflight = [None] * n_streams  # (event, [requests]) per stream
+        self.launches = 0
+        self.evictions = 0
+
+    # --- admission -------------------------------------------------------------
+    def _evict_for(self, need: int) -> bool:
+        for cid in list(self.resident.keys()):  # least recently used first
+            if self.resident_bytes + need <= self.budget:
+                break
+            r = self.resident[cid]
+            if r.busy:
+                continue
+            self.resident_bytes -= case_bytes(r.g, tuple(r.plans.keys()) or (DFP_FP64,))
+            r.free()
+            del self.resident[cid]
+            self.evictions += 1
+        return self.resident_bytes + need <= self.budget
+
+    def _admit(self, req: Request):
+        """The resident state of the request's case, or None if it does not fit now."""
+        r = self.resident.get(req.case_id)
+        if r is not None:
+            if r.busy:
+                return None
+            if req.precision not in r.plans:
+                g = r.g
+                old = case_bytes(g, tuple(r.plans.keys()) or (DFP_FP64,))
+                new = case_bytes(g, tuple(r.plans.keys()) + (req.precision,))
+                if new > old and not self._evict_for(new - old):
+                    return None
+                self.resident_bytes += new - old
+            self.resident.move_to_end(req.case_id)
+            return r
+        g = channel_grid(*req.shape, req.alpha)
+        need = case_bytes(g, (req.precision,))
+        if need > self.budget:
+            raise ValueError(f"request {req.rid}: case needs {need} bytes, over the whole budget")
+        if not self._evict_for(need):
+            return None
+        r = _Resident(req.case_id, g, self.table, self.sweeps)
+        self.resident[req.case_id] = r
+        self.resident_bytes += need
+        return r
+
+    # --- loop ------------------------------------------------------------------
+    def submit(self, req: Request):
+        if not req.arrival:
+            req.arrival = time.perf_counter()
+        self.pending.append(req)
+
+    def _retire(self):
+        now = None
+        for k, fl in enumerate(self.inflight):
+            if fl is None:
+                continue
+            ev, reqs, res = fl
+            if ev.done():
+                now = now or time.perf_counter()
+                for q, r in zip(reqs, res):
+                    q.done = now
+                    r.busy = False
+                    r.last_used = now
+                    self.finished.append(q)
+                self.inflight[k] = None
+
+    def _launch(self, k: int):
+        chosen, res, seen, skipped = [], [], set(), collections.deque()
+        while self.pending and len(chosen) < self.max_batch:
+            q = self.pending.popleft()
+            if q.case_id in seen:
+                skipped.append(q)
+                continue
+            r = self._admit(q)
+            if r is None:
+                skipped.append(q)
+                continue
+            seen.add(q.case_id)
+            chosen.append(q)
+            res.append(r)
+        self.pending.extendleft(reversed(skipped))
+        if not chosen:
+            return False
+        stream = self.streams[k]
+        cases = []
+        now = time.perf_counter()
+        for q, r in zip(chosen, res):
+            r.busy = True
+            plan = r.plan(q.precision)
+            plan.set_stream(stream.handle)
+            plan.synth(r.U, r.V, r.W, q.seed, q.sigma, asynchronous=True)
+            cases.append(Case(plan, r.U, r.V, r.W, r.rhs, r.P, DT, C3, q.save_p))
+            q.launched = now
+        self.batches[k].project(cases, asynchronous=True)
+        ev = Event()
+        ev.record(stream.handle)
+        self.inflight[k] = (ev, chosen, res)
+        self.launches += 1
+        return True
+
+    def poll(self):
+        """Retire finished batches, then launch pending work on free streams."""
+        self._retire()
+        for k in range(len(self.streams)):
+            if self.inflight[k] is None and self.pending:
+                self._launch(k)
+
+    def busy(self) -> bool:
+        return bool(self.pending) or any(f is not None for f in self.inflight)
+
+    def drain(self):
+        while self.busy():
+            self.poll()
+
+    def result(self, case_id: int):
+        """Host copies (U, V, W, rhs_p, P) of a resident case (after drain)."""
+        r = self.resident[case_id]
+        return [a.download() for a in (r.U, r.V, r.W, r.rhs, r.P)]
+
+    def close(self):
+        self.drain()
+        for r in self.resident.values():
+            r.free()
+        self.resident.clear()
+        for b in self.batches:
+            b.close()
+        for s in self.streams:
+            s.close()
+
+
+# ---------------------------------------------------------------------------
+# request streams
+# ---------------------------------------------------------------------------
+def make_case_pool(n_ids: int, seed: int = 11):
+    """Case id -> (shape, alpha): sizes log-uniform between 32^3 and
+    256x192x256 (each dimension snapped to 2^a 3^b), stretching in [1, 2.5]."""
+    rng = np.random.default_rng(seed)
+    return {cid: (serving_size(float(rng.uniform())), float(rng.uniform(1.0, 2.5))) for cid in range(n_ids)}
+
+
+def poisson_requests(n: int, rate: float, pool: dict, fp32_fraction: float = 0.5, seed: int = 3):
+    """n requests with exponential inter-arrival gaps (mean 1/rate s), case ids
+    uniform over the pool, precision FP32+refine with the given probability."""
+    rng = np.random.default_rng(seed)
+    t = 0.0
+    out = []
+    ids = sorted(pool)
+    for i in range(n):
+        t += float(rng.exponential(1.0 / rate))
+        cid = int(rng.choice(ids))
+        shape, alpha = pool[cid]
+        prec = DFP_FP32_REFINE if rng.uniform() < fp32_fraction else DFP_FP64
+        out.append((t, Request(i, cid, shape, alpha, seed=10_000 + i, precision=prec)))
+    return out
+
+
+def run_trace(server: Server, trace, realtime: bool = True):
+    """Feed (offset_s, Request) pairs to the server at their arrival times
+    (all at once when realtime is False) and run until every request is done.
+    Returns (requests, wall seconds)."""
+    t0 = time.perf_counter()
+    i = 0
+    n = len(trace)
+    while i < n or server.busy():
+        now = time.perf_counter()
+        while i < n and (not realtime or t0 + trace[i][0] <= now):
+            off, q = trace[i]
+            q.arrival = t0 + off if realtime else t0
+            server.submit(q)
+            i += 1
+        server.poll()
+    wall = time.perf_counter() - t0
+    return [q for _, q in trace], wall
+
+
+def latency_stats(reqs) -> dict:
+    lat = np.array([q.latency for q in reqs if q.latency is not None])
+    if lat.size == 0:
+        return {}
+    return {"n": int(lat.size), "p50_ms": round(1e3 * float(np.percentile(lat, 50)), 3),
+            "p99_ms": round(1e3 * float(np.percentile(lat, 99)), 3),
+            "mean_ms": round(1e3 * float(lat.mean()), 3), "max_ms": round(1e3 * float(lat.max()), 3)}

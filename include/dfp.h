@@ -76,6 +76,23 @@ enum {
 int dfp_plan_create(const dfp_grid *g, int precision, int device, dfp_plan **out);
 int dfp_plan_destroy(dfp_plan *plan);
 
+/* Launch shapes of a plan (no counterpart in the reference, whose FFT and
+ * tridiagonal libraries choose their own).  0 = the library's choice.
+ *   z_split    radix R in {1,2,3,4} of the first z-FFT stage, applied by the
+ *              x kernels (R must divide nzp; the z kernels then transform
+ *              rows of nzp/R points, one mode per thread, nzp/R <= 256)
+ *   x_columns  z columns per x-kernel workgroup (divides nzp/R)
+ *   z_rows     y rows per group of the fused z-FFT / y-solve kernels
+ * Results do not depend on x_columns or z_rows; z_split changes the FFT's
+ * factorisation (round-off level).  DFP_E_ARG for values out of range,
+ * DFP_E_UNSUPPORTED for shapes that do not fit the device's LDS. */
+typedef struct dfp_tuning {
+    int32_t z_split, x_columns, z_rows;
+} dfp_tuning;
+int dfp_plan_create_tuned(const dfp_grid *g, int precision, int device, const dfp_tuning *tune,
+                          dfp_plan **out);
+int dfp_plan_tuning(const dfp_plan *plan, dfp_tuning *out);
+
 /* Whole projection: divergence -> Poisson solve -> velocity correction.
  * Replaces compute_projection_step (projection.f90:21-34).  On return U, V,
  * W hold the corrected velocity (projection.f90:551-604), rhs_p holds the
@@ -128,6 +145,28 @@ int dfp_field_sizes(const dfp_plan *plan, int64_t sizes[5]);
  * and no-slip y ghosts as boundary_conditions.f90 leaves them.  Normal
  * deviates: SplitMix64 counter hash -> Box-Muller.  Device pointers only. */
 int dfp_synth_fields(dfp_plan *plan, double *U, double *V, double *W, uint64_t seed, double sigma);
+/* the same on the plan's stream; flags may hold DFP_ASYNC */
+int dfp_synth_fields_ex(dfp_plan *plan, double *U, double *V, double *W, uint64_t seed, double sigma, int flags);
+
+/* Batched projections (no counterpart in the reference, which projects one
+ * grid per run): a batch executor runs the projections of many plans - any
+ * grid sizes, either precision - with one persistent launch per stage on one
+ * stream.  Fields are device pointers with each plan's extents; a plan may
+ * appear once per call (it owns its spectral scratch).  flags[i] may hold
+ * DFP_SAVE_P (and DFP_FIELDS_ON_DEVICE, implied); batch_flags DFP_ASYNC.
+ * Results equal those of dfp_project on each plan, bit for bit. */
+typedef struct dfp_batch dfp_batch;
+int dfp_batch_create(int device, int max_cases, dfp_batch **out);
+int dfp_batch_destroy(dfp_batch *batch);
+int dfp_batch_set_stream(dfp_batch *batch, void *hip_stream);
+/* workgroups per CU of the persistent x, z and stencil launches (autotuned) */
+int dfp_batch_set_occupancy(dfp_batch *batch, int x_per_cu, int z_per_cu, int stencil_per_cu);
+/* replay the launch sequence from hipGraphs keyed by its stage list */
+int dfp_batch_use_graphs(dfp_batch *batch, int enable);
+int dfp_batch_project(dfp_batch *batch, int n, dfp_plan *const *plans, double *const *U, double *const *V,
+                      double *const *W, double *const *rhs_p, double *const *P, const double *dt,
+                      const double *c, const int *flags, int batch_flags);
+const char *dfp_batch_last_error(const dfp_batch *batch);
 
 const char *dfp_strerror(int code);
 const char *dfp_last_error(const dfp_plan *plan);

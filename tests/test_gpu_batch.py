@@ -1,0 +1,135 @@
+"""GPU: the batch executor (persistent launches over many plans) against
+per-plan projections on identical inputs, and the autotuned launch shapes.
+
+Bars (written here, as the tests apply them):
+  * a batch of uneven cases (FP64, FP32+refine, or both) gives, case by case,
+    the same phi, P, U, V, W as dfp_project on each plan: bit-identical;
+  * replays of the hipGraph-captured batch, with other work between them,
+    equal the eager batch bit for bit;
+  * a tuned plan (any z split, column and row group) matches the C oracle to
+    the FP64 parity bar 1e-12; column and row groups alone change nothing.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import dfp
+from dfp.batch import Batch, Case
+from dfp.workloads import batched_cases
+from _cases import fields, make_grid, oracle_project, rel_linf
+
+pytestmark = pytest.mark.gpu
+
+DT = 1e-3
+C3 = 0.75
+
+
+def _resident(g, seed, sigma=0.1):
+    U, V, W, P, rhs = fields(g, seed, sigma)
+    return [dfp.DeviceArray.from_host(a) for a in (U, V, W, rhs, P)], (U, V, W, rhs, P)
+
+
+def _single(plan, host, save_p):
+    d = [dfp.DeviceArray.from_host(a) for a in host]
+    plan.project(d[0], d[1], d[2], d[3], d[4], DT, C3, save_p=save_p)
+    return [a.download() for a in d]
+
+
+def _run_batch(specs, precisions, graphs=False, repeat=1, save_every=2):
+    plans, cases, hosts = [], [], []
+    for i, (sp, prec) in enumerate(zip(specs, precisions)):
+        g = make_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
+        plan = dfp.Plan(g, precision=prec)
+        dev, host = _resident(g, sp.seed)
+        save_p = i % save_every == 0
+        plans.append(plan)
+        hosts.append((host, save_p))
+        cases.append(Case(plan, dev[0], dev[1], dev[2], dev[3], dev[4], DT, C3, save_p))
+    b = Batch(max_cases=len(cases), graphs=graphs)
+    outs = []
+    for r in range(repeat):
+        if r:
+            for cs, (host, _) in zip(cases, hosts):
+                for a, h in zip((cs.U, cs.V, cs.W, cs.rhs, cs.P), host):
+                    a.upload(h)
+        b.project(cases)
+        outs.append([[a.download() for a in (cs.U, cs.V, cs.W, cs.rhs, cs.P)] for cs in cases])
+    b.close()
+    return plans, hosts, outs
+
+
+@pytest.mark.parametrize("mix", ["fp64", "fp32", "mixed"])
+def test_batch_equals_single_plans(mix):
+    specs = batched_cases(12, seed=5)
+    if mix == "fp64":
+        precs = [dfp.DFP_FP64] * len(specs)
+    elif mix == "fp32":
+        precs = [dfp.DFP_FP32_REFINE] * len(specs)
+    else:
+        precs = [dfp.DFP_FP64 if i % 3 else dfp.DFP_FP32_REFINE for i in range(len(specs))]
+    print(f"batch {mix}: {[(s.nxp, s.nym, s.nzp) for s in specs]}", flush=True)
+    plans, hosts, outs = _run_batch(specs, precs)
+    for i, (plan, (host, save_p)) in enumerate(zip(plans, hosts)):
+        ref = _single(plan, host, save_p)
+        for lab, a, r in zip(("U", "V", "W", "phi", "P"), outs[0][i], ref):
+            assert np.array_equal(a, r), f"case {i} {lab}: max diff {np.max(np.abs(a - r)):.3e}"
+
+
+def test_batch_48_cases_fp64_vs_oracle():
+    specs = batched_cases(48)
+    print("batch of 48 uneven cases, FP64", flush=True)
+    plans, hosts, outs = _run_batch(specs, [dfp.DFP_FP64] * 48, save_every=1)
+    for i in (0, 7, 19, 33, 47):
+        g = plans[i].g
+        host = [a.copy(order="F") for a in hosts[i][0]]
+        oracle_project(g, host[0], host[1], host[2], host[4], host[3], DT, C3, save_p=True)
+        for lab, a, r in zip(("U", "V", "W", "phi", "P"), outs[0][i], host):
+            assert rel_linf(a, r) <= 1e-12, f"case {i} {lab}"
+
+
+def test_batch_graph_replays_equal_eager():
+    specs = batched_cases(8, seed=9)
+    precs = [dfp.DFP_FP64 if i % 2 else dfp.DFP_FP32_REFINE for i in range(8)]
+    _, _, eager = _run_batch(specs, precs, graphs=False)
+    _, _, replays = _run_batch(specs, precs, graphs=True, repeat=3)
+    for r in range(3):
+        for i in range(8):
+            for a, e in zip(replays[r][i], eager[0][i]):
+                assert np.array_equal(a, e), f"replay {r} case {i}"
+
+
+def test_batch_rejects_shared_plans_and_host_fields():
+    g = make_grid(16, 8, 16, 1.5)
+    plan = dfp.Plan(g)
+    dev, host = _resident(g, 1)
+    b = Batch(max_cases=4)
+    with pytest.raises(dfp.DfpError):
+        b.project([Case(plan, *dev[:4]), Case(plan, *dev[:4])])
+    with pytest.raises(ValueError):
+        b.project([Case(plan, host[0], host[1], host[2], host[3])])
+
+
+@pytest.mark.parametrize("tuning", [(4, 1, 0), (2, 4, 3), (1, 2, 1), (4, 8, 2)])
+def test_tuned_plans_match_oracle(tuning):
+    g = make_grid(64, 40, 64, 2.0)
+    U, V, W, P, rhs = fields(g, 3, 0.1)
+    o = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    oracle_project(g, *o, DT, C3, save_p=True)
+    plan = dfp.Plan(g, tuning=tuning)
+    assert plan.tuning == tuning if tuning[2] else plan.tuning[:2] == tuning[:2]
+    d = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    plan.project(d[0], d[1], d[2], d[4], d[3], DT, C3, save_p=True)
+    for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+        assert rel_linf(a, b) <= 1e-12, lab
+
+
+def test_columns_and_row_groups_do_not_change_results():
+    g = make_grid(64, 40, 64, 2.0)
+    host = fields(g, 4, 0.1)
+    host = (host[0], host[1], host[2], host[4], host[3])
+    ref = _single(dfp.Plan(g, tuning=(4, 1, 1)), host, True)
+    for t in ((4, 2, 5), (4, 16, 0), (4, 4, 2)):
+        out = _single(dfp.Plan(g, tuning=t), host, True)
+        for a, r in zip(out, ref):
+            assert np.array_equal(a, r), t

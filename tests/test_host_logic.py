@@ -1,0 +1,66 @@
+"""Host-side logic of the batched and serving forms (no GPU): workload
+generators, size snapping, autotuner candidates and its table."""
+from __future__ import annotations
+
+import math
+
+import pytest
+
+import dfp
+from dfp import autotune, workloads
+
+
+def test_batched_cases_recipe():
+    cs = workloads.batched_cases(48)
+    assert len(cs) == 48
+    assert cs == workloads.batched_cases(48)  # seeded
+    for c in cs:
+        assert c.nxp in workloads.BATCH_XZ and c.nzp in workloads.BATCH_XZ and c.nym in workloads.BATCH_Y
+        assert 1.0 <= c.alpha <= 2.5
+    assert len({c.seed for c in cs}) == 48
+    assert len({c.key for c in cs}) > 10  # uneven sizes
+
+
+def test_smooth_sizes_and_snap():
+    v = workloads.smooth_sizes(32, 256)
+    assert v[0] == 32 and v[-1] == 256 and 48 in v and 96 in v and 192 in v and 144 in v
+    for x in v:
+        while x % 2 == 0:
+            x //= 2
+        while x % 3 == 0:
+            x //= 3
+        assert x == 1
+    assert workloads.snap(50, v) == 48
+    assert workloads.snap(250, v) == 256
+
+
+def test_serving_size_endpoints():
+    assert workloads.serving_size(0.0) == (32, 32, 32)
+    assert workloads.serving_size(1.0) == (256, 192, 256)
+    for s in (0.1, 0.37, 0.5, 0.81):
+        nx, ny, nz = workloads.serving_size(s)
+        assert nx % 2 == 0 and 32 <= nx <= 256 and 32 <= ny <= 192 and 32 <= nz <= 256
+        d = 32 * (256 / 32) ** s
+        assert abs(math.log(nx / d)) < 0.45
+
+
+def test_autotune_candidates_are_valid_shapes():
+    g = dfp.channel_grid(64, 32, 96, 1.5)
+    cs = autotune.candidates(g)
+    assert cs
+    for R, T, G in cs:
+        assert g.nzp % R == 0 and g.nzp // R <= 256
+        assert (g.nzp // R) % T == 0
+        assert G in (0, -1)
+    assert {R for R, _, _ in cs} == {1, 2, 3, 4}
+
+
+def test_autotune_table_roundtrip(tmp_path):
+    p = tmp_path / "t.json"
+    t = {"plans": {autotune.size_key(64, 32, 64, dfp.DFP_FP64): {"tuning": [4, 2, 8], "us": 10.0}}, "batch": {}}
+    autotune.save_table(t, str(p))
+    t2 = autotune.load_table(str(p))
+    g = dfp.channel_grid(64, 32, 64, 1.0)
+    assert autotune.lookup(t2, g, dfp.DFP_FP64) == (4, 2, 8)
+    assert autotune.lookup(t2, g, dfp.DFP_FP32_REFINE) is None
+    assert autotune.load_table(str(tmp_path / "missing.json")) == {"plans": {}, "batch": {}}
