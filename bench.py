@@ -40,8 +40,20 @@ def algorithmic_bytes(N: int, H: int) -> dict:
         "zsolve_fwd": 32 * H,            # read the spectrum; write the eliminated rows
         "zsolve_bwd": 32 * H,            # read them; write the solved spectrum
         "irfft_x": 16 * H + 8 * N,       # read the spectrum; write phi
-        "correct": 48 * N,               # read u*,v*,w*; write u,v,w
+        "correct": 56 * N,               # read phi, u*,v*,w*; write u,v,w
     }
+
+
+def measured_traffic(kernel: str):
+    """HBM bytes per launch of a kernel from the committed rocprofv3 summary of
+    this workload (2 x FETCH_SIZE + 1 x WRITE_SIZE, copy-calibrated), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic_fp64_256x192x256.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"].get("k_" + kernel)
+        return int(k["traffic_bytes"]) if k else None
+    except (OSError, ValueError, KeyError):
+        return None
 
 
 def cpu_baseline(g, budget_s: float = 20.0) -> dict:
@@ -348,7 +360,8 @@ def main() -> int:
                                "P saved at the third",
                    "grid": [NXP, NYM, NZP], "projections_per_step": 3, "fields_resident": True},
         "roofline": {"bound": "hbm", "kernel": dominant, "achieved": round(dom_gbs, 1), "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4), "traffic": None,
+                     "unit": "GB/s", "frac": round(dom_gbs / HBM_PEAK_GBS, 4),
+                     "traffic": measured_traffic(dominant),
                      "algorithmic_bytes_per_launch": int(ab[dominant]),
                      "avg_launch_us": round(kern[dominant]["avg_us"], 2)},
         "projection": {"us_per_projection": round(proj_us, 2), "B_model_bytes": int(b_model),

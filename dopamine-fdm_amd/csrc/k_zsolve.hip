@@ -55,7 +55,8 @@ hipError_t launch_lu_table(dfp_plan* p) {
 }
 
 constexpr int kZsThreads = 256;
-constexpr int kZsPf = 8;  // group points per thread held in flight
+constexpr int kZsPf = 8;     // group points per thread held in flight
+constexpr int kZsAhead = 2;  // rows read ahead in the elimination sweeps
 
 // Shared layout of one group (G rows of length n):
 //   A[r*n + m]     the rows
@@ -136,18 +137,41 @@ __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned cha
         if (j0 + G < nym) ZS_PREFETCH(j0 + G, min(G, nym - j0 - G), dl, -1);
         lds_fft_inplace<-1, 2>(A, gr, n, zp.fz, twl);
         if (m < n) {
-            for (int r = 0; r < gr; ++r) {
-                const C a = A[r * n + m];
-                double bx = (double)a.x, by = (double)a.y;
-                if (j0 + r > 0) {
-                    const double mult = dus[r] * ipp;  // DL(j-1) / D'(j-1)
-                    bx = bx - mult * bpx;
-                    by = by - mult * bpy;
+            // rows, pivots and coefficients are read kZsAhead rows ahead of
+            // the store of the current row, so LDS latency leaves the chain
+            C ra[kZsAhead];
+            double rl[kZsAhead], rd[kZsAhead];
+#pragma unroll
+            for (int q = 0; q < kZsAhead; ++q) {
+                const int rr = min(q, gr - 1);
+                ra[q] = A[rr * n + m];
+                rl[q] = lus[rr * n + m];
+                rd[q] = dus[rr];
+            }
+            for (int r0 = 0; r0 < gr; r0 += kZsAhead) {
+#pragma unroll
+                for (int q = 0; q < kZsAhead; ++q) {
+                    const int r = r0 + q;
+                    if (r < gr) {
+                        const C a = ra[q];
+                        const double lr = rl[q], dr = rd[q];
+                        // unconditional (clamped) so no select waits on the load
+                        const int rn = min(r + kZsAhead, gr - 1);
+                        ra[q] = A[rn * n + m];
+                        rl[q] = lus[rn * n + m];
+                        rd[q] = dus[rn];
+                        double bx = (double)a.x, by = (double)a.y;
+                        if (j0 + r > 0) {
+                            const double mult = dr * ipp;  // DL(j-1) / D'(j-1)
+                            bx = bx - mult * bpx;
+                            by = by - mult * bpy;
+                        }
+                        A[r * n + m] = CplxT<C>::make((Rl)bx, (Rl)by);
+                        bpx = bx;
+                        bpy = by;
+                        ipp = lr;
+                    }
                 }
-                A[r * n + m] = CplxT<C>::make((Rl)bx, (Rl)by);
-                bpx = bx;
-                bpy = by;
-                ipp = lus[r * n + m];
             }
         }
         __syncthreads();
@@ -197,23 +221,45 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
             ZS_PREFETCH(j0q, j0 - j0q, du, 0);
         }
         if (m < n) {
-            for (int r = gr - 1; r >= 0; --r) {
-                const int j = j0 + r;
-                const C bb = A[r * n + m];
-                const double ip = lus[r * n + m];
-                double xx, xy;
-                if (j == nym - 1) {
-                    xx = (double)bb.x * ip;
-                    xy = (double)bb.y * ip;
-                } else {
-                    // x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
-                    xx = ((double)bb.x - dus[r] * xnx) * ip;
-                    xy = ((double)bb.y - dus[r] * xny) * ip;
+            // rows from the top of the group, read kZsAhead rows ahead
+            C ra[kZsAhead];
+            double rl[kZsAhead], rd[kZsAhead];
+#pragma unroll
+            for (int q = 0; q < kZsAhead; ++q) {
+                const int rr = max(gr - 1 - q, 0);
+                ra[q] = A[rr * n + m];
+                rl[q] = lus[rr * n + m];
+                rd[q] = dus[rr];
+            }
+            for (int t0 = 0; t0 < gr; t0 += kZsAhead) {
+#pragma unroll
+                for (int q = 0; q < kZsAhead; ++q) {
+                    const int t = t0 + q;
+                    if (t < gr) {
+                        const int r = gr - 1 - t;
+                        const int j = j0 + r;
+                        const C bb = ra[q];
+                        const double ip = rl[q], dr = rd[q];
+                        // unconditional (clamped) so no select waits on the load
+                        const int rn = max(gr - 1 - (t + kZsAhead), 0);
+                        ra[q] = A[rn * n + m];
+                        rl[q] = lus[rn * n + m];
+                        rd[q] = dus[rn];
+                        double xx, xy;
+                        if (j == nym - 1) {
+                            xx = (double)bb.x * ip;
+                            xy = (double)bb.y * ip;
+                        } else {
+                            // x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
+                            xx = ((double)bb.x - dr * xnx) * ip;
+                            xy = ((double)bb.y - dr * xny) * ip;
+                        }
+                        if (is00) xx = xy = 0.0;
+                        A[r * n + m] = CplxT<C>::make((Rl)xx, (Rl)xy);
+                        xnx = xx;
+                        xny = xy;
+                    }
                 }
-                if (is00) xx = xy = 0.0;
-                A[r * n + m] = CplxT<C>::make((Rl)xx, (Rl)xy);
-                xnx = xx;
-                xny = xy;
             }
         }
         __syncthreads();

@@ -200,6 +200,7 @@ class Server:
             if r is None:
                 skipped.append(q)
                 continue
+            r.busy = True  # chosen: later admissions in this batch must not evict it
             seen.add(q.case_id)
             chosen.append(q)
             res.append(r)
@@ -210,7 +211,6 @@ class Server:
         cases = []
         now = time.perf_counter()
         for q, r in zip(chosen, res):
-            r.busy = True
             plan = r.plan(q.precision)
             plan.set_stream(stream.handle)
             plan.synth(r.U, r.V, r.W, q.seed, q.sigma, asynchronous=True)
