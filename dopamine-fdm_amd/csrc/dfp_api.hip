@@ -189,7 +189,8 @@ int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, 
     HIPCHK(p, launch_mode00(p));
     HIPCHK(p, launch_zsolve32(p, -1));
     HIPCHK(p, launch_zsolve32(p, +1));
-    HIPCHK(p, launch_irfft_x32(p, rhs, 0, p->d_pbar, p->d_part0));
+    // P = phi/(dt*c) is written by the last inverse transform
+    HIPCHK(p, launch_irfft_x32(p, rhs, 0, p->d_pbar, p->d_part0, P, dtc, save_p && p->sweeps == 0));
     for (int s = 0; s < p->sweeps; ++s) {
         XArgs r{};
         r.rhs = rhs;
@@ -202,9 +203,8 @@ int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, 
         HIPCHK(p, launch_mode00(p));
         HIPCHK(p, launch_zsolve32(p, -1));
         HIPCHK(p, launch_zsolve32(p, +1));
-        HIPCHK(p, launch_irfft_x32(p, rhs, 1, p->d_pbar, p->d_part0));
+        HIPCHK(p, launch_irfft_x32(p, rhs, 1, p->d_pbar, p->d_part0, P, dtc, save_p && s == p->sweeps - 1));
     }
-    if (save_p) HIPCHK(p, launch_save_p(p, rhs, P, dtc));
     return DFP_OK;
 }
 

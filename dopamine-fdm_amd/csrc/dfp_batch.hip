@@ -135,10 +135,12 @@ void destroy_graphs(dfp_batch* b) {
     b->graphs.clear();
 }
 
-// LDS of a batched launch rounded up to 4 KiB (fewer distinct graphs), unless
-// that would leave no room for the kernels' static LDS under the device limit
+// LDS of a batched launch rounded up to a power of two of at least 4 KiB
+// (few distinct graphs), unless that would leave no room for the kernels'
+// static LDS under the device limit
 size_t lds_bucket(size_t v, size_t limit) {
-    const size_t a = align_up(v, 4096);
+    size_t a = 4096;
+    while (a < v) a *= 2;
     return a + 512 <= limit ? a : v;
 }
 

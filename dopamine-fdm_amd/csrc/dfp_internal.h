@@ -185,7 +185,7 @@ struct dfp_plan {
     double* d_red = nullptr;     // reduction partials
     int red_len = 0;
     // FP32 form: the FP64 right-hand side, partial sums and the (0,0) mode
-    int sweeps = 2;              // refinement sweeps after the first FP32 solve
+    int sweeps = 1;              // refinement sweeps after the first FP32 solve
     double* d_b = nullptr;       // rhs_p layout
     double* d_rowpart = nullptr; // nym x (x-kernel blocks per row)
     double* d_part0 = nullptr;   // x-kernel blocks per row
@@ -219,10 +219,10 @@ hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int s
 // FP32 form
 hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a);
 hipError_t launch_zsolve32(dfp_plan* p, int dir);
-hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0);
+hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0, double* P,
+                            double dtc, int save_p);
 hipError_t launch_mode00(dfp_plan* p);  // rowpart -> pbar of the plan's FP32 buffers
 int correct_blocks_z(const Geo& g);
-hipError_t launch_save_p(dfp_plan* p, const double* rhs, double* P, double dtc);
 hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs);
 hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs);
 hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);
@@ -256,5 +256,18 @@ int prof_begin(dfp_plan* p, const char* name);
 void prof_end(dfp_plan* p, int slot);
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// MI355X dispatches workgroup w of a launch to XCD w % 8, and each XCD has its
+// own L2.  xcd_remap(w, n) renumbers the n workgroups so that each XCD gets a
+// contiguous range of logical indices: neighbouring tiles (which read the
+// same halo rows, pivot rows or cache lines) then share an L2.  A bijection
+// on [0, n) for every n; only locality, never correctness, depends on the
+// dispatch order.
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_remap(int w, int n) {
+    const int q = n / kXcds, r = n - q * kXcds;
+    const int x = w % kXcds, i = w / kXcds;
+    return x < r ? x * (q + 1) + i : r * (q + 1) + (x - r) * q + i;
+}
 
 }  // namespace dfp

@@ -16,33 +16,60 @@ namespace dfp {
 // Row means of the field from the x kernel's per-block partial sums, then the
 // (0,0)-mode solve Dyy_pin x = mean with the inverse pivots the plan holds for
 // kx = kz = 0 (lu[j*nzs], pin of projection.f90:292 included) and the
-// elimination order of the spectral kernels.  One block; the solve itself is
-// one thread (nym sequential steps).
+// elimination order of the spectral kernels.  Rows are staged through LDS in
+// chunks by the whole block; one thread runs each recurrence over a chunk
+// from LDS (distinct arrays, so its loads run ahead of its stores).
+constexpr int kM00Chunk = 1024;
+
 __device__ __forceinline__ void mode00_body(const MPar& m) {
+    __shared__ double s_in[kM00Chunk], s_mul[kM00Chunk], s_cu[kM00Chunk], s_out[kM00Chunk];
     const double* __restrict__ rowpart = m.rowpart;
     const double* __restrict__ lu = m.lu;
     double* __restrict__ x = m.x;
     const int nym = m.nym, nparts = m.nparts;
     const long nzs = m.nzs;
-    for (int j = threadIdx.x; j < nym; j += blockDim.x) {
-        double s = 0.0;
-        for (int q = 0; q < nparts; ++q) s += rowpart[(long)j * nparts + q];
-        x[j] = s / m.count;
+    // forward: y(0) = mean(0); y(j) = mean(j) - DL(j-1)/D'(j-1) * y(j-1)
+    double yp = 0.0;
+    for (int c0 = 0; c0 < nym; c0 += kM00Chunk) {
+        const int len = min(kM00Chunk, nym - c0);
+        for (int t = threadIdx.x; t < len; t += blockDim.x) {
+            const int j = c0 + t;
+            double s = 0.0;
+            for (int q = 0; q < nparts; ++q) s += rowpart[(long)j * nparts + q];
+            s_in[t] = s / m.count;
+            s_mul[t] = j > 0 ? m.o.dl[j - 1] * lu[(j - 1) * nzs] : 0.0;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int t = 0; t < len; ++t) {
+                yp = s_in[t] - s_mul[t] * yp;
+                s_out[t] = yp;
+            }
+        }
+        __syncthreads();
+        for (int t = threadIdx.x; t < len; t += blockDim.x) x[c0 + t] = s_out[t];
+        __syncthreads();
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double yp = x[0];
-        for (int j = 1; j < nym; ++j) {
-            const double mult = m.o.dl[j - 1] * lu[(j - 1) * nzs];
-            yp = x[j] - mult * yp;
-            x[j] = yp;
+    // backward: x(nym-1) = y/D'; x(j) = (y(j) - DU(j) x(j+1)) / D'(j)
+    double xn = 0.0;
+    for (int c1 = nym; c1 > 0; c1 -= kM00Chunk) {
+        const int c0 = max(0, c1 - kM00Chunk), len = c1 - c0;
+        for (int t = threadIdx.x; t < len; t += blockDim.x) {
+            const int j = c0 + t;
+            s_in[t] = x[j];
+            s_mul[t] = lu[j * nzs];
+            s_cu[t] = j < nym - 1 ? m.o.du[j] : 0.0;
         }
-        double xn = x[nym - 1] * lu[(nym - 1) * nzs];
-        x[nym - 1] = xn;
-        for (int j = nym - 2; j >= 0; --j) {
-            xn = (x[j] - m.o.du[j] * xn) * lu[j * nzs];
-            x[j] = xn;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            for (int t = len - 1; t >= 0; --t) {
+                xn = c0 + t == nym - 1 ? s_in[t] * s_mul[t] : (s_in[t] - s_cu[t] * xn) * s_mul[t];
+                s_out[t] = xn;
+            }
         }
+        __syncthreads();
+        for (int t = threadIdx.x; t < len; t += blockDim.x) x[c0 + t] = s_out[t];
+        __syncthreads();
     }
 }
 
@@ -85,6 +112,7 @@ hipError_t launch_mode00_batch(hipStream_t s, const MPar* cs, const int* nc, int
 
 // P = phi/(dt*c) with every ghost (projection.f90:361-380, :388-438 on one
 // rank): each P element copies the periodic / wall-clamped interior cell.
+// Used by batches (a single plan writes P in its last inverse transform).
 __device__ __forceinline__ void save_p_body(const SPar& sp, int bx) {
     const Geo& g = sp.g;
     const long t = (long)bx * 256 + threadIdx.x;
@@ -100,8 +128,6 @@ __device__ __forceinline__ void save_p_body(const SPar& sp, int bx) {
     const int kk = (k - 2 + g.nzp) % g.nzp;
     sp.P[t] = sp.rhs[ii + jj * (long)(g.nxg - 1) + kk * (long)(g.nxg - 1) * g.nym] / sp.dtc;
 }
-
-__global__ void __launch_bounds__(256) k_save_p(SPar sp) { save_p_body(sp, blockIdx.x); }
 
 __global__ void __launch_bounds__(256) kb_save_p(const SPar* __restrict__ cs, const int* __restrict__ off,
                                                  const int* __restrict__ pnc) {
@@ -129,14 +155,6 @@ SPar make_spar(const dfp_plan* p, const double* rhs, double* P, double dtc) {
     sp.div_y = FastDiv(g.nyg);
     sp.nblocks = ceil_div((long)g.nxg * g.nyg * g.nzg, 256);
     return sp;
-}
-
-hipError_t launch_save_p(dfp_plan* p, const double* rhs, double* P, double dtc) {
-    const SPar sp = make_spar(p, rhs, P, dtc);
-    const int slot = prof_begin(p, "save_p");
-    hipLaunchKernelGGL(k_save_p, dim3(sp.nblocks), dim3(256), 0, p->stream, sp);
-    prof_end(p, slot);
-    return hipGetLastError();
 }
 
 hipError_t launch_save_p_batch(hipStream_t s, const SPar* cs, const int* off, const int* nc, int grid) {

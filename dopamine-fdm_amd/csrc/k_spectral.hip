@@ -215,7 +215,9 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
 template <int MODE, typename C>
 __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XPar<C> q) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    xfwd_body<MODE, C>(q, blockIdx.x, blockIdx.y, smem_raw);
+    const int L = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int j = L / gridDim.x;
+    xfwd_body<MODE, C>(q, L - j * gridDim.x, j, smem_raw);
 }
 
 // case of work item w: the last c with off[c] <= w (off[0] = 0, off[nc] = total)
@@ -235,7 +237,10 @@ __global__ void __launch_bounds__(kXThreads) kb_div_rfft_x(const XPar<C>* __rest
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int nc = *pnc;
     const int total = off[nc];
-    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    const int me = xcd_remap(blockIdx.x, gridDim.x);
+    for (int base = 0; base < total; base += gridDim.x) {
+        const int w = base + me;
+        if (w >= total) break;
         const int c = find_case(off, nc, w);
         const XPar<C>& q = cs[c];
         const int li = w - off[c];
@@ -397,7 +402,9 @@ __device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsig
 template <typename C, int ACCUM>
 __global__ void __launch_bounds__(kXThreads) k_irfft_x(XPar<C> q) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    xinv_body<C, ACCUM>(q, blockIdx.x, blockIdx.y, smem_raw);
+    const int L = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int j = L / gridDim.x;
+    xinv_body<C, ACCUM>(q, L - j * gridDim.x, j, smem_raw);
 }
 
 template <typename C, int ACCUM>
@@ -406,7 +413,10 @@ __global__ void __launch_bounds__(kXThreads) kb_irfft_x(const XPar<C>* __restric
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int nc = *pnc;
     const int total = off[nc];
-    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    const int me = xcd_remap(blockIdx.x, gridDim.x);
+    for (int base = 0; base < total; base += gridDim.x) {
+        const int w = base + me;
+        if (w >= total) break;
         const int c = find_case(off, nc, w);
         const XPar<C>& q = cs[c];
         const int li = w - off[c];
@@ -501,9 +511,11 @@ hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int s
     return launch_xinv<double2, 0>(p, q);
 }
 
-hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0) {
+hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0, double* P,
+                            double dtc, int save_p) {
     XPar<float2> q = make_xpar<float2>(p);
     q.rhs = rhs; q.pbar = pbar; q.part0 = part0;
+    q.P = P; q.dtc = save_p ? dtc : 1.0; q.save_p = save_p;
     if (accumulate) return launch_xinv<float2, 1>(p, q);
     return launch_xinv<float2, 0>(p, q);
 }

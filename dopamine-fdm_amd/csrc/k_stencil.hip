@@ -95,13 +95,20 @@ __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
     }
 }
 
-__global__ void __launch_bounds__(256) k_correct(CPar c) { correct_body(c, blockIdx.x, blockIdx.y); }
+__global__ void __launch_bounds__(256) k_correct(CPar c) {
+    const int L = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
+    const int bz = L / gridDim.x;
+    correct_body(c, L - bz * gridDim.x, bz);
+}
 
 __global__ void __launch_bounds__(256) kb_correct(const CPar* __restrict__ cs, const int* __restrict__ off,
                                                   const int* __restrict__ pnc) {
     const int nc = *pnc;
     const int total = off[nc];
-    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    const int me = xcd_remap(blockIdx.x, gridDim.x);
+    for (int base = 0; base < total; base += gridDim.x) {
+        const int w = base + me;
+        if (w >= total) break;
         int lo = 0, hi = nc;
         while (hi - lo > 1) {
             const int mid = (lo + hi) >> 1;

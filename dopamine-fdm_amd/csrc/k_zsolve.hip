@@ -272,13 +272,13 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
 template <typename C>
 __global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(ZPar<C> zp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    zfwd_body<C>(zp, blockIdx.x, smem_raw);
+    zfwd_body<C>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
 }
 
 template <typename C>
 __global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(ZPar<C> zp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    zbwd_body<C>(zp, blockIdx.x, smem_raw);
+    zbwd_body<C>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
 }
 
 __device__ __forceinline__ int zfind_case(const int* __restrict__ off, int nc, int w) {
@@ -297,7 +297,10 @@ __global__ void __launch_bounds__(kZsThreads) kb_zsolve(const ZPar<C>* __restric
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int nc = *pnc;
     const int total = off[nc];
-    for (int w = blockIdx.x; w < total; w += gridDim.x) {
+    const int me = xcd_remap(blockIdx.x, gridDim.x);
+    for (int base = 0; base < total; base += gridDim.x) {
+        const int w = base + me;
+        if (w >= total) break;
         const int c = zfind_case(off, nc, w);
         if (DIR < 0) zfwd_body<C>(cs[c], w - off[c], smem_raw);
         else zbwd_body<C>(cs[c], w - off[c], smem_raw);

@@ -86,7 +86,7 @@ class Plan:
             check(lib().dfp_plan_create_tuned(ctypes.byref(self._grid), precision, device, ctypes.byref(t),
                                               ctypes.byref(h)), "dfp_plan_create_tuned")
         self._h = h
-        self.sweeps = 2 if sweeps is None else int(sweeps)
+        self.sweeps = 1 if sweeps is None else int(sweeps)
         if sweeps is not None:
             self.set_refine_sweeps(sweeps)
 

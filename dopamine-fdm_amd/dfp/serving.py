@@ -108,7 +108,7 @@ def case_bytes(g, precisions=(DFP_FP64,)) -> int:
 
 class Server:
     def __init__(self, device: int = 0, hbm_budget: float = 200e9, max_batch: int = 32, n_streams: int = 2,
-                 graphs: bool = True, table: dict | None = None, sweeps: int = 2, occupancy=None):
+                 graphs: bool = True, table: dict | None = None, sweeps: int = 1, occupancy=None):
         self.device = device
         self.budget = float(hbm_budget)
         self.max_batch = max_batch
@@ -129,12 +129,12 @@ class Server:
         self.evictions = 0
 
     # --- admission -------------------------------------------------------------
-    def _evict_for(self, need: int) -> bool:
+    def _evict_for(self, need: int, keep=None) -> bool:
         for cid in list(self.resident.keys()):  # least recently used first
             if self.resident_bytes + need <= self.budget:
                 break
             r = self.resident[cid]
-            if r.busy:
+            if r.busy or cid == keep:
                 continue
             self.resident_bytes -= case_bytes(r.g, tuple(r.plans.keys()) or (DFP_FP64,))
             r.free()
@@ -152,7 +152,7 @@ class Server:
                 g = r.g
                 old = case_bytes(g, tuple(r.plans.keys()) or (DFP_FP64,))
                 new = case_bytes(g, tuple(r.plans.keys()) + (req.precision,))
-                if new > old and not self._evict_for(new - old):
+                if new > old and not self._evict_for(new - old, keep=req.case_id):
                     return None
                 self.resident_bytes += new - old
             self.resident.move_to_end(req.case_id)

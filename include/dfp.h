@@ -48,7 +48,7 @@ typedef struct dfp_grid {
  * DFP_FP32_REFINE: FP32 FFTs on an FP32-stored spectrum (the y elimination
  *   runs in FP64 on the stored values), the kx=kz=0 mode, the mean-pressure
  *   pin, the residual b - L_pin phi and the accumulation of phi in FP64, then
- *   dfp_set_refine_sweeps() refinement sweeps (default 2).  No counterpart in
+ *   dfp_set_refine_sweeps() refinement sweeps (default 1).  No counterpart in
  *   the reference, whose solve is FP64 only. */
 enum { DFP_FP64 = 0, DFP_FP32_REFINE = 1 };
 

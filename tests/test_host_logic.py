@@ -64,3 +64,50 @@ def test_autotune_table_roundtrip(tmp_path):
     assert autotune.lookup(t2, g, dfp.DFP_FP64) == (4, 2, 8)
     assert autotune.lookup(t2, g, dfp.DFP_FP32_REFINE) is None
     assert autotune.load_table(str(tmp_path / "missing.json")) == {"plans": {}, "batch": {}}
+
+
+class _FakeResident:
+    def __init__(self, case_id, g, table, sweeps):
+        self.case_id, self.g, self.plans, self.busy, self.freed = case_id, g, {}, False, False
+
+    def free(self):
+        self.freed = True
+
+
+def _bare_server(budget):
+    import collections
+
+    from dfp import serving
+
+    s = serving.Server.__new__(serving.Server)
+    s.budget, s.table, s.sweeps = float(budget), None, 1
+    s.resident = collections.OrderedDict()
+    s.resident_bytes, s.evictions = 0, 0
+    return s
+
+
+def test_serving_admission_evicts_lru_idle_cases(monkeypatch):
+    from dfp import serving
+    from dfp.serving import Request, case_bytes
+
+    monkeypatch.setattr(serving, "_Resident", _FakeResident)
+    shape = (32, 32, 32)
+    one = case_bytes(dfp.channel_grid(*shape, 1.5), (dfp.DFP_FP64,))
+    s = _bare_server(2.5 * one)
+    reqs = [Request(i, i, shape, 1.5, seed=i) for i in range(4)]
+    r0 = s._admit(reqs[0])
+    r1 = s._admit(reqs[1])
+    assert r0 and r1 and s.resident_bytes == 2 * one
+    r1.busy = True
+    r2 = s._admit(reqs[2])  # evicts case 0 (idle, least recent), not the busy case 1
+    assert r2 is not None and r0.freed and not r1.freed and list(s.resident) == [1, 2]
+    r2.busy = True
+    assert s._admit(reqs[3]) is None  # both residents busy: no room, nothing evicted
+    assert s.resident_bytes <= s.budget and s.evictions == 1
+    # a second precision of a resident case never evicts that case itself
+    r1.busy = False
+    r2.busy = False
+    r1.plans[dfp.DFP_FP64] = object()
+    again = s._admit(Request(9, 1, shape, 1.5, seed=9, precision=dfp.DFP_FP32_REFINE))
+    assert again is r1 and 1 in s.resident and not r1.freed
+    assert s.resident_bytes <= s.budget
