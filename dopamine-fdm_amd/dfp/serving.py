@@ -198,6 +198,8 @@ class Server:
                 continue
             r = self._admit(q)
             if r is None:
+                # later requests of this case wait too: per-case order is arrival order
+                seen.add(q.case_id)
                 skipped.append(q)
                 continue
             r.busy = True  # chosen: later admissions in this batch must not evict it
