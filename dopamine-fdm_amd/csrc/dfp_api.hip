@@ -115,6 +115,90 @@ void dyy(const std::vector<double>& y, const std::vector<double>& yg, std::vecto
     }
 }
 
+// Launch shape of a plan: z-split radix R (the x kernels apply the first
+// radix-R stage of the z FFT), x columns per workgroup T, and either the
+// fused z kernel (whole (kx,u) sub-plane in LDS) or the split pair of z
+// kernels with G rows per group.
+struct Shape {
+    int R = 1, T = 1, G = 1, fused = 0;
+    FftDesc fx{}, fz{};
+};
+
+constexpr int kRadices[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
+constexpr size_t kFusedSweetLds = 56 * 1024;  // three fused workgroups per CU
+
+// x columns per workgroup: the largest divisor of nz2 within about 2048
+// points (or the caller's), if the x kernels fit; 0 if none does
+int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, int want) {
+    const int M = nxp / 2, nz2 = nzp / sh.R;
+    auto fits = [&](int T) {
+        const size_t pts = (size_t)sh.R * T;
+        return pts * M <= (size_t)inplace_capacity(sh.fx, 512, 2) &&
+               cs * (pts * (M + 1) + twiddle_count(sh.fx) + M + 1 + (size_t)nz2 * (sh.R - 1)) + 512 <= lds_limit;
+    };
+    if (want > 0) return (nz2 % want == 0 && fits(want)) ? want : 0;
+    for (int t = std::max(1, std::min(nz2, 2048 / (sh.R * M))); t >= 1; --t)
+        if (nz2 % t == 0 && fits(t)) return t;
+    return 0;
+}
+
+// 0 on success, else a DFP_E_* code
+int choose_shape(const dfp_grid* g, int f32, size_t lds_limit, const dfp_tuning* tune, Shape& out) {
+    const size_t cs = f32 ? sizeof(float2) : sizeof(double2);
+    const int want_R = tune ? tune->z_split : 0, want_T = tune ? tune->x_columns : 0;
+    const int want_G = tune ? tune->z_rows : 0, want_f = tune ? tune->z_fused : 0;
+    if (want_R < 0 || want_T < 0 || want_G < 0 || want_f < 0 || want_f > 2) return DFP_E_ARG;
+    if (want_R > 0) {
+        bool known = false;
+        for (int r : kRadices) known = known || r == want_R;
+        if (!known || g->nzp % want_R != 0) return DFP_E_ARG;
+    }
+    const bool allow_fused = want_f == 2 || (want_f == 0 && want_G == 0);
+    const bool allow_split = want_f != 2;
+    std::vector<Shape> fused, split;
+    for (int R : kRadices) {
+        if (g->nzp % R != 0 || (want_R > 0 && R != want_R)) continue;
+        Shape sh;
+        sh.R = R;
+        if (!factor(g->nxp / 2, sh.fx) || !factor(g->nzp / R, sh.fz)) continue;
+        sh.T = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T);
+        if (sh.T == 0 || sh.fz.n > 256) continue;
+        if (allow_fused && zf_lds_bytes(g->nym, sh.fz.n, twiddle_count(sh.fz), f32) <= lds_limit) {
+            Shape f = sh;
+            f.fused = 1;
+            f.G = zf_rows_per_call(f.fz);
+            fused.push_back(f);
+        }
+        if (allow_split) {
+            Shape q = sh;
+            q.G = zsolve_rows_per_group(q.fz, g->nym, lds_limit);
+            if (want_G > 0) q.G = want_G <= q.G ? want_G : 0;
+            if (q.G >= 1) split.push_back(q);
+        }
+    }
+    // fused where its plane leaves room for three workgroups per CU (smallest
+    // such R), else the split kernels with the classic split (R = 4 when it
+    // leaves at least 8 points per row, else 2, else 3, else none), else the
+    // fused form at any LDS size
+    auto zlds = [&](const Shape& sh) { return zf_lds_bytes(g->nym, sh.fz.n, twiddle_count(sh.fz), f32); };
+    for (const Shape& f : fused)
+        if (zlds(f) <= kFusedSweetLds || want_f == 2) { out = f; return DFP_OK; }
+    if (!split.empty()) {
+        if (want_R > 0) { out = split[0]; return DFP_OK; }
+        const int nzp = g->nzp;
+        int pick = 1;
+        if (nzp % 4 == 0 && nzp / 4 >= 8) pick = 4;
+        else if (nzp % 2 == 0 && nzp / 2 >= 4) pick = 2;
+        else if (nzp % 3 == 0 && nzp / 3 >= 4) pick = 3;
+        for (const Shape& q : split)
+            if (q.R == pick) { out = q; return DFP_OK; }
+        out = split[0];
+        return DFP_OK;
+    }
+    if (!fused.empty()) { out = fused.back(); return DFP_OK; }
+    return DFP_E_UNSUPPORTED;
+}
+
 bool finite_all(const double* a, int n) {
     for (int i = 0; i < n; ++i)
         if (!std::isfinite(a[i])) return false;
@@ -177,6 +261,17 @@ int check_flags(dfp_plan* p, int flags) {
     return DFP_OK;
 }
 
+// z transforms and y solves of the spectrum: the fused kernel or the split pair
+hipError_t z_solve(dfp_plan* p, int f32) {
+    if (p->zfused) {
+        if (f32) return launch_zfused<float2>(p, make_zpar<float2>(p, 1));
+        return launch_zfused<double2>(p, make_zpar<double2>(p, 0));
+    }
+    hipError_t e = f32 ? launch_zsolve32(p, -1) : launch_zsolve(p, -1);
+    if (e != hipSuccess) return e;
+    return f32 ? launch_zsolve32(p, +1) : launch_zsolve(p, +1);
+}
+
 // FP32 form: one FP32 spectral solve with the (0,0) mode in FP64, then
 // p->sweeps refinement sweeps on the FP64 residual b - L_pin phi
 int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs, double* P, double dtc,
@@ -187,8 +282,7 @@ int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, 
     a.rowpart = p->d_rowpart;
     HIPCHK(p, launch_rfft_x32(p, from_fields ? 0 : 1, a));
     HIPCHK(p, launch_mode00(p));
-    HIPCHK(p, launch_zsolve32(p, -1));
-    HIPCHK(p, launch_zsolve32(p, +1));
+    HIPCHK(p, z_solve(p, 1));
     // P = phi/(dt*c) is written by the last inverse transform
     HIPCHK(p, launch_irfft_x32(p, rhs, 0, p->d_pbar, p->d_part0, P, dtc, save_p && p->sweeps == 0));
     for (int s = 0; s < p->sweeps; ++s) {
@@ -201,8 +295,7 @@ int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, 
         r.idz2 = 1.0 / (p->dz * p->dz);
         HIPCHK(p, launch_rfft_x32(p, 2, r));
         HIPCHK(p, launch_mode00(p));
-        HIPCHK(p, launch_zsolve32(p, -1));
-        HIPCHK(p, launch_zsolve32(p, +1));
+        HIPCHK(p, z_solve(p, 1));
         HIPCHK(p, launch_irfft_x32(p, rhs, 1, p->d_pbar, p->d_part0, P, dtc, save_p && s == p->sweeps - 1));
     }
     return DFP_OK;
@@ -213,8 +306,7 @@ int spectral_solve(dfp_plan* p, const double* U, const double* V, const double* 
                    int save_p, int from_fields) {
     if (p->precision == DFP_FP32_REFINE) return mixed_solve(p, U, V, W, rhs, P, dtc, save_p, from_fields);
     HIPCHK(p, launch_div_rfft_x(p, U, V, W, rhs, from_fields ? 0 : 1));
-    HIPCHK(p, launch_zsolve(p, -1));
-    HIPCHK(p, launch_zsolve(p, +1));
+    HIPCHK(p, z_solve(p, 0));
     HIPCHK(p, launch_irfft_x(p, rhs, P, dtc, save_p));
     return DFP_OK;
 }
@@ -277,6 +369,7 @@ int dfp_plan_tuning(const dfp_plan* p, dfp_tuning* t) {
     t->z_split = p->zr;
     t->x_columns = p->tk_x;
     t->z_rows = p->g_zs;
+    t->z_fused = p->zfused ? 2 : 1;
     return DFP_OK;
 }
 
@@ -300,21 +393,24 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     };
     p->device = device;
     p->precision = precision;
-    // z split: the x kernels apply the first radix-zr stage of the z FFT, the
-    // z kernels transform sub-rows of nzp/zr points, one mode per thread
-    int zr = 1;
-    if (tune && tune->z_split > 0) {
-        zr = tune->z_split;
-        if (zr > 4 || g->nzp % zr != 0) return bail(DFP_E_ARG);
-    } else if (g->nzp % 4 == 0 && g->nzp / 4 >= 8) {
-        zr = 4;
-    } else if (g->nzp % 2 == 0 && g->nzp / 2 >= 4) {
-        zr = 2;
-    } else if (g->nzp % 3 == 0 && g->nzp / 3 >= 4) {
-        zr = 3;
+    {
+        // FFT lengths 2^a 3^b, checked before the device is touched
+        FftDesc probe;
+        if (!factor(g->nxp / 2, probe) || !factor(g->nzp, probe)) return bail(DFP_E_UNSUPPORTED);
     }
-    p->zr = zr;
-    if (!factor(g->nxp / 2, p->fx) || !factor(g->nzp / zr, p->fz)) return bail(DFP_E_UNSUPPORTED);
+    if (hipSetDevice(device) != hipSuccess) return bail(DFP_E_HIP);
+    int lds_attr = 0;
+    if (hipDeviceGetAttribute(&lds_attr, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds_attr > 0)
+        p->lds_limit = (size_t)lds_attr;
+    Shape sh;
+    if (int rc = choose_shape(g, precision == DFP_FP32_REFINE, p->lds_limit, tune, sh)) return bail(rc);
+    const int zr = sh.R;
+    p->zr = sh.R;
+    p->fx = sh.fx;
+    p->fz = sh.fz;
+    p->tk_x = sh.T;
+    p->g_zs = sh.G;
+    p->zfused = sh.fused;
     if (hipSetDevice(device) != hipSuccess) return bail(DFP_E_HIP);
     Geo& G = p->geo;
     G.nxp = g->nxp; G.nym = g->nym; G.nzp = g->nzp; G.nxh = g->nxp / 2 + 1;
@@ -409,32 +505,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     p->sz_p = (int64_t)G.nxg * G.nyg * G.nzg;
     p->sz_rhs = (int64_t)(G.nxg - 1) * G.nym * (G.nzg - 1);
 
-    // launch shapes: x blocks hold about 2048 complex points (zr lines per
-    // column), z blocks as many rows as fit their registers and LDS
-    int lds_attr = 0;
-    if (hipDeviceGetAttribute(&lds_attr, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds_attr > 0)
-        p->lds_limit = (size_t)lds_attr;
-    const int M = G.nxp / 2, nz2 = G.nzp / zr;
-    // columns per x block: the largest divisor of nz2 within about 2048 points
-    p->tk_x = 1;
-    if (tune && tune->x_columns > 0) {
-        if (nz2 % tune->x_columns != 0) return bail(DFP_E_ARG);
-        p->tk_x = tune->x_columns;
-    } else {
-        for (int t = std::max(1, std::min(nz2, 2048 / (zr * M))); t >= 1; --t)
-            if (nz2 % t == 0) { p->tk_x = t; break; }
-    }
-    if (sizeof(double2) * ((size_t)zr * p->tk_x * (M + 1) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (zr - 1)) +
-                64 * sizeof(double) >
-            p->lds_limit ||
-        (size_t)zr * p->tk_x * M > (size_t)inplace_capacity(p->fx, 512, 2))
-        return bail(DFP_E_UNSUPPORTED);
-    p->g_zs = zsolve_rows_per_group(p->fz, G.nym, p->lds_limit);
-    if (p->g_zs < 1) return bail(DFP_E_UNSUPPORTED);
-    if (tune && tune->z_rows > 0) {
-        if (tune->z_rows > p->g_zs) return bail(DFP_E_UNSUPPORTED);
-        p->g_zs = tune->z_rows;
-    }
+    const int nz2 = G.nzp / zr;
     if (precision == DFP_FP32_REFINE) {
         const size_t nparts = (size_t)ceil_div(nz2, p->tk_x);
         if (hipMalloc(&p->d_b, sizeof(double) * p->sz_rhs) != hipSuccess ||

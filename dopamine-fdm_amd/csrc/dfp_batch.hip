@@ -13,14 +13,19 @@
 // batch are independent; a plan may appear once per batch.
 //
 // Stage list of one call (FP64 cases a, FP32+refine cases b, the latter
-// ordered by descending sweep count so sweep s runs the first n_s of them):
+// ordered by descending sweep count so sweep s runs the first n_s of them);
+// each x stage is one launch per z-split radix present (the x kernels are
+// compiled per radix), each z stage one launch for the cases with the fused
+// z kernel and one pair for those with the split pair:
 //   x forward  (a: divergence, b: divergence + FP64 copy + row sums)
 //   (0,0) mode (b)
-//   z forward, z backward (a, b)
+//   z transforms and y solves (a, b)
 //   x inverse  (a: phi + P, b: phi + pbar + first-row sums)
-//   per sweep s: x forward residual, (0,0) mode, z fwd, z bwd, x inverse ACCUM (b[0:n_s])
+//   per sweep s: x forward residual, (0,0) mode, z, x inverse ACCUM (b[0:n_s])
 //   save P (b with DFP_SAVE_P)
 //   correction (a and b)
+// A launch reads a case list: the work offsets of the listed cases, their
+// record indices and their count, all in the device buffer.
 #include <algorithm>
 #include <cstdarg>
 #include <cstdio>
@@ -36,23 +41,15 @@ using namespace dfp;
 namespace {
 
 constexpr int kMaxSweepStages = 9;
+constexpr int kRadixList[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};  // z-split radices the x kernels are compiled for
 
 // offsets (in bytes) of the tables inside the staging buffer
 struct Layout {
-    size_t x64, z64, x32, x32s, z32, m32, s32, c, counts, off;
+    size_t x64, z64, x32, x32s, z32, m32, s32, c, counts, lists;
     size_t total;
 };
 
-// per-stage work offsets, each an (n+1)-array of ints, and the case counts
-enum OffId {
-    O_X64 = 0,   // x kernels of the FP64 cases (forward and inverse)
-    O_Z64,       // z kernels of the FP64 cases
-    O_X32,       // x kernels of the FP32 cases
-    O_Z32,       // z kernels of the FP32 cases
-    O_S32,       // save P of the FP32 cases with DFP_SAVE_P
-    O_C,         // correction of every case (FP64 first, then FP32)
-    O_NUM
-};
+constexpr int kMaxLists = 192;  // case lists per call (each launch reads one)
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -72,9 +69,9 @@ Layout layout_for(int cap) {
     L.m32 = take(sizeof(MPar) * cap);
     L.s32 = take(sizeof(SPar) * cap);
     L.c = take(sizeof(CPar) * cap);
-    // counts: one int per (stage offset array, prefix length) used by a launch
-    L.counts = take(sizeof(int) * 64);
-    L.off = take(sizeof(int) * (size_t)O_NUM * (cap + 1));
+    // list counts, then the lists: offsets (count + 1) and record indices (count)
+    L.counts = take(sizeof(int) * kMaxLists);
+    L.lists = take(sizeof(int) * (size_t)kMaxLists * (2 * cap + 1));
     L.total = o;
     return L;
 }
@@ -118,15 +115,14 @@ int bfail(dfp_batch* b, int code, const char* fmt, ...) {
         if (e_ != hipSuccess) return bfail((b), DFP_E_HIP, "%s: %s", #expr, hipGetErrorString(e_));     \
     } while (0)
 
-// One launch of the batch: which kernel, on which tables, with which count slot.
+// One launch of the batch: which kernel, on which records, with which case list.
 struct Launch {
-    enum Kind { XF, XI, ZF, ZB, M00, SAVE, CORR } kind;
-    int f32;       // spectrum precision
-    int mode;      // x forward mode, or ACCUM for the inverse
-    size_t tab;    // offset of the case records
-    int off_id;    // work offsets
-    int count_at;  // index of the case count in the counts table
-    int ncases;    // host-side count (for LDS sizing and mode00's grid)
+    enum Kind { XF, XI, ZF, ZB, ZFUSED, M00, SAVE, CORR } kind;
+    int f32;      // spectrum precision
+    int mode;     // x forward mode, or ACCUM for the inverse
+    int R;        // z-split radix (x kernels)
+    size_t tab;   // offset of the case records
+    int list;     // case list
     size_t lds;
 };
 
@@ -259,11 +255,11 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
     auto* s32 = reinterpret_cast<SPar*>(H + L.s32);
     auto* cc = reinterpret_cast<CPar*>(H + L.c);
     int* counts = reinterpret_cast<int*>(H + L.counts);
-    int* off = reinterpret_cast<int*>(H + L.off);
-    auto offp = [&](int id) { return off + (size_t)id * (b->cap + 1); };
-    for (int id = 0; id < O_NUM; ++id) offp(id)[0] = 0;
+    int* pool = reinterpret_cast<int*>(H + L.lists);
+    const size_t list_stride = 2 * (size_t)b->cap + 1;
 
-    size_t lds_xf64 = 0, lds_xi64 = 0, lds_z64 = 0, lds_xf32 = 0, lds_xi32 = 0, lds_z32 = 0;
+    std::vector<size_t> w64x(n64), w64z(n64), w32x(n32), w32z(n32), wc(n), ws;
+    std::vector<int> R64(n64), F64(n64), R32(n32), F32(n32);
     for (int a = 0; a < n64; ++a) {
         const int i = i64[a];
         dfp_plan* p = plans[i];
@@ -276,14 +272,14 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         q.dtc = q.save_p ? dt[i] * c[i] : 1.0;
         x64[a] = q;
         z64[a] = make_zpar<double2>(p, 0);
-        offp(O_X64)[a + 1] = offp(O_X64)[a] + q.nbx * p->geo.nym;
-        offp(O_Z64)[a + 1] = offp(O_Z64)[a] + z64[a].nblocks;
-        lds_xf64 = std::max(lds_xf64, x_lds_bytes(p, 0, 0));
-        lds_xi64 = std::max(lds_xi64, x_lds_bytes(p, 1, 0));
-        lds_z64 = std::max(lds_z64, z_lds_bytes(p, 0));
+        w64x[a] = (size_t)q.nbx * p->geo.nym;
+        w64z[a] = z64[a].nblocks;
+        R64[a] = p->zr;
+        F64[a] = p->zfused;
     }
     int ns = 0;
     std::vector<int> n_at_sweep(kMaxSweepStages + 1, 0);  // FP32 cases with sweeps > s
+    std::vector<int> save_rec;
     for (int a = 0; a < n32; ++a) {
         const int i = i32[a];
         dfp_plan* p = plans[i];
@@ -307,104 +303,157 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         x32s[a] = r;
         z32[a] = make_zpar<float2>(p, 1);
         m32[a] = make_mpar(p);
-        offp(O_X32)[a + 1] = offp(O_X32)[a] + q.nbx * p->geo.nym;
-        offp(O_Z32)[a + 1] = offp(O_Z32)[a] + z32[a].nblocks;
+        w32x[a] = (size_t)q.nbx * p->geo.nym;
+        w32z[a] = z32[a].nblocks;
+        R32[a] = p->zr;
+        F32[a] = p->zfused;
         for (int s = 0; s < std::min(p->sweeps, kMaxSweepStages); ++s) n_at_sweep[s] = a + 1;
         if (fl & DFP_SAVE_P) {
             s32[ns] = make_spar(p, rhs_p[i], P[i], dt[i] * c[i]);
-            offp(O_S32)[ns + 1] = offp(O_S32)[ns] + s32[ns].nblocks;
+            ws.push_back(s32[ns].nblocks);
+            save_rec.push_back(ns);
             ++ns;
         }
-        lds_xf32 = std::max(lds_xf32, x_lds_bytes(p, 0, 1));
-        lds_xi32 = std::max(lds_xi32, x_lds_bytes(p, 1, 1));
-        lds_z32 = std::max(lds_z32, z_lds_bytes(p, 1));
     }
     for (int a = 0; a < n; ++a) {
         const int i = a < n64 ? i64[a] : i32[a - n64];
         dfp_plan* p = plans[i];
         cc[a] = make_cpar(p, U[i], V[i], W[i], rhs_p[i]);
-        offp(O_C)[a + 1] = offp(O_C)[a] + cc[a].nbx * correct_blocks_z(p->geo);
+        wc[a] = (size_t)cc[a].nbx * correct_blocks_z(p->geo);
     }
 
-    // launch list; counts[k] is the case count a launch reads
-    std::vector<Launch> ls;
-    int nct = 0;
-    const size_t lds_limit = plans[0]->lds_limit;
-    auto add = [&](Launch::Kind k, int f32, int mode, size_t tab, int off_id, int ncase, size_t lds) {
-        if (nct < 64) counts[nct] = ncase;
-        ls.push_back(Launch{k, f32, mode, tab, off_id, nct, ncase, lds_bucket(lds, lds_limit)});
-        ++nct;
+    // case lists: records recs with work items w[rec].  A list slot holds the
+    // offsets at [0, cap] and the record indices at [cap + 1, 2 cap]: fixed
+    // positions, so a launch (and a graph that captured it) reads the same
+    // addresses whatever the list's length
+    int nlists = 0;
+    bool overflow = false;
+    auto make_list = [&](const std::vector<int>& recs, const std::vector<size_t>& w) {
+        if (nlists >= kMaxLists) { overflow = true; return 0; }
+        int* o = pool + (size_t)nlists * list_stride;
+        int* ix = o + b->cap + 1;
+        o[0] = 0;
+        for (size_t k = 0; k < recs.size(); ++k) {
+            o[k + 1] = o[k] + (int)w[recs[k]];
+            ix[k] = recs[k];
+        }
+        counts[nlists] = (int)recs.size();
+        return nlists++;
     };
+    const size_t lds_limit = plans[0]->lds_limit;
+    std::vector<Launch> ls;
+    auto add = [&](Launch::Kind k, int f32, int mode, int R, size_t tab, int list, size_t lds) {
+        ls.push_back(Launch{k, f32, mode, R, tab, list, lds_bucket(lds, lds_limit)});
+    };
+    // the x launches of a set of records, one per radix present; the z launches
+    // for its fused and split members
+    auto x_launches = [&](Launch::Kind k, int f32, int mode, size_t tab, const std::vector<int>& recs,
+                          const std::vector<int>& Rs, const std::vector<size_t>& w, int inv) {
+        for (int R : kRadixList) {
+            std::vector<int> sel;
+            size_t lds = 0;
+            for (int a : recs)
+                if (Rs[a] == R) {
+                    sel.push_back(a);
+                    lds = std::max(lds, x_lds_bytes(plans[f32 ? i32[a] : i64[a]], inv, f32));
+                }
+            if (!sel.empty()) add(k, f32, mode, R, tab, make_list(sel, w), lds);
+        }
+    };
+    auto z_launches = [&](int f32, size_t tab, const std::vector<int>& recs, const std::vector<int>& Fs,
+                          const std::vector<size_t>& w) {
+        std::vector<int> fz, sz;
+        size_t lf = 0, lsz = 0;
+        for (int a : recs) {
+            dfp_plan* p = plans[f32 ? i32[a] : i64[a]];
+            if (Fs[a]) { fz.push_back(a); lf = std::max(lf, zf_lds_bytes(p, f32)); }
+            else { sz.push_back(a); lsz = std::max(lsz, z_lds_bytes(p, f32)); }
+        }
+        if (!fz.empty()) add(Launch::ZFUSED, f32, 0, 0, tab, make_list(fz, w), lf);
+        if (!sz.empty()) {
+            const int li = make_list(sz, w);
+            add(Launch::ZF, f32, 0, 0, tab, li, lsz);
+            add(Launch::ZB, f32, 0, 0, tab, li, lsz);
+        }
+    };
+    auto first = [](int k) { std::vector<int> v(k); for (int a = 0; a < k; ++a) v[a] = a; return v; };
     if (n64) {
-        add(Launch::XF, 0, 0, L.x64, O_X64, n64, lds_xf64);
-        add(Launch::ZF, 0, 0, L.z64, O_Z64, n64, lds_z64);
-        add(Launch::ZB, 0, 0, L.z64, O_Z64, n64, lds_z64);
-        add(Launch::XI, 0, 0, L.x64, O_X64, n64, lds_xi64);
+        const std::vector<int> all = first(n64);
+        x_launches(Launch::XF, 0, 0, L.x64, all, R64, w64x, 0);
+        z_launches(0, L.z64, all, F64, w64z);
+        x_launches(Launch::XI, 0, 0, L.x64, all, R64, w64x, 1);
     }
     if (n32) {
-        add(Launch::XF, 1, 0, L.x32, O_X32, n32, lds_xf32);
-        add(Launch::M00, 1, 0, L.m32, O_X32, n32, 0);
-        add(Launch::ZF, 1, 0, L.z32, O_Z32, n32, lds_z32);
-        add(Launch::ZB, 1, 0, L.z32, O_Z32, n32, lds_z32);
-        add(Launch::XI, 1, 0, L.x32, O_X32, n32, lds_xi32);
+        const std::vector<int> all = first(n32);
+        const std::vector<size_t> one(n32, 1);
+        x_launches(Launch::XF, 1, 0, L.x32, all, R32, w32x, 0);
+        add(Launch::M00, 1, 0, 0, L.m32, make_list(all, one), 0);
+        z_launches(1, L.z32, all, F32, w32z);
+        x_launches(Launch::XI, 1, 0, L.x32, all, R32, w32x, 1);
         for (int s = 0; s < max_sweeps; ++s) {
-            const int m = n_at_sweep[s];
-            add(Launch::XF, 1, 2, L.x32s, O_X32, m, lds_xf32);
-            add(Launch::M00, 1, 0, L.m32, O_X32, m, 0);
-            add(Launch::ZF, 1, 0, L.z32, O_Z32, m, lds_z32);
-            add(Launch::ZB, 1, 0, L.z32, O_Z32, m, lds_z32);
-            add(Launch::XI, 1, 1, L.x32s, O_X32, m, lds_xi32);
+            const std::vector<int> pre = first(n_at_sweep[s]);
+            x_launches(Launch::XF, 1, 2, L.x32s, pre, R32, w32x, 0);
+            add(Launch::M00, 1, 0, 0, L.m32, make_list(pre, one), 0);
+            z_launches(1, L.z32, pre, F32, w32z);
+            x_launches(Launch::XI, 1, 1, L.x32s, pre, R32, w32x, 1);
         }
-        if (ns) add(Launch::SAVE, 1, 0, L.s32, O_S32, ns, 0);
+        if (ns) add(Launch::SAVE, 1, 0, 0, L.s32, make_list(save_rec, ws), 0);
     }
-    add(Launch::CORR, 0, 0, L.c, O_C, n, 0);
-    if (nct > 64) return bfail(b, DFP_E_UNSUPPORTED, "too many stages (%d)", nct);
+    add(Launch::CORR, 0, 0, 0, L.c, make_list(first(n), wc), 0);
+    if (overflow) return bfail(b, DFP_E_UNSUPPORTED, "too many case lists in one call");
 
-    // tables to the device, then the launches (or their graph)
-    BCHK(b, hipMemcpyAsync(b->d_buf, b->h_buf, L.total, hipMemcpyHostToDevice, b->stream));
+    // tables to the device (up to the last list used), then the launches (or their graph)
+    const size_t used = L.lists + sizeof(int) * (size_t)nlists * list_stride;
+    BCHK(b, hipMemcpyAsync(b->d_buf, b->h_buf, used, hipMemcpyHostToDevice, b->stream));
     BCHK(b, hipEventRecord(b->copied, b->stream));
     b->copy_pending = true;
 
     unsigned char* D = b->d_buf;
     const int* d_counts = reinterpret_cast<const int*>(D + L.counts);
-    const int* d_off = reinterpret_cast<const int*>(D + L.off);
-    auto d_offp = [&](int id) { return d_off + (size_t)id * (b->cap + 1); };
+    const int* d_pool = reinterpret_cast<const int*>(D + L.lists);
     const int gx = b->n_cu * b->occ_x, gz = b->n_cu * b->occ_z, gs = b->n_cu * b->occ_s;
     auto run_all = [&](hipStream_t st) -> hipError_t {
         for (const Launch& l : ls) {
-            const int* o = d_offp(l.off_id);
-            const int* nc = d_counts + l.count_at;
+            const int* o = d_pool + (size_t)l.list * list_stride;
+            const int* ix = o + b->cap + 1;
+            const int* nc = d_counts + l.list;
             hipError_t e = hipSuccess;
             switch (l.kind) {
                 case Launch::XF:
-                    e = l.f32 ? launch_xfwd_batch(st, l.mode, reinterpret_cast<const XPar<float2>*>(D + l.tab), o, nc,
-                                                  gx, l.lds)
-                              : launch_xfwd_batch(st, l.mode, reinterpret_cast<const XPar<double2>*>(D + l.tab), o,
-                                                  nc, gx, l.lds);
+                    e = l.f32 ? launch_xfwd_batch(st, l.R, l.mode, reinterpret_cast<const XPar<float2>*>(D + l.tab),
+                                                  o, ix, nc, gx, l.lds)
+                              : launch_xfwd_batch(st, l.R, l.mode, reinterpret_cast<const XPar<double2>*>(D + l.tab),
+                                                  o, ix, nc, gx, l.lds);
                     break;
                 case Launch::XI:
-                    e = l.f32 ? launch_xinv_batch(st, l.mode, reinterpret_cast<const XPar<float2>*>(D + l.tab), o, nc,
-                                                  gx, l.lds)
-                              : launch_xinv_batch(st, l.mode, reinterpret_cast<const XPar<double2>*>(D + l.tab), o,
-                                                  nc, gx, l.lds);
+                    e = l.f32 ? launch_xinv_batch(st, l.R, l.mode, reinterpret_cast<const XPar<float2>*>(D + l.tab),
+                                                  o, ix, nc, gx, l.lds)
+                              : launch_xinv_batch(st, l.R, l.mode, reinterpret_cast<const XPar<double2>*>(D + l.tab),
+                                                  o, ix, nc, gx, l.lds);
                     break;
                 case Launch::ZF:
                 case Launch::ZB: {
                     const int dir = l.kind == Launch::ZF ? -1 : 1;
-                    e = l.f32 ? launch_zsolve_batch(st, dir, reinterpret_cast<const ZPar<float2>*>(D + l.tab), o, nc,
-                                                    gz, l.lds)
-                              : launch_zsolve_batch(st, dir, reinterpret_cast<const ZPar<double2>*>(D + l.tab), o, nc,
-                                                    gz, l.lds);
+                    e = l.f32 ? launch_zsolve_batch(st, dir, reinterpret_cast<const ZPar<float2>*>(D + l.tab), o, ix,
+                                                    nc, gz, l.lds)
+                              : launch_zsolve_batch(st, dir, reinterpret_cast<const ZPar<double2>*>(D + l.tab), o, ix,
+                                                    nc, gz, l.lds);
                     break;
                 }
+                case Launch::ZFUSED:
+                    e = l.f32 ? launch_zfused_batch(st, reinterpret_cast<const ZPar<float2>*>(D + l.tab), o, ix, nc,
+                                                    gz, l.lds)
+                              : launch_zfused_batch(st, reinterpret_cast<const ZPar<double2>*>(D + l.tab), o, ix, nc,
+                                                    gz, l.lds);
+                    break;
                 case Launch::M00:
-                    e = launch_mode00_batch(st, reinterpret_cast<const MPar*>(D + l.tab), nc, std::min(b->cap, 256));
+                    e = launch_mode00_batch(st, reinterpret_cast<const MPar*>(D + l.tab), ix, nc, std::min(b->cap, 256));
                     break;
                 case Launch::SAVE:
-                    e = launch_save_p_batch(st, reinterpret_cast<const SPar*>(D + l.tab), o, nc, gs);
+                    e = launch_save_p_batch(st, reinterpret_cast<const SPar*>(D + l.tab), o, ix, nc, gs);
                     break;
                 case Launch::CORR:
-                    e = launch_correct_batch(st, reinterpret_cast<const CPar*>(D + l.tab), o, nc, gs);
+                    e = launch_correct_batch(st, reinterpret_cast<const CPar*>(D + l.tab), o, ix, nc, gs);
                     break;
             }
             if (e != hipSuccess) return e;
@@ -415,13 +464,14 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
     if (!b->use_graphs) {
         BCHK(b, run_all(b->stream));
     } else {
-        // key: the launch list as the kernels see it (kind, precision, mode, table, counts slot, LDS)
+        // key: the launch list as the kernels see it (kind, precision, mode, radix, records, list, LDS)
         std::vector<long> key;
         for (const Launch& l : ls) {
             key.push_back(l.kind);
             key.push_back(l.f32 * 16 + l.mode);
+            key.push_back(l.R);
             key.push_back((long)l.tab);
-            key.push_back(l.off_id * 64 + l.count_at);
+            key.push_back(l.list);
             key.push_back((long)l.lds);
         }
         auto it = b->graphs.find(key);

@@ -202,7 +202,8 @@ struct dfp_plan {
     std::string last_error;
     // launch shapes
     int tk_x = 8;   // z columns per block of the x-transform kernels (zr lines each)
-    int g_zs = 4;   // rows per group of the fused z-transform / y-solve kernels
+    int g_zs = 4;   // rows per group of the split z kernels, or rows per FFT call of the fused one
+    int zfused = 0; // 1: one fused z kernel (whole sub-plane in LDS); 0: the split pair
     size_t lds_limit = 65536;  // LDS bytes per block on this device
 };
 
@@ -216,6 +217,14 @@ hipError_t launch_lu_table(dfp_plan* p);
 hipError_t launch_zsolve(dfp_plan* p, int dir);
 int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit);
 hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p);
+// fused z transform + y solve (whole sub-plane in LDS)
+template <typename C> hipError_t launch_zfused(dfp_plan* p, const ZPar<C>& zp);
+template <typename C>
+hipError_t launch_zfused_batch(hipStream_t s, const ZPar<C>* cs, const int* off, const int* idx, const int* nc,
+                               int grid, size_t lds);
+size_t zf_lds_bytes(int nym, int n, int ntw, int f32);
+size_t zf_lds_bytes(const dfp_plan* p, int f32);
+int zf_rows_per_call(const FftDesc& fz);
 // FP32 form
 hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a);
 hipError_t launch_zsolve32(dfp_plan* p, int dir);
@@ -228,9 +237,9 @@ hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const do
 hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);
 hipError_t launch_synth(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma);
 
-// parameter records and batched (persistent) launches: cs, off (per-case
-// work offsets, off[0] = 0, off[*nc] = total items) and the case count *nc
-// are in device memory
+// parameter records and batched (persistent) launches: cs (records), off
+// (per-listed-case work offsets, off[0] = 0, off[*nc] = total items), idx
+// (listed case -> record) and the count *nc are in device memory
 template <typename C> XPar<C> make_xpar(const dfp_plan* p);
 template <typename C> ZPar<C> make_zpar(const dfp_plan* p, int zero00);
 CPar make_cpar(const dfp_plan* p, double* U, double* V, double* W, const double* rhs);
@@ -239,23 +248,36 @@ SPar make_spar(const dfp_plan* p, const double* rhs, double* P, double dtc);
 size_t x_lds_bytes(const dfp_plan* p, int inv, int f32);
 size_t z_lds_bytes(const dfp_plan* p, int f32);
 template <typename C>
-hipError_t launch_xfwd_batch(hipStream_t s, int mode, const XPar<C>* cs, const int* off, const int* nc, int grid,
-                             size_t lds);
+hipError_t launch_xfwd_batch(hipStream_t s, int R, int mode, const XPar<C>* cs, const int* off, const int* idx,
+                             const int* nc, int grid, size_t lds);
 template <typename C>
-hipError_t launch_xinv_batch(hipStream_t s, int accum, const XPar<C>* cs, const int* off, const int* nc, int grid,
-                             size_t lds);
+hipError_t launch_xinv_batch(hipStream_t s, int R, int accum, const XPar<C>* cs, const int* off, const int* idx,
+                             const int* nc, int grid, size_t lds);
 template <typename C>
-hipError_t launch_zsolve_batch(hipStream_t s, int dir, const ZPar<C>* cs, const int* off, const int* nc, int grid,
-                               size_t lds);
-hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* nc, int grid);
-hipError_t launch_mode00_batch(hipStream_t s, const MPar* cs, const int* nc, int grid);
-hipError_t launch_save_p_batch(hipStream_t s, const SPar* cs, const int* off, const int* nc, int grid);
+hipError_t launch_zsolve_batch(hipStream_t s, int dir, const ZPar<C>* cs, const int* off, const int* idx,
+                               const int* nc, int grid, size_t lds);
+hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* idx, const int* nc,
+                                int grid);
+hipError_t launch_mode00_batch(hipStream_t s, const MPar* cs, const int* idx, const int* nc, int grid);
+hipError_t launch_save_p_batch(hipStream_t s, const SPar* cs, const int* off, const int* idx, const int* nc,
+                               int grid);
 
 // profiler hooks used by the launchers
 int prof_begin(dfp_plan* p, const char* name);
 void prof_end(dfp_plan* p, int slot);
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
+
+// The pointers in the case records all address global memory.  Saying so
+// lets the compiler use global (not flat) memory instructions where a record
+// is read from memory rather than from the kernel arguments (the batched
+// kernels): flat accesses would also count against the LDS wait counter.
+template <typename T>
+using Glob = __attribute__((address_space(1))) T;
+template <typename T>
+__device__ __forceinline__ Glob<T>* gptr(T* p) {
+    return (Glob<T>*)p;
+}
 
 // MI355X dispatches workgroup w of a launch to XCD w % 8, and each XCD has its
 // own L2.  xcd_remap(w, n) renumbers the n workgroups so that each XCD gets a

@@ -25,6 +25,27 @@ template <> struct CplxT<float2> {
     static __device__ __forceinline__ float2 make(float x, float y) { return make_float2(x, y); }
 };
 
+// complex values through global-address-space pointers (see gptr): via the
+// builtin vector type, since the HIP vector classes do not bind to
+// address-space-qualified references
+template <typename C> struct NativeT;
+template <> struct NativeT<double2> { using type = double __attribute__((ext_vector_type(2))); };
+template <> struct NativeT<float2> { using type = float __attribute__((ext_vector_type(2))); };
+template <typename C>
+__device__ __forceinline__ C gld(const Glob<C>* p) {
+    using N = typename NativeT<C>::type;
+    const N v = *(const Glob<N>*)p;
+    return CplxT<C>::make(v.x, v.y);
+}
+template <typename C>
+__device__ __forceinline__ void gst(Glob<C>* p, C v) {
+    using N = typename NativeT<C>::type;
+    N w;
+    w.x = v.x;
+    w.y = v.y;
+    *(Glob<N>*)p = w;
+}
+
 template <typename C>
 __device__ __forceinline__ C cadd(C a, C b) { return CplxT<C>::make(a.x + b.x, a.y + b.y); }
 template <typename C>
@@ -44,6 +65,127 @@ __device__ __forceinline__ C twiddle(C a, C w) {
 template <int SIGN, typename C>
 __device__ __forceinline__ C mul_si(C a) {
     return SIGN < 0 ? CplxT<C>::make(a.y, -a.x) : CplxT<C>::make(-a.y, a.x);
+}
+
+// ---------------------------------------------------------------------------
+// In-register DFTs of R points, R in {1,2,3,4,6,8,9,12,16}, in place with
+// natural-order input and output: X[u] = sum_t a[t] exp(SIGN 2 pi i t u / R).
+// Composite R = R1*R2 by one Cooley-Tukey step (t = t1*R2 + t2,
+// u = u1 + R1*u2): R2 DFTs of length R1, twiddles w_R^(t2*u1), R1 DFTs of
+// length R2.  The twiddle constants are cos/sin(2 pi k / R) rounded once
+// from 50-digit values (exactly symmetric).
+// ---------------------------------------------------------------------------
+constexpr double kCos6[6] = {1.000000000000000000000e+00, 5.000000000000000000000e-01, -5.000000000000000000000e-01, -1.000000000000000000000e+00, -5.000000000000000000000e-01, 5.000000000000000000000e-01};
+constexpr double kSin6[6] = {0.0, 8.660254037844385965883e-01, 8.660254037844385965883e-01, 0.0, -8.660254037844385965883e-01, -8.660254037844385965883e-01};
+constexpr double kCos8[8] = {1.000000000000000000000e+00, 7.071067811865475727373e-01, 0.0, -7.071067811865475727373e-01, -1.000000000000000000000e+00, -7.071067811865475727373e-01, 0.0, 7.071067811865475727373e-01};
+constexpr double kSin8[8] = {0.0, 7.071067811865475727373e-01, 1.000000000000000000000e+00, 7.071067811865475727373e-01, 0.0, -7.071067811865475727373e-01, -1.000000000000000000000e+00, -7.071067811865475727373e-01};
+constexpr double kCos9[9] = {1.000000000000000000000e+00, 7.660444431189780134517e-01, 1.736481776669303589422e-01, -5.000000000000000000000e-01, -9.396926207859084279050e-01, -9.396926207859084279050e-01, -5.000000000000000000000e-01, 1.736481776669303589422e-01, 7.660444431189780134517e-01};
+constexpr double kSin9[9] = {0.0, 6.427876096865393629187e-01, 9.848077530122080203157e-01, 8.660254037844385965883e-01, 3.420201433256687129081e-01, -3.420201433256687129081e-01, -8.660254037844385965883e-01, -9.848077530122080203157e-01, -6.427876096865393629187e-01};
+constexpr double kCos12[12] = {1.000000000000000000000e+00, 8.660254037844385965883e-01, 5.000000000000000000000e-01, 0.0, -5.000000000000000000000e-01, -8.660254037844385965883e-01, -1.000000000000000000000e+00, -8.660254037844385965883e-01, -5.000000000000000000000e-01, 0.0, 5.000000000000000000000e-01, 8.660254037844385965883e-01};
+constexpr double kSin12[12] = {0.0, 5.000000000000000000000e-01, 8.660254037844385965883e-01, 1.000000000000000000000e+00, 8.660254037844385965883e-01, 5.000000000000000000000e-01, 0.0, -5.000000000000000000000e-01, -8.660254037844385965883e-01, -1.000000000000000000000e+00, -8.660254037844385965883e-01, -5.000000000000000000000e-01};
+constexpr double kCos16[16] = {1.000000000000000000000e+00, 9.238795325112867384831e-01, 7.071067811865475727373e-01, 3.826834323650897817792e-01, 0.0, -3.826834323650897817792e-01, -7.071067811865475727373e-01, -9.238795325112867384831e-01, -1.000000000000000000000e+00, -9.238795325112867384831e-01, -7.071067811865475727373e-01, -3.826834323650897817792e-01, 0.0, 3.826834323650897817792e-01, 7.071067811865475727373e-01, 9.238795325112867384831e-01};
+constexpr double kSin16[16] = {0.0, 3.826834323650897817792e-01, 7.071067811865475727373e-01, 9.238795325112867384831e-01, 1.000000000000000000000e+00, 9.238795325112867384831e-01, 7.071067811865475727373e-01, 3.826834323650897817792e-01, 0.0, -3.826834323650897817792e-01, -7.071067811865475727373e-01, -9.238795325112867384831e-01, -1.000000000000000000000e+00, -9.238795325112867384831e-01, -7.071067811865475727373e-01, -3.826834323650897817792e-01};
+
+template <int R> struct SmallTw;
+template <> struct SmallTw<6> { static __device__ __forceinline__ double c(int k) { return kCos6[k]; } static __device__ __forceinline__ double s(int k) { return kSin6[k]; } };
+template <> struct SmallTw<8> { static __device__ __forceinline__ double c(int k) { return kCos8[k]; } static __device__ __forceinline__ double s(int k) { return kSin8[k]; } };
+template <> struct SmallTw<9> { static __device__ __forceinline__ double c(int k) { return kCos9[k]; } static __device__ __forceinline__ double s(int k) { return kSin9[k]; } };
+template <> struct SmallTw<12> { static __device__ __forceinline__ double c(int k) { return kCos12[k]; } static __device__ __forceinline__ double s(int k) { return kSin12[k]; } };
+template <> struct SmallTw<16> { static __device__ __forceinline__ double c(int k) { return kCos16[k]; } static __device__ __forceinline__ double s(int k) { return kSin16[k]; } };
+
+template <int SIGN, int R, typename C>
+__device__ __forceinline__ void dft_reg(C* a);
+
+template <int SIGN, int R1, int R2, typename C>
+__device__ __forceinline__ void dft_ct(C* a) {
+    using Rl = typename CplxT<C>::real;
+    constexpr int R = R1 * R2;
+    C b[R];
+#pragma unroll
+    for (int t2 = 0; t2 < R2; ++t2) {
+        C v[R1];
+#pragma unroll
+        for (int t1 = 0; t1 < R1; ++t1) v[t1] = a[t1 * R2 + t2];
+        dft_reg<SIGN, R1>(v);
+#pragma unroll
+        for (int u1 = 0; u1 < R1; ++u1) {
+            const int k = (t2 * u1) % R;
+            if (k == 0) {
+                b[u1 * R2 + t2] = v[u1];
+            } else {
+                const C w = CplxT<C>::make((Rl)SmallTw<R>::c(k), (Rl)(SIGN * SmallTw<R>::s(k)));
+                b[u1 * R2 + t2] = cmul(v[u1], w);
+            }
+        }
+    }
+#pragma unroll
+    for (int u1 = 0; u1 < R1; ++u1) {
+        C v[R2];
+#pragma unroll
+        for (int t2 = 0; t2 < R2; ++t2) v[t2] = b[u1 * R2 + t2];
+        dft_reg<SIGN, R2>(v);
+#pragma unroll
+        for (int u2 = 0; u2 < R2; ++u2) a[u1 + R1 * u2] = v[u2];
+    }
+}
+
+template <int SIGN, int R, typename C>
+__device__ __forceinline__ void dft_reg(C* a) {
+    using Rl = typename CplxT<C>::real;
+    if constexpr (R == 2) {
+        const C s = cadd(a[0], a[1]), d = csub(a[0], a[1]);
+        a[0] = s;
+        a[1] = d;
+    } else if constexpr (R == 3) {
+        const Rl s3 = (Rl)0.86602540378443864676372317075294;
+        const C t1 = cadd(a[1], a[2]);
+        const C t2 = CplxT<C>::make(a[0].x - (Rl)0.5 * t1.x, a[0].y - (Rl)0.5 * t1.y);
+        const C dd = csub(a[1], a[2]);
+        const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
+        a[0] = cadd(a[0], t1);
+        a[1] = cadd(t2, jd);
+        a[2] = csub(t2, jd);
+    } else if constexpr (R == 4) {
+        const C s02 = cadd(a[0], a[2]), d02 = csub(a[0], a[2]);
+        const C s13 = cadd(a[1], a[3]), d13 = csub(a[1], a[3]);
+        const C jd13 = mul_si<SIGN>(d13);
+        a[0] = cadd(s02, s13);
+        a[1] = cadd(d02, jd13);
+        a[2] = csub(s02, s13);
+        a[3] = csub(d02, jd13);
+    } else if constexpr (R == 6) {
+        dft_ct<SIGN, 2, 3>(a);
+    } else if constexpr (R == 8) {
+        dft_ct<SIGN, 4, 2>(a);
+    } else if constexpr (R == 9) {
+        dft_ct<SIGN, 3, 3>(a);
+    } else if constexpr (R == 12) {
+        dft_ct<SIGN, 4, 3>(a);
+    } else if constexpr (R == 16) {
+        dft_ct<SIGN, 4, 4>(a);
+    }
+}
+
+// runtime R (a plan constant, uniform) over a register array of RMAX entries
+template <int SIGN, int RMAX, typename C>
+__device__ __forceinline__ void dft_rt(C (&a)[RMAX], int R) {
+    switch (R) {
+        case 2: dft_reg<SIGN, 2>(a); break;
+        case 3: dft_reg<SIGN, 3>(a); break;
+        case 4: dft_reg<SIGN, 4>(a); break;
+        default:
+            if constexpr (RMAX >= 16) {
+                switch (R) {
+                    case 6: dft_reg<SIGN, 6>(a); break;
+                    case 8: dft_reg<SIGN, 8>(a); break;
+                    case 9: dft_reg<SIGN, 9>(a); break;
+                    case 12: dft_reg<SIGN, 12>(a); break;
+                    case 16: dft_reg<SIGN, 16>(a); break;
+                    default: break;
+                }
+            }
+            break;
+    }
 }
 
 // Sum over the block of one value per thread in a fixed order (butterflies

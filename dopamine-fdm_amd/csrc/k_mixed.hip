@@ -23,9 +23,11 @@ constexpr int kM00Chunk = 1024;
 
 __device__ __forceinline__ void mode00_body(const MPar& m) {
     __shared__ double s_in[kM00Chunk], s_mul[kM00Chunk], s_cu[kM00Chunk], s_out[kM00Chunk];
-    const double* __restrict__ rowpart = m.rowpart;
-    const double* __restrict__ lu = m.lu;
-    double* __restrict__ x = m.x;
+    const auto rowpart = gptr(m.rowpart);
+    const auto lu = gptr(m.lu);
+    const auto dl = gptr(m.o.dl);
+    const auto du = gptr(m.o.du);
+    const auto x = gptr(m.x);
     const int nym = m.nym, nparts = m.nparts;
     const long nzs = m.nzs;
     // forward: y(0) = mean(0); y(j) = mean(j) - DL(j-1)/D'(j-1) * y(j-1)
@@ -37,7 +39,7 @@ __device__ __forceinline__ void mode00_body(const MPar& m) {
             double s = 0.0;
             for (int q = 0; q < nparts; ++q) s += rowpart[(long)j * nparts + q];
             s_in[t] = s / m.count;
-            s_mul[t] = j > 0 ? m.o.dl[j - 1] * lu[(j - 1) * nzs] : 0.0;
+            s_mul[t] = j > 0 ? dl[j - 1] * lu[(j - 1) * nzs] : 0.0;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -58,7 +60,7 @@ __device__ __forceinline__ void mode00_body(const MPar& m) {
             const int j = c0 + t;
             s_in[t] = x[j];
             s_mul[t] = lu[j * nzs];
-            s_cu[t] = j < nym - 1 ? m.o.du[j] : 0.0;
+            s_cu[t] = j < nym - 1 ? du[j] : 0.0;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -76,10 +78,11 @@ __device__ __forceinline__ void mode00_body(const MPar& m) {
 __global__ void __launch_bounds__(256) k_mode00(MPar m) { mode00_body(m); }
 
 // cases c = blockIdx.x, blockIdx.x + gridDim.x, ...
-__global__ void __launch_bounds__(256) kb_mode00(const MPar* __restrict__ cs, const int* __restrict__ pnc) {
+__global__ void __launch_bounds__(256) kb_mode00(const MPar* __restrict__ cs, const int* __restrict__ idx,
+                                                 const int* __restrict__ pnc) {
     const int nc = *pnc;
     for (int c = blockIdx.x; c < nc; c += gridDim.x) {
-        mode00_body(cs[c]);
+        mode00_body(cs[idx[c]]);
         __syncthreads();
     }
 }
@@ -105,8 +108,8 @@ hipError_t launch_mode00(dfp_plan* p) {
     return hipGetLastError();
 }
 
-hipError_t launch_mode00_batch(hipStream_t s, const MPar* cs, const int* nc, int grid) {
-    hipLaunchKernelGGL(kb_mode00, dim3(grid), dim3(256), 0, s, cs, nc);
+hipError_t launch_mode00_batch(hipStream_t s, const MPar* cs, const int* idx, const int* nc, int grid) {
+    hipLaunchKernelGGL(kb_mode00, dim3(grid), dim3(256), 0, s, cs, idx, nc);
     return hipGetLastError();
 }
 
@@ -126,11 +129,11 @@ __device__ __forceinline__ void save_p_body(const SPar& sp, int bx) {
     const int ii = (i - 2 + g.nxp) % g.nxp;
     const int jj = j < 2 ? 0 : (j > g.nym + 1 ? g.nym - 1 : j - 2);
     const int kk = (k - 2 + g.nzp) % g.nzp;
-    sp.P[t] = sp.rhs[ii + jj * (long)(g.nxg - 1) + kk * (long)(g.nxg - 1) * g.nym] / sp.dtc;
+    gptr(sp.P)[t] = gptr(sp.rhs)[ii + jj * (long)(g.nxg - 1) + kk * (long)(g.nxg - 1) * g.nym] / sp.dtc;
 }
 
 __global__ void __launch_bounds__(256) kb_save_p(const SPar* __restrict__ cs, const int* __restrict__ off,
-                                                 const int* __restrict__ pnc) {
+                                                 const int* __restrict__ idx, const int* __restrict__ pnc) {
     const int nc = *pnc;
     const int total = off[nc];
     for (int w = blockIdx.x; w < total; w += gridDim.x) {
@@ -140,7 +143,7 @@ __global__ void __launch_bounds__(256) kb_save_p(const SPar* __restrict__ cs, co
             if (off[mid] <= w) lo = mid;
             else hi = mid;
         }
-        save_p_body(cs[lo], w - off[lo]);
+        save_p_body(cs[idx[lo]], w - off[lo]);
     }
 }
 
@@ -157,8 +160,9 @@ SPar make_spar(const dfp_plan* p, const double* rhs, double* P, double dtc) {
     return sp;
 }
 
-hipError_t launch_save_p_batch(hipStream_t s, const SPar* cs, const int* off, const int* nc, int grid) {
-    hipLaunchKernelGGL(kb_save_p, dim3(grid), dim3(256), 0, s, cs, off, nc);
+hipError_t launch_save_p_batch(hipStream_t s, const SPar* cs, const int* off, const int* idx, const int* nc,
+                               int grid) {
+    hipLaunchKernelGGL(kb_save_p, dim3(grid), dim3(256), 0, s, cs, off, idx, nc);
     return hipGetLastError();
 }
 

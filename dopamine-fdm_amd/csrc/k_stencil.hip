@@ -56,10 +56,12 @@ constexpr int kCorrectK = 8;
 __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
 #pragma clang fp contract(off)
     const Geo& g = c.g;
-    double* __restrict__ U = c.U;
-    double* __restrict__ V = c.V;
-    double* __restrict__ W = c.W;
-    const double* __restrict__ rhs = c.rhs;
+    const auto U = gptr(c.U);
+    const auto V = gptr(c.V);
+    const auto W = gptr(c.W);
+    const auto rhs = gptr(c.rhs);
+    const auto inv_dyg = gptr(g.inv_dyg);
+    const auto inv_dzg = gptr(g.inv_dzg);
     const int ni = g.nxg - 2;
     const int t = bx * 256 + threadIdx.x;
     if (t >= ni * g.nym) return;
@@ -70,7 +72,7 @@ __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
     const long pr_k = (long)(g.nxg - 1) * g.nym;
     const long pu_k = (long)g.nx * g.nyg, pv_k = (long)g.nxg * g.ny, pw_k = (long)g.nxg * g.nyg;
     const bool do_u = i <= g.nx - 1, do_v = j <= g.ny - 1;
-    const double idyg = do_v ? g.inv_dyg[j - 2] : 0.0;
+    const double idyg = do_v ? inv_dyg[j - 2] : 0.0;
     long r = ridx(g, i, j, k0);
     long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k0 - 1) * pu_k;
     long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k0 - 1) * pv_k;
@@ -85,7 +87,7 @@ __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
         const double phin = do_w ? rhs[r + pr_k] : 0.0;
         if (do_u) U[iu] = U[iu] - (rhs[r + 1] - phi) * g.inv_dx;
         if (do_v) V[iv] = V[iv] - (rhs[r + (g.nxg - 1)] - phi) * idyg;
-        if (do_w) W[iw] = W[iw] - (phin - phi) * g.inv_dzg[k - 2];
+        if (do_w) W[iw] = W[iw] - (phin - phi) * inv_dzg[k - 2];
         phi = do_w ? phin : phi;
         r += pr_k;
         iu += pu_k;
@@ -102,7 +104,7 @@ __global__ void __launch_bounds__(256) k_correct(CPar c) {
 }
 
 __global__ void __launch_bounds__(256) kb_correct(const CPar* __restrict__ cs, const int* __restrict__ off,
-                                                  const int* __restrict__ pnc) {
+                                                  const int* __restrict__ idx, const int* __restrict__ pnc) {
     const int nc = *pnc;
     const int total = off[nc];
     const int me = xcd_remap(blockIdx.x, gridDim.x);
@@ -115,7 +117,7 @@ __global__ void __launch_bounds__(256) kb_correct(const CPar* __restrict__ cs, c
             if (off[mid] <= w) lo = mid;
             else hi = mid;
         }
-        const CPar& c = cs[lo];
+        const CPar& c = cs[idx[lo]];
         const int li = w - off[lo];
         const int bz = li / c.nbx;
         correct_body(c, li - bz * c.nbx, bz);
@@ -142,8 +144,9 @@ hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const do
     return hipGetLastError();
 }
 
-hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* nc, int grid) {
-    hipLaunchKernelGGL(kb_correct, dim3(grid), dim3(256), 0, s, cs, off, nc);
+hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* idx, const int* nc,
+                                int grid) {
+    hipLaunchKernelGGL(kb_correct, dim3(grid), dim3(256), 0, s, cs, off, idx, nc);
     return hipGetLastError();
 }
 

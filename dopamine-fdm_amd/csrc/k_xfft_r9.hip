@@ -1,0 +1,4 @@
+// x-direction spectral kernels with a radix-9 first z stage
+#include "k_xfft_impl.h"
+
+DFP_INSTANTIATE_XFFT(9)

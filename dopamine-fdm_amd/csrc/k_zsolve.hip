@@ -68,7 +68,7 @@ constexpr int kZsAhead = 2;  // rows read ahead in the elimination sweeps
 #define ZS_LOAD1(K)                                                               \
     {                                                                             \
         const int e = min((int)threadIdx.x + (K) * kZsThreads, lim_);             \
-        pr##K = src_[e];                                                          \
+        pr##K = gld<C>(src_ + e);                                                 \
         const int r = (int)div_n.div((uint32_t)e);                                \
         const int mm = e - r * n;                                                 \
         const int kz = R * mm + u;                                                \
@@ -76,8 +76,8 @@ constexpr int kZsAhead = 2;  // rows read ahead in the elimination sweeps
     }
 #define ZS_PREFETCH(J0, GQ, COEF, COFF)                                           \
     do {                                                                          \
-        const C* src_ = plane + (long)(J0) * n;                                   \
-        const double* lsrc_ = lup + (long)(J0) * nzs;                             \
+        const auto src_ = plane + (long)(J0) * n;                                 \
+        const auto lsrc_ = lup + (long)(J0) * nzs;                                \
         const int lim_ = (GQ) * n - 1;                                            \
         ZS_LOAD1(0) ZS_LOAD1(1) ZS_LOAD1(2) ZS_LOAD1(3)                           \
         ZS_LOAD1(4) ZS_LOAD1(5) ZS_LOAD1(6) ZS_LOAD1(7)                           \
@@ -114,7 +114,7 @@ __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned cha
     const int n = zp.fz.n;
     const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G, ntw = zp.ntw;
     const FastDiv div_n = zp.div_n;
-    const double* __restrict__ dl = zp.dl;
+    const auto dl = gptr(zp.dl);
     const int nzs = nzp / 2 + 1;
     const int kx = b / R;
     const int u = b - kx * R;
@@ -122,9 +122,9 @@ __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned cha
     C* twl = A + G * n;
     double* lus = reinterpret_cast<double*>(twl + ntw);
     double* dus = lus + G * n;
-    C* plane = zp.spec + (long)b * nym * n;
-    const double* lup = zp.lu + (long)kx * nym * nzs;
-    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = zp.tw[e];
+    const auto plane = gptr(zp.spec) + (long)b * nym * n;
+    const auto lup = gptr(zp.lu) + (long)kx * nym * nzs;
+    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = gld<C>(gptr(zp.tw) + e);
     ZS_REGS;
     ZS_PREFETCH(0, min(G, nym), dl, -1);
     double bpx = 0.0, bpy = 0.0, ipp = 0.0;
@@ -175,8 +175,8 @@ __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned cha
             }
         }
         __syncthreads();
-        C* dst = plane + (long)j0 * n;
-        for (int e = threadIdx.x; e < gr * n; e += kZsThreads) dst[e] = A[e];
+        const auto dst = plane + (long)j0 * n;
+        for (int e = threadIdx.x; e < gr * n; e += kZsThreads) gst<C>(dst + e, A[e]);
     }
 }
 
@@ -189,7 +189,7 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
     const int n = zp.fz.n;
     const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G, ntw = zp.ntw;
     const FastDiv div_n = zp.div_n;
-    const double* __restrict__ du = zp.du;
+    const auto du = gptr(zp.du);
     const int nzs = nzp / 2 + 1;
     const int kx = b / R;
     const int u = b - kx * R;
@@ -197,9 +197,9 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
     C* twl = A + G * n;
     double* lus = reinterpret_cast<double*>(twl + ntw);
     double* dus = lus + G * n;
-    C* plane = zp.spec + (long)b * nym * n;
-    const double* lup = zp.lu + (long)kx * nym * nzs;
-    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = zp.tw[e];
+    const auto plane = gptr(zp.spec) + (long)b * nym * n;
+    const auto lup = gptr(zp.lu) + (long)kx * nym * nzs;
+    for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = gld<C>(gptr(zp.tw) + e);
     // groups from the top: [j0, j0+gr), j0 = max(0, nym - (gi+1)*G)
     const int ngroups = (nym + G - 1) / G;
     ZS_REGS;
@@ -264,8 +264,8 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
         }
         __syncthreads();
         lds_fft_inplace<1, 2>(A, gr, n, zp.fz, twl);
-        C* dst = plane + (long)j0 * n;
-        for (int e = threadIdx.x; e < gr * n; e += kZsThreads) dst[e] = A[e];
+        const auto dst = plane + (long)j0 * n;
+        for (int e = threadIdx.x; e < gr * n; e += kZsThreads) gst<C>(dst + e, A[e]);
     }
 }
 
@@ -293,7 +293,7 @@ __device__ __forceinline__ int zfind_case(const int* __restrict__ off, int nc, i
 
 template <typename C, int DIR>
 __global__ void __launch_bounds__(kZsThreads) kb_zsolve(const ZPar<C>* __restrict__ cs, const int* __restrict__ off,
-                                                        const int* __restrict__ pnc) {
+                                                        const int* __restrict__ idx, const int* __restrict__ pnc) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int nc = *pnc;
     const int total = off[nc];
@@ -302,8 +302,8 @@ __global__ void __launch_bounds__(kZsThreads) kb_zsolve(const ZPar<C>* __restric
         const int w = base + me;
         if (w >= total) break;
         const int c = zfind_case(off, nc, w);
-        if (DIR < 0) zfwd_body<C>(cs[c], w - off[c], smem_raw);
-        else zbwd_body<C>(cs[c], w - off[c], smem_raw);
+        if (DIR < 0) zfwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
+        else zbwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
         __syncthreads();
     }
 }
@@ -353,16 +353,16 @@ hipError_t launch_zsolve(dfp_plan* p, int dir) { return launch_zsolve_t<double2>
 hipError_t launch_zsolve32(dfp_plan* p, int dir) { return launch_zsolve_t<float2>(p, dir, make_zpar<float2>(p, 1)); }
 
 template <typename C>
-hipError_t launch_zsolve_batch(hipStream_t s, int dir, const ZPar<C>* cs, const int* off, const int* nc, int grid,
-                               size_t lds) {
-    if (dir < 0) hipLaunchKernelGGL((kb_zsolve<C, -1>), dim3(grid), dim3(kZsThreads), lds, s, cs, off, nc);
-    else hipLaunchKernelGGL((kb_zsolve<C, 1>), dim3(grid), dim3(kZsThreads), lds, s, cs, off, nc);
+hipError_t launch_zsolve_batch(hipStream_t s, int dir, const ZPar<C>* cs, const int* off, const int* idx,
+                               const int* nc, int grid, size_t lds) {
+    if (dir < 0) hipLaunchKernelGGL((kb_zsolve<C, -1>), dim3(grid), dim3(kZsThreads), lds, s, cs, off, idx, nc);
+    else hipLaunchKernelGGL((kb_zsolve<C, 1>), dim3(grid), dim3(kZsThreads), lds, s, cs, off, idx, nc);
     return hipGetLastError();
 }
-template hipError_t launch_zsolve_batch<double2>(hipStream_t, int, const ZPar<double2>*, const int*, const int*, int,
-                                                 size_t);
-template hipError_t launch_zsolve_batch<float2>(hipStream_t, int, const ZPar<float2>*, const int*, const int*, int,
-                                                size_t);
+template hipError_t launch_zsolve_batch<double2>(hipStream_t, int, const ZPar<double2>*, const int*, const int*,
+                                                 const int*, int, size_t);
+template hipError_t launch_zsolve_batch<float2>(hipStream_t, int, const ZPar<float2>*, const int*, const int*,
+                                                const int*, int, size_t);
 
 // rows per group: the group fits the prefetch registers (kZsPf points per
 // thread), the in-place FFT capacity (2 butterflies per thread) and, with the

@@ -27,7 +27,8 @@ class DfpGrid(ctypes.Structure):
 
 
 class DfpTuning(ctypes.Structure):
-    _fields_ = [("z_split", ctypes.c_int32), ("x_columns", ctypes.c_int32), ("z_rows", ctypes.c_int32)]
+    _fields_ = [("z_split", ctypes.c_int32), ("x_columns", ctypes.c_int32), ("z_rows", ctypes.c_int32),
+                ("z_fused", ctypes.c_int32)]
 
 
 class DfpError(RuntimeError):

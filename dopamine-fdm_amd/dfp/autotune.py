@@ -2,12 +2,12 @@
 the batched launches, cached in a JSON table.
 
 For one grid it times, with HIP events on resident fields, every valid
-(z_split, x_columns, z_rows) of ``dfp_plan_create_tuned`` in a bounded
-candidate set and keeps the fastest; for a batch it times the persistent
+(z_split, x_columns, z_rows, z_fused) of ``dfp_plan_create_tuned`` in a
+bounded candidate set and keeps the fastest; for a batch it times the persistent
 launches at a few workgroups-per-CU settings.  The table maps
 "nxp x nym x nzp:fp64|fp32" to the chosen shape and its time:
 
-    {"device": "...", "plans": {"64x64x64:fp64": {"tuning": [4, 16, 8], "us": 41.2}, ...},
+    {"device": "...", "plans": {"64x64x64:fp64": {"tuning": [4, 16, 8, 1], "us": 41.2}, ...},
      "batch": {"occupancy": [4, 4, 8], "us": ...}}
 
 The reference has no counterpart (its FFT and LAPACK libraries pick their
@@ -52,7 +52,7 @@ def save_table(table: dict, path: str = DEFAULT_TABLE):
 
 
 def lookup(table: dict, g: ChannelGrid, precision: int = DFP_FP64):
-    """Tuned (z_split, x_columns, z_rows) of a grid, or None."""
+    """Tuned (z_split, x_columns, z_rows, z_fused) of a grid, or None."""
     e = table.get("plans", {}).get(size_key(g.nxp, g.nym, g.nzp, precision))
     return tuple(e["tuning"]) if e else None
 
@@ -68,20 +68,27 @@ def _divisors(n: int) -> list[int]:
     return [d for d in range(1, n + 1) if n % d == 0]
 
 
-def candidates(g: ChannelGrid, max_points: int = 4096) -> list[tuple[int, int, int]]:
-    """Shapes worth timing: every z split, column counts whose x workgroup holds
-    between 256 and max_points points, and z row groups of full and half size
-    (0 = the library's maximum for that split)."""
+RADICES = (1, 2, 3, 4, 6, 8, 9, 12, 16)
+
+
+def candidates(g: ChannelGrid, max_points: int = 4096) -> list[tuple[int, int, int, int]]:
+    """Shapes worth timing, as (z_split, x_columns, z_rows, z_fused): every
+    z split, column counts whose x workgroup holds between 256 and
+    max_points points, and for each the fused z kernel and the split pair
+    with full and half row groups (z_rows 0 = the library's maximum, -1 =
+    half of it, resolved in tune_plan).  Shapes the device cannot hold are
+    rejected by the library and skipped."""
     out = []
     M = g.nxp // 2
-    for R in (1, 2, 3, 4):
+    for R in RADICES:
         if g.nzp % R or g.nzp // R > 256:
             continue
         nz2 = g.nzp // R
         ts = [t for t in _divisors(nz2) if 256 <= R * t * M <= max_points] or [1]
         for T in ts:
-            out.append((R, T, 0))
-            out.append((R, T, -1))  # half the maximum row group (resolved in tune_plan)
+            out.append((R, T, 0, 2))
+            out.append((R, T, 0, 1))
+            out.append((R, T, -1, 1))
     return out
 
 
@@ -119,16 +126,16 @@ def tune_plan(g: ChannelGrid, precision: int = DFP_FP64, reps: int = 10, device:
     base.close()
     tried = 0
     seen = set()
-    for R, T, G in candidates(g):
+    for R, T, G, F in candidates(g):
         try:
             if G == -1:
-                probe = Plan(g, device=device, precision=precision, tuning=(R, T, 0))
+                probe = Plan(g, device=device, precision=precision, tuning=(R, T, 0, F))
                 gmax = probe.tuning[2]
                 probe.close()
                 if gmax < 2:
                     continue
                 G = gmax // 2
-            p = Plan(g, device=device, precision=precision, tuning=(R, T, G))
+            p = Plan(g, device=device, precision=precision, tuning=(R, T, G, F))
         except DfpError:
             continue
         shape = p.tuning

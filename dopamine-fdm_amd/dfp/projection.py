@@ -70,7 +70,7 @@ class Plan:
 
     def __init__(self, g: ChannelGrid, device: int = 0, precision: int = DFP_FP64, sweeps: int | None = None,
                  tuning: tuple | None = None):
-        """tuning: (z_split, x_columns, z_rows) launch shapes, 0 = the library's choice."""
+        """tuning: (z_split, x_columns, z_rows[, z_fused]) launch shapes, 0 = the library's choice."""
         self.g = g
         self.precision = precision
         self.device = device
@@ -124,10 +124,11 @@ class Plan:
 
     @property
     def tuning(self) -> tuple:
-        """(z_split, x_columns, z_rows) the plan runs with."""
+        """(z_split, x_columns, z_rows, z_fused) the plan runs with (z_fused 2: one
+        fused z kernel, 1: the split pair)."""
         t = DfpTuning()
         check(lib().dfp_plan_tuning(self._h, ctypes.byref(t)), "dfp_plan_tuning", self._h)
-        return (t.z_split, t.x_columns, t.z_rows)
+        return (t.z_split, t.x_columns, t.z_rows, t.z_fused)
 
     def set_stream(self, stream_handle: int | None):
         check(lib().dfp_set_stream(self._h, stream_handle), "dfp_set_stream", self._h)

@@ -78,16 +78,20 @@ int dfp_plan_destroy(dfp_plan *plan);
 
 /* Launch shapes of a plan (no counterpart in the reference, whose FFT and
  * tridiagonal libraries choose their own).  0 = the library's choice.
- *   z_split    radix R in {1,2,3,4} of the first z-FFT stage, applied by the
- *              x kernels (R must divide nzp; the z kernels then transform
- *              rows of nzp/R points, one mode per thread, nzp/R <= 256)
+ *   z_split    radix R in {1,2,3,4,6,8,9,12,16} of the first z-FFT stage,
+ *              applied by the x kernels (R must divide nzp; the z kernels
+ *              then transform rows of nzp/R <= 256 points, one mode per lane)
  *   x_columns  z columns per x-kernel workgroup (divides nzp/R)
- *   z_rows     y rows per group of the fused z-FFT / y-solve kernels
- * Results do not depend on x_columns or z_rows; z_split changes the FFT's
- * factorisation (round-off level).  DFP_E_ARG for values out of range,
- * DFP_E_UNSUPPORTED for shapes that do not fit the device's LDS. */
+ *   z_rows     y rows per group of the split z kernels (z_fused = 1)
+ *   z_fused    1: split pair of z kernels (FFT + elimination, back
+ *              substitution + inverse FFT, spectrum read and written twice);
+ *              2: one fused kernel holding a whole (kx, z-split) sub-plane
+ *              of nym x nzp/R modes in LDS (read and written once)
+ * Results do not depend on x_columns, z_rows or z_fused; z_split changes
+ * the FFT's factorisation (round-off level).  DFP_E_ARG for values out of
+ * range, DFP_E_UNSUPPORTED for shapes that do not fit the device's LDS. */
 typedef struct dfp_tuning {
-    int32_t z_split, x_columns, z_rows;
+    int32_t z_split, x_columns, z_rows, z_fused;
 } dfp_tuning;
 int dfp_plan_create_tuned(const dfp_grid *g, int precision, int device, const dfp_tuning *tune,
                           dfp_plan **out);

@@ -110,26 +110,55 @@ def test_batch_rejects_shared_plans_and_host_fields():
         b.project([Case(plan, host[0], host[1], host[2], host[3])])
 
 
-@pytest.mark.parametrize("tuning", [(4, 1, 0), (2, 4, 3), (1, 2, 1), (4, 8, 2)])
+@pytest.mark.parametrize("tuning", [(4, 1, 0, 1), (2, 4, 3, 1), (1, 2, 1, 1), (4, 8, 2, 1), (4, 2, 0, 2),
+                                    (8, 1, 0, 2), (16, 1, 0, 2), (16, 1, 0, 1), (2, 4, 0, 2)])
 def test_tuned_plans_match_oracle(tuning):
     g = make_grid(64, 40, 64, 2.0)
     U, V, W, P, rhs = fields(g, 3, 0.1)
     o = [a.copy(order="F") for a in (U, V, W, P, rhs)]
     oracle_project(g, *o, DT, C3, save_p=True)
     plan = dfp.Plan(g, tuning=tuning)
-    assert plan.tuning == tuning if tuning[2] else plan.tuning[:2] == tuning[:2]
+    got = plan.tuning
+    assert got[:2] == tuning[:2] and got[3] == tuning[3]
+    if tuning[2]:
+        assert got[2] == tuning[2]
     d = [a.copy(order="F") for a in (U, V, W, P, rhs)]
     plan.project(d[0], d[1], d[2], d[4], d[3], DT, C3, save_p=True)
     for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
         assert rel_linf(a, b) <= 1e-12, lab
 
 
-def test_columns_and_row_groups_do_not_change_results():
+def test_columns_row_groups_and_fusion_do_not_change_results():
     g = make_grid(64, 40, 64, 2.0)
     host = fields(g, 4, 0.1)
     host = (host[0], host[1], host[2], host[4], host[3])
-    ref = _single(dfp.Plan(g, tuning=(4, 1, 1)), host, True)
-    for t in ((4, 2, 5), (4, 16, 0), (4, 4, 2)):
+    ref = _single(dfp.Plan(g, tuning=(4, 1, 1, 1)), host, True)
+    for t in ((4, 2, 5, 1), (4, 16, 0, 1), (4, 4, 2, 1), (4, 2, 0, 2), (4, 16, 0, 2)):
         out = _single(dfp.Plan(g, tuning=t), host, True)
         for a, r in zip(out, ref):
             assert np.array_equal(a, r), t
+
+
+def test_graph_replay_with_changing_list_lengths():
+    """Calls whose launch sequences match (same stages, radices and LDS sizes)
+    share one captured graph while their case lists differ in length; every
+    replay must read the lists of its own call."""
+    g = make_grid(32, 16, 32, 1.5)
+    print("graph replays over lists of 1, 3, 2 cases of one shape", flush=True)
+    plans, devs, hosts = [], [], []
+    for i in range(3):
+        plans.append(dfp.Plan(g))
+        dev, host = _resident(g, 40 + i)
+        devs.append(dev)
+        hosts.append(host)
+    b = Batch(max_cases=4, graphs=True)
+    for sel in ([0], [0, 1, 2], [2, 1]):
+        for i in sel:
+            for a, h in zip(devs[i], hosts[i]):
+                a.upload(h)
+        b.project([Case(plans[i], *devs[i][:4], devs[i][4], DT, C3, True) for i in sel])
+        for i in sel:
+            ref = _single(plans[i], hosts[i], True)
+            for a, r in zip(devs[i], ref):
+                assert np.array_equal(a.download(), r), f"cases {sel}, case {i}"
+    b.close()

@@ -48,20 +48,21 @@ def test_autotune_candidates_are_valid_shapes():
     g = dfp.channel_grid(64, 32, 96, 1.5)
     cs = autotune.candidates(g)
     assert cs
-    for R, T, G in cs:
+    for R, T, G, F in cs:
         assert g.nzp % R == 0 and g.nzp // R <= 256
         assert (g.nzp // R) % T == 0
-        assert G in (0, -1)
-    assert {R for R, _, _ in cs} == {1, 2, 3, 4}
+        assert G in (0, -1) and F in (1, 2)
+        assert not (F == 2 and G != 0)
+    assert {R for R, *_ in cs} == {1, 2, 3, 4, 6, 8, 12, 16}
 
 
 def test_autotune_table_roundtrip(tmp_path):
     p = tmp_path / "t.json"
-    t = {"plans": {autotune.size_key(64, 32, 64, dfp.DFP_FP64): {"tuning": [4, 2, 8], "us": 10.0}}, "batch": {}}
+    t = {"plans": {autotune.size_key(64, 32, 64, dfp.DFP_FP64): {"tuning": [4, 2, 8, 1], "us": 10.0}}, "batch": {}}
     autotune.save_table(t, str(p))
     t2 = autotune.load_table(str(p))
     g = dfp.channel_grid(64, 32, 64, 1.0)
-    assert autotune.lookup(t2, g, dfp.DFP_FP64) == (4, 2, 8)
+    assert autotune.lookup(t2, g, dfp.DFP_FP64) == (4, 2, 8, 1)
     assert autotune.lookup(t2, g, dfp.DFP_FP32_REFINE) is None
     assert autotune.load_table(str(tmp_path / "missing.json")) == {"plans": {}, "batch": {}}
 
