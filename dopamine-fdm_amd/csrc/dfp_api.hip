@@ -125,7 +125,6 @@ struct Shape {
 };
 
 constexpr int kRadices[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
-constexpr size_t kFusedSweetLds = 56 * 1024;  // three fused workgroups per CU
 
 // x columns per workgroup: the largest divisor of nz2 within about 2048
 // points (or the caller's), if the x kernels fit; 0 if none does
@@ -176,13 +175,12 @@ int choose_shape(const dfp_grid* g, int f32, size_t lds_limit, const dfp_tuning*
             if (q.G >= 1) split.push_back(q);
         }
     }
-    // fused where its plane leaves room for three workgroups per CU (smallest
-    // such R), else the split kernels with the classic split (R = 4 when it
-    // leaves at least 8 points per row, else 2, else 3, else none), else the
-    // fused form at any LDS size
-    auto zlds = [&](const Shape& sh) { return zf_lds_bytes(g->nym, sh.fz.n, twiddle_count(sh.fz), f32); };
-    for (const Shape& f : fused)
-        if (zlds(f) <= kFusedSweetLds || want_f == 2) { out = f; return DFP_OK; }
+    // default: the split pair with the classic split (R = 4 when it leaves at
+    // least 8 points per row, else 2, else 3, else none); the fused form when
+    // asked for (or when the split pair cannot run).  Measured at
+    // 256x192x256 the split pair is faster: the fused kernel's sweeps run on
+    // nzp/R lanes per workgroup only.
+    if (want_f == 2 && !fused.empty()) { out = fused[0]; return DFP_OK; }
     if (!split.empty()) {
         if (want_R > 0) { out = split[0]; return DFP_OK; }
         const int nzp = g->nzp;
