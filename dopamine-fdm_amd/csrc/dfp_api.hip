@@ -409,6 +409,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     p->tk_x = sh.T;
     p->g_zs = sh.G;
     p->zfused = sh.fused;
+    if (!sh.fused) z_two_pass(sh.fz.n, p->zn1, p->zn2);
     if (hipSetDevice(device) != hipSuccess) return bail(DFP_E_HIP);
     Geo& G = p->geo;
     G.nxp = g->nxp; G.nym = g->nym; G.nzp = g->nzp; G.nxh = g->nxp / 2 + 1;
@@ -472,6 +473,8 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     stage_twiddles(p->fz, tw);
     tw.push_back(make_double2(1.0, 0.0));
     for (int st = 0; st < p->fz.nst; ++st) p->fz.twoff[st] -= (int)o_tz;
+    const size_t o_t2 = tw.size();
+    for (int e = 0; e < p->fz.n; ++e) tw.push_back(expi(e, p->fz.n));
     if (hipMalloc(&p->d_tw, sizeof(double2) * tw.size()) != hipSuccess) return bail(DFP_E_NOMEM);
     if (hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(DFP_E_HIP);
@@ -479,6 +482,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     p->tw_z = p->d_tw + o_tz;
     p->tw_r2c = p->d_tw + o_r2c;
     p->tw_zs = p->d_tw + o_zs;
+    p->tw_z2 = p->d_tw + o_t2;
     if (precision == DFP_FP32_REFINE) {
         std::vector<float2> tw32(tw.size());
         for (size_t e = 0; e < tw.size(); ++e) tw32[e] = make_float2((float)tw[e].x, (float)tw[e].y);
@@ -487,6 +491,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
             return bail(DFP_E_HIP);
         p->tw_x32 = p->d_tw32;
         p->tw_z32 = p->d_tw32 + o_tz;
+        p->tw_z2_32 = p->d_tw32 + o_t2;
     }
 
     const size_t H = (size_t)G.nxh * G.nym * G.nzp;

@@ -49,7 +49,7 @@ struct Layout {
     size_t total;
 };
 
-constexpr int kMaxLists = 192;  // case lists per call (each launch reads one)
+constexpr int kMaxLists = 512;  // case lists per call (each launch reads one)
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -362,18 +362,26 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
     };
     auto z_launches = [&](int f32, size_t tab, const std::vector<int>& recs, const std::vector<int>& Fs,
                           const std::vector<size_t>& w) {
-        std::vector<int> fz, sz;
-        size_t lf = 0, lsz = 0;
+        // split-pair cases grouped by kernel variant (Launch::R holds it)
+        std::vector<int> fz;
+        std::vector<std::pair<int, std::vector<int>>> sz;
+        std::vector<size_t> lsz;
+        size_t lf = 0;
         for (int a : recs) {
             dfp_plan* p = plans[f32 ? i32[a] : i64[a]];
-            if (Fs[a]) { fz.push_back(a); lf = std::max(lf, zf_lds_bytes(p, f32)); }
-            else { sz.push_back(a); lsz = std::max(lsz, z_lds_bytes(p, f32)); }
+            if (Fs[a]) { fz.push_back(a); lf = std::max(lf, zf_lds_bytes(p, f32)); continue; }
+            const int v = z_variant(p);
+            size_t k = 0;
+            while (k < sz.size() && sz[k].first != v) ++k;
+            if (k == sz.size()) { sz.push_back({v, {}}); lsz.push_back(0); }
+            sz[k].second.push_back(a);
+            lsz[k] = std::max(lsz[k], z_lds_bytes(p, f32));
         }
         if (!fz.empty()) add(Launch::ZFUSED, f32, 0, 0, tab, make_list(fz, w), lf);
-        if (!sz.empty()) {
-            const int li = make_list(sz, w);
-            add(Launch::ZF, f32, 0, 0, tab, li, lsz);
-            add(Launch::ZB, f32, 0, 0, tab, li, lsz);
+        for (size_t k = 0; k < sz.size(); ++k) {
+            const int li = make_list(sz[k].second, w);
+            add(Launch::ZF, f32, 0, sz[k].first, tab, li, lsz[k]);
+            add(Launch::ZB, f32, 0, sz[k].first, tab, li, lsz[k]);
         }
     };
     auto first = [](int k) { std::vector<int> v(k); for (int a = 0; a < k; ++a) v[a] = a; return v; };
@@ -434,10 +442,10 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
                 case Launch::ZF:
                 case Launch::ZB: {
                     const int dir = l.kind == Launch::ZF ? -1 : 1;
-                    e = l.f32 ? launch_zsolve_batch(st, dir, reinterpret_cast<const ZPar<float2>*>(D + l.tab), o, ix,
-                                                    nc, gz, l.lds)
-                              : launch_zsolve_batch(st, dir, reinterpret_cast<const ZPar<double2>*>(D + l.tab), o, ix,
-                                                    nc, gz, l.lds);
+                    e = l.f32 ? launch_zsolve_batch(st, dir, l.R, reinterpret_cast<const ZPar<float2>*>(D + l.tab), o,
+                                                    ix, nc, gz, l.lds)
+                              : launch_zsolve_batch(st, dir, l.R, reinterpret_cast<const ZPar<double2>*>(D + l.tab), o,
+                                                    ix, nc, gz, l.lds);
                     break;
                 }
                 case Launch::ZFUSED:

@@ -120,7 +120,9 @@ struct ZPar {
     const double* du;
     FftDesc fz;          // z FFT of length nzp/R
     const C* tw;
+    const C* tw2;        // w_n^e, e < n, of the two-pass transforms
     int nym, nzp, R, ntw, G, zero00, nblocks;
+    int n1, n2;          // two-pass factorization n = n1*n2 (0: staged transform)
     FastDiv div_n;
 };
 
@@ -174,11 +176,13 @@ struct dfp_plan {
     int zr = 1;                  // radix of the z stage applied by the x kernels (z split)
     const double2* tw_x = nullptr;
     const double2* tw_z = nullptr;
+    const double2* tw_z2 = nullptr;   // exp(-2 pi i e / n), e < n = nzp/zr (two-pass z transforms)
     const double2* tw_r2c = nullptr;  // exp(-2 pi i k / nxp), k = 0..nxp/2
     const double2* tw_zs = nullptr;   // exp(-2 pi i k' u / nzp), [k'*(zr-1) + u-1], k' < nzp/zr, 1 <= u < zr
     float2* d_tw32 = nullptr;         // the same table rounded to FP32 (FP32 form)
     const float2* tw_x32 = nullptr;
     const float2* tw_z32 = nullptr;
+    const float2* tw_z2_32 = nullptr;
     // device work arrays
     double2* d_spec = nullptr;   // (nxp/2+1) x nym x nzp complex, layout [kx][u][j][m], kz = zr*m + u
     double* d_lu = nullptr;      // inverse pivots of the y solve, [kx][j][min(kz,nzp-kz)]
@@ -204,6 +208,7 @@ struct dfp_plan {
     int tk_x = 8;   // z columns per block of the x-transform kernels (zr lines each)
     int g_zs = 4;   // rows per group of the split z kernels, or rows per FFT call of the fused one
     int zfused = 0; // 1: one fused z kernel (whole sub-plane in LDS); 0: the split pair
+    int zn1 = 0, zn2 = 0;  // two-pass z transforms of the split pair (0: staged transform)
     size_t lds_limit = 65536;  // LDS bytes per block on this device
 };
 
@@ -216,6 +221,9 @@ hipError_t launch_lu_table(dfp_plan* p);
 // dir -1: z FFT + forward elimination, +1: back substitution + inverse z FFT
 hipError_t launch_zsolve(dfp_plan* p, int dir);
 int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit);
+bool z_two_pass(int n, int& n1, int& n2);
+// z kernel variant of a plan: -1 fused, else n1*32 + n2 of the split pair (0: staged transform)
+int z_variant(const dfp_plan* p);
 hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p);
 // fused z transform + y solve (whole sub-plane in LDS)
 template <typename C> hipError_t launch_zfused(dfp_plan* p, const ZPar<C>& zp);
@@ -254,7 +262,7 @@ template <typename C>
 hipError_t launch_xinv_batch(hipStream_t s, int R, int accum, const XPar<C>* cs, const int* off, const int* idx,
                              const int* nc, int grid, size_t lds);
 template <typename C>
-hipError_t launch_zsolve_batch(hipStream_t s, int dir, const ZPar<C>* cs, const int* off, const int* idx,
+hipError_t launch_zsolve_batch(hipStream_t s, int dir, int variant, const ZPar<C>* cs, const int* off, const int* idx,
                                const int* nc, int grid, size_t lds);
 hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* idx, const int* nc,
                                 int grid);

@@ -9,17 +9,14 @@
 // plan picks a z-split radix R that makes the plane fit (R = 16 at
 // 256x192x256: 49 KB, three workgroups per CU).
 //
-// The arithmetic is that of k_zsolve_fwd/bwd operation for operation (the
-// eliminated rows b' are held in LDS at the spectrum's precision, as the
-// split kernels hold them in HBM), so for the same R both paths agree bit
-// for bit.
+// The y solves run chunk-parallel over all lanes of the workgroup (see the
+// sweep below), so they round differently from the split kernels' serial
+// sweeps: the two paths agree to round-off, not bit for bit.
 #include "fft_lds.h"
 
 namespace dfp {
 
 constexpr int kZfThreads = 256;
-constexpr int kZfAhead = 8;  // pivot rows read ahead in the sweeps (global memory)
-constexpr int kZfLds = 2;    // plane rows and coefficients read ahead (LDS); divides kZfAhead
 
 // rows of the plane through the in-place LDS FFT, zp.G rows per call (the
 // per-call capacity of two butterflies per thread)
@@ -42,6 +39,8 @@ __device__ __forceinline__ void zfused_body(const ZPar<C>& zp, int b, unsigned c
     C* twl = A + (long)nym * n;                              // ntw
     double* cl = reinterpret_cast<double*>(twl + ntw);       // cl[j] = DL(j-1), cl[0] = 0
     double* cu = cl + nym;                                   // cu[j] = DU(j), cu[nym-1] = 0
+    double* sg = cu + nym;                                   // chunk summaries: kZfThreads x (re, im, gain)
+    double* sc = sg + 3 * kZfThreads;                        // chunk carries: kZfThreads x (re, im)
     const auto plane = gptr(zp.spec) + (long)b * nym * n;
     const int total = nym * n;
     // load the plane: four values per thread in flight per batch (named
@@ -68,88 +67,114 @@ __device__ __forceinline__ void zfused_body(const ZPar<C>& zp, int b, unsigned c
     __syncthreads();
     zf_rows_fft<-1>(A, zp, twl);
 
-    const int m = threadIdx.x;  // mode of this thread (m < n)
-    if (m < n) {
-        const int kz = R * m + u;
-        const auto lcol = gptr(zp.lu) + (long)kx * nym * nzs + min(kz, nzp - kz);
-        // forward elimination: b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1).
-        // Pivots (global memory) are read kZfAhead rows ahead, rows and
-        // coefficients (LDS) kZfLds rows ahead; loads are unconditional with
-        // clamped indices (the clamped reads are never used).
-        C ra[kZfLds];
-        double rc[kZfLds], rl[kZfAhead];
-#pragma unroll
-        for (int q = 0; q < kZfAhead; ++q) rl[q] = lcol[(long)min(q, nym - 1) * nzs];
-#pragma unroll
-        for (int q = 0; q < kZfLds; ++q) {
-            const int jr = min(q, nym - 1);
-            ra[q] = A[jr * n + m];
-            rc[q] = cl[jr];
+    // y solves, chunk-parallel: lane t takes mode m = t % n and the rows
+    // [s, e) of chunk c = t / n (P = kZfThreads / n chunks of L rows).  The
+    // elimination y(j) = b(j) - mu(j) y(j-1), mu(j) = DL(j-1)/D'(j-1), and the
+    // back substitution x(j) = alpha(j) - beta(j) x(j+1), alpha = y/D',
+    // beta = DU(j)/D', are first-order linear recurrences: each chunk runs its
+    // own from a zero start while accumulating the gain of its start value
+    // (phase 1), one lane per mode chains the chunk carries (phase 2), and each
+    // chunk adds gain x carry to its rows (phase 3).  Pivots 1/D'(j) come from
+    // the plan's table (pin of projection.f90:292 included).
+    const int P = max(1, kZfThreads / n);
+    const int L = (nym + P - 1) / P;
+    const int t = threadIdx.x;
+    const int m = t % n;
+    const int c = t / n;
+    const bool act = c < P && t < n * P;
+    const int s = min(c * L, nym), e = min(s + L, nym);
+    const int kz = R * m + u;
+    const auto lcol = gptr(zp.lu) + (long)kx * nym * nzs + min(kz, nzp - kz);
+    // phase 1, elimination: local yh(j) from yh(s-1) = 0, gain g(j) = prod(-mu)
+    if (act && s < e) {
+        double yx = 0.0, yy = 0.0, gn = 1.0;
+        double ipp = s > 0 ? lcol[(long)(s - 1) * nzs] : 0.0;
+        for (int j = s; j < e; ++j) {
+            const double mu = j > 0 ? cl[j] * ipp : 0.0;
+            ipp = lcol[(long)j * nzs];
+            const C a = A[j * n + m];
+            yx = (double)a.x - mu * yx;
+            yy = (double)a.y - mu * yy;
+            gn = -mu * gn;
+            A[j * n + m] = CplxT<C>::make((Rl)yx, (Rl)yy);
         }
-        double bpx = 0.0, bpy = 0.0, ipp = 0.0;
-        for (int j0 = 0; j0 < nym; j0 += kZfAhead) {
-#pragma unroll
-            for (int q = 0; q < kZfAhead; ++q) {
-                const int j = j0 + q;
-                if (j < nym) {
-                    const int qa = q % kZfLds;
-                    const C a = ra[qa];
-                    const double ip = rl[q], dlj = rc[qa];
-                    rl[q] = lcol[(long)min(j + kZfAhead, nym - 1) * nzs];
-                    const int jn = min(j + kZfLds, nym - 1);
-                    ra[qa] = A[jn * n + m];
-                    rc[qa] = cl[jn];
-                    double bx = (double)a.x, by = (double)a.y;
-                    if (j > 0) {
-                        const double mult = dlj * ipp;
-                        bx = bx - mult * bpx;
-                        by = by - mult * bpy;
-                    }
-                    A[j * n + m] = CplxT<C>::make((Rl)bx, (Rl)by);
-                    bpx = bx;
-                    bpy = by;
-                    ipp = ip;
-                }
+        sg[3 * t] = yx;
+        sg[3 * t + 1] = yy;
+        sg[3 * t + 2] = gn;
+    }
+    __syncthreads();
+    // phase 2: y(s_c - 1) for every chunk, chained from chunk 0 (y(-1) = 0)
+    if (t < n) {
+        double cx = 0.0, cy = 0.0;
+        for (int q = 0; q < P; ++q) {
+            const int tq = q * n + t;
+            sc[2 * tq] = cx;
+            sc[2 * tq + 1] = cy;
+            if (min(q * L, nym) < nym) {
+                const double gq = sg[3 * tq + 2];
+                const double nx = sg[3 * tq] + gq * cx, ny = sg[3 * tq + 1] + gq * cy;
+                cx = nx;
+                cy = ny;
             }
         }
-        // back substitution: x(nym-1) = b'/D', x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
-#pragma unroll
-        for (int q = 0; q < kZfAhead; ++q) rl[q] = lcol[(long)max(nym - 1 - q, 0) * nzs];
-#pragma unroll
-        for (int q = 0; q < kZfLds; ++q) {
-            const int jr = max(nym - 1 - q, 0);
-            ra[q] = A[jr * n + m];
-            rc[q] = cu[jr];
+    }
+    __syncthreads();
+    // phase 3: y(j) = yh(j) + g(j) y(s-1); then the local back substitution
+    // xh(j) from xh(e) = 0 with gain h(j) = prod(-beta)
+    if (act && s < e) {
+        const double cx = sc[2 * t], cy = sc[2 * t + 1];
+        double gn = 1.0;
+        double ipp = s > 0 ? lcol[(long)(s - 1) * nzs] : 0.0;
+        for (int j = s; j < e; ++j) {
+            const double mu = j > 0 ? cl[j] * ipp : 0.0;
+            ipp = lcol[(long)j * nzs];
+            gn = -mu * gn;
+            const C a = A[j * n + m];
+            A[j * n + m] = CplxT<C>::make((Rl)((double)a.x + gn * cx), (Rl)((double)a.y + gn * cy));
         }
+        double xx = 0.0, xy = 0.0, hn = 1.0;
+        for (int j = e - 1; j >= s; --j) {
+            const double ip = lcol[(long)j * nzs];
+            const double beta = cu[j] * ip;  // cu[nym-1] = 0
+            const C a = A[j * n + m];
+            xx = (double)a.x * ip - beta * xx;
+            xy = (double)a.y * ip - beta * xy;
+            hn = -beta * hn;
+            A[j * n + m] = CplxT<C>::make((Rl)xx, (Rl)xy);
+        }
+        sg[3 * t] = xx;
+        sg[3 * t + 1] = xy;
+        sg[3 * t + 2] = hn;
+    }
+    __syncthreads();
+    // phase 4: x(e_c) for every chunk, chained from the last (x(nym) = 0)
+    if (t < n) {
+        double cx = 0.0, cy = 0.0;
+        for (int q = P - 1; q >= 0; --q) {
+            const int tq = q * n + t;
+            sc[2 * tq] = cx;
+            sc[2 * tq + 1] = cy;
+            if (min(q * L, nym) < nym) {
+                const double hq = sg[3 * tq + 2];
+                const double nx = sg[3 * tq] + hq * cx, ny = sg[3 * tq + 1] + hq * cy;
+                cx = nx;
+                cy = ny;
+            }
+        }
+    }
+    __syncthreads();
+    // phase 5: x(j) = xh(j) + h(j) x(e)
+    if (act && s < e) {
+        const double cx = sc[2 * t], cy = sc[2 * t + 1];
         const bool is00 = zp.zero00 && kx == 0 && u == 0 && m == 0;
-        double xnx = 0.0, xny = 0.0;
-        for (int t0 = 0; t0 < nym; t0 += kZfAhead) {
-#pragma unroll
-            for (int q = 0; q < kZfAhead; ++q) {
-                const int t = t0 + q;
-                if (t < nym) {
-                    const int j = nym - 1 - t;
-                    const int qa = q % kZfLds;
-                    const C bb = ra[qa];
-                    const double ip = rl[q], duj = rc[qa];
-                    rl[q] = lcol[(long)max(j - kZfAhead, 0) * nzs];
-                    const int jn = max(j - kZfLds, 0);
-                    ra[qa] = A[jn * n + m];
-                    rc[qa] = cu[jn];
-                    double xx, xy;
-                    if (j == nym - 1) {
-                        xx = (double)bb.x * ip;
-                        xy = (double)bb.y * ip;
-                    } else {
-                        xx = ((double)bb.x - duj * xnx) * ip;
-                        xy = ((double)bb.y - duj * xny) * ip;
-                    }
-                    if (is00) xx = xy = 0.0;
-                    A[j * n + m] = CplxT<C>::make((Rl)xx, (Rl)xy);
-                    xnx = xx;
-                    xny = xy;
-                }
-            }
+        double hn = 1.0;
+        for (int j = e - 1; j >= s; --j) {
+            const double beta = cu[j] * lcol[(long)j * nzs];
+            hn = -beta * hn;
+            const C a = A[j * n + m];
+            double xx = (double)a.x + hn * cx, xy = (double)a.y + hn * cy;
+            if (is00) xx = xy = 0.0;
+            A[j * n + m] = CplxT<C>::make((Rl)xx, (Rl)xy);
         }
     }
     __syncthreads();
@@ -187,7 +212,7 @@ __global__ void __launch_bounds__(kZfThreads) kb_zfused(const ZPar<C>* __restric
 // LDS of the fused kernel for a plan (spectrum precision f32 or FP64)
 size_t zf_lds_bytes(int nym, int n, int ntw, int f32) {
     const size_t cs = f32 ? sizeof(float2) : sizeof(double2);
-    return cs * ((size_t)nym * n + ntw) + sizeof(double) * 2 * (size_t)nym;
+    return cs * ((size_t)nym * n + ntw) + sizeof(double) * (2 * (size_t)nym + 5 * kZfThreads);
 }
 
 size_t zf_lds_bytes(const dfp_plan* p, int f32) {

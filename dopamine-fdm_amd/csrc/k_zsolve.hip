@@ -104,6 +104,97 @@ constexpr int kZsAhead = 2;  // rows read ahead in the elimination sweeps
     double pl0, pl1, pl2, pl3, pl4, pl5, pl6, pl7;                                \
     double pd
 
+// forward elimination of one group for mode m: b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1)
+template <typename C>
+__device__ __forceinline__ void fwd_sweep(C* A, const double* lus, const double* dus, int gr, int n, int m, int j0,
+                                          double& bpx, double& bpy, double& ipp) {
+    using Rl = typename CplxT<C>::real;
+    // rows, pivots and coefficients are read kZsAhead rows ahead of
+    // the store of the current row, so LDS latency leaves the chain
+    C ra[kZsAhead];
+    double rl[kZsAhead], rd[kZsAhead];
+#pragma unroll
+    for (int q = 0; q < kZsAhead; ++q) {
+        const int rr = min(q, gr - 1);
+        ra[q] = A[rr * n + m];
+        rl[q] = lus[rr * n + m];
+        rd[q] = dus[rr];
+    }
+    for (int r0 = 0; r0 < gr; r0 += kZsAhead) {
+#pragma unroll
+        for (int q = 0; q < kZsAhead; ++q) {
+            const int r = r0 + q;
+            if (r < gr) {
+                const C a = ra[q];
+                const double lr = rl[q], dr = rd[q];
+                // unconditional (clamped) so no select waits on the load
+                const int rn = min(r + kZsAhead, gr - 1);
+                ra[q] = A[rn * n + m];
+                rl[q] = lus[rn * n + m];
+                rd[q] = dus[rn];
+                double bx = (double)a.x, by = (double)a.y;
+                if (j0 + r > 0) {
+                    const double mult = dr * ipp;  // DL(j-1) / D'(j-1)
+                    bx = bx - mult * bpx;
+                    by = by - mult * bpy;
+                }
+                A[r * n + m] = CplxT<C>::make((Rl)bx, (Rl)by);
+                bpx = bx;
+                bpy = by;
+                ipp = lr;
+            }
+        }
+    }
+}
+
+// back substitution of one group for mode m, rows from the top of the group:
+// x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
+template <typename C>
+__device__ __forceinline__ void bwd_sweep(C* A, const double* lus, const double* dus, int gr, int n, int m, int j0,
+                                          int nym, bool is00, double& xnx, double& xny) {
+    using Rl = typename CplxT<C>::real;
+    // rows from the top of the group, read kZsAhead rows ahead
+    C ra[kZsAhead];
+    double rl[kZsAhead], rd[kZsAhead];
+#pragma unroll
+    for (int q = 0; q < kZsAhead; ++q) {
+        const int rr = max(gr - 1 - q, 0);
+        ra[q] = A[rr * n + m];
+        rl[q] = lus[rr * n + m];
+        rd[q] = dus[rr];
+    }
+    for (int t0 = 0; t0 < gr; t0 += kZsAhead) {
+#pragma unroll
+        for (int q = 0; q < kZsAhead; ++q) {
+            const int t = t0 + q;
+            if (t < gr) {
+                const int r = gr - 1 - t;
+                const int j = j0 + r;
+                const C bb = ra[q];
+                const double ip = rl[q], dr = rd[q];
+                // unconditional (clamped) so no select waits on the load
+                const int rn = max(gr - 1 - (t + kZsAhead), 0);
+                ra[q] = A[rn * n + m];
+                rl[q] = lus[rn * n + m];
+                rd[q] = dus[rn];
+                double xx, xy;
+                if (j == nym - 1) {
+                    xx = (double)bb.x * ip;
+                    xy = (double)bb.y * ip;
+                } else {
+                    // x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
+                    xx = ((double)bb.x - dr * xnx) * ip;
+                    xy = ((double)bb.y - dr * xny) * ip;
+                }
+                if (is00) xx = xy = 0.0;
+                A[r * n + m] = CplxT<C>::make((Rl)xx, (Rl)xy);
+                xnx = xx;
+                xny = xy;
+            }
+        }
+    }
+}
+
 // C: spectrum precision (double2, or float2 for the FP32 form).  The
 // pivots, coefficients and the elimination itself are FP64 in both forms;
 // the FP32 form rounds only what it stores.  b = (kx, u) sub-plane index.
@@ -136,44 +227,7 @@ __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned cha
         __syncthreads();
         if (j0 + G < nym) ZS_PREFETCH(j0 + G, min(G, nym - j0 - G), dl, -1);
         lds_fft_inplace<-1, 2>(A, gr, n, zp.fz, twl);
-        if (m < n) {
-            // rows, pivots and coefficients are read kZsAhead rows ahead of
-            // the store of the current row, so LDS latency leaves the chain
-            C ra[kZsAhead];
-            double rl[kZsAhead], rd[kZsAhead];
-#pragma unroll
-            for (int q = 0; q < kZsAhead; ++q) {
-                const int rr = min(q, gr - 1);
-                ra[q] = A[rr * n + m];
-                rl[q] = lus[rr * n + m];
-                rd[q] = dus[rr];
-            }
-            for (int r0 = 0; r0 < gr; r0 += kZsAhead) {
-#pragma unroll
-                for (int q = 0; q < kZsAhead; ++q) {
-                    const int r = r0 + q;
-                    if (r < gr) {
-                        const C a = ra[q];
-                        const double lr = rl[q], dr = rd[q];
-                        // unconditional (clamped) so no select waits on the load
-                        const int rn = min(r + kZsAhead, gr - 1);
-                        ra[q] = A[rn * n + m];
-                        rl[q] = lus[rn * n + m];
-                        rd[q] = dus[rn];
-                        double bx = (double)a.x, by = (double)a.y;
-                        if (j0 + r > 0) {
-                            const double mult = dr * ipp;  // DL(j-1) / D'(j-1)
-                            bx = bx - mult * bpx;
-                            by = by - mult * bpy;
-                        }
-                        A[r * n + m] = CplxT<C>::make((Rl)bx, (Rl)by);
-                        bpx = bx;
-                        bpy = by;
-                        ipp = lr;
-                    }
-                }
-            }
-        }
+        if (m < n) fwd_sweep<C>(A, lus, dus, gr, n, m, j0, bpx, bpy, ipp);
         __syncthreads();
         const auto dst = plane + (long)j0 * n;
         for (int e = threadIdx.x; e < gr * n; e += kZsThreads) gst<C>(dst + e, A[e]);
@@ -220,48 +274,7 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
             const int j0q = max(0, nym - (gi + 2) * G);
             ZS_PREFETCH(j0q, j0 - j0q, du, 0);
         }
-        if (m < n) {
-            // rows from the top of the group, read kZsAhead rows ahead
-            C ra[kZsAhead];
-            double rl[kZsAhead], rd[kZsAhead];
-#pragma unroll
-            for (int q = 0; q < kZsAhead; ++q) {
-                const int rr = max(gr - 1 - q, 0);
-                ra[q] = A[rr * n + m];
-                rl[q] = lus[rr * n + m];
-                rd[q] = dus[rr];
-            }
-            for (int t0 = 0; t0 < gr; t0 += kZsAhead) {
-#pragma unroll
-                for (int q = 0; q < kZsAhead; ++q) {
-                    const int t = t0 + q;
-                    if (t < gr) {
-                        const int r = gr - 1 - t;
-                        const int j = j0 + r;
-                        const C bb = ra[q];
-                        const double ip = rl[q], dr = rd[q];
-                        // unconditional (clamped) so no select waits on the load
-                        const int rn = max(gr - 1 - (t + kZsAhead), 0);
-                        ra[q] = A[rn * n + m];
-                        rl[q] = lus[rn * n + m];
-                        rd[q] = dus[rn];
-                        double xx, xy;
-                        if (j == nym - 1) {
-                            xx = (double)bb.x * ip;
-                            xy = (double)bb.y * ip;
-                        } else {
-                            // x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
-                            xx = ((double)bb.x - dr * xnx) * ip;
-                            xy = ((double)bb.y - dr * xny) * ip;
-                        }
-                        if (is00) xx = xy = 0.0;
-                        A[r * n + m] = CplxT<C>::make((Rl)xx, (Rl)xy);
-                        xnx = xx;
-                        xny = xy;
-                    }
-                }
-            }
-        }
+        if (m < n) bwd_sweep<C>(A, lus, dus, gr, n, m, j0, nym, is00, xnx, xny);
         __syncthreads();
         lds_fft_inplace<1, 2>(A, gr, n, zp.fz, twl);
         const auto dst = plane + (long)j0 * n;
@@ -269,16 +282,195 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
     }
 }
 
-template <typename C>
-__global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(ZPar<C> zp) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    zfwd_body<C>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
+// Two-pass z transforms for n = N1*N2 (N2 = 4 or 8).  Forward: the thread
+// of item (row r, i1) loads x[i1 + N1*i2], i2 < N2, straight from HBM into
+// registers (the next group's items load while this group is swept),
+// transforms them (length N2), applies w_n^{i1*k2} and writes Y[r][i1*N2 + k2]
+// to LDS; the item (r, k2) then transforms Y[r][k2 + N2*i1] over i1 (length
+// N1) in place, which leaves X[r][k2 + N2*k1] in natural order.  Inverse: the
+// same two steps in the other order, the second one storing x[i1 + N1*i2]
+// from registers to HBM.  Both replace the log(n) LDS passes of the staged
+// transform by one exchange through LDS.
+template <typename C, int N1, int N2>
+__device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
+    constexpr int N = N1 * N2;
+    constexpr int KI = kZsPf / N2;  // first-step items per thread and group
+    C* smem = reinterpret_cast<C*>(smem_raw);
+    const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G;
+    const auto dl = gptr(zp.dl);
+    const int nzs = nzp / 2 + 1;
+    const int kx = b / R;
+    const int u = b - kx * R;
+    C* A = smem;
+    C* twl = A + G * N;
+    double* lus = reinterpret_cast<double*>(twl + N);
+    double* dus = lus + G * N;
+    const auto plane = gptr(zp.spec) + (long)b * nym * N;
+    const auto lup = gptr(zp.lu) + (long)kx * nym * nzs;
+    for (int e = threadIdx.x; e < N; e += kZsThreads) twl[e] = gld<C>(gptr(zp.tw2) + e);
+    C pr[KI][N2];
+    double pl[kZsPf];
+    double pd;
+    auto prefetch = [&](int j0, int gq) {
+        const auto src = plane + (long)j0 * N;
+        const int lim = gq * N1 - 1;
+#pragma unroll
+        for (int k = 0; k < KI; ++k) {
+            const int it = min((int)threadIdx.x + k * kZsThreads, lim);
+            const int r = it / N1, i1 = it - r * N1;
+#pragma unroll
+            for (int i2 = 0; i2 < N2; ++i2) pr[k][i2] = gld<C>(src + r * N + i1 + N1 * i2);
+        }
+        const auto lsrc = lup + (long)j0 * nzs;
+        const int lime = gq * N - 1;
+#pragma unroll
+        for (int k = 0; k < kZsPf; ++k) {
+            const int e = min((int)threadIdx.x + k * kZsThreads, lime);
+            const int r = e / N, mm = e - r * N;
+            const int kz = R * mm + u;
+            pl[k] = lsrc[(long)r * nzs + min(kz, nzp - kz)];
+        }
+        const int jc = j0 + min((int)threadIdx.x, gq - 1) - 1;  // DL(j-1) of row j
+        pd = dl[max(0, min(jc, nym - 2))];
+    };
+    prefetch(0, min(G, nym));
+    double bpx = 0.0, bpy = 0.0, ipp = 0.0;
+    const int m = threadIdx.x;  // mode of this thread (m < N)
+    for (int j0 = 0; j0 < nym; j0 += G) {
+        const int gr = min(G, nym - j0);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KI; ++k) {
+            const int it = (int)threadIdx.x + k * kZsThreads;
+            if (it < gr * N1) {
+                const int r = it / N1, i1 = it - r * N1;
+                C v[N2];
+#pragma unroll
+                for (int i2 = 0; i2 < N2; ++i2) v[i2] = pr[k][i2];
+                dft_reg<-1, N2>(v);
+                C* dst = A + r * N + i1 * N2;
+                dst[0] = v[0];
+#pragma unroll
+                for (int k2 = 1; k2 < N2; ++k2) dst[k2] = twiddle<-1>(v[k2], twl[i1 * k2]);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < kZsPf; ++k) {
+            const int e = (int)threadIdx.x + k * kZsThreads;
+            if (e < gr * N) lus[e] = pl[k];
+        }
+        if ((int)threadIdx.x < gr) dus[threadIdx.x] = pd;
+        __syncthreads();
+        if (j0 + G < nym) prefetch(j0 + G, min(G, nym - j0 - G));
+        for (int it = threadIdx.x; it < gr * N2; it += kZsThreads) {
+            const int r = it / N2, k2 = it - r * N2;
+            C* base = A + r * N + k2;
+            C v[N1];
+#pragma unroll
+            for (int i1 = 0; i1 < N1; ++i1) v[i1] = base[i1 * N2];
+            dft_reg<-1, N1>(v);
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1) base[k1 * N2] = v[k1];
+        }
+        __syncthreads();
+        if (m < N) fwd_sweep<C>(A, lus, dus, gr, N, m, j0, bpx, bpy, ipp);
+        __syncthreads();
+        const auto dst = plane + (long)j0 * N;
+        for (int e = threadIdx.x; e < gr * N; e += kZsThreads) gst<C>(dst + e, A[e]);
+    }
 }
 
-template <typename C>
+template <typename C, int N1, int N2>
+__device__ __forceinline__ void zbwd2_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
+    constexpr int N = N1 * N2;
+    C* smem = reinterpret_cast<C*>(smem_raw);
+    const int n = N;
+    const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G;
+    const FastDiv div_n = zp.div_n;
+    const auto du = gptr(zp.du);
+    const int nzs = nzp / 2 + 1;
+    const int kx = b / R;
+    const int u = b - kx * R;
+    C* A = smem;
+    C* twl = A + G * N;
+    double* lus = reinterpret_cast<double*>(twl + N);
+    double* dus = lus + G * N;
+    const auto plane = gptr(zp.spec) + (long)b * nym * N;
+    const auto lup = gptr(zp.lu) + (long)kx * nym * nzs;
+    for (int e = threadIdx.x; e < N; e += kZsThreads) twl[e] = gld<C>(gptr(zp.tw2) + e);
+    const int ngroups = (nym + G - 1) / G;
+    ZS_REGS;
+    {
+        const int j0q = max(0, nym - G);
+        ZS_PREFETCH(j0q, nym - j0q, du, 0);
+    }
+    double xnx = 0.0, xny = 0.0;
+    const int m = threadIdx.x;
+    const bool is00 = zp.zero00 && kx == 0 && u == 0 && m == 0;
+    for (int gi = 0; gi < ngroups; ++gi) {
+        const int j0 = max(0, nym - (gi + 1) * G);
+        const int gr = nym - gi * G - j0;
+        __syncthreads();
+        ZS_STAGE(gr);
+        __syncthreads();
+        if (gi + 1 < ngroups) {
+            const int j0q = max(0, nym - (gi + 2) * G);
+            ZS_PREFETCH(j0q, j0 - j0q, du, 0);
+        }
+        if (m < N) bwd_sweep<C>(A, lus, dus, gr, N, m, j0, nym, is00, xnx, xny);
+        __syncthreads();
+        for (int it = threadIdx.x; it < gr * N2; it += kZsThreads) {
+            const int r = it / N2, k2 = it - r * N2;
+            C* base = A + r * N + k2;
+            C v[N1];
+#pragma unroll
+            for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[k1 * N2];
+            dft_reg<1, N1>(v);
+            base[0] = v[0];
+#pragma unroll
+            for (int i1 = 1; i1 < N1; ++i1) base[i1 * N2] = twiddle<1>(v[i1], twl[i1 * k2]);
+        }
+        __syncthreads();
+        const auto dst = plane + (long)j0 * N;
+        for (int it = threadIdx.x; it < gr * N1; it += kZsThreads) {
+            const int r = it / N1, i1 = it - r * N1;
+            const C* src = A + r * N + i1 * N2;
+            C v[N2];
+#pragma unroll
+            for (int k2 = 0; k2 < N2; ++k2) v[k2] = src[k2];
+            dft_reg<1, N2>(v);
+#pragma unroll
+            for (int i2 = 0; i2 < N2; ++i2) gst<C>(dst + r * N + i1 + N1 * i2, v[i2]);
+        }
+    }
+}
+
+// the two-pass factorizations (N1, N2) with N1 * N2 = nzp / R; others use the
+// staged transform
+#define DFP_ZV_LIST(X) X(2, 4) X(3, 4) X(2, 8) X(3, 8) X(4, 8) X(9, 4) X(6, 8) X(8, 8) X(9, 8) X(12, 8) X(16, 8)
+
+bool z_two_pass(int n, int& n1, int& n2) {
+#define DFP_ZV_TRY(a, c) \
+    if (a * c == n) { n1 = a; n2 = c; return true; }
+    DFP_ZV_LIST(DFP_ZV_TRY)
+#undef DFP_ZV_TRY
+    n1 = n2 = 0;
+    return false;
+}
+
+// N1 = N2 = 0: the staged transform
+template <typename C, int N1, int N2>
+__global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(ZPar<C> zp) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    if constexpr (N1 == 0) zfwd_body<C>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
+    else zfwd2_body<C, N1, N2>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
+}
+
+template <typename C, int N1, int N2>
 __global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(ZPar<C> zp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    zbwd_body<C>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
+    if constexpr (N1 == 0) zbwd_body<C>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
+    else zbwd2_body<C, N1, N2>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
 }
 
 __device__ __forceinline__ int zfind_case(const int* __restrict__ off, int nc, int w) {
@@ -291,7 +483,7 @@ __device__ __forceinline__ int zfind_case(const int* __restrict__ off, int nc, i
     return lo;
 }
 
-template <typename C, int DIR>
+template <typename C, int DIR, int N1, int N2>
 __global__ void __launch_bounds__(kZsThreads) kb_zsolve(const ZPar<C>* __restrict__ cs, const int* __restrict__ off,
                                                         const int* __restrict__ idx, const int* __restrict__ pnc) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -302,8 +494,13 @@ __global__ void __launch_bounds__(kZsThreads) kb_zsolve(const ZPar<C>* __restric
         const int w = base + me;
         if (w >= total) break;
         const int c = zfind_case(off, nc, w);
-        if (DIR < 0) zfwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
-        else zbwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
+        if constexpr (N1 == 0) {
+            if (DIR < 0) zfwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
+            else zbwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
+        } else {
+            if (DIR < 0) zfwd2_body<C, N1, N2>(cs[idx[c]], w - off[c], smem_raw);
+            else zbwd2_body<C, N1, N2>(cs[idx[c]], w - off[c], smem_raw);
+        }
         __syncthreads();
     }
 }
@@ -322,6 +519,10 @@ ZPar<C> make_zpar(const dfp_plan* p, int f32_zero00) {
     z.nzp = g.nzp;
     z.R = p->zr;
     z.ntw = twiddle_count(p->fz);
+    z.n1 = p->zfused ? 0 : p->zn1;
+    z.n2 = p->zfused ? 0 : p->zn2;
+    z.tw2 = sizeof(C) == sizeof(double2) ? reinterpret_cast<const C*>(p->tw_z2) : reinterpret_cast<const C*>(p->tw_z2_32);
+    if (z.n1) z.ntw = p->fz.n;
     z.G = p->g_zs;
     z.zero00 = f32_zero00;
     z.nblocks = g.nxh * p->zr;
@@ -332,18 +533,30 @@ template ZPar<double2> make_zpar<double2>(const dfp_plan*, int);
 template ZPar<float2> make_zpar<float2>(const dfp_plan*, int);
 
 size_t z_lds_bytes(const dfp_plan* p, int f32) {
-    const int G = p->g_zs, n = p->fz.n, ntw = twiddle_count(p->fz);
+    const int G = p->g_zs, n = p->fz.n, ntw = p->zn1 ? n : twiddle_count(p->fz);
     return (f32 ? sizeof(float2) : sizeof(double2)) * ((size_t)G * n + ntw) + sizeof(double) * ((size_t)G * n + G);
 }
+
+int z_variant(const dfp_plan* p) { return p->zfused ? -1 : p->zn1 * 32 + p->zn2; }
 
 template <typename C>
 static hipError_t launch_zsolve_t(dfp_plan* p, int dir, const ZPar<C>& zp) {
     const bool f32 = sizeof(C) == 8;
     const size_t lds = z_lds_bytes(p, f32);
     const int slot = prof_begin(p, dir < 0 ? (f32 ? "zsolve_fwd32" : "zsolve_fwd") : (f32 ? "zsolve_bwd32" : "zsolve_bwd"));
-    const dim3 grid(zp.nblocks);
-    if (dir < 0) hipLaunchKernelGGL(k_zsolve_fwd<C>, grid, dim3(kZsThreads), lds, p->stream, zp);
-    else hipLaunchKernelGGL(k_zsolve_bwd<C>, grid, dim3(kZsThreads), lds, p->stream, zp);
+    const dim3 grid(zp.nblocks), blk(kZsThreads);
+    switch (z_variant(p)) {
+#define DFP_ZV_CASE(a, c)                                                                          \
+    case a * 32 + c:                                                                               \
+        if (dir < 0) hipLaunchKernelGGL((k_zsolve_fwd<C, a, c>), grid, blk, lds, p->stream, zp);  \
+        else hipLaunchKernelGGL((k_zsolve_bwd<C, a, c>), grid, blk, lds, p->stream, zp);          \
+        break;
+        DFP_ZV_LIST(DFP_ZV_CASE)
+#undef DFP_ZV_CASE
+        default:
+            if (dir < 0) hipLaunchKernelGGL((k_zsolve_fwd<C, 0, 0>), grid, blk, lds, p->stream, zp);
+            else hipLaunchKernelGGL((k_zsolve_bwd<C, 0, 0>), grid, blk, lds, p->stream, zp);
+    }
     prof_end(p, slot);
     return hipGetLastError();
 }
@@ -352,26 +565,41 @@ hipError_t launch_zsolve(dfp_plan* p, int dir) { return launch_zsolve_t<double2>
 
 hipError_t launch_zsolve32(dfp_plan* p, int dir) { return launch_zsolve_t<float2>(p, dir, make_zpar<float2>(p, 1)); }
 
+// variant: z_variant of the cases of the list (all alike)
 template <typename C>
-hipError_t launch_zsolve_batch(hipStream_t s, int dir, const ZPar<C>* cs, const int* off, const int* idx,
+hipError_t launch_zsolve_batch(hipStream_t s, int dir, int variant, const ZPar<C>* cs, const int* off, const int* idx,
                                const int* nc, int grid, size_t lds) {
-    if (dir < 0) hipLaunchKernelGGL((kb_zsolve<C, -1>), dim3(grid), dim3(kZsThreads), lds, s, cs, off, idx, nc);
-    else hipLaunchKernelGGL((kb_zsolve<C, 1>), dim3(grid), dim3(kZsThreads), lds, s, cs, off, idx, nc);
+    const dim3 g(grid), blk(kZsThreads);
+    switch (variant) {
+#define DFP_ZV_CASE(a, c)                                                                                   \
+    case a * 32 + c:                                                                                        \
+        if (dir < 0) hipLaunchKernelGGL((kb_zsolve<C, -1, a, c>), g, blk, lds, s, cs, off, idx, nc);        \
+        else hipLaunchKernelGGL((kb_zsolve<C, 1, a, c>), g, blk, lds, s, cs, off, idx, nc);                 \
+        break;
+        DFP_ZV_LIST(DFP_ZV_CASE)
+#undef DFP_ZV_CASE
+        default:
+            if (dir < 0) hipLaunchKernelGGL((kb_zsolve<C, -1, 0, 0>), g, blk, lds, s, cs, off, idx, nc);
+            else hipLaunchKernelGGL((kb_zsolve<C, 1, 0, 0>), g, blk, lds, s, cs, off, idx, nc);
+    }
     return hipGetLastError();
 }
-template hipError_t launch_zsolve_batch<double2>(hipStream_t, int, const ZPar<double2>*, const int*, const int*,
+template hipError_t launch_zsolve_batch<double2>(hipStream_t, int, int, const ZPar<double2>*, const int*, const int*,
                                                  const int*, int, size_t);
-template hipError_t launch_zsolve_batch<float2>(hipStream_t, int, const ZPar<float2>*, const int*, const int*,
+template hipError_t launch_zsolve_batch<float2>(hipStream_t, int, int, const ZPar<float2>*, const int*, const int*,
                                                 const int*, int, size_t);
 
 // rows per group: the group fits the prefetch registers (kZsPf points per
-// thread), the in-place FFT capacity (2 butterflies per thread) and, with the
+// thread), the in-place FFT capacity (2 butterflies per thread; not a limit of
+// the two-pass transforms) and, with the
 // twiddles and pivots, the device's LDS per block; 0 if one row does not fit
 int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit) {
     const int n = fz.n;
     if (n > kZsThreads) return 0;  // one mode per thread
-    const size_t ntw = (size_t)twiddle_count(fz);
-    int G = std::min(kZsPf * kZsThreads, inplace_capacity(fz, kZsThreads, 2)) / n;
+    int n1, n2;
+    const bool two = z_two_pass(n, n1, n2);
+    const size_t ntw = two ? (size_t)n : (size_t)twiddle_count(fz);
+    int G = (two ? kZsPf * kZsThreads : std::min(kZsPf * kZsThreads, inplace_capacity(fz, kZsThreads, 2))) / n;
     G = std::max(0, std::min(G, nym));
     auto bytes = [&](int g) { return sizeof(double2) * ((size_t)g * n + ntw) + sizeof(double) * ((size_t)g * n + g); };
     while (G > 0 && bytes(G) > lds_limit) --G;
