@@ -104,10 +104,11 @@ constexpr int kZsAhead = 2;  // rows read ahead in the elimination sweeps
     double pl0, pl1, pl2, pl3, pl4, pl5, pl6, pl7;                                \
     double pd
 
-// forward elimination of one group for mode m: b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1)
-template <typename C>
-__device__ __forceinline__ void fwd_sweep(C* A, const double* lus, const double* dus, int gr, int n, int m, int j0,
-                                          double& bpx, double& bpy, double& ipp) {
+// forward elimination of one group for mode m: b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1);
+// put(r, b') keeps the result of group row r
+template <typename C, typename Put>
+__device__ __forceinline__ void fwd_sweep(const C* A, const double* lus, const double* dus, int gr, int n, int m,
+                                          int j0, double& bpx, double& bpy, double& ipp, Put put) {
     using Rl = typename CplxT<C>::real;
     // rows, pivots and coefficients are read kZsAhead rows ahead of
     // the store of the current row, so LDS latency leaves the chain
@@ -138,7 +139,7 @@ __device__ __forceinline__ void fwd_sweep(C* A, const double* lus, const double*
                     bx = bx - mult * bpx;
                     by = by - mult * bpy;
                 }
-                A[r * n + m] = CplxT<C>::make((Rl)bx, (Rl)by);
+                put(r, CplxT<C>::make((Rl)bx, (Rl)by));
                 bpx = bx;
                 bpy = by;
                 ipp = lr;
@@ -200,7 +201,6 @@ __device__ __forceinline__ void bwd_sweep(C* A, const double* lus, const double*
 // the FP32 form rounds only what it stores.  b = (kx, u) sub-plane index.
 template <typename C>
 __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
-    using Rl = typename CplxT<C>::real;
     C* smem = reinterpret_cast<C*>(smem_raw);
     const int n = zp.fz.n;
     const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G, ntw = zp.ntw;
@@ -227,7 +227,7 @@ __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned cha
         __syncthreads();
         if (j0 + G < nym) ZS_PREFETCH(j0 + G, min(G, nym - j0 - G), dl, -1);
         lds_fft_inplace<-1, 2>(A, gr, n, zp.fz, twl);
-        if (m < n) fwd_sweep<C>(A, lus, dus, gr, n, m, j0, bpx, bpy, ipp);
+        if (m < n) fwd_sweep<C>(A, lus, dus, gr, n, m, j0, bpx, bpy, ipp, [&](int r, C v) { A[r * n + m] = v; });
         __syncthreads();
         const auto dst = plane + (long)j0 * n;
         for (int e = threadIdx.x; e < gr * n; e += kZsThreads) gst<C>(dst + e, A[e]);
@@ -238,7 +238,6 @@ __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned cha
 // in FP64 separately
 template <typename C>
 __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
-    using Rl = typename CplxT<C>::real;
     C* smem = reinterpret_cast<C*>(smem_raw);
     const int n = zp.fz.n;
     const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G, ntw = zp.ntw;
@@ -373,10 +372,9 @@ __device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int b, unsigned ch
             for (int k1 = 0; k1 < N1; ++k1) base[k1 * N2] = v[k1];
         }
         __syncthreads();
-        if (m < N) fwd_sweep<C>(A, lus, dus, gr, N, m, j0, bpx, bpy, ipp);
-        __syncthreads();
-        const auto dst = plane + (long)j0 * N;
-        for (int e = threadIdx.x; e < gr * N; e += kZsThreads) gst<C>(dst + e, A[e]);
+        // the lanes of the sweep store their rows: N contiguous modes per row
+        const auto dst = plane + (long)j0 * N + m;
+        if (m < N) fwd_sweep<C>(A, lus, dus, gr, N, m, j0, bpx, bpy, ipp, [&](int r, C v) { gst<C>(dst + r * N, v); });
     }
 }
 
