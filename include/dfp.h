@@ -87,8 +87,9 @@ int dfp_plan_destroy(dfp_plan *plan);
  *              substitution + inverse FFT, spectrum read and written twice);
  *              2: one fused kernel holding a whole (kx, z-split) sub-plane
  *              of nym x nzp/R modes in LDS (read and written once)
- * Results do not depend on x_columns, z_rows or z_fused; z_split changes
- * the FFT's factorisation (round-off level).  DFP_E_ARG for values out of
+ * Results do not depend on x_columns or z_rows; z_split changes the FFT's
+ * factorisation and z_fused = 2 the association of the y solve (chunked
+ * across lanes), both at round-off level.  DFP_E_ARG for values out of
  * range, DFP_E_UNSUPPORTED for shapes that do not fit the device's LDS. */
 typedef struct dfp_tuning {
     int32_t z_split, x_columns, z_rows, z_fused;
