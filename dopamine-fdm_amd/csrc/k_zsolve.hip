@@ -105,95 +105,88 @@ constexpr int kZsAhead = 2;  // rows read ahead in the elimination sweeps
     double pd
 
 // forward elimination of one group for mode m: b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1);
-// put(r, b') keeps the result of group row r
-template <typename C, typename Put>
+// at(r) is the LDS index of the mode in group row r, put(r, b') keeps the result.
+// The carry (b'(j0-1), 1/D'(j0-1)) starts at zero, which leaves row 0 as it is.
+// Rows, pivots and coefficients are read kZsAhead rows ahead (clamped, so the
+// loads are unconditional) into a ring of registers whose slots are fixed by
+// the unrolled loop: no copies wait on the loads in flight.
+template <typename C, typename At, typename Put>
 __device__ __forceinline__ void fwd_sweep(const C* A, const double* lus, const double* dus, int gr, int n, int m,
-                                          int j0, double& bpx, double& bpy, double& ipp, Put put) {
+                                          double& bpx, double& bpy, double& ipp, At at, Put put) {
     using Rl = typename CplxT<C>::real;
-    // rows, pivots and coefficients are read kZsAhead rows ahead of
-    // the store of the current row, so LDS latency leaves the chain
     C ra[kZsAhead];
     double rl[kZsAhead], rd[kZsAhead];
 #pragma unroll
     for (int q = 0; q < kZsAhead; ++q) {
         const int rr = min(q, gr - 1);
-        ra[q] = A[rr * n + m];
+        ra[q] = A[at(rr)];
         rl[q] = lus[rr * n + m];
         rd[q] = dus[rr];
     }
-    for (int r0 = 0; r0 < gr; r0 += kZsAhead) {
+    auto row = [&](int r, C& sa, double& sl, double& sd) {
+        const C a = sa;
+        const double lr = sl, dr = sd;
+        const int rn = min(r + kZsAhead, gr - 1);
+        sa = A[at(rn)];
+        sl = lus[rn * n + m];
+        sd = dus[rn];
+        const double mult = dr * ipp;  // DL(j-1) / D'(j-1)
+        const double bx = (double)a.x - mult * bpx;
+        const double by = (double)a.y - mult * bpy;
+        put(r, CplxT<C>::make((Rl)bx, (Rl)by));
+        bpx = bx;
+        bpy = by;
+        ipp = lr;
+    };
+    int r0 = 0;
+    for (; r0 + kZsAhead <= gr; r0 += kZsAhead) {
 #pragma unroll
-        for (int q = 0; q < kZsAhead; ++q) {
-            const int r = r0 + q;
-            if (r < gr) {
-                const C a = ra[q];
-                const double lr = rl[q], dr = rd[q];
-                // unconditional (clamped) so no select waits on the load
-                const int rn = min(r + kZsAhead, gr - 1);
-                ra[q] = A[rn * n + m];
-                rl[q] = lus[rn * n + m];
-                rd[q] = dus[rn];
-                double bx = (double)a.x, by = (double)a.y;
-                if (j0 + r > 0) {
-                    const double mult = dr * ipp;  // DL(j-1) / D'(j-1)
-                    bx = bx - mult * bpx;
-                    by = by - mult * bpy;
-                }
-                put(r, CplxT<C>::make((Rl)bx, (Rl)by));
-                bpx = bx;
-                bpy = by;
-                ipp = lr;
-            }
-        }
+        for (int q = 0; q < kZsAhead; ++q) row(r0 + q, ra[q], rl[q], rd[q]);
     }
+#pragma unroll
+    for (int q = 0; q < kZsAhead; ++q)
+        if (r0 + q < gr) row(r0 + q, ra[q], rl[q], rd[q]);
 }
 
 // back substitution of one group for mode m, rows from the top of the group:
-// x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
-template <typename C>
-__device__ __forceinline__ void bwd_sweep(C* A, const double* lus, const double* dus, int gr, int n, int m, int j0,
-                                          int nym, bool is00, double& xnx, double& xny) {
+// x(j) = (b'(j) - DU(j) x(j+1)) / D'(j), with x(nym) = 0 as the first carry;
+// at(r) and the read-ahead as for fwd_sweep
+template <typename C, typename At>
+__device__ __forceinline__ void bwd_sweep(C* A, const double* lus, const double* dus, int gr, int n, int m,
+                                          bool is00, double& xnx, double& xny, At at) {
     using Rl = typename CplxT<C>::real;
-    // rows from the top of the group, read kZsAhead rows ahead
     C ra[kZsAhead];
     double rl[kZsAhead], rd[kZsAhead];
 #pragma unroll
     for (int q = 0; q < kZsAhead; ++q) {
         const int rr = max(gr - 1 - q, 0);
-        ra[q] = A[rr * n + m];
+        ra[q] = A[at(rr)];
         rl[q] = lus[rr * n + m];
         rd[q] = dus[rr];
     }
-    for (int t0 = 0; t0 < gr; t0 += kZsAhead) {
+    auto row = [&](int t, C& sa, double& sl, double& sd) {
+        const int r = gr - 1 - t;
+        const C bb = sa;
+        const double ip = sl, dr = sd;
+        const int rn = max(gr - 1 - (t + kZsAhead), 0);
+        sa = A[at(rn)];
+        sl = lus[rn * n + m];
+        sd = dus[rn];
+        double xx = ((double)bb.x - dr * xnx) * ip;
+        double xy = ((double)bb.y - dr * xny) * ip;
+        if (is00) xx = xy = 0.0;
+        A[at(r)] = CplxT<C>::make((Rl)xx, (Rl)xy);
+        xnx = xx;
+        xny = xy;
+    };
+    int t0 = 0;
+    for (; t0 + kZsAhead <= gr; t0 += kZsAhead) {
 #pragma unroll
-        for (int q = 0; q < kZsAhead; ++q) {
-            const int t = t0 + q;
-            if (t < gr) {
-                const int r = gr - 1 - t;
-                const int j = j0 + r;
-                const C bb = ra[q];
-                const double ip = rl[q], dr = rd[q];
-                // unconditional (clamped) so no select waits on the load
-                const int rn = max(gr - 1 - (t + kZsAhead), 0);
-                ra[q] = A[rn * n + m];
-                rl[q] = lus[rn * n + m];
-                rd[q] = dus[rn];
-                double xx, xy;
-                if (j == nym - 1) {
-                    xx = (double)bb.x * ip;
-                    xy = (double)bb.y * ip;
-                } else {
-                    // x(j) = (b'(j) - DU(j) x(j+1)) / D'(j)
-                    xx = ((double)bb.x - dr * xnx) * ip;
-                    xy = ((double)bb.y - dr * xny) * ip;
-                }
-                if (is00) xx = xy = 0.0;
-                A[r * n + m] = CplxT<C>::make((Rl)xx, (Rl)xy);
-                xnx = xx;
-                xny = xy;
-            }
-        }
+        for (int q = 0; q < kZsAhead; ++q) row(t0 + q, ra[q], rl[q], rd[q]);
     }
+#pragma unroll
+    for (int q = 0; q < kZsAhead; ++q)
+        if (t0 + q < gr) row(t0 + q, ra[q], rl[q], rd[q]);
 }
 
 // C: spectrum precision (double2, or float2 for the FP32 form).  The
@@ -227,7 +220,8 @@ __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned cha
         __syncthreads();
         if (j0 + G < nym) ZS_PREFETCH(j0 + G, min(G, nym - j0 - G), dl, -1);
         lds_fft_inplace<-1, 2>(A, gr, n, zp.fz, twl);
-        if (m < n) fwd_sweep<C>(A, lus, dus, gr, n, m, j0, bpx, bpy, ipp, [&](int r, C v) { A[r * n + m] = v; });
+        auto at = [&](int r) { return r * n + m; };
+        if (m < n) fwd_sweep<C>(A, lus, dus, gr, n, m, bpx, bpy, ipp, at, [&](int r, C v) { A[at(r)] = v; });
         __syncthreads();
         const auto dst = plane + (long)j0 * n;
         for (int e = threadIdx.x; e < gr * n; e += kZsThreads) gst<C>(dst + e, A[e]);
@@ -273,7 +267,7 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
             const int j0q = max(0, nym - (gi + 2) * G);
             ZS_PREFETCH(j0q, j0 - j0q, du, 0);
         }
-        if (m < n) bwd_sweep<C>(A, lus, dus, gr, n, m, j0, nym, is00, xnx, xny);
+        if (m < n) bwd_sweep<C>(A, lus, dus, gr, n, m, is00, xnx, xny, [&](int r) { return r * n + m; });
         __syncthreads();
         lds_fft_inplace<1, 2>(A, gr, n, zp.fz, twl);
         const auto dst = plane + (long)j0 * n;
@@ -290,6 +284,19 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
 // same two steps in the other order, the second one storing x[i1 + N1*i2]
 // from registers to HBM.  Both replace the log(n) LDS passes of the staged
 // transform by one exchange through LDS.
+// LDS position of element (i1, k2) of row r in the two-pass layouts (i1*N2 + k2
+// unswizzled): the N2 elements of each i1 are rotated by i1 and, for N2 = 8
+// and rows of a multiple of 16 points, odd rows swap 8-element halves, so the
+// 16 lanes of an LDS pass (FP64) hit distinct banks in both passes and the
+// sweep; the same map serves the Y and X layouts, so both passes stay in place
+template <int N1, int N2>
+__device__ __forceinline__ int zsw(int r, int i1, int k2) {
+    constexpr int N = N1 * N2;
+    int p = i1 * N2 + (k2 + i1) % N2;
+    if constexpr (N2 == 8 && N % 16 == 0) p ^= (r & 1) << 3;
+    return r * N + p;
+}
+
 template <typename C, int N1, int N2>
 __device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
     constexpr int N = N1 * N2;
@@ -347,10 +354,9 @@ __device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int b, unsigned ch
 #pragma unroll
                 for (int i2 = 0; i2 < N2; ++i2) v[i2] = pr[k][i2];
                 dft_reg<-1, N2>(v);
-                C* dst = A + r * N + i1 * N2;
-                dst[0] = v[0];
+                A[zsw<N1, N2>(r, i1, 0)] = v[0];
 #pragma unroll
-                for (int k2 = 1; k2 < N2; ++k2) dst[k2] = twiddle<-1>(v[k2], twl[i1 * k2]);
+                for (int k2 = 1; k2 < N2; ++k2) A[zsw<N1, N2>(r, i1, k2)] = twiddle<-1>(v[k2], twl[i1 * k2]);
             }
         }
 #pragma unroll
@@ -363,18 +369,19 @@ __device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int b, unsigned ch
         if (j0 + G < nym) prefetch(j0 + G, min(G, nym - j0 - G));
         for (int it = threadIdx.x; it < gr * N2; it += kZsThreads) {
             const int r = it / N2, k2 = it - r * N2;
-            C* base = A + r * N + k2;
             C v[N1];
 #pragma unroll
-            for (int i1 = 0; i1 < N1; ++i1) v[i1] = base[i1 * N2];
+            for (int i1 = 0; i1 < N1; ++i1) v[i1] = A[zsw<N1, N2>(r, i1, k2)];
             dft_reg<-1, N1>(v);
 #pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) base[k1 * N2] = v[k1];
+            for (int k1 = 0; k1 < N1; ++k1) A[zsw<N1, N2>(r, k1, k2)] = v[k1];
         }
         __syncthreads();
         // the lanes of the sweep store their rows: N contiguous modes per row
         const auto dst = plane + (long)j0 * N + m;
-        if (m < N) fwd_sweep<C>(A, lus, dus, gr, N, m, j0, bpx, bpy, ipp, [&](int r, C v) { gst<C>(dst + r * N, v); });
+        if (m < N)
+            fwd_sweep<C>(A, lus, dus, gr, N, m, bpx, bpy, ipp, [&](int r) { return zsw<N1, N2>(r, m / N2, m % N2); },
+                         [&](int r, C v) { gst<C>(dst + r * N, v); });
     }
 }
 
@@ -409,33 +416,49 @@ __device__ __forceinline__ void zbwd2_body(const ZPar<C>& zp, int b, unsigned ch
         const int j0 = max(0, nym - (gi + 1) * G);
         const int gr = nym - gi * G - j0;
         __syncthreads();
-        ZS_STAGE(gr);
+        {
+            // stage rows (natural order in HBM) into the swizzled layout
+            const int np_ = gr * N;
+#define ZS_STORE2(K)                                                              \
+    {                                                                             \
+        const int e = (int)threadIdx.x + (K) * kZsThreads;                        \
+        if (e < np_) {                                                            \
+            const int r = e / N, mm = e - r * N;                                  \
+            A[zsw<N1, N2>(r, mm / N2, mm % N2)] = pr##K;                          \
+            lus[e] = pl##K;                                                       \
+        }                                                                         \
+    }
+            ZS_STORE2(0) ZS_STORE2(1) ZS_STORE2(2) ZS_STORE2(3)
+            ZS_STORE2(4) ZS_STORE2(5) ZS_STORE2(6) ZS_STORE2(7)
+#undef ZS_STORE2
+            if ((int)threadIdx.x < gr) dus[threadIdx.x] = pd;
+        }
         __syncthreads();
         if (gi + 1 < ngroups) {
             const int j0q = max(0, nym - (gi + 2) * G);
             ZS_PREFETCH(j0q, j0 - j0q, du, 0);
         }
-        if (m < N) bwd_sweep<C>(A, lus, dus, gr, N, m, j0, nym, is00, xnx, xny);
+        if (m < N)
+            bwd_sweep<C>(A, lus, dus, gr, N, m, is00, xnx, xny,
+                         [&](int r) { return zsw<N1, N2>(r, m / N2, m % N2); });
         __syncthreads();
         for (int it = threadIdx.x; it < gr * N2; it += kZsThreads) {
             const int r = it / N2, k2 = it - r * N2;
-            C* base = A + r * N + k2;
             C v[N1];
 #pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) v[k1] = base[k1 * N2];
+            for (int k1 = 0; k1 < N1; ++k1) v[k1] = A[zsw<N1, N2>(r, k1, k2)];
             dft_reg<1, N1>(v);
-            base[0] = v[0];
+            A[zsw<N1, N2>(r, 0, k2)] = v[0];
 #pragma unroll
-            for (int i1 = 1; i1 < N1; ++i1) base[i1 * N2] = twiddle<1>(v[i1], twl[i1 * k2]);
+            for (int i1 = 1; i1 < N1; ++i1) A[zsw<N1, N2>(r, i1, k2)] = twiddle<1>(v[i1], twl[i1 * k2]);
         }
         __syncthreads();
         const auto dst = plane + (long)j0 * N;
         for (int it = threadIdx.x; it < gr * N1; it += kZsThreads) {
             const int r = it / N1, i1 = it - r * N1;
-            const C* src = A + r * N + i1 * N2;
             C v[N2];
 #pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) v[k2] = src[k2];
+            for (int k2 = 0; k2 < N2; ++k2) v[k2] = A[zsw<N1, N2>(r, i1, k2)];
             dft_reg<1, N2>(v);
 #pragma unroll
             for (int i2 = 0; i2 < N2; ++i2) gst<C>(dst + r * N + i1 + N1 * i2, v[i2]);
