@@ -124,3 +124,19 @@ def test_mixed_config2_size():
     dfp.refresh_velocity_ghosts(g, U, V, W)
     md = dfp.Plan(g).max_divergence(U, V, W)
     print(f"  max div after FP32+refine {md:.3e} (FP64 bar {div_bound(g, U, V, W):.3e})", flush=True)
+
+
+def test_mixed_config3_size():
+    # 512x384x512, stretching 2.0, sigma 0.15: sweeps 1..3 against the C oracle and the FP64 GPU path
+    g = make_grid(512, 384, 512, 2.0)
+    print("case 512x384x512 FP32+refine", flush=True)
+    inputs = fields(g, 3, 0.15)
+    o = _oracle(g, inputs)
+    ref64 = _project(g, inputs, dfp.DFP_FP64)
+    for s in (1, 2, 3):
+        d = _project(g, inputs, dfp.DFP_FP32_REFINE, sweeps=s)
+        e_or = max(rel_linf(a, b) for a, b in zip(d, o))
+        e_64 = max(rel_linf(a, b) for a, b in zip(d, ref64))
+        print(f"  sweeps {s}: max over fields vs oracle {e_or:.3e}, vs FP64 GPU {e_64:.3e}", flush=True)
+        assert e_or <= MIX_TOL and e_64 <= MIX_TOL
+        del d
