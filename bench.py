@@ -190,29 +190,43 @@ def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
     if save_tuning:
         autotune.save_table(table, save_tuning)
     out = {"cases": 48, "distinct_sizes": len(sizes), "tuning_s": round(tune_s, 1)}
-    for prec, name in ((dfp.DFP_FP64, "fp64"), (dfp.DFP_FP32_REFINE, "fp32_refine")):
-        plans, cases = [], []
-        for sp in specs:
-            g = dfp.channel_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
-            plan = autotune.tuned_plan(g, prec, table)
-            f = [dfp.DeviceArray(sh) for sh in (g.shape_u, g.shape_v, g.shape_w)]
-            plan.synth(*f, seed=sp.seed, sigma=sp.sigma)
-            rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
-            P = dfp.DeviceArray(g.shape_p, zero=True)
-            plans.append(plan)
-            cases.append(Case(plan, f[0], f[1], f[2], rhs, P, DT, RK_C[2], True))
-        b = Batch(max_cases=48, graphs=True)
-        occ = autotune.tune_batch(b, cases)
+    e0, e1 = hipmem.Event(), hipmem.Event()
+
+    def time_batch(b, cases):
         b.project(cases)
         hipmem.synchronize()
-        e0, e1 = hipmem.Event(), hipmem.Event()
         e0.record()
         for _ in range(reps):
             b.project(cases, asynchronous=True)
         e1.record()
         e1.synchronize()
-        batch_ms = e0.elapsed_ms(e1) / reps
-        # the same projections one plan at a time (async, one stream)
+        return e0.elapsed_ms(e1) / reps
+
+    for prec, name in ((dfp.DFP_FP64, "fp64"), (dfp.DFP_FP32_REFINE, "fp32_refine")):
+        # two shape sets for the batch: each size's own tuned shape (fastest
+        # one plan at a time) and the library defaults (one z-split radix
+        # family and the split z pair, which group into fewer, fuller
+        # launches); the batch keeps whichever runs the whole call faster
+        by_shape = {}
+        for shapes in ("tuned", "default"):
+            plans, cases = [], []
+            for sp in specs:
+                g = dfp.channel_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
+                plan = autotune.tuned_plan(g, prec, table if shapes == "tuned" else None)
+                f = [dfp.DeviceArray(sh) for sh in (g.shape_u, g.shape_v, g.shape_w)]
+                plan.synth(*f, seed=sp.seed, sigma=sp.sigma)
+                rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
+                P = dfp.DeviceArray(g.shape_p, zero=True)
+                plans.append(plan)
+                cases.append(Case(plan, f[0], f[1], f[2], rhs, P, DT, RK_C[2], True))
+            b = Batch(max_cases=48, graphs=True)
+            occ = autotune.tune_batch(b, cases)
+            by_shape[shapes] = (time_batch(b, cases), occ, plans, cases, b)
+        shapes = min(by_shape, key=lambda k: by_shape[k][0])
+        batch_ms, occ = by_shape[shapes][0], by_shape[shapes][1]
+        # the baseline: the same projections one plan at a time with each
+        # size's tuned shape
+        cases = by_shape["tuned"][3]
         for cs in cases:
             cs.plan.project(cs.U, cs.V, cs.W, cs.rhs, cs.P, DT, RK_C[2], save_p=True, asynchronous=True)
         hipmem.synchronize()
@@ -225,10 +239,12 @@ def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
         seq_ms = e0.elapsed_ms(e1) / reps
         out[name] = {"batch_ms": round(batch_ms, 4), "projections_per_s": round(48 / (batch_ms / 1e3), 1),
                      "sequential_ms": round(seq_ms, 4), "speedup_vs_sequential": round(seq_ms / batch_ms, 3),
-                     "occupancy": occ["occupancy"]}
-        b.close()
-        for pl in plans:
-            pl.close()
+                     "occupancy": occ["occupancy"], "shapes": shapes,
+                     "batch_ms_by_shapes": {k: round(v[0], 4) for k, v in by_shape.items()}}
+        for v in by_shape.values():
+            v[4].close()
+            for pl in v[2]:
+                pl.close()
     return out
 
 

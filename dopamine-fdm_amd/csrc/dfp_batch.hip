@@ -91,8 +91,15 @@ struct dfp_batch {
     bool copy_pending = false;
     bool use_graphs = false;
     std::map<std::vector<long>, hipGraphExec_t> graphs;
+    // launches of one phase (independent case lists) run side by side on the
+    // caller's stream and these, forked and joined with events
+    std::vector<hipStream_t> aux;
+    hipEvent_t ev_fork = nullptr;
+    std::vector<hipEvent_t> ev_join;
     std::string last_error;
 };
+
+constexpr int kAuxStreams = 3;  // with the caller's stream: four, the process's hardware queues
 
 namespace {
 
@@ -124,7 +131,17 @@ struct Launch {
     size_t tab;   // offset of the case records
     int list;     // case list
     size_t lds;
+    int phase;    // launches of one phase are independent and may run concurrently
 };
+
+void release_streams(dfp_batch* b) {
+    for (hipStream_t st : b->aux) (void)hipStreamDestroy(st);
+    for (hipEvent_t ev : b->ev_join) (void)hipEventDestroy(ev);
+    if (b->ev_fork) (void)hipEventDestroy(b->ev_fork);
+    b->aux.clear();
+    b->ev_join.clear();
+    b->ev_fork = nullptr;
+}
 
 void destroy_graphs(dfp_batch* b) {
     for (auto& kv : b->graphs) (void)hipGraphExecDestroy(kv.second);
@@ -157,6 +174,7 @@ int dfp_batch_create(int device, int max_cases, dfp_batch** out) {
         (void)hipHostFree(b->h_buf);
         (void)hipFree(b->d_buf);
         if (b->copied) (void)hipEventDestroy(b->copied);
+        release_streams(b);
         delete b;
         return code;
     };
@@ -166,6 +184,15 @@ int dfp_batch_create(int device, int max_cases, dfp_batch** out) {
     if (hipHostMalloc((void**)&b->h_buf, b->L.total, hipHostMallocDefault) != hipSuccess) return bail(DFP_E_NOMEM);
     if (hipMalloc(&b->d_buf, b->L.total) != hipSuccess) return bail(DFP_E_NOMEM);
     if (hipEventCreateWithFlags(&b->copied, hipEventDisableTiming) != hipSuccess) return bail(DFP_E_HIP);
+    if (hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming) != hipSuccess) return bail(DFP_E_HIP);
+    for (int q = 0; q < kAuxStreams; ++q) {
+        hipStream_t st = nullptr;
+        hipEvent_t ev = nullptr;
+        if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) return bail(DFP_E_HIP);
+        b->aux.push_back(st);
+        if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return bail(DFP_E_HIP);
+        b->ev_join.push_back(ev);
+    }
     memset(b->h_buf, 0, b->L.total);
     *out = b;
     return DFP_OK;
@@ -176,9 +203,11 @@ int dfp_batch_destroy(dfp_batch* b) {
     (void)hipSetDevice(b->device);
     (void)hipStreamSynchronize(b->stream);
     destroy_graphs(b);
+    for (hipStream_t st : b->aux) (void)hipStreamSynchronize(st);
     (void)hipHostFree(b->h_buf);
     (void)hipFree(b->d_buf);
     (void)hipEventDestroy(b->copied);
+    release_streams(b);
     delete b;
     return DFP_OK;
 }
@@ -342,8 +371,9 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
     };
     const size_t lds_limit = plans[0]->lds_limit;
     std::vector<Launch> ls;
+    int phase = 0;
     auto add = [&](Launch::Kind k, int f32, int mode, int R, size_t tab, int list, size_t lds) {
-        ls.push_back(Launch{k, f32, mode, R, tab, list, lds_bucket(lds, lds_limit)});
+        ls.push_back(Launch{k, f32, mode, R, tab, list, lds_bucket(lds, lds_limit), phase});
     };
     // the x launches of a set of records, one per radix present; the z launches
     // for its fused and split members
@@ -359,6 +389,7 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
                 }
             if (!sel.empty()) add(k, f32, mode, R, tab, make_list(sel, w), lds);
         }
+        ++phase;
     };
     auto z_launches = [&](int f32, size_t tab, const std::vector<int>& recs, const std::vector<int>& Fs,
                           const std::vector<size_t>& w) {
@@ -377,12 +408,17 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
             sz[k].second.push_back(a);
             lsz[k] = std::max(lsz[k], z_lds_bytes(p, f32));
         }
+        // phase 1: the fused kernel and the forward halves of the split pairs;
+        // phase 2: the backward halves
         if (!fz.empty()) add(Launch::ZFUSED, f32, 0, 0, tab, make_list(fz, w), lf);
+        std::vector<int> lists(sz.size());
         for (size_t k = 0; k < sz.size(); ++k) {
-            const int li = make_list(sz[k].second, w);
-            add(Launch::ZF, f32, 0, sz[k].first, tab, li, lsz[k]);
-            add(Launch::ZB, f32, 0, sz[k].first, tab, li, lsz[k]);
+            lists[k] = make_list(sz[k].second, w);
+            add(Launch::ZF, f32, 0, sz[k].first, tab, lists[k], lsz[k]);
         }
+        ++phase;
+        for (size_t k = 0; k < sz.size(); ++k) add(Launch::ZB, f32, 0, sz[k].first, tab, lists[k], lsz[k]);
+        ++phase;
     };
     auto first = [](int k) { std::vector<int> v(k); for (int a = 0; a < k; ++a) v[a] = a; return v; };
     if (n64) {
@@ -396,16 +432,19 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         const std::vector<size_t> one(n32, 1);
         x_launches(Launch::XF, 1, 0, L.x32, all, R32, w32x, 0);
         add(Launch::M00, 1, 0, 0, L.m32, make_list(all, one), 0);
+        ++phase;
         z_launches(1, L.z32, all, F32, w32z);
         x_launches(Launch::XI, 1, 0, L.x32, all, R32, w32x, 1);
         for (int s = 0; s < max_sweeps; ++s) {
             const std::vector<int> pre = first(n_at_sweep[s]);
             x_launches(Launch::XF, 1, 2, L.x32s, pre, R32, w32x, 0);
             add(Launch::M00, 1, 0, 0, L.m32, make_list(pre, one), 0);
+            ++phase;
             z_launches(1, L.z32, pre, F32, w32z);
             x_launches(Launch::XI, 1, 1, L.x32s, pre, R32, w32x, 1);
         }
         if (ns) add(Launch::SAVE, 1, 0, 0, L.s32, make_list(save_rec, ws), 0);
+        ++phase;
     }
     add(Launch::CORR, 0, 0, 0, L.c, make_list(first(n), wc), 0);
     if (overflow) return bfail(b, DFP_E_UNSUPPORTED, "too many case lists in one call");
@@ -420,8 +459,9 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
     const int* d_counts = reinterpret_cast<const int*>(D + L.counts);
     const int* d_pool = reinterpret_cast<const int*>(D + L.lists);
     const int gx = b->n_cu * b->occ_x, gz = b->n_cu * b->occ_z, gs = b->n_cu * b->occ_s;
-    auto run_all = [&](hipStream_t st) -> hipError_t {
-        for (const Launch& l : ls) {
+    // one launch on stream st
+    auto run_one = [&](const Launch& l, hipStream_t st) -> hipError_t {
+        {
             const int* o = d_pool + (size_t)l.list * list_stride;
             const int* ix = o + b->cap + 1;
             const int* nc = d_counts + l.list;
@@ -468,6 +508,33 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         }
         return hipSuccess;
     };
+    // every launch in order; a phase with several launches forks onto the
+    // auxiliary streams and joins back into st before the next phase
+    auto run_all = [&](hipStream_t st) -> hipError_t {
+        size_t i = 0;
+        while (i < ls.size()) {
+            size_t k = i;
+            while (k < ls.size() && ls[k].phase == ls[i].phase) ++k;
+            if (k - i == 1) {
+                if (hipError_t e = run_one(ls[i], st)) return e;
+            } else {
+                const int ns = std::min((int)(k - i), kAuxStreams + 1);
+                if (hipError_t e = hipEventRecord(b->ev_fork, st)) return e;
+                for (int q = 1; q < ns; ++q)
+                    if (hipError_t e = hipStreamWaitEvent(b->aux[q - 1], b->ev_fork, 0)) return e;
+                for (size_t m = i; m < k; ++m) {
+                    const int q = (int)((m - i) % ns);
+                    if (hipError_t e = run_one(ls[m], q == 0 ? st : b->aux[q - 1])) return e;
+                }
+                for (int q = 1; q < ns; ++q) {
+                    if (hipError_t e = hipEventRecord(b->ev_join[q - 1], b->aux[q - 1])) return e;
+                    if (hipError_t e = hipStreamWaitEvent(st, b->ev_join[q - 1], 0)) return e;
+                }
+            }
+            i = k;
+        }
+        return hipSuccess;
+    };
 
     if (!b->use_graphs) {
         BCHK(b, run_all(b->stream));
@@ -481,6 +548,7 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
             key.push_back((long)l.tab);
             key.push_back(l.list);
             key.push_back((long)l.lds);
+            key.push_back(l.phase);
         }
         auto it = b->graphs.find(key);
         if (it == b->graphs.end()) {

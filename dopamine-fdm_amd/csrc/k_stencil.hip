@@ -56,10 +56,12 @@ constexpr int kCorrectK = 8;
 __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
 #pragma clang fp contract(off)
     const Geo& g = c.g;
-    const auto U = gptr(c.U);
-    const auto V = gptr(c.V);
-    const auto W = gptr(c.W);
-    const auto rhs = gptr(c.rhs);
+    // U, V, W and phi are distinct arrays: the loads of later planes may be
+    // issued ahead of the stores of earlier ones
+    auto* __restrict__ U = gptr(c.U);
+    auto* __restrict__ V = gptr(c.V);
+    auto* __restrict__ W = gptr(c.W);
+    const auto* __restrict__ rhs = gptr(c.rhs);
     const auto inv_dyg = gptr(g.inv_dyg);
     const auto inv_dzg = gptr(g.inv_dzg);
     const int ni = g.nxg - 2;
@@ -73,27 +75,37 @@ __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
     const long pu_k = (long)g.nx * g.nyg, pv_k = (long)g.nxg * g.ny, pw_k = (long)g.nxg * g.nyg;
     const bool do_u = i <= g.nx - 1, do_v = j <= g.ny - 1;
     const double idyg = do_v ? inv_dyg[j - 2] : 0.0;
-    long r = ridx(g, i, j, k0);
-    long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k0 - 1) * pu_k;
-    long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k0 - 1) * pv_k;
-    long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k0 - 1) * pw_k;
-    double phi = rhs[r];
+    const long r = ridx(g, i, j, k0);
+    const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k0 - 1) * pu_k;
+    const long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k0 - 1) * pv_k;
+    const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k0 - 1) * pw_k;
+    const int nk = k1 - k0 + 1;  // planes of this block, 1..kCorrectK
+    // Every load of the block's planes is issued before any store (the
+    // arrays are distinct): phi on planes k0..k1+1 (rhs_p holds the planes up
+    // to nzg), phi(i+1), phi(j+1), u, v, w.  Plane indices are clamped to the
+    // block's last plane and phi(j+1) to row j on the last row, so every
+    // address is in range; values of unused planes are never stored.
+    double pc[kCorrectK + 1], px[kCorrectK], py[kCorrectK], uo[kCorrectK], vo[kCorrectK], wo[kCorrectK];
+#pragma unroll
+    for (int q = 0; q <= kCorrectK; ++q) pc[q] = rhs[r + min(q, nk) * pr_k];
+    const long oy = do_v ? (long)(g.nxg - 1) : 0;
 #pragma unroll
     for (int q = 0; q < kCorrectK; ++q) {
+        const int qq = min(q, nk - 1);
+        px[q] = rhs[r + qq * pr_k + 1];
+        py[q] = rhs[r + qq * pr_k + oy];
+        uo[q] = U[iu + qq * pu_k];
+        vo[q] = V[iv + qq * pv_k];
+        wo[q] = W[iw + qq * pw_k];
+    }
+#pragma unroll
+    for (int q = 0; q < kCorrectK; ++q) {
+        if (q >= nk) break;
         const int k = k0 + q;
-        if (k > k1) break;
-        // phi(k+1) exists up to k = nzg-1 (rhs_p's last plane is nzg); only W uses it, for k <= nz-1
-        const bool do_w = k <= g.nz - 1;
-        const double phin = do_w ? rhs[r + pr_k] : 0.0;
-        if (do_u) U[iu] = U[iu] - (rhs[r + 1] - phi) * g.inv_dx;
-        if (do_v) V[iv] = V[iv] - (rhs[r + (g.nxg - 1)] - phi) * idyg;
-        if (do_w) W[iw] = W[iw] - (phin - phi) * inv_dzg[k - 2];
-        phi = do_w ? phin : phi;
-        r += pr_k;
-        iu += pu_k;
-        iv += pv_k;
-        iw += pw_k;
-        if (!do_w && k < k1) phi = rhs[r];
+        // phi(k+1) is used for W only, for k <= nz-1
+        if (do_u) U[iu + q * pu_k] = uo[q] - (px[q] - pc[q]) * g.inv_dx;
+        if (do_v) V[iv + q * pv_k] = vo[q] - (py[q] - pc[q]) * idyg;
+        if (k <= g.nz - 1) W[iw + q * pw_k] = wo[q] - (pc[q + 1] - pc[q]) * inv_dzg[k - 2];
     }
 }
 
