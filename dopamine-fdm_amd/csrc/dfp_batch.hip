@@ -94,6 +94,7 @@ struct dfp_batch {
     // launches of one phase (independent case lists) run side by side on the
     // caller's stream and these, forked and joined with events
     std::vector<hipStream_t> aux;
+    int n_par = 3;  // auxiliary streams in use (<= aux.size())
     hipEvent_t ev_fork = nullptr;
     std::vector<hipEvent_t> ev_join;
     std::string last_error;
@@ -234,6 +235,15 @@ int dfp_batch_use_graphs(dfp_batch* b, int enable) {
     if (!b) return DFP_E_ARG;
     b->use_graphs = enable != 0;
     if (!enable) destroy_graphs(b);
+    return DFP_OK;
+}
+
+int dfp_batch_set_parallel(dfp_batch* b, int aux_streams) {
+    if (!b) return DFP_E_ARG;
+    if (aux_streams < 0 || aux_streams > (int)b->aux.size())
+        return bfail(b, DFP_E_ARG, "auxiliary streams out of range (0..%d)", (int)b->aux.size());
+    if (aux_streams != b->n_par) destroy_graphs(b);
+    b->n_par = aux_streams;
     return DFP_OK;
 }
 
@@ -518,7 +528,7 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
             if (k - i == 1) {
                 if (hipError_t e = run_one(ls[i], st)) return e;
             } else {
-                const int ns = std::min((int)(k - i), kAuxStreams + 1);
+                const int ns = std::min((int)(k - i), b->n_par + 1);
                 if (hipError_t e = hipEventRecord(b->ev_fork, st)) return e;
                 for (int q = 1; q < ns; ++q)
                     if (hipError_t e = hipStreamWaitEvent(b->aux[q - 1], b->ev_fork, 0)) return e;

@@ -65,6 +65,7 @@ def lib() -> ctypes.CDLL:
         "dfp_batch_set_stream": ([vp, vp], i32),
         "dfp_batch_set_occupancy": ([vp, i32, i32, i32], i32),
         "dfp_batch_use_graphs": ([vp, i32], i32),
+        "dfp_batch_set_parallel": ([vp, i32], i32),
         "dfp_batch_project": ([vp, i32, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
                                ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(d), ctypes.POINTER(d),
                                ctypes.POINTER(i32), i32], i32),

@@ -56,6 +56,11 @@ class Batch:
     def use_graphs(self, enable: bool):
         self._check(lib().dfp_batch_use_graphs(self._h, 1 if enable else 0), "dfp_batch_use_graphs")
 
+    def set_parallel(self, aux_streams: int):
+        """Independent launches of one stage run on up to 1 + aux_streams
+        streams (0..3, default 3); 0 where several batches already overlap."""
+        self._check(lib().dfp_batch_set_parallel(self._h, int(aux_streams)), "dfp_batch_set_parallel")
+
     def set_stream(self, stream_handle):
         self._check(lib().dfp_batch_set_stream(self._h, stream_handle), "dfp_batch_set_stream")
 

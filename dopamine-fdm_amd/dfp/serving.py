@@ -122,6 +122,8 @@ class Server:
         self.batches = [Batch(device, max_cases=max_batch, graphs=graphs) for _ in range(n_streams)]
         for b, s in zip(self.batches, self.streams):
             b.set_stream(s.handle)
+            if n_streams > 1:
+                b.set_parallel(0)  # the batches themselves overlap on the server's streams
             if occupancy:
                 b.set_occupancy(*occupancy)
         self.inflight = [None] * n_streams  # (event, [requests]) per stream

@@ -168,6 +168,10 @@ int dfp_batch_set_stream(dfp_batch *batch, void *hip_stream);
 int dfp_batch_set_occupancy(dfp_batch *batch, int x_per_cu, int z_per_cu, int stencil_per_cu);
 /* replay the launch sequence from hipGraphs keyed by its stage list */
 int dfp_batch_use_graphs(dfp_batch *batch, int enable);
+/* independent launches of one stage (different z splits or z kernels) run
+ * side by side on up to 1 + aux_streams streams (0..3, default 3); callers
+ * that already overlap several batches on their own streams use 0 */
+int dfp_batch_set_parallel(dfp_batch *batch, int aux_streams);
 int dfp_batch_project(dfp_batch *batch, int n, dfp_plan *const *plans, double *const *U, double *const *V,
                       double *const *W, double *const *rhs_p, double *const *P, const double *dt,
                       const double *c, const int *flags, int batch_flags);

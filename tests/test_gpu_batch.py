@@ -168,3 +168,32 @@ def test_graph_replay_with_changing_list_lengths():
             for a, r in zip(devs[i], ref):
                 assert np.array_equal(a.download(), r), f"cases {sel}, case {i}"
     b.close()
+
+
+def test_parallel_phases_do_not_change_results():
+    """Stage launches on one stream or side by side on four give the same
+    bits (they touch disjoint cases); out-of-range stream counts are refused."""
+    specs = batched_cases(10, seed=21)
+    precs = [dfp.DFP_FP64 if i % 2 else dfp.DFP_FP32_REFINE for i in range(len(specs))]
+    outs = []
+    for par in (0, 3):
+        plans, cases = [], []
+        for i, (sp, prec) in enumerate(zip(specs, precs)):
+            g = make_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
+            # two z splits, so the x stages have two launches per phase as well as the z stages
+            plan = dfp.Plan(g, precision=prec, tuning=(4 if i % 4 < 2 else 2, 0, 0, 0))
+            dev, _ = _resident(g, sp.seed)
+            plans.append(plan)
+            cases.append(Case(plan, dev[0], dev[1], dev[2], dev[3], dev[4], DT, C3, True))
+        b = Batch(max_cases=len(cases), graphs=True)
+        b.set_parallel(par)
+        b.project(cases)
+        outs.append([[a.download() for a in (cs.U, cs.V, cs.W, cs.rhs, cs.P)] for cs in cases])
+        if par == 3:
+            for bad in (-1, 4):
+                with pytest.raises(dfp.DfpError):
+                    b.set_parallel(bad)
+        b.close()
+    for i in range(len(specs)):
+        for a, r in zip(outs[1][i], outs[0][i]):
+            assert np.array_equal(a, r), f"case {i}"
