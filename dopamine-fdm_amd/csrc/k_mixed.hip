@@ -36,16 +36,40 @@ __device__ __forceinline__ void mode00_body(const MPar& m) {
         const int len = min(kM00Chunk, nym - c0);
         for (int t = threadIdx.x; t < len; t += blockDim.x) {
             const int j = c0 + t;
+            // the parts in order, eight loads in flight at a time
+            const auto rp = rowpart + (long)j * nparts;
             double s = 0.0;
-            for (int q = 0; q < nparts; ++q) s += rowpart[(long)j * nparts + q];
+            int q = 0;
+            for (; q + 8 <= nparts; q += 8) {
+                double v[8];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) v[u] = rp[q + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) s += v[u];
+            }
+            for (; q < nparts; ++q) s += rp[q];
             s_in[t] = s / m.count;
             s_mul[t] = j > 0 ? dl[j - 1] * lu[(j - 1) * nzs] : 0.0;
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            for (int t = 0; t < len; ++t) {
-                yp = s_in[t] - s_mul[t] * yp;
-                s_out[t] = yp;
+            // eight rows' operands are read together, so one LDS round trip
+            // serves eight steps of the recurrence
+            for (int t0 = 0; t0 < len; t0 += 8) {
+                double a[8], mu[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int t = min(t0 + q, len - 1);
+                    a[q] = s_in[t];
+                    mu[q] = s_mul[t];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    if (t0 + q < len) {
+                        yp = a[q] - mu[q] * yp;
+                        s_out[t0 + q] = yp;
+                    }
+                }
             }
         }
         __syncthreads();
@@ -64,9 +88,23 @@ __device__ __forceinline__ void mode00_body(const MPar& m) {
         }
         __syncthreads();
         if (threadIdx.x == 0) {
-            for (int t = len - 1; t >= 0; --t) {
-                xn = c0 + t == nym - 1 ? s_in[t] * s_mul[t] : (s_in[t] - s_cu[t] * xn) * s_mul[t];
-                s_out[t] = xn;
+            for (int t1 = len - 1; t1 >= 0; t1 -= 8) {
+                double a[8], ip[8], cu[8];
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int t = max(t1 - q, 0);
+                    a[q] = s_in[t];
+                    ip[q] = s_mul[t];
+                    cu[q] = s_cu[t];
+                }
+#pragma unroll
+                for (int q = 0; q < 8; ++q) {
+                    const int t = t1 - q;
+                    if (t >= 0) {
+                        xn = c0 + t == nym - 1 ? a[q] * ip[q] : (a[q] - cu[q] * xn) * ip[q];
+                        s_out[t] = xn;
+                    }
+                }
             }
         }
         __syncthreads();
