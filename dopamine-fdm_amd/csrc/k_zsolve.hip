@@ -13,10 +13,12 @@
 // (mult = DL/D', D' <- D - mult*DU); LAPACK interchanges rows only in a few
 // low modes, which moves the solution at the 1e-15 level.
 //
-// k_zsolve_fwd: rows j = 0..nym-1 in groups of G: FFT of the rows in LDS,
-//   then per mode b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1); b' overwrites them.
+// k_zsolve_fwd: rows j = 0..nym-1 in groups of G: FFT of the rows (two
+//   register passes with one LDS exchange for n = N1*N2, else the staged LDS
+//   transform), then per mode b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1).
 // k_zsolve_bwd: rows j = nym-1..0: x(j) = (b'(j) - DU(j) x(j+1)) / D'(j), then
-//   the inverse FFT of the rows in LDS.
+//   the inverse FFT of the rows.
+// One lane per mode runs the elimination of its column of the group.
 // Both read and write the spectrum once (64 H bytes for the pair in FP64,
 // 32 H in the FP32 form, whose z FFTs run in FP32 while the elimination
 // stays FP64 on values rounded to FP32 only where stored).  While a
@@ -59,7 +61,8 @@ constexpr int kZsPf = 8;     // group points per thread held in flight
 constexpr int kZsAhead = 2;  // rows read ahead in the elimination sweeps
 
 // Shared layout of one group (G rows of length n):
-//   A[r*n + m]     the rows
+//   A[r*n + m]     the rows (the two-pass kernels hold element (i1, k2) of row
+//                  r at zsw(r, i1, k2), a permutation within the row)
 //   lus[r*n + m]   inverse pivot of mode m at row j0 + r
 //   dus[r]         DL(j-1) (forward) or DU(j) (backward) of row j = j0 + r
 // The prefetch registers are named variables (pr0..pr7, pl0..pl7, pd) so
