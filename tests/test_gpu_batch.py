@@ -197,3 +197,23 @@ def test_parallel_phases_do_not_change_results():
     for i in range(len(specs)):
         for a, r in zip(outs[1][i], outs[0][i]):
             assert np.array_equal(a, r), f"case {i}"
+
+
+@pytest.mark.parametrize("nzp,R", [(16, 2), (24, 2), (32, 1), (48, 2), (72, 2), (96, 2), (144, 2), (96, 1), (128, 1),
+                                   (18, 1), (54, 1), (256, 1)])
+def test_z_row_lengths_match_oracle(nzp, R):
+    """Every row length of the split z kernels (n = nzp/R: the two-pass
+    factorizations 8..128 and the staged transform for the others, 18, 54,
+    256) against the C oracle, FP64 and FP32+refine."""
+    g = make_grid(16, 12, nzp, 1.7)
+    print(f"z rows of {nzp // R} points (nzp {nzp}, R {R})", flush=True)
+    U, V, W, P, rhs = fields(g, nzp + R, 0.1)
+    o = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    oracle_project(g, *o, DT, C3, save_p=True)
+    for prec, tol in ((dfp.DFP_FP64, 1e-12), (dfp.DFP_FP32_REFINE, 1e-11)):
+        plan = dfp.Plan(g, precision=prec, tuning=(R, 0, 0, 1))
+        d = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+        plan.project(d[0], d[1], d[2], d[4], d[3], DT, C3, save_p=True)
+        plan.close()
+        for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+            assert rel_linf(a, b) <= tol, (prec, lab)
