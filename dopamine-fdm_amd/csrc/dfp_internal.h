@@ -122,6 +122,7 @@ struct ZPar {
     const C* tw;
     const C* tw2;        // w_n^e, e < n, of the two-pass transforms
     int nym, nzp, R, ntw, G, zero00, nblocks;
+    int nplanes;         // (kx, u) sub-planes: nxh * R
     int n1, n2;          // two-pass factorization n = n1*n2 (0: staged transform)
     FastDiv div_n;
 };

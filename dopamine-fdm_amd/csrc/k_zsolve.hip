@@ -300,171 +300,218 @@ __device__ __forceinline__ int zsw(int r, int i1, int k2) {
     return r * N + p;
 }
 
+// Rows of 24 or 32 points: a block takes P = 2 consecutive sub-planes, so the
+// sweeping wave has a mode in (nearly) every lane and a launch needs half the
+// blocks.  Group row r of plane p is the virtual row v = r*P + p (n points,
+// LDS layout of the two-pass transforms); lane t = p*n + m sweeps mode m of
+// plane p.  A partial last block loads its last plane again and stores
+// nothing of it.  Shorter rows keep one plane per block: with P = 64/n their
+// blocks hold more groups of fewer rows, and a launch covers fewer CUs; at
+// 8, 12 and 16 points that measured slower, while 24 and 32 gained.
+__host__ __device__ constexpr int zplanes(int n) { return n > 16 && n <= 32 ? 64 / n : 1; }
+
+// Point k of a thread's share of a group of gq rows, each row holding N
+// points of each of P planes: row r, plane p, position m.  Points past the
+// group are clamped to its last (loaded, never staged).
+template <int N, int P>
+__device__ __forceinline__ void zpoint(int k, int gq, int& r, int& p, int& m) {
+    const int e = min((int)threadIdx.x + k * kZsThreads, gq * P * N - 1);
+    const int v = e / N;
+    m = e - v * N;
+    r = v / P;
+    p = v - r * P;
+}
+
 template <typename C, int N1, int N2>
-__device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
+__device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int blk, unsigned char* smem_raw) {
     constexpr int N = N1 * N2;
+    constexpr int P = zplanes(N);
     constexpr int KI = kZsPf / N2;  // first-step items per thread and group
     C* smem = reinterpret_cast<C*>(smem_raw);
     const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G;
     const auto dl = gptr(zp.dl);
     const int nzs = nzp / 2 + 1;
-    const int kx = b / R;
-    const int u = b - kx * R;
+    const int b0 = blk * P;
     C* A = smem;
-    C* twl = A + G * N;
+    C* twl = A + G * P * N;
     double* lus = reinterpret_cast<double*>(twl + N);
-    double* dus = lus + G * N;
-    const auto plane = gptr(zp.spec) + (long)b * nym * N;
-    const auto lup = gptr(zp.lu) + (long)kx * nym * nzs;
+    double* dus = lus + G * P * N;
+    const auto spec = gptr(zp.spec);
+    const auto lu = gptr(zp.lu);
+    auto plane_of = [&](int p) { return P == 1 ? b0 : min(b0 + p, zp.nplanes - 1); };
     for (int e = threadIdx.x; e < N; e += kZsThreads) twl[e] = gld<C>(gptr(zp.tw2) + e);
     C pr[KI][N2];
     double pl[kZsPf];
     double pd;
+    // offset of row 0 of the pivots of position mm of plane p
+    auto lu_col = [&](int p, int mm) {
+        const int bp = plane_of(p), kx = bp / R, kz = R * mm + (bp - kx * R);
+        return (long)kx * nym * nzs + min(kz, nzp - kz);
+    };
     auto prefetch = [&](int j0, int gq) {
-        const auto src = plane + (long)j0 * N;
-        const int lim = gq * N1 - 1;
 #pragma unroll
         for (int k = 0; k < KI; ++k) {
-            const int it = min((int)threadIdx.x + k * kZsThreads, lim);
-            const int r = it / N1, i1 = it - r * N1;
+            int r, p, i1;
+            zpoint<N1, P>(k, gq, r, p, i1);
+            const auto src = spec + ((long)plane_of(p) * nym + j0 + r) * N + i1;
 #pragma unroll
-            for (int i2 = 0; i2 < N2; ++i2) pr[k][i2] = gld<C>(src + r * N + i1 + N1 * i2);
+            for (int i2 = 0; i2 < N2; ++i2) pr[k][i2] = gld<C>(src + N1 * i2);
         }
-        const auto lsrc = lup + (long)j0 * nzs;
-        const int lime = gq * N - 1;
 #pragma unroll
         for (int k = 0; k < kZsPf; ++k) {
-            const int e = min((int)threadIdx.x + k * kZsThreads, lime);
-            const int r = e / N, mm = e - r * N;
-            const int kz = R * mm + u;
-            pl[k] = lsrc[(long)r * nzs + min(kz, nzp - kz)];
+            int r, p, mm;
+            zpoint<N, P>(k, gq, r, p, mm);
+            pl[k] = lu[lu_col(p, mm) + (long)(j0 + r) * nzs];
         }
         const int jc = j0 + min((int)threadIdx.x, gq - 1) - 1;  // DL(j-1) of row j
         pd = dl[max(0, min(jc, nym - 2))];
     };
     prefetch(0, min(G, nym));
     double bpx = 0.0, bpy = 0.0, ipp = 0.0;
-    const int m = threadIdx.x;  // mode of this thread (m < N)
+    const int tp = (int)threadIdx.x / N, m = (int)threadIdx.x - tp * N;  // sweeping lane: plane, mode
+    const bool sweeps = tp < P;
+    const bool stores = sweeps && (P == 1 || b0 + tp < zp.nplanes);
     for (int j0 = 0; j0 < nym; j0 += G) {
         const int gr = min(G, nym - j0);
         __syncthreads();
 #pragma unroll
         for (int k = 0; k < KI; ++k) {
             const int it = (int)threadIdx.x + k * kZsThreads;
-            if (it < gr * N1) {
-                const int r = it / N1, i1 = it - r * N1;
-                C v[N2];
+            if (it < gr * P * N1) {
+                const int v = it / N1, i1 = it - v * N1;
+                C w[N2];
 #pragma unroll
-                for (int i2 = 0; i2 < N2; ++i2) v[i2] = pr[k][i2];
-                dft_reg<-1, N2>(v);
-                A[zsw<N1, N2>(r, i1, 0)] = v[0];
+                for (int i2 = 0; i2 < N2; ++i2) w[i2] = pr[k][i2];
+                dft_reg<-1, N2>(w);
+                A[zsw<N1, N2>(v, i1, 0)] = w[0];
 #pragma unroll
-                for (int k2 = 1; k2 < N2; ++k2) A[zsw<N1, N2>(r, i1, k2)] = twiddle<-1>(v[k2], twl[i1 * k2]);
+                for (int k2 = 1; k2 < N2; ++k2) A[zsw<N1, N2>(v, i1, k2)] = twiddle<-1>(w[k2], twl[i1 * k2]);
             }
         }
 #pragma unroll
         for (int k = 0; k < kZsPf; ++k) {
             const int e = (int)threadIdx.x + k * kZsThreads;
-            if (e < gr * N) lus[e] = pl[k];
+            if (e < gr * P * N) lus[e] = pl[k];
         }
         if ((int)threadIdx.x < gr) dus[threadIdx.x] = pd;
         __syncthreads();
         if (j0 + G < nym) prefetch(j0 + G, min(G, nym - j0 - G));
-        for (int it = threadIdx.x; it < gr * N2; it += kZsThreads) {
-            const int r = it / N2, k2 = it - r * N2;
-            C v[N1];
+        for (int it = threadIdx.x; it < gr * P * N2; it += kZsThreads) {
+            const int v = it / N2, k2 = it - v * N2;
+            C w[N1];
 #pragma unroll
-            for (int i1 = 0; i1 < N1; ++i1) v[i1] = A[zsw<N1, N2>(r, i1, k2)];
-            dft_reg<-1, N1>(v);
+            for (int i1 = 0; i1 < N1; ++i1) w[i1] = A[zsw<N1, N2>(v, i1, k2)];
+            dft_reg<-1, N1>(w);
 #pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) A[zsw<N1, N2>(r, k1, k2)] = v[k1];
+            for (int k1 = 0; k1 < N1; ++k1) A[zsw<N1, N2>(v, k1, k2)] = w[k1];
         }
         __syncthreads();
         // the lanes of the sweep store their rows: N contiguous modes per row
-        const auto dst = plane + (long)j0 * N + m;
-        if (m < N)
-            fwd_sweep<C>(A, lus, dus, gr, N, m, bpx, bpy, ipp, [&](int r) { return zsw<N1, N2>(r, m / N2, m % N2); },
-                         [&](int r, C v) { gst<C>(dst + r * N, v); });
+        if (sweeps) {
+            const auto dst = spec + ((long)plane_of(tp) * nym + j0) * N + m;
+            fwd_sweep<C>(
+                A, lus + tp * N, dus, gr, P * N, m, bpx, bpy, ipp,
+                [&](int r) { return zsw<N1, N2>(r * P + tp, m / N2, m % N2); },
+                [&](int r, C w) {
+                    if (stores) gst<C>(dst + r * N, w);
+                });
+        }
     }
 }
 
 template <typename C, int N1, int N2>
-__device__ __forceinline__ void zbwd2_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
+__device__ __forceinline__ void zbwd2_body(const ZPar<C>& zp, int blk, unsigned char* smem_raw) {
     constexpr int N = N1 * N2;
+    constexpr int P = zplanes(N);
     C* smem = reinterpret_cast<C*>(smem_raw);
-    const int n = N;
     const int nym = zp.nym, nzp = zp.nzp, R = zp.R, G = zp.G;
-    const FastDiv div_n = zp.div_n;
     const auto du = gptr(zp.du);
     const int nzs = nzp / 2 + 1;
-    const int kx = b / R;
-    const int u = b - kx * R;
+    const int b0 = blk * P;
     C* A = smem;
-    C* twl = A + G * N;
+    C* twl = A + G * P * N;
     double* lus = reinterpret_cast<double*>(twl + N);
-    double* dus = lus + G * N;
-    const auto plane = gptr(zp.spec) + (long)b * nym * N;
-    const auto lup = gptr(zp.lu) + (long)kx * nym * nzs;
+    double* dus = lus + G * P * N;
+    const auto spec = gptr(zp.spec);
+    const auto lu = gptr(zp.lu);
+    auto plane_of = [&](int p) { return P == 1 ? b0 : min(b0 + p, zp.nplanes - 1); };
     for (int e = threadIdx.x; e < N; e += kZsThreads) twl[e] = gld<C>(gptr(zp.tw2) + e);
     const int ngroups = (nym + G - 1) / G;
-    ZS_REGS;
+    C pr[kZsPf];
+    double pl[kZsPf];
+    double pd;
+    auto lu_col = [&](int p, int mm) {
+        const int bp = plane_of(p), kx = bp / R, kz = R * mm + (bp - kx * R);
+        return (long)kx * nym * nzs + min(kz, nzp - kz);
+    };
+    // rows j0..j0+gq-1 of the block's planes
+    auto prefetch = [&](int j0, int gq) {
+#pragma unroll
+        for (int k = 0; k < kZsPf; ++k) {
+            int r, p, mm;
+            zpoint<N, P>(k, gq, r, p, mm);
+            pr[k] = gld<C>(spec + ((long)plane_of(p) * nym + j0 + r) * N + mm);
+            pl[k] = lu[lu_col(p, mm) + (long)(j0 + r) * nzs];
+        }
+        const int jc = j0 + min((int)threadIdx.x, gq - 1);  // DU(j) of row j
+        pd = du[max(0, min(jc, nym - 2))];
+    };
     {
         const int j0q = max(0, nym - G);
-        ZS_PREFETCH(j0q, nym - j0q, du, 0);
+        prefetch(j0q, nym - j0q);
     }
     double xnx = 0.0, xny = 0.0;
-    const int m = threadIdx.x;
-    const bool is00 = zp.zero00 && kx == 0 && u == 0 && m == 0;
+    const int tp = (int)threadIdx.x / N, m = (int)threadIdx.x - tp * N;  // sweeping lane: plane, mode
+    const bool sweeps = tp < P;
+    const int bt = plane_of(tp);
+    const bool is00 = zp.zero00 && sweeps && bt == 0 && m == 0;  // kx = 0, u = 0
     for (int gi = 0; gi < ngroups; ++gi) {
         const int j0 = max(0, nym - (gi + 1) * G);
         const int gr = nym - gi * G - j0;
         __syncthreads();
-        {
-            // stage rows (natural order in HBM) into the swizzled layout
-            const int np_ = gr * N;
-#define ZS_STORE2(K)                                                              \
-    {                                                                             \
-        const int e = (int)threadIdx.x + (K) * kZsThreads;                        \
-        if (e < np_) {                                                            \
-            const int r = e / N, mm = e - r * N;                                  \
-            A[zsw<N1, N2>(r, mm / N2, mm % N2)] = pr##K;                          \
-            lus[e] = pl##K;                                                       \
-        }                                                                         \
-    }
-            ZS_STORE2(0) ZS_STORE2(1) ZS_STORE2(2) ZS_STORE2(3)
-            ZS_STORE2(4) ZS_STORE2(5) ZS_STORE2(6) ZS_STORE2(7)
-#undef ZS_STORE2
-            if ((int)threadIdx.x < gr) dus[threadIdx.x] = pd;
+        // stage rows (natural order in HBM) into the swizzled layout
+#pragma unroll
+        for (int k = 0; k < kZsPf; ++k) {
+            const int e = (int)threadIdx.x + k * kZsThreads;
+            if (e < gr * P * N) {
+                const int v = e / N, mm = e - v * N;
+                A[zsw<N1, N2>(v, mm / N2, mm % N2)] = pr[k];
+                lus[e] = pl[k];
+            }
         }
+        if ((int)threadIdx.x < gr) dus[threadIdx.x] = pd;
         __syncthreads();
         if (gi + 1 < ngroups) {
             const int j0q = max(0, nym - (gi + 2) * G);
-            ZS_PREFETCH(j0q, j0 - j0q, du, 0);
+            prefetch(j0q, j0 - j0q);
         }
-        if (m < N)
-            bwd_sweep<C>(A, lus, dus, gr, N, m, is00, xnx, xny,
-                         [&](int r) { return zsw<N1, N2>(r, m / N2, m % N2); });
+        if (sweeps)
+            bwd_sweep<C>(A, lus + tp * N, dus, gr, P * N, m, is00, xnx, xny,
+                         [&](int r) { return zsw<N1, N2>(r * P + tp, m / N2, m % N2); });
         __syncthreads();
-        for (int it = threadIdx.x; it < gr * N2; it += kZsThreads) {
-            const int r = it / N2, k2 = it - r * N2;
-            C v[N1];
+        for (int it = threadIdx.x; it < gr * P * N2; it += kZsThreads) {
+            const int v = it / N2, k2 = it - v * N2;
+            C w[N1];
 #pragma unroll
-            for (int k1 = 0; k1 < N1; ++k1) v[k1] = A[zsw<N1, N2>(r, k1, k2)];
-            dft_reg<1, N1>(v);
-            A[zsw<N1, N2>(r, 0, k2)] = v[0];
+            for (int k1 = 0; k1 < N1; ++k1) w[k1] = A[zsw<N1, N2>(v, k1, k2)];
+            dft_reg<1, N1>(w);
+            A[zsw<N1, N2>(v, 0, k2)] = w[0];
 #pragma unroll
-            for (int i1 = 1; i1 < N1; ++i1) A[zsw<N1, N2>(r, i1, k2)] = twiddle<1>(v[i1], twl[i1 * k2]);
+            for (int i1 = 1; i1 < N1; ++i1) A[zsw<N1, N2>(v, i1, k2)] = twiddle<1>(w[i1], twl[i1 * k2]);
         }
         __syncthreads();
-        const auto dst = plane + (long)j0 * N;
-        for (int it = threadIdx.x; it < gr * N1; it += kZsThreads) {
-            const int r = it / N1, i1 = it - r * N1;
-            C v[N2];
+        for (int it = threadIdx.x; it < gr * P * N1; it += kZsThreads) {
+            const int v = it / N1, i1 = it - v * N1;
+            const int r = v / P, p = v - r * P;
+            C w[N2];
 #pragma unroll
-            for (int k2 = 0; k2 < N2; ++k2) v[k2] = A[zsw<N1, N2>(r, i1, k2)];
-            dft_reg<1, N2>(v);
+            for (int k2 = 0; k2 < N2; ++k2) w[k2] = A[zsw<N1, N2>(v, i1, k2)];
+            dft_reg<1, N2>(w);
+            if (P == 1 || b0 + p < zp.nplanes) {
+                const auto dst = spec + ((long)(b0 + p) * nym + j0 + r) * N + i1;
 #pragma unroll
-            for (int i2 = 0; i2 < N2; ++i2) gst<C>(dst + r * N + i1 + N1 * i2, v[i2]);
+                for (int i2 = 0; i2 < N2; ++i2) gst<C>(dst + N1 * i2, w[i2]);
+            }
         }
     }
 }
@@ -549,7 +596,8 @@ ZPar<C> make_zpar(const dfp_plan* p, int f32_zero00) {
     if (z.n1) z.ntw = p->fz.n;
     z.G = p->g_zs;
     z.zero00 = f32_zero00;
-    z.nblocks = g.nxh * p->zr;
+    z.nplanes = g.nxh * p->zr;
+    z.nblocks = ceil_div(z.nplanes, z.n1 ? zplanes(p->fz.n) : 1);
     z.div_n = FastDiv(p->fz.n);
     return z;
 }
@@ -558,7 +606,8 @@ template ZPar<float2> make_zpar<float2>(const dfp_plan*, int);
 
 size_t z_lds_bytes(const dfp_plan* p, int f32) {
     const int G = p->g_zs, n = p->fz.n, ntw = p->zn1 ? n : twiddle_count(p->fz);
-    return (f32 ? sizeof(float2) : sizeof(double2)) * ((size_t)G * n + ntw) + sizeof(double) * ((size_t)G * n + G);
+    const size_t pts = (size_t)G * n * (p->zn1 ? zplanes(n) : 1);
+    return (f32 ? sizeof(float2) : sizeof(double2)) * (pts + ntw) + sizeof(double) * (pts + G);
 }
 
 int z_variant(const dfp_plan* p) { return p->zfused ? -1 : p->zn1 * 32 + p->zn2; }
@@ -613,19 +662,21 @@ template hipError_t launch_zsolve_batch<double2>(hipStream_t, int, int, const ZP
 template hipError_t launch_zsolve_batch<float2>(hipStream_t, int, int, const ZPar<float2>*, const int*, const int*,
                                                 const int*, int, size_t);
 
-// rows per group: the group fits the prefetch registers (kZsPf points per
-// thread), the in-place FFT capacity (2 butterflies per thread; not a limit of
-// the two-pass transforms) and, with the
-// twiddles and pivots, the device's LDS per block; 0 if one row does not fit
+// rows per group: the group (its rows in each of the block's planes) fits
+// the prefetch registers (kZsPf points per thread), the in-place FFT capacity
+// (2 butterflies per thread; not a limit of the two-pass transforms) and,
+// with the twiddles and pivots, the device's LDS per block; 0 if one row does
+// not fit
 int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit) {
     const int n = fz.n;
     if (n > kZsThreads) return 0;  // one mode per thread
     int n1, n2;
     const bool two = z_two_pass(n, n1, n2);
     const size_t ntw = two ? (size_t)n : (size_t)twiddle_count(fz);
-    int G = (two ? kZsPf * kZsThreads : std::min(kZsPf * kZsThreads, inplace_capacity(fz, kZsThreads, 2))) / n;
+    const int pn = n * (two ? zplanes(n) : 1);  // points of one group row
+    int G = (two ? kZsPf * kZsThreads : std::min(kZsPf * kZsThreads, inplace_capacity(fz, kZsThreads, 2))) / pn;
     G = std::max(0, std::min(G, nym));
-    auto bytes = [&](int g) { return sizeof(double2) * ((size_t)g * n + ntw) + sizeof(double) * ((size_t)g * n + g); };
+    auto bytes = [&](int g) { return sizeof(double2) * ((size_t)g * pn + ntw) + sizeof(double) * ((size_t)g * pn + g); };
     while (G > 0 && bytes(G) > lds_limit) --G;
     return G;
 }
