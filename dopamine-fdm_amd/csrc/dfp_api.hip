@@ -224,6 +224,9 @@ void free_plan_memory(dfp_plan* p) {
         (void)hipEventDestroy(pe.second.second);
     }
     for (auto ev : p->event_pool) (void)hipEventDestroy(ev);
+    if (p->fork_ev) (void)hipEventDestroy(p->fork_ev);
+    if (p->join_ev) (void)hipEventDestroy(p->join_ev);
+    if (p->aux) (void)hipStreamDestroy(p->aux);
 }
 
 int ensure_staging(dfp_plan* p) {
@@ -270,6 +273,26 @@ hipError_t z_solve(dfp_plan* p, int f32) {
     return f32 ? launch_zsolve32(p, +1) : launch_zsolve(p, +1);
 }
 
+// The (0,0)-mode solve (row sums -> pbar) and the z solves (the spectrum,
+// whose (0,0) mode they leave at zero) touch disjoint data, so the former runs
+// on the plan's second stream beside the latter and joins before the inverse x
+// transform reads pbar.  With per-kernel profiling on, both stay in order on
+// the plan's stream.
+int mode00_and_z(dfp_plan* p) {
+    if (p->prof) {
+        HIPCHK(p, launch_mode00(p, p->stream));
+        HIPCHK(p, z_solve(p, 1));
+        return DFP_OK;
+    }
+    HIPCHK(p, hipEventRecord(p->fork_ev, p->stream));
+    HIPCHK(p, hipStreamWaitEvent(p->aux, p->fork_ev, 0));
+    HIPCHK(p, launch_mode00(p, p->aux));
+    HIPCHK(p, hipEventRecord(p->join_ev, p->aux));
+    HIPCHK(p, z_solve(p, 1));
+    HIPCHK(p, hipStreamWaitEvent(p->stream, p->join_ev, 0));
+    return DFP_OK;
+}
+
 // FP32 form: one FP32 spectral solve with the (0,0) mode in FP64, then
 // p->sweeps refinement sweeps on the FP64 residual b - L_pin phi
 int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs, double* P, double dtc,
@@ -279,8 +302,7 @@ int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, 
     a.b_out = p->d_b;
     a.rowpart = p->d_rowpart;
     HIPCHK(p, launch_rfft_x32(p, from_fields ? 0 : 1, a));
-    HIPCHK(p, launch_mode00(p));
-    HIPCHK(p, z_solve(p, 1));
+    if (int rc = mode00_and_z(p)) return rc;
     // P = phi/(dt*c) is written by the last inverse transform
     HIPCHK(p, launch_irfft_x32(p, rhs, 0, p->d_pbar, p->d_part0, P, dtc, save_p && p->sweeps == 0));
     for (int s = 0; s < p->sweeps; ++s) {
@@ -292,8 +314,7 @@ int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, 
         r.idx2 = 1.0 / (p->dx * p->dx);
         r.idz2 = 1.0 / (p->dz * p->dz);
         HIPCHK(p, launch_rfft_x32(p, 2, r));
-        HIPCHK(p, launch_mode00(p));
-        HIPCHK(p, z_solve(p, 1));
+        if (int rc = mode00_and_z(p)) return rc;
         HIPCHK(p, launch_irfft_x32(p, rhs, 1, p->d_pbar, p->d_part0, P, dtc, save_p && s == p->sweeps - 1));
     }
     return DFP_OK;
@@ -516,6 +537,10 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
             hipMalloc(&p->d_part0, sizeof(double) * nparts) != hipSuccess ||
             hipMalloc(&p->d_pbar, sizeof(double) * G.nym) != hipSuccess)
             return bail(DFP_E_NOMEM);
+        if (hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming) != hipSuccess)
+            return bail(DFP_E_HIP);
     }
     *out = p;
     return DFP_OK;

@@ -199,6 +199,9 @@ struct dfp_plan {
     double *d_U = nullptr, *d_V = nullptr, *d_W = nullptr, *d_rhs = nullptr, *d_P = nullptr;
     int64_t sz_u = 0, sz_v = 0, sz_w = 0, sz_p = 0, sz_rhs = 0;
     hipStream_t stream = nullptr;
+    // FP32 form: second stream for the (0,0)-mode solve beside the z solves
+    hipStream_t aux = nullptr;
+    hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // profiling
     bool prof = false;
     std::vector<dfp::ProfSlot> slots;
@@ -239,7 +242,7 @@ hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a);
 hipError_t launch_zsolve32(dfp_plan* p, int dir);
 hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0, double* P,
                             double dtc, int save_p);
-hipError_t launch_mode00(dfp_plan* p);  // rowpart -> pbar of the plan's FP32 buffers
+hipError_t launch_mode00(dfp_plan* p, hipStream_t s);  // rowpart -> pbar of the plan's FP32 buffers
 int correct_blocks_z(const Geo& g);
 hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs);
 hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs);

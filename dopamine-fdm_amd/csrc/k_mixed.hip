@@ -139,9 +139,9 @@ MPar make_mpar(const dfp_plan* p) {
     return m;
 }
 
-hipError_t launch_mode00(dfp_plan* p) {
+hipError_t launch_mode00(dfp_plan* p, hipStream_t s) {
     const int slot = prof_begin(p, "mode00");
-    hipLaunchKernelGGL(k_mode00, dim3(1), dim3(256), 0, p->stream, make_mpar(p));
+    hipLaunchKernelGGL(k_mode00, dim3(1), dim3(256), 0, s, make_mpar(p));
     prof_end(p, slot);
     return hipGetLastError();
 }
