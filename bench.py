@@ -44,6 +44,20 @@ def algorithmic_bytes(N: int, H: int) -> dict:
     }
 
 
+def b_model(N: int, H: int) -> int:
+    """Bytes of one plain FP64 projection under the canonical fused model."""
+    return 80 * N + 128 * H
+
+
+def b_model_mixed(N: int, H: int, sweeps: int) -> int:
+    """Bytes of one FP32+refine projection as the implementation moves them:
+    the first solve reads u*, v*, w* (24N), writes b (8N), moves the FP32
+    spectrum through the x, z and inverse x kernels (8H + 32H + 8H) and writes
+    phi (8N); each sweep reads phi and b (16N), moves the spectrum again (48H)
+    and reads and writes phi (16N); the correction moves 56N."""
+    return 96 * N + 48 * H + sweeps * (32 * N + 48 * H)
+
+
 def measured_traffic(kernel: str):
     """HBM bytes per launch of a kernel from the committed rocprofv3 summary of
     this workload (2 x FETCH_SIZE + 1 x WRITE_SIZE, copy-calibrated), or None."""
@@ -173,6 +187,12 @@ def bench_mixed(g, K, Wm, sweeps, n_sets):
     return 3 * K / (ms / 1e3), {k: round(1e3 * m / max(n, 1), 2) for k, (n, m) in prof.items()}
 
 
+def form_gbs(bytes_per_projection: int, projections_per_s: float) -> dict:
+    gbs = bytes_per_projection * projections_per_s / 1e9
+    return {"B_model_bytes": int(bytes_per_projection), "achieved_GBs": round(gbs, 1),
+            "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4)}
+
+
 def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
     """48 uneven cases (nxp, nzp in {32..128}, nym in {32,64,96}): one batched
     call (persistent launch per stage) against the same 48 projections issued
@@ -238,7 +258,12 @@ def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
         e1.record()
         e1.synchronize()
         seq_ms = e0.elapsed_ms(e1) / reps
+        case_bytes = sum(b_model(cs.plan.g.n_cells, cs.plan.g.n_modes) if prec == dfp.DFP_FP64
+                         else b_model_mixed(cs.plan.g.n_cells, cs.plan.g.n_modes, cs.plan.sweeps) for cs in cases)
+        gbs = case_bytes / (batch_ms * 1e-3) / 1e9
         out[name] = {"batch_ms": round(batch_ms, 4), "projections_per_s": round(48 / (batch_ms / 1e3), 1),
+                     "B_model_bytes": int(case_bytes), "achieved_GBs": round(gbs, 1),
+                     "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
                      "sequential_ms": round(seq_ms, 4), "speedup_vs_sequential": round(seq_ms / batch_ms, 3),
                      "occupancy": occ["occupancy"], "shapes": shapes,
                      "batch_ms_by_shapes": {k: round(v[0], 4) for k, v in by_shape.items()}}
@@ -278,6 +303,16 @@ def bench_serving(n_requests: int = 600, rate: float = 200.0, n_ids: int = 64) -
                                                   "throughput_req_s": round(len(reqs2) / wall2, 1),
                                                   "batches": srv.launches - n0}
     best = min(v["wall_s"] for v in sat.values())
+
+    def req_bytes(q):
+        nx, ny, nz = q.shape
+        N, H = nx * ny * nz, (nx // 2 + 1) * ny * nz
+        return b_model(N, H) if q.precision == dfp.DFP_FP64 else b_model_mixed(N, H, srv.sweeps)
+
+    total_bytes = sum(req_bytes(q) for _, q in serving.poisson_requests(n_requests, rate, pool))
+    sat["B_model_bytes"] = int(total_bytes)
+    sat["achieved_GBs"] = round(total_bytes / best / 1e9, 1)
+    sat["frac_of_hbm_peak"] = round(total_bytes / best / 1e9 / HBM_PEAK_GBS, 4)
     # plain sequential: one request at a time, synchronous dfp_project on resident state
     t0 = time.perf_counter()
     for _, q in serving.poisson_requests(n_requests, rate, pool):
@@ -363,9 +398,9 @@ def main() -> int:
     kern = {k: {"launches": n, "avg_us": 1e3 * ms / max(n, 1)} for k, (n, ms) in prof.items()}
     dominant = max((k for k in kern if k in ab), key=lambda k: kern[k]["avg_us"] * kern[k]["launches"])
     dom_gbs = ab[dominant] / (kern[dominant]["avg_us"] * 1e-6) / 1e9
-    b_model = 80 * N + 128 * H
+    bm = b_model(N, H)
     proj_us = 1e3 * total_ms / (3 * K)
-    proj_gbs = b_model / (proj_us * 1e-6) / 1e9
+    proj_gbs = bm / (proj_us * 1e-6) / 1e9
 
     out = {
         "metric": "projections/s at 256x192x256 (FP64 vs FP32+refine) + serving p50/p99 latency",
@@ -390,7 +425,7 @@ def main() -> int:
                      "traffic": measured_traffic(dominant),
                      "algorithmic_bytes_per_launch": int(ab[dominant]),
                      "avg_launch_us": round(kern[dominant]["avg_us"], 2)},
-        "projection": {"us_per_projection": round(proj_us, 2), "B_model_bytes": int(b_model),
+        "projection": {"us_per_projection": round(proj_us, 2), "B_model_bytes": int(bm),
                        "achieved_GBs": round(proj_gbs, 1), "frac_of_hbm_peak": round(proj_gbs / HBM_PEAK_GBS, 4),
                        "kernels_us": {k: round(v["avg_us"], 2) for k, v in kern.items()},
                        "wall_s": round(wall, 3)},
@@ -402,6 +437,7 @@ def main() -> int:
         mixed = {"sweeps": args.sweeps}
         v32, k32 = bench_mixed(g, K, Wm, args.sweeps, n_mix)
         mixed["256x192x256"] = {"projections_per_s": round(v32, 2), "speedup_vs_fp64": round(v32 / value, 3),
+                                **form_gbs(b_model_mixed(N, H, args.sweeps), v32),
                                 "kernels_us": k32, "rel_err_vs_fp64_by_sweeps": mixed_errors(g, (0, 1, 2, 3), 4242)}
         gl = dfp.channel_grid(2 * NXP, 2 * NYM, 2 * NZP, ALPHA)
         pl = dfp.Plan(gl)
@@ -414,7 +450,10 @@ def main() -> int:
         del sl, rl, Pl
         v32l, k32l = bench_mixed(gl, kl, 1, args.sweeps, 3)
         mixed["512x384x512"] = {"fp64_projections_per_s": round(v64l, 2), "projections_per_s": round(v32l, 2),
-                                "speedup_vs_fp64": round(v32l / v64l, 3), "kernels_us": k32l,
+                                "speedup_vs_fp64": round(v32l / v64l, 3),
+                                "fp64": form_gbs(b_model(gl.n_cells, gl.n_modes), v64l),
+                                **form_gbs(b_model_mixed(gl.n_cells, gl.n_modes, args.sweeps), v32l),
+                                "kernels_us": k32l,
                                 "rel_err_vs_fp64_by_sweeps": mixed_errors(gl, (1, 2, 3), 4343)}
         out["mixed"] = mixed
     if not args.no_batched:

@@ -101,6 +101,7 @@ struct XPar {
     const C* tw;         // x stages | R2C packing (nxp/2+1) | first z stage (nz2*(R-1))
     C* spec;
     int ntw, R, T, nbx;  // stage-twiddle count, z split radix, columns per block, blocks per row
+    int strip;           // column blocks per strip of the workgroup order (nbx: row-major)
     FastDiv div_nxp, div_t, div_half;
     XArgs a;             // forward kernel inputs
     double* rhs;         // inverse: phi out (and in, when accumulating)

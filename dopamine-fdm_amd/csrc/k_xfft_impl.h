@@ -38,6 +38,22 @@ namespace dfp {
 
 constexpr int kXThreads = 512;
 
+// Tile (column block bx, y row j) of the L-th workgroup of a case: strips of
+// S column blocks, rows j in order within a strip.  Consecutive workgroups
+// (which xcd_remap keeps on one XCD and its L2) then read the rows beside
+// their own (the v* and phi halos of the forward kernel) and the column
+// blocks beside them (the w* and phi halos; the other half of each 128-byte
+// spectrum line) shortly after another workgroup did.  S = nbx is plain
+// row-major order.
+__device__ __forceinline__ void xtile(int L, int nbx, int nym, int S, int& bx, int& j) {
+    const int sw = S * nym;
+    const int s = L / sw;
+    const int r = L - s * sw;
+    const int w = min(S, nbx - s * S);  // column blocks of this strip
+    j = r / w;
+    bx = s * S + (r - j * w);
+}
+
 // ---------------------------------------------------------------------------
 // divergence + x R2C + first z stage
 // ---------------------------------------------------------------------------
@@ -197,8 +213,9 @@ template <int MODE, typename C, int R>
 __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XPar<C> q) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int L = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int j = L / gridDim.x;
-    xfwd_body<MODE, C, R>(q, L - j * gridDim.x, j, smem_raw);
+    int bx, j;
+    xtile(L, gridDim.x, gridDim.y, q.strip, bx, j);
+    xfwd_body<MODE, C, R>(q, bx, j, smem_raw);
 }
 
 // case of work item w: the last c with off[c] <= w (off[0] = 0, off[nc] = total)
@@ -224,9 +241,9 @@ __global__ void __launch_bounds__(kXThreads) kb_div_rfft_x(const XPar<C>* __rest
         if (w >= total) break;
         const int c = xfind_case(off, nc, w);
         const XPar<C>& q = cs[idx[c]];
-        const int li = w - off[c];
-        const int j = li / q.nbx;
-        xfwd_body<MODE, C, R>(q, li - j * q.nbx, j, smem_raw);
+        int bx, j;
+        xtile(w - off[c], q.nbx, q.g.nym, q.strip, bx, j);
+        xfwd_body<MODE, C, R>(q, bx, j, smem_raw);
         __syncthreads();
     }
 }
@@ -384,8 +401,9 @@ template <typename C, int ACCUM, int R>
 __global__ void __launch_bounds__(kXThreads) k_irfft_x(XPar<C> q) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int L = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
-    const int j = L / gridDim.x;
-    xinv_body<C, ACCUM, R>(q, L - j * gridDim.x, j, smem_raw);
+    int bx, j;
+    xtile(L, gridDim.x, gridDim.y, q.strip, bx, j);
+    xinv_body<C, ACCUM, R>(q, bx, j, smem_raw);
 }
 
 template <typename C, int ACCUM, int R>
@@ -400,9 +418,9 @@ __global__ void __launch_bounds__(kXThreads) kb_irfft_x(const XPar<C>* __restric
         if (w >= total) break;
         const int c = xfind_case(off, nc, w);
         const XPar<C>& q = cs[idx[c]];
-        const int li = w - off[c];
-        const int j = li / q.nbx;
-        xinv_body<C, ACCUM, R>(q, li - j * q.nbx, j, smem_raw);
+        int bx, j;
+        xtile(w - off[c], q.nbx, q.g.nym, q.strip, bx, j);
+        xinv_body<C, ACCUM, R>(q, bx, j, smem_raw);
         __syncthreads();
     }
 }
