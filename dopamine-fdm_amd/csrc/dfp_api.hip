@@ -127,7 +127,10 @@ struct Shape {
 constexpr int kRadices[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
 
 // x columns per workgroup: the largest divisor of nz2 within about 2048
-// points (or the caller's), if the x kernels fit; 0 if none does
+// points (from nxp = 512 up: as many as the LDS transform holds, 4096 when
+// every x stage is radix 4, for 64-byte rather than 32-byte spectrum
+// segments; measured at 512x384x512 FP64 3.90 -> 3.58 ms, FP32+refine
+// 6.12 -> 5.47 ms) or the caller's, if the x kernels fit; 0 if none does
 int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, int want) {
     const int M = nxp / 2, nz2 = nzp / sh.R;
     auto fits = [&](int T) {
@@ -136,7 +139,8 @@ int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, in
                cs * (pts * (M + 1) + twiddle_count(sh.fx) + M + 1 + (size_t)nz2 * (sh.R - 1)) + 512 <= lds_limit;
     };
     if (want > 0) return (nz2 % want == 0 && fits(want)) ? want : 0;
-    for (int t = std::max(1, std::min(nz2, 2048 / (sh.R * M))); t >= 1; --t)
+    const int pts = nxp >= 512 ? inplace_capacity(sh.fx, 512, 2) : 2048;
+    for (int t = std::max(1, std::min(nz2, pts / (sh.R * M))); t >= 1; --t)
         if (nz2 % t == 0 && fits(t)) return t;
     return 0;
 }

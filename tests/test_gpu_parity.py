@@ -159,3 +159,22 @@ def test_gpu_matches_golden_fixtures(path):
     plan.project(U, V, W, rhs, P, float(f["dt"]), float(f["c"]), save_p=True)
     for lab, x, key in (("U", U, "U_out"), ("V", V, "V_out"), ("W", W, "W_out"), ("P", P, "P_out"), ("phi", rhs, "rhs_out")):
         assert rel_linf(x, f[key]) <= TOL, lab
+
+
+@pytest.mark.parametrize("nxp,nym,nzp", [(512, 10, 192), (576, 6, 144)])
+def test_wide_x_strip_order(nxp, nym, nzp):
+    """From nxp = 512 the x kernels walk their tiles in strips of 8 column
+    blocks (12 and 36 blocks here, so the last strip is partial): every tile
+    is still visited once, FP64 and FP32+refine against the C oracle."""
+    g = make_grid(nxp, nym, nzp, 1.8)
+    print(f"wide x {nxp}x{nym}x{nzp}", flush=True)
+    U, V, W, P, rhs = fields(g, nxp + nzp, 0.1)
+    o = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    oracle_project(g, *o, DT, C3, save_p=True)
+    for prec, tol in ((dfp.DFP_FP64, TOL), (dfp.DFP_FP32_REFINE, 1e-11)):
+        plan = dfp.Plan(g, precision=prec)
+        d = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+        plan.project(d[0], d[1], d[2], d[4], d[3], DT, C3, save_p=True)
+        plan.close()
+        for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+            assert rel_linf(a, b) <= tol, (prec, lab)
