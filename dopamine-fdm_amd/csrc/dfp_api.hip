@@ -127,19 +127,25 @@ struct Shape {
 constexpr int kRadices[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
 
 // x columns per workgroup: the largest divisor of nz2 within about 2048
-// points (from nxp = 512 up: as many as the LDS transform holds, 4096 when
-// every x stage is radix 4, for 64-byte rather than 32-byte spectrum
-// segments; measured at 512x384x512 FP64 3.90 -> 3.58 ms, FP32+refine
-// 6.12 -> 5.47 ms) or the caller's, if the x kernels fit; 0 if none does
+// points, or the caller's, if the x kernels fit; 0 if none does.  Where the
+// x lines are long (nxp >= 512, or nxp >= 256 in the FP32 form, whose
+// transforms pair their radix-2 butterflies) the workgroup takes as many
+// points as its LDS transform holds (4096 when every stage is radix 4, or
+// radix 2 paired), so its spectrum segments are 64 bytes rather than 32.
+// Measured at 512x384x512: FP64 3.90 -> 3.58 ms, FP32+refine 6.12 -> 5.47 ms;
+// at 256x192x256 FP32+refine 650 -> 628 us (FP64 there keeps 2048 points:
+// 4096 gained nothing, two workgroups per CU instead of three).
 int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, int want) {
     const int M = nxp / 2, nz2 = nzp / sh.R;
+    const bool f32 = cs == sizeof(float2);
+    const int cap = inplace_capacity(sh.fx, 512, 2, f32);
     auto fits = [&](int T) {
         const size_t pts = (size_t)sh.R * T;
-        return pts * M <= (size_t)inplace_capacity(sh.fx, 512, 2) &&
+        return pts * M <= (size_t)cap &&
                cs * (pts * (M + 1) + twiddle_count(sh.fx) + M + 1 + (size_t)nz2 * (sh.R - 1)) + 512 <= lds_limit;
     };
     if (want > 0) return (nz2 % want == 0 && fits(want)) ? want : 0;
-    const int pts = nxp >= 512 ? inplace_capacity(sh.fx, 512, 2) : 2048;
+    const int pts = nxp >= 512 || (f32 && nxp >= 256) ? cap : 2048;
     for (int t = std::max(1, std::min(nz2, pts / (sh.R * M))); t >= 1; --t)
         if (nz2 % t == 0 && fits(t)) return t;
     return 0;

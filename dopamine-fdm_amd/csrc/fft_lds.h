@@ -204,9 +204,13 @@ __device__ __forceinline__ double block_sum(double v, double* wsum) {
 
 // Transforms nl lines in LDS (line l, element e at buf[l*ld + e]) in place:
 // each thread holds the outputs of up to KMAX butterflies in registers
-// between a read phase and a write phase, so one LDS buffer suffices.  Requires nl * n / r <= KMAX * blockDim.x for every stage radix r.
+// between a read phase and a write phase, so one LDS buffer suffices.
+// Requires nl * n / r <= KMAX * blockDim.x for every stage radix r; with
+// PAIR2 a radix-2 stage with more butterflies than that puts two in each
+// register slot (b and b + KMAX * blockDim.x), which doubles what its stages
+// hold: nl * n / 2 <= 2 * KMAX * blockDim.x.
 // Ends with a barrier.
-template <int SIGN, int KMAX, typename C>
+template <int SIGN, int KMAX, typename C, bool PAIR2 = false>
 __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const C* tw) {
     using R = typename CplxT<C>::real;
     int ncur = d.n, s = 1;
@@ -216,6 +220,45 @@ __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const 
         const int nb = m * s;
         const int tot = nb * nl;
         const C* tws = tw + d.twoff[st];
+        if (PAIR2 && r == 2 && tot > KMAX * (int)blockDim.x) {
+            C v[KMAX][4];
+            int dst[KMAX][2];
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int b = threadIdx.x + (k + h * KMAX) * blockDim.x;
+                    dst[k][h] = -1;
+                    if (b < tot) {
+                        const int l = (int)d.nbdiv[st].div((uint32_t)b);
+                        const int bi = b - l * nb;
+                        const int p = (int)d.sdiv[st].div((uint32_t)bi);
+                        const int q = bi - p * s;
+                        const C* sl = buf + l * ld + q + s * p;
+                        dst[k][h] = l * ld + q + s * 2 * p;
+                        const C a0 = sl[0], a1 = sl[s * m];
+                        v[k][2 * h] = cadd(a0, a1);
+                        v[k][2 * h + 1] = twiddle<SIGN>(csub(a0, a1), tws[p]);
+                    }
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int k = 0; k < KMAX; ++k) {
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    if (dst[k][h] >= 0) {
+                        C* dl = buf + dst[k][h];
+                        dl[0] = v[k][2 * h];
+                        dl[s] = v[k][2 * h + 1];
+                    }
+                }
+            }
+            __syncthreads();
+            s *= r;
+            ncur = m;
+            continue;
+        }
         C v[KMAX][4];
         int dsto[KMAX];
 #pragma unroll
@@ -275,10 +318,14 @@ __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const 
 }
 
 // number of complex points one block can transform in place with KMAX
-// butterflies per thread and the smallest radix of the plan
-inline int inplace_capacity(const FftDesc& d, int threads, int kmax) {
+// butterflies per thread and the smallest radix of the plan (a radix-2
+// stage counts as 4 with PAIR2)
+inline int inplace_capacity(const FftDesc& d, int threads, int kmax, bool pair2 = false) {
     int rmin = 4;
-    for (int st = 0; st < d.nst; ++st) rmin = d.rad[st] < rmin ? d.rad[st] : rmin;
+    for (int st = 0; st < d.nst; ++st) {
+        const int r = d.rad[st] == 2 && pair2 ? 4 : d.rad[st];
+        rmin = r < rmin ? r : rmin;
+    }
     return kmax * threads * rmin;
 }
 

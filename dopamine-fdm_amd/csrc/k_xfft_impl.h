@@ -181,7 +181,8 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
         if (threadIdx.x == 0) gptr(a.rowpart)[(long)j * q.nbx + bx] = sb;
     }
     __syncthreads();
-    lds_fft_inplace<-1, 2>(A, nl, M, q.fd, twl);
+    // FP32 tiles hold up to 4096 points (radix-2 stages paired) for 64-byte spectrum segments
+    lds_fft_inplace<-1, 2, C, sizeof(C) == sizeof(float2)>(A, nl, M, q.fd, twl);
 
     // unpack the R2C modes of the R lines of each column, apply the first z stage
     for (int pidx = threadIdx.x; pidx < (M + 1) * nt; pidx += kXThreads) {
@@ -339,7 +340,7 @@ __device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsig
         }
     }
     __syncthreads();
-    lds_fft_inplace<1, 2>(A, nl, ld, q.fd, twl);
+    lds_fft_inplace<1, 2, C, sizeof(C) == sizeof(float2)>(A, nl, ld, q.fd, twl);
     const long pr_j = (long)(g.nxg - 1), pr_k = (long)(g.nxg - 1) * g.nym;
     const long pp_j = (long)g.nxg, pp_k = (long)g.nxg * g.nyg;
     const auto rr = gptr(q.rhs) + (long)j * pr_j;
