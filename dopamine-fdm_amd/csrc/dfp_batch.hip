@@ -7,7 +7,8 @@
 // host into one pinned staging buffer, copied to the device with one
 // asynchronous copy, and read by the kernels.  The kernels' own arguments are
 // then only addresses inside that device buffer, the case counts live in it
-// too, and the grid is fixed (workgroups per CU x CUs), so a launch sequence
+// too, and the grid is fixed (workgroups per CU x CUs, capped at what the
+// device holds at once of the kernel with its LDS), so a launch sequence
 // captured once in a hipGraph replays for any batch with the same stage list
 // and LDS sizes.  Each plan keeps its own spectrum scratch, so the cases of a
 // batch are independent; a plan may appear once per batch.
@@ -34,9 +35,36 @@
 #include <new>
 #include <tuple>
 
+#include <mutex>
+
 #include "dfp_internal.h"
 
 using namespace dfp;
+
+namespace dfp {
+
+int resident_limit(const void* kernel, int threads, size_t lds) {
+    static std::mutex mu;
+    static std::map<std::tuple<int, const void*, int, size_t>, int> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    const auto key = std::make_tuple(dev, kernel, threads, lds);
+    {
+        std::lock_guard<std::mutex> lock(mu);
+        auto it = cache.find(key);
+        if (it != cache.end()) return it->second;
+    }
+    int per_cu = 0, ncu = 0;
+    int cap = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, threads, lds) == hipSuccess &&
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && per_cu > 0 && ncu > 0)
+        cap = per_cu * ncu;
+    std::lock_guard<std::mutex> lock(mu);
+    cache[key] = cap;
+    return cap;
+}
+
+}  // namespace dfp
 
 namespace {
 

@@ -281,6 +281,17 @@ void prof_end(dfp_plan* p, int slot);
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+// Workgroups of a persistent (batched) launch: the caller's count, capped at
+// what the device holds at once of this kernel with this LDS.  A workgroup
+// beyond that would start only once another has finished all of its work
+// items, and then run its own items with the GPU mostly idle.
+int resident_limit(const void* kernel, int threads, size_t lds);  // per device, cached; 0 if unknown
+template <typename K>
+inline int resident_grid(K kernel, int grid, int threads, size_t lds) {
+    const int cap = resident_limit(reinterpret_cast<const void*>(kernel), threads, lds);
+    return cap > 0 && cap < grid ? cap : grid;
+}
+
 // The pointers in the case records all address global memory.  Saying so
 // lets the compiler use global (not flat) memory instructions where a record
 // is read from memory rather than from the kernel arguments (the batched

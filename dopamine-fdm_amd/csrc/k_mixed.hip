@@ -200,7 +200,7 @@ SPar make_spar(const dfp_plan* p, const double* rhs, double* P, double dtc) {
 
 hipError_t launch_save_p_batch(hipStream_t s, const SPar* cs, const int* off, const int* idx, const int* nc,
                                int grid) {
-    hipLaunchKernelGGL(kb_save_p, dim3(grid), dim3(256), 0, s, cs, off, idx, nc);
+    hipLaunchKernelGGL(kb_save_p, dim3(resident_grid(kb_save_p, grid, 256, 0)), dim3(256), 0, s, cs, off, idx, nc);
     return hipGetLastError();
 }
 

@@ -158,7 +158,7 @@ hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const do
 
 hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* idx, const int* nc,
                                 int grid) {
-    hipLaunchKernelGGL(kb_correct, dim3(grid), dim3(256), 0, s, cs, off, idx, nc);
+    hipLaunchKernelGGL(kb_correct, dim3(resident_grid(kb_correct, grid, 256, 0)), dim3(256), 0, s, cs, off, idx, nc);
     return hipGetLastError();
 }
 

@@ -463,12 +463,16 @@ template <int R, typename C>
 hipError_t launch_xfwd_batch_r(hipStream_t s, int mode, const XPar<C>* cs, const int* off, const int* idx,
                                const int* nc, int grid, size_t lds) {
     if (mode == 0) {
-        hipLaunchKernelGGL((kb_div_rfft_x<0, C, R>), dim3(grid), dim3(kXThreads), lds, s, cs, off, idx, nc);
+        const auto k = kb_div_rfft_x<0, C, R>;
+        hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kXThreads, lds)), dim3(kXThreads), lds, s, cs, off, idx, nc);
     } else if (mode == 1) {
-        hipLaunchKernelGGL((kb_div_rfft_x<1, C, R>), dim3(grid), dim3(kXThreads), lds, s, cs, off, idx, nc);
+        const auto k = kb_div_rfft_x<1, C, R>;
+        hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kXThreads, lds)), dim3(kXThreads), lds, s, cs, off, idx, nc);
     } else {
         if constexpr (sizeof(C) == sizeof(float2)) {
-            hipLaunchKernelGGL((kb_div_rfft_x<2, C, R>), dim3(grid), dim3(kXThreads), lds, s, cs, off, idx, nc);
+            const auto k = kb_div_rfft_x<2, C, R>;
+            hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kXThreads, lds)), dim3(kXThreads), lds, s, cs, off, idx,
+                               nc);
         } else {
             return hipErrorInvalidValue;
         }
@@ -480,10 +484,13 @@ template <int R, typename C>
 hipError_t launch_xinv_batch_r(hipStream_t s, int accum, const XPar<C>* cs, const int* off, const int* idx,
                                const int* nc, int grid, size_t lds) {
     if (!accum) {
-        hipLaunchKernelGGL((kb_irfft_x<C, 0, R>), dim3(grid), dim3(kXThreads), lds, s, cs, off, idx, nc);
+        const auto k = kb_irfft_x<C, 0, R>;
+        hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kXThreads, lds)), dim3(kXThreads), lds, s, cs, off, idx, nc);
     } else {
         if constexpr (sizeof(C) == sizeof(float2)) {
-            hipLaunchKernelGGL((kb_irfft_x<C, 1, R>), dim3(grid), dim3(kXThreads), lds, s, cs, off, idx, nc);
+            const auto k = kb_irfft_x<C, 1, R>;
+            hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kXThreads, lds)), dim3(kXThreads), lds, s, cs, off, idx,
+                               nc);
         } else {
             return hipErrorInvalidValue;
         }

@@ -240,7 +240,8 @@ template hipError_t launch_zfused<float2>(dfp_plan*, const ZPar<float2>&);
 template <typename C>
 hipError_t launch_zfused_batch(hipStream_t s, const ZPar<C>* cs, const int* off, const int* idx, const int* nc,
                                int grid, size_t lds) {
-    hipLaunchKernelGGL(kb_zfused<C>, dim3(grid), dim3(kZfThreads), lds, s, cs, off, idx, nc);
+    const auto k = kb_zfused<C>;
+    hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kZfThreads, lds)), dim3(kZfThreads), lds, s, cs, off, idx, nc);
     return hipGetLastError();
 }
 template hipError_t launch_zfused_batch<double2>(hipStream_t, const ZPar<double2>*, const int*, const int*,

@@ -642,18 +642,21 @@ hipError_t launch_zsolve32(dfp_plan* p, int dir) { return launch_zsolve_t<float2
 template <typename C>
 hipError_t launch_zsolve_batch(hipStream_t s, int dir, int variant, const ZPar<C>* cs, const int* off, const int* idx,
                                const int* nc, int grid, size_t lds) {
-    const dim3 g(grid), blk(kZsThreads);
+    const dim3 blk(kZsThreads);
+    auto go = [&](auto k) {
+        hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kZsThreads, lds)), blk, lds, s, cs, off, idx, nc);
+    };
     switch (variant) {
-#define DFP_ZV_CASE(a, c)                                                                                   \
-    case a * 32 + c:                                                                                        \
-        if (dir < 0) hipLaunchKernelGGL((kb_zsolve<C, -1, a, c>), g, blk, lds, s, cs, off, idx, nc);        \
-        else hipLaunchKernelGGL((kb_zsolve<C, 1, a, c>), g, blk, lds, s, cs, off, idx, nc);                 \
+#define DFP_ZV_CASE(a, c)                              \
+    case a * 32 + c:                                   \
+        if (dir < 0) go(kb_zsolve<C, -1, a, c>);       \
+        else go(kb_zsolve<C, 1, a, c>);                \
         break;
         DFP_ZV_LIST(DFP_ZV_CASE)
 #undef DFP_ZV_CASE
         default:
-            if (dir < 0) hipLaunchKernelGGL((kb_zsolve<C, -1, 0, 0>), g, blk, lds, s, cs, off, idx, nc);
-            else hipLaunchKernelGGL((kb_zsolve<C, 1, 0, 0>), g, blk, lds, s, cs, off, idx, nc);
+            if (dir < 0) go(kb_zsolve<C, -1, 0, 0>);
+            else go(kb_zsolve<C, 1, 0, 0>);
     }
     return hipGetLastError();
 }
