@@ -288,7 +288,8 @@ def bench_serving(n_requests: int = 600, rate: float = 200.0, n_ids: int = 64) -
     srv = serving.Server(hbm_budget=200e9, max_batch=32, n_streams=2, graphs=True, table=table)
     reqs, wall = serving.run_trace(srv, trace, realtime=True)
     live = {"requests": len(reqs), "rate_req_s": rate, "wall_s": round(wall, 3),
-            "throughput_req_s": round(len(reqs) / wall, 1), "batches": srv.launches,
+            "throughput_req_s": round(len(reqs) / wall, 1), "launches": srv.launches,
+            "alone": srv.solo_launches, "solo_cells": srv.solo_cells,
             "latency": serving.latency_stats(reqs), "resident_GB": round(srv.resident_bytes / 1e9, 2)}
     # saturation: the same requests all queued at once (fields and plans already
     # resident), replayed graphs and direct launches
@@ -296,12 +297,12 @@ def bench_serving(n_requests: int = 600, rate: float = 200.0, n_ids: int = 64) -
     for graphs in (True, False):
         for b in srv.batches:
             b.use_graphs(graphs)
-        n0 = srv.launches
+        n0, a0 = srv.launches, srv.solo_launches
         trace2 = serving.poisson_requests(n_requests, rate, pool)
         reqs2, wall2 = serving.run_trace(srv, trace2, realtime=False)
         sat["graphs" if graphs else "direct"] = {"wall_s": round(wall2, 3),
                                                   "throughput_req_s": round(len(reqs2) / wall2, 1),
-                                                  "batches": srv.launches - n0}
+                                                  "launches": srv.launches - n0, "alone": srv.solo_launches - a0}
     best = min(v["wall_s"] for v in sat.values())
 
     def req_bytes(q):

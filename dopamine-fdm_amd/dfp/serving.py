@@ -13,7 +13,8 @@ sweep count).  The server
   * batches the admitted requests into one ``dfp_batch_project`` call (one
     persistent launch per stage, hipGraph replay) on one of a few streams,
     so requests that arrive while others run are launched as soon as a
-    stream is free and overlap on the device;
+    stream is free and overlap on the device; a request for a large case
+    (``solo_cells``) takes a stream alone with its plan's own launches;
   * records arrival, launch and completion times per request.
 
 A case is in at most one in-flight batch (its plans own their scratch), so
@@ -108,8 +109,16 @@ def case_bytes(g, precisions=(DFP_FP64,)) -> int:
 
 class Server:
     def __init__(self, device: int = 0, hbm_budget: float = 200e9, max_batch: int = 32, n_streams: int = 2,
-                 graphs: bool = True, table: dict | None = None, sweeps: int = 1, occupancy=None):
+                 graphs: bool = True, table: dict | None = None, sweeps: int = 1, occupancy=None,
+                 solo_cells: int | None = 1 << 20):
         self.device = device
+        # requests for cases of at least solo_cells cells run alone through
+        # their plan's own launches (which fill the GPU by themselves) rather
+        # than inside a batch; None batches every request.  On the log-uniform
+        # mix of 32^3 .. 256x192x256 the default (about 1M cells) served all
+        # requests queued at once 3-5 % faster than batching every one.
+        self.solo_cells = solo_cells
+        self.solo_launches = 0
         self.budget = float(hbm_budget)
         self.max_batch = max_batch
         self.table = table
@@ -191,11 +200,17 @@ class Server:
                     self.finished.append(q)
                 self.inflight[k] = None
 
+    def _solo(self, q: Request) -> bool:
+        nx, ny, nz = q.shape
+        return self.solo_cells is not None and nx * ny * nz >= self.solo_cells
+
     def _launch(self, k: int):
         chosen, res, seen, skipped = [], [], set(), collections.deque()
         while self.pending and len(chosen) < self.max_batch:
             q = self.pending.popleft()
-            if q.case_id in seen:
+            if q.case_id in seen or (chosen and self._solo(q)):
+                # a large request waits for a stream of its own
+                seen.add(q.case_id)
                 skipped.append(q)
                 continue
             r = self._admit(q)
@@ -208,10 +223,25 @@ class Server:
             seen.add(q.case_id)
             chosen.append(q)
             res.append(r)
+            if self._solo(q):
+                break
         self.pending.extendleft(reversed(skipped))
         if not chosen:
             return False
         stream = self.streams[k]
+        if len(chosen) == 1 and self._solo(chosen[0]):
+            q, r = chosen[0], res[0]
+            plan = r.plan(q.precision)
+            plan.set_stream(stream.handle)
+            plan.synth(r.U, r.V, r.W, q.seed, q.sigma, asynchronous=True)
+            q.launched = time.perf_counter()
+            plan.project(r.U, r.V, r.W, r.rhs, r.P, DT, C3, save_p=q.save_p, asynchronous=True)
+            ev = Event()
+            ev.record(stream.handle)
+            self.inflight[k] = (ev, chosen, res)
+            self.launches += 1
+            self.solo_launches += 1
+            return True
         cases = []
         now = time.perf_counter()
         for q, r in zip(chosen, res):

@@ -37,7 +37,7 @@ def _last_per_case(trace):
 
 
 def _alone(q):
-    s = Server(max_batch=4, n_streams=1)
+    s = Server(max_batch=4, n_streams=1, solo_cells=None)
     r = Request(0, q.case_id, q.shape, q.alpha, q.seed, q.precision)
     s.submit(r)
     s.drain()
@@ -46,13 +46,19 @@ def _alone(q):
     return out
 
 
-def test_mix_equals_alone():
+@pytest.mark.parametrize("solo_cells", [None, 200_000])
+def test_mix_equals_alone(solo_cells):
+    # with solo_cells the larger requests of the mix take a stream alone (their
+    # plan's own launches), while alone every request here is batched
     pool, trace = _trace()
-    s = Server(max_batch=8, n_streams=2)
+    s = Server(max_batch=8, n_streams=2, solo_cells=solo_cells)
     reqs, wall = serving.run_trace(s, trace, realtime=False)
     assert all(q.done is not None for q in reqs)
     stats = serving.latency_stats(reqs)
-    print(f"mix of {len(reqs)} requests in {wall:.3f} s, {s.launches} batches, {stats}", flush=True)
+    print(f"mix of {len(reqs)} requests in {wall:.3f} s, {s.launches} launches ({s.solo_launches} alone), {stats}",
+          flush=True)
+    if solo_cells:
+        assert s.solo_launches > 0
     for cid, q in _last_per_case(trace).items():
         mixed = s.result(cid)
         alone = _alone(q)
