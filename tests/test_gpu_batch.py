@@ -217,3 +217,30 @@ def test_z_row_lengths_match_oracle(nzp, R):
         plan.close()
         for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
             assert rel_linf(a, b) <= tol, (prec, lab)
+
+
+def test_occupancy_beyond_residency_same_bits():
+    """Workgroups per CU far beyond what the device holds (the persistent
+    grids are capped at residency) and the smallest setting give the same
+    bits as the defaults."""
+    specs = batched_cases(6, seed=33)
+    precs = [dfp.DFP_FP64 if i % 2 else dfp.DFP_FP32_REFINE for i in range(len(specs))]
+    outs = []
+    for occ in (None, (16, 16, 32), (1, 1, 1)):
+        plans, cases = [], []
+        for sp, prec in zip(specs, precs):
+            g = make_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
+            plan = dfp.Plan(g, precision=prec)
+            dev, _ = _resident(g, sp.seed)
+            plans.append(plan)
+            cases.append(Case(plan, dev[0], dev[1], dev[2], dev[3], dev[4], DT, C3, True))
+        b = Batch(max_cases=len(cases), graphs=False)
+        if occ:
+            b.set_occupancy(*occ)
+        b.project(cases)
+        outs.append([[a.download() for a in (cs.U, cs.V, cs.W, cs.rhs, cs.P)] for cs in cases])
+        b.close()
+    for o in outs[1:]:
+        for i in range(len(specs)):
+            for a, r in zip(o[i], outs[0][i]):
+                assert np.array_equal(a, r), f"case {i}"
