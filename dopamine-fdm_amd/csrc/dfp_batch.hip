@@ -282,9 +282,13 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
                       const int* flags, int batch_flags) {
     if (!b) return DFP_E_ARG;
     if (n < 0 || n > b->cap) return bfail(b, DFP_E_ARG, "batch of %d cases (capacity %d)", n, b->cap);
-    if (batch_flags & ~DFP_ASYNC) return bfail(b, DFP_E_ARG, "unknown batch flag bits 0x%x", batch_flags);
+    if (batch_flags & ~(DFP_ASYNC | DFP_SOLVE_ONLY))
+        return bfail(b, DFP_E_ARG, "unknown batch flag bits 0x%x", batch_flags);
+    // solve only: rhs_p -> phi (+ P) as dfp_solve, no divergence, no correction
+    const bool solve = (batch_flags & DFP_SOLVE_ONLY) != 0;
     if (n == 0) return DFP_OK;
-    if (!plans || !U || !V || !W || !rhs_p || !dt || !c) return bfail(b, DFP_E_ARG, "null argument array");
+    if (!plans || !rhs_p || !dt || !c || (!solve && (!U || !V || !W)))
+        return bfail(b, DFP_E_ARG, "null argument array");
     BCHK(b, hipSetDevice(b->device));
     // validate and split by precision (FP32 cases by descending sweeps)
     std::vector<int> i64, i32;
@@ -294,7 +298,7 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         if (p->device != b->device) return bfail(b, DFP_E_ARG, "case %d: plan on device %d", i, p->device);
         for (int k = 0; k < i; ++k)
             if (plans[k] == p) return bfail(b, DFP_E_ARG, "cases %d and %d share a plan", k, i);
-        if (!U[i] || !V[i] || !W[i] || !rhs_p[i]) return bfail(b, DFP_E_ARG, "case %d: null field", i);
+        if (!rhs_p[i] || (!solve && (!U[i] || !V[i] || !W[i]))) return bfail(b, DFP_E_ARG, "case %d: null field", i);
         const int fl = flags ? flags[i] : 0;
         if (fl & ~(DFP_SAVE_P | DFP_FIELDS_ON_DEVICE))
             return bfail(b, DFP_E_ARG, "case %d: flags 0x%x (device fields, DFP_SAVE_P only)", i, fl);
@@ -332,7 +336,8 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         dfp_plan* p = plans[i];
         const int fl = flags ? flags[i] : 0;
         XPar<double2> q = make_xpar<double2>(p);
-        q.a.U = U[i]; q.a.V = V[i]; q.a.W = W[i];
+        if (solve) q.a.rhs = rhs_p[i];
+        else { q.a.U = U[i]; q.a.V = V[i]; q.a.W = W[i]; }
         q.rhs = rhs_p[i];
         q.save_p = (fl & DFP_SAVE_P) ? 1 : 0;
         q.P = q.save_p ? P[i] : nullptr;
@@ -352,7 +357,8 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         dfp_plan* p = plans[i];
         const int fl = flags ? flags[i] : 0;
         XPar<float2> q = make_xpar<float2>(p);
-        q.a.U = U[i]; q.a.V = V[i]; q.a.W = W[i];
+        if (solve) q.a.rhs = rhs_p[i];
+        else { q.a.U = U[i]; q.a.V = V[i]; q.a.W = W[i]; }
         q.a.b_out = p->d_b;
         q.a.rowpart = p->d_rowpart;
         q.rhs = rhs_p[i];
@@ -382,7 +388,7 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
             ++ns;
         }
     }
-    for (int a = 0; a < n; ++a) {
+    for (int a = 0; a < n && !solve; ++a) {
         const int i = a < n64 ? i64[a] : i32[a - n64];
         dfp_plan* p = plans[i];
         cc[a] = make_cpar(p, U[i], V[i], W[i], rhs_p[i]);
@@ -461,14 +467,14 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
     auto first = [](int k) { std::vector<int> v(k); for (int a = 0; a < k; ++a) v[a] = a; return v; };
     if (n64) {
         const std::vector<int> all = first(n64);
-        x_launches(Launch::XF, 0, 0, L.x64, all, R64, w64x, 0);
+        x_launches(Launch::XF, 0, solve ? 1 : 0, L.x64, all, R64, w64x, 0);
         z_launches(0, L.z64, all, F64, w64z);
         x_launches(Launch::XI, 0, 0, L.x64, all, R64, w64x, 1);
     }
     if (n32) {
         const std::vector<int> all = first(n32);
         const std::vector<size_t> one(n32, 1);
-        x_launches(Launch::XF, 1, 0, L.x32, all, R32, w32x, 0);
+        x_launches(Launch::XF, 1, solve ? 1 : 0, L.x32, all, R32, w32x, 0);
         add(Launch::M00, 1, 0, 0, L.m32, make_list(all, one), 0);
         ++phase;
         z_launches(1, L.z32, all, F32, w32z);
@@ -484,7 +490,7 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         if (ns) add(Launch::SAVE, 1, 0, 0, L.s32, make_list(save_rec, ws), 0);
         ++phase;
     }
-    add(Launch::CORR, 0, 0, 0, L.c, make_list(first(n), wc), 0);
+    if (!solve) add(Launch::CORR, 0, 0, 0, L.c, make_list(first(n), wc), 0);
     if (overflow) return bfail(b, DFP_E_UNSUPPORTED, "too many case lists in one call");
 
     // tables to the device (up to the last list used), then the launches (or their graph)

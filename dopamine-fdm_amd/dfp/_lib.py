@@ -13,6 +13,7 @@ DFP_FP32_REFINE = 1
 DFP_FIELDS_ON_DEVICE = 1
 DFP_SAVE_P = 2
 DFP_ASYNC = 4
+DFP_SOLVE_ONLY = 8
 
 _dp = ctypes.POINTER(ctypes.c_double)
 

@@ -11,7 +11,7 @@ from __future__ import annotations
 import ctypes
 from dataclasses import dataclass
 
-from ._lib import DFP_ASYNC, DFP_FIELDS_ON_DEVICE, DFP_SAVE_P, DfpError, lib
+from ._lib import DFP_ASYNC, DFP_FIELDS_ON_DEVICE, DFP_SAVE_P, DFP_SOLVE_ONLY, DfpError, lib
 from .projection import Plan, _ptr
 
 
@@ -64,7 +64,10 @@ class Batch:
     def set_stream(self, stream_handle):
         self._check(lib().dfp_batch_set_stream(self._h, stream_handle), "dfp_batch_set_stream")
 
-    def project(self, cases: list[Case], asynchronous: bool = False):
+    def project(self, cases: list[Case], asynchronous: bool = False, solve_only: bool = False):
+        """Project every case (one persistent launch per stage).  With
+        solve_only, each case's rhs holds the right-hand side and receives phi
+        (and P with save_p), as Plan.solve; U, V, W are not read (may be None)."""
         n = len(cases)
         if n > self.max_cases:
             raise ValueError(f"{n} cases exceed the batch capacity {self.max_cases}")
@@ -78,6 +81,9 @@ class Batch:
             ptrs = []
             for a, shape, name in ((cs.U, g.shape_u, "U"), (cs.V, g.shape_v, "V"), (cs.W, g.shape_w, "W"),
                                    (cs.rhs, g.shape_rhs, "rhs_p")):
+                if solve_only and name != "rhs_p" and a is None:
+                    ptrs.append(None)
+                    continue
                 ptr, dev = _ptr(a, shape, name)
                 if not dev:
                     raise ValueError(f"case {i}: {name} must be device-resident")
@@ -91,8 +97,8 @@ class Batch:
                 P[i] = pp
             dt[i], cc[i] = float(cs.dt), float(cs.c)
             fl[i] = DFP_FIELDS_ON_DEVICE | (DFP_SAVE_P if cs.save_p else 0)
-        self._check(lib().dfp_batch_project(self._h, n, plans, U, V, W, R, P, dt, cc, fl,
-                                            DFP_ASYNC if asynchronous else 0), "dfp_batch_project")
+        bf = (DFP_ASYNC if asynchronous else 0) | (DFP_SOLVE_ONLY if solve_only else 0)
+        self._check(lib().dfp_batch_project(self._h, n, plans, U, V, W, R, P, dt, cc, fl, bf), "dfp_batch_project")
 
     def close(self):
         if self._h:

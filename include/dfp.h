@@ -56,7 +56,8 @@ enum { DFP_FP64 = 0, DFP_FP32_REFINE = 1 };
 enum {
     DFP_FIELDS_ON_DEVICE = 1, /* array arguments are device pointers (resident mode)  */
     DFP_SAVE_P = 2,           /* also write P = phi/(dt*c) with its ghosts (rk_step 3) */
-    DFP_ASYNC = 4             /* do not synchronise the stream before returning       */
+    DFP_ASYNC = 4,            /* do not synchronise the stream before returning       */
+    DFP_SOLVE_ONLY = 8        /* dfp_batch_project: solve rhs_p -> phi as dfp_solve   */
 };
 
 /* error codes */
@@ -158,8 +159,13 @@ int dfp_synth_fields_ex(dfp_plan *plan, double *U, double *V, double *W, uint64_
  * grid sizes, either precision - with one persistent launch per stage on one
  * stream.  Fields are device pointers with each plan's extents; a plan may
  * appear once per call (it owns its spectral scratch).  flags[i] may hold
- * DFP_SAVE_P (and DFP_FIELDS_ON_DEVICE, implied); batch_flags DFP_ASYNC.
- * Results equal those of dfp_project on each plan, bit for bit. */
+ * DFP_SAVE_P (and DFP_FIELDS_ON_DEVICE, implied); batch_flags DFP_ASYNC and
+ * DFP_SOLVE_ONLY.  Results equal those of dfp_project on each plan, bit for
+ * bit; with DFP_SOLVE_ONLY, those of dfp_solve (rhs_p holds the right-hand
+ * side and receives phi, U/V/W are not read and may be null): the repeated
+ * solve_poisson_equation(skip_p_save=.True.) of a caller such as the
+ * reference's VOF preconditioner (vof_pressure.f90:707-731), for many cases
+ * in one call. */
 typedef struct dfp_batch dfp_batch;
 int dfp_batch_create(int device, int max_cases, dfp_batch **out);
 int dfp_batch_destroy(dfp_batch *batch);

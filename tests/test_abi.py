@@ -97,3 +97,15 @@ def test_precision_argument_is_checked():
         assert L.dfp_plan_create(ctypes.byref(s), bad, 0, ctypes.byref(h)) == -1
         assert not h.value
     assert L.dfp_set_refine_sweeps(None, 2) == -1
+
+
+def test_flag_values_match_the_header():
+    txt = open(_lib.HEADER_PATH).read()
+    vals = {k: int(v) for k, v in re.findall(r"\b(DFP_[A-Z0-9_]+)\s*=\s*(\d+)", txt)}
+    for name in ("DFP_FIELDS_ON_DEVICE", "DFP_SAVE_P", "DFP_ASYNC", "DFP_SOLVE_ONLY", "DFP_FP64", "DFP_FP32_REFINE"):
+        assert getattr(dfp, name) == vals[name], name
+
+
+def test_batch_rejects_unknown_batch_flags_without_a_device():
+    # argument checks come before any device call
+    assert dfp.lib().dfp_batch_project(None, 0, None, None, None, None, None, None, None, None, None, 0) != 0

@@ -244,3 +244,37 @@ def test_occupancy_beyond_residency_same_bits():
         for i in range(len(specs)):
             for a, r in zip(o[i], outs[0][i]):
                 assert np.array_equal(a, r), f"case {i}"
+
+
+def test_batch_solve_only_equals_plan_solve():
+    """DFP_SOLVE_ONLY: each case's rhs_p holds a right-hand side and receives
+    phi (P where asked), bit for bit as Plan.solve on the same plan; U, V, W
+    are neither read nor written (None here).  FP64 and FP32+refine cases in
+    one call; graph replays of the solve-only sequence give the same bits."""
+    specs = batched_cases(8, seed=47)
+    precs = [dfp.DFP_FP64 if i % 2 else dfp.DFP_FP32_REFINE for i in range(len(specs))]
+    plans, cases, hosts = [], [], []
+    for i, (sp, prec) in enumerate(zip(specs, precs)):
+        g = make_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
+        plan = dfp.Plan(g, precision=prec)
+        _, (U, V, W, rhs, P) = _resident(g, sp.seed)
+        rng = np.random.default_rng(sp.seed)
+        b = np.asfortranarray(rng.standard_normal(rhs.shape))
+        save_p = i % 3 == 0
+        d_rhs, d_P = dfp.DeviceArray.from_host(b), dfp.DeviceArray.from_host(P)
+        plans.append(plan)
+        hosts.append((b, P, save_p))
+        cases.append(Case(plan, None, None, None, d_rhs, d_P, DT, C3, save_p))
+    for graphs in (False, True):
+        bt = Batch(max_cases=len(cases), graphs=graphs)
+        for rep in range(2 if graphs else 1):
+            for cs, (b, P, _) in zip(cases, hosts):
+                cs.rhs.upload(b)
+                cs.P.upload(P)
+            bt.project(cases, solve_only=True)
+            for i, (plan, cs, (b, P, save_p)) in enumerate(zip(plans, cases, hosts)):
+                r = [b.copy(order="F"), P.copy(order="F")]
+                plan.solve(r[0], r[1], DT, C3, save_p=save_p)
+                assert np.array_equal(cs.rhs.download(), r[0]), f"graphs {graphs} rep {rep} case {i} phi"
+                assert np.array_equal(cs.P.download(), r[1]), f"graphs {graphs} rep {rep} case {i} P"
+        bt.close()
