@@ -267,6 +267,28 @@ def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
                      "sequential_ms": round(seq_ms, 4), "speedup_vs_sequential": round(seq_ms / batch_ms, 3),
                      "occupancy": occ["occupancy"], "shapes": shapes,
                      "batch_ms_by_shapes": {k: round(v[0], 4) for k, v in by_shape.items()}}
+        # repeated solves (rhs_p -> phi, as the reference's skip_p_save solves of
+        # a preconditioner): one solve-only batched call against plan.solve
+        # one case at a time, on the batch's shapes
+        bsolve, scases = by_shape[shapes][4], by_shape[shapes][3]
+        bsolve.project(scases, solve_only=True)
+        hipmem.synchronize()
+        e0.record()
+        for _ in range(reps):
+            bsolve.project(scases, solve_only=True, asynchronous=True)
+        e1.record()
+        e1.synchronize()
+        solve_ms = e0.elapsed_ms(e1) / reps
+        e0.record()
+        for _ in range(reps):
+            for cs in scases:
+                cs.plan.solve(cs.rhs, None, DT, RK_C[2], save_p=False, asynchronous=True)
+        e1.record()
+        e1.synchronize()
+        solve_seq_ms = e0.elapsed_ms(e1) / reps
+        out[name]["solve_only"] = {"batch_ms": round(solve_ms, 4), "solves_per_s": round(48 / (solve_ms / 1e3), 1),
+                                   "sequential_ms": round(solve_seq_ms, 4),
+                                   "speedup_vs_sequential": round(solve_seq_ms / solve_ms, 3)}
         for v in by_shape.values():
             v[4].close()
             for pl in v[2]:

@@ -153,11 +153,11 @@ class Plan:
         f = self._flags(du, dv, dw, dr)
         check(lib().dfp_divergence(self._h, pu, pv, pw, pr, f), "dfp_divergence", self._h)
 
-    def solve(self, rhs_p, P=None, dt=0.0, c=0.0, save_p=False):
+    def solve(self, rhs_p, P=None, dt=0.0, c=0.0, save_p=False, asynchronous=False):
         g = self.g
         pr, dr = _ptr(rhs_p, g.shape_rhs, "rhs_p")
         pp, dpp = _ptr(P, g.shape_p, "P")
-        f = self._flags(dr, dpp, save_p=save_p)
+        f = self._flags(dr, dpp, save_p=save_p, asynchronous=asynchronous)
         check(lib().dfp_solve(self._h, pr, pp, float(dt), float(c), f), "dfp_solve", self._h)
 
     def correct(self, U, V, W, rhs_p):
