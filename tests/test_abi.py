@@ -106,6 +106,8 @@ def test_flag_values_match_the_header():
         assert getattr(dfp, name) == vals[name], name
 
 
-def test_batch_rejects_unknown_batch_flags_without_a_device():
+def test_batch_project_rejects_a_null_batch_without_a_device():
     # argument checks come before any device call
     assert dfp.lib().dfp_batch_project(None, 0, None, None, None, None, None, None, None, None, None, 0) != 0
+    assert dfp.lib().dfp_batch_project(None, 0, None, None, None, None, None, None, None, None, None,
+                                       dfp.DFP_SOLVE_ONLY) != 0
