@@ -120,32 +120,35 @@ void dyy(const std::vector<double>& y, const std::vector<double>& yg, std::vecto
 // fused z kernel (whole (kx,u) sub-plane in LDS) or the split pair of z
 // kernels with G rows per group.
 struct Shape {
-    int R = 1, T = 1, G = 1, fused = 0;
+    int R = 1, T = 1, Ti = 1, G = 1, fused = 0;  // T, Ti: columns of the forward, inverse x kernels
     FftDesc fx{}, fz{};
 };
 
 constexpr int kRadices[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
 
-// x columns per workgroup: the largest divisor of nz2 within about 2048
-// points, or the caller's, if the x kernels fit; 0 if none does.  Where the
-// x lines are long (nxp >= 512, or nxp >= 256 in the FP32 form, whose
-// transforms pair their radix-2 butterflies) the workgroup takes as many
-// points as its LDS transform holds (4096 when every stage is radix 4, or
-// radix 2 paired), so its spectrum segments are 64 bytes rather than 32.
-// Measured at 512x384x512: FP64 3.90 -> 3.58 ms, FP32+refine 6.12 -> 5.47 ms;
-// at 256x192x256 FP32+refine 650 -> 628 us (FP64 there keeps 2048 points:
-// 4096 gained nothing, two workgroups per CU instead of three).
-int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, int want) {
+// x columns per workgroup of the forward (inv = 0) or inverse x kernel: the
+// largest divisor of nz2 within about 2048 points, or the caller's, if the
+// kernel fits; 0 if none does.  Where the x lines are long the workgroup takes
+// as many points as its LDS transform holds (4096 when every stage is radix 4,
+// or radix 2 paired), for longer spectrum segments: from nxp = 256 up in the
+// forward kernels (which pair their radix-2 butterflies) and in the FP32
+// inverse, from nxp = 512 up in the FP64 inverse.  Measured: 512x384x512 FP64
+// 3.90 -> 3.58 ms, FP32+refine 6.12 -> 5.47 ms; 256x192x256 FP32+refine
+// 650 -> 628 us, the FP64 forward kernel 113 -> 106 us (its inverse with 8
+// columns holds 66 KB of LDS, two workgroups per CU instead of three, and
+// lost 8 us, so it keeps 4).
+int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, int want, int inv) {
     const int M = nxp / 2, nz2 = nzp / sh.R;
     const bool f32 = cs == sizeof(float2);
-    const int cap = inplace_capacity(sh.fx, 512, 2, f32);
+    const bool pair = f32 || !inv;  // the kernels that pair radix-2 butterflies
+    const int cap = inplace_capacity(sh.fx, 512, 2, pair);
     auto fits = [&](int T) {
         const size_t pts = (size_t)sh.R * T;
         return pts * M <= (size_t)cap &&
                cs * (pts * (M + 1) + twiddle_count(sh.fx) + M + 1 + (size_t)nz2 * (sh.R - 1)) + 512 <= lds_limit;
     };
     if (want > 0) return (nz2 % want == 0 && fits(want)) ? want : 0;
-    const int pts = nxp >= 512 || (f32 && nxp >= 256) ? cap : 2048;
+    const int pts = nxp >= 512 || (nxp >= 256 && pair) ? cap : 2048;
     for (int t = std::max(1, std::min(nz2, pts / (sh.R * M))); t >= 1; --t)
         if (nz2 % t == 0 && fits(t)) return t;
     return 0;
@@ -170,8 +173,9 @@ int choose_shape(const dfp_grid* g, int f32, size_t lds_limit, const dfp_tuning*
         Shape sh;
         sh.R = R;
         if (!factor(g->nxp / 2, sh.fx) || !factor(g->nzp / R, sh.fz)) continue;
-        sh.T = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T);
-        if (sh.T == 0 || sh.fz.n > 256) continue;
+        sh.T = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 0);
+        sh.Ti = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 1);
+        if (sh.T == 0 || sh.Ti == 0 || sh.fz.n > 256) continue;
         if (allow_fused && zf_lds_bytes(g->nym, sh.fz.n, twiddle_count(sh.fz), f32) <= lds_limit) {
             Shape f = sh;
             f.fused = 1;
@@ -438,6 +442,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     p->fx = sh.fx;
     p->fz = sh.fz;
     p->tk_x = sh.T;
+    p->tk_xi = sh.Ti;
     p->g_zs = sh.G;
     p->zfused = sh.fused;
     if (!sh.fused) z_two_pass(sh.fz.n, p->zn1, p->zn2);
@@ -541,10 +546,11 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
 
     const int nz2 = G.nzp / zr;
     if (precision == DFP_FP32_REFINE) {
-        const size_t nparts = (size_t)ceil_div(nz2, p->tk_x);
+        const size_t nparts = (size_t)ceil_div(nz2, p->tk_x);     // forward kernel blocks per row
+        const size_t nparts_i = (size_t)ceil_div(nz2, p->tk_xi);  // inverse kernel blocks per row
         if (hipMalloc(&p->d_b, sizeof(double) * p->sz_rhs) != hipSuccess ||
             hipMalloc(&p->d_rowpart, sizeof(double) * nparts * G.nym) != hipSuccess ||
-            hipMalloc(&p->d_part0, sizeof(double) * nparts) != hipSuccess ||
+            hipMalloc(&p->d_part0, sizeof(double) * nparts_i) != hipSuccess ||
             hipMalloc(&p->d_pbar, sizeof(double) * G.nym) != hipSuccess)
             return bail(DFP_E_NOMEM);
         if (hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking) != hipSuccess ||

@@ -73,7 +73,7 @@ constexpr int kRadixList[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};  // z-split radices 
 
 // offsets (in bytes) of the tables inside the staging buffer
 struct Layout {
-    size_t x64, z64, x32, x32s, z32, m32, s32, c, counts, lists;
+    size_t x64, x64i, z64, x32, x32i, x32s, x32si, z32, m32, s32, c, counts, lists;  // *i: inverse x records
     size_t total;
 };
 
@@ -90,9 +90,12 @@ Layout layout_for(int cap) {
         return at;
     };
     L.x64 = take(sizeof(XPar<double2>) * cap);
+    L.x64i = take(sizeof(XPar<double2>) * cap);
     L.z64 = take(sizeof(ZPar<double2>) * cap);
     L.x32 = take(sizeof(XPar<float2>) * cap);
+    L.x32i = take(sizeof(XPar<float2>) * cap);
     L.x32s = take(sizeof(XPar<float2>) * cap);
+    L.x32si = take(sizeof(XPar<float2>) * cap);
     L.z32 = take(sizeof(ZPar<float2>) * cap);
     L.m32 = take(sizeof(MPar) * cap);
     L.s32 = take(sizeof(SPar) * cap);
@@ -318,6 +321,9 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
     unsigned char* H = b->h_buf;
     const Layout& L = b->L;
     auto* x64 = reinterpret_cast<XPar<double2>*>(H + L.x64);
+    auto* x64i = reinterpret_cast<XPar<double2>*>(H + L.x64i);
+    auto* x32i = reinterpret_cast<XPar<float2>*>(H + L.x32i);
+    auto* x32si = reinterpret_cast<XPar<float2>*>(H + L.x32si);
     auto* z64 = reinterpret_cast<ZPar<double2>*>(H + L.z64);
     auto* x32 = reinterpret_cast<XPar<float2>*>(H + L.x32);
     auto* x32s = reinterpret_cast<XPar<float2>*>(H + L.x32s);
@@ -329,7 +335,7 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
     int* pool = reinterpret_cast<int*>(H + L.lists);
     const size_t list_stride = 2 * (size_t)b->cap + 1;
 
-    std::vector<size_t> w64x(n64), w64z(n64), w32x(n32), w32z(n32), wc(n), ws;
+    std::vector<size_t> w64x(n64), w64xi(n64), w64z(n64), w32x(n32), w32xi(n32), w32z(n32), wc(n), ws;
     std::vector<int> R64(n64), F64(n64), R32(n32), F32(n32);
     for (int a = 0; a < n64; ++a) {
         const int i = i64[a];
@@ -343,8 +349,11 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         q.P = q.save_p ? P[i] : nullptr;
         q.dtc = q.save_p ? dt[i] * c[i] : 1.0;
         x64[a] = q;
+        xpar_columns(q, p, 1);
+        x64i[a] = q;
         z64[a] = make_zpar<double2>(p, 0);
-        w64x[a] = (size_t)q.nbx * p->geo.nym;
+        w64x[a] = (size_t)x64[a].nbx * p->geo.nym;
+        w64xi[a] = (size_t)x64i[a].nbx * p->geo.nym;
         w64z[a] = z64[a].nblocks;
         R64[a] = p->zr;
         F64[a] = p->zfused;
@@ -365,6 +374,8 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         q.pbar = p->d_pbar;
         q.part0 = p->d_part0;
         x32[a] = q;
+        x32i[a] = q;
+        xpar_columns(x32i[a], p, 1);
         XPar<float2> r = q;  // sweeps: residual forward, accumulating inverse
         r.a = XArgs{};
         r.a.rhs = rhs_p[i];
@@ -374,9 +385,12 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         r.a.idx2 = 1.0 / (p->dx * p->dx);
         r.a.idz2 = 1.0 / (p->dz * p->dz);
         x32s[a] = r;
+        x32si[a] = r;
+        xpar_columns(x32si[a], p, 1);
         z32[a] = make_zpar<float2>(p, 1);
         m32[a] = make_mpar(p);
         w32x[a] = (size_t)q.nbx * p->geo.nym;
+        w32xi[a] = (size_t)x32i[a].nbx * p->geo.nym;
         w32z[a] = z32[a].nblocks;
         R32[a] = p->zr;
         F32[a] = p->zfused;
@@ -469,7 +483,7 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         const std::vector<int> all = first(n64);
         x_launches(Launch::XF, 0, solve ? 1 : 0, L.x64, all, R64, w64x, 0);
         z_launches(0, L.z64, all, F64, w64z);
-        x_launches(Launch::XI, 0, 0, L.x64, all, R64, w64x, 1);
+        x_launches(Launch::XI, 0, 0, L.x64i, all, R64, w64xi, 1);
     }
     if (n32) {
         const std::vector<int> all = first(n32);
@@ -478,14 +492,14 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         add(Launch::M00, 1, 0, 0, L.m32, make_list(all, one), 0);
         ++phase;
         z_launches(1, L.z32, all, F32, w32z);
-        x_launches(Launch::XI, 1, 0, L.x32, all, R32, w32x, 1);
+        x_launches(Launch::XI, 1, 0, L.x32i, all, R32, w32xi, 1);
         for (int s = 0; s < max_sweeps; ++s) {
             const std::vector<int> pre = first(n_at_sweep[s]);
             x_launches(Launch::XF, 1, 2, L.x32s, pre, R32, w32x, 0);
             add(Launch::M00, 1, 0, 0, L.m32, make_list(pre, one), 0);
             ++phase;
             z_launches(1, L.z32, pre, F32, w32z);
-            x_launches(Launch::XI, 1, 1, L.x32s, pre, R32, w32x, 1);
+            x_launches(Launch::XI, 1, 1, L.x32si, pre, R32, w32xi, 1);
         }
         if (ns) add(Launch::SAVE, 1, 0, 0, L.s32, make_list(save_rec, ws), 0);
         ++phase;

@@ -102,6 +102,7 @@ struct XPar {
     C* spec;
     int ntw, R, T, nbx;  // stage-twiddle count, z split radix, columns per block, blocks per row
     int strip;           // column blocks per strip of the workgroup order (nbx: row-major)
+    int npart0;          // blocks per row of the inverse kernel: entries of part0
     FastDiv div_nxp, div_t, div_half;
     XArgs a;             // forward kernel inputs
     double* rhs;         // inverse: phi out (and in, when accumulating)
@@ -210,7 +211,8 @@ struct dfp_plan {
     std::vector<hipEvent_t> event_pool;
     std::string last_error;
     // launch shapes
-    int tk_x = 8;   // z columns per block of the x-transform kernels (zr lines each)
+    int tk_x = 8;   // z columns per block of the forward x kernel (zr lines each)
+    int tk_xi = 8;  // z columns per block of the inverse x kernel
     int g_zs = 4;   // rows per group of the split z kernels, or rows per FFT call of the fused one
     int zfused = 0; // 1: one fused z kernel (whole sub-plane in LDS); 0: the split pair
     int zn1 = 0, zn2 = 0;  // two-pass z transforms of the split pair (0: staged transform)
@@ -253,7 +255,9 @@ hipError_t launch_synth(dfp_plan* p, double* U, double* V, double* W, uint64_t s
 // parameter records and batched (persistent) launches: cs (records), off
 // (per-listed-case work offsets, off[0] = 0, off[*nc] = total items), idx
 // (listed case -> record) and the count *nc are in device memory
-template <typename C> XPar<C> make_xpar(const dfp_plan* p);
+template <typename C> XPar<C> make_xpar(const dfp_plan* p, int inv = 0);
+// set the column fields (T, nbx, strip, div_t) of q for the forward (inv = 0) or inverse x kernel
+template <typename C> void xpar_columns(XPar<C>& q, const dfp_plan* p, int inv);
 template <typename C> ZPar<C> make_zpar(const dfp_plan* p, int zero00);
 CPar make_cpar(const dfp_plan* p, double* U, double* V, double* W, const double* rhs);
 MPar make_mpar(const dfp_plan* p);

@@ -20,15 +20,10 @@ hipError_t launch_xinv_batch_r(hipStream_t s, int accum, const XPar<C>* cs, cons
 // host side: parameters, LDS sizes, launches
 // ---------------------------------------------------------------------------
 template <typename C>
-XPar<C> make_xpar(const dfp_plan* p) {
-    XPar<C> q{};
+void xpar_columns(XPar<C>& q, const dfp_plan* p, int inv) {
     const Geo& g = p->geo;
-    q.g = g;
-    q.o = p->ops;
-    q.fd = p->fx;
-    q.R = p->zr;
-    q.T = p->tk_x;
-    q.nbx = (g.nzp / p->zr) / p->tk_x;
+    q.T = inv ? p->tk_xi : p->tk_x;
+    q.nbx = (g.nzp / p->zr) / q.T;
     // A row of tiles moves about 24 * nxp * nzp bytes of u*, v*, w* (or phi and
     // b); from nxp = 512 up that exceeds an XCD's 4 MB L2 before the next row
     // reads it again as its halo, so the workgroups walk strips of 8 column
@@ -36,9 +31,23 @@ XPar<C> make_xpar(const dfp_plan* p) {
     // ran 4 % of a projection faster, FP64 unchanged; at 256x192x256
     // row-major order was 0.7 % faster).
     q.strip = g.nxp >= 512 ? std::min(q.nbx, 8) : q.nbx;
+    q.div_t = FastDiv(q.T);
+}
+template void xpar_columns<double2>(XPar<double2>&, const dfp_plan*, int);
+template void xpar_columns<float2>(XPar<float2>&, const dfp_plan*, int);
+
+template <typename C>
+XPar<C> make_xpar(const dfp_plan* p, int inv) {
+    XPar<C> q{};
+    const Geo& g = p->geo;
+    q.g = g;
+    q.o = p->ops;
+    q.fd = p->fx;
+    q.R = p->zr;
+    xpar_columns(q, p, inv);
+    q.npart0 = (g.nzp / p->zr) / p->tk_xi;
     q.ntw = twiddle_count(p->fx);
     q.div_nxp = FastDiv(g.nxp);
-    q.div_t = FastDiv(p->tk_x);
     q.div_half = FastDiv(p->fx.n / 2 + 1);
     q.scale = 1.0 / ((double)g.nxp * (double)g.nzp);
     q.dtc = 1.0;
@@ -51,12 +60,12 @@ XPar<C> make_xpar(const dfp_plan* p) {
     }
     return q;
 }
-template XPar<double2> make_xpar<double2>(const dfp_plan*);
-template XPar<float2> make_xpar<float2>(const dfp_plan*);
+template XPar<double2> make_xpar<double2>(const dfp_plan*, int);
+template XPar<float2> make_xpar<float2>(const dfp_plan*, int);
 
 // dynamic LDS of the forward (inv = 0) or inverse (inv = 1) x kernel of a plan
 size_t x_lds_bytes(const dfp_plan* p, int inv, int f32) {
-    const int R = p->zr, T = p->tk_x, nz2 = p->geo.nzp / R, M = p->fx.n;
+    const int R = p->zr, T = inv ? p->tk_xi : p->tk_x, nz2 = p->geo.nzp / R, M = p->fx.n;
     const size_t cs = f32 ? sizeof(float2) : sizeof(double2);
     return cs * ((size_t)R * T * (M + inv) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (R - 1));
 }
@@ -153,14 +162,14 @@ static hipError_t launch_xinv(dfp_plan* p, const XPar<C>& q) {
 }
 
 hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p) {
-    XPar<double2> q = make_xpar<double2>(p);
+    XPar<double2> q = make_xpar<double2>(p, 1);
     q.rhs = rhs; q.P = P; q.dtc = dtc; q.save_p = save_p;
     return launch_xinv<double2, 0>(p, q);
 }
 
 hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0, double* P,
                             double dtc, int save_p) {
-    XPar<float2> q = make_xpar<float2>(p);
+    XPar<float2> q = make_xpar<float2>(p, 1);
     q.rhs = rhs; q.pbar = pbar; q.part0 = part0;
     q.P = P; q.dtc = save_p ? dtc : 1.0; q.save_p = save_p;
     if (accumulate) return launch_xinv<float2, 1>(p, q);

@@ -110,7 +110,7 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
     const double inv_dy = inv_dyf[j];
     double mean0 = 0.0;
     if (MODE == 2 && j == 0) {
-        for (int t = 0; t < q.nbx; ++t) mean0 += gptr(a.part0)[t];
+        for (int t = 0; t < q.npart0; ++t) mean0 += gptr(a.part0)[t];
         mean0 = mean0 / ((double)nxp * (double)g.nzp);
     }
     double rsum = 0.0;
@@ -181,8 +181,8 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
         if (threadIdx.x == 0) gptr(a.rowpart)[(long)j * q.nbx + bx] = sb;
     }
     __syncthreads();
-    // FP32 tiles hold up to 4096 points (radix-2 stages paired) for 64-byte spectrum segments
-    lds_fft_inplace<-1, 2, C, sizeof(C) == sizeof(float2)>(A, nl, M, q.fd, twl);
+    // tiles of up to 4096 points (radix-2 stages paired): 64-byte FP32 and 128-byte FP64 spectrum segments
+    lds_fft_inplace<-1, 2, C, true>(A, nl, M, q.fd, twl);
 
     // unpack the R2C modes of the R lines of each column, apply the first z stage
     for (int pidx = threadIdx.x; pidx < (M + 1) * nt; pidx += kXThreads) {
@@ -231,7 +231,7 @@ __device__ __forceinline__ int xfind_case(const int* __restrict__ off, int nc, i
 }
 
 template <int MODE, typename C, int R>
-__global__ void __launch_bounds__(kXThreads) kb_div_rfft_x(const XPar<C>* __restrict__ cs, const int* __restrict__ off,
+__global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(4))) kb_div_rfft_x(const XPar<C>* __restrict__ cs, const int* __restrict__ off,
                                                            const int* __restrict__ idx, const int* __restrict__ pnc) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int nc = *pnc;
