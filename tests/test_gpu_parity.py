@@ -178,3 +178,41 @@ def test_wide_x_strip_order(nxp, nym, nzp):
         plan.close()
         for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
             assert rel_linf(a, b) <= tol, (prec, lab)
+
+
+def _sweep_shapes(n, seed):
+    """n seeded grid shapes with 2^a 3^b extents: nxp 4..576 (even), nym 2..80,
+    nzp 1..288, stretching 0.5..2.5; wide ones are kept thin so the oracle
+    stays quick."""
+    rng = np.random.default_rng(seed)
+    ext = sorted({2 ** a * 3 ** b for a in range(0, 10) for b in range(0, 3) if 2 ** a * 3 ** b <= 576})
+    out = []
+    while len(out) < n:
+        nxp = int(rng.choice([e for e in ext if e % 2 == 0 and e >= 4]))
+        nzp = int(rng.choice([e for e in ext if e <= 288]))
+        nym = int(rng.integers(2, 81))
+        if nxp * nym * nzp > 600_000:
+            continue
+        out.append((nxp, nym, nzp, float(rng.uniform(0.5, 2.5))))
+    return out
+
+
+@pytest.mark.parametrize("shape", _sweep_shapes(24, 2026), ids=lambda s: f"{s[0]}x{s[1]}x{s[2]}")
+def test_seeded_shape_sweep(shape):
+    """Seeded sweep over grid shapes (z splits, row lengths, column counts and
+    x tile rules the fixed cases do not reach): FP64 to 1e-12 and
+    FP32+refine to 1e-11 against the C oracle."""
+    nxp, nym, nzp, alpha = shape
+    g = make_grid(nxp, nym, nzp, alpha)
+    print(f"sweep {nxp}x{nym}x{nzp} alpha {alpha:.3f}", flush=True)
+    U, V, W, P, rhs = fields(g, nxp * 7 + nym * 3 + nzp, 0.1)
+    o = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    oracle_project(g, *o, DT, C3, save_p=True)
+    for prec, tol in ((dfp.DFP_FP64, TOL), (dfp.DFP_FP32_REFINE, 1e-11)):
+        plan = dfp.Plan(g, precision=prec)
+        d = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+        plan.project(d[0], d[1], d[2], d[4], d[3], DT, C3, save_p=True)
+        plan.close()
+        for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+            err = rel_linf(a, b)
+            assert err <= tol, f"{shape} prec {prec} {lab}: {err:.3e}"
