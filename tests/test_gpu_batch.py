@@ -278,3 +278,29 @@ def test_batch_solve_only_equals_plan_solve():
                 assert np.array_equal(cs.rhs.download(), r[0]), f"graphs {graphs} rep {rep} case {i} phi"
                 assert np.array_equal(cs.P.download(), r[1]), f"graphs {graphs} rep {rep} case {i} P"
         bt.close()
+
+
+def test_batch_wide_grids_equal_single_plans():
+    """Grids whose x kernels take the wide-tile rules (4096-point forward
+    tiles from nxp = 256, strips of column blocks from nxp = 512, separate
+    forward and inverse column counts) in one batch with narrow ones, both
+    precisions: bit for bit as each plan alone."""
+    shapes = [(512, 16, 72, 1.8), (288, 12, 24, 1.1), (256, 5, 8, 1.5), (256, 10, 64, 2.0), (64, 20, 48, 1.3)]
+    plans, cases, hosts = [], [], []
+    for i, (nxp, nym, nzp, alpha) in enumerate(shapes):
+        for prec in (dfp.DFP_FP64, dfp.DFP_FP32_REFINE):
+            g = make_grid(nxp, nym, nzp, alpha)
+            plan = dfp.Plan(g, precision=prec)
+            dev, host = _resident(g, 100 + i)
+            save_p = (i + prec) % 2 == 0
+            plans.append(plan)
+            hosts.append((host, save_p))
+            cases.append(Case(plan, dev[0], dev[1], dev[2], dev[3], dev[4], DT, C3, save_p))
+    print(f"wide batch: {[(p.g.nxp, p.g.nym, p.g.nzp) for p in plans]}", flush=True)
+    b = Batch(max_cases=len(cases))
+    b.project(cases)
+    for i, (plan, cs, (host, save_p)) in enumerate(zip(plans, cases, hosts)):
+        ref = _single(plan, host, save_p)
+        for lab, a, r in zip(("U", "V", "W", "phi", "P"), (cs.U, cs.V, cs.W, cs.rhs, cs.P), ref):
+            assert np.array_equal(a.download(), r), f"case {i} {lab}"
+    b.close()
