@@ -82,7 +82,9 @@ int dfp_plan_destroy(dfp_plan *plan);
  *   z_split    radix R in {1,2,3,4,6,8,9,12,16} of the first z-FFT stage,
  *              applied by the x kernels (R must divide nzp; the z kernels
  *              then transform rows of nzp/R <= 256 points, one mode per lane)
- *   x_columns  z columns per x-kernel workgroup (divides nzp/R)
+ *   x_columns  z columns per x-kernel workgroup (divides nzp/R); given, it
+ *              sets both x kernels, else each picks its own (dfp_plan_tuning
+ *              reports the forward kernel's)
  *   z_rows     y rows per group of the split z kernels (z_fused = 1)
  *   z_fused    1: split pair of z kernels (FFT + elimination, back
  *              substitution + inverse FFT, spectrum read and written twice);
