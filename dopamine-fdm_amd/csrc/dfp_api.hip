@@ -132,23 +132,26 @@ constexpr int kRadices[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
 // as many points as its LDS transform holds (4096 when every stage is radix 4,
 // or radix 2 paired), for longer spectrum segments: from nxp = 256 up in the
 // forward kernels (which pair their radix-2 butterflies) and in the FP32
-// inverse, from nxp = 512 up in the FP64 inverse.  Measured: 512x384x512 FP64
-// 3.90 -> 3.58 ms, FP32+refine 6.12 -> 5.47 ms; 256x192x256 FP32+refine
-// 650 -> 628 us, the FP64 forward kernel 113 -> 106 us (its inverse with 8
-// columns holds 66 KB of LDS, two workgroups per CU instead of three, and
-// lost 8 us, so it keeps 4).
+// inverse, from nxp = 512 up in the FP64 inverse; from nxp = 512 the FP32
+// forward kernel takes up to 8192 points (four butterfly slots per thread).
+// Measured: 512x384x512 FP64 3.90 -> 3.58 ms, FP32+refine 6.12 -> 5.47 ms
+// (8192-point FP32 forward tiles: forward kernels -12 % and -23 %);
+// 256x192x256 FP32+refine 650 -> 628 us, the FP64 forward kernel
+// 113 -> 106 us (its inverse with 8 columns holds 66 KB of LDS, two
+// workgroups per CU instead of three, and lost 8 us, so it keeps 4).
 int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, int want, int inv) {
     const int M = nxp / 2, nz2 = nzp / sh.R;
     const bool f32 = cs == sizeof(float2);
-    const bool pair = f32 || !inv;  // the kernels that pair radix-2 butterflies
-    const int cap = inplace_capacity(sh.fx, 512, 2, pair);
+    const bool pair = f32 || !inv;         // the kernels that pair radix-2 butterflies
+    const int slots = f32 && !inv ? 4 : 2;  // FP32 forward tiles may take four butterfly slots per thread
+    const int cap = inplace_capacity(sh.fx, 512, slots, pair);
     auto fits = [&](int T) {
         const size_t pts = (size_t)sh.R * T;
         return pts * M <= (size_t)cap &&
                cs * (pts * (M + 1) + twiddle_count(sh.fx) + M + 1 + (size_t)nz2 * (sh.R - 1)) + 512 <= lds_limit;
     };
     if (want > 0) return (nz2 % want == 0 && fits(want)) ? want : 0;
-    const int pts = nxp >= 512 || (nxp >= 256 && pair) ? cap : 2048;
+    const int pts = nxp >= 512 ? cap : nxp >= 256 && pair ? std::min(cap, 4096) : 2048;
     for (int t = std::max(1, std::min(nz2, pts / (sh.R * M))); t >= 1; --t)
         if (nz2 % t == 0 && fits(t)) return t;
     return 0;

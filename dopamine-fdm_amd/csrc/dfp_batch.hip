@@ -440,12 +440,15 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         for (int R : kRadixList) {
             std::vector<int> sel;
             size_t lds = 0;
+            bool wide = false;  // a forward tile too large for two butterfly slots: the four-slot kernel
             for (int a : recs)
                 if (Rs[a] == R) {
+                    const dfp_plan* p = plans[f32 ? i32[a] : i64[a]];
                     sel.push_back(a);
-                    lds = std::max(lds, x_lds_bytes(plans[f32 ? i32[a] : i64[a]], inv, f32));
+                    lds = std::max(lds, x_lds_bytes(p, inv, f32));
+                    wide = wide || (!inv && xfwd_wide(p->fx, R, p->tk_x));
                 }
-            if (!sel.empty()) add(k, f32, mode, R, tab, make_list(sel, w), lds);
+            if (!sel.empty()) add(k, f32, wide ? mode | 8 : mode, R, tab, make_list(sel, w), lds);
         }
         ++phase;
     };

@@ -285,6 +285,11 @@ void prof_end(dfp_plan* p, int slot);
 
 inline int ceil_div(long a, long b) { return (int)((a + b - 1) / b); }
 
+// A forward x tile of R*T lines of the x transform fx holds more points than
+// two butterfly slots per thread (512 threads, radix-2 stages paired) can
+// transform: its kernel takes four slots each (FP32 form only).
+bool xfwd_wide(const FftDesc& fx, int R, int T);
+
 // Workgroups of a persistent (batched) launch: the caller's count, capped at
 // what the device holds at once of this kernel with this LDS.  A workgroup
 // beyond that would start only once another has finished all of its work

@@ -19,6 +19,10 @@ hipError_t launch_xinv_batch_r(hipStream_t s, int accum, const XPar<C>* cs, cons
 // ---------------------------------------------------------------------------
 // host side: parameters, LDS sizes, launches
 // ---------------------------------------------------------------------------
+bool xfwd_wide(const FftDesc& fx, int R, int T) {
+    return (long)R * T * fx.n > (long)inplace_capacity(fx, 512, 2, true);  // 512 threads per x workgroup
+}
+
 template <typename C>
 void xpar_columns(XPar<C>& q, const dfp_plan* p, int inv) {
     const Geo& g = p->geo;

@@ -67,7 +67,7 @@ __device__ __forceinline__ void xtile(int L, int nbx, int nym, int S, int& bx, i
 // formed in FP64 and rounded once).  a.b_out receives the FP64 field in the
 // rhs_p layout; a.rowpart the per-block sums of the field (FP64, fixed
 // order), rowpart[j*nbx + bx].
-template <int MODE, typename C, int R>
+template <int MODE, typename C, int R, int KX>
 __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsigned char* smem_raw) {
     using Rl = typename CplxT<C>::real;
     __shared__ double wsum[kXThreads / 64];
@@ -181,8 +181,8 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
         if (threadIdx.x == 0) gptr(a.rowpart)[(long)j * q.nbx + bx] = sb;
     }
     __syncthreads();
-    // tiles of up to 4096 points (radix-2 stages paired): 64-byte FP32 and 128-byte FP64 spectrum segments
-    lds_fft_inplace<-1, 2, C, true>(A, nl, M, q.fd, twl);
+    // tiles of up to 2048 * KX points (radix-2 stages paired): 64-byte FP32 and 128-byte FP64 spectrum segments
+    lds_fft_inplace<-1, KX, C, true>(A, nl, M, q.fd, twl);
 
     // unpack the R2C modes of the R lines of each column, apply the first z stage
     for (int pidx = threadIdx.x; pidx < (M + 1) * nt; pidx += kXThreads) {
@@ -210,13 +210,13 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
     }
 }
 
-template <int MODE, typename C, int R>
+template <int MODE, typename C, int R, int KX>
 __global__ void __launch_bounds__(kXThreads) k_div_rfft_x(XPar<C> q) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int L = xcd_remap(blockIdx.x + gridDim.x * blockIdx.y, gridDim.x * gridDim.y);
     int bx, j;
     xtile(L, gridDim.x, gridDim.y, q.strip, bx, j);
-    xfwd_body<MODE, C, R>(q, bx, j, smem_raw);
+    xfwd_body<MODE, C, R, KX>(q, bx, j, smem_raw);
 }
 
 // case of work item w: the last c with off[c] <= w (off[0] = 0, off[nc] = total)
@@ -230,9 +230,12 @@ __device__ __forceinline__ int xfind_case(const int* __restrict__ off, int nc, i
     return lo;
 }
 
-template <int MODE, typename C, int R>
-__global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(4))) kb_div_rfft_x(const XPar<C>* __restrict__ cs, const int* __restrict__ off,
-                                                           const int* __restrict__ idx, const int* __restrict__ pnc) {
+// KX butterfly slots per thread (4: FP32 tiles too large for two);
+// with two, at most four waves per SIMD (128 VGPRs): two workgroups per CU
+template <int MODE, typename C, int R, int KX>
+__global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(KX == 2 ? 4 : 1)))
+kb_div_rfft_x(const XPar<C>* __restrict__ cs, const int* __restrict__ off, const int* __restrict__ idx,
+              const int* __restrict__ pnc) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     const int nc = *pnc;
     const int total = off[nc];
@@ -244,7 +247,7 @@ __global__ void __launch_bounds__(kXThreads) __attribute__((amdgpu_waves_per_eu(
         const XPar<C>& q = cs[idx[c]];
         int bx, j;
         xtile(w - off[c], q.nbx, q.g.nym, q.strip, bx, j);
-        xfwd_body<MODE, C, R>(q, bx, j, smem_raw);
+        xfwd_body<MODE, C, R, KX>(q, bx, j, smem_raw);
         __syncthreads();
     }
 }
@@ -431,13 +434,22 @@ __global__ void __launch_bounds__(kXThreads) kb_irfft_x(const XPar<C>* __restric
 // in k_xfft_r<R>.hip, so the radices compile in parallel)
 template <int R, typename C>
 hipError_t launch_xfwd_r(hipStream_t s, int mode, const XPar<C>& q, dim3 grid, size_t lds) {
+    auto go = [&](auto k) { hipLaunchKernelGGL(k, grid, dim3(kXThreads), lds, s, q); };
+    if constexpr (sizeof(C) == sizeof(float2)) {
+        if (xfwd_wide(q.fd, R, q.T)) {
+            if (mode == 0) go(k_div_rfft_x<0, C, R, 4>);
+            else if (mode == 1) go(k_div_rfft_x<1, C, R, 4>);
+            else go(k_div_rfft_x<2, C, R, 4>);
+            return hipGetLastError();
+        }
+    }
     if (mode == 0) {
-        hipLaunchKernelGGL((k_div_rfft_x<0, C, R>), grid, dim3(kXThreads), lds, s, q);
+        go(k_div_rfft_x<0, C, R, 2>);
     } else if (mode == 1) {
-        hipLaunchKernelGGL((k_div_rfft_x<1, C, R>), grid, dim3(kXThreads), lds, s, q);
+        go(k_div_rfft_x<1, C, R, 2>);
     } else {
         if constexpr (sizeof(C) == sizeof(float2)) {
-            hipLaunchKernelGGL((k_div_rfft_x<2, C, R>), grid, dim3(kXThreads), lds, s, q);
+            go(k_div_rfft_x<2, C, R, 2>);
         } else {
             return hipErrorInvalidValue;  // the residual form exists for the FP32 spectrum only
         }
@@ -459,20 +471,32 @@ hipError_t launch_xinv_r(hipStream_t s, int accum, const XPar<C>& q, dim3 grid, 
     return hipGetLastError();
 }
 
+// mode: 0, 1, 2 as for the single launch, + 8 when a case of the list has
+// forward tiles too large for two butterfly slots per thread (FP32 only)
 template <int R, typename C>
 hipError_t launch_xfwd_batch_r(hipStream_t s, int mode, const XPar<C>* cs, const int* off, const int* idx,
                                const int* nc, int grid, size_t lds) {
+    auto go = [&](auto k) {
+        hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kXThreads, lds)), dim3(kXThreads), lds, s, cs, off, idx, nc);
+    };
+    const bool wide = (mode & 8) != 0;
+    mode &= 7;
+    if constexpr (sizeof(C) == sizeof(float2)) {
+        if (wide) {
+            if (mode == 0) go(kb_div_rfft_x<0, C, R, 4>);
+            else if (mode == 1) go(kb_div_rfft_x<1, C, R, 4>);
+            else go(kb_div_rfft_x<2, C, R, 4>);
+            return hipGetLastError();
+        }
+    }
+    if (wide) return hipErrorInvalidValue;
     if (mode == 0) {
-        const auto k = kb_div_rfft_x<0, C, R>;
-        hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kXThreads, lds)), dim3(kXThreads), lds, s, cs, off, idx, nc);
+        go(kb_div_rfft_x<0, C, R, 2>);
     } else if (mode == 1) {
-        const auto k = kb_div_rfft_x<1, C, R>;
-        hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kXThreads, lds)), dim3(kXThreads), lds, s, cs, off, idx, nc);
+        go(kb_div_rfft_x<1, C, R, 2>);
     } else {
         if constexpr (sizeof(C) == sizeof(float2)) {
-            const auto k = kb_div_rfft_x<2, C, R>;
-            hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kXThreads, lds)), dim3(kXThreads), lds, s, cs, off, idx,
-                               nc);
+            go(kb_div_rfft_x<2, C, R, 2>);
         } else {
             return hipErrorInvalidValue;
         }
