@@ -679,7 +679,9 @@ int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit) {
     const int pn = n * (two ? zplanes(n) : 1);  // points of one group row
     int G = (two ? kZsPf * kZsThreads : std::min(kZsPf * kZsThreads, inplace_capacity(fz, kZsThreads, 2))) / pn;
     G = std::max(0, std::min(G, nym));
-    auto bytes = [&](int g) { return sizeof(double2) * ((size_t)g * pn + ntw) + sizeof(double) * ((size_t)g * pn + g); };
+    auto bytes = [&](int g) {
+        return sizeof(double2) * ((size_t)g * pn + ntw) + sizeof(double) * ((size_t)g * pn + g);
+    };
     while (G > 0 && bytes(G) > lds_limit) --G;
     return G;
 }

@@ -14,6 +14,7 @@ from .hipmem import DeviceArray, Event, device_count, synchronize
 
 __all__ = [
     "DFP_ASYNC", "DFP_FIELDS_ON_DEVICE", "DFP_FP32_REFINE", "DFP_FP64", "DFP_SAVE_P", "DFP_SOLVE_ONLY", "DfpError",
-    "LIB_PATH", "HEADER_PATH", "lib", "ChannelGrid", "channel_grid", "namelist_grid", "RK_COEF_DIAG", "ChannelState", "Plan",
-    "Projection", "refresh_velocity_ghosts", "synth_fields", "DeviceArray", "Event", "device_count", "synchronize",
+    "LIB_PATH", "HEADER_PATH", "lib", "ChannelGrid", "channel_grid", "namelist_grid", "RK_COEF_DIAG", "ChannelState",
+    "Plan", "Projection", "refresh_velocity_ghosts", "synth_fields", "DeviceArray", "Event", "device_count",
+    "synchronize",
 ]
