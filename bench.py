@@ -307,7 +307,8 @@ def bench_serving(n_requests: int = 600, rate: float = 200.0, n_ids: int = 64) -
     pool = serving.make_case_pool(n_ids)
     table = autotune.load_table()
     trace = serving.poisson_requests(n_requests, rate, pool)
-    srv = serving.Server(hbm_budget=200e9, max_batch=32, n_streams=2, graphs=True, table=table)
+    srv = serving.Server(hbm_budget=200e9, max_batch=32, n_streams=2, backlog_streams=2, graphs=True,
+                         table=table)
     reqs, wall = serving.run_trace(srv, trace, realtime=True)
     live = {"requests": len(reqs), "rate_req_s": rate, "wall_s": round(wall, 3),
             "throughput_req_s": round(len(reqs) / wall, 1), "launches": srv.launches,

@@ -110,7 +110,7 @@ def case_bytes(g, precisions=(DFP_FP64,)) -> int:
 class Server:
     def __init__(self, device: int = 0, hbm_budget: float = 200e9, max_batch: int = 32, n_streams: int = 2,
                  graphs: bool = True, table: dict | None = None, sweeps: int = 1, occupancy=None,
-                 solo_cells: int | None = 1 << 20):
+                 solo_cells: int | None = 1 << 20, backlog_streams: int = 2):
         self.device = device
         # requests for cases of at least solo_cells cells run alone through
         # their plan's own launches (which fill the GPU by themselves) rather
@@ -119,6 +119,15 @@ class Server:
         # requests queued at once 3-5 % faster than batching every one.
         self.solo_cells = solo_cells
         self.solo_launches = 0
+        # backlog_streams more streams take work only while a full batch of
+        # requests is waiting: a backlog drains faster, while at light load the
+        # requests keep the GPU shared n_streams ways (their latency).  Two of
+        # them (four streams in all, the hardware queues HIP gives a process by
+        # default) served 600 requests queued at once 10 % faster, with the p99
+        # latency at 200 req/s unchanged (four streams used at any load raised
+        # it from 2.6 to 4.6 ms).
+        self.n_base = n_streams
+        n_streams += backlog_streams
         self.budget = float(hbm_budget)
         self.max_batch = max_batch
         self.table = table
@@ -262,6 +271,8 @@ class Server:
         self._retire()
         for k in range(len(self.streams)):
             if self.inflight[k] is None and self.pending:
+                if k >= self.n_base and len(self.pending) < self.max_batch:
+                    continue
                 self._launch(k)
 
     def busy(self) -> bool:
