@@ -304,3 +304,34 @@ def test_batch_wide_grids_equal_single_plans():
         for lab, a, r in zip(("U", "V", "W", "phi", "P"), (cs.U, cs.V, cs.W, cs.rhs, cs.P), ref):
             assert np.array_equal(a.download(), r), f"case {i} {lab}"
     b.close()
+
+
+def test_batch_mixed_sweep_counts_equal_single_plans():
+    """FP32+refine cases with 0, 1, 2 and 3 refinement sweeps (the batch runs
+    sweep s for the cases that have more than s) beside FP64 cases, P saved
+    for some: bit for bit as each plan alone, eager and from graph replays."""
+    specs = batched_cases(10, seed=61)
+    sweeps = [0, 1, 2, 3, None, 3, 1, None, 0, 2]
+    plans, cases, hosts = [], [], []
+    for i, (sp, sw) in enumerate(zip(specs, sweeps)):
+        g = make_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
+        prec = dfp.DFP_FP64 if sw is None else dfp.DFP_FP32_REFINE
+        plan = dfp.Plan(g, precision=prec, sweeps=sw)
+        dev, host = _resident(g, sp.seed)
+        save_p = i % 2 == 1
+        plans.append(plan)
+        hosts.append((host, save_p))
+        cases.append(Case(plan, dev[0], dev[1], dev[2], dev[3], dev[4], DT, C3, save_p))
+    print(f"sweeps {sweeps}", flush=True)
+    for graphs in (False, True):
+        b = Batch(max_cases=len(cases), graphs=graphs)
+        for rep in range(2 if graphs else 1):
+            for cs, (host, _) in zip(cases, hosts):
+                for a, h in zip((cs.U, cs.V, cs.W, cs.rhs, cs.P), host):
+                    a.upload(h)
+            b.project(cases)
+            for i, (plan, cs, (host, save_p)) in enumerate(zip(plans, cases, hosts)):
+                ref = _single(plan, host, save_p)
+                for lab, a, r in zip(("U", "V", "W", "phi", "P"), (cs.U, cs.V, cs.W, cs.rhs, cs.P), ref):
+                    assert np.array_equal(a.download(), r), f"graphs {graphs} rep {rep} case {i} {lab}"
+        b.close()
