@@ -24,6 +24,7 @@ reproduces single-plan projections bit for bit.
 from __future__ import annotations
 
 import collections
+import math
 import time
 from dataclasses import dataclass, field
 
@@ -95,16 +96,17 @@ class _Resident:
 def case_bytes(g, precisions=(DFP_FP64,)) -> int:
     """HBM a resident case holds: its five fields and, per precision, the
     plan's spectrum, pivot table and (FP32+refine) FP64 right-hand side."""
-    f = 8 * (np.prod(g.shape_u) + np.prod(g.shape_v) + np.prod(g.shape_w) + np.prod(g.shape_rhs) +
-             np.prod(g.shape_p))
+    # integer products (math.prod): this runs for every submitted request
+    f = 8 * (math.prod(g.shape_u) + math.prod(g.shape_v) + math.prod(g.shape_w) + math.prod(g.shape_rhs) +
+             math.prod(g.shape_p))
     H = (g.nxp // 2 + 1) * g.nym * g.nzp
     lu = 8 * (g.nxp // 2 + 1) * g.nym * (g.nzp // 2 + 1)
     extra = 0
     for prec in set(precisions) | {DFP_FP64}:
         extra += 16 * H + lu
         if prec == DFP_FP32_REFINE:
-            extra += 8 * np.prod(g.shape_rhs)
-    return int(f + extra)
+            extra += 8 * math.prod(g.shape_rhs)
+    return f + extra
 
 
 class Server:
@@ -147,6 +149,12 @@ class Server:
         self.inflight = [None] * n_streams  # (event, [requests]) per stream
         self.launches = 0
         self.evictions = 0
+        # (request, exception) for requests dropped because admitting them or
+        # building their plan raised; the other requests of that launch went
+        # back to the queue in their order
+        self.failed: list = []
+        self._grids: dict = {}
+        self._bytes: dict = {}
 
     # --- admission -------------------------------------------------------------
     def _evict_for(self, need: int, keep=None) -> bool:
@@ -156,7 +164,7 @@ class Server:
             r = self.resident[cid]
             if r.busy or cid == keep:
                 continue
-            self.resident_bytes -= case_bytes(r.g, tuple(r.plans.keys()) or (DFP_FP64,))
+            self.resident_bytes -= self._case_bytes(r.g, tuple(r.plans.keys()) or (DFP_FP64,))
             r.free()
             del self.resident[cid]
             self.evictions += 1
@@ -166,32 +174,69 @@ class Server:
         """The resident state of the request's case, or None if it does not fit now."""
         r = self.resident.get(req.case_id)
         if r is not None:
+            self._check_grid(r, req)
             if r.busy:
                 return None
             if req.precision not in r.plans:
                 g = r.g
-                old = case_bytes(g, tuple(r.plans.keys()) or (DFP_FP64,))
-                new = case_bytes(g, tuple(r.plans.keys()) + (req.precision,))
+                old = self._case_bytes(g, tuple(r.plans.keys()) or (DFP_FP64,))
+                new = self._case_bytes(g, tuple(r.plans.keys()) + (req.precision,))
                 if new > old and not self._evict_for(new - old, keep=req.case_id):
                     return None
+                r.plan(req.precision)  # made before it is counted: a plan that fails counts nothing
                 self.resident_bytes += new - old
             self.resident.move_to_end(req.case_id)
             return r
-        g = channel_grid(*req.shape, req.alpha)
-        need = case_bytes(g, (req.precision,))
-        if need > self.budget:
-            raise ValueError(f"request {req.rid}: case needs {need} bytes, over the whole budget")
+        g = self._grid(req)
+        need = self._case_bytes(g, (req.precision,))
         if not self._evict_for(need):
             return None
         r = _Resident(req.case_id, g, self.table, self.sweeps)
+        try:
+            r.plan(req.precision)
+        except BaseException:
+            r.free()
+            raise
         self.resident[req.case_id] = r
         self.resident_bytes += need
         return r
 
+    def _case_bytes(self, g, precisions):
+        key = (g.nxp, g.nym, g.nzp, frozenset(precisions))
+        v = self._bytes.get(key)
+        if v is None:
+            v = self._bytes[key] = case_bytes(g, precisions)
+        return v
+
     # --- loop ------------------------------------------------------------------
+    def _grid(self, req: Request):
+        key = (tuple(req.shape), req.alpha)
+        g = self._grids.get(key)
+        if g is None:
+            g = self._grids[key] = channel_grid(*req.shape, req.alpha)
+        return g
+
+    @staticmethod
+    def _check_grid(r, req: Request):
+        g = r.g
+        if (g.nxp, g.nym, g.nzp, g.alpha) != (*req.shape, req.alpha):
+            raise ValueError(f"request {req.rid}: case {req.case_id} is resident as "
+                             f"{g.nxp}x{g.nym}x{g.nzp} alpha {g.alpha}, not {tuple(req.shape)} alpha {req.alpha}")
+
     def submit(self, req: Request):
+        """Queue a request.  Raises ValueError, and queues nothing, when its
+        case could not fit the HBM budget even alone, or when its case id is
+        resident with another grid."""
         if not req.arrival:
             req.arrival = time.perf_counter()
+        r = self.resident.get(req.case_id)
+        if r is not None:
+            self._check_grid(r, req)
+            need = self._case_bytes(r.g, tuple(r.plans.keys()) + (req.precision,))
+        else:
+            need = self._case_bytes(self._grid(req), (req.precision,))
+        if need > self.budget:
+            raise ValueError(f"request {req.rid}: case needs {need} bytes, over the whole budget")
         self.pending.append(req)
 
     def _retire(self):
@@ -214,54 +259,62 @@ class Server:
         return self.solo_cells is not None and nx * ny * nz >= self.solo_cells
 
     def _launch(self, k: int):
-        chosen, res, seen, skipped = [], [], set(), collections.deque()
-        while self.pending and len(chosen) < self.max_batch:
-            q = self.pending.popleft()
-            if q.case_id in seen or (chosen and self._solo(q)):
-                # a large request waits for a stream of its own
+        chosen, res, seen, skipped = [], [], set(), []
+        taken, cur = [], None  # every request taken off the queue; the one being prepared
+        try:
+            while self.pending and len(chosen) < self.max_batch:
+                q = self.pending.popleft()
+                taken.append(q)
+                if q.case_id in seen or (chosen and self._solo(q)):
+                    # a large request waits for a stream of its own
+                    seen.add(q.case_id)
+                    skipped.append(q)
+                    continue
+                cur = q
+                r = self._admit(q)
+                cur = None
+                if r is None:
+                    # later requests of this case wait too: per-case order is arrival order
+                    seen.add(q.case_id)
+                    skipped.append(q)
+                    continue
+                r.busy = True  # chosen: later admissions in this batch must not evict it
                 seen.add(q.case_id)
-                skipped.append(q)
-                continue
-            r = self._admit(q)
-            if r is None:
-                # later requests of this case wait too: per-case order is arrival order
-                seen.add(q.case_id)
-                skipped.append(q)
-                continue
-            r.busy = True  # chosen: later admissions in this batch must not evict it
-            seen.add(q.case_id)
-            chosen.append(q)
-            res.append(r)
-            if self._solo(q):
-                break
-        self.pending.extendleft(reversed(skipped))
-        if not chosen:
-            return False
-        stream = self.streams[k]
-        if len(chosen) == 1 and self._solo(chosen[0]):
-            q, r = chosen[0], res[0]
-            plan = r.plan(q.precision)
-            plan.set_stream(stream.handle)
-            plan.synth(r.U, r.V, r.W, q.seed, q.sigma, asynchronous=True)
-            q.launched = time.perf_counter()
-            plan.project(r.U, r.V, r.W, r.rhs, r.P, DT, C3, save_p=q.save_p, asynchronous=True)
+                chosen.append(q)
+                res.append(r)
+                if self._solo(q):
+                    break
+            if not chosen:
+                self.pending.extendleft(reversed(skipped))
+                return False
+            stream = self.streams[k]
+            plans = [r.plans[q.precision] for q, r in zip(chosen, res)]
+            if len(chosen) == 1 and self._solo(chosen[0]):
+                q, r, plan = chosen[0], res[0], plans[0]
+                plan.set_stream(stream.handle)
+                plan.synth(r.U, r.V, r.W, q.seed, q.sigma, asynchronous=True)
+                q.launched = time.perf_counter()
+                plan.project(r.U, r.V, r.W, r.rhs, r.P, DT, C3, save_p=q.save_p, asynchronous=True)
+                self.solo_launches += 1
+            else:
+                cases = []
+                now = time.perf_counter()
+                for q, r, plan in zip(chosen, res, plans):
+                    plan.set_stream(stream.handle)
+                    plan.synth(r.U, r.V, r.W, q.seed, q.sigma, asynchronous=True)
+                    cases.append(Case(plan, r.U, r.V, r.W, r.rhs, r.P, DT, C3, q.save_p))
+                    q.launched = now
+                self.batches[k].project(cases, asynchronous=True)
             ev = Event()
             ev.record(stream.handle)
-            self.inflight[k] = (ev, chosen, res)
-            self.launches += 1
-            self.solo_launches += 1
-            return True
-        cases = []
-        now = time.perf_counter()
-        for q, r in zip(chosen, res):
-            plan = r.plan(q.precision)
-            plan.set_stream(stream.handle)
-            plan.synth(r.U, r.V, r.W, q.seed, q.sigma, asynchronous=True)
-            cases.append(Case(plan, r.U, r.V, r.W, r.rhs, r.P, DT, C3, q.save_p))
-            q.launched = now
-        self.batches[k].project(cases, asynchronous=True)
-        ev = Event()
-        ev.record(stream.handle)
+        except BaseException as e:
+            for r in res:
+                r.busy = False
+            if cur is not None:
+                self.failed.append((cur, e))
+            self.pending.extendleft(reversed([q for q in taken if q is not cur]))
+            raise
+        self.pending.extendleft(reversed(skipped))
         self.inflight[k] = (ev, chosen, res)
         self.launches += 1
         return True
@@ -274,6 +327,10 @@ class Server:
                 if k >= self.n_base and len(self.pending) < self.max_batch:
                     continue
                 self._launch(k)
+        if self.pending and all(f is None for f in self.inflight):
+            # with nothing in flight no case is busy and every request fits
+            # the budget (checked at submit), so this does not happen
+            raise RuntimeError(f"{len(self.pending)} requests pending and none could be launched")
 
     def busy(self) -> bool:
         return bool(self.pending) or any(f is not None for f in self.inflight)

@@ -7,7 +7,10 @@ Bars (written here, as the tests apply them):
   * with a budget that holds only a few cases, idle cases are evicted and the
     results are unchanged;
   * served results match the C oracle within the precision's parity bar
-    (1e-12 for FP64, 1e-11 for FP32+refine).
+    (1e-12 for FP64, 1e-11 for FP32+refine);
+  * a request that cannot be served (its case over the budget, a grid other
+    than its resident case id's, a size the library rejects) raises and is
+    dropped, and the requests around it are served as if it never came.
 """
 from __future__ import annotations
 
@@ -99,4 +102,48 @@ def test_served_results_match_oracle():
         tol = 1e-12 if q.precision == dfp.DFP_FP64 else 1e-11
         for lab, a, b in zip(("U", "V", "W", "phi", "P"), s.result(cid), (U, V, W, rhs, P)):
             assert rel_linf(a, b) <= tol, f"case {cid} {lab} ({'fp64' if tol == 1e-12 else 'fp32'})"
+    s.close()
+
+
+def test_requests_over_budget_or_against_the_resident_grid_are_refused():
+    shape, alpha = (16, 8, 12), 1.5
+    g = dfp.channel_grid(*shape, alpha)
+    b64 = case_bytes(g, (dfp.DFP_FP64,))
+    assert case_bytes(g, (dfp.DFP_FP32_REFINE,)) > b64
+    s = Server(max_batch=4, n_streams=1, hbm_budget=b64 + 1, solo_cells=None)
+    with pytest.raises(ValueError, match="over the whole budget"):
+        s.submit(Request(0, 0, shape, alpha, seed=5, precision=dfp.DFP_FP32_REFINE))
+    q = Request(1, 0, shape, alpha, seed=6)
+    s.submit(q)
+    s.drain()
+    assert q.done is not None and not s.pending
+    with pytest.raises(ValueError, match="over the whole budget"):
+        s.submit(Request(2, 0, shape, alpha, seed=7, precision=dfp.DFP_FP32_REFINE))
+    with pytest.raises(ValueError, match="resident as"):
+        s.submit(Request(3, 0, (16, 8, 16), alpha, seed=8))
+    assert not s.pending and s.resident_bytes <= b64
+    for a, b in zip(s.result(0), _alone(q)):
+        assert np.array_equal(a, b)
+    s.close()
+
+
+def test_a_rejected_size_leaves_the_other_requests_served():
+    # nzp = 14 has the factor 7: plan creation rejects it (DFP_E_UNSUPPORTED)
+    # before any kernel runs
+    good = [Request(0, 0, (16, 8, 12), 1.5, seed=11), Request(2, 2, (24, 6, 16), 2.0, seed=12,
+                                                                 precision=dfp.DFP_FP32_REFINE)]
+    bad = Request(1, 1, (16, 8, 14), 1.5, seed=13)
+    s = Server(max_batch=8, n_streams=1, solo_cells=None)
+    for q in (good[0], bad, good[1]):
+        s.submit(q)
+    with pytest.raises(dfp.DfpError):
+        s.poll()
+    assert [f[0].rid for f in s.failed] == [1]
+    assert [q.rid for q in s.pending] == [0, 2]
+    assert 1 not in s.resident and not any(r.busy for r in s.resident.values())
+    s.drain()
+    assert all(q.done is not None for q in good) and bad.done is None
+    for q in good:
+        for a, b in zip(s.result(q.case_id), _alone(q)):
+            assert np.array_equal(a, b), f"case {q.case_id}"
     s.close()

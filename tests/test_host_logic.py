@@ -71,6 +71,11 @@ class _FakeResident:
     def __init__(self, case_id, g, table, sweeps):
         self.case_id, self.g, self.plans, self.busy, self.freed = case_id, g, {}, False, False
 
+    def plan(self, precision):
+        if self.g.nzp == 14:  # stands for a size the library rejects
+            raise ValueError("size rejected")
+        return self.plans.setdefault(precision, object())
+
     def free(self):
         self.freed = True
 
@@ -84,6 +89,8 @@ def _bare_server(budget):
     s.budget, s.table, s.sweeps = float(budget), None, 1
     s.resident = collections.OrderedDict()
     s.resident_bytes, s.evictions = 0, 0
+    s.pending, s.failed, s._grids, s._bytes = collections.deque(), [], {}, {}
+    s.max_batch, s.solo_cells = 8, None
     return s
 
 
@@ -112,3 +119,30 @@ def test_serving_admission_evicts_lru_idle_cases(monkeypatch):
     again = s._admit(Request(9, 1, shape, 1.5, seed=9, precision=dfp.DFP_FP32_REFINE))
     assert again is r1 and 1 in s.resident and not r1.freed
     assert s.resident_bytes <= s.budget
+
+
+def test_serving_submit_refuses_and_a_failed_admission_restores_the_queue(monkeypatch):
+    from dfp import serving
+    from dfp.serving import Request, case_bytes
+
+    monkeypatch.setattr(serving, "_Resident", _FakeResident)
+    shape = (16, 8, 12)
+    one = case_bytes(dfp.channel_grid(*shape, 1.5), (dfp.DFP_FP64,))
+    s = _bare_server(one + 1)
+    with pytest.raises(ValueError, match="over the whole budget"):
+        s.submit(Request(0, 0, shape, 1.5, seed=0, precision=dfp.DFP_FP32_REFINE))
+    assert not s.pending
+    s.budget = 10.0 * one
+    a, bad, c = Request(1, 0, shape, 1.5, seed=1), Request(2, 1, (16, 8, 14), 1.5, seed=2), Request(3, 2, shape, 2.0, seed=3)
+    for q in (a, bad, c):
+        s.submit(q)
+    with pytest.raises(ValueError, match="size rejected"):
+        s._launch(0)
+    assert [f[0].rid for f in s.failed] == [2]
+    assert [q.rid for q in s.pending] == [1, 3]
+    assert list(s.resident) == [0] and not s.resident[0].busy
+    assert s.resident_bytes == one
+    # case 0 is resident now: a request naming it with another grid is refused
+    with pytest.raises(ValueError, match="resident as"):
+        s.submit(Request(4, 0, (16, 8, 16), 1.5, seed=4))
+    assert [q.rid for q in s.pending] == [1, 3]
