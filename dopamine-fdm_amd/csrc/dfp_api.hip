@@ -415,7 +415,11 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     if (!g || !g->y || !g->z || !g->yg || !g->zg) return DFP_E_ARG;
     if (g->x_bc_type != 0 || g->y_bc_type != 1 || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
     if (precision != DFP_FP64 && precision != DFP_FP32_REFINE) return DFP_E_ARG;
-    if (g->nxp < 2 || (g->nxp & 1) || g->nym < 1 || g->nzp < 1) return DFP_E_UNSUPPORTED;
+    if (g->nxp < 2 || (g->nxp & 1) || g->nzp < 1) return DFP_E_UNSUPPORTED;
+    // one cell between the walls: Dyy is zero there, so the (0,0) mode's
+    // diagonal 3/2 * 0 is singular (the reference's zgtsv returns info = 1,
+    // projection.f90:295, and leaves that mode unsolved)
+    if (g->nym < 2) return DFP_E_UNSUPPORTED;
     // an x block holds zr lines of nxp/2 complex points in 32 KiB of LDS; a z
     // block holds one mode per thread
     if (g->nxp > 1024 || g->nzp > 1024 || g->nym > 65535) return DFP_E_UNSUPPORTED;
