@@ -113,6 +113,9 @@ class Server:
     def __init__(self, device: int = 0, hbm_budget: float = 200e9, max_batch: int = 32, n_streams: int = 2,
                  graphs: bool = True, table: dict | None = None, sweeps: int = 1, occupancy=None,
                  solo_cells: int | None = 1 << 20, backlog_streams: int = 2):
+        if n_streams < 1 or backlog_streams < 0 or max_batch < 1:
+            raise ValueError(f"need n_streams >= 1, backlog_streams >= 0, max_batch >= 1 "
+                             f"(got {n_streams}, {backlog_streams}, {max_batch})")
         self.device = device
         # requests for cases of at least solo_cells cells run alone through
         # their plan's own launches (which fill the GPU by themselves) rather

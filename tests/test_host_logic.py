@@ -146,3 +146,11 @@ def test_serving_submit_refuses_and_a_failed_admission_restores_the_queue(monkey
     with pytest.raises(ValueError, match="resident as"):
         s.submit(Request(4, 0, (16, 8, 16), 1.5, seed=4))
     assert [q.rid for q in s.pending] == [1, 3]
+
+
+def test_serving_rejects_a_server_without_streams():
+    from dfp import serving
+
+    for kw in (dict(n_streams=0), dict(backlog_streams=-1), dict(max_batch=0)):
+        with pytest.raises(ValueError):
+            serving.Server(**kw)  # raises before any device call
