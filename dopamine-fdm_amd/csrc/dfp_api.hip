@@ -100,7 +100,9 @@ void dyy(const std::vector<double>& y, const std::vector<double>& yg, std::vecto
         const double c = 1.0 / (Y(j) - Y(j - 1)) / (G(j) - G(j - 1));
         const int jj = j - 2;
         if (j == 2 && j == nyg - 1) {
-            dd[jj] = b + c + a;
+            // one cell between the walls: the reference assigns the row twice,
+            // b+c for the first wall, then a+b for the last, which stays
+            dd[jj] = a + b;
         } else if (j == 2) {
             dd[jj] = b + c;
             du[jj] = a;
@@ -416,10 +418,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     if (g->x_bc_type != 0 || g->y_bc_type != 1 || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
     if (precision != DFP_FP64 && precision != DFP_FP32_REFINE) return DFP_E_ARG;
     if (g->nxp < 2 || (g->nxp & 1) || g->nzp < 1) return DFP_E_UNSUPPORTED;
-    // one cell between the walls: Dyy is zero there, so the (0,0) mode's
-    // diagonal 3/2 * 0 is singular (the reference's zgtsv returns info = 1,
-    // projection.f90:295, and leaves that mode unsolved)
-    if (g->nym < 2) return DFP_E_UNSUPPORTED;
+    if (g->nym < 1) return DFP_E_UNSUPPORTED;
     // an x block holds zr lines of nxp/2 complex points in 32 KiB of LDS; a z
     // block holds one mode per thread
     if (g->nxp > 1024 || g->nzp > 1024 || g->nym > 65535) return DFP_E_UNSUPPORTED;

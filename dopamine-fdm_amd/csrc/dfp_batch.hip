@@ -305,8 +305,8 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         const int fl = flags ? flags[i] : 0;
         if (fl & ~(DFP_SAVE_P | DFP_FIELDS_ON_DEVICE))
             return bfail(b, DFP_E_ARG, "case %d: flags 0x%x (device fields, DFP_SAVE_P only)", i, fl);
-        if ((fl & DFP_SAVE_P) && (!P || !P[i] || !(dt[i] * c[i] != 0.0)))
-            return bfail(b, DFP_E_ARG, "case %d: DFP_SAVE_P needs P and a nonzero dt*c", i);
+        if ((fl & DFP_SAVE_P) && (!P || !P[i] || !(dt[i] * c[i] != 0.0) || !std::isfinite(dt[i] * c[i])))
+            return bfail(b, DFP_E_ARG, "case %d: DFP_SAVE_P needs P and a finite nonzero dt*c", i);
         (p->precision == DFP_FP32_REFINE ? i32 : i64).push_back(i);
     }
     std::stable_sort(i32.begin(), i32.end(), [&](int a, int q) { return plans[a]->sweeps > plans[q]->sweeps; });

@@ -264,6 +264,7 @@ class Server:
     def _launch(self, k: int):
         chosen, res, seen, skipped = [], [], set(), []
         taken, cur = [], None  # every request taken off the queue; the one being prepared
+        enqueued = False  # work of this launch is queued on stream k
         try:
             while self.pending and len(chosen) < self.max_batch:
                 q = self.pending.popleft()
@@ -292,6 +293,7 @@ class Server:
                 return False
             stream = self.streams[k]
             plans = [r.plans[q.precision] for q, r in zip(chosen, res)]
+            enqueued = True
             if len(chosen) == 1 and self._solo(chosen[0]):
                 q, r, plan = chosen[0], res[0], plans[0]
                 plan.set_stream(stream.handle)
@@ -311,6 +313,14 @@ class Server:
             ev = Event()
             ev.record(stream.handle)
         except BaseException as e:
+            if enqueued:
+                # kernels already queued on stream k may still write these
+                # cases' fields: let them finish before another stream may
+                # take the cases again
+                try:
+                    self.streams[k].synchronize()
+                except BaseException:
+                    pass
             for r in res:
                 r.busy = False
             if cur is not None:

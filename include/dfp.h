@@ -75,9 +75,10 @@ enum {
  * Replaces the reference's set-up in initialization.f90:322-539 and
  * gpu_poisson_init / gpu_gtsv_init (poisson_gpu.f90:60-135, :388-428).
  * DFP_E_UNSUPPORTED for other boundary conditions, odd nxp, nxp/2 or nzp
- * with a prime factor other than 2 and 3, nxp or nzp above 1024, nym above
- * 65535, and nym < 2
- * (one cell between the walls: the (0,0) mode's system is singular). */
+ * with a prime factor other than 2 and 3, nxp or nzp above 1024, and nym
+ * above 65535.  With one cell between the walls (nym = 1) Dyy(2,2) is a+b,
+ * the last of the reference's two assignments to it
+ * (initialization.f90:523-535). */
 int dfp_plan_create(const dfp_grid *g, int precision, int device, dfp_plan **out);
 int dfp_plan_destroy(dfp_plan *plan);
 

@@ -359,7 +359,9 @@ ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
         double c = 1.0 / (YF(j) - YF(j - 1)) / (YG(j) - YG(j - 1));
         int jj = j - 2;
         if (j == 2 && j == nyg - 1) {
-            p->dyy_d[jj] = b + c + a; /* one-cell column: both wall rows (not used by any config) */
+            /* one cell between the walls: initialization.f90:523-535 assigns
+               Dyy(2,2) = b+c and then Dyy(nyg-1,nyg-1) = a+b, the same entry */
+            p->dyy_d[jj] = a + b;
         } else if (j == 2) {
             p->dyy_d[jj] = b + c;
             p->dyy_u[jj] = a;

@@ -16,6 +16,7 @@ SMALL_CASES = [
     ("odd_nym_32x33x48", dict(nxp=32, nym=33, nzp=48, alpha=2.5), 0.15),
     ("flat_uniform_16x2x16", dict(nxp=16, nym=2, nzp=16, alpha=0.0, grid_type=1), 0.1),
     ("nzp1_6x7x1", dict(nxp=6, nym=7, nzp=1, alpha=1.0), 0.1),
+    ("one_cell_8x1x12", dict(nxp=8, nym=1, nzp=12, alpha=1.5), 0.1),
 ]
 
 

@@ -335,3 +335,23 @@ def test_batch_mixed_sweep_counts_equal_single_plans():
                 for lab, a, r in zip(("U", "V", "W", "phi", "P"), (cs.U, cs.V, cs.W, cs.rhs, cs.P), ref):
                     assert np.array_equal(a.download(), r), f"graphs {graphs} rep {rep} case {i} {lab}"
         b.close()
+
+
+def test_batch_rejects_a_non_finite_save_p_scale_as_dfp_project_does():
+    """DFP_SAVE_P with dt*c infinite or NaN: dfp_project returns DFP_E_ARG,
+    and so does a batch holding such a case (nothing runs)."""
+    g = make_grid(16, 8, 16, 1.5)
+    plan = dfp.Plan(g)
+    dev, host = _resident(g, 3)
+    b = Batch(max_cases=2)
+    try:
+        for dt, c in ((1e308, 1e308), (float("nan"), 0.75), (float("inf"), 0.75)):
+            with pytest.raises(RuntimeError):
+                plan.project(dev[0], dev[1], dev[2], dev[3], dev[4], dt, c, save_p=True)
+            with pytest.raises(RuntimeError):
+                b.project([Case(plan, dev[0], dev[1], dev[2], dev[3], dev[4], dt, c, True)])
+        # the fields are untouched by the refused calls
+        assert np.array_equal(dev[0].download(), host[0])
+    finally:
+        b.close()
+        plan.close()

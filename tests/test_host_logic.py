@@ -154,3 +154,50 @@ def test_serving_rejects_a_server_without_streams():
     for kw in (dict(n_streams=0), dict(backlog_streams=-1), dict(max_batch=0)):
         with pytest.raises(ValueError):
             serving.Server(**kw)  # raises before any device call
+
+
+def test_serving_failed_launch_waits_for_queued_work_before_freeing_cases(monkeypatch):
+    """A launch that raises after kernels were queued on its stream (here the
+    projection call, after the field synthesis) must wait for that stream
+    before its cases can be taken by another stream."""
+    from dfp import serving
+    from dfp.serving import Request
+
+    log = []
+
+    class _Plan:
+        def set_stream(self, h):
+            pass
+
+        def synth(self, *a, **k):
+            log.append("synth")
+
+        def project(self, *a, **k):
+            raise RuntimeError("launch failed")
+
+    class _Res(_FakeResident):
+        def __init__(self, *a):
+            super().__init__(*a)
+            self.U = self.V = self.W = self.rhs = self.P = None
+
+        def plan(self, precision):
+            return self.plans.setdefault(precision, _Plan())
+
+    class _Stream:
+        handle = 0
+
+        def synchronize(self):
+            # the case must still be marked busy while its stream drains
+            log.append(("sync", [r.busy for r in s.resident.values()]))
+
+    monkeypatch.setattr(serving, "_Resident", _Res)
+    s = _bare_server(1e12)
+    s.solo_cells = 1  # one request runs alone through its plan
+    s.streams = [_Stream()]
+    q = Request(0, 7, (16, 8, 16), 1.5, seed=0)
+    s.submit(q)
+    with pytest.raises(RuntimeError, match="launch failed"):
+        s._launch(0)
+    assert log == ["synth", ("sync", [True])]
+    assert not s.resident[7].busy
+    assert [p.rid for p in s.pending] == [0]

@@ -98,7 +98,17 @@ def test_fft_stage_matches_naive_dft():
     assert np.allclose(np.fft.ifft(a, axis=1) * 12, R.naive_dft(a, 1, +1), atol=1e-12)
 
 
-@pytest.mark.parametrize("dims", [(8, 6, 4), (6, 5, 6), (4, 3, 9)])
+def test_dyy_of_one_cell_between_the_walls():
+    # initialization.f90:523-535 assigns Dyy(2,2) twice when nyg-1 = 2: b+c,
+    # then a+b, which stays.  a+b = -c = -1/((y(2)-y(1)) (yg(2)-yg(1))).
+    g = R.channel_grid(8, 1, 4, 1.5)
+    D = R.dyy_matrix(g.y, g.yg)
+    assert D.shape == (1, 1)
+    c = 1.0 / ((g.y[1] - g.y[0]) * (g.yg[1] - g.yg[0]))
+    assert D[0, 0] == pytest.approx(-c, rel=1e-14)
+
+
+@pytest.mark.parametrize("dims", [(8, 6, 4), (6, 5, 6), (4, 3, 9), (8, 1, 6)])
 def test_solve_matches_dense_pinned_operator(dims):
     nxp, nym, nzp = dims
     g = R.channel_grid(nxp, nym, nzp, 1.7)
