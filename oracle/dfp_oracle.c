@@ -19,12 +19,21 @@
  *                       boundary_conditions.f90:277-289, :591-602
  *   correction ........ projection.f90:551-604
  *   max divergence .... projection.f90:606-639
+ *   x inflow/outflow .. nxp = nxm (initialization.f90:331-338), DCT-IV
+ *                       wavenumbers :436-449, REDFT11 forward
+ *                       projection.f90:126-129 and inverse :333-339 with
+ *                       /(2*nxp*nzp), no (0,0) pin (:292), backflow term
+ *                       :65-96, no x periodic fill :446-468, Neumann x
+ *                       ghosts of P :413-422, outflow face :578-589
  * Third-party algorithms restated from their published definitions:
  *   LAPACK zgtsv (Gaussian elimination with partial pivoting on a
  *   tridiagonal matrix; LAPACK version unpinned by the reference,
  *   CMakeLists.txt:50-51) and FFTW's unnormalised complex DFT
  *   (FFTW_FORWARD = e^{-2 pi i jk/n}); the DFT is computed with this file's
- *   own mixed-radix Stockham FFT.
+ *   own mixed-radix Stockham FFT.  FFTW's REDFT11 (DCT-IV,
+ *   y_k = 2 sum_n x_n cos(pi (2n+1)(2k+1) / 4n)) is computed from a
+ *   half-length complex DFT with pre- and post-twiddles for even n, by its
+ *   definition for odd n.
  * At one rank the 2decomp&fft transposes are data copies that do not change
  * values; they are left out.
  *
@@ -277,7 +286,13 @@ static void centres_with_ghosts(int n, const double *f, double *g) {
 void ora_channel_grid(int nxp, int nym, int nzp, double Lx_i, double Ly_i, double Lz_i,
                       double alpha, int grid_type, double *x, double *y, double *z,
                       double *xg, double *yg, double *zg, double *dx, double *dz) {
-    int nx = nxp + 2, ny = nym + 1, nz = nzp + 2;
+    ora_channel_grid_x(nxp, nym, nzp, Lx_i, Ly_i, Lz_i, alpha, grid_type, 0, x, y, z, xg, yg, zg, dx, dz);
+}
+
+void ora_channel_grid_x(int nxp, int nym, int nzp, double Lx_i, double Ly_i, double Lz_i,
+                        double alpha, int grid_type, int x_bc_type, double *x, double *y, double *z,
+                        double *xg, double *yg, double *zg, double *dx, double *dz) {
+    int nx = nxp + (x_bc_type ? 1 : 2), ny = nym + 1, nz = nzp + 2;
     for (int i = 1; i <= nx; i++) x[i - 1] = Lx_i * (double)(i - 1) / (double)(nx - 1);
     for (int i = 1; i <= nz; i++) z[i - 1] = Lz_i * (double)(i - 1) / (double)(nz - 1);
     if (grid_type == 1) {
@@ -303,6 +318,9 @@ void ora_channel_grid(int nxp, int nym, int nzp, double Lx_i, double Ly_i, doubl
  * plan
  * ------------------------------------------------------------------------- */
 struct ora_plan {
+    int xbc;  /* x_bc_type */
+    double beta, outflow_Uc;
+    double *Pb, *phib; /* (nyg-2) x (nzg-2) */
     int nxp, nym, nzp;
     int nx, ny, nz, nxg, nyg, nzg;
     double dx, dz;
@@ -313,6 +331,76 @@ struct ora_plan {
     cplx *work; /* poisson_y_c at np = 1: nxp*nym*nzp */
     int nthreads;
 };
+
+static void wavenumbers_dct(int n, double d, double *k) {
+    /* initialization.f90:443-445 */
+    const double pi = 4.0 * atan(1.0);
+    for (int i = 0; i < n; i++) k[i] = 2.0 * (cos(pi * (double)(2 * i + 1) / (2.0 * (double)n)) - 1.0) / pow(d, 2.0);
+}
+
+/* DCT-IV, REDFT11 scaling: y_k = 2 sum_n x_n cos(pi (2n+1)(2k+1) / (4N)).
+ * Even N = 2M: v_n = (x_{2n} + i x_{N-1-2n}) exp(-i pi (4n+1) / (4N)),
+ * V = DFT_M(v), Z_k = V_k exp(-i pi k / N); then y_{2k} = 2 Re Z_k and
+ * y_{N-1-2k} = -2 Im Z_k.  Odd N: the definition (O(N^2)). */
+static void dct4_many(int n, long howmany, double *data, int nthreads) {
+    const long double pil = 3.14159265358979323846264338327950288L;
+    if (n % 2) {
+#pragma omp parallel num_threads(nthreads)
+        {
+            double *t = (double *)malloc(sizeof(double) * (size_t)n);
+#pragma omp for schedule(static)
+            for (long b = 0; b < howmany; b++) {
+                double *x = data + b * (long)n;
+                for (int k = 0; k < n; k++) {
+                    long double acc = 0.0L;
+                    for (int q = 0; q < n; q++) {
+                        long m8 = ((long)(2 * q + 1) * (2 * k + 1)) % (8L * n);
+                        acc += (long double)x[q] * cosl(pil * (long double)m8 / (4.0L * n));
+                    }
+                    t[k] = (double)(2.0L * acc);
+                }
+                memcpy(x, t, sizeof(double) * (size_t)n);
+            }
+            free(t);
+        }
+        return;
+    }
+    const int M = n / 2;
+    cplx *pre = (cplx *)malloc(sizeof(cplx) * (size_t)M);
+    cplx *post = (cplx *)malloc(sizeof(cplx) * (size_t)M);
+    for (int q = 0; q < M; q++) {
+        /* exp(-i pi (4q+1)/(4N)) = exp(-2 pi i (4q+1) / (8N)); exp(-i pi q / N) = exp(-2 pi i q / (2N)) */
+        pre[q] = expi_index(4L * q + 1, 8L * n, -1);
+        post[q] = expi_index(q, 2L * n, -1);
+    }
+    fplan f;
+    fplan_init(&f, M);
+    cplx *v = (cplx *)malloc(sizeof(cplx) * (size_t)M * (size_t)howmany);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (long b = 0; b < howmany; b++) {
+        const double *x = data + b * (long)n;
+        for (int q = 0; q < M; q++) {
+            cplx a = {x[2 * q], x[n - 1 - 2 * q]};
+            v[b * M + q] = cmul(a, pre[q]);
+        }
+    }
+    fft_many(&f, v, howmany, 1, M, -1, nthreads);
+#pragma omp parallel for schedule(static) num_threads(nthreads)
+    for (long b = 0; b < howmany; b++) {
+        double *x = data + b * (long)n;
+        for (int q = 0; q < M; q++) {
+            cplx z = cmul(v[b * M + q], post[q]);
+            x[2 * q] = 2.0 * z.re;
+            x[n - 1 - 2 * q] = -2.0 * z.im;
+        }
+    }
+    free(v);
+    free(pre);
+    free(post);
+    fplan_free(&f);
+}
+
+void ora_dct4_many(int n, int howmany, double *data) { dct4_many(n, howmany, data, 1); }
 
 static double *dup_array(const double *a, int n) {
     double *r = (double *)malloc(sizeof(double) * (size_t)n);
@@ -331,10 +419,12 @@ static void wavenumbers(int n, double d, double *k) {
 
 ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
     if (g->nxp < 1 || g->nym < 1 || g->nzp < 1) return NULL;
+    if (g->x_bc_type != 0 && g->x_bc_type != 1) return NULL;
     ora_plan *p = (ora_plan *)calloc(1, sizeof(ora_plan));
+    p->xbc = g->x_bc_type;
     p->nxp = g->nxp; p->nym = g->nym; p->nzp = g->nzp;
-    p->nx = g->nxp + 2; p->ny = g->nym + 1; p->nz = g->nzp + 2;
-    p->nxg = g->nxp + 3; p->nyg = g->nym + 2; p->nzg = g->nzp + 3;
+    p->nx = g->nxp + (p->xbc ? 1 : 2); p->ny = g->nym + 1; p->nz = g->nzp + 2;
+    p->nxg = p->nx + 1; p->nyg = g->nym + 2; p->nzg = g->nzp + 3;
     p->dx = g->dx; p->dz = g->dz;
     p->x = dup_array(g->x, p->nx);
     p->y = dup_array(g->y, p->ny);
@@ -343,8 +433,10 @@ ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
     p->zg = dup_array(g->zg, p->nzg);
     p->kxx = (double *)malloc(sizeof(double) * (size_t)p->nxp);
     p->kzz = (double *)malloc(sizeof(double) * (size_t)p->nzp);
-    wavenumbers(p->nxp, p->dx, p->kxx);
+    if (p->xbc) wavenumbers_dct(p->nxp, p->dx, p->kxx);
+    else wavenumbers(p->nxp, p->dx, p->kxx);
     wavenumbers(p->nzp, p->dz, p->kzz);
+    p->phib = (double *)calloc((size_t)(p->nyg - 2) * (size_t)(p->nzg - 2), sizeof(double));
     /* Dyy, initialization.f90:506-535; index jj = j-2 for reference row j */
     int nym = p->nym, nyg = p->nyg;
     p->dyy_l = (double *)calloc((size_t)(nym > 1 ? nym - 1 : 1), sizeof(double));
@@ -392,6 +484,7 @@ void ora_plan_destroy(ora_plan *p) {
     free(p->x); free(p->y); free(p->z); free(p->yg); free(p->zg);
     free(p->kxx); free(p->kzz);
     free(p->dyy_l); free(p->dyy_d); free(p->dyy_u);
+    free(p->Pb); free(p->phib);
     fplan_free(&p->fx); fplan_free(&p->fz);
     free(p->work);
     free(p);
@@ -435,10 +528,50 @@ void ora_divergence(const ora_plan *p, const double *U, const double *V, const d
     }
 }
 
+int ora_set_outflow(ora_plan *p, double beta, double outflow_Uc, const double *Pb) {
+    if (beta < 0.0 || (beta > 0.0 && (!p->xbc || !Pb))) return -1;  /* input_output.f90:312-314 */
+    p->beta = beta;
+    p->outflow_Uc = outflow_Uc;
+    size_t n = (size_t)(p->nyg - 2) * (size_t)(p->nzg - 2);
+    free(p->Pb);
+    p->Pb = NULL;
+    if (Pb) {
+        p->Pb = (double *)malloc(sizeof(double) * n);
+        memcpy(p->Pb, Pb, sizeof(double) * n);
+    }
+    return 0;
+}
+
+void ora_phib(const ora_plan *p, double *phib) {
+    memcpy(phib, p->phib, sizeof(double) * (size_t)(p->nyg - 2) * (size_t)(p->nzg - 2));
+}
+
+#define PB_(j, k) p->Pb[(size_t)((k) - 2) * (size_t)(p->nyg - 2) + (size_t)((j) - 2)]
+#define PHIB_(j, k) p->phib[(size_t)((k) - 2) * (size_t)(p->nyg - 2) + (size_t)((j) - 2)]
+
+static void set_outflow_backflow_pressure(ora_plan *p, const double *U, const double *V, const double *W,
+                                          double *rhs_p, double c) {
+    /* projection.f90:73-94, c = dt*rk_coef(rk_step,rk_step) */
+    const int nx = p->nx;
+    const double uref = 0.05 * (p->outflow_Uc > 1e-2 ? p->outflow_Uc : 1e-2);
+    const double inv_dx2 = 1.0 / pow(p->dx, 2);
+    for (int k = 2; k <= p->nzg - 1; k++)
+        for (int j = 2; j <= p->nyg - 1; j++) {
+            double un = U_(nx, j, k);
+            double vt = 0.5 * (V_(nx, j, k) + V_(nx, j - 1, k));
+            double wt = 0.5 * (W_(nx, j, k) + W_(nx, j, k - 1));
+            double th = 0.5 * (1.0 - tanh(un / uref));
+            PHIB_(j, k) = c * (-0.5 * p->beta * (un * un + vt * vt + wt * wt) * th - PB_(j, k));
+            R_(nx, j, k) = R_(nx, j, k) - 2.0 * PHIB_(j, k) * inv_dx2;
+        }
+}
+
 static void apply_periodic_xz_pressure(const ora_plan *p, double *rhs_p) {
-    /* projection.f90:453-456 and :479-482 (single rank) */
-    for (int k = 2; k <= p->nzg; k++)
-        for (int j = 2; j <= p->nyg - 1; j++) R_(p->nxg - 1, j, k) = R_(2, j, k);
+    /* projection.f90:453-456 and :479-482 (single rank); no x fill for the
+       inflow/outflow x (:446-448) */
+    if (!p->xbc)
+        for (int k = 2; k <= p->nzg; k++)
+            for (int j = 2; j <= p->nyg - 1; j++) R_(p->nxg - 1, j, k) = R_(2, j, k);
     for (int j = 2; j <= p->nyg - 1; j++)
         for (int i = 2; i <= p->nxg - 1; i++) R_(i, j, p->nzp + 2) = R_(i, j, 2);
 }
@@ -453,9 +586,15 @@ static void finish_pressure_ghosts(const ora_plan *p, double *P) {
         }
     for (int k = 1; k <= nzg; k++)
         for (int j = 1; j <= nyg; j++) {
-            P_(1, j, k) = P_(nxg - 2, j, k);
-            P_(nxg - 1, j, k) = P_(2, j, k);
-            P_(nxg, j, k) = P_(3, j, k);
+            if (p->xbc) {
+                /* :413-422 Neumann copies at the two x ends */
+                P_(1, j, k) = P_(2, j, k);
+                P_(nxg, j, k) = P_(nxg - 1, j, k);
+            } else {
+                P_(1, j, k) = P_(nxg - 2, j, k);
+                P_(nxg - 1, j, k) = P_(2, j, k);
+                P_(nxg, j, k) = P_(3, j, k);
+            }
         }
     for (int j = 1; j <= nyg; j++)
         for (int i = 1; i <= nxg; i++) {
@@ -468,16 +607,32 @@ static void finish_pressure_ghosts(const ora_plan *p, double *P) {
 int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
     const int nxp = p->nxp, nym = p->nym, nzp = p->nzp;
     cplx *wk = p->work;
-    /* :121 poisson_y_r = rhs_p(2:nxp+1, 2:nym+1, 2:nzp+1); :124 dcmplx */
+    if (p->xbc) {
+        /* :121 copy, :127 REDFT11 along x, :128 dcmplx */
+        double *xr = (double *)malloc(sizeof(double) * (size_t)nxp * nym * nzp);
 #pragma omp parallel for schedule(static) num_threads(p->nthreads)
-    for (int k = 0; k < nzp; k++)
-        for (int j = 0; j < nym; j++)
-            for (int i = 0; i < nxp; i++) {
-                cplx v = {R_(i + 2, j + 2, k + 2), 0.0};
-                wk[((size_t)k * nym + j) * nxp + i] = v;
-            }
-    /* :125 C2C forward along x, batch nym*nzp, contiguous lines */
-    fft_many(&p->fx, wk, (long)nym * nzp, 1, nxp, -1, p->nthreads);
+        for (int k = 0; k < nzp; k++)
+            for (int j = 0; j < nym; j++)
+                for (int i = 0; i < nxp; i++) xr[((size_t)k * nym + j) * nxp + i] = R_(i + 2, j + 2, k + 2);
+        dct4_many(nxp, (long)nym * nzp, xr, p->nthreads);
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+        for (size_t e = 0; e < (size_t)nxp * nym * nzp; e++) {
+            cplx v = {xr[e], 0.0};
+            wk[e] = v;
+        }
+        free(xr);
+    } else {
+        /* :121 poisson_y_r = rhs_p(2:nxp+1, 2:nym+1, 2:nzp+1); :124 dcmplx */
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+        for (int k = 0; k < nzp; k++)
+            for (int j = 0; j < nym; j++)
+                for (int i = 0; i < nxp; i++) {
+                    cplx v = {R_(i + 2, j + 2, k + 2), 0.0};
+                    wk[((size_t)k * nym + j) * nxp + i] = v;
+                }
+        /* :125 C2C forward along x, batch nym*nzp, contiguous lines */
+        fft_many(&p->fx, wk, (long)nym * nzp, 1, nxp, -1, p->nthreads);
+    }
     /* :134-136 C2C forward along z, stride nxp*nym, batch nxp*nym */
     fft_many(&p->fz, wk, (long)nxp * nym, (long)nxp * nym, 1, -1, p->nthreads);
     /* :277-297 one Zgtsv per (i,k) mode */
@@ -497,7 +652,7 @@ int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
                     DU[j] = p->dyy_u[j];
                 }
                 for (int j = 0; j < nym; j++) D[j] = p->dyy_d[j] + wavenum_sum;
-                if (i == 0 && k == 0) D[0] = 3.0 / 2.0 * D[0];
+                if (!p->xbc && i == 0 && k == 0) D[0] = 3.0 / 2.0 * D[0];
                 cplx *line = wk + (size_t)k * nym * nxp + i;
                 for (int j = 0; j < nym; j++) b[j] = line[(size_t)j * nxp];
                 int info = zgtsv_real_coef(nym, DL, D, DU, b);
@@ -510,15 +665,29 @@ int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
         }
         free(DL); free(DU); free(D); free(b);
     }
-    /* :308-310 inverse z; :325 inverse x */
+    /* :308-310 inverse z */
     fft_many(&p->fz, wk, (long)nxp * nym, (long)nxp * nym, 1, +1, p->nthreads);
-    fft_many(&p->fx, wk, (long)nym * nzp, 1, nxp, +1, p->nthreads);
-    /* :327 Real()/(nxp*nzp); :342 back into rhs_p */
-    const double norm = (double)((long)nxp * nzp);
+    if (p->xbc) {
+        /* :333 Real(), :334 REDFT11, :336 /(2*nxp*nzp); :342 back into rhs_p */
+        double *xr = (double *)malloc(sizeof(double) * (size_t)nxp * nym * nzp);
 #pragma omp parallel for schedule(static) num_threads(p->nthreads)
-    for (int k = 0; k < nzp; k++)
-        for (int j = 0; j < nym; j++)
-            for (int i = 0; i < nxp; i++) R_(i + 2, j + 2, k + 2) = wk[((size_t)k * nym + j) * nxp + i].re / norm;
+        for (size_t e = 0; e < (size_t)nxp * nym * nzp; e++) xr[e] = wk[e].re;
+        dct4_many(nxp, (long)nym * nzp, xr, p->nthreads);
+        const double norm = (double)(2L * nxp * nzp);
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+        for (int k = 0; k < nzp; k++)
+            for (int j = 0; j < nym; j++)
+                for (int i = 0; i < nxp; i++) R_(i + 2, j + 2, k + 2) = xr[((size_t)k * nym + j) * nxp + i] / norm;
+        free(xr);
+    } else {
+        /* :325 inverse x; :327 Real()/(nxp*nzp); :342 back into rhs_p */
+        fft_many(&p->fx, wk, (long)nym * nzp, 1, nxp, +1, p->nthreads);
+        const double norm = (double)((long)nxp * nzp);
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+        for (int k = 0; k < nzp; k++)
+            for (int j = 0; j < nym; j++)
+                for (int i = 0; i < nxp; i++) R_(i + 2, j + 2, k + 2) = wk[((size_t)k * nym + j) * nxp + i].re / norm;
+    }
     apply_periodic_xz_pressure(p, rhs_p);
     /* :357-358 single rank: no interior halo neighbours */
     if (P) {
@@ -540,6 +709,15 @@ void ora_correct(const ora_plan *p, double *U, double *V, double *W, const doubl
     for (int k = 2; k <= p->nzg - 1; k++)
         for (int j = 2; j <= p->nyg - 1; j++)
             for (int i = 2; i <= p->nx - 1; i++) U_(i, j, k) = U_(i, j, k) - (R_(i + 1, j, k) - R_(i, j, k)) * inv_dx;
+    if (p->xbc) {
+        /* :579-589 outflow face */
+        const int nx = p->nx;
+        for (int k = 2; k <= p->nzg - 1; k++)
+            for (int j = 2; j <= p->nyg - 1; j++) {
+                if (p->beta > 0.0) U_(nx, j, k) = U_(nx, j, k) + 2.0 * (R_(nx, j, k) - PHIB_(j, k)) * inv_dx;
+                else U_(nx, j, k) = U_(nx, j, k) + 2.0 * R_(nx, j, k) * inv_dx;
+            }
+    }
 #pragma omp parallel for schedule(static) num_threads(p->nthreads)
     for (int k = 2; k <= p->nzg - 1; k++)
         for (int j = 2; j <= p->ny - 1; j++) {
@@ -556,6 +734,7 @@ void ora_correct(const ora_plan *p, double *U, double *V, double *W, const doubl
 
 int ora_project(ora_plan *p, double *U, double *V, double *W, double *rhs_p, double *P, double dt, double c) {
     ora_divergence(p, U, V, W, rhs_p);
+    if (p->beta > 0.0) set_outflow_backflow_pressure(p, U, V, W, rhs_p, dt * c);
     int rc = ora_solve(p, rhs_p, P, dt, c);
     ora_correct(p, U, V, W, rhs_p);
     return rc;

@@ -6,14 +6,16 @@
  * the checker and as the host-CPU baseline.
  *
  * It restates the reference's CPU (FFTW + LAPACK) branch of the projection
- * for the channel case (x periodic, y walls, z periodic) on one rank; every
- * function cites the reference lines it follows in dfp_oracle.c.
+ * for the channel case (x periodic, y walls, z periodic) and its x
+ * inflow/outflow branch on one rank; every function cites the reference lines
+ * it follows in dfp_oracle.c.
  *
  * Arrays are the reference's Fortran arrays by address (column-major, x
  * fastest):
  *   U(nx,nyg,nzg)  V(nxg,ny,nzg)  W(nxg,nyg,nz)  P(nxg,nyg,nzg)
  *   rhs_p(2:nxg, 2:nyg-1, 2:nzg)  -> extents (nxg-1, nyg-2, nzg-1)
- * with nx=nxp+2, ny=nym+1, nz=nzp+2, nxg=nxp+3, nyg=nym+2, nzg=nzp+3.
+ * with nx=nxp+2, ny=nym+1, nz=nzp+2, nxg=nxp+3, nyg=nym+2, nzg=nzp+3 (x
+ * inflow/outflow: nx=nxp+1, nxg=nxp+2).
  */
 #ifndef DFP_ORACLE_H
 #define DFP_ORACLE_H
@@ -30,6 +32,7 @@ typedef struct ora_grid {
     const double *z;      /* faces  z(1:nz)   */
     const double *yg;     /* centres yg(1:nyg) */
     const double *zg;     /* centres zg(1:nzg) */
+    int x_bc_type;        /* 0 periodic; 1 inflow/outflow (nx = nxp+1, nxg = nxp+2) */
 } ora_grid;
 
 typedef struct ora_plan ora_plan;
@@ -41,11 +44,27 @@ void ora_channel_grid(int nxp, int nym, int nzp, double Lx_i, double Ly_i, doubl
                       double alpha, int grid_type, double *x, double *y, double *z,
                       double *xg, double *yg, double *zg, double *dx, double *dz);
 
+/* the same with the x boundary type (1: nx = nxp+1 faces over Lx_i) */
+void ora_channel_grid_x(int nxp, int nym, int nzp, double Lx_i, double Ly_i, double Lz_i,
+                        double alpha, int grid_type, int x_bc_type, double *x, double *y, double *z,
+                        double *xg, double *yg, double *zg, double *dx, double *dz);
+
 ora_plan *ora_plan_create(const ora_grid *g, int nthreads);
 void ora_plan_destroy(ora_plan *p);
 
 /* operators: kxx[nxp], kzz[nzp]; Dyy tridiagonal as lower[nym-1], diag[nym], upper[nym-1] */
 void ora_operators(const ora_plan *p, double *kxx, double *kzz, double *dl, double *d, double *du);
+
+/* x inflow/outflow: outlet backflow state (outflow_backflow_beta, outflow_Uc,
+ * Pb(2:nyg-1, 2:nzg-1) as a (nyg-2) x (nzg-2) Fortran array); beta = 0 turns
+ * the term off.  ora_project then applies set_outflow_backflow_pressure
+ * (projection.f90:65-96) with c = rk_coef(rk_step,rk_step) and the outflow
+ * face correction with phib (:578-589).  ora_phib copies phib out. */
+int ora_set_outflow(ora_plan *p, double beta, double outflow_Uc, const double *Pb);
+void ora_phib(const ora_plan *p, double *phib);
+
+/* DCT-IV with FFTW's REDFT11 scaling along contiguous lines of n points */
+void ora_dct4_many(int n, int howmany, double *data);
 
 /* projection.f90:37-63 */
 void ora_divergence(const ora_plan *p, const double *U, const double *V, const double *W, double *rhs_p);

@@ -17,6 +17,7 @@ class _Grid(ctypes.Structure):
         ("nxp", ctypes.c_int), ("nym", ctypes.c_int), ("nzp", ctypes.c_int),
         ("dx", ctypes.c_double), ("dz", ctypes.c_double),
         ("x", _dp), ("y", _dp), ("z", _dp), ("yg", _dp), ("zg", _dp),
+        ("x_bc_type", ctypes.c_int),
     ]
 
 
@@ -49,6 +50,11 @@ def lib():
         L.ora_threads.argtypes = [vp]
         L.ora_threads.restype = ctypes.c_int
         L.ora_channel_grid.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [ctypes.c_int] + [vp] * 8
+        L.ora_channel_grid_x.argtypes = [ctypes.c_int] * 3 + [ctypes.c_double] * 4 + [ctypes.c_int] * 2 + [vp] * 8
+        L.ora_set_outflow.argtypes = [vp, ctypes.c_double, ctypes.c_double, vp]
+        L.ora_set_outflow.restype = ctypes.c_int
+        L.ora_phib.argtypes = [vp, vp]
+        L.ora_dct4_many.argtypes = [ctypes.c_int, ctypes.c_int, vp]
         _lib = L
     return _lib
 
@@ -71,7 +77,7 @@ class CProjection:
         x, y, z, yg, zg = self._keep
         gd = _Grid(g.nxp, g.nym, g.nzp, g.dx, g.dz,
                    x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), z.ctypes.data_as(_dp),
-                   yg.ctypes.data_as(_dp), zg.ctypes.data_as(_dp))
+                   yg.ctypes.data_as(_dp), zg.ctypes.data_as(_dp), int(getattr(g, "x_bc_type", 0)))
         self._p = lib().ora_plan_create(ctypes.byref(gd), int(nthreads))
         if not self._p:
             raise ValueError("ora_plan_create failed")
@@ -86,6 +92,25 @@ class CProjection:
             self.close()
         except Exception:
             pass
+
+    def set_outflow(self, beta, outflow_Uc, Pb):
+        """Outlet backflow state (x inflow/outflow): Pb has shape (nyg-2, nzg-2)."""
+        g = self.g
+        pb = None
+        if Pb is not None:
+            pb = np.asfortranarray(Pb, dtype=np.float64)
+            if pb.shape != (g.nyg - 2, g.nzg - 2):
+                raise ValueError("Pb must have shape (nyg-2, nzg-2)")
+        self._pb = pb
+        if lib().ora_set_outflow(self._p, float(beta), float(outflow_Uc), _ptr(pb)) != 0:
+            raise ValueError("ora_set_outflow refused the backflow state")
+
+    @property
+    def phib(self):
+        g = self.g
+        out = np.zeros((g.nyg - 2, g.nzg - 2), order="F")
+        lib().ora_phib(self._p, _ptr(out))
+        return out
 
     @property
     def threads(self):
@@ -144,12 +169,20 @@ def zgtsv(dl, d, du, b):
     return x, info
 
 
-def channel_grid_arrays(nxp, nym, nzp, Lx_i, Ly_i, Lz_i, alpha, grid_type=2):
-    nx, ny, nz = nxp + 2, nym + 1, nzp + 2
+def channel_grid_arrays(nxp, nym, nzp, Lx_i, Ly_i, Lz_i, alpha, grid_type=2, x_bc_type=0):
+    nx, ny, nz = nxp + (1 if x_bc_type else 2), nym + 1, nzp + 2
     x = np.zeros(nx); y = np.zeros(ny); z = np.zeros(nz)
     xg = np.zeros(nx + 1); yg = np.zeros(ny + 1); zg = np.zeros(nz + 1)
     dx = ctypes.c_double(); dz = ctypes.c_double()
-    lib().ora_channel_grid(nxp, nym, nzp, Lx_i, Ly_i, Lz_i, alpha, grid_type,
-                           _ptr(x), _ptr(y), _ptr(z), _ptr(xg), _ptr(yg), _ptr(zg),
-                           ctypes.addressof(dx), ctypes.addressof(dz))
+    lib().ora_channel_grid_x(nxp, nym, nzp, Lx_i, Ly_i, Lz_i, alpha, grid_type, int(x_bc_type),
+                             _ptr(x), _ptr(y), _ptr(z), _ptr(xg), _ptr(yg), _ptr(zg),
+                             ctypes.addressof(dx), ctypes.addressof(dz))
     return dict(x=x, y=y, z=z, xg=xg, yg=yg, zg=zg, dx=dx.value, dz=dz.value)
+
+
+def dct4(a, axis=0):
+    """DCT-IV (REDFT11 scaling) through the C restatement, along ``axis``."""
+    b = np.array(np.moveaxis(np.asarray(a, dtype=np.float64), axis, -1), order="C", copy=True)
+    n = b.shape[-1]
+    lib().ora_dct4_many(n, b.size // n, _ptr(b))
+    return np.moveaxis(b, -1, axis)

@@ -23,13 +23,26 @@ periodic: ``x_bc_type=0, y_bc_type=1, z_bc_type=0``) on one rank:
 * velocity correction ......... src/projection.f90:551-604 (single rank)
 * max divergence .............. src/projection.f90:606-639 (no IBM)
 
+and the x inflow/outflow branch (``x_bc_type=1``, y walls, z periodic):
+
+* grid: all nxm cells kept ..... src/initialization.f90:331-338 (nxp = nxm, Lxp = Lx)
+* DCT-IV wavenumbers .......... src/initialization.f90:436-449
+* x transform: FFTW REDFT11 ... src/projection.f90:126-129 (forward),
+                                :333-339 (inverse, /(2*nxp*nzp)); no (0,0) pin
+* outlet backflow term ........ src/projection.f90:65-96 (set_outflow_backflow_pressure)
+* no x periodic fill .......... src/projection.f90:446-468
+* Neumann x ghosts of P ....... src/projection.f90:413-422
+* outflow face correction ..... src/projection.f90:578-589
+
 LAPACK ``zgtsv`` (unpinned LAPACK version, CMakeLists.txt:50-51; the routine's
 algorithm is unchanged since LAPACK 3.0) is restated from its published
 algorithm: Gaussian elimination with partial pivoting on the tridiagonal,
 then back substitution with the second superdiagonal created by row
 interchanges.  FFTW3 (``fftw_plan_many_dft``, FFTW_FORWARD = e^{-i}) is
 represented by ``numpy.fft`` (pocketfft) with the same unnormalised
-conventions; FFT roundoff differs from FFTW's at the 1e-16 level.
+conventions; FFT roundoff differs from FFTW's at the 1e-16 level.  FFTW's
+REDFT11 (DCT-IV, y_k = 2 sum_n x_n cos(pi (2n+1)(2k+1) / 4N)) is represented
+by ``scipy.fft.dct(type=4)``, whose definition is the same.
 
 Array conventions: every 3-D array is a numpy array in Fortran order whose
 element ``[i-1, j-1, k-1]`` is the reference's ``A(i,j,k)`` (1-based).  The
@@ -43,7 +56,12 @@ naive O(n^2) DFT; ``zgtsv`` by dense ``numpy.linalg.solve`` and by a
 case forced through the row-interchange branch; the whole solve by a dense
 solve of the assembled 7-point operator with the pin on small grids; the
 projection by a manufactured solution u* = curl_h(A) + grad_h(phi_true) and
-by idempotence.
+by idempotence.  The inflow/outflow branch: the DCT-IV by a naive O(n^2) sum,
+its wavenumbers by the eigenvalue identity of the 3-point Laplacian with a
+Neumann inflow ghost and a zero outflow face value, the solve (backflow face
+value included) by a dense solve of the assembled 7-point operator, the
+projection by a manufactured solution and by a divergence-free result with
+the backflow term on.
 """
 from __future__ import annotations
 
@@ -51,6 +69,7 @@ import math
 from dataclasses import dataclass, field
 
 import numpy as np
+import scipy.fft
 
 # global.f90:21  pi = 4d0*datan(1d0)
 PI = 4.0 * math.atan(1.0)
@@ -80,6 +99,7 @@ class ChannelGrid:
     Lz_i: float
     alpha: float
     grid_type: int = 2
+    x_bc_type: int = 0   # 0 periodic, 1 inflow/outflow (global.f90:185-186)
     x: np.ndarray = field(init=False)
     y: np.ndarray = field(init=False)
     z: np.ndarray = field(init=False)
@@ -114,10 +134,12 @@ class ChannelGrid:
         self.dx = float(np.min(self.xg[1:] - self.xg[:-1]))
         self.dz = float(np.min(self.zg[1:] - self.zg[:-1]))
 
-    # face counts and ghosted cell counts (initialization.f90:170-191)
+    # face counts and ghosted cell counts (initialization.f90:170-191); the
+    # periodic x drops the duplicate cell from the transform (nxp = nxm - 1),
+    # the inflow/outflow x keeps every cell (nxp = nxm, :331-338)
     @property
     def nx(self):
-        return self.nxp + 2
+        return self.nxp + 2 if self.x_bc_type == 0 else self.nxp + 1
 
     @property
     def ny(self):
@@ -129,7 +151,7 @@ class ChannelGrid:
 
     @property
     def nxg(self):
-        return self.nxp + 3
+        return self.nx + 1
 
     @property
     def nyg(self):
@@ -173,18 +195,20 @@ def _centres_with_ghosts(f: np.ndarray) -> np.ndarray:
     return g
 
 
-def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * PI, Lz=PI, Ly=2.0, grid_type=2):
+def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * PI, Lz=PI, Ly=2.0, grid_type=2, x_bc_type=0):
     """Channel grid whose periodic lengths are ``Lx`` and ``Lz``.
 
     The reference's periodic period is ``Lx_i - dx`` (initialization.f90:332),
-    so the namelist length is ``Lx_i = Lx*(nxp+1)/nxp``.
+    so the namelist length is ``Lx_i = Lx*(nxp+1)/nxp``.  With the x
+    inflow/outflow branch (``x_bc_type=1``) the nxp cells span ``Lx``.
     """
-    return ChannelGrid(nxp, nym, nzp, Lx * (nxp + 1) / nxp, Ly, Lz * (nzp + 1) / nzp, alpha, grid_type)
+    Lx_i = Lx * (nxp + 1) / nxp if x_bc_type == 0 else Lx
+    return ChannelGrid(nxp, nym, nzp, Lx_i, Ly, Lz * (nzp + 1) / nzp, alpha, grid_type, x_bc_type)
 
 
-def namelist_grid(nx, ny, nz, Lx, Ly, Lz, alpha, grid_type=2):
+def namelist_grid(nx, ny, nz, Lx, Ly, Lz, alpha, grid_type=2, x_bc_type=0):
     """Grid from the reference's own namelist values (face counts and lengths)."""
-    return ChannelGrid(nx - 2, ny - 1, nz - 2, Lx, Ly, Lz, alpha, grid_type)
+    return ChannelGrid(nx - 2 if x_bc_type == 0 else nx - 1, ny - 1, nz - 2, Lx, Ly, Lz, alpha, grid_type, x_bc_type)
 
 
 # ---------------------------------------------------------------------------
@@ -200,6 +224,12 @@ def wavenumbers(n: int, d: float) -> np.ndarray:
     for i in range(half + 1, n):
         k[i] = 2.0 * (math.cos(2.0 * PI * float(-n + i) / float(n)) - 1.0) / d ** 2.0
     return k
+
+
+def wavenumbers_dct(n: int, d: float) -> np.ndarray:
+    """initialization.f90:443-445: DCT-IV (quarter-wave) modified wavenumbers
+    kxx(i) = 2 (cos(pi (2i+1) / (2n)) - 1) / d^2, i = 0..n-1."""
+    return np.array([2.0 * (math.cos(PI * float(2 * i + 1) / (2.0 * float(n))) - 1.0) / d ** 2.0 for i in range(n)])
 
 
 def dyy_matrix(y: np.ndarray, yg: np.ndarray) -> np.ndarray:
@@ -311,9 +341,43 @@ class Projection:
 
     def __init__(self, g: ChannelGrid):
         self.g = g
-        self.kxx = wavenumbers(g.nxp, g.dx)
+        self.kxx = wavenumbers(g.nxp, g.dx) if g.x_bc_type == 0 else wavenumbers_dct(g.nxp, g.dx)
         self.kzz = wavenumbers(g.nzp, g.dz)
         self.Dyy = dyy_matrix(g.y, g.yg)
+        # outlet backflow stabilisation (global.f90:188-197): off unless beta > 0
+        self.beta = 0.0
+        self.outflow_Uc = 0.0
+        self.Pb = None     # (nyg-2, nzg-2): Pb(2:nyg-1, 2:nzg-1)
+        self.phib = None   # same shape, written by set_outflow_backflow_pressure
+
+    def set_outflow(self, beta, outflow_Uc, Pb):
+        """Outlet backflow state of the x inflow/outflow case: beta
+        (outflow_backflow_beta), the plane-mean outlet speed and the outlet
+        face pressure Pb(2:nyg-1, 2:nzg-1) of the last stage."""
+        g = self.g
+        if g.x_bc_type != 1 and beta > 0.0:
+            raise ValueError("outflow_backflow_beta > 0 needs x_bc_type = 1 (input_output.f90:313-314)")
+        self.beta, self.outflow_Uc = float(beta), float(outflow_Uc)
+        self.Pb = np.array(Pb, dtype=np.float64, order="F") if Pb is not None else None
+        self.phib = np.zeros((g.nyg - 2, g.nzg - 2), order="F")
+
+    # -- projection.f90:65-96 ---------------------------------------------------
+    def set_outflow_backflow_pressure(self, U, V, W, rhs_p, dtc):
+        """phib(j,k) = c*(-beta/2*|u|^2*H(-u_n) - Pb(j,k)), H smoothed over 5 %
+        of the outlet speed; rhs_p(nx,j,k) -= 2*phib/dx^2; c = dt*rk_coef(s,s)."""
+        g = self.g
+        nx = g.nx
+        uref = 0.05 * max(self.outflow_Uc, 1e-2)
+        inv_dx2 = 1.0 / g.dx ** 2
+        for k in range(2, g.nzg):
+            for j in range(2, g.nyg):
+                un = U[nx - 1, j - 1, k - 1]
+                vt = 0.5 * (V[nx - 1, j - 1, k - 1] + V[nx - 1, j - 2, k - 1])
+                wt = 0.5 * (W[nx - 1, j - 1, k - 1] + W[nx - 1, j - 1, k - 2])
+                th = 0.5 * (1.0 - math.tanh(un / uref))
+                ph = dtc * (-0.5 * self.beta * (un * un + vt * vt + wt * wt) * th - self.Pb[j - 2, k - 2])
+                self.phib[j - 2, k - 2] = ph
+                rhs_p[nx - 2, j - 2, k - 2] = rhs_p[nx - 2, j - 2, k - 2] - 2.0 * ph * inv_dx2
 
     # -- projection.f90:37-63 ------------------------------------------------
     def compute_pseudo_pressure_rhs(self, U, V, W, rhs_p):
@@ -335,10 +399,15 @@ class Projection:
     def solve_poisson_equation(self, rhs_p, P=None, dt=None, rk_coef33=None):
         g = self.g
         nxp, nym, nzp = g.nxp, g.nym, g.nzp
+        xdct = g.x_bc_type == 1
         # :121 copy of the interior (decomp_poisson%ysz = nxp, nym, nzp at np=1)
         ry = np.array(rhs_p[0:nxp, 0:nym, 0:nzp], dtype=np.float64)
-        # :124-125 dcmplx + complex FFT along x (FFTW_FORWARD, unnormalised)
-        c = np.fft.fft(ry.astype(np.complex128), axis=0)
+        if not xdct:
+            # :124-125 dcmplx + complex FFT along x (FFTW_FORWARD, unnormalised)
+            c = np.fft.fft(ry.astype(np.complex128), axis=0)
+        else:
+            # :127-128 REDFT11 along x, then dcmplx
+            c = scipy.fft.dct(ry, type=4, axis=0).astype(np.complex128)
         # :134-136 complex FFT along z
         c = np.fft.fft(c, axis=2)
         # :277-297 one Zgtsv per (i,k) mode
@@ -349,7 +418,8 @@ class Projection:
         M = nxp * nzp
         D = diag[None, :] + lam.reshape(M, order="F")[:, None]  # (M, nym), mode m = i + nxp*k
         D = D.copy()
-        D[0, 0] = 3.0 / 2.0 * D[0, 0]                           # :292 pin of the (0,0) mode
+        if not xdct:
+            D[0, 0] = 3.0 / 2.0 * D[0, 0]                       # :292 pin of the (0,0) mode (x periodic only)
         B = np.transpose(c, (0, 2, 1)).reshape(M, nym, order="F")  # B[i + nxp*k, j]
         DL = np.broadcast_to(dl1, (M, nym - 1))
         DU = np.broadcast_to(du1, (M, nym - 1))
@@ -357,10 +427,15 @@ class Projection:
         if np.any(info != 0):
             raise FloatingPointError("zgtsv reported a zero pivot")
         c = np.transpose(X.reshape(nxp, nzp, nym, order="F"), (0, 2, 1))
-        # :308-310 inverse z, :325-327 inverse x, Real()/(nxp*nzp)
+        # :308-310 inverse z
         c = np.fft.ifft(c, axis=2) * nzp
-        c = np.fft.ifft(c, axis=0) * nxp
-        phi = c.real / float(nxp * nzp)
+        if not xdct:
+            # :325-327 inverse x, Real()/(nxp*nzp)
+            c = np.fft.ifft(c, axis=0) * nxp
+            phi = c.real / float(nxp * nzp)
+        else:
+            # :333-336 Real(), REDFT11, /(2*nxp*nzp)
+            phi = scipy.fft.dct(c.real, type=4, axis=0) / float(2 * nxp * nzp)
         # :342
         rhs_p[0:nxp, 0:nym, 0:nzp] = phi
         self.apply_periodic_xz_pressure(rhs_p)
@@ -373,8 +448,9 @@ class Projection:
     # -- projection.f90:441-500 single rank ---------------------------------
     def apply_periodic_xz_pressure(self, rhs_p):
         g = self.g
-        # :455 rhs_p(nxg-1,:,:) = rhs_p(2,:,:)
-        rhs_p[g.nxg - 1 - 2, :, :] = rhs_p[0, :, :]
+        if g.x_bc_type == 0:
+            # :455 rhs_p(nxg-1,:,:) = rhs_p(2,:,:) (inflow/outflow x: nothing, :446-448)
+            rhs_p[g.nxg - 1 - 2, :, :] = rhs_p[0, :, :]
         # :481 rhs_p(2:nxg-1,:,nzp+2) = rhs_p(2:nxg-1,:,2)   (local nzp = nzm-1 = nzp_global)
         rhs_p[0:g.nxg - 2, :, g.nzp + 2 - 2] = rhs_p[0:g.nxg - 2, :, 0]
 
@@ -384,9 +460,14 @@ class Projection:
         Pf[:, 0, :] = Pf[:, 1, :]
         Pf[:, g.nyg - 1, :] = Pf[:, g.nyg - 2, :]
         # update_ghost_interior_planes(_x): no neighbours on one rank
-        Pf[0, :, :] = Pf[g.nxg - 3, :, :]
-        Pf[g.nxg - 2, :, :] = Pf[1, :, :]
-        Pf[g.nxg - 1, :, :] = Pf[2, :, :]
+        if g.x_bc_type == 0:
+            Pf[0, :, :] = Pf[g.nxg - 3, :, :]
+            Pf[g.nxg - 2, :, :] = Pf[1, :, :]
+            Pf[g.nxg - 1, :, :] = Pf[2, :, :]
+        else:
+            # :413-422 Neumann copies at the inflow and outflow ends
+            Pf[0, :, :] = Pf[1, :, :]
+            Pf[g.nxg - 1, :, :] = Pf[g.nxg - 2, :, :]
         Pf[:, :, 0] = Pf[:, :, g.nzg - 3]
         Pf[:, :, g.nzg - 2] = Pf[:, :, 1]
         Pf[:, :, g.nzg - 1] = Pf[:, :, 2]
@@ -402,6 +483,12 @@ class Projection:
 
         # U, i = 2..nx-1 (the seam face nx is skipped on the last x row)
         U[1:nx - 1, 1:nyg - 1, 1:nzg - 1] -= (R(3, nx, 2, nyg - 1, 2, nzg - 1) - R(2, nx - 1, 2, nyg - 1, 2, nzg - 1)) * inv_dx
+        if g.x_bc_type == 1:
+            # :578-589 outflow face, face value phib (0 without the backflow term)
+            if self.beta > 0.0:
+                U[nx - 1, 1:nyg - 1, 1:nzg - 1] = U[nx - 1, 1:nyg - 1, 1:nzg - 1] + 2.0 * (R(nx, nx, 2, nyg - 1, 2, nzg - 1)[0] - self.phib) * inv_dx
+            else:
+                U[nx - 1, 1:nyg - 1, 1:nzg - 1] = U[nx - 1, 1:nyg - 1, 1:nzg - 1] + 2.0 * R(nx, nx, 2, nyg - 1, 2, nzg - 1)[0] * inv_dx
         # V, j = 2..ny-1, inv_dygj = 1/(yg(j+1)-yg(j))
         jj = np.arange(2, ny)
         inv_dyg = 1.0 / (g.yg[jj] - g.yg[jj - 1])
@@ -412,8 +499,12 @@ class Projection:
         W[1:nxg - 1, 1:nyg - 1, 1:nz - 1] -= (R(2, nxg - 1, 2, nyg - 1, 3, nz) - R(2, nxg - 1, 2, nyg - 1, 2, nz - 1)) * inv_dzg[None, None, :]
 
     # -- projection.f90:21-34 -------------------------------------------------
-    def compute_projection_step(self, U, V, W, rhs_p, P=None, dt=None, rk_coef33=None):
+    def compute_projection_step(self, U, V, W, rhs_p, P=None, dt=None, rk_coef33=None, c_stage=None):
+        """c_stage = rk_coef(rk_step, rk_step) of the backflow term (defaults
+        to rk_coef33: the P save runs at stage 3, where the two agree)."""
         self.compute_pseudo_pressure_rhs(U, V, W, rhs_p)
+        if self.beta > 0.0:
+            self.set_outflow_backflow_pressure(U, V, W, rhs_p, dt * (rk_coef33 if c_stage is None else c_stage))
         self.solve_poisson_equation(rhs_p, P, dt, rk_coef33)
         self.project_velocity(U, V, W, rhs_p)
 
@@ -492,3 +583,51 @@ def naive_dft(a: np.ndarray, axis: int, sign: int = -1) -> np.ndarray:
     w = np.exp(sign * 2j * np.pi * jk / n)
     out = a @ w.T
     return np.moveaxis(out, -1, axis)
+
+
+def naive_dct4(a: np.ndarray, axis: int) -> np.ndarray:
+    """O(n^2) DCT-IV with FFTW's REDFT11 scaling, y_k = 2 sum_n x_n
+    cos(pi (2n+1)(2k+1) / (4n)), exact-angle cosines (pins the x transform of
+    the inflow/outflow branch)."""
+    a = np.moveaxis(np.asarray(a, dtype=np.float64), axis, -1)
+    n = a.shape[-1]
+    nn = np.arange(n)
+    ang = (2 * nn[:, None] + 1) * (2 * nn[None, :] + 1) % (8 * n)
+    w = 2.0 * np.cos(np.pi * ang / (4.0 * n))
+    return np.moveaxis(a @ w, -1, axis)
+
+
+def dense_inflow_outflow_operator(g: ChannelGrid):
+    """The discrete Poisson operator of the x inflow/outflow branch, assembled
+    densely on the nxp*nym*nzp cells: 3-point second differences in x with a
+    Neumann inflow ghost (phi_{-1} = phi_0: the inflow face velocity is not
+    corrected) and a zero outflow face value (phi_N = -phi_{N-1}: the face
+    value enters the right-hand side, projection.f90:91), periodic z, Dyy in
+    y, no pin (the operator is not singular).  Index i + nxp*(j + nym*k)."""
+    nxp, nym, nzp = g.nxp, g.nym, g.nzp
+    n = nxp * nym * nzp
+    A = np.zeros((n, n))
+    Dyy = dyy_matrix(g.y, g.yg)
+    cx = 1.0 / g.dx ** 2
+    cz = 1.0 / g.dz ** 2
+
+    def idx(i, j, k):
+        return i + nxp * (j + nym * (k % nzp))
+
+    for k in range(nzp):
+        for j in range(nym):
+            for i in range(nxp):
+                r = idx(i, j, k)
+                # x: left neighbour (or the Neumann ghost), right neighbour (or -phi)
+                A[r, idx(i - 1 if i > 0 else 0, j, k)] += cx
+                if i < nxp - 1:
+                    A[r, idx(i + 1, j, k)] += cx
+                else:
+                    A[r, r] -= cx
+                A[r, r] -= 2.0 * cx
+                A[r, idx(i, j, k + 1)] += cz
+                A[r, idx(i, j, k - 1)] += cz
+                A[r, r] -= 2.0 * cz
+                for jj in range(max(0, j - 1), min(nym, j + 2)):
+                    A[r, idx(i, jj, k)] += Dyy[j, jj]
+    return A
