@@ -85,6 +85,15 @@ void wavenumbers(int n, double d, std::vector<double>& k) {
         k[i] = 2.0 * (std::cos(2.0 * pi * (double)(-n + i) / (double)n) - 1.0) / std::pow(d, 2.0);
 }
 
+// initialization.f90:443-445: DCT-IV (quarter-wave) modified wavenumbers of the
+// inflow/outflow x, kxx(i) = 2 (cos(pi (2i+1) / (2n)) - 1) / d^2, i = 0..n-1
+void wavenumbers_dct(int n, double d, std::vector<double>& k) {
+    const double pi = 4.0 * std::atan(1.0);
+    k.assign(n, 0.0);
+    for (int i = 0; i < n; ++i)
+        k[i] = 2.0 * (std::cos(pi * (double)(2 * i + 1) / (2.0 * (double)n)) - 1.0) / std::pow(d, 2.0);
+}
+
 // initialization.f90:506-535, rows j = 2..nyg-1 stored at j-2
 void dyy(const std::vector<double>& y, const std::vector<double>& yg, std::vector<double>& dl, std::vector<double>& dd,
          std::vector<double>& du) {
@@ -141,7 +150,9 @@ constexpr int kRadices[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
 // 256x192x256 FP32+refine 650 -> 628 us, the FP64 forward kernel
 // 113 -> 106 us (its inverse with 8 columns holds 66 KB of LDS, two
 // workgroups per CU instead of three, and lost 8 us, so it keeps 4).
-int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, int want, int inv) {
+// dct: the DCT-IV kernels of the inflow/outflow x (M points per line, M pre-
+// and M post-twiddles; the inverse holds at most 4096 points)
+int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, int want, int inv, int dct = 0) {
     const int M = nxp / 2, nz2 = nzp / sh.R;
     const bool f32 = cs == sizeof(float2);
     const bool pair = f32 || !inv;         // the kernels that pair radix-2 butterflies
@@ -149,6 +160,9 @@ int x_columns(const Shape& sh, int nxp, int nzp, size_t lds_limit, size_t cs, in
     const int cap = inplace_capacity(sh.fx, 512, slots, pair);
     auto fits = [&](int T) {
         const size_t pts = (size_t)sh.R * T;
+        if (dct)
+            return pts * M <= (size_t)cap && (!inv || pts * M <= 4096) &&
+                   cs * (pts * M + twiddle_count(sh.fx) + 2 * M + (size_t)nz2 * (sh.R - 1)) + 512 <= lds_limit;
         return pts * M <= (size_t)cap &&
                cs * (pts * (M + 1) + twiddle_count(sh.fx) + M + 1 + (size_t)nz2 * (sh.R - 1)) + 512 <= lds_limit;
     };
@@ -178,8 +192,9 @@ int choose_shape(const dfp_grid* g, int f32, size_t lds_limit, const dfp_tuning*
         Shape sh;
         sh.R = R;
         if (!factor(g->nxp / 2, sh.fx) || !factor(g->nzp / R, sh.fz)) continue;
-        sh.T = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 0);
-        sh.Ti = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 1);
+        const int dct = g->x_bc_type == 1 ? 1 : 0;
+        sh.T = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 0, dct);
+        sh.Ti = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 1, dct);
         if (sh.T == 0 || sh.Ti == 0 || sh.fz.n > 256) continue;
         if (allow_fused && zf_lds_bytes(g->nym, sh.fz.n, twiddle_count(sh.fz), f32) <= lds_limit) {
             Shape f = sh;
@@ -238,6 +253,8 @@ void free_plan_memory(dfp_plan* p) {
     (void)hipFree(p->d_W);
     (void)hipFree(p->d_rhs);
     (void)hipFree(p->d_P);
+    (void)hipFree(p->d_pb);
+    (void)hipFree(p->d_phib);
     for (auto& pe : p->pending) {
         (void)hipEventDestroy(pe.second.first);
         (void)hipEventDestroy(pe.second.second);
@@ -247,6 +264,21 @@ void free_plan_memory(dfp_plan* p) {
     if (p->join_ev) (void)hipEventDestroy(p->join_ev);
     if (p->aux) (void)hipStreamDestroy(p->aux);
 }
+
+// the outlet backflow term is on (dfp_set_outflow with beta > 0)
+bool backflow(const dfp_plan* p) { return p->geo.xbc == 1 && p->beta > 0.0; }
+
+// device staging of Pb and phib for host-resident calls
+int ensure_outflow_staging(dfp_plan* p) {
+    if (p->d_pb) return DFP_OK;
+    const size_t n = (size_t)(p->geo.nyg - 2) * (size_t)(p->geo.nzg - 2);
+    HIPCHK(p, hipMalloc(&p->d_pb, sizeof(double) * n));
+    HIPCHK(p, hipMalloc(&p->d_phib, sizeof(double) * n));
+    HIPCHK(p, hipMemset(p->d_phib, 0, sizeof(double) * n));
+    return DFP_OK;
+}
+
+int64_t face_size(const dfp_plan* p) { return (int64_t)(p->geo.nyg - 2) * (p->geo.nzg - 2); }
 
 int ensure_staging(dfp_plan* p) {
     if (p->d_U) return DFP_OK;
@@ -339,11 +371,12 @@ int mixed_solve(dfp_plan* p, const double* U, const double* V, const double* W, 
     return DFP_OK;
 }
 
-// the spectral chain on device arrays: rhs (read unless from_fields) -> phi in rhs (+P)
+// the spectral chain on device arrays: rhs (read unless from_fields) -> phi in rhs (+P);
+// phib: outlet face values of the backflow term (inflow/outflow x, from the fields only)
 int spectral_solve(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs, double* P, double dtc,
-                   int save_p, int from_fields) {
+                   int save_p, int from_fields, const double* phib = nullptr) {
     if (p->precision == DFP_FP32_REFINE) return mixed_solve(p, U, V, W, rhs, P, dtc, save_p, from_fields);
-    HIPCHK(p, launch_div_rfft_x(p, U, V, W, rhs, from_fields ? 0 : 1));
+    HIPCHK(p, launch_div_rfft_x(p, U, V, W, rhs, from_fields ? 0 : 1, phib));
     HIPCHK(p, z_solve(p, 0));
     HIPCHK(p, launch_irfft_x(p, rhs, P, dtc, save_p));
     return DFP_OK;
@@ -415,8 +448,11 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     if (!out) return DFP_E_ARG;
     *out = nullptr;
     if (!g || !g->y || !g->z || !g->yg || !g->zg) return DFP_E_ARG;
-    if (g->x_bc_type != 0 || g->y_bc_type != 1 || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
+    if (g->x_bc_type != 0 && g->x_bc_type != 1) return DFP_E_UNSUPPORTED;
+    if (g->y_bc_type != 1 || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
     if (precision != DFP_FP64 && precision != DFP_FP32_REFINE) return DFP_E_ARG;
+    // the inflow/outflow x is solved in FP64 only
+    if (g->x_bc_type == 1 && precision != DFP_FP64) return DFP_E_UNSUPPORTED;
     if (g->nxp < 2 || (g->nxp & 1) || g->nzp < 1) return DFP_E_UNSUPPORTED;
     if (g->nym < 1) return DFP_E_UNSUPPORTED;
     // an x block holds zr lines of nxp/2 complex points in 32 KiB of LDS; a z
@@ -454,9 +490,15 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     if (!sh.fused) z_two_pass(sh.fz.n, p->zn1, p->zn2);
     if (hipSetDevice(device) != hipSuccess) return bail(DFP_E_HIP);
     Geo& G = p->geo;
-    G.nxp = g->nxp; G.nym = g->nym; G.nzp = g->nzp; G.nxh = g->nxp / 2 + 1;
-    G.nx = g->nxp + 2; G.ny = g->nym + 1; G.nz = g->nzp + 2;
-    G.nxg = g->nxp + 3; G.nyg = g->nym + 2; G.nzg = g->nzp + 3;
+    const bool dct = g->x_bc_type == 1;
+    G.xbc = g->x_bc_type;
+    G.nxp = g->nxp; G.nym = g->nym; G.nzp = g->nzp;
+    // the periodic x keeps the R2C half of its modes (and drops the duplicate
+    // cell, nxp = nxm - 1); the inflow/outflow x keeps all nxm = nxp cells and
+    // all nxp DCT-IV modes (initialization.f90:331-338)
+    G.nxh = dct ? g->nxp : g->nxp / 2 + 1;
+    G.nx = g->nxp + (dct ? 1 : 2); G.ny = g->nym + 1; G.nz = g->nzp + 2;
+    G.nxg = G.nx + 1; G.nyg = g->nym + 2; G.nzg = g->nzp + 3;
     p->dx = g->dx; p->dz = g->dz;
     G.inv_dx = 1.0 / g->dx;
     if (g->x) p->hx.assign(g->x, g->x + G.nx);
@@ -474,7 +516,8 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     for (int k = 2; k <= G.nzg - 1; ++k) inv_dzf[k - 2] = 1.0 / (p->hz[k - 1] - p->hz[k - 2]);
     for (int j = 2; j <= G.ny - 1; ++j) inv_dyg[j - 2] = 1.0 / (p->hyg[j] - p->hyg[j - 1]);
     for (int k = 2; k <= G.nz - 1; ++k) inv_dzg[k - 2] = 1.0 / (p->hzg[k] - p->hzg[k - 1]);
-    wavenumbers(G.nxp, g->dx, p->hkxx);
+    if (dct) wavenumbers_dct(G.nxp, g->dx, p->hkxx);
+    else wavenumbers(G.nxp, g->dx, p->hkxx);
     wavenumbers(G.nzp, g->dz, p->hkzz);
     dyy(p->hy, p->hyg, p->hdl, p->hdd, p->hdu);
 
@@ -502,12 +545,21 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     p->ops.dl = T + o_dl; p->ops.dd = T + o_dd; p->ops.du = T + o_du;
 
     // twiddles, in the order the x kernels stage them into LDS with one
-    // copy: x stages, R2C/C2R packing (M+1), first z stage (nz2*(zr-1)); then
-    // the z-kernel stages
+    // copy: x stages, R2C/C2R packing (M+1) or the DCT-IV pre-twiddles
+    // e^{-i pi (4n+1)/(4 nxp)} and post-twiddles 2 e^{-i pi n/nxp} (M each),
+    // first z stage (nz2*(zr-1)); then the z-kernel stages
     std::vector<double2> tw;
     stage_twiddles(p->fx, tw);
     const size_t o_r2c = tw.size();
-    for (int k = 0; k <= G.nxp / 2; ++k) tw.push_back(expi(k, G.nxp));
+    if (dct) {
+        for (int n = 0; n < G.nxp / 2; ++n) tw.push_back(expi(4L * n + 1, 8L * G.nxp));
+        for (int n = 0; n < G.nxp / 2; ++n) {
+            const double2 w = expi(n, 2L * G.nxp);
+            tw.push_back(make_double2(2.0 * w.x, 2.0 * w.y));
+        }
+    } else {
+        for (int k = 0; k <= G.nxp / 2; ++k) tw.push_back(expi(k, G.nxp));
+    }
     const size_t o_zs = tw.size();
     for (int k = 0; k < G.nzp / zr; ++k)
         for (int u = 1; u < zr; ++u) tw.push_back(expi((long)k * u, G.nzp));
@@ -605,24 +657,72 @@ int dfp_project(dfp_plan* p, double* U, double* V, double* W, double* rhs_p, dou
     if (!U || !V || !W) return fail(p, DFP_E_ARG, "U, V and W are required");
     if (save_p && (!P || !(dt * c != 0.0) || !std::isfinite(dt * c)))
         return fail(p, DFP_E_ARG, "DFP_SAVE_P needs P and a finite nonzero dt*c");
+    const bool bf = backflow(p);
+    if (bf && (!(dt * c != 0.0) || !std::isfinite(dt * c)))
+        return fail(p, DFP_E_ARG, "the outlet backflow term needs a finite nonzero dt*c");
     if (flags & DFP_FIELDS_ON_DEVICE) {
         if (!rhs_p) return fail(p, DFP_E_ARG, "rhs_p is required with DFP_FIELDS_ON_DEVICE");
-        rc = spectral_solve(p, U, V, W, rhs_p, P, dt * c, save_p, 1);
+        // compute_projection_step (projection.f90:21-34): divergence, backflow
+        // face values, solve, correction
+        if (bf) HIPCHK(p, launch_outflow_phib(p, U, V, W, p->pb, p->phib, nullptr, dt * c));
+        rc = spectral_solve(p, U, V, W, rhs_p, P, dt * c, save_p, 1, bf ? p->phib : nullptr);
         if (rc) return rc;
-        HIPCHK(p, launch_correct(p, U, V, W, rhs_p));
+        HIPCHK(p, launch_correct(p, U, V, W, rhs_p, bf ? p->phib : nullptr));
         return finish(p, flags);
     }
     if ((rc = ensure_staging(p))) return rc;
+    if (bf && (rc = ensure_outflow_staging(p))) return rc;
     if ((rc = h2d(p, p->d_U, U, p->sz_u)) || (rc = h2d(p, p->d_V, V, p->sz_v)) || (rc = h2d(p, p->d_W, W, p->sz_w)))
         return rc;
     if (rhs_p && (rc = h2d(p, p->d_rhs, rhs_p, p->sz_rhs))) return rc;
-    rc = spectral_solve(p, p->d_U, p->d_V, p->d_W, p->d_rhs, p->d_P, dt * c, save_p, 1);
+    if (bf) {
+        if ((rc = h2d(p, p->d_pb, p->pb, face_size(p)))) return rc;
+        HIPCHK(p, launch_outflow_phib(p, p->d_U, p->d_V, p->d_W, p->d_pb, p->d_phib, nullptr, dt * c));
+    }
+    rc = spectral_solve(p, p->d_U, p->d_V, p->d_W, p->d_rhs, p->d_P, dt * c, save_p, 1, bf ? p->d_phib : nullptr);
     if (rc) return rc;
-    HIPCHK(p, launch_correct(p, p->d_U, p->d_V, p->d_W, p->d_rhs));
+    HIPCHK(p, launch_correct(p, p->d_U, p->d_V, p->d_W, p->d_rhs, bf ? p->d_phib : nullptr));
     if ((rc = d2h(p, U, p->d_U, p->sz_u)) || (rc = d2h(p, V, p->d_V, p->sz_v)) || (rc = d2h(p, W, p->d_W, p->sz_w)))
         return rc;
     if (rhs_p && (rc = d2h(p, rhs_p, p->d_rhs, p->sz_rhs))) return rc;
     if (save_p && (rc = d2h(p, P, p->d_P, p->sz_p))) return rc;
+    if (bf && (rc = d2h(p, p->phib, p->d_phib, face_size(p)))) return rc;
+    return finish(p, flags);
+}
+
+int dfp_set_outflow(dfp_plan* p, double beta, double outflow_uc, const double* Pb, double* phib) {
+    if (!p) return DFP_E_ARG;
+    if (!(beta >= 0.0) || !std::isfinite(beta) || !std::isfinite(outflow_uc))
+        return fail(p, DFP_E_ARG, "outflow_backflow_beta must be finite and >= 0, outflow_Uc finite");
+    if (beta > 0.0 && p->geo.xbc != 1)
+        return fail(p, DFP_E_ARG, "outflow_backflow_beta > 0 needs x_bc_type = 1");
+    if (beta > 0.0 && (!Pb || !phib)) return fail(p, DFP_E_ARG, "outflow_backflow_beta > 0 needs Pb and phib");
+    p->beta = beta;
+    p->outflow_uc = outflow_uc;
+    p->pb = Pb;
+    p->phib = phib;
+    return DFP_OK;
+}
+
+int dfp_outflow_backflow(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs_p, double dt,
+                         double c, int flags) {
+    if (!p) return DFP_E_ARG;
+    int rc = check_flags(p, flags);
+    if (rc) return rc;
+    if (!U || !V || !W || !rhs_p) return fail(p, DFP_E_ARG, "U, V, W and rhs_p are required");
+    if (!backflow(p)) return fail(p, DFP_E_ARG, "no outlet backflow term is set (dfp_set_outflow, beta > 0)");
+    if (!(dt * c != 0.0) || !std::isfinite(dt * c))
+        return fail(p, DFP_E_ARG, "the outlet backflow term needs a finite nonzero dt*c");
+    if (flags & DFP_FIELDS_ON_DEVICE) {
+        HIPCHK(p, launch_outflow_phib(p, U, V, W, p->pb, p->phib, rhs_p, dt * c));
+        return finish(p, flags);
+    }
+    if ((rc = ensure_staging(p)) || (rc = ensure_outflow_staging(p))) return rc;
+    if ((rc = h2d(p, p->d_U, U, p->sz_u)) || (rc = h2d(p, p->d_V, V, p->sz_v)) || (rc = h2d(p, p->d_W, W, p->sz_w)) ||
+        (rc = h2d(p, p->d_rhs, rhs_p, p->sz_rhs)) || (rc = h2d(p, p->d_pb, p->pb, face_size(p))))
+        return rc;
+    HIPCHK(p, launch_outflow_phib(p, p->d_U, p->d_V, p->d_W, p->d_pb, p->d_phib, p->d_rhs, dt * c));
+    if ((rc = d2h(p, rhs_p, p->d_rhs, p->sz_rhs)) || (rc = d2h(p, p->phib, p->d_phib, face_size(p)))) return rc;
     return finish(p, flags);
 }
 
@@ -671,15 +771,18 @@ int dfp_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs_
     int rc = check_flags(p, flags);
     if (rc) return rc;
     if (!U || !V || !W || !rhs_p) return fail(p, DFP_E_ARG, "U, V, W and rhs_p are required");
+    const bool bf = backflow(p);
     if (flags & DFP_FIELDS_ON_DEVICE) {
-        HIPCHK(p, launch_correct(p, U, V, W, rhs_p));
+        HIPCHK(p, launch_correct(p, U, V, W, rhs_p, bf ? p->phib : nullptr));
         return finish(p, flags);
     }
     if ((rc = ensure_staging(p))) return rc;
+    if (bf && (rc = ensure_outflow_staging(p))) return rc;
     if ((rc = h2d(p, p->d_U, U, p->sz_u)) || (rc = h2d(p, p->d_V, V, p->sz_v)) || (rc = h2d(p, p->d_W, W, p->sz_w)) ||
         (rc = h2d(p, p->d_rhs, rhs_p, p->sz_rhs)))
         return rc;
-    HIPCHK(p, launch_correct(p, p->d_U, p->d_V, p->d_W, p->d_rhs));
+    if (bf && (rc = h2d(p, p->d_phib, p->phib, face_size(p)))) return rc;
+    HIPCHK(p, launch_correct(p, p->d_U, p->d_V, p->d_W, p->d_rhs, bf ? p->d_phib : nullptr));
     if ((rc = d2h(p, U, p->d_U, p->sz_u)) || (rc = d2h(p, V, p->d_V, p->sz_v)) || (rc = d2h(p, W, p->d_W, p->sz_w)))
         return rc;
     return finish(p, flags);

@@ -53,7 +53,8 @@ struct FftDesc {
 //   inv_dyg[j-2] = 1/(yg(j+1)-yg(j)),  j = 2..ny-1    (nym-1 entries)
 //   inv_dzg[k-2] = 1/(zg(k+1)-zg(k)),  k = 2..nz-1    (nzp entries)
 struct Geo {
-    int nxp, nym, nzp, nxh;
+    int nxp, nym, nzp, nxh;   // nxh: x planes of the spectrum (nxp/2+1 periodic, nxp inflow/outflow)
+    int xbc;                  // x_bc_type: 0 periodic, 1 inflow/outflow (DCT-IV in x)
     int nx, ny, nz, nxg, nyg, nzg;
     double inv_dx;
     const double* inv_dyf;
@@ -72,7 +73,7 @@ struct Ops {
     const double* dl;   // nym-1
     const double* dd;   // nym
     const double* du;   // nym-1
-    const double* kxx;  // nxh (kx = 0..nxp/2)
+    const double* kxx;  // nxh: kx = 0..nxp/2 (periodic) or the nxp DCT-IV modes
     const double* kzz;  // nzp
 };
 
@@ -89,6 +90,8 @@ struct XArgs {
     double* rowpart;        // per-(row, block) sums of the field, or null
     const double* part0;    // mode 2: per-block sums of phi's first row
     double idx2, idz2;      // mode 2: 1/dx^2, 1/dz^2
+    const double* phib;     // mode 0, inflow/outflow x: outlet face values phib(2:nyg-1,2:nzg-1)
+                            // (the last cell's equation takes -2 phib/dx^2), or null
 };
 
 // Per-case parameters of the x-direction kernels (k_spectral.hip).  C is
@@ -135,6 +138,7 @@ struct CPar {
     double* V;
     double* W;
     const double* rhs;
+    const double* phib;  // inflow/outflow x with the backflow term: outlet face values, else null
     Geo g;
     FastDiv div_ni;
     int nbx;             // blocks over the (x,y) plane
@@ -197,6 +201,12 @@ struct dfp_plan {
     double* d_rowpart = nullptr; // nym x (x-kernel blocks per row)
     double* d_part0 = nullptr;   // x-kernel blocks per row
     double* d_pbar = nullptr;    // nym
+    // x inflow/outflow: outlet backflow state (dfp_set_outflow) and the staging
+    // of its face arrays for host-resident calls, (nyg-2) x (nzg-2) each
+    double beta = 0.0, outflow_uc = 0.0;
+    const double* pb = nullptr;  // caller's Pb(2:nyg-1,2:nzg-1)
+    double* phib = nullptr;      // caller's phib(2:nyg-1,2:nzg-1)
+    double *d_pb = nullptr, *d_phib = nullptr;
     // staging for host-resident fields
     double *d_U = nullptr, *d_V = nullptr, *d_W = nullptr, *d_rhs = nullptr, *d_P = nullptr;
     int64_t sz_u = 0, sz_v = 0, sz_w = 0, sz_p = 0, sz_rhs = 0;
@@ -222,9 +232,10 @@ struct dfp_plan {
 namespace dfp {
 
 // launchers (defined next to their kernels)
+// inflow/outflow x: DCT-IV kernels; phib (mode 0 only) adds the outlet backflow term
 hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W,
-                             const double* rhs, int from_rhs);
-hipError_t launch_lu_table(dfp_plan* p);
+                             const double* rhs, int from_rhs, const double* phib = nullptr);
+hipError_t launch_lu_table(dfp_plan* p);  // pins the (0,0) mode of the periodic x only
 // dir -1: z FFT + forward elimination, +1: back substitution + inverse z FFT
 hipError_t launch_zsolve(dfp_plan* p, int dir);
 int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit);
@@ -248,7 +259,14 @@ hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const doub
 hipError_t launch_mode00(dfp_plan* p, hipStream_t s);  // rowpart -> pbar of the plan's FP32 buffers
 int correct_blocks_z(const Geo& g);
 hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs);
-hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs);
+// phib: outlet face values of the inflow/outflow x with the backflow term, or null
+hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs,
+                          const double* phib = nullptr);
+// set_outflow_backflow_pressure (projection.f90:65-96): phib from u* at the
+// outlet and Pb, with dtc = dt*rk_coef(rk_step,rk_step); rhs, if given, takes
+// -2 phib/dx^2 in its last x cell (the fused x kernel applies it itself)
+hipError_t launch_outflow_phib(dfp_plan* p, const double* U, const double* V, const double* W, const double* Pb,
+                               double* phib, double* rhs, double dtc);
 hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);
 hipError_t launch_synth(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma);
 

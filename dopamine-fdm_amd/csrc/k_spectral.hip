@@ -1,5 +1,7 @@
 // Host side of the x-direction spectral kernels (k_xfft_impl.h): parameter
 // records, LDS sizes, and launches dispatched on the plan's z-split radix.
+#include <cmath>
+
 #include "fft_lds.h"
 
 namespace dfp {
@@ -15,6 +17,10 @@ hipError_t launch_xfwd_batch_r(hipStream_t s, int mode, const XPar<C>* cs, const
 template <int R, typename C>
 hipError_t launch_xinv_batch_r(hipStream_t s, int accum, const XPar<C>* cs, const int* off, const int* idx,
                                const int* nc, int grid, size_t lds);
+template <int R>
+hipError_t launch_xdct_fwd_r(hipStream_t s, int mode, const XPar<double2>& q, dim3 grid, size_t lds);
+template <int R>
+hipError_t launch_xdct_inv_r(hipStream_t s, const XPar<double2>& q, dim3 grid, size_t lds);
 
 // ---------------------------------------------------------------------------
 // host side: parameters, LDS sizes, launches
@@ -52,8 +58,14 @@ XPar<C> make_xpar(const dfp_plan* p, int inv) {
     q.npart0 = (g.nzp / p->zr) / p->tk_xi;
     q.ntw = twiddle_count(p->fx);
     q.div_nxp = FastDiv(g.nxp);
-    q.div_half = FastDiv(p->fx.n / 2 + 1);
-    q.scale = 1.0 / ((double)g.nxp * (double)g.nzp);
+    if (g.xbc == 1) {
+        // DCT-IV kernels: lines of M = nxp/2 packed points; REDFT11 twice is 2 nxp
+        q.div_half = FastDiv(p->fx.n);
+        q.scale = 1.0 / (2.0 * (double)g.nxp * (double)g.nzp);
+    } else {
+        q.div_half = FastDiv(p->fx.n / 2 + 1);
+        q.scale = 1.0 / ((double)g.nxp * (double)g.nzp);
+    }
     q.dtc = 1.0;
     if (sizeof(C) == sizeof(double2)) {
         q.tw = reinterpret_cast<const C*>(p->tw_x);
@@ -71,6 +83,8 @@ template XPar<float2> make_xpar<float2>(const dfp_plan*, int);
 size_t x_lds_bytes(const dfp_plan* p, int inv, int f32) {
     const int R = p->zr, T = inv ? p->tk_xi : p->tk_x, nz2 = p->geo.nzp / R, M = p->fx.n;
     const size_t cs = f32 ? sizeof(float2) : sizeof(double2);
+    if (p->geo.xbc == 1)  // DCT-IV: M packed points per line, pre- and post-twiddles
+        return cs * ((size_t)R * T * M + twiddle_count(p->fx) + 2 * M + (size_t)nz2 * (R - 1));
     return cs * ((size_t)R * T * (M + inv) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (R - 1));
 }
 
@@ -132,6 +146,18 @@ template hipError_t launch_xinv_batch<double2>(hipStream_t, int, int, const XPar
 template hipError_t launch_xinv_batch<float2>(hipStream_t, int, int, const XPar<float2>*, const int*, const int*,
                                               const int*, int, size_t);
 
+static hipError_t xdct_fwd_any(int R, hipStream_t s, int mode, const XPar<double2>& q, dim3 grid, size_t lds) {
+#define DFP_CALL(RV) launch_xdct_fwd_r<RV>(s, mode, q, grid, lds)
+    DFP_R_SWITCH(R, DFP_CALL)
+#undef DFP_CALL
+}
+
+static hipError_t xdct_inv_any(int R, hipStream_t s, const XPar<double2>& q, dim3 grid, size_t lds) {
+#define DFP_CALL(RV) launch_xdct_inv_r<RV>(s, q, grid, lds)
+    DFP_R_SWITCH(R, DFP_CALL)
+#undef DFP_CALL
+}
+
 template <typename C>
 static hipError_t launch_xfwd(dfp_plan* p, int mode, const XPar<C>& q) {
     const dim3 grid(q.nbx, q.g.nym);
@@ -143,9 +169,19 @@ static hipError_t launch_xfwd(dfp_plan* p, int mode, const XPar<C>& q) {
 }
 
 hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W, const double* rhs,
-                             int from_rhs) {
+                             int from_rhs, const double* phib) {
     XPar<double2> q = make_xpar<double2>(p);
     q.a.U = U; q.a.V = V; q.a.W = W; q.a.rhs = rhs;
+    if (p->geo.xbc == 1) {
+        q.a.phib = from_rhs ? nullptr : phib;
+        q.a.idx2 = 1.0 / std::pow(p->dx, 2.0);
+        const dim3 grid(q.nbx, q.g.nym);
+        const size_t lds = x_lds_bytes(p, 0, 0);
+        const int slot = prof_begin(p, from_rhs ? "dct_x_rhs" : "div_dct_x");
+        const hipError_t e = xdct_fwd_any(p->zr, p->stream, from_rhs ? 1 : 0, q, grid, lds);
+        prof_end(p, slot);
+        return e;
+    }
     return launch_xfwd(p, from_rhs ? 1 : 0, q);
 }
 
@@ -168,6 +204,14 @@ static hipError_t launch_xinv(dfp_plan* p, const XPar<C>& q) {
 hipError_t launch_irfft_x(dfp_plan* p, double* rhs, double* P, double dtc, int save_p) {
     XPar<double2> q = make_xpar<double2>(p, 1);
     q.rhs = rhs; q.P = P; q.dtc = dtc; q.save_p = save_p;
+    if (p->geo.xbc == 1) {
+        const dim3 grid(q.nbx, q.g.nym);
+        const size_t lds = x_lds_bytes(p, 1, 0);
+        const int slot = prof_begin(p, "idct_x");
+        const hipError_t e = xdct_inv_any(p->zr, p->stream, q, grid, lds);
+        prof_end(p, slot);
+        return e;
+    }
     return launch_xinv<double2, 0>(p, q);
 }
 

@@ -2,12 +2,16 @@
 //   k_divergence   rhs_p = div(U,V,W)            projection.f90:37-63
 //   k_correct      U,V,W -= grad(rhs_p)          projection.f90:551-604
 //   k_max_div      max |div(U,V,W)|              projection.f90:607-639
+//   k_outflow_phib outlet backflow face values   projection.f90:65-96
 //   k_synth        seeded synthetic channel fields (benchmark/test inputs)
 // Threads run over the (x,y) plane of one z index, x fastest (the contiguous
 // index of every field), blockIdx.y is the z index; consecutive lanes read
 // consecutive addresses of one row.  The
 // stencil arithmetic is kept operation-for-operation as the reference writes
 // it (no FMA contraction), so these kernels reproduce the CPU path bit for bit.
+#include <algorithm>
+#include <cmath>
+
 #include "dfp_internal.h"
 
 namespace dfp {
@@ -46,6 +50,8 @@ hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, cons
 }
 
 // U(i,j,k) -= (phi(i+1)-phi(i))/dx          i=2..nx-1,  j=2..nyg-1, k=2..nzg-1
+//   inflow/outflow x, outlet face i = nx: U += 2 (phi(nx) - phib)/dx (phib = 0
+//   without the backflow term)
 // V(i,j,k) -= (phi(j+1)-phi(j))/dyg(j)      i=2..nxg-1, j=2..ny-1,  k=2..nzg-1
 // W(i,j,k) -= (phi(k+1)-phi(k))/dzg(k)      i=2..nxg-1, j=2..nyg-1, k=2..nz-1
 // Each thread walks kCorrectK consecutive z planes of one (i,j) column, so
@@ -74,6 +80,9 @@ __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
     const long pr_k = (long)(g.nxg - 1) * g.nym;
     const long pu_k = (long)g.nx * g.nyg, pv_k = (long)g.nxg * g.ny, pw_k = (long)g.nxg * g.nyg;
     const bool do_u = i <= g.nx - 1, do_v = j <= g.ny - 1;
+    // inflow/outflow x: the outlet face U(nx) takes the face value of phi,
+    // zero or phib (projection.f90:578-589)
+    const bool face_out = g.xbc == 1 && i == g.nx;
     const double idyg = do_v ? inv_dyg[j - 2] : 0.0;
     const long r = ridx(g, i, j, k0);
     const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k0 - 1) * pu_k;
@@ -103,7 +112,16 @@ __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
         if (q >= nk) break;
         const int k = k0 + q;
         // phi(k+1) is used for W only, for k <= nz-1
-        if (do_u) U[iu + q * pu_k] = uo[q] - (px[q] - pc[q]) * g.inv_dx;
+        if (do_u) {
+            U[iu + q * pu_k] = uo[q] - (px[q] - pc[q]) * g.inv_dx;
+        } else if (face_out) {
+            if (c.phib) {
+                const double fb = gptr(c.phib)[(j - 2) + (long)(k - 2) * g.nym];
+                U[iu + q * pu_k] = uo[q] + 2.0 * (pc[q] - fb) * g.inv_dx;
+            } else {
+                U[iu + q * pu_k] = uo[q] + 2.0 * pc[q] * g.inv_dx;
+            }
+        }
         if (do_v) V[iv + q * pv_k] = vo[q] - (py[q] - pc[q]) * idyg;
         if (k <= g.nz - 1) W[iw + q * pw_k] = wo[q] - (pc[q + 1] - pc[q]) * inv_dzg[k - 2];
     }
@@ -139,6 +157,7 @@ __global__ void __launch_bounds__(256) kb_correct(const CPar* __restrict__ cs, c
 CPar make_cpar(const dfp_plan* p, double* U, double* V, double* W, const double* rhs) {
     CPar c{};
     c.U = U; c.V = V; c.W = W; c.rhs = rhs;
+    c.phib = nullptr;
     c.g = p->geo;
     c.div_ni = FastDiv(p->geo.nxg - 2);
     c.nbx = ceil_div((long)(p->geo.nxg - 2) * p->geo.nym, 256);
@@ -147,8 +166,9 @@ CPar make_cpar(const dfp_plan* p, double* U, double* V, double* W, const double*
 
 int correct_blocks_z(const Geo& g) { return ceil_div(g.nzg - 2, kCorrectK); }
 
-hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs) {
-    const CPar c = make_cpar(p, U, V, W, rhs);
+hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const double* rhs, const double* phib) {
+    CPar c = make_cpar(p, U, V, W, rhs);
+    c.phib = phib;
     dim3 grid(c.nbx, correct_blocks_z(p->geo));
     const int slot = prof_begin(p, "correct");
     hipLaunchKernelGGL(k_correct, grid, dim3(256), 0, p->stream, c);
@@ -159,6 +179,50 @@ hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const do
 hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* idx, const int* nc,
                                 int grid) {
     hipLaunchKernelGGL(kb_correct, dim3(resident_grid(kb_correct, grid, 256, 0)), dim3(256), 0, s, cs, off, idx, nc);
+    return hipGetLastError();
+}
+
+// Outlet backflow term of the inflow/outflow x, set_outflow_backflow_pressure
+// (projection.f90:73-94) on one rank (which owns the outlet): one thread per
+// outlet face (j,k), j = 2..nyg-1, k = 2..nzg-1,
+//   phib(j,k) = dtc*(-beta/2*(un^2+vt^2+wt^2)*th - Pb(j,k)),
+//   th = (1 - tanh(un/uref))/2, un = U(nx,j,k), vt, wt the face averages of
+//   V and W, and, with rhs given, rhs_p(nx,j,k) -= 2*phib(j,k)/dx^2.
+// Pb and phib are (nyg-2) x (nzg-2) arrays, (j,k) at (j-2) + (k-2)*(nyg-2).
+__global__ void __launch_bounds__(256) k_outflow_phib(const double* __restrict__ U, const double* __restrict__ V,
+                                                      const double* __restrict__ W, const double* __restrict__ Pb,
+                                                      double* __restrict__ phib, double* __restrict__ rhs, Geo g,
+                                                      double beta, double uref, double dtc, double inv_dx2) {
+#pragma clang fp contract(off)
+    const int t = blockIdx.x * 256 + threadIdx.x;
+    if (t >= g.nym * (g.nzp + 1)) return;
+    const int k = 2 + t / g.nym;
+    const int j = 2 + t - (k - 2) * g.nym;
+    const int nx = g.nx;
+    const double un = U[(nx - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg];
+    const long iv = (nx - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.ny;
+    const long iw = (nx - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.nyg;
+    const double vt = 0.5 * (V[iv] + V[iv - g.nxg]);
+    const double wt = 0.5 * (W[iw] + W[iw - (long)g.nxg * g.nyg]);
+    const double th = 0.5 * (1.0 - tanh(un / uref));
+    const double ph = dtc * (-0.5 * beta * (un * un + vt * vt + wt * wt) * th - Pb[t]);
+    phib[t] = ph;
+    if (rhs) {
+        const long r = ridx(g, nx, j, k);
+        rhs[r] = rhs[r] - 2.0 * ph * inv_dx2;
+    }
+}
+
+hipError_t launch_outflow_phib(dfp_plan* p, const double* U, const double* V, const double* W, const double* Pb,
+                               double* phib, double* rhs, double dtc) {
+    const Geo& g = p->geo;
+    const double uref = 0.05 * std::max(p->outflow_uc, 1e-2);
+    const double inv_dx2 = 1.0 / std::pow(p->dx, 2.0);
+    const int n = g.nym * (g.nzp + 1);
+    const int slot = prof_begin(p, "outflow_phib");
+    hipLaunchKernelGGL(k_outflow_phib, dim3(ceil_div(n, 256)), dim3(256), 0, p->stream, U, V, W, Pb, phib, rhs, g,
+                       p->beta, uref, dtc, inv_dx2);
+    prof_end(p, slot);
     return hipGetLastError();
 }
 
@@ -250,7 +314,8 @@ __global__ void __launch_bounds__(256) k_synth(double* __restrict__ F, int f, Ge
     const int j = 1 + blockIdx.y;
     const int k = 1 + blockIdx.z;
     if (i > n1) return;
-    const int is = (f == 0) ? per_face(i, n1) : per_cell(i, n1);
+    // the inflow/outflow x has no periodic ghosts: every x index is its own source
+    const int is = g.xbc == 1 ? i : ((f == 0) ? per_face(i, n1) : per_cell(i, n1));
     const int ks = (f == 2) ? per_face(k, n3) : per_cell(k, n3);
     double sgn = 1.0;
     int js = j;

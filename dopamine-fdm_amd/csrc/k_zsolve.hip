@@ -30,7 +30,10 @@
 
 namespace dfp {
 
-__global__ void __launch_bounds__(256) k_lu_table(double* __restrict__ lu, Ops o, int nxh, int nym, int nzp) {
+// pin: the periodic x has the singular (0,0) mode, pinned by D(2) *= 3/2
+// (projection.f90:292); the DCT-IV modes of the inflow/outflow x are all
+// nonzero and nothing is pinned
+__global__ void __launch_bounds__(256) k_lu_table(double* __restrict__ lu, Ops o, int nxh, int nym, int nzp, int pin) {
     const int nzs = nzp / 2 + 1;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nxh * nzs) return;
@@ -38,7 +41,7 @@ __global__ void __launch_bounds__(256) k_lu_table(double* __restrict__ lu, Ops o
     const int kzs = (int)(t - (long)kx * nzs);
     const double lam = o.kxx[kx] + o.kzz[kzs];
     double dp = o.dd[0] + lam;
-    if (kx == 0 && kzs == 0) dp = 3.0 / 2.0 * dp;
+    if (pin && kx == 0 && kzs == 0) dp = 3.0 / 2.0 * dp;
     double* col = lu + (long)kx * nym * nzs + kzs;
     col[0] = 1.0 / dp;
     for (int j = 1; j < nym; ++j) {
@@ -52,7 +55,7 @@ hipError_t launch_lu_table(dfp_plan* p) {
     const Geo& g = p->geo;
     const long n = (long)g.nxh * (g.nzp / 2 + 1);
     hipLaunchKernelGGL(k_lu_table, dim3(ceil_div(n, 256)), dim3(256), 0, p->stream, p->d_lu, p->ops, g.nxh, g.nym,
-                       g.nzp);
+                       g.nzp, g.xbc == 0 ? 1 : 0);
     return hipGetLastError();
 }
 

@@ -25,6 +25,7 @@ class ChannelGrid:
     Lz_i: float
     alpha: float
     grid_type: int = 2
+    x_bc_type: int = 0   # 0 periodic x, 1 inflow/outflow x (global.f90:185-186)
     x: np.ndarray = field(init=False)
     y: np.ndarray = field(init=False)
     z: np.ndarray = field(init=False)
@@ -37,6 +38,8 @@ class ChannelGrid:
     def __post_init__(self):
         if self.nxp < 2 or self.nxp % 2 or self.nym < 1 or self.nzp < 1:
             raise ValueError("need nxp even >= 2, nym >= 1, nzp >= 1")
+        if self.x_bc_type not in (0, 1):
+            raise ValueError("x_bc_type must be 0 (periodic) or 1 (inflow/outflow)")
         nx, ny, nz = self.nx, self.ny, self.nz
         self.x = np.array([self.Lx_i * float(i) / float(nx - 1) for i in range(nx)])
         self.z = np.array([self.Lz_i * float(i) / float(nz - 1) for i in range(nz)])
@@ -54,10 +57,12 @@ class ChannelGrid:
         self.dx = float(np.min(np.diff(self.xg)))
         self.dz = float(np.min(np.diff(self.zg)))
 
-    nx = property(lambda s: s.nxp + 2)
+    # the periodic x transforms nxp = nxm - 1 cells (the last duplicates the
+    # first), the inflow/outflow x all nxm = nxp (initialization.f90:331-338)
+    nx = property(lambda s: s.nxp + (2 if s.x_bc_type == 0 else 1))
     ny = property(lambda s: s.nym + 1)
     nz = property(lambda s: s.nzp + 2)
-    nxg = property(lambda s: s.nxp + 3)
+    nxg = property(lambda s: s.nx + 1)
     nyg = property(lambda s: s.nym + 2)
     nzg = property(lambda s: s.nzp + 3)
     shape_u = property(lambda s: (s.nx, s.nyg, s.nzg))
@@ -72,7 +77,10 @@ class ChannelGrid:
 
     @property
     def n_modes(self):
-        return (self.nxp // 2 + 1) * self.nym * self.nzp
+        """complex modes of the spectrum: the R2C half of the x modes (periodic)
+        or all nxp DCT-IV modes (inflow/outflow)"""
+        nk = self.nxp // 2 + 1 if self.x_bc_type == 0 else self.nxp
+        return nk * self.nym * self.nzp
 
     @property
     def min_spacing(self):
@@ -88,12 +96,15 @@ def _centres(f: np.ndarray) -> np.ndarray:
     return g
 
 
-def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * math.pi, Lz=math.pi, Ly=2.0, grid_type=2) -> ChannelGrid:
+def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * math.pi, Lz=math.pi, Ly=2.0, grid_type=2,
+                 x_bc_type=0) -> ChannelGrid:
     """Channel whose periodic lengths are Lx and Lz (the reference's period is
-    namelist Lx minus one cell, src/initialization.f90:332)."""
-    return ChannelGrid(nxp, nym, nzp, Lx * (nxp + 1) / nxp, Ly, Lz * (nzp + 1) / nzp, alpha, grid_type)
+    namelist Lx minus one cell, src/initialization.f90:332).  With the
+    inflow/outflow x (x_bc_type = 1) the nxp cells span Lx (Lxp = Lx, :333-334)."""
+    Lx_i = Lx * (nxp + 1) / nxp if x_bc_type == 0 else Lx
+    return ChannelGrid(nxp, nym, nzp, Lx_i, Ly, Lz * (nzp + 1) / nzp, alpha, grid_type, x_bc_type)
 
 
-def namelist_grid(nx, ny, nz, Lx, Ly, Lz, alpha, grid_type=2) -> ChannelGrid:
+def namelist_grid(nx, ny, nz, Lx, Ly, Lz, alpha, grid_type=2, x_bc_type=0) -> ChannelGrid:
     """Grid from the reference's namelist values (face counts, lengths)."""
-    return ChannelGrid(nx - 2, ny - 1, nz - 2, Lx, Ly, Lz, alpha, grid_type)
+    return ChannelGrid(nx - (2 if x_bc_type == 0 else 1), ny - 1, nz - 2, Lx, Ly, Lz, alpha, grid_type, x_bc_type)

@@ -77,7 +77,8 @@ class Plan:
         self._arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in (g.x, g.y, g.z, g.yg, g.zg)]
         dp = ctypes.POINTER(ctypes.c_double)
         x, y, z, yg, zg = (a.ctypes.data_as(dp) for a in self._arrays)
-        self._grid = DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, x, y, z, yg, zg, 0, 1, 0)
+        self._grid = DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, x, y, z, yg, zg, int(getattr(g, "x_bc_type", 0)), 1, 0)
+        self._outflow = None  # keeps the arrays registered with dfp_set_outflow alive
         h = ctypes.c_void_p()
         if tuning is None:
             check(lib().dfp_plan_create(ctypes.byref(self._grid), precision, device, ctypes.byref(h)), "dfp_plan_create")
@@ -143,6 +144,30 @@ class Plan:
         pp, dpp = _ptr(P, g.shape_p, "P")
         f = self._flags(du, dv, dw, dr, dpp, save_p=save_p, asynchronous=asynchronous)
         check(lib().dfp_project(self._h, pu, pv, pw, pr, pp, float(dt), float(c), f), "dfp_project", self._h)
+
+    def set_outflow(self, beta: float, outflow_Uc: float, Pb=None, phib=None):
+        """Outlet backflow state of the inflow/outflow x (global.f90:188-197):
+        Pb and phib are (nyg-2, nzg-2) arrays, host or device alike the fields of
+        the calls that follow; the plan reads Pb and writes phib at each call."""
+        g = self.g
+        shape = (g.nyg - 2, g.nzg - 2)
+        pb, dpb = _ptr(Pb, shape, "Pb")
+        pf, dpf = _ptr(phib, shape, "phib")
+        if Pb is not None and phib is not None and dpb != dpf:
+            raise ValueError("Pb and phib must both be host or both be device arrays")
+        check(lib().dfp_set_outflow(self._h, float(beta), float(outflow_Uc), pb, pf), "dfp_set_outflow", self._h)
+        self._outflow = (Pb, phib)
+
+    def outflow_backflow(self, U, V, W, rhs_p, dt, c):
+        """set_outflow_backflow_pressure (projection.f90:65-96) on its own."""
+        g = self.g
+        pu, du = _ptr(U, g.shape_u, "U")
+        pv, dv = _ptr(V, g.shape_v, "V")
+        pw, dw = _ptr(W, g.shape_w, "W")
+        pr, dr = _ptr(rhs_p, g.shape_rhs, "rhs_p")
+        f = self._flags(du, dv, dw, dr)
+        check(lib().dfp_outflow_backflow(self._h, pu, pv, pw, pr, float(dt), float(c), f), "dfp_outflow_backflow",
+              self._h)
 
     def divergence(self, U, V, W, rhs_p):
         g = self.g
@@ -225,6 +250,11 @@ class ChannelState:
     rhs_p: np.ndarray = field(init=False)
     dt: float = 1e-3
     rk_step: int = 1
+    # outlet backflow stabilisation of the inflow/outflow x (global.f90:188-197)
+    outflow_backflow_beta: float = 0.0
+    outflow_Uc: float = 0.0
+    Pb: np.ndarray = field(init=False)
+    phib: np.ndarray = field(init=False)
 
     def __post_init__(self):
         g = self.g
@@ -233,6 +263,8 @@ class ChannelState:
         self.W = np.zeros(g.shape_w, order="F")
         self.P = np.zeros(g.shape_p, order="F")
         self.rhs_p = np.zeros(g.shape_rhs, order="F")
+        self.Pb = np.zeros((g.nyg - 2, g.nzg - 2), order="F")
+        self.phib = np.zeros((g.nyg - 2, g.nzg - 2), order="F")
 
 
 class Projection:
@@ -246,11 +278,27 @@ class Projection:
         # projection.f90:109-110, :362
         return self.st.rk_step == 3 and not bool(skip_p_save)
 
+    def _sync_outflow(self):
+        st = self.st
+        if st.outflow_backflow_beta > 0.0:
+            self.plan.set_outflow(st.outflow_backflow_beta, st.outflow_Uc, st.Pb, st.phib)
+        else:
+            self.plan.set_outflow(0.0, st.outflow_Uc)
+
     def compute_projection_step(self):
         """projection.f90:21-34 (the fused device path)."""
         st = self.st
-        # P = phi/(dt*rk_coef(3,3)) at the last stage only (projection.f90:362-377)
-        self.plan.project(st.U, st.V, st.W, st.rhs_p, st.P, st.dt, RK_COEF_DIAG[2], save_p=self._save_p(False))
+        self._sync_outflow()
+        # c = rk_coef(rk_step,rk_step): the backflow term's factor (projection.f90:82)
+        # and, at the last stage, P = phi/(dt*rk_coef(3,3)) (projection.f90:362-377)
+        self.plan.project(st.U, st.V, st.W, st.rhs_p, st.P, st.dt, RK_COEF_DIAG[st.rk_step - 1],
+                          save_p=self._save_p(False))
+
+    def set_outflow_backflow_pressure(self):
+        """projection.f90:65-96"""
+        st = self.st
+        self._sync_outflow()
+        self.plan.outflow_backflow(st.U, st.V, st.W, st.rhs_p, st.dt, RK_COEF_DIAG[st.rk_step - 1])
 
     def compute_pseudo_pressure_rhs(self):
         st = self.st
@@ -262,6 +310,7 @@ class Projection:
 
     def project_velocity(self):
         st = self.st
+        self._sync_outflow()
         self.plan.correct(st.U, st.V, st.W, st.rhs_p)
 
     def check_divergence(self) -> float:

@@ -3,7 +3,8 @@
 
 U = 1 - ((yg - yc)/h)^2 + sigma*N(0,1), V = W = sigma*N(0,1), V = 0 on the two
 walls, periodic ghosts in x and z as boundary_conditions.f90:277-289 and
-:591-602 leave them, no-slip ghosts in y (U(:,1,:) = -U(:,2,:)).  The normal
+:591-602 leave them (the inflow/outflow x, x_bc_type = 1, has none: every x
+index draws its own deviate), no-slip ghosts in y (U(:,1,:) = -U(:,2,:)).  The normal
 deviate of element e of field f is Box-Muller on two SplitMix64 hashes of
 (key_f, e), key_f = mix64(seed + (f+1)*G), G = 0x9E3779B97F4A7C15, where e
 is the Fortran linear index of the element's independent source cell.
@@ -58,7 +59,10 @@ def synth_fields(g, seed: int, sigma: float):
         i = np.arange(1, n1 + 1)[:, None, None]
         j = np.arange(1, n2 + 1)[None, :, None]
         k = np.arange(1, n3 + 1)[None, None, :]
-        i_s = _per_face(i, n1) if f == 0 else _per_cell(i, n1)
+        if getattr(g, "x_bc_type", 0) == 1:
+            i_s = i  # the inflow/outflow x has no periodic ghosts
+        else:
+            i_s = _per_face(i, n1) if f == 0 else _per_cell(i, n1)
         k_s = _per_face(k, n3) if f == 2 else _per_cell(k, n3)
         if f == 1:
             j_s = j
@@ -86,9 +90,10 @@ def refresh_velocity_ghosts(g, U, V, W):
     apply_boundary_conditions leaves them after the projection; y ghosts are
     not touched."""
     nx, nxg, nz, nzg = g.nx, g.nxg, g.nz, g.nzg
-    U[0] = U[nx - 2]; U[nx - 1] = U[1]
-    for F in (V, W):
-        F[0] = F[nxg - 3]; F[nxg - 2] = F[1]; F[nxg - 1] = F[2]
+    if getattr(g, "x_bc_type", 0) == 0:  # the inflow/outflow x keeps its faces and ghosts
+        U[0] = U[nx - 2]; U[nx - 1] = U[1]
+        for F in (V, W):
+            F[0] = F[nxg - 3]; F[nxg - 2] = F[1]; F[nxg - 1] = F[2]
     for F in (U, V):
         F[:, :, 0] = F[:, :, nzg - 3]; F[:, :, nzg - 2] = F[:, :, 1]; F[:, :, nzg - 1] = F[:, :, 2]
     W[:, :, 0] = W[:, :, nz - 2]; W[:, :, nz - 1] = W[:, :, 1]

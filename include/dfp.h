@@ -3,15 +3,19 @@
  *
  * Replaces the Poisson/projection path of the reference's Fortran module
  * `projection` (src/projection.f90) and its GPU solver (src/poisson_gpu.f90)
- * for the channel case: x periodic, y walls, z periodic
- * (x_bc_type = 0, y_bc_type = 1, z_bc_type = 0), one rank.
+ * for y walls and z periodic with either x boundary type, one rank:
+ *   the channel, x periodic (x_bc_type = 0, y_bc_type = 1, z_bc_type = 0);
+ *   the inflow/outflow x (x_bc_type = 1: DCT-IV in x, no mean-pressure pin,
+ *   optional outlet backflow term, projection.f90:65-96, :126-129, :333-339).
  *
  * All arrays are the reference's Fortran arrays, passed by address
  * (column-major, x fastest), with these extents (initialization.f90:204-220):
  *     U(nx,nyg,nzg)   V(nxg,ny,nzg)   W(nxg,nyg,nz)   P(nxg,nyg,nzg)
  *     rhs_p(2:nxg, 2:nyg-1, 2:nzg)   ->  (nxg-1) x (nyg-2) x (nzg-1)
  * where nxp, nym, nzp are the Fourier/cell counts and
- *     nx = nxp+2, ny = nym+1, nz = nzp+2, nxg = nxp+3, nyg = nym+2, nzg = nzp+3.
+ *     nx = nxp+2, ny = nym+1, nz = nzp+2, nxg = nxp+3, nyg = nym+2, nzg = nzp+3
+ * (x periodic: the last cell duplicates the first), or nx = nxp+1, nxg = nxp+2
+ * (x inflow/outflow: every cell is transformed, initialization.f90:331-338).
  *
  * Calls are synchronous unless DFP_ASYNC is given.  A plan is not
  * thread-safe: use one plan per host thread.  Every function returns 0 on
@@ -40,7 +44,7 @@ typedef struct dfp_grid {
     const double *z;        /* z faces  z(1:nz)                                     */
     const double *yg;       /* y centres with ghosts yg(1:nyg)                      */
     const double *zg;       /* z centres with ghosts zg(1:nzg)                      */
-    int32_t x_bc_type, y_bc_type, z_bc_type; /* must be 0, 1, 0 in this build       */
+    int32_t x_bc_type, y_bc_type, z_bc_type; /* x 0 or 1; y 1; z 0 in this build   */
 } dfp_grid;
 
 /* Precision of the spectral solve.
@@ -70,15 +74,16 @@ enum {
     DFP_E_NONFINITE = -5    /* NaN or Inf found where a finite value was required     */
 };
 
-/* Build a plan: operators (kxx, kzz, Dyy as in initialization.f90:416-539),
- * FFT twiddles and device scratch for one grid on one device.
+/* Build a plan: operators (kxx, kzz, Dyy as in initialization.f90:416-539;
+ * with x_bc_type = 1 the DCT-IV kxx of :443-445), FFT twiddles and device
+ * scratch for one grid on one device.
  * Replaces the reference's set-up in initialization.f90:322-539 and
  * gpu_poisson_init / gpu_gtsv_init (poisson_gpu.f90:60-135, :388-428).
  * DFP_E_UNSUPPORTED for other boundary conditions, odd nxp, nxp/2 or nzp
- * with a prime factor other than 2 and 3, nxp or nzp above 1024, and nym
- * above 65535.  With one cell between the walls (nym = 1) Dyy(2,2) is a+b,
- * the last of the reference's two assignments to it
- * (initialization.f90:523-535). */
+ * with a prime factor other than 2 and 3, nxp or nzp above 1024, nym
+ * above 65535, and DFP_FP32_REFINE with x_bc_type = 1.  With one cell
+ * between the walls (nym = 1) Dyy(2,2) is a+b, the last of the reference's
+ * two assignments to it (initialization.f90:523-535). */
 int dfp_plan_create(const dfp_grid *g, int precision, int device, dfp_plan **out);
 int dfp_plan_destroy(dfp_plan *plan);
 
@@ -115,6 +120,28 @@ int dfp_plan_tuning(const dfp_plan *plan, dfp_tuning *out);
 int dfp_project(dfp_plan *plan, double *U, double *V, double *W, double *rhs_p, double *P,
                 double dt, double c, int flags);
 
+/* Outlet backflow stabilisation of the x inflow/outflow case: the module
+ * state set_outflow_backflow_pressure reads and writes (global.f90:188-197).
+ * beta = outflow_backflow_beta (>= 0; 0 turns the term off), outflow_Uc =
+ * the plane-mean outlet speed, Pb(2:nyg-1,2:nzg-1) the outlet face pressure
+ * (read) and phib(2:nyg-1,2:nzg-1) the face value of the solve (written by
+ * dfp_project and dfp_outflow_backflow, read by dfp_project and
+ * dfp_correct), both (nyg-2) x (nzg-2) arrays.  The plan keeps the pointers
+ * and reads them at each call: device pointers for calls with
+ * DFP_FIELDS_ON_DEVICE, host pointers otherwise.  With beta > 0
+ * dfp_project runs compute_projection_step with the term
+ * (projection.f90:21-34, :65-96): c is then rk_coef(rk_step,rk_step), which
+ * also scales P at the last stage.  DFP_E_ARG for beta > 0 on a periodic x
+ * (input_output.f90:312-314) or without Pb and phib.  Call again whenever
+ * outflow_Uc changes (boundary_conditions.f90:208). */
+int dfp_set_outflow(dfp_plan *plan, double beta, double outflow_Uc, const double *Pb, double *phib);
+
+/* set_outflow_backflow_pressure (projection.f90:65-96) on its own: phib from
+ * U, V, W at the outlet and Pb, and rhs_p(nx,:,:) -= 2 phib/dx^2, with
+ * c = rk_coef(rk_step,rk_step).  Needs dfp_set_outflow with beta > 0. */
+int dfp_outflow_backflow(dfp_plan *plan, const double *U, const double *V, const double *W, double *rhs_p,
+                         double dt, double c, int flags);
+
 /* rhs_p = div(U,V,W) on i=2..nxg-1, j=2..nyg-1, k=2..nzg-1.
  * Replaces compute_pseudo_pressure_rhs (projection.f90:37-63). */
 int dfp_divergence(dfp_plan *plan, const double *U, const double *V, const double *W,
@@ -126,7 +153,9 @@ int dfp_divergence(dfp_plan *plan, const double *U, const double *V, const doubl
  * skip_p_save=.True. corresponds to leaving DFP_SAVE_P out). */
 int dfp_solve(dfp_plan *plan, double *rhs_p, double *P, double dt, double c, int flags);
 
-/* U,V,W -= grad(rhs_p).  Replaces project_velocity (projection.f90:551-604). */
+/* U,V,W -= grad(rhs_p).  Replaces project_velocity (projection.f90:551-604);
+ * with x_bc_type = 1 also the outlet face U(nx) += 2 (rhs_p(nx) - phib)/dx,
+ * phib = 0 without the backflow term (:578-589). */
 int dfp_correct(dfp_plan *plan, double *U, double *V, double *W, const double *rhs_p, int flags);
 
 /* max |div u| over the interior cells, with the face spacings of
@@ -185,6 +214,7 @@ int dfp_batch_use_graphs(dfp_batch *batch, int enable);
  * side by side on up to 1 + aux_streams streams (0..3, default 3); callers
  * that already overlap several batches on their own streams use 0 */
 int dfp_batch_set_parallel(dfp_batch *batch, int aux_streams);
+/* plans with x_bc_type = 1 are refused (DFP_E_UNSUPPORTED) */
 int dfp_batch_project(dfp_batch *batch, int n, dfp_plan *const *plans, double *const *U, double *const *V,
                       double *const *W, double *const *rhs_p, double *const *P, const double *dt,
                       const double *c, const int *flags, int batch_flags);

@@ -48,7 +48,7 @@ def _grid_struct(g, bc=(0, 1, 0)):
     return _lib.DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, *ptrs, *bc), keep
 
 
-@pytest.mark.parametrize("bc", [(1, 1, 0), (0, 0, 0), (0, 1, 1)])
+@pytest.mark.parametrize("bc", [(2, 1, 0), (1, 0, 0), (0, 0, 0), (0, 1, 1), (1, 1, 1)])
 def test_other_boundary_conditions_are_rejected(bc):
     g = dfp.channel_grid(16, 8, 16, 1.5)
     s, keep = _grid_struct(g, bc)
@@ -111,3 +111,25 @@ def test_batch_project_rejects_a_null_batch_without_a_device():
     assert dfp.lib().dfp_batch_project(None, 0, None, None, None, None, None, None, None, None, None, 0) != 0
     assert dfp.lib().dfp_batch_project(None, 0, None, None, None, None, None, None, None, None, None,
                                        dfp.DFP_SOLVE_ONLY) != 0
+
+
+def test_inflow_outflow_precision_and_outflow_arguments():
+    """x inflow/outflow plans are FP64 only, refused before any device call;
+    the outlet backflow state is checked before it is kept."""
+    L = dfp.lib()
+    g = dfp.channel_grid(16, 8, 16, 1.5, x_bc_type=1)
+    s, keep = _grid_struct(g, (1, 1, 0))
+    h = ctypes.c_void_p()
+    assert L.dfp_plan_create(ctypes.byref(s), dfp.DFP_FP32_REFINE, 0, ctypes.byref(h)) == -2
+    assert not h.value
+    assert L.dfp_set_outflow(None, 0.5, 1.0, None, None) == -1
+    assert L.dfp_outflow_backflow(None, None, None, None, None, 1e-3, 0.75, 0) == -1
+
+
+def test_inflow_outflow_host_grid_rules():
+    # initialization.f90:331-338: the inflow/outflow x keeps all nxm = nx-1 cells
+    g = dfp.namelist_grid(17, 9, 10, 4.0, 2.0, 1.0, 1.5, x_bc_type=1)
+    assert (g.nxp, g.nx, g.nxg) == (16, 17, 18)
+    assert g.shape_u == (17, 10, 11) and g.shape_v == (18, 9, 11) and g.shape_rhs == (17, 8, 10)
+    assert g.dx == pytest.approx(4.0 / 16, rel=1e-14)
+    assert g.n_modes == 16 * 8 * 8
