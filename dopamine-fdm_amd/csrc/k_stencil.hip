@@ -11,6 +11,8 @@
 // it (no FMA contraction), so these kernels reproduce the CPU path bit for bit.
 #include <algorithm>
 #include <cmath>
+#include <mutex>
+#include <string>
 
 #include "dfp_internal.h"
 
@@ -226,6 +228,30 @@ hipError_t launch_outflow_phib(dfp_plan* p, const double* U, const double* V, co
     return hipGetLastError();
 }
 
+// |div u| of cell (i,j,k) (reference indices) with the face spacings of
+// check_divergence (projection.f90:631-633)
+__device__ __forceinline__ double abs_div(const double* __restrict__ U, const double* __restrict__ V,
+                                          const double* __restrict__ W, const Geo& g, int i, int j, int k) {
+#pragma clang fp contract(off)
+    const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg;
+    const long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.ny;
+    const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.nyg;
+    const double dxi = g.x ? (g.x[i - 1] - g.x[i - 2]) : 1.0 / g.inv_dx;
+    return fabs((U[iu] - U[iu - 1]) / dxi + (V[iv] - V[iv - g.nxg]) / (g.y[j - 1] - g.y[j - 2]) +
+                (W[iw] - W[iw - (long)g.nxg * g.nyg]) / (g.z[k - 1] - g.z[k - 2]));
+}
+
+// max over the block of one value per thread; thread 0 receives it
+__device__ __forceinline__ double block_max(double v, double* red) {
+    red[threadIdx.x] = v;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    return red[0];
+}
+
 // Block-level max of |div| over one (j,k) row segment; partials[block] holds it.
 __global__ void __launch_bounds__(256) k_max_div(const double* __restrict__ U, const double* __restrict__ V,
                                                  const double* __restrict__ W, double* __restrict__ partials, Geo g) {
@@ -235,24 +261,31 @@ __global__ void __launch_bounds__(256) k_max_div(const double* __restrict__ U, c
     const int j = 2 + t / ni;
     const int i = 2 + t - (j - 2) * ni;
     const int k = 2 + blockIdx.y;
+    const double v = t < ni * g.nym ? abs_div(U, V, W, g, i, j, k) : 0.0;
+    const double m = block_max(v, red);
+    if (threadIdx.x == 0) partials[blockIdx.x + (long)gridDim.x * blockIdx.y] = m;
+}
+
+// The same maximum with a bounded number of blocks: block b covers the
+// cells c = b*256 + tid, + gridDim.x*256, ... of the interior in (i, j, k)
+// order; partials[b] receives its maximum.
+__global__ void __launch_bounds__(256) k_max_div_strided(const double* __restrict__ U, const double* __restrict__ V,
+                                                         const double* __restrict__ W, double* __restrict__ partials,
+                                                         Geo g) {
+    __shared__ double red[256];
+    const long ni = g.nxg - 2, nj = g.nym;
+    const long total = ni * nj * (g.nzg - 2);
     double v = 0.0;
-    if (t < ni * g.nym) {
-        const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k - 1) * g.nx * g.nyg;
-        const long iv = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.ny;
-        const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k - 1) * (long)g.nxg * g.nyg;
-        const double dxi = g.x ? (g.x[i - 1] - g.x[i - 2]) : 1.0 / g.inv_dx;
-        v = fabs((U[iu] - U[iu - 1]) / dxi + (V[iv] - V[iv - g.nxg]) / (g.y[j - 1] - g.y[j - 2]) +
-                 (W[iw] - W[iw - (long)g.nxg * g.nyg]) / (g.z[k - 1] - g.z[k - 2]));
+    for (long c = (long)blockIdx.x * 256 + threadIdx.x; c < total; c += (long)gridDim.x * 256) {
+        const long q = c / ni;  // interior row (j, k), j fastest
+        const long kq = q / nj;
+        const int i = 2 + (int)(c - q * ni);
+        const int j = 2 + (int)(q - kq * nj);
+        const int k = 2 + (int)kq;
+        v = fmax(v, abs_div(U, V, W, g, i, j, k));
     }
-    red[threadIdx.x] = v;
-    __syncthreads();
-    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
-        if (threadIdx.x < s) red[threadIdx.x] = fmax(red[threadIdx.x], red[threadIdx.x + s]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        partials[blockIdx.x + (long)gridDim.x * blockIdx.y] = red[0];
-    }
+    const double m = block_max(v, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = m;
 }
 
 __global__ void __launch_bounds__(256) k_reduce_max(const double* __restrict__ partials, long n,
@@ -280,6 +313,76 @@ hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const d
     prof_end(p, slot);
     return hipGetLastError();
 }
+
+// ---------------------------------------------------------------------------
+// Allocation-free max-divergence (dfp_max_divergence_noalloc).  Its partial
+// maxima and result live in a fixed array of device memory instead of the
+// plan's workspace: kNoAllocParts partials + the result, 8200 bytes, held
+// by every GPU the library runs on until the process ends.  Host code never
+// uses the array's name as an address: at a device's first call, with that
+// device current, hipGetSymbolAddress gives the device's own copy, kept in
+// scratch_dev[device].  Every call on a device shares its copy; calls that
+// overlap on one device race (each call overwrites all partials it reads,
+// so nothing has to start at zero).
+// ---------------------------------------------------------------------------
+constexpr int kNoAllocParts = 1024;
+constexpr int kNoAllocDoubles = kNoAllocParts + 1;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) double scratch[kNoAllocDoubles];
+
+static std::mutex scratch_mu;
+static double* scratch_dev[kMaxDevices] = {};
+
+// address of this device's copy of scratch; the device must be current
+static hipError_t scratch_ptr(int device, double** out) {
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_dev[device]) {
+        void* ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        scratch_dev[device] = static_cast<double*>(ptr);
+    }
+    *out = scratch_dev[device];
+    return hipSuccess;
+}
+
+}  // namespace dfp
+
+extern "C" int dfp_max_divergence_noalloc(dfp_plan* p, const double* U, const double* V, const double* W,
+                                          double* out, int flags) {
+    using namespace dfp;
+    if (!p) return DFP_E_ARG;
+    auto fail = [&](int code, const std::string& msg) {
+        p->last_error = msg;
+        return code;
+    };
+    if (!U || !V || !W || !out) return fail(DFP_E_ARG, "U, V, W and out are required");
+    if (flags != DFP_FIELDS_ON_DEVICE)
+        return fail(DFP_E_ARG, "dfp_max_divergence_noalloc takes device-resident fields (DFP_FIELDS_ON_DEVICE) only");
+    if (p->device < 0 || p->device >= kMaxDevices) return fail(DFP_E_UNSUPPORTED, "device index beyond 63");
+    const Geo& g = p->geo;
+    const long cells = (long)(g.nxg - 2) * g.nym * (g.nzg - 2);
+    // at most kNoAllocParts blocks, each striding over its share of the
+    // cells: any grid a plan takes fits the fixed array
+    const long blocks = std::max<long>(1, std::min<long>(kNoAllocParts, (cells + 255) / 256));
+    if (blocks + 1 > kNoAllocDoubles)
+        return fail(DFP_E_UNSUPPORTED, "max-divergence partials exceed the fixed workspace of 1025 doubles");
+    hipError_t e = hipSetDevice(p->device);
+    double* ws = nullptr;
+    if (e == hipSuccess) e = scratch_ptr(p->device, &ws);
+    if (e != hipSuccess) return fail(DFP_E_HIP, std::string("workspace address: ") + hipGetErrorString(e));
+    const int slot = prof_begin(p, "max_div_noalloc");
+    hipLaunchKernelGGL(k_max_div_strided, dim3((unsigned)blocks), dim3(256), 0, p->stream, U, V, W, ws, g);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, p->stream, ws, blocks, ws + kNoAllocParts);
+    prof_end(p, slot);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, ws + kNoAllocParts, sizeof(double), hipMemcpyDeviceToHost, p->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(p->stream);
+    if (e != hipSuccess) return fail(DFP_E_HIP, std::string("max divergence: ") + hipGetErrorString(e));
+    return DFP_OK;
+}
+
+namespace dfp {
 
 // ---------------------------------------------------------------------------
 // synthetic fields: SplitMix64 counter hash -> Box-Muller normal deviates

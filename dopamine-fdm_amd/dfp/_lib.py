@@ -78,6 +78,7 @@ def lib() -> ctypes.CDLL:
         "dfp_set_outflow": ([vp, d, d, vp, vp], i32),
         "dfp_outflow_backflow": ([vp, vp, vp, vp, vp, d, d, i32], i32),
         "dfp_max_divergence": ([vp, vp, vp, vp, ctypes.POINTER(d), i32], i32),
+        "dfp_max_divergence_noalloc": ([vp, vp, vp, vp, ctypes.POINTER(d), i32], i32),
         "dfp_set_stream": ([vp, vp], i32),
         "dfp_set_refine_sweeps": ([vp, i32], i32),
         "dfp_profile_enable": ([vp, i32], i32),

@@ -163,6 +163,20 @@ int dfp_correct(dfp_plan *plan, double *U, double *V, double *W, const double *r
 int dfp_max_divergence(dfp_plan *plan, const double *U, const double *V, const double *W,
                        double *out, int flags);
 
+/* No counterpart in the reference: dfp_max_divergence for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Device-resident fields only
+ * (flags must be DFP_FIELDS_ON_DEVICE); the same value as
+ * dfp_max_divergence, bit for bit.  Its reduction partials and result live in
+ * a fixed array of library device memory (at most 1024 partials + the
+ * result, 8200 bytes per GPU, held until the process ends) instead of the
+ * plan's workspace, so the call allocates nothing; the result is copied to
+ * the host double *out and the plan's stream is synchronised.  All calls on
+ * one GPU share that array: calls that overlap on one GPU (several host
+ * threads, or several streams) race. */
+int dfp_max_divergence_noalloc(dfp_plan *plan, const double *U, const double *V, const double *W,
+                               double *out, int flags);
+
 /* Refinement sweeps of a DFP_FP32_REFINE plan, 0..8 (no effect on an FP64
  * plan).  No counterpart in the reference. */
 int dfp_set_refine_sweeps(dfp_plan *plan, int sweeps);
