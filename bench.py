@@ -187,6 +187,51 @@ def bench_mixed(g, K, Wm, sweeps, n_sets):
     return 3 * K / (ms / 1e3), {k: round(1e3 * m / max(n, 1), 2) for k, (n, m) in prof.items()}
 
 
+def inflow_bytes(N: int) -> dict:
+    """Per-launch bytes of the inflow/outflow x kernels (DCT-IV in x): the
+    spectrum keeps all nxp DCT modes as complex values, 16 N bytes per pass."""
+    return {"div_dct_x": 24 * N + 16 * N, "zsolve_fwd": 32 * N, "zsolve_bwd": 32 * N,
+            "idct_x": 16 * N + 8 * N, "correct": 56 * N}
+
+
+def bench_inflow(K: int, Wm: int, beta: float = 0.5) -> dict:
+    """The x inflow/outflow branch at 256x192x256 (nx = 257 faces, DCT-IV in
+    x) with the outlet backflow term on: same RK3 loop as the FP64 line, fields,
+    Pb and phib resident in HBM."""
+    import dfp
+    from dfp import hipmem
+
+    g = dfp.channel_grid(NXP, NYM, NZP, ALPHA, x_bc_type=1)
+    N = g.n_cells
+    plan = dfp.Plan(g)
+    rng = np.random.default_rng(9)
+    Pb = dfp.DeviceArray.from_host(np.asfortranarray(0.1 * rng.standard_normal((g.nyg - 2, g.nzg - 2))))
+    phib = dfp.DeviceArray((g.nyg - 2, g.nzg - 2), zero=True)
+    plan.set_outflow(beta, 1.0, Pb, phib)
+    sets = make_sets(plan, g, 12, 0x5EED2000)
+    rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
+    P = dfp.DeviceArray(g.shape_p, zero=True)
+    hipmem.synchronize()
+    ms = time_steps(plan, sets, rhs, P, K, Wm)
+    plan.profile(True)
+    plan.profile_reset()
+    time_steps(plan, sets, rhs, P, max(1, K // 5), 0, first=Wm + K)
+    prof = plan.profile_read()
+    plan.profile(False)
+    plan.close()
+    rate = 3 * K / (ms / 1e3)
+    impl = sum(inflow_bytes(N).values())
+    # the information the spectral stages must carry: real DCT-IV modes whose z
+    # transforms are Hermitian, nxp/2 complex values per (y, z) line
+    H = (g.nxp // 2) * g.nym * g.nzp
+    gbs = impl * rate / 1e9
+    return {"grid": [g.nxp, g.nym, g.nzp], "namelist_nx": g.nx, "outflow_backflow_beta": beta,
+            "projections_per_s": round(rate, 2), "us_per_projection": round(1e6 / rate, 2),
+            "impl_bytes": int(impl), "impl_GBs": round(gbs, 1), "impl_frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+            **form_gbs(b_model(N, H), rate),
+            "kernels_us": {k: round(1e3 * m / max(n, 1), 2) for k, (n, m) in prof.items()}}
+
+
 def form_gbs(bytes_per_projection: int, projections_per_s: float) -> dict:
     gbs = bytes_per_projection * projections_per_s / 1e9
     return {"B_model_bytes": int(bytes_per_projection), "achieved_GBs": round(gbs, 1),
@@ -364,6 +409,7 @@ def main() -> int:
     ap.add_argument("--sweeps", type=int, default=1, help="refinement sweeps of the timed FP32+refine form")
     ap.add_argument("--no-batched", action="store_true", help="skip the 48-case batched measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
+    ap.add_argument("--no-inflow", action="store_true", help="skip the inflow/outflow x measurement")
     ap.add_argument("--save-tuning", default=None, help="write the autotuner table (with new entries) here")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
@@ -480,6 +526,11 @@ def main() -> int:
                                 "kernels_us": k32l,
                                 "rel_err_vs_fp64_by_sweeps": mixed_errors(gl, (1, 2, 3), 4343)}
         out["mixed"] = mixed
+    if not args.no_inflow:
+        if args.no_mixed:
+            plan.close()
+            del sets, rhs, P
+        out["inflow_outflow"] = bench_inflow(max(2, K // 2), Wm)
     if not args.no_batched:
         out["batched"] = bench_batched(save_tuning=args.save_tuning)
     if not args.no_serving:

@@ -60,6 +60,24 @@ Module dfp_capi
        Integer(c_int), Value :: flags
      End Function dfp_correct
 
+     Integer(c_int) Function dfp_set_outflow(plan, beta, outflow_Uc, Pb, phib) Bind(C, name='dfp_set_outflow')
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: plan
+       Real(c_double), Value :: beta, outflow_Uc
+       Real(c_double), Intent(In) :: Pb(*)
+       Real(c_double), Intent(InOut) :: phib(*)
+     End Function dfp_set_outflow
+
+     Integer(c_int) Function dfp_outflow_backflow(plan, U, V, W, rhs_p, dt, c, flags) &
+          Bind(C, name='dfp_outflow_backflow')
+       Import :: c_int, c_ptr, c_double
+       Type(c_ptr), Value :: plan
+       Real(c_double), Intent(In) :: U(*), V(*), W(*)
+       Real(c_double), Intent(InOut) :: rhs_p(*)
+       Real(c_double), Value :: dt, c
+       Integer(c_int), Value :: flags
+     End Function dfp_outflow_backflow
+
      Integer(c_int) Function dfp_max_divergence(plan, U, V, W, out, flags) Bind(C, name='dfp_max_divergence')
        Import :: c_int, c_ptr, c_double
        Type(c_ptr), Value :: plan

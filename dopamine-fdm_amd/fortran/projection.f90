@@ -1,8 +1,9 @@
-!> Drop-in replacement of the reference's module projection (src/projection.f90) for the channel case
-!  (x periodic, y walls, z periodic) on one rank: the same subroutine names and arguments, backed by the
-!  dfp HIP library through its C ABI.  The fields are the global-module arrays (global.f90:139-146); they
-!  are copied to the device and back on each call.  A caller that keeps them on the device passes device
-!  addresses with DFP_FIELDS_ON_DEVICE instead (see dfp_project in include/dfp.h).
+!> Drop-in replacement of the reference's module projection (src/projection.f90) for y walls and z periodic
+!  with a periodic or an inflow/outflow x (x_bc_type 0 or 1, with the outlet backflow term) on one rank: the
+!  same subroutine names and arguments, backed by the dfp HIP library through its C ABI.  The fields are the
+!  global-module arrays (global.f90:139-146, :188-197); they are copied to the device and back on each call.
+!  A caller that keeps them on the device passes device addresses with DFP_FIELDS_ON_DEVICE instead (see
+!  dfp_project in include/dfp.h).
 Module projection
 
   Use iso_c_binding
@@ -13,15 +14,17 @@ Module projection
   Implicit None
 
   Private
-  Public :: compute_projection_step, compute_pseudo_pressure_rhs, solve_poisson_equation, project_velocity, &
-            apply_periodic_xz_pressure, update_ghost_interior_planes_pressure, &
-            update_ghost_interior_planes_pressure_x, finish_pressure_ghosts, check_divergence, projection_release
+  Public :: compute_projection_step, compute_pseudo_pressure_rhs, set_outflow_backflow_pressure, &
+            solve_poisson_equation, project_velocity, apply_periodic_xz_pressure, &
+            update_ghost_interior_planes_pressure, update_ghost_interior_planes_pressure_x, &
+            finish_pressure_ghosts, check_divergence, projection_release
 
   Type(c_ptr), Save :: plan = c_null_ptr
   ! the grid the plan was built from (c_loc needs TARGET arrays)
   Real(c_double), Allocatable, Target, Save :: gx(:), gy(:), gz(:), gyg(:), gzg(:)
-  ! stands in for P when no pressure is saved
+  ! stands in for P when no pressure is saved, and for Pb and phib without the backflow term
   Real(c_double), Save :: no_p(1) = 0d0
+  Real(c_double), Save :: no_face(1) = 0d0
 
 Contains
 
@@ -40,12 +43,16 @@ Contains
   Subroutine ensure_plan
     Type(dfp_grid_t) :: g
     If ( c_associated(plan) ) Return
-    If ( x_bc_type /= 0 .Or. y_bc_type /= 1 .Or. z_bc_type /= 0 ) Then
-       Write(error_unit,'(A)') 'ERROR: the dfp projection supports x periodic, y walls, z periodic only'
+    If ( ( x_bc_type /= 0 .And. x_bc_type /= 1 ) .Or. y_bc_type /= 1 .Or. z_bc_type /= 0 ) Then
+       Write(error_unit,'(A)') 'ERROR: the dfp projection supports x periodic or inflow/outflow, y walls, ' // &
+            'z periodic only'
        Error Stop 1
     End If
     gx = x;  gy = y;  gz = z;  gyg = yg;  gzg = zg
-    g%nxp = nx - 2;  g%nym = ny - 1;  g%nzp = nz - 2
+    ! the periodic x drops its duplicate cell, the inflow/outflow x keeps all nxm (initialization.f90:331-338)
+    g%nxp = nx - 2
+    If ( x_bc_type == 1 ) g%nxp = nx - 1
+    g%nym = ny - 1;  g%nzp = nz - 2
     g%dx = dx;  g%dz = dz
     g%x = c_loc(gx);  g%y = c_loc(gy);  g%z = c_loc(gz);  g%yg = c_loc(gyg);  g%zg = c_loc(gzg)
     g%x_bc_type = x_bc_type;  g%y_bc_type = y_bc_type;  g%z_bc_type = z_bc_type
@@ -59,18 +66,38 @@ Contains
     plan = c_null_ptr
   End Subroutine projection_release
 
-  !> projection.f90:21-34
+  !> Hands the outlet backflow state (global.f90:188-197) to the plan before a call that reads it: the plan
+  !  reads Pb and writes phib during that call (outflow_Uc changes every step, boundary_conditions.f90:208)
+  Subroutine sync_outflow
+    If ( outflow_backflow_beta > 0d0 ) Then
+       Call dfp_check(dfp_set_outflow(plan, outflow_backflow_beta, outflow_Uc, Pb, phib), 'dfp_set_outflow')
+    Else
+       Call dfp_check(dfp_set_outflow(plan, 0d0, outflow_Uc, no_face, no_face), 'dfp_set_outflow')
+    End If
+  End Subroutine sync_outflow
+
+  !> projection.f90:21-34 (the backflow term of :24 included); c = rk_coef(rk_step,rk_step), which at the last
+  !  stage is the rk_coef(3,3) of the pressure save
   Subroutine compute_projection_step
     Integer(c_int) :: flags
     Call ensure_plan
+    Call sync_outflow
     flags = 0
     If ( rk_step == 3 ) flags = DFP_SAVE_P
     If ( rk_step == 3 ) Then
-       Call dfp_check(dfp_project(plan, U, V, W, rhs_p, P, dt, rk_coef(3,3), flags), 'dfp_project')
+       Call dfp_check(dfp_project(plan, U, V, W, rhs_p, P, dt, rk_coef(rk_step,rk_step), flags), 'dfp_project')
     Else
-       Call dfp_check(dfp_project(plan, U, V, W, rhs_p, no_p, dt, rk_coef(3,3), flags), 'dfp_project')
+       Call dfp_check(dfp_project(plan, U, V, W, rhs_p, no_p, dt, rk_coef(rk_step,rk_step), flags), 'dfp_project')
     End If
   End Subroutine compute_projection_step
+
+  !> projection.f90:65-96
+  Subroutine set_outflow_backflow_pressure
+    Call ensure_plan
+    Call sync_outflow
+    Call dfp_check(dfp_outflow_backflow(plan, U, V, W, rhs_p, dt, rk_coef(rk_step,rk_step), 0_c_int), &
+         'dfp_outflow_backflow')
+  End Subroutine set_outflow_backflow_pressure
 
   !> projection.f90:37-63
   Subroutine compute_pseudo_pressure_rhs
@@ -92,15 +119,17 @@ Contains
     End If
   End Subroutine solve_poisson_equation
 
-  !> projection.f90:551-604
+  !> projection.f90:551-604 (the outlet face with phib, :578-589)
   Subroutine project_velocity
     Call ensure_plan
+    Call sync_outflow
     Call dfp_check(dfp_correct(plan, U, V, W, rhs_p, 0_c_int), 'dfp_correct')
   End Subroutine project_velocity
 
-  !> projection.f90:441-500 on one rank: the periodic duplicate planes of rhs_p
+  !> projection.f90:441-500 on one rank: the periodic duplicate planes of rhs_p (the inflow/outflow x has
+  !  none in x, :446-448)
   Subroutine apply_periodic_xz_pressure
-    rhs_p ( nxg-1, :, : ) = rhs_p ( 2, :, : )
+    If ( x_bc_type == 0 ) rhs_p ( nxg-1, :, : ) = rhs_p ( 2, :, : )
     rhs_p ( 2:nxg-1, :, nz ) = rhs_p ( 2:nxg-1, :, 2 )
   End Subroutine apply_periodic_xz_pressure
 
@@ -112,14 +141,20 @@ Contains
   Subroutine update_ghost_interior_planes_pressure_x
   End Subroutine update_ghost_interior_planes_pressure_x
 
-  !> projection.f90:388-438 with boundary_conditions.f90:277-289, :591-602 (id = 4) on one rank
+  !> projection.f90:388-438 with boundary_conditions.f90:277-289, :591-602 (id = 4) on one rank; Neumann
+  !  copies at the two ends of the inflow/outflow x (:413-422)
   Subroutine finish_pressure_ghosts(Pf)
     Real(Int64), Intent(InOut) :: Pf(nxg,nyg,nzg)
     Pf(:,  1,:) = Pf(:,    2,:)
     Pf(:,nyg,:) = Pf(:,nyg-1,:)
-    Pf(    1,:,:) = Pf(nxg-2,:,:)
-    Pf(nxg-1,:,:) = Pf(    2,:,:)
-    Pf(nxg  ,:,:) = Pf(    3,:,:)
+    If ( x_bc_type == 0 ) Then
+       Pf(    1,:,:) = Pf(nxg-2,:,:)
+       Pf(nxg-1,:,:) = Pf(    2,:,:)
+       Pf(nxg  ,:,:) = Pf(    3,:,:)
+    Else
+       Pf(  1,:,:) = Pf(    2,:,:)
+       Pf(nxg,:,:) = Pf(nxg-1,:,:)
+    End If
     Pf(:,:,    1) = Pf(:,:,nzg-2)
     Pf(:,:,nzg-1) = Pf(:,:,    2)
     Pf(:,:,nzg  ) = Pf(:,:,    3)

@@ -1,4 +1,4 @@
-!> Test stand-in for the part of the solver's global module (global.f90:130-184, :293-294) that the
+!> Test stand-in for the part of the solver's global module (global.f90:130-197, :293-294) that the
 !  drop-in projection module reads: the grid, the fields and the time-step data of one rank.
 Module global
   Use iso_fortran_env, Only : Int32, Int64
@@ -10,4 +10,7 @@ Module global
   Real(Int64), Allocatable, Dimension(:) :: x, y, z, yg, zg
   Real(Int64), Allocatable, Dimension(:,:,:) :: U, V, W, P, rhs_p
   Real(Int64), Allocatable, Dimension(:,:) :: rk_coef
+  ! outlet backflow stabilisation of the inflow/outflow x (global.f90:188-197)
+  Real(Int64) :: outflow_backflow_beta = 0d0, outflow_Uc = 0d0
+  Real(Int64), Allocatable, Dimension(:,:) :: Pb, phib
 End Module global

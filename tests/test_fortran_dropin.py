@@ -39,13 +39,15 @@ def test_dropin_compiles_and_links_against_the_c_abi():
         assert os.path.exists(exe)
 
 
-def _write_case(path, g, U, V, W, dt):
+def _write_case(path, g, U, V, W, dt, beta=0.0, uc=0.0, Pb=None):
+    if Pb is None:
+        Pb = np.zeros((g.nyg - 2, g.nzg - 2), order="F")
     with open(path, "wb") as f:
-        np.array([g.nxp, g.nym, g.nzp], dtype=np.int32).tofile(f)
-        np.array([g.dx, g.dz, dt]).tofile(f)
+        np.array([g.nxp, g.nym, g.nzp, getattr(g, "x_bc_type", 0)], dtype=np.int32).tofile(f)
+        np.array([g.dx, g.dz, dt, beta, uc]).tofile(f)
         for a in (g.x, g.y, g.z, g.yg, g.zg):
             np.ascontiguousarray(a, dtype=np.float64).tofile(f)
-        for a in (U, V, W):
+        for a in (U, V, W, Pb):
             a.ravel(order="F").tofile(f)
 
 
@@ -56,25 +58,37 @@ def _read_out(path, g):
         n = int(np.prod(shape))
         out.append(raw[o:o + n].reshape(shape, order="F"))
         o += n
-    return out, float(raw[o])
+    maxdiv = float(raw[o])
+    phib = raw[o + 1:].reshape((g.nyg - 2, g.nzg - 2), order="F")
+    return out, maxdiv, phib
+
+
+# (grid dims, x_bc_type, outflow_backflow_beta)
+DROPIN_CASES = [((16, 12, 8), 0, 0.0), ((24, 17, 18), 0, 0.0), ((16, 10, 12), 1, 0.0), ((12, 9, 8), 1, 0.8)]
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("dims", [(16, 12, 8), (24, 17, 18)])
-def test_dropin_projection_matches_oracle(dims, tmp_path):
+@pytest.mark.parametrize("dims,xbc,beta", DROPIN_CASES, ids=lambda v: str(v))
+def test_dropin_projection_matches_oracle(dims, xbc, beta, tmp_path):
     if not os.path.exists(DRIVER):
         raise RuntimeError(f"Fortran driver not built: {DRIVER} (run __graft_entry__.build())")
-    g = make_grid(*dims, 1.8)
+    g = make_grid(*dims, 1.8, x_bc_type=xbc)
     U, V, W, P, rhs = fields(g, 21, 0.1)
+    Pb = np.asfortranarray(np.random.default_rng(2).standard_normal((g.nyg - 2, g.nzg - 2)))
+    uc = 0.9
     fin, fout = tmp_path / "in.bin", tmp_path / "out.bin"
-    _write_case(fin, g, U, V, W, 1e-3)
-    print(f"fortran driver {dims}", flush=True)
+    _write_case(fin, g, U, V, W, 1e-3, beta, uc, Pb)
+    print(f"fortran driver {dims} x_bc_type {xbc} beta {beta}", flush=True)
     subprocess.run([DRIVER, str(fin), str(fout)], check=True, timeout=120)
-    (Uf, Vf, Wf, rf, Pf), maxdiv = _read_out(fout, g)
+    (Uf, Vf, Wf, rf, Pf), maxdiv, phibf = _read_out(fout, g)
     op = ref_c.CProjection(g)
+    if beta > 0:
+        op.set_outflow(beta, uc, Pb)
     op.compute_projection_step(U, V, W, rhs, P, 1e-3, 0.75)
     for lab, a, b in zip(("U", "V", "W", "phi", "P"), (Uf, Vf, Wf, rf, Pf), (U, V, W, rhs, P)):
         assert rel_linf(a, b) <= 1e-12, lab
+    if beta > 0:
+        assert rel_linf(phibf, op.phib) <= 1e-12
     # check_divergence on the driver's own corrected fields: the same stencil, bit-identical
     assert maxdiv == op.check_divergence(Uf, Vf, Wf)
     # and close to the oracle's value on its own fields (which differ at round-off)
