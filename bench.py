@@ -70,10 +70,12 @@ def measured_traffic(kernel: str):
         return None
 
 
-def cpu_baseline(g, budget_s: float = 10.0) -> dict:
-    """The C oracle (restatement of the reference's CPU FFTW path) on the host
-    cores, on a bounded sample of the same workload: as many projections as
-    finish in about budget_s seconds."""
+def cpu_baseline(g, budget_s: float = 10.0, budget_4_s: float = 6.0) -> dict:
+    """The C oracle (restatement of the reference's CPU FFTW path, with its
+    own mixed-radix FFT: no FFTW or MKL is built here) on the host cores, on a
+    bounded sample of the same workload: as many projections as finish in about
+    budget_s seconds on every core, then budget_4_s seconds on 4 threads (the
+    reference's configuration is quoted at 4 MPI ranks)."""
     from oracle import ref_c
 
     import dfp
@@ -86,18 +88,48 @@ def cpu_baseline(g, budget_s: float = 10.0) -> dict:
     U, V, W = dfp.synth_fields(g, 777, SIGMA)
     P = np.zeros(g.shape_p, order="F")
     rhs = np.zeros(g.shape_rhs, order="F")
-    op = ref_c.CProjection(g, cores)
-    op.compute_projection_step(U, V, W, rhs, None, DT, RK_C[0])  # warm-up
-    n, t0 = 0, time.perf_counter()
-    while True:
-        op.compute_projection_step(U, V, W, rhs, P if n % 3 == 2 else None, DT, RK_C[2])
-        n += 1
-        el = time.perf_counter() - t0
-        if el >= budget_s:
-            break
-    op.close()
+
+    def run(threads, budget):
+        op = ref_c.CProjection(g, threads)
+        op.compute_projection_step(U, V, W, rhs, None, DT, RK_C[0])  # warm-up
+        n, t0 = 0, time.perf_counter()
+        while True:
+            op.compute_projection_step(U, V, W, rhs, P if n % 3 == 2 else None, DT, RK_C[2])
+            n += 1
+            el = time.perf_counter() - t0
+            if el >= budget:
+                break
+        op.close()
+        return n, el
+
+    n, el = run(cores, budget_s)
+    n4, el4 = run(min(4, cores), budget_4_s)
     return {"value": n / el, "unit": "projections/s", "cores": int(cores), "kind": "port",
-            "sample": f"{n} projections at {g.nxp}x{g.nym}x{g.nzp} FP64 (C oracle, OpenMP, own FFT), {el:.1f} s"}
+            "sample": f"{n} projections at {g.nxp}x{g.nym}x{g.nzp} FP64 (C oracle, OpenMP, own FFT), {el:.1f} s",
+            "threads_4": {"value": round(n4 / el4, 3), "cores": min(4, int(cores)),
+                          "sample": f"{n4} projections, {el4:.1f} s"},
+            "fft": "own mixed-radix Stockham FFT (no FFTW/MKL in the image)"}
+
+
+def bench_host_staged(g, reps: int = 6) -> dict:
+    """The projection as the Fortran drop-in module calls it: host arrays,
+    copied to the device and back inside dfp_project (flags 0), so PCIe is in
+    the time.  Pageable numpy arrays, like a Fortran module's arrays."""
+    import dfp
+
+    plan = dfp.Plan(g)
+    U, V, W = dfp.synth_fields(g, 99, SIGMA)
+    P = np.zeros(g.shape_p, order="F")
+    rhs = np.zeros(g.shape_rhs, order="F")
+    plan.project(U, V, W, rhs, P, DT, RK_C[2], save_p=True)  # warm-up (allocates the staging)
+    t0 = time.perf_counter()
+    for i in range(reps):
+        plan.project(U, V, W, rhs, P, DT, RK_C[2], save_p=(i % 3 == 2))
+    el = time.perf_counter() - t0
+    plan.close()
+    moved = 8 * (2 * (np.prod(g.shape_u) + np.prod(g.shape_v) + np.prod(g.shape_w)) + 2 * np.prod(g.shape_rhs))
+    return {"projections_per_s": round(reps / el, 2), "us_per_projection": round(1e6 * el / reps, 1),
+            "pcie_bytes_per_projection": int(moved), "reps": reps}
 
 
 def make_sets(plan, g, n_sets, seed0):
@@ -410,6 +442,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 48-case batched measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-inflow", action="store_true", help="skip the inflow/outflow x measurement")
+    ap.add_argument("--no-host-staged", action="store_true", help="skip the host-staged (PCIe) measurement")
     ap.add_argument("--save-tuning", default=None, help="write the autotuner table (with new entries) here")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
@@ -526,6 +559,8 @@ def main() -> int:
                                 "kernels_us": k32l,
                                 "rel_err_vs_fp64_by_sweeps": mixed_errors(gl, (1, 2, 3), 4343)}
         out["mixed"] = mixed
+    if not args.no_host_staged:
+        out["host_staged"] = bench_host_staged(g)
     if not args.no_inflow:
         if args.no_mixed:
             plan.close()

@@ -301,14 +301,27 @@ int d2h(dfp_plan* p, double* dst, const double* src, int64_t n) {
     return DFP_OK;
 }
 
+// the monitor's NaN/overflow guard (monitor.f90:32-38) on device fields:
+// DFP_E_NONFINITE if a value is NaN or above 1e8 in magnitude
+int guard_finite(dfp_plan* p, const double* U, const double* V, const double* W) {
+    double* d_out = p->d_red + p->red_len;
+    double bad = 0.0;
+    HIPCHK(p, launch_guard_finite(p, U, V, W, d_out));
+    HIPCHK(p, hipMemcpyAsync(&bad, d_out, sizeof(double), hipMemcpyDeviceToHost, p->stream));
+    HIPCHK(p, hipStreamSynchronize(p->stream));
+    if (bad != 0.0) return fail(p, DFP_E_NONFINITE, "u* holds NaN or a value above 1e8 in magnitude");
+    return DFP_OK;
+}
+
 int finish(dfp_plan* p, int flags) {
     HIPCHK(p, hipGetLastError());
     if (!(flags & DFP_ASYNC) || !(flags & DFP_FIELDS_ON_DEVICE)) HIPCHK(p, hipStreamSynchronize(p->stream));
     return DFP_OK;
 }
 
-int check_flags(dfp_plan* p, int flags) {
-    if (flags & ~(DFP_FIELDS_ON_DEVICE | DFP_SAVE_P | DFP_ASYNC)) return fail(p, DFP_E_ARG, "unknown flag bits 0x%x", flags);
+int check_flags(dfp_plan* p, int flags, int extra = 0) {
+    if (flags & ~(DFP_FIELDS_ON_DEVICE | DFP_SAVE_P | DFP_ASYNC | extra))
+        return fail(p, DFP_E_ARG, "unknown flag bits 0x%x", flags);
     HIPCHK(p, hipSetDevice(p->device));
     return DFP_OK;
 }
@@ -651,7 +664,7 @@ int dfp_field_sizes(const dfp_plan* p, int64_t sizes[5]) {
 int dfp_project(dfp_plan* p, double* U, double* V, double* W, double* rhs_p, double* P, double dt, double c,
                 int flags) {
     if (!p) return DFP_E_ARG;
-    int rc = check_flags(p, flags);
+    int rc = check_flags(p, flags, DFP_CHECK_FINITE);
     if (rc) return rc;
     const int save_p = (flags & DFP_SAVE_P) ? 1 : 0;
     if (!U || !V || !W) return fail(p, DFP_E_ARG, "U, V and W are required");
@@ -660,8 +673,10 @@ int dfp_project(dfp_plan* p, double* U, double* V, double* W, double* rhs_p, dou
     const bool bf = backflow(p);
     if (bf && (!(dt * c != 0.0) || !std::isfinite(dt * c)))
         return fail(p, DFP_E_ARG, "the outlet backflow term needs a finite nonzero dt*c");
+    const bool guard = (flags & DFP_CHECK_FINITE) != 0;
     if (flags & DFP_FIELDS_ON_DEVICE) {
         if (!rhs_p) return fail(p, DFP_E_ARG, "rhs_p is required with DFP_FIELDS_ON_DEVICE");
+        if (guard && (rc = guard_finite(p, U, V, W))) return rc;
         // compute_projection_step (projection.f90:21-34): divergence, backflow
         // face values, solve, correction
         if (bf) HIPCHK(p, launch_outflow_phib(p, U, V, W, p->pb, p->phib, nullptr, dt * c));
@@ -674,6 +689,7 @@ int dfp_project(dfp_plan* p, double* U, double* V, double* W, double* rhs_p, dou
     if (bf && (rc = ensure_outflow_staging(p))) return rc;
     if ((rc = h2d(p, p->d_U, U, p->sz_u)) || (rc = h2d(p, p->d_V, V, p->sz_v)) || (rc = h2d(p, p->d_W, W, p->sz_w)))
         return rc;
+    if (guard && (rc = guard_finite(p, p->d_U, p->d_V, p->d_W))) return rc;
     if (rhs_p && (rc = h2d(p, p->d_rhs, rhs_p, p->sz_rhs))) return rc;
     if (bf) {
         if ((rc = h2d(p, p->d_pb, p->pb, face_size(p)))) return rc;

@@ -268,6 +268,8 @@ hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const do
 hipError_t launch_outflow_phib(dfp_plan* p, const double* U, const double* V, const double* W, const double* Pb,
                                double* phib, double* rhs, double dtc);
 hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);
+// out_dev = 1 if a velocity value in the monitor's ranges is NaN or above 1e8 in magnitude, else 0
+hipError_t launch_guard_finite(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);
 hipError_t launch_synth(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma);
 
 // parameter records and batched (persistent) launches: cs (records), off

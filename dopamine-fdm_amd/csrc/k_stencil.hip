@@ -314,6 +314,47 @@ hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const d
     return hipGetLastError();
 }
 
+// Field guard of the reference's monitor (monitor.f90:32-38): a velocity value
+// in U(2:nx-1,2:nyg-1,2:nzg-1), V(2:nxg-1,2:ny-1,2:nzg-1) or
+// W(2:nxg-1,2:nyg-1,2:nz-1) that is NaN or exceeds 1e8 in magnitude.  Block b
+// strides over the three ranges and writes 1 (found) or 0 to partials[b].
+__global__ void __launch_bounds__(256) k_guard_finite(const double* __restrict__ U, const double* __restrict__ V,
+                                                      const double* __restrict__ W, double* __restrict__ partials,
+                                                      Geo g) {
+    __shared__ double red[256];
+    const long nk = g.nzg - 2;
+    const long su = (long)(g.nx - 2) * g.nym * nk;             // U interior
+    const long sv = (long)(g.nxg - 2) * (g.ny - 2) * nk;       // V interior faces
+    const long sw = (long)(g.nxg - 2) * g.nym * (g.nz - 2);    // W interior faces
+    double bad = 0.0;
+    for (long e = (long)blockIdx.x * 256 + threadIdx.x; e < su + sv + sw; e += (long)gridDim.x * 256) {
+        const double* F;
+        long r, n1, n2, s1, s2;
+        if (e < su) {
+            F = U; r = e; n1 = g.nx - 2; n2 = g.nym; s1 = g.nx; s2 = (long)g.nx * g.nyg;
+        } else if (e < su + sv) {
+            F = V; r = e - su; n1 = g.nxg - 2; n2 = g.ny - 2; s1 = g.nxg; s2 = (long)g.nxg * g.ny;
+        } else {
+            F = W; r = e - su - sv; n1 = g.nxg - 2; n2 = g.nym; s1 = g.nxg; s2 = (long)g.nxg * g.nyg;
+        }
+        const long q = r / n1, k = q / n2;
+        const long i = r - q * n1, j = q - k * n2;
+        const double x = F[(i + 1) + (j + 1) * s1 + (k + 1) * s2];
+        if (x != x || fabs(x) > 1e8) bad = 1.0;
+    }
+    const double m = block_max(bad, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = m;
+}
+
+hipError_t launch_guard_finite(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev) {
+    const int blocks = std::max(1, std::min(1024, p->red_len));
+    const int slot = prof_begin(p, "guard_finite");
+    hipLaunchKernelGGL(k_guard_finite, dim3(blocks), dim3(256), 0, p->stream, U, V, W, p->d_red, p->geo);
+    hipLaunchKernelGGL(k_reduce_max, dim3(1), dim3(256), 0, p->stream, p->d_red, (long)blocks, out_dev);
+    prof_end(p, slot);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // Allocation-free max-divergence (dfp_max_divergence_noalloc).  Its partial
 // maxima and result live in a fixed array of device memory instead of the

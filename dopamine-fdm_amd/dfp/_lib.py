@@ -14,6 +14,7 @@ DFP_FIELDS_ON_DEVICE = 1
 DFP_SAVE_P = 2
 DFP_ASYNC = 4
 DFP_SOLVE_ONLY = 8
+DFP_CHECK_FINITE = 16
 
 _dp = ctypes.POINTER(ctypes.c_double)
 

@@ -22,7 +22,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from . import _lib
-from ._lib import DFP_ASYNC, DFP_FIELDS_ON_DEVICE, DFP_FP64, DFP_SAVE_P, DfpGrid, DfpTuning, check, lib
+from ._lib import DFP_ASYNC, DFP_CHECK_FINITE, DFP_FIELDS_ON_DEVICE, DFP_FP64, DFP_SAVE_P, DfpGrid, DfpTuning, check, lib
 from .grid import ChannelGrid
 from .hipmem import DeviceArray
 
@@ -135,7 +135,9 @@ class Plan:
         check(lib().dfp_set_stream(self._h, stream_handle), "dfp_set_stream", self._h)
 
     # --- entry points -----------------------------------------------------------
-    def project(self, U, V, W, rhs_p, P=None, dt=0.0, c=0.0, save_p=False, asynchronous=False):
+    def project(self, U, V, W, rhs_p, P=None, dt=0.0, c=0.0, save_p=False, asynchronous=False, check_finite=False):
+        """check_finite: refuse u* holding NaN or |u| > 1e8 (the monitor's guard,
+        monitor.f90:32-38) with DfpError code DFP_E_NONFINITE before writing anything."""
         g = self.g
         pu, du = _ptr(U, g.shape_u, "U")
         pv, dv = _ptr(V, g.shape_v, "V")
@@ -143,6 +145,8 @@ class Plan:
         pr, dr = _ptr(rhs_p, g.shape_rhs, "rhs_p")
         pp, dpp = _ptr(P, g.shape_p, "P")
         f = self._flags(du, dv, dw, dr, dpp, save_p=save_p, asynchronous=asynchronous)
+        if check_finite:
+            f |= DFP_CHECK_FINITE
         check(lib().dfp_project(self._h, pu, pv, pw, pr, pp, float(dt), float(c), f), "dfp_project", self._h)
 
     def set_outflow(self, beta: float, outflow_Uc: float, Pb=None, phib=None):

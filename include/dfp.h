@@ -61,7 +61,10 @@ enum {
     DFP_FIELDS_ON_DEVICE = 1, /* array arguments are device pointers (resident mode)  */
     DFP_SAVE_P = 2,           /* also write P = phi/(dt*c) with its ghosts (rk_step 3) */
     DFP_ASYNC = 4,            /* do not synchronise the stream before returning       */
-    DFP_SOLVE_ONLY = 8        /* dfp_batch_project: solve rhs_p -> phi as dfp_solve   */
+    DFP_SOLVE_ONLY = 8,       /* dfp_batch_project: solve rhs_p -> phi as dfp_solve   */
+    DFP_CHECK_FINITE = 16     /* dfp_project: refuse u* holding NaN or |u| > 1e8 in the
+                                 monitor's ranges (monitor.f90:32-38), DFP_E_NONFINITE,
+                                 before anything is written                            */
 };
 
 /* error codes */

@@ -102,7 +102,8 @@ def test_precision_argument_is_checked():
 def test_flag_values_match_the_header():
     txt = open(_lib.HEADER_PATH).read()
     vals = {k: int(v) for k, v in re.findall(r"\b(DFP_[A-Z0-9_]+)\s*=\s*(\d+)", txt)}
-    for name in ("DFP_FIELDS_ON_DEVICE", "DFP_SAVE_P", "DFP_ASYNC", "DFP_SOLVE_ONLY", "DFP_FP64", "DFP_FP32_REFINE"):
+    for name in ("DFP_FIELDS_ON_DEVICE", "DFP_SAVE_P", "DFP_ASYNC", "DFP_SOLVE_ONLY", "DFP_CHECK_FINITE", "DFP_FP64",
+                 "DFP_FP32_REFINE"):
         assert getattr(dfp, name) == vals[name], name
 
 

@@ -142,7 +142,9 @@ def test_config2_full_size_parity(seed):
     assert md <= div_bound(g, U, V, W)
 
 
-GOLDEN = sorted(glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz")))
+# projection fixtures (the snapshot-reader fixture has its own test)
+GOLDEN = sorted(p for p in glob.glob(os.path.join(os.path.dirname(__file__), "golden", "*.npz"))
+                if not os.path.basename(p).startswith("snapshot_"))
 
 
 @pytest.mark.parametrize("path", GOLDEN, ids=[os.path.basename(p) for p in GOLDEN])
@@ -216,3 +218,36 @@ def test_seeded_shape_sweep(shape):
         for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
             err = rel_linf(a, b)
             assert err <= tol, f"{shape} prec {prec} {lab}: {err:.3e}"
+
+
+@pytest.mark.parametrize("field,value", [("U", np.nan), ("V", np.inf), ("W", -2.0e8)])
+def test_check_finite_refuses_bad_fields(field, value):
+    """The monitor's guard (monitor.f90:32-38) as an opt-in of dfp_project: a
+    NaN, an Inf or |u| > 1e8 in u* is refused with DFP_E_NONFINITE before
+    anything is written, host-staged and device-resident; a clean field
+    projects as without the guard."""
+    g = make_grid(16, 10, 8, 1.5)
+    U, V, W, P, rhs = fields(g, 8, 0.1)
+    bad = [a.copy(order="F") for a in (U, V, W)]
+    Fb = bad["UVW".index(field)]
+    Fb[3, 4, 5] = value
+    plan = dfp.Plan(g)
+    rhs0 = rhs.copy(order="F")
+    with pytest.raises(dfp.DfpError) as ei:
+        plan.project(bad[0], bad[1], bad[2], rhs0, P, DT, C3, save_p=True, check_finite=True)
+    assert ei.value.code == -5
+    assert np.array_equal(rhs0, rhs) and np.isnan(Fb[3, 4, 5]) == np.isnan(value)
+    dev = [dfp.DeviceArray.from_host(a) for a in (*bad, P, rhs)]
+    with pytest.raises(dfp.DfpError):
+        plan.project(dev[0], dev[1], dev[2], dev[4], dev[3], DT, C3, save_p=True, check_finite=True)
+    assert np.array_equal(dev[4].download(), rhs)
+    # a value outside the monitor's ranges (a ghost) is not refused; a clean field projects
+    ok = [a.copy(order="F") for a in (U, V, W)]
+    ok[0][0, 0, 0] = np.nan
+    a = [x.copy(order="F") for x in (*ok, P, rhs)]
+    b = [x.copy(order="F") for x in (*ok, P, rhs)]
+    plan.project(a[0], a[1], a[2], a[4], a[3], DT, C3, save_p=True, check_finite=True)
+    plan.project(b[0], b[1], b[2], b[4], b[3], DT, C3, save_p=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y, equal_nan=True)
+    plan.close()

@@ -102,3 +102,50 @@ def test_truncated_or_short_files_raise(tmp_path):
     short.write_bytes(raw[: len(raw) - (12 + 8 * int(np.prod(g.shape_p)))])
     with pytest.raises(ValueError):
         read_snapshot(short)
+
+
+# ---------------------------------------------------------------------------
+# the reference's own reader (postProcessing/dopamine_post/fields.py,
+# FieldSnapshot.read, :188-247), through the fixture
+# tests/golden/make_snapshot_golden.py wrote by running it on files from
+# write_snapshot
+# ---------------------------------------------------------------------------
+def _golden_cases():
+    import hashlib
+    import os
+    import sys
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden"))
+    import make_snapshot_golden as mk
+
+    return mk.cases(), hashlib
+
+
+def _centre(U, V, W, P):
+    # cell centres with the ghost ring stripped, as the reader presents fields
+    return (0.5 * (U[:-1, 1:-1, 1:-1] + U[1:, 1:-1, 1:-1]), 0.5 * (V[1:-1, :-1, 1:-1] + V[1:-1, 1:, 1:-1]),
+            0.5 * (W[1:-1, 1:-1, :-1] + W[1:-1, 1:-1, 1:]), P[1:-1, 1:-1, 1:-1])
+
+
+def test_written_files_are_the_ones_the_reference_reader_decoded(tmp_path):
+    """write_snapshot reproduces, byte for byte, the files the reference's
+    reader decoded into the fixture; read_snapshot then gives back the inputs,
+    whose coordinates and cell-centred fields are the reader's, bit for bit."""
+    import os
+
+    f = np.load(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "snapshot_reader.npz"))
+    cases, hashlib = _golden_cases()
+    for name, g, U, V, W, P, extra, sed in cases:
+        assert np.array_equal(f[f"{name}_inputs"], np.concatenate([a.ravel(order="F") for a in (U, V, W, P, *extra)]))
+        path = tmp_path / f"snap_{name}.00000100"
+        write_snapshot(path, g, U, V, W, P, extra)
+        assert hashlib.sha256(path.read_bytes()).digest() == f[f"{name}_sha256"].tobytes(), name
+        got = read_snapshot(path, g)
+        for c in ("x", "y", "z", "xm", "ym", "zm"):
+            assert np.array_equal(got[c], f[f"{name}_{c}"]), (name, c)
+        for fld, a in zip("UVWP", _centre(got["U"], got["V"], got["W"], got["P"])):
+            assert np.array_equal(a, f[f"{name}_field_{fld}"]), (name, fld)
+        if sed:
+            assert np.array_equal(got["extra"][0][1:-1, 1:-1, 1:-1], f[f"{name}_field_C"]), name
+        else:
+            assert f"{name}_field_C" not in f
