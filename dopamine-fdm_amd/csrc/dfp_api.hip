@@ -742,6 +742,84 @@ int dfp_outflow_backflow(dfp_plan* p, const double* U, const double* V, const do
     return finish(p, flags);
 }
 
+// the step either side of the projection: device-resident fields, channel case
+static int step_checks(dfp_plan* p, int flags) {
+    if (flags & ~(DFP_FIELDS_ON_DEVICE | DFP_ASYNC | DFP_SAVE_P))
+        return fail(p, DFP_E_ARG, "unknown flag bits 0x%x", flags);
+    if (!(flags & DFP_FIELDS_ON_DEVICE))
+        return fail(p, DFP_E_ARG, "the RK update and boundary conditions take device-resident fields only");
+    if (p->geo.xbc != 0)
+        return fail(p, DFP_E_UNSUPPORTED, "the boundary conditions here are the channel's (x periodic) only");
+    HIPCHK(p, hipSetDevice(p->device));
+    return DFP_OK;
+}
+
+static int rk_args(dfp_plan* p, int stage, const double* U, const double* V, const double* W, const double* Uo,
+                   const double* Vo, const double* Wo, const double* const* Fu, const double* const* Fv,
+                   const double* const* Fw, const double* coef, double dt) {
+    if (stage < 1 || stage > 3) return fail(p, DFP_E_ARG, "stage must be 1, 2 or 3, got %d", stage);
+    if (!U || !V || !W || !Uo || !Vo || !Wo || !Fu || !Fv || !Fw || !coef)
+        return fail(p, DFP_E_ARG, "U, V, W, Uo, Vo, Wo, Fu, Fv, Fw and coef are required");
+    for (int s = 0; s < stage; ++s)
+        if (!Fu[s] || !Fv[s] || !Fw[s]) return fail(p, DFP_E_ARG, "right-hand side %d of the stage is null", s + 1);
+    for (int s = 0; s < stage; ++s)
+        if (!std::isfinite(coef[s])) return fail(p, DFP_E_ARG, "rk_coef(%d,%d) is not finite", stage, s + 1);
+    if (!std::isfinite(dt)) return fail(p, DFP_E_ARG, "dt is not finite");
+    return DFP_OK;
+}
+
+int dfp_rk_update(dfp_plan* p, int stage, double* U, double* V, double* W, const double* Uo, const double* Vo,
+                  const double* Wo, const double* const* Fu, const double* const* Fv, const double* const* Fw,
+                  const double* coef, double dt, int flags) {
+    if (!p) return DFP_E_ARG;
+    int rc = step_checks(p, flags & ~DFP_SAVE_P);
+    if (rc) return rc;
+    if ((rc = rk_args(p, stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef, dt))) return rc;
+    const double* Fo[3] = {Uo, Vo, Wo};
+    const double* const* R[3] = {Fu, Fv, Fw};
+    HIPCHK(p, launch_rk_update(p, stage, U, V, W, Fo, R, coef, dt));
+    return finish(p, flags);
+}
+
+int dfp_apply_bc(dfp_plan* p, double* U, double* V, double* W, const double* alpha_x, const double* alpha_y,
+                 const double* alpha_z, int flags) {
+    if (!p) return DFP_E_ARG;
+    int rc = step_checks(p, flags & ~DFP_SAVE_P);
+    if (rc) return rc;
+    if (!U || !V || !W || !alpha_x || !alpha_y || !alpha_z)
+        return fail(p, DFP_E_ARG, "U, V, W and alpha_x, alpha_y, alpha_z are required");
+    HIPCHK(p, launch_apply_bc(p, U, V, W, alpha_x, alpha_y, alpha_z));
+    return finish(p, flags);
+}
+
+int dfp_substep(dfp_plan* p, int stage, double* U, double* V, double* W, const double* Uo, const double* Vo,
+                const double* Wo, const double* const* Fu, const double* const* Fv, const double* const* Fw,
+                const double* coef, const double* alpha_x, const double* alpha_y, const double* alpha_z,
+                double* rhs_p, double* P, double dt, int flags) {
+    if (!p) return DFP_E_ARG;
+    int rc = step_checks(p, flags);
+    if (rc) return rc;
+    if ((rc = rk_args(p, stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef, dt))) return rc;
+    if (!alpha_x || !alpha_y || !alpha_z || !rhs_p)
+        return fail(p, DFP_E_ARG, "alpha_x, alpha_y, alpha_z and rhs_p are required");
+    const int save_p = (flags & DFP_SAVE_P) ? 1 : 0;
+    const double c = coef[stage - 1];
+    if (save_p && (!P || !(dt * c != 0.0) || !std::isfinite(dt * c)))
+        return fail(p, DFP_E_ARG, "DFP_SAVE_P needs P and a finite nonzero dt*rk_coef(stage,stage)");
+    const double* Fo[3] = {Uo, Vo, Wo};
+    const double* const* R[3] = {Fu, Fv, Fw};
+    // one substage of time_integration.f90 without immersed boundary, VOF or
+    // scalars: the update :256-258 (:394-396, :526-531), apply_boundary_conditions
+    // :297 (:428, :564), the projection :300 (:431, :567) and the boundary
+    // conditions after it :304 (:435, :571), boundary_conditions.f90:186-260
+    HIPCHK(p, launch_rk_update(p, stage, U, V, W, Fo, R, coef, dt));
+    HIPCHK(p, launch_apply_bc(p, U, V, W, alpha_x, alpha_y, alpha_z));
+    if ((rc = spectral_solve(p, U, V, W, rhs_p, P, dt * c, save_p, 1))) return rc;
+    HIPCHK(p, launch_correct(p, U, V, W, rhs_p, nullptr));
+    HIPCHK(p, launch_apply_bc(p, U, V, W, alpha_x, alpha_y, alpha_z));
+    return finish(p, flags);
+}
+
 int dfp_divergence(dfp_plan* p, const double* U, const double* V, const double* W, double* rhs_p, int flags) {
     if (!p) return DFP_E_ARG;
     int rc = check_flags(p, flags);

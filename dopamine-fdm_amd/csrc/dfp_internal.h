@@ -268,6 +268,12 @@ hipError_t launch_correct(dfp_plan* p, double* U, double* V, double* W, const do
 hipError_t launch_outflow_phib(dfp_plan* p, const double* U, const double* V, const double* W, const double* Pb,
                                double* phib, double* rhs, double dtc);
 hipError_t launch_max_div(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);
+// RK3 stage update of the interiors (Fo: Uo, Vo, Wo; R[field][s]: right-hand sides, s < stage;
+// c = rk_coef(stage, 1:stage)) and the channel's apply_boundary_conditions (k_step.hip)
+hipError_t launch_rk_update(dfp_plan* p, int stage, double* U, double* V, double* W, const double* const Fo[3],
+                            const double* const* R[3], const double* c, double dt);
+hipError_t launch_apply_bc(dfp_plan* p, double* U, double* V, double* W, const double* ax, const double* ay,
+                           const double* az);
 // out_dev = 1 if a velocity value in the monitor's ranges is NaN or above 1e8 in magnitude, else 0
 hipError_t launch_guard_finite(dfp_plan* p, const double* U, const double* V, const double* W, double* out_dev);
 hipError_t launch_synth(dfp_plan* p, double* U, double* V, double* W, uint64_t seed, double sigma);

@@ -77,6 +77,11 @@ def lib() -> ctypes.CDLL:
         "dfp_solve": ([vp, vp, vp, d, d, i32], i32),
         "dfp_correct": ([vp, vp, vp, vp, vp, i32], i32),
         "dfp_set_outflow": ([vp, d, d, vp, vp], i32),
+        "dfp_rk_update": ([vp, i32, vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                           ctypes.POINTER(d), d, i32], i32),
+        "dfp_apply_bc": ([vp, vp, vp, vp, vp, vp, vp, i32], i32),
+        "dfp_substep": ([vp, i32, vp, vp, vp, vp, vp, vp, ctypes.POINTER(vp), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                         ctypes.POINTER(d), vp, vp, vp, vp, vp, d, i32], i32),
         "dfp_outflow_backflow": ([vp, vp, vp, vp, vp, d, d, i32], i32),
         "dfp_max_divergence": ([vp, vp, vp, vp, ctypes.POINTER(d), i32], i32),
         "dfp_max_divergence_noalloc": ([vp, vp, vp, vp, ctypes.POINTER(d), i32], i32),

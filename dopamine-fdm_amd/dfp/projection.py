@@ -173,6 +173,73 @@ class Plan:
         check(lib().dfp_outflow_backflow(self._h, pu, pv, pw, pr, float(dt), float(c), f), "dfp_outflow_backflow",
               self._h)
 
+    # --- the step either side of the projection (channel, device arrays) -------
+    def _step_arrays(self, stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef):
+        g = self.g
+        ptrs = []
+        for a, sh, name in ((U, g.shape_u, "U"), (V, g.shape_v, "V"), (W, g.shape_w, "W"),
+                            (Uo, g.shape_u, "Uo"), (Vo, g.shape_v, "Vo"), (Wo, g.shape_w, "Wo")):
+            p, dev = _ptr(a, sh, name)
+            if not dev:
+                raise ValueError(f"{name} must be device-resident")
+            ptrs.append(p)
+        shapes = ((g.nx - 2, g.nyg - 2, g.nzg - 2), (g.nxg - 2, g.ny - 2, g.nzg - 2), (g.nxg - 2, g.nyg - 2, g.nz - 2))
+        rhs = []
+        for F, sh, name in zip((Fu, Fv, Fw), shapes, ("Fu", "Fv", "Fw")):
+            if len(F) < stage:
+                raise ValueError(f"{name}: {stage} right-hand sides needed")
+            arr = (ctypes.c_void_p * 3)()
+            for s_ in range(stage):
+                p, dev = _ptr(F[s_], sh, name)
+                if not dev:
+                    raise ValueError(f"{name} must be device-resident")
+                arr[s_] = p
+            rhs.append(arr)
+        c = (ctypes.c_double * 3)(*[float(v) for v in list(coef)[:stage]])
+        return ptrs, rhs, c
+
+    def _alpha_ptrs(self, alpha_x, alpha_y, alpha_z):
+        g = self.g
+        out = []
+        for a, sh, name in ((alpha_x, (g.nx, 2, g.nzg), "alpha_x"), (alpha_y, (g.nxg, 2, g.nzg), "alpha_y"),
+                            (alpha_z, (g.nxg, 2, g.nz), "alpha_z")):
+            p, dev = _ptr(a, sh, name)
+            if not dev:
+                raise ValueError(f"{name} must be device-resident")
+            out.append(p)
+        return out
+
+    def rk_update(self, stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef, dt, asynchronous=False):
+        """time_integration.f90:256-258 / :394-396 / :526-531 on device arrays;
+        Fu, Fv, Fw: sequences of interior-shaped right-hand sides, coef =
+        rk_coef(stage, 1:stage)."""
+        ptrs, rhs, c = self._step_arrays(stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef)
+        f = DFP_FIELDS_ON_DEVICE | (DFP_ASYNC if asynchronous else 0)
+        check(lib().dfp_rk_update(self._h, int(stage), *ptrs, *rhs, c, float(dt), f), "dfp_rk_update", self._h)
+
+    def apply_bc(self, U, V, W, alpha_x, alpha_y, alpha_z, asynchronous=False):
+        """apply_boundary_conditions of the channel (boundary_conditions.f90:186-260)."""
+        g = self.g
+        pu, du = _ptr(U, g.shape_u, "U")
+        pv, dv = _ptr(V, g.shape_v, "V")
+        pw, dw = _ptr(W, g.shape_w, "W")
+        f = self._flags(du, dv, dw, asynchronous=asynchronous)
+        check(lib().dfp_apply_bc(self._h, pu, pv, pw, *self._alpha_ptrs(alpha_x, alpha_y, alpha_z), f),
+              "dfp_apply_bc", self._h)
+
+    def substep(self, stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef, alpha_x, alpha_y, alpha_z, rhs_p, P=None,
+                dt=0.0, save_p=False, asynchronous=False):
+        """One RK3 substage around the projection (time_integration.f90:256-304)."""
+        g = self.g
+        ptrs, rhs, c = self._step_arrays(stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef)
+        pr, dr = _ptr(rhs_p, g.shape_rhs, "rhs_p")
+        pp, dpp = _ptr(P, g.shape_p, "P")
+        if not dr or (P is not None and not dpp):
+            raise ValueError("rhs_p and P must be device-resident")
+        f = DFP_FIELDS_ON_DEVICE | (DFP_SAVE_P if save_p else 0) | (DFP_ASYNC if asynchronous else 0)
+        check(lib().dfp_substep(self._h, int(stage), *ptrs, *rhs, c, *self._alpha_ptrs(alpha_x, alpha_y, alpha_z),
+                                pr, pp, float(dt), f), "dfp_substep", self._h)
+
     def divergence(self, U, V, W, rhs_p):
         g = self.g
         pu, du = _ptr(U, g.shape_u, "U")

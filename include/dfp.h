@@ -180,6 +180,35 @@ int dfp_max_divergence(dfp_plan *plan, const double *U, const double *V, const d
 int dfp_max_divergence_noalloc(dfp_plan *plan, const double *U, const double *V, const double *W,
                                double *out, int flags);
 
+/* The step either side of the projection, for a channel plan (x_bc_type =
+ * 0) and device-resident fields (flags must hold DFP_FIELDS_ON_DEVICE;
+ * DFP_ASYNC allowed).  Fu, Fv, Fw are host arrays of `stage` device pointers
+ * to the right-hand sides Fu_s(2:nx-1,2:nyg-1,2:nzg-1),
+ * Fv_s(2:nxg-1,2:ny-1,2:nzg-1), Fw_s(2:nxg-1,2:nyg-1,2:nz-1)
+ * (initialization.f90:662-670); coef = rk_coef(stage, 1:stage).
+ *
+ * dfp_rk_update: the RK3 stage update of the interiors of U, V, W from
+ * Uo, Vo, Wo (time_integration.f90:256-258, :394-396, :526-531).
+ * dfp_apply_bc: apply_boundary_conditions of the channel on one rank
+ * (boundary_conditions.f90:186-260): periodic x and z ghost planes and the
+ * Robin (slip-length) wall rows with alpha_x(nx,2,nzg), alpha_y(nxg,2,nzg),
+ * alpha_z(nxg,2,nz) (global.f90:328; 0 = no slip).
+ * dfp_substep: one RK3 substage around the projection as time_integration.f90
+ * runs it without immersed boundaries, VOF or scalars (:256-304): update,
+ * boundary conditions, dfp_project with c = rk_coef(stage,stage) (P saved
+ * with DFP_SAVE_P), boundary conditions, all on the plan's stream.
+ * DFP_E_UNSUPPORTED for an inflow/outflow plan, whose x boundary conditions
+ * (inflow data, convective outflow) are not built here. */
+int dfp_rk_update(dfp_plan *plan, int stage, double *U, double *V, double *W, const double *Uo, const double *Vo,
+                  const double *Wo, const double *const *Fu, const double *const *Fv, const double *const *Fw,
+                  const double *coef, double dt, int flags);
+int dfp_apply_bc(dfp_plan *plan, double *U, double *V, double *W, const double *alpha_x, const double *alpha_y,
+                 const double *alpha_z, int flags);
+int dfp_substep(dfp_plan *plan, int stage, double *U, double *V, double *W, const double *Uo, const double *Vo,
+                const double *Wo, const double *const *Fu, const double *const *Fv, const double *const *Fw,
+                const double *coef, const double *alpha_x, const double *alpha_y, const double *alpha_z,
+                double *rhs_p, double *P, double dt, int flags);
+
 /* Refinement sweeps of a DFP_FP32_REFINE plan, 0..8 (no effect on an FP64
  * plan).  No counterpart in the reference. */
 int dfp_set_refine_sweeps(dfp_plan *plan, int sweeps);

@@ -19,6 +19,11 @@
  *                       boundary_conditions.f90:277-289, :591-602
  *   correction ........ projection.f90:551-604
  *   max divergence .... projection.f90:606-639
+ *   RK3 stage update .. time_integration.f90:256-258, :394-396, :526-531
+ *   channel BCs ....... apply_boundary_conditions, boundary_conditions.f90:186-260
+ *                       (x periodic :263-289, z periodic :580-602, y Robin
+ *                       walls :784-804; one rank: the halo exchanges
+ *                       :170-183, :809-883 do nothing)
  *   x inflow/outflow .. nxp = nxm (initialization.f90:331-338), DCT-IV
  *                       wavenumbers :436-449, REDFT11 forward
  *                       projection.f90:126-129 and inverse :333-339 with
@@ -752,4 +757,111 @@ double ora_max_divergence(const ora_plan *p, const double *U, const double *V, c
                 if (d > div) div = d;
             }
     return div;
+}
+
+
+/* -------------------------------------------------------------------------
+ * the step either side of the projection (channel case, one rank)
+ * ------------------------------------------------------------------------- */
+/* Interior arrays Fu(2:nx-1,2:nyg-1,2:nzg-1), Fv(2:nxg-1,2:ny-1,2:nzg-1),
+ * Fw(2:nxg-1,2:nyg-1,2:nz-1) (initialization.f90:662-670). */
+#define FI_(a, n1, n2, i, j, k) IDX3(a, (n1) - 2, (n2) - 2, (i) - 1, (j) - 1, (k) - 1)
+
+void ora_rk_update(const ora_plan *p, int stage, double *U, double *V, double *W, const double *Uo, const double *Vo,
+                   const double *Wo, const double *const *Fu, const double *const *Fv, const double *const *Fw,
+                   const double *c, double dt) {
+    /* time_integration.f90:256-258 (stage 1: Uo + dt*c11*Fu1), :394-396
+       (stage 2: Uo + dt*(c21*Fu1 + c22*Fu2)), :526-531 (stage 3:
+       Uo + dt*(c31*Fu1 + c32*Fu2 + c33*Fu3)), operation for operation */
+    for (int f = 0; f < 3; f++) {
+        double *F = f == 0 ? U : (f == 1 ? V : W);
+        const double *Fo = f == 0 ? Uo : (f == 1 ? Vo : Wo);
+        const double *const *R = f == 0 ? Fu : (f == 1 ? Fv : Fw);
+        const int n1 = f == 0 ? p->nx : p->nxg, n2 = f == 1 ? p->ny : p->nyg, n3 = f == 2 ? p->nz : p->nzg;
+        for (int k = 2; k <= n3 - 1; k++)
+            for (int j = 2; j <= n2 - 1; j++)
+                for (int i = 2; i <= n1 - 1; i++) {
+                    double v;
+                    if (stage == 1) {
+                        v = IDX3(Fo, n1, n2, i, j, k) + dt * c[0] * FI_(R[0], n1, n2, i, j, k);
+                    } else if (stage == 2) {
+                        v = IDX3(Fo, n1, n2, i, j, k) +
+                            dt * (c[0] * FI_(R[0], n1, n2, i, j, k) + c[1] * FI_(R[1], n1, n2, i, j, k));
+                    } else {
+                        v = IDX3(Fo, n1, n2, i, j, k) +
+                            dt * (c[0] * FI_(R[0], n1, n2, i, j, k) + c[1] * FI_(R[1], n1, n2, i, j, k) +
+                                  c[2] * FI_(R[2], n1, n2, i, j, k));
+                    }
+                    IDX3(F, n1, n2, i, j, k) = v;
+                }
+    }
+}
+
+/* boundary_conditions.f90:263-289 (id 1: x faces, else x centres), one rank */
+static void periodic_x(double *F, int n1, int n2, int n3, int faces) {
+    for (int k = 1; k <= n3; k++)
+        for (int j = 1; j <= n2; j++) {
+            if (faces) {
+                IDX3(F, n1, n2, 1, j, k) = IDX3(F, n1, n2, n1 - 1, j, k);
+                IDX3(F, n1, n2, n1, j, k) = IDX3(F, n1, n2, 2, j, k);
+            } else {
+                IDX3(F, n1, n2, 1, j, k) = IDX3(F, n1, n2, n1 - 2, j, k);
+                IDX3(F, n1, n2, n1 - 1, j, k) = IDX3(F, n1, n2, 2, j, k);
+                IDX3(F, n1, n2, n1, j, k) = IDX3(F, n1, n2, 3, j, k);
+            }
+        }
+}
+
+/* boundary_conditions.f90:580-602 (id 3: z faces, else z centres), one rank */
+static void periodic_z(double *F, int n1, int n2, int n3, int faces) {
+    for (int j = 1; j <= n2; j++)
+        for (int i = 1; i <= n1; i++) {
+            if (faces) {
+                IDX3(F, n1, n2, i, j, 1) = IDX3(F, n1, n2, i, j, n3 - 1);
+                IDX3(F, n1, n2, i, j, n3) = IDX3(F, n1, n2, i, j, 2);
+            } else {
+                IDX3(F, n1, n2, i, j, 1) = IDX3(F, n1, n2, i, j, n3 - 2);
+                IDX3(F, n1, n2, i, j, n3 - 1) = IDX3(F, n1, n2, i, j, 2);
+                IDX3(F, n1, n2, i, j, n3) = IDX3(F, n1, n2, i, j, 3);
+            }
+        }
+}
+
+/* boundary_conditions.f90:784-804: Robin (slip-length) walls; alpha(n1,2,n3)
+   holds the lower (1) and upper (2) wall coefficients */
+static void robin_y(const ora_plan *p, double *F, int n1, int n2, int n3, const double *alpha, int faces) {
+    for (int k = 1; k <= n3; k++)
+        for (int i = 1; i <= n1; i++) {
+            const double a1 = IDX3(alpha, n1, 2, i, 1, k), a2 = IDX3(alpha, n1, 2, i, 2, k);
+            if (faces) {
+                const double d1 = p->y[1] - p->y[0], d2 = p->y[p->ny - 1] - p->y[p->ny - 2];
+                IDX3(F, n1, n2, i, 1, k) = a1 * IDX3(F, n1, n2, i, 2, k) / d1 / (a1 / d1 + 1.0);
+                IDX3(F, n1, n2, i, n2, k) = a2 * IDX3(F, n1, n2, i, n2 - 1, k) / d2 / (a2 / d2 + 1.0);
+            } else {
+                const double g1 = p->yg[1] - p->yg[0], g2 = p->yg[p->nyg - 1] - p->yg[p->nyg - 2];
+                IDX3(F, n1, n2, i, 1, k) = (2.0 * a1 / g1 - 1.0) * IDX3(F, n1, n2, i, 2, k) / (2.0 * a1 / g1 + 1.0);
+                IDX3(F, n1, n2, i, n2, k) =
+                    (2.0 * a2 / g2 - 1.0) * IDX3(F, n1, n2, i, n2 - 1, k) / (2.0 * a2 / g2 + 1.0);
+            }
+        }
+}
+
+void ora_apply_bc(const ora_plan *p, double *U, double *V, double *W, const double *alpha_x, const double *alpha_y,
+                  const double *alpha_z) {
+    /* boundary_conditions.f90:186-260 for x_bc_type = 0, z_bc_type = 0,
+       y_bc_type = 1: periodic x, periodic z, Robin y, then the x and z wraps
+       again (:247-258) */
+    for (int pass = 0; pass < 2; pass++) {
+        periodic_x(U, p->nx, p->nyg, p->nzg, 1);
+        periodic_x(V, p->nxg, p->ny, p->nzg, 0);
+        periodic_x(W, p->nxg, p->nyg, p->nz, 0);
+        periodic_z(U, p->nx, p->nyg, p->nzg, 0);
+        periodic_z(V, p->nxg, p->ny, p->nzg, 0);
+        periodic_z(W, p->nxg, p->nyg, p->nz, 1);
+        if (pass == 0) {
+            robin_y(p, U, p->nx, p->nyg, p->nzg, alpha_x, 0);
+            robin_y(p, V, p->nxg, p->ny, p->nzg, alpha_y, 1);
+            robin_y(p, W, p->nxg, p->nyg, p->nz, alpha_z, 0);
+        }
+    }
 }

@@ -87,6 +87,17 @@ int ora_zgtsv(int n, double *dl, double *d, double *du, double *b);
  * plan_many semantics (interleaved complex doubles) */
 int ora_fft_many(int n, int howmany, double *data, long stride, long dist, int sign);
 
+/* RK3 stage update (time_integration.f90:256-258, :394-396, :526-531): the
+ * interior of U, V, W from Uo, Vo, Wo and the right-hand sides Fu[s], Fv[s],
+ * Fw[s] (interior-shaped arrays, s < stage) with c = rk_coef(stage,1:stage) */
+void ora_rk_update(const ora_plan *p, int stage, double *U, double *V, double *W, const double *Uo, const double *Vo,
+                   const double *Wo, const double *const *Fu, const double *const *Fv, const double *const *Fw,
+                   const double *c, double dt);
+/* apply_boundary_conditions of the channel (boundary_conditions.f90:186-260):
+ * alpha_x(nx,2,nzg), alpha_y(nxg,2,nzg), alpha_z(nxg,2,nz) */
+void ora_apply_bc(const ora_plan *p, double *U, double *V, double *W, const double *alpha_x, const double *alpha_y,
+                  const double *alpha_z);
+
 /* threads the plan's loops use (OpenMP); returns the count in effect */
 int ora_threads(const ora_plan *p);
 

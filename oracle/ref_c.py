@@ -55,6 +55,8 @@ def lib():
         L.ora_set_outflow.restype = ctypes.c_int
         L.ora_phib.argtypes = [vp, vp]
         L.ora_dct4_many.argtypes = [ctypes.c_int, ctypes.c_int, vp]
+        L.ora_rk_update.argtypes = [vp, ctypes.c_int, vp, vp, vp, vp, vp, vp, vp, vp, vp, vp, ctypes.c_double]
+        L.ora_apply_bc.argtypes = [vp, vp, vp, vp, vp, vp, vp]
         _lib = L
     return _lib
 
@@ -159,6 +161,34 @@ class CProjection:
     def check_divergence(self, U, V, W):
         self._chk(U, V, W)
         return lib().ora_max_divergence(self._p, _ptr(U), _ptr(V), _ptr(W))
+
+    def rk_update(self, stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef, dt):
+        """time_integration.f90:256-258/:394-396/:526-531; Fu, Fv, Fw: lists of
+        interior-shaped arrays (stage entries), coef = rk_coef(stage,1:stage)."""
+        self._chk(U, V, W)
+        self._chk(Uo, Vo, Wo)
+        g = self.g
+        shapes = ((g.nx - 2, g.nyg - 2, g.nzg - 2), (g.nxg - 2, g.ny - 2, g.nzg - 2), (g.nxg - 2, g.nyg - 2, g.nz - 2))
+        arrs = []
+        for F, sh, name in zip((Fu, Fv, Fw), shapes, ("Fu", "Fv", "Fw")):
+            if len(F) < stage:
+                raise ValueError(f"{name}: {stage} arrays needed")
+            for a in F[:stage]:
+                _check_f64_fortran(a, sh, name)
+            arrs.append((ctypes.c_void_p * 3)(*[_ptr(a) for a in F[:stage]]))
+        c = (ctypes.c_double * 3)(*[float(v) for v in coef[:stage]])
+        lib().ora_rk_update(self._p, int(stage), _ptr(U), _ptr(V), _ptr(W), _ptr(Uo), _ptr(Vo), _ptr(Wo),
+                            ctypes.cast(arrs[0], ctypes.c_void_p), ctypes.cast(arrs[1], ctypes.c_void_p),
+                            ctypes.cast(arrs[2], ctypes.c_void_p), ctypes.cast(c, ctypes.c_void_p), float(dt))
+
+    def apply_boundary_conditions(self, U, V, W, alpha_x, alpha_y, alpha_z):
+        """boundary_conditions.f90:186-260 of the channel, one rank."""
+        self._chk(U, V, W)
+        g = self.g
+        _check_f64_fortran(alpha_x, (g.nx, 2, g.nzg), "alpha_x")
+        _check_f64_fortran(alpha_y, (g.nxg, 2, g.nzg), "alpha_y")
+        _check_f64_fortran(alpha_z, (g.nxg, 2, g.nz), "alpha_z")
+        lib().ora_apply_bc(self._p, _ptr(U), _ptr(V), _ptr(W), _ptr(alpha_x), _ptr(alpha_y), _ptr(alpha_z))
 
 
 def zgtsv(dl, d, du, b):

@@ -522,6 +522,81 @@ class Projection:
         return float(np.max(np.abs(div))) if div.size else 0.0
 
 
+# ---------------------------------------------------------------------------
+# the step either side of the projection (channel case, one rank)
+# ---------------------------------------------------------------------------
+def rk_update(g: ChannelGrid, stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef, dt):
+    """time_integration.f90:256-258 (stage 1), :394-396 (2), :526-531 (3):
+    the interiors U(2:nx-1,2:nyg-1,2:nzg-1), V(2:nxg-1,2:ny-1,2:nzg-1),
+    W(2:nxg-1,2:nyg-1,2:nz-1) from the stage-start velocity and the
+    interior-shaped right-hand sides Fu[s], Fv[s], Fw[s] (s < stage) with
+    coef = rk_coef(stage, 1:stage), in the reference's operation order."""
+    for F, Fo, R in ((U, Uo, Fu), (V, Vo, Fv), (W, Wo, Fw)):
+        n1, n2, n3 = F.shape
+        sl = (slice(1, n1 - 1), slice(1, n2 - 1), slice(1, n3 - 1))
+        if stage == 1:
+            F[sl] = Fo[sl] + dt * coef[0] * R[0]
+        elif stage == 2:
+            F[sl] = Fo[sl] + dt * (coef[0] * R[0] + coef[1] * R[1])
+        else:
+            F[sl] = Fo[sl] + dt * (coef[0] * R[0] + coef[1] * R[1] + coef[2] * R[2])
+
+
+def _periodic_x(F, faces):
+    # boundary_conditions.f90:275-284 (one rank): id 1 x faces, else x centres
+    n = F.shape[0]
+    if faces:
+        F[0] = F[n - 2]
+        F[n - 1] = F[1]
+    else:
+        F[0] = F[n - 3]
+        F[n - 2] = F[1]
+        F[n - 1] = F[2]
+
+
+def _periodic_z(F, faces):
+    # boundary_conditions.f90:591-600 (one rank): id 3 z faces, else z centres
+    n = F.shape[2]
+    if faces:
+        F[:, :, 0] = F[:, :, n - 2]
+        F[:, :, n - 1] = F[:, :, 1]
+    else:
+        F[:, :, 0] = F[:, :, n - 3]
+        F[:, :, n - 2] = F[:, :, 1]
+        F[:, :, n - 1] = F[:, :, 2]
+
+
+def _robin_y(g, F, alpha, faces):
+    # boundary_conditions.f90:784-804: slip length alpha(:,1,:) at the lower,
+    # alpha(:,2,:) at the upper wall
+    a1, a2 = alpha[:, 0, :], alpha[:, 1, :]
+    n = F.shape[1]
+    if faces:
+        d1, d2 = g.y[1] - g.y[0], g.y[-1] - g.y[-2]
+        F[:, 0, :] = a1 * F[:, 1, :] / d1 / (a1 / d1 + 1.0)
+        F[:, n - 1, :] = a2 * F[:, n - 2, :] / d2 / (a2 / d2 + 1.0)
+    else:
+        g1, g2 = g.yg[1] - g.yg[0], g.yg[-1] - g.yg[-2]
+        F[:, 0, :] = (2.0 * a1 / g1 - 1.0) * F[:, 1, :] / (2.0 * a1 / g1 + 1.0)
+        F[:, n - 1, :] = (2.0 * a2 / g2 - 1.0) * F[:, n - 2, :] / (2.0 * a2 / g2 + 1.0)
+
+
+def apply_boundary_conditions(g: ChannelGrid, U, V, W, alpha_x, alpha_y, alpha_z):
+    """boundary_conditions.f90:186-260 for the channel (x_bc_type = 0,
+    z_bc_type = 0, y_bc_type = 1) on one rank, where the halo exchanges
+    (:170-183) exchange nothing: periodic x (U faces, V and W centres),
+    periodic z (U and V centres, W faces), Robin y walls (U with alpha_x, V
+    faces with alpha_y, W with alpha_z), then the x and z wraps again
+    (:247-258)."""
+    for first in (True, False):
+        _periodic_x(U, True); _periodic_x(V, False); _periodic_x(W, False)
+        _periodic_z(U, False); _periodic_z(V, False); _periodic_z(W, True)
+        if first:
+            _robin_y(g, U, alpha_x, False)
+            _robin_y(g, V, alpha_y, True)
+            _robin_y(g, W, alpha_z, False)
+
+
 def new_fields(g: ChannelGrid):
     """Zeroed U, V, W, P (Fortran order) and rhs_p, shaped as initialization.f90:204-220."""
     U = np.zeros(g.shape_u, order="F")
