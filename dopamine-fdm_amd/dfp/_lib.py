@@ -5,7 +5,9 @@ import ctypes
 import os
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.normpath(os.path.join(_HERE, "..", "lib", "libdfp.so"))
+# DFP_LIB selects another build of the same library (the host-sanitizer build
+# that the CPU tests load); by default the in-tree libdfp.so
+LIB_PATH = os.environ.get("DFP_LIB") or os.path.normpath(os.path.join(_HERE, "..", "lib", "libdfp.so"))
 HEADER_PATH = os.path.normpath(os.path.join(_HERE, "..", "..", "include", "dfp.h"))
 
 DFP_FP64 = 0
