@@ -19,6 +19,12 @@
  *                       boundary_conditions.f90:277-289, :591-602
  *   correction ........ projection.f90:551-604
  *   max divergence .... projection.f90:606-639
+ *   periodic y ........ y_bc_type = 0 with periodic x and z: kyy
+ *                       (initialization.f90:467-491, nyp = nym-1, dy =
+ *                       y(2)-y(1)), y C2C FFT per z slab, division by
+ *                       kxx+kyy+kzz with the (0,0,0) mode set to zero, inverse
+ *                       y FFT /nyp (projection.f90:246-273), duplicate row
+ *                       rhs_p(:,nyg-1,:) = rhs_p(:,2,:) (:350-354)
  *   RK3 stage update .. time_integration.f90:256-258, :394-396, :526-531
  *   channel BCs ....... apply_boundary_conditions, boundary_conditions.f90:186-260
  *                       (x periodic :263-289, z periodic :580-602, y Robin
@@ -324,6 +330,9 @@ void ora_channel_grid_x(int nxp, int nym, int nzp, double Lx_i, double Ly_i, dou
  * ------------------------------------------------------------------------- */
 struct ora_plan {
     int xbc;  /* x_bc_type */
+    int ybc;  /* y_bc_type */
+    double *kyy;  /* periodic y: nym-1 modified wavenumbers */
+    fplan fy;     /* periodic y: length nym-1 */
     double beta, outflow_Uc;
     double *Pb, *phib; /* (nyg-2) x (nzg-2) */
     int nxp, nym, nzp;
@@ -425,8 +434,12 @@ static void wavenumbers(int n, double d, double *k) {
 ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
     if (g->nxp < 1 || g->nym < 1 || g->nzp < 1) return NULL;
     if (g->x_bc_type != 0 && g->x_bc_type != 1) return NULL;
+    if (g->y_bc_type != 0 && g->y_bc_type != 1) return NULL;
+    /* periodic y needs periodic x (initialization.f90:96-97) and two rows */
+    if (g->y_bc_type == 0 && (g->x_bc_type != 0 || g->nym < 2)) return NULL;
     ora_plan *p = (ora_plan *)calloc(1, sizeof(ora_plan));
     p->xbc = g->x_bc_type;
+    p->ybc = g->y_bc_type;
     p->nxp = g->nxp; p->nym = g->nym; p->nzp = g->nzp;
     p->nx = g->nxp + (p->xbc ? 1 : 2); p->ny = g->nym + 1; p->nz = g->nzp + 2;
     p->nxg = p->nx + 1; p->nyg = g->nym + 2; p->nzg = g->nzp + 3;
@@ -475,6 +488,12 @@ ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
 #undef YG
     fplan_init(&p->fx, p->nxp);
     fplan_init(&p->fz, p->nzp);
+    if (!p->ybc) {
+        /* initialization.f90:481-489, dy = y(2)-y(1) on the uniform grid */
+        p->kyy = (double *)malloc(sizeof(double) * (size_t)(p->nym - 1));
+        wavenumbers(p->nym - 1, p->y[1] - p->y[0], p->kyy);
+        fplan_init(&p->fy, p->nym - 1);
+    }
     p->work = (cplx *)malloc(sizeof(cplx) * (size_t)p->nxp * p->nym * p->nzp);
 #ifdef _OPENMP
     p->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads();
@@ -491,6 +510,7 @@ void ora_plan_destroy(ora_plan *p) {
     free(p->dyy_l); free(p->dyy_d); free(p->dyy_u);
     free(p->Pb); free(p->phib);
     fplan_free(&p->fx); fplan_free(&p->fz);
+    if (!p->ybc) { free(p->kyy); fplan_free(&p->fy); }
     free(p->work);
     free(p);
 }
@@ -640,8 +660,39 @@ int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
     }
     /* :134-136 C2C forward along z, stride nxp*nym, batch nxp*nym */
     fft_many(&p->fz, wk, (long)nxp * nym, (long)nxp * nym, 1, -1, p->nthreads);
-    /* :277-297 one Zgtsv per (i,k) mode */
     int info_all = 0;
+    if (!p->ybc) {
+        /* periodic y (:246-273): per z slab, C2C forward along y over the nyp =
+           nym-1 independent rows (stride nxp, batch nxp), divide by
+           kxx+kyy+kzz with the (0,0,0) mode set to zero, backward, /nyp */
+        const int nyp = nym - 1;
+        for (int k = 0; k < nzp; k++) fft_many(&p->fy, wk + (size_t)k * nym * nxp, nxp, nxp, 1, -1, p->nthreads);
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+        for (int k = 0; k < nzp; k++)
+            for (int j = 0; j < nyp; j++)
+                for (int i = 0; i < nxp; i++) {
+                    cplx *v = wk + ((size_t)k * nym + j) * nxp + i;
+                    const double ws = p->kxx[i] + p->kyy[j] + p->kzz[k];
+                    if (i == 0 && j == 0 && k == 0) {
+                        v->re = 0.0;
+                        v->im = 0.0;
+                    } else {
+                        v->re = v->re / ws;
+                        v->im = v->im / ws;
+                    }
+                }
+        for (int k = 0; k < nzp; k++) fft_many(&p->fy, wk + (size_t)k * nym * nxp, nxp, nxp, 1, +1, p->nthreads);
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+        for (int k = 0; k < nzp; k++)
+            for (int j = 0; j < nyp; j++)
+                for (int i = 0; i < nxp; i++) {
+                    cplx *v = wk + ((size_t)k * nym + j) * nxp + i;
+                    v->re = v->re / (double)nyp;
+                    v->im = v->im / (double)nyp;
+                }
+        goto inverse;
+    }
+    /* :277-297 one Zgtsv per (i,k) mode */
 #pragma omp parallel num_threads(p->nthreads)
     {
         double *DL = (double *)malloc(sizeof(double) * (size_t)(nym > 1 ? nym - 1 : 1));
@@ -670,6 +721,7 @@ int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
         }
         free(DL); free(DU); free(D); free(b);
     }
+inverse:
     /* :308-310 inverse z */
     fft_many(&p->fz, wk, (long)nxp * nym, (long)nxp * nym, 1, +1, p->nthreads);
     if (p->xbc) {
@@ -694,6 +746,11 @@ int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
                 for (int i = 0; i < nxp; i++) R_(i + 2, j + 2, k + 2) = wk[((size_t)k * nym + j) * nxp + i].re / norm;
     }
     apply_periodic_xz_pressure(p, rhs_p);
+    if (!p->ybc) {
+        /* :350-354 periodic y: the duplicate row, over the whole x-z extent */
+        for (int k = 2; k <= p->nzg; k++)
+            for (int i = 2; i <= p->nxg; i++) R_(i, p->nyg - 1, k) = R_(i, 2, k);
+    }
     /* :357-358 single rank: no interior halo neighbours */
     if (P) {
         /* :377 */

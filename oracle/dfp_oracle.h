@@ -33,6 +33,7 @@ typedef struct ora_grid {
     const double *yg;     /* centres yg(1:nyg) */
     const double *zg;     /* centres zg(1:nzg) */
     int x_bc_type;        /* 0 periodic; 1 inflow/outflow (nx = nxp+1, nxg = nxp+2) */
+    int y_bc_type;        /* 1 walls; 0 periodic (uniform y, nym-1 independent rows) */
 } ora_grid;
 
 typedef struct ora_plan ora_plan;

@@ -100,6 +100,7 @@ class ChannelGrid:
     alpha: float
     grid_type: int = 2
     x_bc_type: int = 0   # 0 periodic, 1 inflow/outflow (global.f90:185-186)
+    y_bc_type: int = 1   # 1 walls, 0 periodic (uniform y only, initialization.f90:128-129)
     x: np.ndarray = field(init=False)
     y: np.ndarray = field(init=False)
     z: np.ndarray = field(init=False)
@@ -110,6 +111,9 @@ class ChannelGrid:
     dz: float = field(init=False)
 
     def __post_init__(self):
+        if self.y_bc_type == 0 and (self.grid_type != 1 or self.x_bc_type != 0 or self.nym < 2):
+            # initialization.f90:96-97, :128-129
+            raise ValueError("periodic y needs a uniform y grid, a periodic x and nym >= 2")
         nx, ny, nz = self.nx, self.ny, self.nz
         # genGridandIC.f90:23-25, :38-40  x(i) = Lx*(i-1)/(nx-1)
         self.x = np.array([self.Lx_i * float(i - 1) / float(nx - 1) for i in range(1, nx + 1)])
@@ -193,6 +197,13 @@ def _centres_with_ghosts(f: np.ndarray) -> np.ndarray:
     g[0] = fm[0] - 2.0 * (fm[0] - f[0])
     g[n] = fm[n - 2] + 2.0 * (f[n - 1] - fm[n - 2])
     return g
+
+
+def periodic_y_grid(nxp, nym, nzp, Lx=2.0 * PI, Ly=2.0, Lz=PI):
+    """Triply periodic box (y_bc_type = 0, uniform y): the last of the nym
+    y cells duplicates the first, so the namelist length spans nym cells of
+    Ly/(nym-1) and the period is Ly."""
+    return ChannelGrid(nxp, nym, nzp, Lx * (nxp + 1) / nxp, Ly * nym / (nym - 1), Lz * (nzp + 1) / nzp, 0.0, 1, 0, 0)
 
 
 def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * PI, Lz=PI, Ly=2.0, grid_type=2, x_bc_type=0):
@@ -344,6 +355,8 @@ class Projection:
         self.kxx = wavenumbers(g.nxp, g.dx) if g.x_bc_type == 0 else wavenumbers_dct(g.nxp, g.dx)
         self.kzz = wavenumbers(g.nzp, g.dz)
         self.Dyy = dyy_matrix(g.y, g.yg)
+        # periodic y: initialization.f90:481-489 (nyp = nym-1, dy = y(2)-y(1))
+        self.kyy = wavenumbers(g.nym - 1, float(g.y[1] - g.y[0])) if getattr(g, "y_bc_type", 1) == 0 else None
         # outlet backflow stabilisation (global.f90:188-197): off unless beta > 0
         self.beta = 0.0
         self.outflow_Uc = 0.0
@@ -410,6 +423,18 @@ class Projection:
             c = scipy.fft.dct(ry, type=4, axis=0).astype(np.complex128)
         # :134-136 complex FFT along z
         c = np.fft.fft(c, axis=2)
+        if self.kyy is not None:
+            # :246-273 periodic y: FFT over the nyp = nym-1 independent rows, divide
+            # by kxx+kyy+kzz, the (0,0,0) mode set to zero, inverse FFT, /nyp
+            nyp = nym - 1
+            cy = np.fft.fft(c[:, :nyp, :], axis=1)
+            lam3 = self.kxx[:, None, None] + self.kyy[None, :, None] + self.kzz[None, None, :]
+            lam3[0, 0, 0] = 1.0
+            cy = cy / lam3
+            cy[0, 0, 0] = 0.0
+            c = c.copy()
+            c[:, :nyp, :] = np.fft.ifft(cy, axis=1) * nyp / float(nyp)
+            return self._inverse_xz(c, rhs_p, P, dt, rk_coef33)
         # :277-297 one Zgtsv per (i,k) mode
         lam = self.kxx[:, None] + self.kzz[None, :]          # wavenum_sum, (nxp, nzp)
         diag = np.diag(self.Dyy)                              # Dyy(j,j)
@@ -427,6 +452,12 @@ class Projection:
         if np.any(info != 0):
             raise FloatingPointError("zgtsv reported a zero pivot")
         c = np.transpose(X.reshape(nxp, nzp, nym, order="F"), (0, 2, 1))
+        return self._inverse_xz(c, rhs_p, P, dt, rk_coef33)
+
+    def _inverse_xz(self, c, rhs_p, P, dt, rk_coef33):
+        g = self.g
+        nxp, nym, nzp = g.nxp, g.nym, g.nzp
+        xdct = g.x_bc_type == 1
         # :308-310 inverse z
         c = np.fft.ifft(c, axis=2) * nzp
         if not xdct:
@@ -439,6 +470,9 @@ class Projection:
         # :342
         rhs_p[0:nxp, 0:nym, 0:nzp] = phi
         self.apply_periodic_xz_pressure(rhs_p)
+        if self.kyy is not None:
+            # :350-354 periodic y: the duplicate row rhs_p(:,nyg-1,:) = rhs_p(:,2,:)
+            rhs_p[:, nym - 1, :] = rhs_p[:, 0, :]
         # :357-358 single rank: no interior halo neighbours, nothing to exchange
         if P is not None:
             # :377 P = rhs_p/(dt*rk_coef(3,3)) on 2:nxg-1, 2:nyg-1, 2:nzg-1
