@@ -132,7 +132,8 @@ void dyy(const std::vector<double>& y, const std::vector<double>& yg, std::vecto
 // kernels with G rows per group.
 struct Shape {
     int R = 1, T = 1, Ti = 1, G = 1, fused = 0;  // T, Ti: columns of the forward, inverse x kernels
-    FftDesc fx{}, fz{};
+    int gy = 1;                                  // periodic y: columns per y-transform call
+    FftDesc fx{}, fz{}, fy{};
 };
 
 constexpr int kRadices[] = {1, 2, 3, 4, 6, 8, 9, 12, 16};
@@ -184,8 +185,13 @@ int choose_shape(const dfp_grid* g, int f32, size_t lds_limit, const dfp_tuning*
         for (int r : kRadices) known = known || r == want_R;
         if (!known || g->nzp % want_R != 0) return DFP_E_ARG;
     }
+    // the periodic y (y FFT in place of the wall solve) has the fused kernel only
+    const bool py = g->y_bc_type == 0;
+    FftDesc fy{};
+    if (py && (want_f == 1 || want_G > 0)) return DFP_E_UNSUPPORTED;
+    if (py && !factor(g->nym - 1, fy)) return DFP_E_UNSUPPORTED;
     const bool allow_fused = want_f == 2 || (want_f == 0 && want_G == 0);
-    const bool allow_split = want_f != 2;
+    const bool allow_split = want_f != 2 && !py;
     std::vector<Shape> fused, split;
     for (int R : kRadices) {
         if (g->nzp % R != 0 || (want_R > 0 && R != want_R)) continue;
@@ -196,7 +202,17 @@ int choose_shape(const dfp_grid* g, int f32, size_t lds_limit, const dfp_tuning*
         sh.T = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 0, dct);
         sh.Ti = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 1, dct);
         if (sh.T == 0 || sh.Ti == 0 || sh.fz.n > 256) continue;
-        if (allow_fused && zf_lds_bytes(g->nym, sh.fz.n, twiddle_count(sh.fz), f32) <= lds_limit) {
+        if (py) {
+            const int gy = zf_py_columns(fy, sh.fz.n);
+            if (gy > 0 && zf_py_lds_bytes(g->nym, sh.fz.n, twiddle_count(sh.fz), twiddle_count(fy)) <= lds_limit) {
+                Shape f = sh;
+                f.fused = 1;
+                f.G = zf_rows_per_call(f.fz);
+                f.fy = fy;
+                f.gy = gy;
+                fused.push_back(f);
+            }
+        } else if (allow_fused && zf_lds_bytes(g->nym, sh.fz.n, twiddle_count(sh.fz), f32) <= lds_limit) {
             Shape f = sh;
             f.fused = 1;
             f.G = zf_rows_per_call(f.fz);
@@ -328,6 +344,7 @@ int check_flags(dfp_plan* p, int flags, int extra = 0) {
 
 // z transforms and y solves of the spectrum: the fused kernel or the split pair
 hipError_t z_solve(dfp_plan* p, int f32) {
+    if (p->geo.ybc == 0) return launch_zfused_py(p);
     if (p->zfused) {
         if (f32) return launch_zfused<float2>(p, make_zpar<float2>(p, 1));
         return launch_zfused<double2>(p, make_zpar<double2>(p, 0));
@@ -462,7 +479,11 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     *out = nullptr;
     if (!g || !g->y || !g->z || !g->yg || !g->zg) return DFP_E_ARG;
     if (g->x_bc_type != 0 && g->x_bc_type != 1) return DFP_E_UNSUPPORTED;
-    if (g->y_bc_type != 1 || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
+    if ((g->y_bc_type != 0 && g->y_bc_type != 1) || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
+    // periodic y (initialization.f90:96-97, :128-129): periodic x, a uniform
+    // y grid and nym-1 = 2^a 3^b independent rows; solved in FP64
+    if (g->y_bc_type == 0 && (g->x_bc_type != 0 || precision != DFP_FP64 || g->nym < 2))
+        return DFP_E_UNSUPPORTED;
     if (precision != DFP_FP64 && precision != DFP_FP32_REFINE) return DFP_E_ARG;
     // the inflow/outflow x is solved in FP64 only
     if (g->x_bc_type == 1 && precision != DFP_FP64) return DFP_E_UNSUPPORTED;
@@ -485,6 +506,14 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
         // FFT lengths 2^a 3^b, checked before the device is touched
         FftDesc probe;
         if (!factor(g->nxp / 2, probe) || !factor(g->nzp, probe)) return bail(DFP_E_UNSUPPORTED);
+        if (g->y_bc_type == 0) {
+            if (!factor(g->nym - 1, probe)) return bail(DFP_E_UNSUPPORTED);
+            // the reference takes dy = y(2) - y(1) for every row (:481-489)
+            const double dy = g->y[1] - g->y[0];
+            if (!(dy > 0.0)) return bail(DFP_E_ARG);
+            for (int j = 1; j < g->nym + 1; ++j)
+                if (std::fabs((g->y[j] - g->y[j - 1]) - dy) > 1e-9 * dy) return bail(DFP_E_UNSUPPORTED);
+        }
     }
     if (hipSetDevice(device) != hipSuccess) return bail(DFP_E_HIP);
     int lds_attr = 0;
@@ -500,11 +529,14 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     p->tk_xi = sh.Ti;
     p->g_zs = sh.G;
     p->zfused = sh.fused;
+    p->fy = sh.fy;
+    p->gy = sh.gy;
     if (!sh.fused) z_two_pass(sh.fz.n, p->zn1, p->zn2);
     if (hipSetDevice(device) != hipSuccess) return bail(DFP_E_HIP);
     Geo& G = p->geo;
     const bool dct = g->x_bc_type == 1;
     G.xbc = g->x_bc_type;
+    G.ybc = g->y_bc_type;
     G.nxp = g->nxp; G.nym = g->nym; G.nzp = g->nzp;
     // the periodic x keeps the R2C half of its modes (and drops the duplicate
     // cell, nxp = nxm - 1); the inflow/outflow x keeps all nxm = nxp cells and
@@ -533,6 +565,10 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     else wavenumbers(G.nxp, g->dx, p->hkxx);
     wavenumbers(G.nzp, g->dz, p->hkzz);
     dyy(p->hy, p->hyg, p->hdl, p->hdd, p->hdu);
+    // periodic y: the modified wavenumbers of the nym-1 independent rows,
+    // spacing y(2) - y(1) (initialization.f90:481-489)
+    std::vector<double> hkyy;
+    if (G.ybc == 0) wavenumbers(G.nym - 1, p->hy[1] - p->hy[0], hkyy);
 
     std::vector<double> tab;
     auto put = [&](const std::vector<double>& v) {
@@ -545,6 +581,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     const size_t o_dyf = put(inv_dyf), o_dzf = put(inv_dzf), o_dyg = put(inv_dyg), o_dzg = put(inv_dzg);
     const size_t o_kxx = put(kxx_half), o_kzz = put(p->hkzz);
     const size_t o_dl = put(p->hdl), o_dd = put(p->hdd), o_du = put(p->hdu);
+    const size_t o_kyy = hkyy.empty() ? (size_t)-1 : put(hkyy);
     const size_t o_x = p->hx.empty() ? (size_t)-1 : put(p->hx);
     const size_t o_y = put(p->hy), o_z = put(p->hz), o_yg = put(p->hyg);
     if (hipMalloc(&p->d_tables, sizeof(double) * tab.size()) != hipSuccess) return bail(DFP_E_NOMEM);
@@ -556,6 +593,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     G.y = T + o_y; G.z = T + o_z; G.yg = T + o_yg;
     p->ops.kxx = T + o_kxx; p->ops.kzz = T + o_kzz;
     p->ops.dl = T + o_dl; p->ops.dd = T + o_dd; p->ops.du = T + o_du;
+    p->ops.kyy = o_kyy == (size_t)-1 ? nullptr : T + o_kyy;
 
     // twiddles, in the order the x kernels stage them into LDS with one
     // copy: x stages, R2C/C2R packing (M+1) or the DCT-IV pre-twiddles
@@ -582,6 +620,11 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     for (int st = 0; st < p->fz.nst; ++st) p->fz.twoff[st] -= (int)o_tz;
     const size_t o_t2 = tw.size();
     for (int e = 0; e < p->fz.n; ++e) tw.push_back(expi(e, p->fz.n));
+    const size_t o_ty = tw.size();
+    if (G.ybc == 0) {
+        stage_twiddles(p->fy, tw);
+        for (int st = 0; st < p->fy.nst; ++st) p->fy.twoff[st] -= (int)o_ty;
+    }
     if (hipMalloc(&p->d_tw, sizeof(double2) * tw.size()) != hipSuccess) return bail(DFP_E_NOMEM);
     if (hipMemcpy(p->d_tw, tw.data(), sizeof(double2) * tw.size(), hipMemcpyHostToDevice) != hipSuccess)
         return bail(DFP_E_HIP);
@@ -590,6 +633,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     p->tw_r2c = p->d_tw + o_r2c;
     p->tw_zs = p->d_tw + o_zs;
     p->tw_z2 = p->d_tw + o_t2;
+    p->tw_y = p->d_tw + o_ty;
     if (precision == DFP_FP32_REFINE) {
         std::vector<float2> tw32(tw.size());
         for (size_t e = 0; e < tw.size(); ++e) tw32[e] = make_float2((float)tw[e].x, (float)tw[e].y);
@@ -603,9 +647,11 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
 
     const size_t H = (size_t)G.nxh * G.nym * G.nzp;
     if (hipMalloc(&p->d_spec, sizeof(double2) * H) != hipSuccess) return bail(DFP_E_NOMEM);
-    const size_t nlu = (size_t)G.nxh * G.nym * (G.nzp / 2 + 1);
-    if (hipMalloc(&p->d_lu, sizeof(double) * nlu) != hipSuccess) return bail(DFP_E_NOMEM);
-    if (launch_lu_table(p) != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) return bail(DFP_E_HIP);
+    if (G.ybc == 1) {  // pivots of the wall solve
+        const size_t nlu = (size_t)G.nxh * G.nym * (G.nzp / 2 + 1);
+        if (hipMalloc(&p->d_lu, sizeof(double) * nlu) != hipSuccess) return bail(DFP_E_NOMEM);
+        if (launch_lu_table(p) != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) return bail(DFP_E_HIP);
+    }
     p->red_len = (int)((long)ceil_div((long)(G.nxg - 2) * G.nym, 256) * (G.nzg - 2));
     if (hipMalloc(&p->d_red, sizeof(double) * (p->red_len + 8)) != hipSuccess) return bail(DFP_E_NOMEM);
 
@@ -786,8 +832,9 @@ int dfp_apply_bc(dfp_plan* p, double* U, double* V, double* W, const double* alp
     if (!p) return DFP_E_ARG;
     int rc = step_checks(p, flags & ~DFP_SAVE_P);
     if (rc) return rc;
-    if (!U || !V || !W || !alpha_x || !alpha_y || !alpha_z)
-        return fail(p, DFP_E_ARG, "U, V, W and alpha_x, alpha_y, alpha_z are required");
+    if (!U || !V || !W) return fail(p, DFP_E_ARG, "U, V and W are required");
+    if (p->geo.ybc == 1 && (!alpha_x || !alpha_y || !alpha_z))
+        return fail(p, DFP_E_ARG, "alpha_x, alpha_y and alpha_z are required with y walls");
     HIPCHK(p, launch_apply_bc(p, U, V, W, alpha_x, alpha_y, alpha_z));
     return finish(p, flags);
 }
@@ -800,8 +847,9 @@ int dfp_substep(dfp_plan* p, int stage, double* U, double* V, double* W, const d
     int rc = step_checks(p, flags);
     if (rc) return rc;
     if ((rc = rk_args(p, stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef, dt))) return rc;
-    if (!alpha_x || !alpha_y || !alpha_z || !rhs_p)
-        return fail(p, DFP_E_ARG, "alpha_x, alpha_y, alpha_z and rhs_p are required");
+    if (!rhs_p) return fail(p, DFP_E_ARG, "rhs_p is required");
+    if (p->geo.ybc == 1 && (!alpha_x || !alpha_y || !alpha_z))
+        return fail(p, DFP_E_ARG, "alpha_x, alpha_y and alpha_z are required with y walls");
     const int save_p = (flags & DFP_SAVE_P) ? 1 : 0;
     const double c = coef[stage - 1];
     if (save_p && (!P || !(dt * c != 0.0) || !std::isfinite(dt * c)))

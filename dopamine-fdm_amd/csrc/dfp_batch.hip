@@ -299,8 +299,9 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         dfp_plan* p = plans[i];
         if (!p) return bfail(b, DFP_E_ARG, "case %d: null plan", i);
         if (p->device != b->device) return bfail(b, DFP_E_ARG, "case %d: plan on device %d", i, p->device);
-        if (p->geo.xbc != 0)
-            return bfail(b, DFP_E_UNSUPPORTED, "case %d: the batch executor takes periodic-x plans only", i);
+        if (p->geo.xbc != 0 || p->geo.ybc != 1)
+            return bfail(b, DFP_E_UNSUPPORTED, "case %d: the batch executor takes channel plans (periodic x, y walls) only",
+                         i);
         for (int k = 0; k < i; ++k)
             if (plans[k] == p) return bfail(b, DFP_E_ARG, "cases %d and %d share a plan", k, i);
         if (!rhs_p[i] || (!solve && (!U[i] || !V[i] || !W[i]))) return bfail(b, DFP_E_ARG, "case %d: null field", i);

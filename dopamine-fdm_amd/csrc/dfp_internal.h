@@ -55,6 +55,7 @@ struct FftDesc {
 struct Geo {
     int nxp, nym, nzp, nxh;   // nxh: x planes of the spectrum (nxp/2+1 periodic, nxp inflow/outflow)
     int xbc;                  // x_bc_type: 0 periodic, 1 inflow/outflow (DCT-IV in x)
+    int ybc;                  // y_bc_type: 1 walls (tridiagonal y solve), 0 periodic (y FFT, uniform y)
     int nx, ny, nz, nxg, nyg, nzg;
     double inv_dx;
     const double* inv_dyf;
@@ -75,6 +76,7 @@ struct Ops {
     const double* du;   // nym-1
     const double* kxx;  // nxh: kx = 0..nxp/2 (periodic) or the nxp DCT-IV modes
     const double* kzz;  // nzp
+    const double* kyy;  // nym-1 (periodic y), else null
 };
 
 // Inputs of the forward x-transform kernel beyond the spectrum.
@@ -132,6 +134,23 @@ struct ZPar {
     FastDiv div_n;
 };
 
+// Per-case parameters of the periodic-y z/y kernel (k_zfused.hip): the fused
+// kernel's record (no y solve tables) plus the y transform over the nym-1
+// independent rows and the three modified-wavenumber tables.
+template <typename C>
+struct ZyPar {
+    ZPar<C> z;
+    FftDesc fy;          // y FFT of length nym-1
+    const C* twy;        // its stage twiddles
+    const double* kxx;   // nxh
+    const double* kyy;   // nym-1
+    const double* kzz;   // nzp
+    int ntwy;            // twiddle count of fy
+    int gy;              // columns per y-transform call (divides nzp/R)
+    FastDiv div_gy;
+    double inv_ny;       // 1/(nym-1)
+};
+
 // Per-case parameters of the stencil kernels (k_stencil.hip, k_mixed.hip).
 struct CPar {
     double* U;
@@ -180,6 +199,9 @@ struct dfp_plan {
     double* d_tables = nullptr;  // one allocation holding every 1-D table
     double2* d_tw = nullptr;     // FFT twiddles: x stages, z stages, R2C post twiddles
     dfp::FftDesc fx{}, fz{};     // x: length nxp/2 (half-length complex), z: length nzp/zr
+    dfp::FftDesc fy{};           // periodic y: length nym-1
+    const double2* tw_y = nullptr;  // periodic y: stage twiddles of fy
+    int gy = 1;                  // periodic y: columns per y-transform call
     int zr = 1;                  // radix of the z stage applied by the x kernels (z split)
     const double2* tw_x = nullptr;
     const double2* tw_z = nullptr;
@@ -249,6 +271,11 @@ template <typename C>
 hipError_t launch_zfused_batch(hipStream_t s, const ZPar<C>* cs, const int* off, const int* idx, const int* nc,
                                int grid, size_t lds);
 size_t zf_lds_bytes(int nym, int n, int ntw, int f32);
+// periodic y: z transform + y FFT solve of each (kx, u) sub-plane in LDS (FP64)
+hipError_t launch_zfused_py(dfp_plan* p);
+size_t zf_py_lds_bytes(int nym, int n, int ntwz, int ntwy);
+// columns per y-transform call: the largest divisor of n whose columns one call holds
+int zf_py_columns(const FftDesc& fy, int n);
 size_t zf_lds_bytes(const dfp_plan* p, int f32);
 int zf_rows_per_call(const FftDesc& fz);
 // FP32 form

@@ -202,6 +202,39 @@ __device__ __forceinline__ double block_sum(double v, double* wsum) {
     return s;
 }
 
+// One Stockham butterfly of radix r (2, 3 or 4): a_t = sl[t*step], outputs
+// v[u] = (sum_t a_t w_r^{t*u}) * tws[p*(r-1) + u-1] (u >= 1)
+template <int SIGN, typename C>
+__device__ __forceinline__ void stockham_bfly(const C* sl, int step, int r, int p, const C* tws, C* v) {
+    using R = typename CplxT<C>::real;
+    if (r == 4) {
+        const C a0 = sl[0], a1 = sl[step], a2 = sl[2 * step], a3 = sl[3 * step];
+        const C s02 = cadd(a0, a2), d02 = csub(a0, a2);
+        const C s13 = cadd(a1, a3), d13 = csub(a1, a3);
+        const C jd13 = mul_si<SIGN>(d13);
+        const C* w = tws + p * 3;
+        v[0] = cadd(s02, s13);
+        v[1] = twiddle<SIGN>(cadd(d02, jd13), w[0]);
+        v[2] = twiddle<SIGN>(csub(s02, s13), w[1]);
+        v[3] = twiddle<SIGN>(csub(d02, jd13), w[2]);
+    } else if (r == 2) {
+        const C a0 = sl[0], a1 = sl[step];
+        v[0] = cadd(a0, a1);
+        v[1] = twiddle<SIGN>(csub(a0, a1), tws[p]);
+    } else {
+        const R s3 = (R)0.86602540378443864676372317075294;
+        const C a0 = sl[0], a1 = sl[step], a2 = sl[2 * step];
+        const C t1 = cadd(a1, a2);
+        const C t2 = CplxT<C>::make(a0.x - (R)0.5 * t1.x, a0.y - (R)0.5 * t1.y);
+        const C dd = csub(a1, a2);
+        const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
+        const C* w = tws + p * 2;
+        v[0] = cadd(a0, t1);
+        v[1] = twiddle<SIGN>(cadd(t2, jd), w[0]);
+        v[2] = twiddle<SIGN>(csub(t2, jd), w[1]);
+    }
+}
+
 // Transforms nl lines in LDS (line l, element e at buf[l*ld + e]) in place:
 // each thread holds the outputs of up to KMAX butterflies in registers
 // between a read phase and a write phase, so one LDS buffer suffices.
@@ -212,7 +245,6 @@ __device__ __forceinline__ double block_sum(double v, double* wsum) {
 // Ends with a barrier.
 template <int SIGN, int KMAX, typename C, bool PAIR2 = false>
 __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const C* tw) {
-    using R = typename CplxT<C>::real;
     int ncur = d.n, s = 1;
     for (int st = 0; st < d.nst; ++st) {
         const int r = d.rad[st];
@@ -272,32 +304,7 @@ __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const 
                 const int q = bi - p * s;
                 const C* sl = buf + l * ld + q + s * p;
                 dsto[k] = l * ld + q + s * r * p;
-                if (r == 4) {
-                    const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m], a3 = sl[3 * s * m];
-                    const C s02 = cadd(a0, a2), d02 = csub(a0, a2);
-                    const C s13 = cadd(a1, a3), d13 = csub(a1, a3);
-                    const C jd13 = mul_si<SIGN>(d13);
-                    const C* w = tws + p * 3;
-                    v[k][0] = cadd(s02, s13);
-                    v[k][1] = twiddle<SIGN>(cadd(d02, jd13), w[0]);
-                    v[k][2] = twiddle<SIGN>(csub(s02, s13), w[1]);
-                    v[k][3] = twiddle<SIGN>(csub(d02, jd13), w[2]);
-                } else if (r == 2) {
-                    const C a0 = sl[0], a1 = sl[s * m];
-                    v[k][0] = cadd(a0, a1);
-                    v[k][1] = twiddle<SIGN>(csub(a0, a1), tws[p]);
-                } else {
-                    const R s3 = (R)0.86602540378443864676372317075294;
-                    const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m];
-                    const C t1 = cadd(a1, a2);
-                    const C t2 = CplxT<C>::make(a0.x - (R)0.5 * t1.x, a0.y - (R)0.5 * t1.y);
-                    const C dd = csub(a1, a2);
-                    const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
-                    const C* w = tws + p * 2;
-                    v[k][0] = cadd(a0, t1);
-                    v[k][1] = twiddle<SIGN>(cadd(t2, jd), w[0]);
-                    v[k][2] = twiddle<SIGN>(csub(t2, jd), w[1]);
-                }
+                stockham_bfly<SIGN>(sl, s * m, r, p, tws, v[k]);
             }
         }
         __syncthreads();
@@ -309,6 +316,53 @@ __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const 
                 dl[s] = v[k][1];
                 if (r >= 3) dl[2 * s] = v[k][2];
                 if (r == 4) dl[3 * s] = v[k][3];
+            }
+        }
+        __syncthreads();
+        s *= r;
+        ncur = m;
+    }
+}
+
+// Transforms the columns c0 .. c0+nc-1 of a row-major LDS array (element e of
+// column c at buf[e*ld + c]) in place, the same stages as lds_fft_inplace.
+// Butterfly b takes column c0 + b % nc, so neighbouring lanes read
+// neighbouring columns (no bank conflicts whatever ld is).  divc divides by
+// nc.  Requires nc * n / r <= KMAX * blockDim.x for every stage radix r.
+// Ends with a barrier.
+template <int SIGN, int KMAX, typename C>
+__device__ void lds_fft_cols(C* buf, int c0, int nc, int ld, const FftDesc& d, const C* tw, const FastDiv& divc) {
+    int ncur = d.n, s = 1;
+    for (int st = 0; st < d.nst; ++st) {
+        const int r = d.rad[st];
+        const int m = ncur / r;
+        const int tot = m * s * nc;
+        const C* tws = tw + d.twoff[st];
+        C v[KMAX][4];
+        int dsto[KMAX];
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            const int b = threadIdx.x + k * blockDim.x;
+            dsto[k] = -1;
+            if (b < tot) {
+                const int bi = (int)divc.div((uint32_t)b);
+                const int c = c0 + b - bi * nc;
+                const int p = (int)d.sdiv[st].div((uint32_t)bi);
+                const int q = bi - p * s;
+                dsto[k] = (q + s * r * p) * ld + c;
+                stockham_bfly<SIGN>(buf + (q + s * p) * ld + c, s * m * ld, r, p, tws, v[k]);
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < KMAX; ++k) {
+            if (dsto[k] >= 0) {
+                C* dl = buf + dsto[k];
+                const int step = s * ld;
+                dl[0] = v[k][0];
+                dl[step] = v[k][1];
+                if (r >= 3) dl[2 * step] = v[k][2];
+                if (r == 4) dl[3 * step] = v[k][3];
             }
         }
         __syncthreads();

@@ -5,9 +5,11 @@
 //   k_wrap_x/z      periodic ghost planes             boundary_conditions.f90:263-289,
 //                                                     :580-602
 //   k_robin_y       Robin (slip-length) wall rows     boundary_conditions.f90:784-804
+//   k_wrap_y        periodic y ghost rows (y_bc_type = 0, the periodic box)
+//                                                     boundary_conditions.f90:762-781
 // apply_boundary_conditions (:186-260) is the sequence x wrap, z wrap, y
-// walls, x wrap, z wrap; on one rank its halo exchanges (:170-183) exchange
-// nothing.  Each launch covers the three fields (blockIdx.z); arithmetic is
+// walls (or y wrap, :229-233), x wrap, z wrap; on one rank its halo
+// exchanges (:170-183) exchange nothing.  Each launch covers the three fields (blockIdx.z); arithmetic is
 // kept operation for operation (no contraction), so results equal the CPU
 // path bit for bit.
 #include <algorithm>
@@ -104,6 +106,25 @@ __global__ void __launch_bounds__(256) k_wrap_z(Field3 f) {
     }
 }
 
+// y ghost rows over (i, k): V faces (1 <- ny-1, ny <- 2), U and W centres
+// (1 <- nyg-2, nyg-1 <- 2, nyg <- 3), in the reference's statement order
+__global__ void __launch_bounds__(256) k_wrap_y(Field3 f) {
+    const int fi = blockIdx.z;
+    const int n1 = f.n1[fi], n2 = f.n2[fi], n3 = f.n3[fi];
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    if (t >= (long)n1 * n3) return;
+    const long k = t / n1, i = t - k * n1;
+    double* F = f.F[fi] + i + k * (long)n1 * n2;  // F(i, 1, k), stride n1 in y
+    if (fi == 1) {
+        F[0] = F[(long)(n2 - 2) * n1];
+        F[(long)(n2 - 1) * n1] = F[n1];
+    } else {
+        F[0] = F[(long)(n2 - 3) * n1];
+        F[(long)(n2 - 2) * n1] = F[n1];
+        F[(long)(n2 - 1) * n1] = F[2 * (long)n1];
+    }
+}
+
 struct RobinArgs {
     const double* alpha[3];  // alpha_x (nx,2,nzg), alpha_y (nxg,2,nzg), alpha_z (nxg,2,nz)
     double dy1, dy2;         // y(2)-y(1), y(ny)-y(ny-1)
@@ -176,7 +197,10 @@ hipError_t launch_apply_bc(dfp_plan* p, double* U, double* V, double* W, const d
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_wrap_x, dim3(ceil_div(px, 256), 1, 3), dim3(256), 0, p->stream, f);
         hipLaunchKernelGGL(k_wrap_z, dim3(ceil_div(pz, 256), 1, 3), dim3(256), 0, p->stream, f);
-        if (pass == 0) hipLaunchKernelGGL(k_robin_y, dim3(ceil_div(py, 256), 1, 3), dim3(256), 0, p->stream, f, r);
+        if (pass == 0 && p->geo.ybc == 0)
+            hipLaunchKernelGGL(k_wrap_y, dim3(ceil_div(py, 256), 1, 3), dim3(256), 0, p->stream, f);
+        else if (pass == 0)
+            hipLaunchKernelGGL(k_robin_y, dim3(ceil_div(py, 256), 1, 3), dim3(256), 0, p->stream, f, r);
     }
     prof_end(p, slot);
     return hipGetLastError();

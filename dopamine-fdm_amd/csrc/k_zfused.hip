@@ -26,6 +26,23 @@ __device__ __forceinline__ void zf_rows_fft(C* A, const ZPar<C>& zp, const C* tw
     for (int l0 = 0; l0 < zp.nym; l0 += zp.G) lds_fft_inplace<SIGN, 2>(A + (long)l0 * n, min(zp.G, zp.nym - l0), n, zp.fz, twl);
 }
 
+// the plane (total values) into LDS: four values per thread in flight per
+// batch (named registers, clamped unconditional loads)
+template <typename C>
+__device__ __forceinline__ void zf_load_plane(C* A, const Glob<C>* plane, int total) {
+    for (int base = 0; base < total; base += 4 * kZfThreads) {
+        const int e0 = base + (int)threadIdx.x;
+        const C v0 = gld<C>(plane + min(e0, total - 1));
+        const C v1 = gld<C>(plane + min(e0 + kZfThreads, total - 1));
+        const C v2 = gld<C>(plane + min(e0 + 2 * kZfThreads, total - 1));
+        const C v3 = gld<C>(plane + min(e0 + 3 * kZfThreads, total - 1));
+        if (e0 < total) A[e0] = v0;
+        if (e0 + kZfThreads < total) A[e0 + kZfThreads] = v1;
+        if (e0 + 2 * kZfThreads < total) A[e0 + 2 * kZfThreads] = v2;
+        if (e0 + 3 * kZfThreads < total) A[e0 + 3 * kZfThreads] = v3;
+    }
+}
+
 template <typename C>
 __device__ __forceinline__ void zfused_body(const ZPar<C>& zp, int b, unsigned char* smem_raw) {
     using Rl = typename CplxT<C>::real;
@@ -43,19 +60,7 @@ __device__ __forceinline__ void zfused_body(const ZPar<C>& zp, int b, unsigned c
     double* sc = sg + 3 * kZfThreads;                        // chunk carries: kZfThreads x (re, im)
     const auto plane = gptr(zp.spec) + (long)b * nym * n;
     const int total = nym * n;
-    // load the plane: four values per thread in flight per batch (named
-    // registers, clamped unconditional loads)
-    for (int base = 0; base < total; base += 4 * kZfThreads) {
-        const int e0 = base + (int)threadIdx.x;
-        const C v0 = gld<C>(plane + min(e0, total - 1));
-        const C v1 = gld<C>(plane + min(e0 + kZfThreads, total - 1));
-        const C v2 = gld<C>(plane + min(e0 + 2 * kZfThreads, total - 1));
-        const C v3 = gld<C>(plane + min(e0 + 3 * kZfThreads, total - 1));
-        if (e0 < total) A[e0] = v0;
-        if (e0 + kZfThreads < total) A[e0 + kZfThreads] = v1;
-        if (e0 + 2 * kZfThreads < total) A[e0 + 2 * kZfThreads] = v2;
-        if (e0 + 3 * kZfThreads < total) A[e0 + 3 * kZfThreads] = v3;
-    }
+    zf_load_plane(A, plane, total);
     const auto tw = gptr(zp.tw);
     const auto dl = gptr(zp.dl);
     const auto du = gptr(zp.du);
@@ -207,6 +212,97 @@ __global__ void __launch_bounds__(kZfThreads) kb_zfused(const ZPar<C>* __restric
         zfused_body<C>(cs[idx[lo]], w - off[lo], smem_raw);
         __syncthreads();
     }
+}
+
+// Periodic y (y_bc_type = 0 on a uniform y grid, projection.f90:246-273):
+// the wall solve's place is taken by a y FFT over the nyp = nym - 1
+// independent rows of the sub-plane, a division of each (kx, ky, kz) mode by
+// kxx + kyy + kzz with the (0,0,0) mode set to zero, and the inverse y FFT
+// scaled by 1/nyp (folded into the division).  Row nym - 1, the duplicate
+// the reference leaves out of its transform, then takes row 0
+// (projection.f90:350-354): the inverse z and x transforms reproduce row 0 in
+// it bit for bit.  (The reference's copy also spans rhs_p's ghost positions
+// i = nxg and k = nzg; it zeroes rhs_p at allocation, initialization.f90:220,
+// and only ever copies those positions among themselves, so they stay zero,
+// and the library leaves them as the caller's array holds them.)  One
+// workgroup per (kx, u) sub-plane, as above.
+__device__ __forceinline__ void zf_py_body(const ZyPar<double2>& q, int b, unsigned char* smem_raw) {
+    using C = double2;
+    const ZPar<C>& zp = q.z;
+    C* smem = reinterpret_cast<C*>(smem_raw);
+    const int n = zp.fz.n;
+    const int nym = zp.nym, nyp = nym - 1, R = zp.R, ntw = zp.ntw;
+    const int kx = b / R;
+    const int u = b - kx * R;
+    C* A = smem;                       // nym * n
+    C* twl = A + (long)nym * n;        // ntw: z stages
+    C* twy = twl + ntw;                // q.ntwy: y stages
+    const auto plane = gptr(zp.spec) + (long)b * nym * n;
+    const int total = nym * n;
+    zf_load_plane(A, plane, total);
+    const auto tw = gptr(zp.tw);
+    const auto tyg = gptr(q.twy);
+    for (int e = threadIdx.x; e < ntw; e += kZfThreads) twl[e] = gld<C>(tw + e);
+    for (int e = threadIdx.x; e < q.ntwy; e += kZfThreads) twy[e] = gld<C>(tyg + e);
+    __syncthreads();
+    zf_rows_fft<-1>(A, zp, twl);
+    for (int c0 = 0; c0 < n; c0 += q.gy) lds_fft_cols<-1, 4>(A, c0, q.gy, n, q.fy, twy, q.div_gy);
+    // element (ky, m) of the sub-plane is mode (kx, ky, kz = R m + u)
+    const double lx = q.kxx[kx];
+    for (int e = threadIdx.x; e < nyp * n; e += kZfThreads) {
+        const int ky = (int)zp.div_n.div((uint32_t)e);
+        const int m = e - ky * n;
+        const int kz = R * m + u;
+        const double lam = lx + q.kyy[ky] + q.kzz[kz];
+        const C a = A[e];
+        if (kx == 0 && ky == 0 && kz == 0) {
+            A[e] = make_double2(0.0, 0.0);
+        } else {
+            const double s = q.inv_ny / lam;
+            A[e] = make_double2(a.x * s, a.y * s);
+        }
+    }
+    __syncthreads();
+    for (int c0 = 0; c0 < n; c0 += q.gy) lds_fft_cols<1, 4>(A, c0, q.gy, n, q.fy, twy, q.div_gy);
+    for (int m = threadIdx.x; m < n; m += kZfThreads) A[(long)nyp * n + m] = A[m];
+    __syncthreads();
+    zf_rows_fft<1>(A, zp, twl);
+    for (int e = threadIdx.x; e < total; e += kZfThreads) gst<C>(plane + e, A[e]);
+}
+
+__global__ void __launch_bounds__(kZfThreads) k_zfused_py(ZyPar<double2> q) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    zf_py_body(q, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
+}
+
+size_t zf_py_lds_bytes(int nym, int n, int ntwz, int ntwy) {
+    return sizeof(double2) * ((size_t)nym * n + ntwz + ntwy);
+}
+
+int zf_py_columns(const FftDesc& fy, int n) {
+    const int cap = inplace_capacity(fy, kZfThreads, 4);  // butterfly slots x smallest radix
+    for (int g = n; g >= 1; --g)
+        if (n % g == 0 && (long)g * fy.n <= cap) return g;
+    return 0;
+}
+
+hipError_t launch_zfused_py(dfp_plan* p) {
+    ZyPar<double2> q{};
+    q.z = make_zpar<double2>(p, 0);
+    q.fy = p->fy;
+    q.twy = p->tw_y;
+    q.kxx = p->ops.kxx;
+    q.kyy = p->ops.kyy;
+    q.kzz = p->ops.kzz;
+    q.ntwy = twiddle_count(p->fy);
+    q.gy = p->gy;
+    q.div_gy = FastDiv((uint32_t)p->gy);
+    q.inv_ny = 1.0 / (double)(p->geo.nym - 1);
+    const size_t lds = zf_py_lds_bytes(p->geo.nym, p->fz.n, q.z.ntw, q.ntwy);
+    const int slot = prof_begin(p, "zfused_py");
+    hipLaunchKernelGGL(k_zfused_py, dim3(q.z.nblocks), dim3(kZfThreads), lds, p->stream, q);
+    prof_end(p, slot);
+    return hipGetLastError();
 }
 
 // LDS of the fused kernel for a plan (spectrum precision f32 or FP64)

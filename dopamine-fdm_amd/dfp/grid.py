@@ -26,6 +26,7 @@ class ChannelGrid:
     alpha: float
     grid_type: int = 2
     x_bc_type: int = 0   # 0 periodic x, 1 inflow/outflow x (global.f90:185-186)
+    y_bc_type: int = 1   # 1 walls, 0 periodic y (uniform y and periodic x only, initialization.f90:96-97, :128-129)
     x: np.ndarray = field(init=False)
     y: np.ndarray = field(init=False)
     z: np.ndarray = field(init=False)
@@ -40,6 +41,10 @@ class ChannelGrid:
             raise ValueError("need nxp even >= 2, nym >= 1, nzp >= 1")
         if self.x_bc_type not in (0, 1):
             raise ValueError("x_bc_type must be 0 (periodic) or 1 (inflow/outflow)")
+        if self.y_bc_type not in (0, 1):
+            raise ValueError("y_bc_type must be 1 (walls) or 0 (periodic)")
+        if self.y_bc_type == 0 and (self.grid_type != 1 or self.x_bc_type != 0 or self.nym < 2):
+            raise ValueError("periodic y needs a uniform y grid (grid_type 1), a periodic x and nym >= 2")
         nx, ny, nz = self.nx, self.ny, self.nz
         self.x = np.array([self.Lx_i * float(i) / float(nx - 1) for i in range(nx)])
         self.z = np.array([self.Lz_i * float(i) / float(nz - 1) for i in range(nz)])
@@ -103,6 +108,13 @@ def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * math.pi, Lz=math.pi, Ly=2.0, gri
     inflow/outflow x (x_bc_type = 1) the nxp cells span Lx (Lxp = Lx, :333-334)."""
     Lx_i = Lx * (nxp + 1) / nxp if x_bc_type == 0 else Lx
     return ChannelGrid(nxp, nym, nzp, Lx_i, Ly, Lz * (nzp + 1) / nzp, alpha, grid_type, x_bc_type)
+
+
+def periodic_y_grid(nxp, nym, nzp, Lx=2.0 * math.pi, Ly=2.0, Lz=math.pi) -> ChannelGrid:
+    """Triply periodic box (y_bc_type = 0, uniform y): the last of the nym
+    y cells duplicates the first (projection.f90:350-354), so the namelist
+    length spans nym cells of Ly/(nym-1) and the y period is Ly."""
+    return ChannelGrid(nxp, nym, nzp, Lx * (nxp + 1) / nxp, Ly * nym / (nym - 1), Lz * (nzp + 1) / nzp, 0.0, 1, 0, 0)
 
 
 def namelist_grid(nx, ny, nz, Lx, Ly, Lz, alpha, grid_type=2, x_bc_type=0) -> ChannelGrid:

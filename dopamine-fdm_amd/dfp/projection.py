@@ -77,7 +77,8 @@ class Plan:
         self._arrays = [np.ascontiguousarray(a, dtype=np.float64) for a in (g.x, g.y, g.z, g.yg, g.zg)]
         dp = ctypes.POINTER(ctypes.c_double)
         x, y, z, yg, zg = (a.ctypes.data_as(dp) for a in self._arrays)
-        self._grid = DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, x, y, z, yg, zg, int(getattr(g, "x_bc_type", 0)), 1, 0)
+        self._grid = DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, x, y, z, yg, zg, int(getattr(g, "x_bc_type", 0)),
+                             int(getattr(g, "y_bc_type", 1)), 0)
         self._outflow = None  # keeps the arrays registered with dfp_set_outflow alive
         h = ctypes.c_void_p()
         if tuning is None:
@@ -204,9 +205,9 @@ class Plan:
         for a, sh, name in ((alpha_x, (g.nx, 2, g.nzg), "alpha_x"), (alpha_y, (g.nxg, 2, g.nzg), "alpha_y"),
                             (alpha_z, (g.nxg, 2, g.nz), "alpha_z")):
             p, dev = _ptr(a, sh, name)
-            if not dev:
+            if a is not None and not dev:
                 raise ValueError(f"{name} must be device-resident")
-            out.append(p)
+            out.append(p)  # None (periodic y takes no alphas) passes a null pointer
         return out
 
     def rk_update(self, stage, U, V, W, Uo, Vo, Wo, Fu, Fv, Fw, coef, dt, asynchronous=False):
@@ -217,8 +218,9 @@ class Plan:
         f = DFP_FIELDS_ON_DEVICE | (DFP_ASYNC if asynchronous else 0)
         check(lib().dfp_rk_update(self._h, int(stage), *ptrs, *rhs, c, float(dt), f), "dfp_rk_update", self._h)
 
-    def apply_bc(self, U, V, W, alpha_x, alpha_y, alpha_z, asynchronous=False):
-        """apply_boundary_conditions of the channel (boundary_conditions.f90:186-260)."""
+    def apply_bc(self, U, V, W, alpha_x=None, alpha_y=None, alpha_z=None, asynchronous=False):
+        """apply_boundary_conditions (boundary_conditions.f90:186-260): the
+        channel's Robin walls (alphas required) or the periodic y wrap."""
         g = self.g
         pu, du = _ptr(U, g.shape_u, "U")
         pv, dv = _ptr(V, g.shape_v, "V")

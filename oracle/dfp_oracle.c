@@ -884,6 +884,22 @@ static void periodic_z(double *F, int n1, int n2, int n3, int faces) {
         }
 }
 
+/* boundary_conditions.f90:762-781: periodic y (one rank); faces: V (id 1,
+   1 <- ny-1, ny <- 2), else U and W centres (1 <- nyg-2, nyg-1 <- 2, nyg <- 3) */
+static void periodic_y(double *F, int n1, int n2, int n3, int faces) {
+    for (int k = 1; k <= n3; k++)
+        for (int i = 1; i <= n1; i++) {
+            if (faces) {
+                IDX3(F, n1, n2, i, 1, k) = IDX3(F, n1, n2, i, n2 - 1, k);
+                IDX3(F, n1, n2, i, n2, k) = IDX3(F, n1, n2, i, 2, k);
+            } else {
+                IDX3(F, n1, n2, i, 1, k) = IDX3(F, n1, n2, i, n2 - 2, k);
+                IDX3(F, n1, n2, i, n2 - 1, k) = IDX3(F, n1, n2, i, 2, k);
+                IDX3(F, n1, n2, i, n2, k) = IDX3(F, n1, n2, i, 3, k);
+            }
+        }
+}
+
 /* boundary_conditions.f90:784-804: Robin (slip-length) walls; alpha(n1,2,n3)
    holds the lower (1) and upper (2) wall coefficients */
 static void robin_y(const ora_plan *p, double *F, int n1, int n2, int n3, const double *alpha, int faces) {
@@ -905,9 +921,10 @@ static void robin_y(const ora_plan *p, double *F, int n1, int n2, int n3, const 
 
 void ora_apply_bc(const ora_plan *p, double *U, double *V, double *W, const double *alpha_x, const double *alpha_y,
                   const double *alpha_z) {
-    /* boundary_conditions.f90:186-260 for x_bc_type = 0, z_bc_type = 0,
-       y_bc_type = 1: periodic x, periodic z, Robin y, then the x and z wraps
-       again (:247-258) */
+    /* boundary_conditions.f90:186-260 for x_bc_type = 0, z_bc_type = 0:
+       periodic x, periodic z, then Robin y (y_bc_type = 1) or periodic y
+       (y_bc_type = 0, :229-233; alphas unused), then the x and z wraps again
+       (:247-258) */
     for (int pass = 0; pass < 2; pass++) {
         periodic_x(U, p->nx, p->nyg, p->nzg, 1);
         periodic_x(V, p->nxg, p->ny, p->nzg, 0);
@@ -915,7 +932,11 @@ void ora_apply_bc(const ora_plan *p, double *U, double *V, double *W, const doub
         periodic_z(U, p->nx, p->nyg, p->nzg, 0);
         periodic_z(V, p->nxg, p->ny, p->nzg, 0);
         periodic_z(W, p->nxg, p->nyg, p->nz, 1);
-        if (pass == 0) {
+        if (pass == 0 && !p->ybc) {
+            periodic_y(U, p->nx, p->nyg, p->nzg, 0);
+            periodic_y(V, p->nxg, p->ny, p->nzg, 1);
+            periodic_y(W, p->nxg, p->nyg, p->nz, 0);
+        } else if (pass == 0) {
             robin_y(p, U, p->nx, p->nyg, p->nzg, alpha_x, 0);
             robin_y(p, V, p->nxg, p->ny, p->nzg, alpha_y, 1);
             robin_y(p, W, p->nxg, p->nyg, p->nz, alpha_z, 0);

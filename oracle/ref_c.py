@@ -183,9 +183,13 @@ class CProjection:
                             ctypes.cast(arrs[2], ctypes.c_void_p), ctypes.cast(c, ctypes.c_void_p), float(dt))
 
     def apply_boundary_conditions(self, U, V, W, alpha_x, alpha_y, alpha_z):
-        """boundary_conditions.f90:186-260 of the channel, one rank."""
+        """boundary_conditions.f90:186-260 of the channel (or, y_bc_type = 0,
+        the periodic box; its alphas may be None), one rank."""
         self._chk(U, V, W)
         g = self.g
+        if getattr(g, "y_bc_type", 1) == 0 and alpha_x is None:
+            lib().ora_apply_bc(self._p, _ptr(U), _ptr(V), _ptr(W), None, None, None)
+            return
         _check_f64_fortran(alpha_x, (g.nx, 2, g.nzg), "alpha_x")
         _check_f64_fortran(alpha_y, (g.nxg, 2, g.nzg), "alpha_y")
         _check_f64_fortran(alpha_z, (g.nxg, 2, g.nz), "alpha_z")

@@ -600,6 +600,18 @@ def _periodic_z(F, faces):
         F[:, :, n - 1] = F[:, :, 2]
 
 
+def _periodic_y(F, faces):
+    # boundary_conditions.f90:762-781: id 1 y faces (V), else y centres (U, W)
+    n = F.shape[1]
+    if faces:
+        F[:, 0, :] = F[:, n - 2, :]
+        F[:, n - 1, :] = F[:, 1, :]
+    else:
+        F[:, 0, :] = F[:, n - 3, :]
+        F[:, n - 2, :] = F[:, 1, :]
+        F[:, n - 1, :] = F[:, 2, :]
+
+
 def _robin_y(g, F, alpha, faces):
     # boundary_conditions.f90:784-804: slip length alpha(:,1,:) at the lower,
     # alpha(:,2,:) at the upper wall
@@ -620,12 +632,15 @@ def apply_boundary_conditions(g: ChannelGrid, U, V, W, alpha_x, alpha_y, alpha_z
     z_bc_type = 0, y_bc_type = 1) on one rank, where the halo exchanges
     (:170-183) exchange nothing: periodic x (U faces, V and W centres),
     periodic z (U and V centres, W faces), Robin y walls (U with alpha_x, V
-    faces with alpha_y, W with alpha_z), then the x and z wraps again
-    (:247-258)."""
+    faces with alpha_y, W with alpha_z) or, with y_bc_type = 0, the periodic
+    y wrap (:229-233, V faces, U and W centres; the alphas are unused), then
+    the x and z wraps again (:247-258)."""
     for first in (True, False):
         _periodic_x(U, True); _periodic_x(V, False); _periodic_x(W, False)
         _periodic_z(U, False); _periodic_z(V, False); _periodic_z(W, True)
-        if first:
+        if first and getattr(g, "y_bc_type", 1) == 0:
+            _periodic_y(U, False); _periodic_y(V, True); _periodic_y(W, False)
+        elif first:
             _robin_y(g, U, alpha_x, False)
             _robin_y(g, V, alpha_y, True)
             _robin_y(g, W, alpha_z, False)

@@ -48,7 +48,7 @@ def _grid_struct(g, bc=(0, 1, 0)):
     return _lib.DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, *ptrs, *bc), keep
 
 
-@pytest.mark.parametrize("bc", [(2, 1, 0), (1, 0, 0), (0, 0, 0), (0, 1, 1), (1, 1, 1)])
+@pytest.mark.parametrize("bc", [(2, 1, 0), (1, 0, 0), (0, 2, 0), (0, 1, 1), (1, 1, 1), (0, 0, 1)])
 def test_other_boundary_conditions_are_rejected(bc):
     g = dfp.channel_grid(16, 8, 16, 1.5)
     s, keep = _grid_struct(g, bc)
@@ -134,3 +134,28 @@ def test_inflow_outflow_host_grid_rules():
     assert g.shape_u == (17, 10, 11) and g.shape_v == (18, 9, 11) and g.shape_rhs == (17, 8, 10)
     assert g.dx == pytest.approx(4.0 / 16, rel=1e-14)
     assert g.n_modes == 16 * 8 * 8
+
+
+def test_periodic_y_rules_before_any_device_call():
+    """y_bc_type = 0 needs a periodic x (initialization.f90:96-97), a uniform
+    y grid (:128-129), nym - 1 = 2^a 3^b independent rows and FP64; each is
+    refused before the device is touched."""
+    L = dfp.lib()
+    h = ctypes.c_void_p()
+    g = dfp.periodic_y_grid(16, 9, 16)
+    s, keep = _grid_struct(g, (0, 0, 0))
+    assert L.dfp_plan_create(ctypes.byref(s), dfp.DFP_FP32_REFINE, 0, ctypes.byref(h)) == -2
+    s, keep = _grid_struct(dfp.periodic_y_grid(16, 8, 16), (0, 0, 0))       # nym - 1 = 7
+    assert L.dfp_plan_create(ctypes.byref(s), 0, 0, ctypes.byref(h)) == -2
+    s, keep = _grid_struct(dfp.channel_grid(16, 9, 16, 1.5), (0, 0, 0))     # tanh-stretched y
+    assert L.dfp_plan_create(ctypes.byref(s), 0, 0, ctypes.byref(h)) == -2
+    s, keep = _grid_struct(dfp.channel_grid(16, 1, 16, 0.0, grid_type=1), (0, 0, 0))  # nym = 1
+    assert L.dfp_plan_create(ctypes.byref(s), 0, 0, ctypes.byref(h)) == -2
+    assert not h.value
+    with pytest.raises(ValueError):
+        dfp.ChannelGrid(16, 9, 16, 7.0, 2.0, 3.5, 1.5, 2, 0, 0)  # periodic y on a stretched grid
+    with pytest.raises(ValueError):
+        dfp.ChannelGrid(16, 9, 16, 7.0, 2.0, 3.5, 0.0, 1, 1, 0)  # with the inflow/outflow x
+    # the last y face of the box is one period above the first (projection.f90:350-354)
+    g = dfp.periodic_y_grid(8, 9, 6, Ly=2.0)
+    assert g.y[-2] - g.y[0] == pytest.approx(2.0, rel=1e-14)

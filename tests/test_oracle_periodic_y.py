@@ -131,3 +131,31 @@ def test_c_oracle_equals_numpy_periodic_y():
     op.close()
     for x, y in zip(a, b):
         assert np.max(np.abs(x - y)) <= 1e-13 * max(1.0, np.max(np.abs(y)))
+
+
+def test_periodic_y_boundary_conditions():
+    # boundary_conditions.f90:186-260 with y_bc_type = 0 (:229-233): every y
+    # ghost row equals its periodic source row; the interior is untouched
+    g = R.periodic_y_grid(8, 7, 6)
+    rng = np.random.default_rng(21)
+    U, V, W = (np.asfortranarray(rng.standard_normal(s)) for s in (g.shape_u, g.shape_v, g.shape_w))
+    U0, V0, W0 = U.copy(order="F"), V.copy(order="F"), W.copy(order="F")
+    a = [x.copy(order="F") for x in (U, V, W)]
+    R.apply_boundary_conditions(g, *a, None, None, None)
+    Ua, Va, Wa = a
+    ny, nyg = g.ny, g.nyg
+    assert np.array_equal(Va[:, 0, :], Va[:, ny - 2, :]) and np.array_equal(Va[:, ny - 1, :], Va[:, 1, :])
+    for F in (Ua, Wa):
+        assert np.array_equal(F[:, 0, :], F[:, nyg - 3, :])
+        assert np.array_equal(F[:, nyg - 2, :], F[:, 1, :]) and np.array_equal(F[:, nyg - 1, :], F[:, 2, :])
+    assert np.array_equal(Ua[1:-1, 1:-2, 1:-2], U0[1:-1, 1:-2, 1:-2])
+    assert np.array_equal(Va[1:-2, 1:-1, 1:-2], V0[1:-2, 1:-1, 1:-2])
+    assert np.array_equal(Wa[1:-2, 1:-2, 1:-1], W0[1:-2, 1:-2, 1:-1])
+    # x and z wraps still hold after the y wrap (the second pass, :247-258)
+    assert np.array_equal(Ua[0], Ua[g.nx - 2]) and np.array_equal(Wa[:, :, 0], Wa[:, :, g.nz - 2])
+    b = [x.copy(order="F") for x in (U, V, W)]
+    op = ref_c.CProjection(g)
+    op.apply_boundary_conditions(*b, None, None, None)
+    op.close()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
