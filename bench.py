@@ -264,6 +264,42 @@ def bench_inflow(K: int, Wm: int, beta: float = 0.5) -> dict:
             "kernels_us": {k: round(1e3 * m / max(n, 1), 2) for k, (n, m) in prof.items()}}
 
 
+def periodic_y_bytes(N: int, H: int) -> dict:
+    """Per-launch bytes of the periodic-box kernels: the wall solve's pair
+    becomes one fused z-FFT / y-FFT pass over the spectrum (32 H)."""
+    return {"div_rfft_x": 24 * N + 16 * H, "zfused_py": 32 * H, "irfft_x": 16 * H + 8 * N, "correct": 56 * N}
+
+
+def bench_periodic_y(K: int, Wm: int, n: int = 256) -> dict:
+    """The periodic box (y_bc_type = 0) at n x (n+1) x n (the last y row
+    duplicates the first, so n independent rows): same RK3 loop as the FP64
+    line, fields resident in HBM."""
+    import dfp
+    from dfp import hipmem
+
+    g = dfp.periodic_y_grid(n, n + 1, n)
+    N, H = g.n_cells, g.n_modes
+    plan = dfp.Plan(g)
+    sets = make_sets(plan, g, 12, 0x5EED3000)
+    rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
+    P = dfp.DeviceArray(g.shape_p, zero=True)
+    hipmem.synchronize()
+    ms = time_steps(plan, sets, rhs, P, K, Wm)
+    plan.profile(True)
+    plan.profile_reset()
+    time_steps(plan, sets, rhs, P, max(1, K // 5), 0, first=Wm + K)
+    prof = plan.profile_read()
+    plan.profile(False)
+    plan.close()
+    rate = 3 * K / (ms / 1e3)
+    impl = sum(periodic_y_bytes(N, H).values())
+    gbs = impl * rate / 1e9
+    return {"grid": [g.nxp, g.nym, g.nzp], "projections_per_s": round(rate, 2),
+            "us_per_projection": round(1e6 / rate, 2), "impl_bytes": int(impl), "impl_GBs": round(gbs, 1),
+            "impl_frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "kernels_us": {k: round(1e3 * m / max(n_, 1), 2) for k, (n_, m) in prof.items()}}
+
+
 def form_gbs(bytes_per_projection: int, projections_per_s: float) -> dict:
     gbs = bytes_per_projection * projections_per_s / 1e9
     return {"B_model_bytes": int(bytes_per_projection), "achieved_GBs": round(gbs, 1),
@@ -442,6 +478,7 @@ def main() -> int:
     ap.add_argument("--no-batched", action="store_true", help="skip the 48-case batched measurement")
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-inflow", action="store_true", help="skip the inflow/outflow x measurement")
+    ap.add_argument("--no-periodic-y", action="store_true", help="skip the periodic-box measurement")
     ap.add_argument("--no-host-staged", action="store_true", help="skip the host-staged (PCIe) measurement")
     ap.add_argument("--save-tuning", default=None, help="write the autotuner table (with new entries) here")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -566,6 +603,8 @@ def main() -> int:
             plan.close()
             del sets, rhs, P
         out["inflow_outflow"] = bench_inflow(max(2, K // 2), Wm)
+    if not args.no_periodic_y:
+        out["periodic_y"] = bench_periodic_y(max(2, K // 2), Wm)
     if not args.no_batched:
         out["batched"] = bench_batched(save_tuning=args.save_tuning)
     if not args.no_serving:

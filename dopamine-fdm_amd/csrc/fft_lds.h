@@ -202,39 +202,6 @@ __device__ __forceinline__ double block_sum(double v, double* wsum) {
     return s;
 }
 
-// One Stockham butterfly of radix r (2, 3 or 4): a_t = sl[t*step], outputs
-// v[u] = (sum_t a_t w_r^{t*u}) * tws[p*(r-1) + u-1] (u >= 1)
-template <int SIGN, typename C>
-__device__ __forceinline__ void stockham_bfly(const C* sl, int step, int r, int p, const C* tws, C* v) {
-    using R = typename CplxT<C>::real;
-    if (r == 4) {
-        const C a0 = sl[0], a1 = sl[step], a2 = sl[2 * step], a3 = sl[3 * step];
-        const C s02 = cadd(a0, a2), d02 = csub(a0, a2);
-        const C s13 = cadd(a1, a3), d13 = csub(a1, a3);
-        const C jd13 = mul_si<SIGN>(d13);
-        const C* w = tws + p * 3;
-        v[0] = cadd(s02, s13);
-        v[1] = twiddle<SIGN>(cadd(d02, jd13), w[0]);
-        v[2] = twiddle<SIGN>(csub(s02, s13), w[1]);
-        v[3] = twiddle<SIGN>(csub(d02, jd13), w[2]);
-    } else if (r == 2) {
-        const C a0 = sl[0], a1 = sl[step];
-        v[0] = cadd(a0, a1);
-        v[1] = twiddle<SIGN>(csub(a0, a1), tws[p]);
-    } else {
-        const R s3 = (R)0.86602540378443864676372317075294;
-        const C a0 = sl[0], a1 = sl[step], a2 = sl[2 * step];
-        const C t1 = cadd(a1, a2);
-        const C t2 = CplxT<C>::make(a0.x - (R)0.5 * t1.x, a0.y - (R)0.5 * t1.y);
-        const C dd = csub(a1, a2);
-        const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
-        const C* w = tws + p * 2;
-        v[0] = cadd(a0, t1);
-        v[1] = twiddle<SIGN>(cadd(t2, jd), w[0]);
-        v[2] = twiddle<SIGN>(csub(t2, jd), w[1]);
-    }
-}
-
 // Transforms nl lines in LDS (line l, element e at buf[l*ld + e]) in place:
 // each thread holds the outputs of up to KMAX butterflies in registers
 // between a read phase and a write phase, so one LDS buffer suffices.
@@ -245,6 +212,7 @@ __device__ __forceinline__ void stockham_bfly(const C* sl, int step, int r, int 
 // Ends with a barrier.
 template <int SIGN, int KMAX, typename C, bool PAIR2 = false>
 __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const C* tw) {
+    using R = typename CplxT<C>::real;
     int ncur = d.n, s = 1;
     for (int st = 0; st < d.nst; ++st) {
         const int r = d.rad[st];
@@ -304,7 +272,32 @@ __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const 
                 const int q = bi - p * s;
                 const C* sl = buf + l * ld + q + s * p;
                 dsto[k] = l * ld + q + s * r * p;
-                stockham_bfly<SIGN>(sl, s * m, r, p, tws, v[k]);
+                if (r == 4) {
+                    const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m], a3 = sl[3 * s * m];
+                    const C s02 = cadd(a0, a2), d02 = csub(a0, a2);
+                    const C s13 = cadd(a1, a3), d13 = csub(a1, a3);
+                    const C jd13 = mul_si<SIGN>(d13);
+                    const C* w = tws + p * 3;
+                    v[k][0] = cadd(s02, s13);
+                    v[k][1] = twiddle<SIGN>(cadd(d02, jd13), w[0]);
+                    v[k][2] = twiddle<SIGN>(csub(s02, s13), w[1]);
+                    v[k][3] = twiddle<SIGN>(csub(d02, jd13), w[2]);
+                } else if (r == 2) {
+                    const C a0 = sl[0], a1 = sl[s * m];
+                    v[k][0] = cadd(a0, a1);
+                    v[k][1] = twiddle<SIGN>(csub(a0, a1), tws[p]);
+                } else {
+                    const R s3 = (R)0.86602540378443864676372317075294;
+                    const C a0 = sl[0], a1 = sl[s * m], a2 = sl[2 * s * m];
+                    const C t1 = cadd(a1, a2);
+                    const C t2 = CplxT<C>::make(a0.x - (R)0.5 * t1.x, a0.y - (R)0.5 * t1.y);
+                    const C dd = csub(a1, a2);
+                    const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
+                    const C* w = tws + p * 2;
+                    v[k][0] = cadd(a0, t1);
+                    v[k][1] = twiddle<SIGN>(cadd(t2, jd), w[0]);
+                    v[k][2] = twiddle<SIGN>(csub(t2, jd), w[1]);
+                }
             }
         }
         __syncthreads();
@@ -322,6 +315,47 @@ __device__ void lds_fft_inplace(C* buf, int nl, int ld, const FftDesc& d, const 
         s *= r;
         ncur = m;
     }
+}
+
+// One Stockham butterfly of radix r (2, 3 or 4): a_t = sl[t*step], outputs
+// v[u] = (sum_t a_t w_r^{t*u}) * tws[p*(r-1) + u-1] (u >= 1), returned by
+// value so that the callers' register arrays stay in registers
+template <typename C>
+struct Bfly4 {
+    C v[4];
+};
+template <int SIGN, typename C>
+__device__ __forceinline__ Bfly4<C> stockham_bfly(const C* sl, int step, int r, int p, const C* tws) {
+    using R = typename CplxT<C>::real;
+    Bfly4<C> o;
+    C* v = o.v;
+    if (r == 4) {
+        const C a0 = sl[0], a1 = sl[step], a2 = sl[2 * step], a3 = sl[3 * step];
+        const C s02 = cadd(a0, a2), d02 = csub(a0, a2);
+        const C s13 = cadd(a1, a3), d13 = csub(a1, a3);
+        const C jd13 = mul_si<SIGN>(d13);
+        const C* w = tws + p * 3;
+        v[0] = cadd(s02, s13);
+        v[1] = twiddle<SIGN>(cadd(d02, jd13), w[0]);
+        v[2] = twiddle<SIGN>(csub(s02, s13), w[1]);
+        v[3] = twiddle<SIGN>(csub(d02, jd13), w[2]);
+    } else if (r == 2) {
+        const C a0 = sl[0], a1 = sl[step];
+        v[0] = cadd(a0, a1);
+        v[1] = twiddle<SIGN>(csub(a0, a1), tws[p]);
+    } else {
+        const R s3 = (R)0.86602540378443864676372317075294;
+        const C a0 = sl[0], a1 = sl[step], a2 = sl[2 * step];
+        const C t1 = cadd(a1, a2);
+        const C t2 = CplxT<C>::make(a0.x - (R)0.5 * t1.x, a0.y - (R)0.5 * t1.y);
+        const C dd = csub(a1, a2);
+        const C jd = mul_si<SIGN>(CplxT<C>::make(s3 * dd.x, s3 * dd.y));
+        const C* w = tws + p * 2;
+        v[0] = cadd(a0, t1);
+        v[1] = twiddle<SIGN>(cadd(t2, jd), w[0]);
+        v[2] = twiddle<SIGN>(csub(t2, jd), w[1]);
+    }
+    return o;
 }
 
 // Transforms the columns c0 .. c0+nc-1 of a row-major LDS array (element e of
@@ -350,7 +384,11 @@ __device__ void lds_fft_cols(C* buf, int c0, int nc, int ld, const FftDesc& d, c
                 const int p = (int)d.sdiv[st].div((uint32_t)bi);
                 const int q = bi - p * s;
                 dsto[k] = (q + s * r * p) * ld + c;
-                stockham_bfly<SIGN>(buf + (q + s * p) * ld + c, s * m * ld, r, p, tws, v[k]);
+                const Bfly4<C> o = stockham_bfly<SIGN>(buf + (q + s * p) * ld + c, s * m * ld, r, p, tws);
+                v[k][0] = o.v[0];
+                v[k][1] = o.v[1];
+                v[k][2] = o.v[2];
+                v[k][3] = o.v[3];
             }
         }
         __syncthreads();

@@ -85,7 +85,7 @@ size_t x_lds_bytes(const dfp_plan* p, int inv, int f32) {
     const size_t cs = f32 ? sizeof(float2) : sizeof(double2);
     if (p->geo.xbc == 1)  // DCT-IV: M packed points per line, pre- and post-twiddles
         return cs * ((size_t)R * T * M + twiddle_count(p->fx) + 2 * M + (size_t)nz2 * (R - 1));
-    return cs * ((size_t)R * T * (M + inv) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (R - 1));
+    return cs * ((size_t)R * T * (M + 1) + twiddle_count(p->fx) + M + 1 + (size_t)nz2 * (R - 1));
 }
 
 static const char* xname(int mode, bool f32) {

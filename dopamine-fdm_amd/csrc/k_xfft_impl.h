@@ -87,6 +87,10 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
     const auto spec = gptr(q.spec);
     C* smem = reinterpret_cast<C*>(smem_raw);
     const int M = q.fd.n;
+    // lines padded to M+1 points: the unpack below reads the same mode of
+    // neighbouring columns in neighbouring lanes, which would otherwise all
+    // fall in one LDS bank
+    const int ld = M + 1;
     const int nxp = g.nxp;
     const int nz2 = g.nzp / R;
     const int k0 = bx * q.T;
@@ -94,7 +98,7 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
     const int nl = R * nt;
     const int ntw = q.ntw;
     C* A = smem;
-    C* twl = smem + nl * M;         // FFT stage twiddles
+    C* twl = smem + nl * ld;        // FFT stage twiddles
     C* rtl = twl + ntw;             // R2C unpack twiddles, M+1
     C* tzl = rtl + (M + 1);         // first z stage twiddles, nz2*(R-1)
     Rl* Ar = reinterpret_cast<Rl*>(A);
@@ -127,7 +131,7 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
             const int i = e - l * nxp;
             const int t = (int)q.div_t.div((uint32_t)l);
             const int k = k0 + (l - t * nt) + t * nz2;
-            dst[u] = e0 < total ? e : -1;
+            dst[u] = e0 < total ? l * 2 * ld + i : -1;
             cell[u] = i + j * pr_j + k * pr_k;
             if (MODE == 1) {
                 v[u] = arhs[cell[u]];
@@ -182,7 +186,7 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
     }
     __syncthreads();
     // tiles of up to 2048 * KX points (radix-2 stages paired): 64-byte FP32 and 128-byte FP64 spectrum segments
-    lds_fft_inplace<-1, KX, C, true>(A, nl, M, q.fd, twl);
+    lds_fft_inplace<-1, KX, C, true>(A, nl, ld, q.fd, twl);
 
     // unpack the R2C modes of the R lines of each column, apply the first z stage
     for (int pidx = threadIdx.x; pidx < (M + 1) * nt; pidx += kXThreads) {
@@ -191,7 +195,7 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
         C av[R];
 #pragma unroll
         for (int t = 0; t < R; ++t) {
-            const C* Z = A + (t * nt + c) * M;
+            const C* Z = A + (t * nt + c) * ld;
             const C zk = Z[kx == M ? 0 : kx];
             const C zm = Z[kx == 0 ? 0 : M - kx];
             const C E = CplxT<C>::make((Rl)0.5 * (zk.x + zm.x), (Rl)0.5 * (zk.y - zm.y));

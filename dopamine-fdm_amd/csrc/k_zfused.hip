@@ -17,6 +17,7 @@
 namespace dfp {
 
 constexpr int kZfThreads = 256;
+constexpr int kZfPyThreads = 512;  // periodic-y kernel: no serial sweeps, more waves to hide the plane loads
 
 // rows of the plane through the in-place LDS FFT, zp.G rows per call (the
 // per-call capacity of two butterflies per thread)
@@ -26,20 +27,20 @@ __device__ __forceinline__ void zf_rows_fft(C* A, const ZPar<C>& zp, const C* tw
     for (int l0 = 0; l0 < zp.nym; l0 += zp.G) lds_fft_inplace<SIGN, 2>(A + (long)l0 * n, min(zp.G, zp.nym - l0), n, zp.fz, twl);
 }
 
-// the plane (total values) into LDS: four values per thread in flight per
-// batch (named registers, clamped unconditional loads)
-template <typename C>
+// the plane (total values) into LDS by NT threads: four values per thread in
+// flight per batch (named registers, clamped unconditional loads)
+template <int NT, typename C>
 __device__ __forceinline__ void zf_load_plane(C* A, const Glob<C>* plane, int total) {
-    for (int base = 0; base < total; base += 4 * kZfThreads) {
+    for (int base = 0; base < total; base += 4 * NT) {
         const int e0 = base + (int)threadIdx.x;
         const C v0 = gld<C>(plane + min(e0, total - 1));
-        const C v1 = gld<C>(plane + min(e0 + kZfThreads, total - 1));
-        const C v2 = gld<C>(plane + min(e0 + 2 * kZfThreads, total - 1));
-        const C v3 = gld<C>(plane + min(e0 + 3 * kZfThreads, total - 1));
+        const C v1 = gld<C>(plane + min(e0 + NT, total - 1));
+        const C v2 = gld<C>(plane + min(e0 + 2 * NT, total - 1));
+        const C v3 = gld<C>(plane + min(e0 + 3 * NT, total - 1));
         if (e0 < total) A[e0] = v0;
-        if (e0 + kZfThreads < total) A[e0 + kZfThreads] = v1;
-        if (e0 + 2 * kZfThreads < total) A[e0 + 2 * kZfThreads] = v2;
-        if (e0 + 3 * kZfThreads < total) A[e0 + 3 * kZfThreads] = v3;
+        if (e0 + NT < total) A[e0 + NT] = v1;
+        if (e0 + 2 * NT < total) A[e0 + 2 * NT] = v2;
+        if (e0 + 3 * NT < total) A[e0 + 3 * NT] = v3;
     }
 }
 
@@ -60,7 +61,7 @@ __device__ __forceinline__ void zfused_body(const ZPar<C>& zp, int b, unsigned c
     double* sc = sg + 3 * kZfThreads;                        // chunk carries: kZfThreads x (re, im)
     const auto plane = gptr(zp.spec) + (long)b * nym * n;
     const int total = nym * n;
-    zf_load_plane(A, plane, total);
+    zf_load_plane<kZfThreads>(A, plane, total);
     const auto tw = gptr(zp.tw);
     const auto dl = gptr(zp.dl);
     const auto du = gptr(zp.du);
@@ -239,17 +240,17 @@ __device__ __forceinline__ void zf_py_body(const ZyPar<double2>& q, int b, unsig
     C* twy = twl + ntw;                // q.ntwy: y stages
     const auto plane = gptr(zp.spec) + (long)b * nym * n;
     const int total = nym * n;
-    zf_load_plane(A, plane, total);
+    zf_load_plane<kZfPyThreads>(A, plane, total);
     const auto tw = gptr(zp.tw);
     const auto tyg = gptr(q.twy);
-    for (int e = threadIdx.x; e < ntw; e += kZfThreads) twl[e] = gld<C>(tw + e);
-    for (int e = threadIdx.x; e < q.ntwy; e += kZfThreads) twy[e] = gld<C>(tyg + e);
+    for (int e = threadIdx.x; e < ntw; e += kZfPyThreads) twl[e] = gld<C>(tw + e);
+    for (int e = threadIdx.x; e < q.ntwy; e += kZfPyThreads) twy[e] = gld<C>(tyg + e);
     __syncthreads();
     zf_rows_fft<-1>(A, zp, twl);
     for (int c0 = 0; c0 < n; c0 += q.gy) lds_fft_cols<-1, 4>(A, c0, q.gy, n, q.fy, twy, q.div_gy);
     // element (ky, m) of the sub-plane is mode (kx, ky, kz = R m + u)
     const double lx = q.kxx[kx];
-    for (int e = threadIdx.x; e < nyp * n; e += kZfThreads) {
+    for (int e = threadIdx.x; e < nyp * n; e += kZfPyThreads) {
         const int ky = (int)zp.div_n.div((uint32_t)e);
         const int m = e - ky * n;
         const int kz = R * m + u;
@@ -264,13 +265,13 @@ __device__ __forceinline__ void zf_py_body(const ZyPar<double2>& q, int b, unsig
     }
     __syncthreads();
     for (int c0 = 0; c0 < n; c0 += q.gy) lds_fft_cols<1, 4>(A, c0, q.gy, n, q.fy, twy, q.div_gy);
-    for (int m = threadIdx.x; m < n; m += kZfThreads) A[(long)nyp * n + m] = A[m];
+    for (int m = threadIdx.x; m < n; m += kZfPyThreads) A[(long)nyp * n + m] = A[m];
     __syncthreads();
     zf_rows_fft<1>(A, zp, twl);
-    for (int e = threadIdx.x; e < total; e += kZfThreads) gst<C>(plane + e, A[e]);
+    for (int e = threadIdx.x; e < total; e += kZfPyThreads) gst<C>(plane + e, A[e]);
 }
 
-__global__ void __launch_bounds__(kZfThreads) k_zfused_py(ZyPar<double2> q) {
+__global__ void __launch_bounds__(kZfPyThreads) k_zfused_py(ZyPar<double2> q) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
     zf_py_body(q, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
 }
@@ -280,7 +281,7 @@ size_t zf_py_lds_bytes(int nym, int n, int ntwz, int ntwy) {
 }
 
 int zf_py_columns(const FftDesc& fy, int n) {
-    const int cap = inplace_capacity(fy, kZfThreads, 4);  // butterfly slots x smallest radix
+    const int cap = inplace_capacity(fy, kZfPyThreads, 4);  // butterfly slots x smallest radix
     for (int g = n; g >= 1; --g)
         if (n % g == 0 && (long)g * fy.n <= cap) return g;
     return 0;
@@ -300,7 +301,7 @@ hipError_t launch_zfused_py(dfp_plan* p) {
     q.inv_ny = 1.0 / (double)(p->geo.nym - 1);
     const size_t lds = zf_py_lds_bytes(p->geo.nym, p->fz.n, q.z.ntw, q.ntwy);
     const int slot = prof_begin(p, "zfused_py");
-    hipLaunchKernelGGL(k_zfused_py, dim3(q.z.nblocks), dim3(kZfThreads), lds, p->stream, q);
+    hipLaunchKernelGGL(k_zfused_py, dim3(q.z.nblocks), dim3(kZfPyThreads), lds, p->stream, q);
     prof_end(p, slot);
     return hipGetLastError();
 }

@@ -1,5 +1,6 @@
-!> Drop-in replacement of the reference's module projection (src/projection.f90) for y walls and z periodic
-!  with a periodic or an inflow/outflow x (x_bc_type 0 or 1, with the outlet backflow term) on one rank: the
+!> Drop-in replacement of the reference's module projection (src/projection.f90) for z periodic with y walls
+!  and a periodic or an inflow/outflow x (x_bc_type 0 or 1, with the outlet backflow term), or with y and x
+!  periodic (y_bc_type 0, the periodic box), on one rank: the
 !  same subroutine names and arguments, backed by the dfp HIP library through its C ABI.  The fields are the
 !  global-module arrays (global.f90:139-146, :188-197); they are copied to the device and back on each call.
 !  A caller that keeps them on the device passes device addresses with DFP_FIELDS_ON_DEVICE instead (see
@@ -43,9 +44,11 @@ Contains
   Subroutine ensure_plan
     Type(dfp_grid_t) :: g
     If ( c_associated(plan) ) Return
-    If ( ( x_bc_type /= 0 .And. x_bc_type /= 1 ) .Or. y_bc_type /= 1 .Or. z_bc_type /= 0 ) Then
-       Write(error_unit,'(A)') 'ERROR: the dfp projection supports x periodic or inflow/outflow, y walls, ' // &
-            'z periodic only'
+    ! the channel (either x type), or the periodic box (y_bc_type = 0 with a periodic x, initialization.f90:96-97)
+    If ( ( x_bc_type /= 0 .And. x_bc_type /= 1 ) .Or. ( y_bc_type /= 0 .And. y_bc_type /= 1 ) .Or. &
+         ( y_bc_type == 0 .And. x_bc_type /= 0 ) .Or. z_bc_type /= 0 ) Then
+       Write(error_unit,'(A)') 'ERROR: the dfp projection supports z periodic with y walls (x periodic or ' // &
+            'inflow/outflow) or with y and x periodic only'
        Error Stop 1
     End If
     gx = x;  gy = y;  gz = z;  gyg = yg;  gzg = zg
