@@ -300,6 +300,61 @@ def bench_periodic_y(K: int, Wm: int, n: int = 256) -> dict:
             "kernels_us": {k: round(1e3 * m / max(n_, 1), 2) for k, (n_, m) in prof.items()}}
 
 
+def substep_bytes(N: int, H: int, stage: int) -> dict:
+    """Per-substage bytes of the step around the projection: the RK3 update
+    reads u_old and `stage` right-hand sides and writes u (3 fields of about N
+    values each); the boundary conditions move ghost planes only (not
+    counted); the projection is the FP64 one."""
+    return {"rk_update": 3 * 8 * N * (stage + 2), **algorithmic_bytes(N, H)}
+
+
+def bench_substep(K: int, Wm: int, stage: int = 3) -> dict:
+    """One whole RK3 substage on the device (dfp_substep: update, boundary
+    conditions, projection, boundary conditions) at 256x192x256, the third
+    stage (three right-hand sides, P saved), no-slip walls (alpha = 0)."""
+    import dfp
+    from dfp import hipmem
+
+    g = dfp.channel_grid(NXP, NYM, NZP, ALPHA)
+    N, H = g.n_cells, g.n_modes
+    plan = dfp.Plan(g)
+    U, V, W, Uo, Vo, Wo = (dfp.DeviceArray(s) for s in (g.shape_u, g.shape_v, g.shape_w) * 2)
+    plan.synth(U, V, W, seed=0x5EED4000, sigma=SIGMA)
+    plan.synth(Uo, Vo, Wo, seed=0x5EED4001, sigma=SIGMA)
+    inner = ((g.nx - 2, g.nyg - 2, g.nzg - 2), (g.nxg - 2, g.ny - 2, g.nzg - 2), (g.nxg - 2, g.nyg - 2, g.nz - 2))
+    F = [[dfp.DeviceArray(sh, zero=True) for _ in range(3)] for sh in inner]
+    al = (dfp.DeviceArray((g.nx, 2, g.nzg), zero=True), dfp.DeviceArray((g.nxg, 2, g.nzg), zero=True),
+          dfp.DeviceArray((g.nxg, 2, g.nz), zero=True))
+    rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
+    P = dfp.DeviceArray(g.shape_p, zero=True)
+    coef = {1: (8 / 15, 0.0, 0.0), 2: (1 / 4, 5 / 12, 0.0), 3: (1 / 4, 0.0, 3 / 4)}[stage]  # rk_coef(stage, :)
+
+    def run(n):
+        for _ in range(n):
+            plan.substep(stage, U, V, W, Uo, Vo, Wo, *F, coef, *al, rhs, P, DT, save_p=True, asynchronous=True)
+
+    run(Wm)
+    hipmem.synchronize()
+    e0, e1 = hipmem.Event(), hipmem.Event()
+    e0.record()
+    run(K)
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_ms(e1)
+    plan.profile(True)
+    plan.profile_reset()
+    run(max(1, K // 5))
+    prof = plan.profile_read()
+    plan.profile(False)
+    plan.close()
+    us = 1e3 * ms / K
+    impl = sum(substep_bytes(N, H, stage).values())
+    gbs = impl / (us * 1e-6) / 1e9
+    return {"grid": [g.nxp, g.nym, g.nzp], "stage": stage, "us_per_substep": round(us, 2),
+            "impl_bytes": int(impl), "impl_GBs": round(gbs, 1), "impl_frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
+            "kernels_us": {k: round(1e3 * m / max(n, 1), 2) for k, (n, m) in prof.items()}}
+
+
 def form_gbs(bytes_per_projection: int, projections_per_s: float) -> dict:
     gbs = bytes_per_projection * projections_per_s / 1e9
     return {"B_model_bytes": int(bytes_per_projection), "achieved_GBs": round(gbs, 1),
@@ -479,6 +534,7 @@ def main() -> int:
     ap.add_argument("--no-serving", action="store_true", help="skip the serving-loop measurement")
     ap.add_argument("--no-inflow", action="store_true", help="skip the inflow/outflow x measurement")
     ap.add_argument("--no-periodic-y", action="store_true", help="skip the periodic-box measurement")
+    ap.add_argument("--no-substep", action="store_true", help="skip the whole-substage measurement")
     ap.add_argument("--no-host-staged", action="store_true", help="skip the host-staged (PCIe) measurement")
     ap.add_argument("--save-tuning", default=None, help="write the autotuner table (with new entries) here")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -605,6 +661,8 @@ def main() -> int:
         out["inflow_outflow"] = bench_inflow(max(2, K // 2), Wm)
     if not args.no_periodic_y:
         out["periodic_y"] = bench_periodic_y(max(2, K // 2), Wm)
+    if not args.no_substep:
+        out["substep"] = bench_substep(max(2, K), Wm)
     if not args.no_batched:
         out["batched"] = bench_batched(save_tuning=args.save_tuning)
     if not args.no_serving:
