@@ -3,10 +3,13 @@
  *
  * Replaces the Poisson/projection path of the reference's Fortran module
  * `projection` (src/projection.f90) and its GPU solver (src/poisson_gpu.f90)
- * for y walls and z periodic with either x boundary type, one rank:
+ * for z periodic, one rank:
  *   the channel, x periodic (x_bc_type = 0, y_bc_type = 1, z_bc_type = 0);
  *   the inflow/outflow x (x_bc_type = 1: DCT-IV in x, no mean-pressure pin,
- *   optional outlet backflow term, projection.f90:65-96, :126-129, :333-339).
+ *   optional outlet backflow term, projection.f90:65-96, :126-129, :333-339);
+ *   the periodic box (x_bc_type = 0, y_bc_type = 0 on a uniform y grid: a y
+ *   FFT over the nym-1 independent rows in place of the wall solve, the last
+ *   row a duplicate of the first, projection.f90:246-273, :350-354; FP64).
  *
  * All arrays are the reference's Fortran arrays, passed by address
  * (column-major, x fastest), with these extents (initialization.f90:204-220):
@@ -44,7 +47,7 @@ typedef struct dfp_grid {
     const double *z;        /* z faces  z(1:nz)                                     */
     const double *yg;       /* y centres with ghosts yg(1:nyg)                      */
     const double *zg;       /* z centres with ghosts zg(1:nzg)                      */
-    int32_t x_bc_type, y_bc_type, z_bc_type; /* x 0 or 1; y 1; z 0 in this build   */
+    int32_t x_bc_type, y_bc_type, z_bc_type; /* x 0 or 1; y 1 (or 0 with x 0); z 0 */
 } dfp_grid;
 
 /* Precision of the spectral solve.
@@ -180,8 +183,8 @@ int dfp_max_divergence(dfp_plan *plan, const double *U, const double *V, const d
 int dfp_max_divergence_noalloc(dfp_plan *plan, const double *U, const double *V, const double *W,
                                double *out, int flags);
 
-/* The step either side of the projection, for a channel plan (x_bc_type =
- * 0) and device-resident fields (flags must hold DFP_FIELDS_ON_DEVICE;
+/* The step either side of the projection, for a plan with a periodic x
+ * (x_bc_type = 0: the channel or the periodic box) and device-resident fields (flags must hold DFP_FIELDS_ON_DEVICE;
  * DFP_ASYNC allowed).  Fu, Fv, Fw are host arrays of `stage` device pointers
  * to the right-hand sides Fu_s(2:nx-1,2:nyg-1,2:nzg-1),
  * Fv_s(2:nxg-1,2:ny-1,2:nzg-1), Fw_s(2:nxg-1,2:nyg-1,2:nz-1)
@@ -192,7 +195,8 @@ int dfp_max_divergence_noalloc(dfp_plan *plan, const double *U, const double *V,
  * dfp_apply_bc: apply_boundary_conditions of the channel on one rank
  * (boundary_conditions.f90:186-260): periodic x and z ghost planes and the
  * Robin (slip-length) wall rows with alpha_x(nx,2,nzg), alpha_y(nxg,2,nzg),
- * alpha_z(nxg,2,nz) (global.f90:328; 0 = no slip).
+ * alpha_z(nxg,2,nz) (global.f90:328; 0 = no slip), or, for the periodic box,
+ * the periodic y ghost rows (:229-233, :762-781; the alphas may be NULL).
  * dfp_substep: one RK3 substage around the projection as time_integration.f90
  * runs it without immersed boundaries, VOF or scalars (:256-304): update,
  * boundary conditions, dfp_project with c = rk_coef(stage,stage) (P saved
