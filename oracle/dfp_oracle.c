@@ -342,6 +342,10 @@ struct ora_plan {
     double *kxx, *kzz;
     double *dyy_l, *dyy_d, *dyy_u; /* Dyy(j+1,j), Dyy(j,j), Dyy(j,j+1) */
     fplan fx, fz;
+    int zbc;          /* z_bc_type: 1 = the duct's z walls */
+    double *lam_z;    /* duct: eigenvalues of the z-wall operator, ascending (nzp) */
+    double *Qz;       /* duct: orthonormal eigenvectors of S, column m at Qz + m*nzp */
+    double *sqrt_w;   /* duct: square roots of the z cell widths */
     cplx *work; /* poisson_y_c at np = 1: nxp*nym*nzp */
     int nthreads;
 };
@@ -431,18 +435,122 @@ static void wavenumbers(int n, double d, double *k) {
     for (int i = half + 1; i <= n - 1; i++) k[i] = 2.0 * (cos(2.0 * pi * (double)(-n + i) / (double)n) - 1.0) / pow(d, 2.0);
 }
 
+/* Cyclic Jacobi eigenvalue iteration for a dense symmetric n x n matrix a
+   (column-major, overwritten): eigenvalues in d, ascending, and orthonormal
+   eigenvectors in the columns of v.  Row-cyclic sweeps of symmetric Schur
+   rotations (Golub & Van Loan, Matrix Computations, Algorithms 8.4.1-8.4.3)
+   until the off-diagonal part vanishes against the diagonal. */
+static void jacobi_eigen(int n, double *a, double *d, double *v) {
+#define A_(i, j) a[(size_t)(j) * n + (i)]
+#define V_E(i, j) v[(size_t)(j) * n + (i)]
+    for (int j = 0; j < n; j++)
+        for (int i = 0; i < n; i++) V_E(i, j) = i == j ? 1.0 : 0.0;
+    for (int sweep = 0; sweep < 100; sweep++) {
+        double off = 0.0, diag = 0.0;
+        for (int j = 0; j < n; j++)
+            for (int i = 0; i < n; i++) {
+                if (i == j) diag += A_(i, j) * A_(i, j);
+                else off += A_(i, j) * A_(i, j);
+            }
+        if (off <= 1e-34 * diag) break;
+        for (int p = 0; p < n - 1; p++)
+            for (int q = p + 1; q < n; q++) {
+                const double apq = A_(p, q);
+                if (apq == 0.0) continue;
+                const double tau = (A_(q, q) - A_(p, p)) / (2.0 * apq);
+                const double t = tau >= 0.0 ? 1.0 / (tau + sqrt(1.0 + tau * tau)) : -1.0 / (-tau + sqrt(1.0 + tau * tau));
+                const double c = 1.0 / sqrt(1.0 + t * t), s = t * c;
+                for (int k = 0; k < n; k++) {  /* A <- A J */
+                    const double akp = A_(k, p), akq = A_(k, q);
+                    A_(k, p) = c * akp - s * akq;
+                    A_(k, q) = s * akp + c * akq;
+                }
+                for (int k = 0; k < n; k++) {  /* A <- J^T A */
+                    const double apk = A_(p, k), aqk = A_(q, k);
+                    A_(p, k) = c * apk - s * aqk;
+                    A_(q, k) = s * apk + c * aqk;
+                }
+                for (int k = 0; k < n; k++) {  /* V <- V J */
+                    const double vkp = V_E(k, p), vkq = V_E(k, q);
+                    V_E(k, p) = c * vkp - s * vkq;
+                    V_E(k, q) = s * vkp + c * vkq;
+                }
+            }
+    }
+    for (int i = 0; i < n; i++) d[i] = A_(i, i);
+    /* ascending order, eigenvectors along */
+    for (int i = 0; i < n - 1; i++) {
+        int m = i;
+        for (int j = i + 1; j < n; j++)
+            if (d[j] < d[m]) m = j;
+        if (m != i) {
+            const double t = d[i];
+            d[i] = d[m];
+            d[m] = t;
+            for (int k = 0; k < n; k++) {
+                const double w = V_E(k, i);
+                V_E(k, i) = V_E(k, m);
+                V_E(k, m) = w;
+            }
+        }
+    }
+#undef A_
+#undef V_E
+}
+
+/* initialization.f90:545-580 (Dzz, Dyy's construction on z, zg) and
+   :586-611: S = W^(1/2) Dzz W^(-1/2) with W the z cell widths, its
+   eigenvalues lambda_z and orthonormal eigenvectors Qz, and sqrt_w */
+static void duct_eigenbasis(ora_plan *p) {
+    const int n = p->nzp;
+    const double *z = p->z, *zg = p->zg;
+    double *S = (double *)calloc((size_t)n * n, sizeof(double));
+    p->lam_z = (double *)malloc(sizeof(double) * (size_t)n);
+    p->Qz = (double *)malloc(sizeof(double) * (size_t)n * n);
+    p->sqrt_w = (double *)malloc(sizeof(double) * (size_t)n);
+    for (int k = 0; k < n; k++) p->sqrt_w[k] = sqrt(z[k + 1] - z[k]);
+#define ZF(k) z[(k) - 1]
+#define ZG(k) zg[(k) - 1]
+    for (int k = 2; k <= p->nzg - 1; k++) {
+        const double a = 1.0 / (ZF(k) - ZF(k - 1)) / (ZG(k + 1) - ZG(k));
+        const double b = 1.0 / (ZF(k) - ZF(k - 1)) * (-1.0 / (ZG(k + 1) - ZG(k)) - 1.0 / (ZG(k) - ZG(k - 1)));
+        const double c = 1.0 / (ZF(k) - ZF(k - 1)) / (ZG(k) - ZG(k - 1));
+        const int kk = k - 2;
+        double dkk;
+        if (k == 2 && k == p->nzg - 1) dkk = a + b;
+        else if (k == 2) dkk = b + c;
+        else if (k == p->nzg - 1) dkk = a + b;
+        else dkk = b;
+        S[(size_t)kk * n + kk] = dkk;
+        if (kk + 1 < n) {
+            /* S(k,k+1) = S(k+1,k) = 1 / ((zg(k+2)-zg(k+1)) sqrt(w(k) w(k+1))), 1-based cell k */
+            const double off = 1.0 / ((ZG(k + 1) - ZG(k)) * sqrt((ZF(k) - ZF(k - 1)) * (ZF(k + 1) - ZF(k))));
+            S[(size_t)(kk + 1) * n + kk] = off;
+            S[(size_t)kk * n + kk + 1] = off;
+        }
+    }
+#undef ZF
+#undef ZG
+    jacobi_eigen(n, S, p->lam_z, p->Qz);
+    free(S);
+}
+
 ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
     if (g->nxp < 1 || g->nym < 1 || g->nzp < 1) return NULL;
     if (g->x_bc_type != 0 && g->x_bc_type != 1) return NULL;
     if (g->y_bc_type != 0 && g->y_bc_type != 1) return NULL;
     /* periodic y needs periodic x (initialization.f90:96-97) and two rows */
     if (g->y_bc_type == 0 && (g->x_bc_type != 0 || g->nym < 2)) return NULL;
+    /* z walls: the 4-wall duct with a periodic x (initialization.f90:98-100) */
+    if (g->z_bc_type != 0 && g->z_bc_type != 1) return NULL;
+    if (g->z_bc_type == 1 && (g->x_bc_type != 0 || g->y_bc_type != 1)) return NULL;
     ora_plan *p = (ora_plan *)calloc(1, sizeof(ora_plan));
     p->xbc = g->x_bc_type;
     p->ybc = g->y_bc_type;
+    p->zbc = g->z_bc_type;
     p->nxp = g->nxp; p->nym = g->nym; p->nzp = g->nzp;
-    p->nx = g->nxp + (p->xbc ? 1 : 2); p->ny = g->nym + 1; p->nz = g->nzp + 2;
-    p->nxg = p->nx + 1; p->nyg = g->nym + 2; p->nzg = g->nzp + 3;
+    p->nx = g->nxp + (p->xbc ? 1 : 2); p->ny = g->nym + 1; p->nz = g->nzp + (p->zbc ? 1 : 2);
+    p->nxg = p->nx + 1; p->nyg = g->nym + 2; p->nzg = p->nz + 1;
     p->dx = g->dx; p->dz = g->dz;
     p->x = dup_array(g->x, p->nx);
     p->y = dup_array(g->y, p->ny);
@@ -494,6 +602,7 @@ ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
         wavenumbers(p->nym - 1, p->y[1] - p->y[0], p->kyy);
         fplan_init(&p->fy, p->nym - 1);
     }
+    if (p->zbc) duct_eigenbasis(p);
     p->work = (cplx *)malloc(sizeof(cplx) * (size_t)p->nxp * p->nym * p->nzp);
 #ifdef _OPENMP
     p->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads();
@@ -511,6 +620,7 @@ void ora_plan_destroy(ora_plan *p) {
     free(p->Pb); free(p->phib);
     fplan_free(&p->fx); fplan_free(&p->fz);
     if (!p->ybc) { free(p->kyy); fplan_free(&p->fy); }
+    if (p->zbc) { free(p->lam_z); free(p->Qz); free(p->sqrt_w); }
     free(p->work);
     free(p);
 }
@@ -597,6 +707,7 @@ static void apply_periodic_xz_pressure(const ora_plan *p, double *rhs_p) {
     if (!p->xbc)
         for (int k = 2; k <= p->nzg; k++)
             for (int j = 2; j <= p->nyg - 1; j++) R_(p->nxg - 1, j, k) = R_(2, j, k);
+    if (p->zbc) return; /* z walls: no z fill (:477-497) */
     for (int j = 2; j <= p->nyg - 1; j++)
         for (int i = 2; i <= p->nxg - 1; i++) R_(i, j, p->nzp + 2) = R_(i, j, 2);
 }
@@ -623,10 +734,79 @@ static void finish_pressure_ghosts(const ora_plan *p, double *P) {
         }
     for (int j = 1; j <= nyg; j++)
         for (int i = 1; i <= nxg; i++) {
-            P_(i, j, 1) = P_(i, j, nzg - 2);
-            P_(i, j, nzg - 1) = P_(i, j, 2);
-            P_(i, j, nzg) = P_(i, j, 3);
+            if (p->zbc) {
+                /* :430-437 Neumann copies at the z walls */
+                P_(i, j, 1) = P_(i, j, 2);
+                P_(i, j, nzg) = P_(i, j, nzg - 1);
+            } else {
+                P_(i, j, 1) = P_(i, j, nzg - 2);
+                P_(i, j, nzg - 1) = P_(i, j, 2);
+                P_(i, j, nzg) = P_(i, j, 3);
+            }
         }
+}
+
+/* projection.f90:162-206, the duct: per x mode i, z_hat = (p * sqrt_w) Qz,
+   one Zgtsv per (i, m) with kxx(i) + lambda_z(m) (the pin where i = 0 and
+   |lambda_z(m)| < 1e-8 max|lambda_z|), p = z_hat Qz^T / sqrt_w */
+static int duct_solve(ora_plan *p, cplx *wk) {
+    const int nxp = p->nxp, nym = p->nym, nzm = p->nzp;
+    double lmax = 0.0;
+    for (int m = 0; m < nzm; m++) lmax = fmax(lmax, fabs(p->lam_z[m]));
+    int info_all = 0;
+#pragma omp parallel num_threads(p->nthreads)
+    {
+        cplx *zh = (cplx *)malloc(sizeof(cplx) * (size_t)nym * nzm);
+        cplx *b = (cplx *)malloc(sizeof(cplx) * (size_t)nym);
+        double *DL = (double *)malloc(sizeof(double) * (size_t)(nym > 1 ? nym - 1 : 1));
+        double *DU = (double *)malloc(sizeof(double) * (size_t)(nym > 1 ? nym - 1 : 1));
+        double *D = (double *)malloc(sizeof(double) * (size_t)nym);
+#pragma omp for schedule(static)
+        for (int i = 0; i < nxp; i++) {
+            for (int j = 0; j < nym; j++)
+                for (int m = 0; m < nzm; m++) {
+                    cplx acc = {0.0, 0.0};
+                    for (int k = 0; k < nzm; k++) {
+                        const cplx v = wk[((size_t)k * nym + j) * nxp + i];
+                        const double q = p->Qz[(size_t)m * nzm + k];
+                        acc.re += v.re * p->sqrt_w[k] * q;
+                        acc.im += v.im * p->sqrt_w[k] * q;
+                    }
+                    zh[(size_t)m * nym + j] = acc;
+                }
+            for (int m = 0; m < nzm; m++) {
+                const double ws = p->kxx[i] + p->lam_z[m];
+                for (int j = 0; j < nym - 1; j++) {
+                    DL[j] = p->dyy_l[j];
+                    DU[j] = p->dyy_u[j];
+                }
+                for (int j = 0; j < nym; j++) D[j] = p->dyy_d[j] + ws;
+                if (i == 0 && fabs(p->lam_z[m]) < 1e-8 * lmax) D[0] = 3.0 / 2.0 * D[0];
+                for (int j = 0; j < nym; j++) b[j] = zh[(size_t)m * nym + j];
+                const int info = zgtsv_real_coef(nym, DL, D, DU, b);
+                if (info != 0) {
+#pragma omp atomic write
+                    info_all = info;
+                }
+                for (int j = 0; j < nym; j++) zh[(size_t)m * nym + j] = b[j];
+            }
+            for (int j = 0; j < nym; j++)
+                for (int k = 0; k < nzm; k++) {
+                    cplx acc = {0.0, 0.0};
+                    for (int m = 0; m < nzm; m++) {
+                        const cplx v = zh[(size_t)m * nym + j];
+                        const double q = p->Qz[(size_t)m * nzm + k];
+                        acc.re += v.re * q;
+                        acc.im += v.im * q;
+                    }
+                    cplx *o = wk + ((size_t)k * nym + j) * nxp + i;
+                    o->re = acc.re / p->sqrt_w[k];
+                    o->im = acc.im / p->sqrt_w[k];
+                }
+        }
+        free(zh); free(b); free(DL); free(DU); free(D);
+    }
+    return info_all;
 }
 
 int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
@@ -658,9 +838,13 @@ int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
         /* :125 C2C forward along x, batch nym*nzp, contiguous lines */
         fft_many(&p->fx, wk, (long)nym * nzp, 1, nxp, -1, p->nthreads);
     }
+    int info_all = 0;
+    if (p->zbc) {
+        info_all = duct_solve(p, wk);
+        goto inverse;
+    }
     /* :134-136 C2C forward along z, stride nxp*nym, batch nxp*nym */
     fft_many(&p->fz, wk, (long)nxp * nym, (long)nxp * nym, 1, -1, p->nthreads);
-    int info_all = 0;
     if (!p->ybc) {
         /* periodic y (:246-273): per z slab, C2C forward along y over the nyp =
            nym-1 independent rows (stride nxp, batch nxp), divide by
@@ -722,8 +906,8 @@ int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
         free(DL); free(DU); free(D); free(b);
     }
 inverse:
-    /* :308-310 inverse z */
-    fft_many(&p->fz, wk, (long)nxp * nym, (long)nxp * nym, 1, +1, p->nthreads);
+    /* :308-310 inverse z (the duct solved z in physical space) */
+    if (!p->zbc) fft_many(&p->fz, wk, (long)nxp * nym, (long)nxp * nym, 1, +1, p->nthreads);
     if (p->xbc) {
         /* :333 Real(), :334 REDFT11, :336 /(2*nxp*nzp); :342 back into rhs_p */
         double *xr = (double *)malloc(sizeof(double) * (size_t)nxp * nym * nzp);
@@ -737,9 +921,9 @@ inverse:
                 for (int i = 0; i < nxp; i++) R_(i + 2, j + 2, k + 2) = xr[((size_t)k * nym + j) * nxp + i] / norm;
         free(xr);
     } else {
-        /* :325 inverse x; :327 Real()/(nxp*nzp); :342 back into rhs_p */
+        /* :325 inverse x; :327 Real()/(nxp*nzp), the duct Real()/nxp (:330); :342 back into rhs_p */
         fft_many(&p->fx, wk, (long)nym * nzp, 1, nxp, +1, p->nthreads);
-        const double norm = (double)((long)nxp * nzp);
+        const double norm = p->zbc ? (double)nxp : (double)((long)nxp * nzp);
 #pragma omp parallel for schedule(static) num_threads(p->nthreads)
         for (int k = 0; k < nzp; k++)
             for (int j = 0; j < nym; j++)

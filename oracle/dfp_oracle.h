@@ -6,9 +6,9 @@
  * the checker and as the host-CPU baseline.
  *
  * It restates the reference's CPU (FFTW + LAPACK) branch of the projection
- * for the channel case (x periodic, y walls, z periodic) and its x
- * inflow/outflow branch on one rank; every function cites the reference lines
- * it follows in dfp_oracle.c.
+ * for the channel case (x periodic, y walls, z periodic), its x
+ * inflow/outflow, periodic-y and 4-wall duct branches on one rank; every
+ * function cites the reference lines it follows in dfp_oracle.c.
  *
  * Arrays are the reference's Fortran arrays by address (column-major, x
  * fastest):
@@ -34,6 +34,8 @@ typedef struct ora_grid {
     const double *zg;     /* centres zg(1:nzg) */
     int x_bc_type;        /* 0 periodic; 1 inflow/outflow (nx = nxp+1, nxg = nxp+2) */
     int y_bc_type;        /* 1 walls; 0 periodic (uniform y, nym-1 independent rows) */
+    int z_bc_type;        /* 0 periodic; 1 walls (the 4-wall duct: x periodic, y walls;
+                             nzp = nzm cells, nz = nzp+1, nzg = nzp+2) */
 } ora_grid;
 
 typedef struct ora_plan ora_plan;

@@ -17,7 +17,7 @@ class _Grid(ctypes.Structure):
         ("nxp", ctypes.c_int), ("nym", ctypes.c_int), ("nzp", ctypes.c_int),
         ("dx", ctypes.c_double), ("dz", ctypes.c_double),
         ("x", _dp), ("y", _dp), ("z", _dp), ("yg", _dp), ("zg", _dp),
-        ("x_bc_type", ctypes.c_int), ("y_bc_type", ctypes.c_int),
+        ("x_bc_type", ctypes.c_int), ("y_bc_type", ctypes.c_int), ("z_bc_type", ctypes.c_int),
     ]
 
 
@@ -80,7 +80,7 @@ class CProjection:
         gd = _Grid(g.nxp, g.nym, g.nzp, g.dx, g.dz,
                    x.ctypes.data_as(_dp), y.ctypes.data_as(_dp), z.ctypes.data_as(_dp),
                    yg.ctypes.data_as(_dp), zg.ctypes.data_as(_dp), int(getattr(g, "x_bc_type", 0)),
-                   int(getattr(g, "y_bc_type", 1)))
+                   int(getattr(g, "y_bc_type", 1)), int(getattr(g, "z_bc_type", 0)))
         self._p = lib().ora_plan_create(ctypes.byref(gd), int(nthreads))
         if not self._p:
             raise ValueError("ora_plan_create failed")

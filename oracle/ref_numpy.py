@@ -34,6 +34,20 @@ and the x inflow/outflow branch (``x_bc_type=1``, y walls, z periodic):
 * Neumann x ghosts of P ....... src/projection.f90:413-422
 * outflow face correction ..... src/projection.f90:578-589
 
+and the 4-wall duct (``x_bc_type=0, y_bc_type=1, z_bc_type=1``):
+
+* grid: all nzm cells kept ..... src/initialization.f90:339-345 (nzp = nzm), z faces uniform or
+                                symmetric tanh (alpha_grid_z > 0), src/genGridandIC.f90:27-40
+* Dzz ......................... src/initialization.f90:545-580 (Dyy's construction on z, zg)
+* z eigenbasis ................ src/initialization.f90:586-618: S = W^(1/2) Dzz W^(-1/2)
+                                (W = cell widths), LAPACK dstev -> lambda_z, Qz
+* solve ....................... src/projection.f90:162-206 (CPU duct branch): x FFT, per kx
+                                z_hat = (p * sqrt_w) Qz, one Zgtsv per (kx, m) with
+                                kxx + lambda_z(m) and the pin where kx = 0 and
+                                |lambda_z(m)| < 1e-8 max|lambda_z|, p = z_hat Qz^T / sqrt_w;
+                                inverse x with Real()/nxp (:326-331)
+* no z periodic fill .......... src/projection.f90:477-497; Neumann z ghosts of P (:430-437)
+
 LAPACK ``zgtsv`` (unpinned LAPACK version, CMakeLists.txt:50-51; the routine's
 algorithm is unchanged since LAPACK 3.0) is restated from its published
 algorithm: Gaussian elimination with partial pivoting on the tridiagonal,
@@ -101,6 +115,8 @@ class ChannelGrid:
     grid_type: int = 2
     x_bc_type: int = 0   # 0 periodic, 1 inflow/outflow (global.f90:185-186)
     y_bc_type: int = 1   # 1 walls, 0 periodic (uniform y only, initialization.f90:128-129)
+    z_bc_type: int = 0   # 0 periodic, 1 walls (the 4-wall duct with y walls and a periodic x)
+    alpha_z: float = 0.0  # z walls: symmetric tanh stretching of the z faces (alpha_grid_z, 0 = uniform)
     x: np.ndarray = field(init=False)
     y: np.ndarray = field(init=False)
     z: np.ndarray = field(init=False)
@@ -114,10 +130,19 @@ class ChannelGrid:
         if self.y_bc_type == 0 and (self.grid_type != 1 or self.x_bc_type != 0 or self.nym < 2):
             # initialization.f90:96-97, :128-129
             raise ValueError("periodic y needs a uniform y grid, a periodic x and nym >= 2")
+        if self.z_bc_type == 1 and (self.x_bc_type != 0 or self.y_bc_type != 1):
+            # the duct: initialization.f90:98-100 (GPU builds), y walls here
+            raise ValueError("z walls are restated for the duct: periodic x and y walls")
         nx, ny, nz = self.nx, self.ny, self.nz
         # genGridandIC.f90:23-25, :38-40  x(i) = Lx*(i-1)/(nx-1)
         self.x = np.array([self.Lx_i * float(i - 1) / float(nx - 1) for i in range(1, nx + 1)])
-        self.z = np.array([self.Lz_i * float(i - 1) / float(nz - 1) for i in range(1, nz + 1)])
+        if self.z_bc_type == 1 and self.alpha_z > 0.0:
+            # genGridandIC.f90:30-35 symmetric tanh in z
+            t_half = math.tanh(0.5 * self.alpha_z)
+            self.z = np.array([self.Lz_i * 0.5 * (1.0 + math.tanh(self.alpha_z * (float(i - 1) / float(nz - 1) - 0.5))
+                                                 / t_half) for i in range(1, nz + 1)])
+        else:
+            self.z = np.array([self.Lz_i * float(i - 1) / float(nz - 1) for i in range(1, nz + 1)])
         if self.grid_type == 1:
             # genGridandIC.f90:48-52
             self.y = np.array([self.Ly_i * float(i - 1) / float(ny - 1) for i in range(1, ny + 1)])
@@ -151,7 +176,8 @@ class ChannelGrid:
 
     @property
     def nz(self):
-        return self.nzp + 2
+        # z walls keep every cell (nzp = nzm, initialization.f90:339-345)
+        return self.nzp + 2 if self.z_bc_type == 0 else self.nzp + 1
 
     @property
     def nxg(self):
@@ -163,7 +189,7 @@ class ChannelGrid:
 
     @property
     def nzg(self):
-        return self.nzp + 3
+        return self.nz + 1
 
     # array shapes of the field contract (initialization.f90:204-220)
     @property
@@ -206,6 +232,13 @@ def periodic_y_grid(nxp, nym, nzp, Lx=2.0 * PI, Ly=2.0, Lz=PI):
     return ChannelGrid(nxp, nym, nzp, Lx * (nxp + 1) / nxp, Ly * nym / (nym - 1), Lz * (nzp + 1) / nzp, 0.0, 1, 0, 0)
 
 
+def duct_grid(nxp, nym, nzm, alpha, alpha_z=0.0, Lx=2.0 * PI, Ly=2.0, Lz=2.0):
+    """The 4-wall duct: periodic x of period Lx, y walls (tanh stretching
+    alpha), z walls Lz apart with nzm cells (uniform faces, or symmetric tanh
+    with alpha_z > 0)."""
+    return ChannelGrid(nxp, nym, nzm, Lx * (nxp + 1) / nxp, Ly, Lz, alpha, 2 if alpha > 0 else 1, 0, 1, 1, alpha_z)
+
+
 def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * PI, Lz=PI, Ly=2.0, grid_type=2, x_bc_type=0):
     """Channel grid whose periodic lengths are ``Lx`` and ``Lz``.
 
@@ -241,6 +274,27 @@ def wavenumbers_dct(n: int, d: float) -> np.ndarray:
     """initialization.f90:443-445: DCT-IV (quarter-wave) modified wavenumbers
     kxx(i) = 2 (cos(pi (2i+1) / (2n)) - 1) / d^2, i = 0..n-1."""
     return np.array([2.0 * (math.cos(PI * float(2 * i + 1) / (2.0 * float(n))) - 1.0) / d ** 2.0 for i in range(n)])
+
+
+def z_eigenbasis(z: np.ndarray, zg: np.ndarray):
+    """initialization.f90:586-611: the symmetric tridiagonal S = W^(1/2) Dzz
+    W^(-1/2) of the z-wall operator (W = diag of the cell widths z(k+1)-z(k);
+    Dzz is Dyy's construction on z, zg, initialization.f90:545-580), its
+    eigenvalues lambda_z (ascending) and orthonormal eigenvectors Qz[:, m]
+    (LAPACK dstev there, scipy's LAPACK-backed eigh_tridiagonal here), and
+    sqrt_w = sqrt(cell widths)."""
+    import scipy.linalg
+
+    Dzz = dyy_matrix(z, zg)
+    nzm = Dzz.shape[0]
+    cell_w = z[1:nzm + 1] - z[0:nzm]
+    sqrt_w = np.sqrt(cell_w)
+    diag = np.diag(Dzz).copy()
+    sub = np.array([1.0 / ((zg[k + 1] - zg[k]) * math.sqrt(cell_w[k - 1] * cell_w[k])) for k in range(1, nzm)])
+    if nzm == 1:
+        return diag.copy(), np.ones((1, 1)), sqrt_w
+    lam, Qz = scipy.linalg.eigh_tridiagonal(diag, sub)
+    return lam, Qz, sqrt_w
 
 
 def dyy_matrix(y: np.ndarray, yg: np.ndarray) -> np.ndarray:
@@ -357,6 +411,10 @@ class Projection:
         self.Dyy = dyy_matrix(g.y, g.yg)
         # periodic y: initialization.f90:481-489 (nyp = nym-1, dy = y(2)-y(1))
         self.kyy = wavenumbers(g.nym - 1, float(g.y[1] - g.y[0])) if getattr(g, "y_bc_type", 1) == 0 else None
+        self.duct = getattr(g, "z_bc_type", 0) == 1
+        if self.duct:
+            self.kzz = None
+            self.lambda_z, self.Qz, self.sqrt_w = z_eigenbasis(g.z, g.zg)
         # outlet backflow stabilisation (global.f90:188-197): off unless beta > 0
         self.beta = 0.0
         self.outflow_Uc = 0.0
@@ -421,6 +479,8 @@ class Projection:
         else:
             # :127-128 REDFT11 along x, then dcmplx
             c = scipy.fft.dct(ry, type=4, axis=0).astype(np.complex128)
+        if self.duct:
+            return self._duct_solve(c, rhs_p, P, dt, rk_coef33)
         # :134-136 complex FFT along z
         c = np.fft.fft(c, axis=2)
         if self.kyy is not None:
@@ -454,10 +514,46 @@ class Projection:
         c = np.transpose(X.reshape(nxp, nzp, nym, order="F"), (0, 2, 1))
         return self._inverse_xz(c, rhs_p, P, dt, rk_coef33)
 
+    def _duct_solve(self, c, rhs_p, P, dt, rk_coef33):
+        # projection.f90:162-206: per kx, z eigenmode transform, one Zgtsv
+        # per (kx, m) with kxx + lambda_z(m), the pin at kx = 0 on the zero
+        # eigenvalue (found by magnitude), inverse transform
+        g = self.g
+        nxp, nym, nzm = g.nxp, g.nym, g.nzp
+        lam = self.lambda_z
+        tol = 1e-8 * np.max(np.abs(lam))
+        diag = np.diag(self.Dyy)
+        dl1 = np.array([self.Dyy[jj + 1, jj] for jj in range(nym - 1)])
+        du1 = np.array([self.Dyy[jj, jj + 1] for jj in range(nym - 1)])
+        zh = np.einsum("ijk,km->ijm", c * self.sqrt_w[None, None, :], self.Qz)   # (nxp, nym, nzm)
+        M = nxp * nzm
+        D = diag[None, :] + (self.kxx[:, None] + lam[None, :]).reshape(M, order="F")[:, None]  # mode i + nxp*m
+        D = D.copy()
+        for m in range(nzm):
+            if abs(lam[m]) < tol:
+                D[0 + nxp * m, 0] = 3.0 / 2.0 * D[0 + nxp * m, 0]
+        B = np.transpose(zh, (0, 2, 1)).reshape(M, nym, order="F")
+        X, info = zgtsv(np.broadcast_to(dl1, (M, nym - 1)), D, np.broadcast_to(du1, (M, nym - 1)), B)
+        if np.any(info != 0):
+            raise FloatingPointError("zgtsv reported a zero pivot")
+        zs = np.transpose(X.reshape(nxp, nzm, nym, order="F"), (0, 2, 1))
+        c = np.einsum("ijm,km->ijk", zs, self.Qz) / self.sqrt_w[None, None, :]
+        return self._inverse_xz(c, rhs_p, P, dt, rk_coef33)
+
     def _inverse_xz(self, c, rhs_p, P, dt, rk_coef33):
         g = self.g
         nxp, nym, nzp = g.nxp, g.nym, g.nzp
         xdct = g.x_bc_type == 1
+        if self.duct:
+            # :326-331 z solved in physical space: Real()/nxp
+            c = np.fft.ifft(c, axis=0) * nxp
+            phi = c.real / float(nxp)
+            rhs_p[0:nxp, 0:nym, 0:nzp] = phi
+            self.apply_periodic_xz_pressure(rhs_p)
+            if P is not None:
+                P[1:g.nxg - 1, 1:g.nyg - 1, 1:g.nzg - 1] = rhs_p[0:g.nxg - 2, :, 0:g.nzg - 2] / (dt * rk_coef33)
+                self.finish_pressure_ghosts(P)
+            return
         # :308-310 inverse z
         c = np.fft.ifft(c, axis=2) * nzp
         if not xdct:
@@ -485,8 +581,9 @@ class Projection:
         if g.x_bc_type == 0:
             # :455 rhs_p(nxg-1,:,:) = rhs_p(2,:,:) (inflow/outflow x: nothing, :446-448)
             rhs_p[g.nxg - 1 - 2, :, :] = rhs_p[0, :, :]
-        # :481 rhs_p(2:nxg-1,:,nzp+2) = rhs_p(2:nxg-1,:,2)   (local nzp = nzm-1 = nzp_global)
-        rhs_p[0:g.nxg - 2, :, g.nzp + 2 - 2] = rhs_p[0:g.nxg - 2, :, 0]
+        if getattr(g, "z_bc_type", 0) == 0:
+            # :481 rhs_p(2:nxg-1,:,nzp+2) = rhs_p(2:nxg-1,:,2)   (local nzp = nzm-1 = nzp_global)
+            rhs_p[0:g.nxg - 2, :, g.nzp + 2 - 2] = rhs_p[0:g.nxg - 2, :, 0]
 
     # -- projection.f90:388-438 with boundary_conditions.f90:277-289, :591-602 (id=4)
     def finish_pressure_ghosts(self, Pf):
@@ -502,6 +599,11 @@ class Projection:
             # :413-422 Neumann copies at the inflow and outflow ends
             Pf[0, :, :] = Pf[1, :, :]
             Pf[g.nxg - 1, :, :] = Pf[g.nxg - 2, :, :]
+        if getattr(g, "z_bc_type", 0) == 1:
+            # :430-437 Neumann copies at the z walls
+            Pf[:, :, 0] = Pf[:, :, 1]
+            Pf[:, :, g.nzg - 1] = Pf[:, :, g.nzg - 2]
+            return
         Pf[:, :, 0] = Pf[:, :, g.nzg - 3]
         Pf[:, :, g.nzg - 2] = Pf[:, :, 1]
         Pf[:, :, g.nzg - 1] = Pf[:, :, 2]
