@@ -300,6 +300,49 @@ def bench_periodic_y(K: int, Wm: int, n: int = 256) -> dict:
             "kernels_us": {k: round(1e3 * m / max(n_, 1), 2) for k, (n_, m) in prof.items()}}
 
 
+def duct_bytes(N: int, H: int) -> dict:
+    """Per-projection bytes of the duct's kernels: the x kernels and the
+    correction as the channel's, each z eigenbasis GEMM reads and writes the
+    spectrum (its real matrix stays in L2), the y sweeps read and write it
+    twice and read the pivots (8 bytes per mode and row) twice."""
+    return {"div_rfft_x": 24 * N + 16 * H, "zeig_fwd": 32 * H, "duct_ysolve": 64 * H + 16 * H,
+            "zeig_inv": 32 * H, "irfft_x": 16 * H + 8 * N, "correct": 56 * N}
+
+
+def bench_duct(K: int, Wm: int) -> dict:
+    """The 4-wall duct at 256x192x256 cells (y and z walls, tanh stretching
+    2.0 in both): same RK3 loop as the FP64 line, fields resident in HBM.
+    The z eigenbasis GEMMs are FP64 compute: 2 x 4 x nxh*nym*nzm^2 flops."""
+    import dfp
+    from dfp import hipmem
+
+    g = dfp.duct_grid(NXP, NYM, NZP, ALPHA, ALPHA)
+    N, H = g.n_cells, g.n_modes
+    plan = dfp.Plan(g)
+    sets = make_sets(plan, g, 12, 0x5EED5000)
+    rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
+    P = dfp.DeviceArray(g.shape_p, zero=True)
+    hipmem.synchronize()
+    ms = time_steps(plan, sets, rhs, P, K, Wm)
+    plan.profile(True)
+    plan.profile_reset()
+    time_steps(plan, sets, rhs, P, max(1, K // 5), 0, first=Wm + K)
+    prof = plan.profile_read()
+    plan.profile(False)
+    plan.close()
+    rate = 3 * K / (ms / 1e3)
+    kus = {k: round(1e3 * m / max(n, 1), 2) for k, (n, m) in prof.items()}
+    flops = 4.0 * (g.nxp // 2 + 1) * g.nym * g.nzp * g.nzp  # one GEMM
+    gemm_us = kus.get("zeig_fwd", 0.0)
+    impl = sum(duct_bytes(N, H).values())
+    return {"grid": [g.nxp, g.nym, g.nzp], "projections_per_s": round(rate, 2),
+            "us_per_projection": round(1e6 / rate, 2), "impl_bytes": int(impl),
+            "impl_GBs": round(impl * rate / 1e9, 1),
+            "zeig_fwd_TFLOPs": round(flops / (gemm_us * 1e-6) / 1e12, 2) if gemm_us else None,
+            "zeig_frac_of_fp64_peak": round(flops / (gemm_us * 1e-6) / 78.6e12, 4) if gemm_us else None,
+            "kernels_us": kus}
+
+
 def substep_bytes(N: int, H: int, stage: int) -> dict:
     """Per-substage bytes of the step around the projection: the RK3 update
     reads u_old and `stage` right-hand sides and writes u (3 fields of about N
@@ -535,6 +578,7 @@ def main() -> int:
     ap.add_argument("--no-inflow", action="store_true", help="skip the inflow/outflow x measurement")
     ap.add_argument("--no-periodic-y", action="store_true", help="skip the periodic-box measurement")
     ap.add_argument("--no-substep", action="store_true", help="skip the whole-substage measurement")
+    ap.add_argument("--no-duct", action="store_true", help="skip the 4-wall duct measurement")
     ap.add_argument("--no-host-staged", action="store_true", help="skip the host-staged (PCIe) measurement")
     ap.add_argument("--save-tuning", default=None, help="write the autotuner table (with new entries) here")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
@@ -663,6 +707,8 @@ def main() -> int:
         out["periodic_y"] = bench_periodic_y(max(2, K // 2), Wm)
     if not args.no_substep:
         out["substep"] = bench_substep(max(2, K), Wm)
+    if not args.no_duct:
+        out["duct"] = bench_duct(max(2, K // 2), Wm)
     if not args.no_batched:
         out["batched"] = bench_batched(save_tuning=args.save_tuning)
     if not args.no_serving:

@@ -8,11 +8,14 @@
 #include <cmath>
 #include <cstdarg>
 #include <algorithm>
+#include <limits>
+#include <numeric>
 #include <cstdio>
 #include <cstring>
 #include <new>
 
 #include "fft_lds.h"
+#include "symtri_eigen.h"
 
 using namespace dfp;
 
@@ -180,6 +183,18 @@ int choose_shape(const dfp_grid* g, int f32, size_t lds_limit, const dfp_tuning*
     const int want_R = tune ? tune->z_split : 0, want_T = tune ? tune->x_columns : 0;
     const int want_G = tune ? tune->z_rows : 0, want_f = tune ? tune->z_fused : 0;
     if (want_R < 0 || want_T < 0 || want_G < 0 || want_f < 0 || want_f > 2) return DFP_E_ARG;
+    if (g->z_bc_type == 1) {
+        // the duct: no z transform (R = 1), the z eigenbasis kernels instead
+        if ((want_R > 1) || want_G > 0 || want_f > 0) return DFP_E_UNSUPPORTED;
+        Shape sh;
+        if (!factor(g->nxp / 2, sh.fx)) return DFP_E_UNSUPPORTED;
+        factor(1, sh.fz);
+        sh.T = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 0);
+        sh.Ti = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 1);
+        if (sh.T == 0 || sh.Ti == 0) return DFP_E_UNSUPPORTED;
+        out = sh;
+        return DFP_OK;
+    }
     if (want_R > 0) {
         bool known = false;
         for (int r : kRadices) known = known || r == want_R;
@@ -263,6 +278,8 @@ void free_plan_memory(dfp_plan* p) {
     (void)hipFree(p->d_pbar);
     (void)hipFree(p->d_spec);
     (void)hipFree(p->d_lu);
+    (void)hipFree(p->d_spec2);
+    (void)hipFree(p->d_duct);
     (void)hipFree(p->d_red);
     (void)hipFree(p->d_U);
     (void)hipFree(p->d_V);
@@ -344,6 +361,7 @@ int check_flags(dfp_plan* p, int flags, int extra = 0) {
 
 // z transforms and y solves of the spectrum: the fused kernel or the split pair
 hipError_t z_solve(dfp_plan* p, int f32) {
+    if (p->geo.zbc == 1) return launch_duct_solve(p);
     if (p->geo.ybc == 0) return launch_zfused_py(p);
     if (p->zfused) {
         if (f32) return launch_zfused<float2>(p, make_zpar<float2>(p, 1));
@@ -479,7 +497,10 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     *out = nullptr;
     if (!g || !g->y || !g->z || !g->yg || !g->zg) return DFP_E_ARG;
     if (g->x_bc_type != 0 && g->x_bc_type != 1) return DFP_E_UNSUPPORTED;
-    if ((g->y_bc_type != 0 && g->y_bc_type != 1) || g->z_bc_type != 0) return DFP_E_UNSUPPORTED;
+    if ((g->y_bc_type != 0 && g->y_bc_type != 1) || (g->z_bc_type != 0 && g->z_bc_type != 1)) return DFP_E_UNSUPPORTED;
+    // z walls: the 4-wall duct with a periodic x and y walls (initialization.f90:98-100), FP64
+    if (g->z_bc_type == 1 && (g->x_bc_type != 0 || g->y_bc_type != 1 || precision != DFP_FP64))
+        return DFP_E_UNSUPPORTED;
     // periodic y (initialization.f90:96-97, :128-129): periodic x, a uniform
     // y grid and nym-1 = 2^a 3^b independent rows; solved in FP64
     if (g->y_bc_type == 0 && (g->x_bc_type != 0 || precision != DFP_FP64 || g->nym < 2))
@@ -505,7 +526,7 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     {
         // FFT lengths 2^a 3^b, checked before the device is touched
         FftDesc probe;
-        if (!factor(g->nxp / 2, probe) || !factor(g->nzp, probe)) return bail(DFP_E_UNSUPPORTED);
+        if (!factor(g->nxp / 2, probe) || (g->z_bc_type == 0 && !factor(g->nzp, probe))) return bail(DFP_E_UNSUPPORTED);
         if (g->y_bc_type == 0) {
             if (!factor(g->nym - 1, probe)) return bail(DFP_E_UNSUPPORTED);
             // the reference takes dy = y(2) - y(1) for every row (:481-489)
@@ -537,13 +558,15 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     const bool dct = g->x_bc_type == 1;
     G.xbc = g->x_bc_type;
     G.ybc = g->y_bc_type;
+    G.zbc = g->z_bc_type;
     G.nxp = g->nxp; G.nym = g->nym; G.nzp = g->nzp;
     // the periodic x keeps the R2C half of its modes (and drops the duplicate
     // cell, nxp = nxm - 1); the inflow/outflow x keeps all nxm = nxp cells and
     // all nxp DCT-IV modes (initialization.f90:331-338)
     G.nxh = dct ? g->nxp : g->nxp / 2 + 1;
-    G.nx = g->nxp + (dct ? 1 : 2); G.ny = g->nym + 1; G.nz = g->nzp + 2;
-    G.nxg = G.nx + 1; G.nyg = g->nym + 2; G.nzg = g->nzp + 3;
+    // z walls keep all nzm = nzp cells (initialization.f90:339-345)
+    G.nx = g->nxp + (dct ? 1 : 2); G.ny = g->nym + 1; G.nz = g->nzp + (G.zbc ? 1 : 2);
+    G.nxg = G.nx + 1; G.nyg = g->nym + 2; G.nzg = G.nz + 1;
     p->dx = g->dx; p->dz = g->dz;
     G.inv_dx = 1.0 / g->dx;
     if (g->x) p->hx.assign(g->x, g->x + G.nx);
@@ -569,6 +592,40 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     // spacing y(2) - y(1) (initialization.f90:481-489)
     std::vector<double> hkyy;
     if (G.ybc == 0) wavenumbers(G.nym - 1, p->hy[1] - p->hy[0], hkyy);
+    // the duct (initialization.f90:544-611): S = W^(1/2) Dzz W^(-1/2), its
+    // eigenvalues lambda_z and eigenvectors Qz; Qf = W^(1/2) Qz takes a plane
+    // to the eigenbasis, Qi = Qz^T W^(-1/2) back; the eigenmode of the zero
+    // eigenvalue (by magnitude, :207) is pinned at kx = 0
+    std::vector<double> duct_tab;
+    if (G.zbc == 1) {
+        const int n = G.nzp;
+        auto Z = [&](int k) { return p->hz[k - 1]; };
+        auto ZG = [&](int k) { return p->hzg[k - 1]; };
+        std::vector<double> sd(n), so(n > 1 ? n - 1 : 1, 0.0), sw(n), Q;
+        for (int k = 2; k <= G.nzg - 1; ++k) {
+            const double a = 1.0 / (Z(k) - Z(k - 1)) / (ZG(k + 1) - ZG(k));
+            const double b = 1.0 / (Z(k) - Z(k - 1)) * (-1.0 / (ZG(k + 1) - ZG(k)) - 1.0 / (ZG(k) - ZG(k - 1)));
+            const double c = 1.0 / (Z(k) - Z(k - 1)) / (ZG(k) - ZG(k - 1));
+            const int kk = k - 2;
+            sd[kk] = (k == 2 && k == G.nzg - 1) ? a + b : (k == 2 ? b + c : (k == G.nzg - 1 ? a + b : b));
+            if (kk + 1 < n) so[kk] = 1.0 / ((ZG(k + 1) - ZG(k)) * std::sqrt((Z(k) - Z(k - 1)) * (Z(k + 1) - Z(k))));
+        }
+        for (int k = 0; k < n; ++k) sw[k] = std::sqrt(Z(k + 2) - Z(k + 1));
+        if (!std::isfinite(std::accumulate(sd.begin(), sd.end(), 0.0)) || !symtri_eigen(n, sd, so, Q))
+            return bail(DFP_E_NONFINITE);
+        double lmax = 0.0;
+        for (double l : sd) lmax = std::max(lmax, std::fabs(l));
+        for (int m = 0; m < n && p->pin_m < 0; ++m)
+            if (std::fabs(sd[m]) < 1e-8 * lmax) p->pin_m = m;
+        duct_tab.resize((size_t)2 * n * n + n);
+        for (int k = 0; k < n; ++k)
+            for (int m = 0; m < n; ++m) {
+                const double q = Q[(size_t)m * n + k];
+                duct_tab[(size_t)k * n + m] = sw[k] * q;                    // Qf(k, m)
+                duct_tab[(size_t)n * n + (size_t)m * n + k] = q / sw[k];    // Qi(m, k)
+            }
+        std::copy(sd.begin(), sd.end(), duct_tab.begin() + (size_t)2 * n * n);
+    }
 
     std::vector<double> tab;
     auto put = [&](const std::vector<double>& v) {
@@ -647,7 +704,19 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
 
     const size_t H = (size_t)G.nxh * G.nym * G.nzp;
     if (hipMalloc(&p->d_spec, sizeof(double2) * H) != hipSuccess) return bail(DFP_E_NOMEM);
-    if (G.ybc == 1) {  // pivots of the wall solve
+    if (G.zbc == 1) {  // the duct: eigenbasis tables, second spectrum, pivots per (kx, eigenmode)
+        if (hipMalloc(&p->d_duct, sizeof(double) * duct_tab.size()) != hipSuccess ||
+            hipMalloc(&p->d_spec2, sizeof(double2) * H) != hipSuccess ||
+            hipMalloc(&p->d_lu, sizeof(double) * (size_t)G.nxh * G.nym * G.nzp) != hipSuccess)
+            return bail(DFP_E_NOMEM);
+        if (hipMemcpy(p->d_duct, duct_tab.data(), sizeof(double) * duct_tab.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(DFP_E_HIP);
+        const size_t nn = (size_t)G.nzp * G.nzp;
+        p->duct_qf = p->d_duct;
+        p->duct_qi = p->d_duct + nn;
+        p->duct_lam = p->d_duct + 2 * nn;
+        if (launch_duct_lu(p) != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) return bail(DFP_E_HIP);
+    } else if (G.ybc == 1) {  // pivots of the wall solve
         const size_t nlu = (size_t)G.nxh * G.nym * (G.nzp / 2 + 1);
         if (hipMalloc(&p->d_lu, sizeof(double) * nlu) != hipSuccess) return bail(DFP_E_NOMEM);
         if (launch_lu_table(p) != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) return bail(DFP_E_HIP);
@@ -794,8 +863,8 @@ static int step_checks(dfp_plan* p, int flags) {
         return fail(p, DFP_E_ARG, "unknown flag bits 0x%x", flags);
     if (!(flags & DFP_FIELDS_ON_DEVICE))
         return fail(p, DFP_E_ARG, "the RK update and boundary conditions take device-resident fields only");
-    if (p->geo.xbc != 0)
-        return fail(p, DFP_E_UNSUPPORTED, "the boundary conditions here are the channel's (x periodic) only");
+    if (p->geo.xbc != 0 || p->geo.zbc != 0)
+        return fail(p, DFP_E_UNSUPPORTED, "the boundary conditions here are those of a periodic x and z");
     HIPCHK(p, hipSetDevice(p->device));
     return DFP_OK;
 }

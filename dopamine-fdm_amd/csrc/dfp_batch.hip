@@ -299,7 +299,7 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
         dfp_plan* p = plans[i];
         if (!p) return bfail(b, DFP_E_ARG, "case %d: null plan", i);
         if (p->device != b->device) return bfail(b, DFP_E_ARG, "case %d: plan on device %d", i, p->device);
-        if (p->geo.xbc != 0 || p->geo.ybc != 1)
+        if (p->geo.xbc != 0 || p->geo.ybc != 1 || p->geo.zbc != 0)
             return bfail(b, DFP_E_UNSUPPORTED, "case %d: the batch executor takes channel plans (periodic x, y walls) only",
                          i);
         for (int k = 0; k < i; ++k)

@@ -56,6 +56,7 @@ struct Geo {
     int nxp, nym, nzp, nxh;   // nxh: x planes of the spectrum (nxp/2+1 periodic, nxp inflow/outflow)
     int xbc;                  // x_bc_type: 0 periodic, 1 inflow/outflow (DCT-IV in x)
     int ybc;                  // y_bc_type: 1 walls (tridiagonal y solve), 0 periodic (y FFT, uniform y)
+    int zbc;                  // z_bc_type: 0 periodic (FFT in z), 1 walls (the duct: z eigenbasis, nzp = nzm)
     int nx, ny, nz, nxg, nyg, nzg;
     double inv_dx;
     const double* inv_dyf;
@@ -202,6 +203,14 @@ struct dfp_plan {
     dfp::FftDesc fy{};           // periodic y: length nym-1
     const double2* tw_y = nullptr;  // periodic y: stage twiddles of fy
     int gy = 1;                  // periodic y: columns per y-transform call
+    // the duct (z walls): second spectrum (the eigenmode planes), the z
+    // eigenbasis tables and the pinned eigenmode (the zero eigenvalue)
+    double2* d_spec2 = nullptr;
+    double* d_duct = nullptr;    // Qf (nzp x nzp, row k) | Qi (nzp x nzp, row m) | lambda_z (nzp)
+    const double* duct_qf = nullptr;
+    const double* duct_qi = nullptr;
+    const double* duct_lam = nullptr;
+    int pin_m = -1;
     int zr = 1;                  // radix of the z stage applied by the x kernels (z split)
     const double2* tw_x = nullptr;
     const double2* tw_z = nullptr;
@@ -271,6 +280,9 @@ template <typename C>
 hipError_t launch_zfused_batch(hipStream_t s, const ZPar<C>* cs, const int* off, const int* idx, const int* nc,
                                int grid, size_t lds);
 size_t zf_lds_bytes(int nym, int n, int ntw, int f32);
+// the duct: z eigenbasis transforms and y solves of the spectrum (k_duct.hip)
+hipError_t launch_duct_lu(dfp_plan* p);
+hipError_t launch_duct_solve(dfp_plan* p);
 // periodic y: z transform + y FFT solve of each (kx, u) sub-plane in LDS (FP64)
 hipError_t launch_zfused_py(dfp_plan* p);
 size_t zf_py_lds_bytes(int nym, int n, int ntwz, int ntwy);

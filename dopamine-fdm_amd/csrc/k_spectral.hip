@@ -64,7 +64,8 @@ XPar<C> make_xpar(const dfp_plan* p, int inv) {
         q.scale = 1.0 / (2.0 * (double)g.nxp * (double)g.nzp);
     } else {
         q.div_half = FastDiv(p->fx.n / 2 + 1);
-        q.scale = 1.0 / ((double)g.nxp * (double)g.nzp);
+        // the duct solves z in physical space: Real()/nxp (projection.f90:326-331)
+        q.scale = g.zbc == 1 ? 1.0 / (double)g.nxp : 1.0 / ((double)g.nxp * (double)g.nzp);
     }
     q.dtc = 1.0;
     if (sizeof(C) == sizeof(double2)) {

@@ -365,10 +365,11 @@ __device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsig
         if (pbar) phi += pbar[j];
         if (ACCUM) phi += rr[i + k * pr_k];
         psum += phi;
-        // rhs_p(i+2, j+2, k+2) and the duplicates (i = nxp -> nxg-1, k = nzp -> nzp+2)
+        // rhs_p(i+2, j+2, k+2) and the duplicates (i = nxp -> nxg-1, k = nzp -> nzp+2;
+        // z walls have no z duplicate, projection.f90:472-497)
         rr[i + k * pr_k] = phi;
         if (i == 0) rr[nxp + k * pr_k] = phi;
-        if (k == 0) {
+        if (k == 0 && g.zbc == 0) {
             rr[i + (long)g.nzp * pr_k] = phi;
             if (i == 0) rr[nxp + (long)g.nzp * pr_k] = phi;
         }
@@ -376,11 +377,15 @@ __device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsig
             // every copy of this cell in P (reference 1-based indices; 0 = none):
             // x: i+2, 1 if i = nxp-1, nxg-1 if i = 0, nxg if i = 1
             // y: j+2, 1 if j = 0, nyg if j = nym-1
-            // z: k+2, 1 if k = nzp-1, nzg-1 if k = 0, nzg if k = 1 (mod nzp)
+            // z: k+2, 1 if k = nzp-1, nzg-1 if k = 0, nzg if k = 1 (mod nzp);
+            //    z walls (Neumann, projection.f90:423-436): 1 if k = 0, nzg if k = nzp-1
             const double pv = phi / dtc;
             const int x1 = i == nxp - 1 ? 1 : 0, x2 = i == 0 ? g.nxg - 1 : 0, x3 = i == 1 ? g.nxg : 0;
             const int y1 = j == 0 ? 1 : 0, y2 = j == g.nym - 1 ? g.nyg : 0;
-            const int z1 = k == g.nzp - 1 ? 1 : 0, z2 = k == 0 ? g.nzg - 1 : 0, z3 = k == 1 % g.nzp ? g.nzg : 0;
+            const bool zw = g.zbc == 1;
+            const int z1 = zw ? (k == 0 ? 1 : 0) : (k == g.nzp - 1 ? 1 : 0);
+            const int z2 = zw ? (k == g.nzp - 1 ? g.nzg : 0) : (k == 0 ? g.nzg - 1 : 0);
+            const int z3 = zw ? 0 : (k == 1 % g.nzp ? g.nzg : 0);
 #pragma unroll
             for (int c3 = 0; c3 < 4; ++c3) {
                 const int zz = c3 == 0 ? k + 2 : (c3 == 1 ? z1 : (c3 == 2 ? z2 : z3));

@@ -27,6 +27,8 @@ class ChannelGrid:
     grid_type: int = 2
     x_bc_type: int = 0   # 0 periodic x, 1 inflow/outflow x (global.f90:185-186)
     y_bc_type: int = 1   # 1 walls, 0 periodic y (uniform y and periodic x only, initialization.f90:96-97, :128-129)
+    z_bc_type: int = 0   # 0 periodic z, 1 walls (the 4-wall duct: periodic x, y walls, initialization.f90:98-100)
+    alpha_z: float = 0.0  # z walls: symmetric tanh stretching of the z faces (alpha_grid_z; 0 uniform)
     x: np.ndarray = field(init=False)
     y: np.ndarray = field(init=False)
     z: np.ndarray = field(init=False)
@@ -45,9 +47,19 @@ class ChannelGrid:
             raise ValueError("y_bc_type must be 1 (walls) or 0 (periodic)")
         if self.y_bc_type == 0 and (self.grid_type != 1 or self.x_bc_type != 0 or self.nym < 2):
             raise ValueError("periodic y needs a uniform y grid (grid_type 1), a periodic x and nym >= 2")
+        if self.z_bc_type not in (0, 1):
+            raise ValueError("z_bc_type must be 0 (periodic) or 1 (walls)")
+        if self.z_bc_type == 1 and (self.x_bc_type != 0 or self.y_bc_type != 1):
+            raise ValueError("z walls are built as the 4-wall duct: periodic x and y walls")
         nx, ny, nz = self.nx, self.ny, self.nz
         self.x = np.array([self.Lx_i * float(i) / float(nx - 1) for i in range(nx)])
-        self.z = np.array([self.Lz_i * float(i) / float(nz - 1) for i in range(nz)])
+        if self.z_bc_type == 1 and self.alpha_z > 0.0:
+            # genGridandIC.f90:30-35: symmetric tanh between the z walls
+            th = math.tanh(0.5 * self.alpha_z)
+            self.z = np.array([self.Lz_i * 0.5 * (1.0 + math.tanh(self.alpha_z * (float(i) / float(nz - 1) - 0.5)) / th)
+                               for i in range(nz)])
+        else:
+            self.z = np.array([self.Lz_i * float(i) / float(nz - 1) for i in range(nz)])
         if self.grid_type == 1:
             self.y = np.array([self.Ly_i * float(i) / float(ny - 1) for i in range(ny)])
         elif self.grid_type == 2:
@@ -66,10 +78,10 @@ class ChannelGrid:
     # first), the inflow/outflow x all nxm = nxp (initialization.f90:331-338)
     nx = property(lambda s: s.nxp + (2 if s.x_bc_type == 0 else 1))
     ny = property(lambda s: s.nym + 1)
-    nz = property(lambda s: s.nzp + 2)
+    nz = property(lambda s: s.nzp + (2 if s.z_bc_type == 0 else 1))  # z walls keep every cell (nzp = nzm)
     nxg = property(lambda s: s.nx + 1)
     nyg = property(lambda s: s.nym + 2)
-    nzg = property(lambda s: s.nzp + 3)
+    nzg = property(lambda s: s.nz + 1)
     shape_u = property(lambda s: (s.nx, s.nyg, s.nzg))
     shape_v = property(lambda s: (s.nxg, s.ny, s.nzg))
     shape_w = property(lambda s: (s.nxg, s.nyg, s.nz))
@@ -108,6 +120,13 @@ def channel_grid(nxp, nym, nzp, alpha, Lx=2.0 * math.pi, Lz=math.pi, Ly=2.0, gri
     inflow/outflow x (x_bc_type = 1) the nxp cells span Lx (Lxp = Lx, :333-334)."""
     Lx_i = Lx * (nxp + 1) / nxp if x_bc_type == 0 else Lx
     return ChannelGrid(nxp, nym, nzp, Lx_i, Ly, Lz * (nzp + 1) / nzp, alpha, grid_type, x_bc_type)
+
+
+def duct_grid(nxp, nym, nzm, alpha, alpha_z=0.0, Lx=2.0 * math.pi, Ly=2.0, Lz=2.0) -> ChannelGrid:
+    """The 4-wall duct: periodic x of period Lx, y walls Ly apart (tanh
+    stretching alpha, uniform if 0), z walls Lz apart with nzm cells (uniform,
+    or symmetric tanh with alpha_z > 0)."""
+    return ChannelGrid(nxp, nym, nzm, Lx * (nxp + 1) / nxp, Ly, Lz, alpha, 2 if alpha > 0 else 1, 0, 1, 1, alpha_z)
 
 
 def periodic_y_grid(nxp, nym, nzp, Lx=2.0 * math.pi, Ly=2.0, Lz=math.pi) -> ChannelGrid:

@@ -78,7 +78,7 @@ class Plan:
         dp = ctypes.POINTER(ctypes.c_double)
         x, y, z, yg, zg = (a.ctypes.data_as(dp) for a in self._arrays)
         self._grid = DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, x, y, z, yg, zg, int(getattr(g, "x_bc_type", 0)),
-                             int(getattr(g, "y_bc_type", 1)), 0)
+                             int(getattr(g, "y_bc_type", 1)), int(getattr(g, "z_bc_type", 0)))
         self._outflow = None  # keeps the arrays registered with dfp_set_outflow alive
         h = ctypes.c_void_p()
         if tuning is None:

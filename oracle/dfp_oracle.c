@@ -498,8 +498,8 @@ static void jacobi_eigen(int n, double *a, double *d, double *v) {
 #undef V_E
 }
 
-/* initialization.f90:545-580 (Dzz, Dyy's construction on z, zg) and
-   :586-611: S = W^(1/2) Dzz W^(-1/2) with W the z cell widths, its
+/* initialization.f90:544-577 (Dzz, Dyy's construction on z, zg) and
+   :583-611: S = W^(1/2) Dzz W^(-1/2) with W the z cell widths, its
    eigenvalues lambda_z and orthonormal eigenvectors Qz, and sqrt_w */
 static void duct_eigenbasis(ora_plan *p) {
     const int n = p->nzp;
@@ -707,7 +707,7 @@ static void apply_periodic_xz_pressure(const ora_plan *p, double *rhs_p) {
     if (!p->xbc)
         for (int k = 2; k <= p->nzg; k++)
             for (int j = 2; j <= p->nyg - 1; j++) R_(p->nxg - 1, j, k) = R_(2, j, k);
-    if (p->zbc) return; /* z walls: no z fill (:477-497) */
+    if (p->zbc) return; /* z walls: no z fill (:472-497) */
     for (int j = 2; j <= p->nyg - 1; j++)
         for (int i = 2; i <= p->nxg - 1; i++) R_(i, j, p->nzp + 2) = R_(i, j, 2);
 }
@@ -735,7 +735,7 @@ static void finish_pressure_ghosts(const ora_plan *p, double *P) {
     for (int j = 1; j <= nyg; j++)
         for (int i = 1; i <= nxg; i++) {
             if (p->zbc) {
-                /* :430-437 Neumann copies at the z walls */
+                /* :423-436 Neumann copies at the z walls */
                 P_(i, j, 1) = P_(i, j, 2);
                 P_(i, j, nzg) = P_(i, j, nzg - 1);
             } else {
@@ -746,7 +746,7 @@ static void finish_pressure_ghosts(const ora_plan *p, double *P) {
         }
 }
 
-/* projection.f90:162-206, the duct: per x mode i, z_hat = (p * sqrt_w) Qz,
+/* projection.f90:172-218, the duct: per x mode i, z_hat = (p * sqrt_w) Qz,
    one Zgtsv per (i, m) with kxx(i) + lambda_z(m) (the pin where i = 0 and
    |lambda_z(m)| < 1e-8 max|lambda_z|), p = z_hat Qz^T / sqrt_w */
 static int duct_solve(ora_plan *p, cplx *wk) {

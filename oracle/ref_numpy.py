@@ -38,15 +38,15 @@ and the 4-wall duct (``x_bc_type=0, y_bc_type=1, z_bc_type=1``):
 
 * grid: all nzm cells kept ..... src/initialization.f90:339-345 (nzp = nzm), z faces uniform or
                                 symmetric tanh (alpha_grid_z > 0), src/genGridandIC.f90:27-40
-* Dzz ......................... src/initialization.f90:545-580 (Dyy's construction on z, zg)
-* z eigenbasis ................ src/initialization.f90:586-618: S = W^(1/2) Dzz W^(-1/2)
+* Dzz ......................... src/initialization.f90:544-577 (Dyy's construction on z, zg)
+* z eigenbasis ................ src/initialization.f90:583-618: S = W^(1/2) Dzz W^(-1/2)
                                 (W = cell widths), LAPACK dstev -> lambda_z, Qz
-* solve ....................... src/projection.f90:162-206 (CPU duct branch): x FFT, per kx
+* solve ....................... src/projection.f90:172-218 (CPU duct branch): x FFT, per kx
                                 z_hat = (p * sqrt_w) Qz, one Zgtsv per (kx, m) with
                                 kxx + lambda_z(m) and the pin where kx = 0 and
                                 |lambda_z(m)| < 1e-8 max|lambda_z|, p = z_hat Qz^T / sqrt_w;
                                 inverse x with Real()/nxp (:326-331)
-* no z periodic fill .......... src/projection.f90:477-497; Neumann z ghosts of P (:430-437)
+* no z periodic fill .......... src/projection.f90:472-497; Neumann z ghosts of P (:423-436)
 
 LAPACK ``zgtsv`` (unpinned LAPACK version, CMakeLists.txt:50-51; the routine's
 algorithm is unchanged since LAPACK 3.0) is restated from its published
@@ -277,9 +277,9 @@ def wavenumbers_dct(n: int, d: float) -> np.ndarray:
 
 
 def z_eigenbasis(z: np.ndarray, zg: np.ndarray):
-    """initialization.f90:586-611: the symmetric tridiagonal S = W^(1/2) Dzz
+    """initialization.f90:583-611: the symmetric tridiagonal S = W^(1/2) Dzz
     W^(-1/2) of the z-wall operator (W = diag of the cell widths z(k+1)-z(k);
-    Dzz is Dyy's construction on z, zg, initialization.f90:545-580), its
+    Dzz is Dyy's construction on z, zg, initialization.f90:544-577), its
     eigenvalues lambda_z (ascending) and orthonormal eigenvectors Qz[:, m]
     (LAPACK dstev there, scipy's LAPACK-backed eigh_tridiagonal here), and
     sqrt_w = sqrt(cell widths)."""
@@ -515,7 +515,7 @@ class Projection:
         return self._inverse_xz(c, rhs_p, P, dt, rk_coef33)
 
     def _duct_solve(self, c, rhs_p, P, dt, rk_coef33):
-        # projection.f90:162-206: per kx, z eigenmode transform, one Zgtsv
+        # projection.f90:172-218: per kx, z eigenmode transform, one Zgtsv
         # per (kx, m) with kxx + lambda_z(m), the pin at kx = 0 on the zero
         # eigenvalue (found by magnitude), inverse transform
         g = self.g
@@ -600,7 +600,7 @@ class Projection:
             Pf[0, :, :] = Pf[1, :, :]
             Pf[g.nxg - 1, :, :] = Pf[g.nxg - 2, :, :]
         if getattr(g, "z_bc_type", 0) == 1:
-            # :430-437 Neumann copies at the z walls
+            # :423-436 Neumann copies at the z walls
             Pf[:, :, 0] = Pf[:, :, 1]
             Pf[:, :, g.nzg - 1] = Pf[:, :, g.nzg - 2]
             return

@@ -48,7 +48,7 @@ def _grid_struct(g, bc=(0, 1, 0)):
     return _lib.DfpGrid(g.nxp, g.nym, g.nzp, g.dx, g.dz, *ptrs, *bc), keep
 
 
-@pytest.mark.parametrize("bc", [(2, 1, 0), (1, 0, 0), (0, 2, 0), (0, 1, 1), (1, 1, 1), (0, 0, 1)])
+@pytest.mark.parametrize("bc", [(2, 1, 0), (1, 0, 0), (0, 2, 0), (0, 1, 2), (1, 1, 1), (0, 0, 1)])
 def test_other_boundary_conditions_are_rejected(bc):
     g = dfp.channel_grid(16, 8, 16, 1.5)
     s, keep = _grid_struct(g, bc)
@@ -159,3 +159,23 @@ def test_periodic_y_rules_before_any_device_call():
     # the last y face of the box is one period above the first (projection.f90:350-354)
     g = dfp.periodic_y_grid(8, 9, 6, Ly=2.0)
     assert g.y[-2] - g.y[0] == pytest.approx(2.0, rel=1e-14)
+
+
+def test_duct_rules_before_any_device_call():
+    """z walls are built as the 4-wall duct (periodic x, y walls,
+    initialization.f90:98-100) in FP64; the rest is refused before the device
+    is touched."""
+    L = dfp.lib()
+    h = ctypes.c_void_p()
+    g = dfp.duct_grid(16, 9, 12, 1.5, 1.2)
+    s, keep = _grid_struct(g, (0, 1, 1))
+    assert L.dfp_plan_create(ctypes.byref(s), dfp.DFP_FP32_REFINE, 0, ctypes.byref(h)) == -2
+    for bc in ((1, 1, 1), (0, 0, 1)):
+        s, keep = _grid_struct(g, bc)
+        assert L.dfp_plan_create(ctypes.byref(s), 0, 0, ctypes.byref(h)) == -2
+    assert not h.value
+    with pytest.raises(ValueError):
+        dfp.ChannelGrid(16, 9, 12, 7.0, 2.0, 2.0, 1.5, 2, 1, 1, 1)   # inflow x with z walls
+    # z walls keep every cell: nz = nzm + 1 faces, nzg = nzm + 2
+    assert (g.nz, g.nzg) == (13, 14) and g.shape_w == (g.nxg, g.nyg, 13) and g.shape_rhs == (g.nxg - 1, 9, 13)
+    assert g.z[0] == 0.0 and abs(g.z[-1] - 2.0) < 1e-15

@@ -1,5 +1,5 @@
 """Pins the 4-wall duct branch of the oracles (x periodic, y walls, z walls:
-src/projection.f90:162-206, src/initialization.f90:545-618) by what the
+src/projection.f90:172-218, src/initialization.f90:544-618) by what the
 mathematics fixes:
   * the z eigenbasis: v_m = Qz[:, m] / sqrt_w are eigenvectors of the
     assembled z-wall operator Dzz with eigenvalue lambda_z(m); Qz is
