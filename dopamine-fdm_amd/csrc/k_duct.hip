@@ -4,9 +4,10 @@
 // symmetric S = W^(1/2) Dzz W^(-1/2) = Qz diag(lambda_z) Qz^T (W = the z cell
 // widths), and after the x transform every x mode's (y, z) plane is taken to
 // the z eigenbasis, solved along y per eigenmode and taken back:
-//   k_zeig         C(j, n) = sum_k A(j, k) B(k, n) for every x mode: A the
+//   k_zeig_mfma    C(j, n) = sum_k A(j, k) B(k, n) for every x mode: A the
 //                  complex plane, B real: Qf = W^(1/2) Qz forward,
-//                  Qi = Qz^T W^(-1/2) back (a batched complex-by-real GEMM)
+//                  Qi = Qz^T W^(-1/2) back (a batched complex-by-real GEMM
+//                  on the FP64 matrix cores)
 //   k_duct_lu      inverse pivots of each (kx, m) system Dyy + kxx + lambda_z(m)
 //                  (the pin where kx = 0 on the zero eigenvalue)
 //   k_duct_ysolve  elimination and back substitution along y, one lane per
@@ -20,74 +21,105 @@ namespace dfp {
 constexpr int kGemmThreads = 256;
 constexpr int kGemmTJ = 64;  // rows (j) per tile
 constexpr int kGemmTN = 64;  // columns (n) per tile
-constexpr int kGemmKC = 16;  // k per LDS stage
+constexpr int kGemmKC = 32;  // k per LDS stage
 
-// One (n-tile, j-tile, kx) tile of C = A B.  Thread (tx, ty) = (t % 16,
-// t / 16) accumulates rows ty + 16 r and columns tx + 16 c (r, c < 4): in
-// the inner loop a wave reads 4 distinct A values (broadcast) and 16
-// consecutive B values, and the stores of a row run over 16 consecutive
-// columns.  A is staged transposed (As[k][j], rows padded by one value so
-// the transposing writes fall in distinct banks).
-__global__ void __launch_bounds__(kGemmThreads) k_zeig(const double2* __restrict__ A, const double* __restrict__ B,
-                                                      double2* __restrict__ C, int nym, int K, int N) {
-    __shared__ double2 As[kGemmKC][kGemmTJ + 1];
+// C = A B for every x mode on the matrix cores: v_mfma_f64_16x16x4f64 (D += A B on
+// 16x16x4 FP64 tiles; lane l holds A(l%16, l/16), B(l/16, l%16) and
+// D(l/16 + 4v, l%16) in its v-th accumulator, as measured on gfx950).  The
+// complex rows of A are taken as pairs of real rows (re, im), so the real
+// GEMM has M = 2 nym.  Within a 16-row tile, real row i holds part
+// (i/4) % 2 of complex row (i%4) + 4 (i/8): lane l's accumulators v = 0, 1
+// are then the re and im parts of complex row l/16 and v = 2, 3 those of
+// row l/16 + 4, each pair stored as one 16-byte value.  A block's tile is 64
+// complex rows (128 real) x 64 columns; wave w computes real rows
+// 32w..32w+31 (two tile rows) x all 64 columns (four tile columns).
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__global__ void __launch_bounds__(kGemmThreads) k_zeig_mfma(const double2* __restrict__ A,
+                                                           const double* __restrict__ B, double2* __restrict__ C,
+                                                           int nym, int K, int N) {
+    __shared__ double As[kGemmKC][2 * kGemmTJ + 2];  // real rows, padded
     __shared__ double Bs[kGemmKC][kGemmTN];
     const int t = threadIdx.x;
-    const int tx = t % 16, ty = t / 16;
+    const int lane = t & 63, w = t >> 6;
+    const int li = lane & 15, lk = lane >> 4;
     const int n0 = blockIdx.x * kGemmTN, j0 = blockIdx.y * kGemmTJ;
     const long plane = (long)blockIdx.z * nym;
     const auto a = gptr(A) + plane * K;
     const auto b = gptr(B);
-    double2 acc[4][4];
+    constexpr int QA = kGemmTJ * kGemmKC / kGemmThreads, QB = kGemmKC * kGemmTN / kGemmThreads;
+    // the next stage's tile values, in flight while the current stage multiplies
+    double2 pa[QA];
+    double pb[QB];
+    auto fetch = [&](int k0) {
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-        for (int c = 0; c < 4; ++c) acc[r][c] = make_double2(0.0, 0.0);
-    for (int k0 = 0; k0 < K; k0 += kGemmKC) {
-        // A tile: 64 rows x 16 k, consecutive lanes along k (256-byte runs)
-#pragma unroll
-        for (int q = 0; q < kGemmTJ * kGemmKC / kGemmThreads; ++q) {
+        for (int q = 0; q < QA; ++q) {
             const int e = t + q * kGemmThreads;
             const int jj = e / kGemmKC, kk = e - jj * kGemmKC;
             const int j = j0 + jj, k = k0 + kk;
-            As[kk][jj] = (j < nym && k < K) ? gld<double2>(a + (long)j * K + k) : make_double2(0.0, 0.0);
+            pa[q] = (j < nym && k < K) ? gld<double2>(a + (long)j * K + k) : make_double2(0.0, 0.0);
         }
-        // B tile: 16 k x 64 columns, consecutive lanes along n
 #pragma unroll
-        for (int q = 0; q < kGemmKC * kGemmTN / kGemmThreads; ++q) {
+        for (int q = 0; q < QB; ++q) {
             const int e = t + q * kGemmThreads;
             const int kk = e / kGemmTN, nn = e - kk * kGemmTN;
             const int k = k0 + kk, n = n0 + nn;
-            Bs[kk][nn] = (k < K && n < N) ? b[(long)k * N + n] : 0.0;
+            pb[q] = (k < K && n < N) ? b[(long)k * N + n] : 0.0;
         }
+    };
+    auto stage = [&]() {
+#pragma unroll
+        for (int q = 0; q < QA; ++q) {
+            const int e = t + q * kGemmThreads;
+            const int jj = e / kGemmKC, kk = e - jj * kGemmKC;
+            const int cr = jj & 7;                                     // complex row within its tile
+            const int i0 = 16 * (jj >> 3) + (cr & 3) + 8 * (cr >> 2);  // its re row; im is 4 rows on
+            As[kk][i0] = pa[q].x;
+            As[kk][i0 + 4] = pa[q].y;
+        }
+#pragma unroll
+        for (int q = 0; q < QB; ++q) {
+            const int e = t + q * kGemmThreads;
+            const int kk = e / kGemmTN, nn = e - kk * kGemmTN;
+            Bs[kk][nn] = pb[q];
+        }
+    };
+    v4d acc[2][4];
+#pragma unroll
+    for (int r = 0; r < 2; ++r)
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc[r][c] = v4d{0.0, 0.0, 0.0, 0.0};
+    fetch(0);
+    for (int k0 = 0; k0 < K; k0 += kGemmKC) {
+        stage();
         __syncthreads();
+        if (k0 + kGemmKC < K) fetch(k0 + kGemmKC);
 #pragma unroll
-        for (int kk = 0; kk < kGemmKC; ++kk) {
-            double2 av[4];
-            double bv[4];
+        for (int ks = 0; ks < kGemmKC / 4; ++ks) {
+            const int kk = 4 * ks + lk;
+            double av[2], bv[4];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) av[r] = As[kk][ty + 16 * r];
+            for (int r = 0; r < 2; ++r) av[r] = As[kk][32 * w + 16 * r + li];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) bv[c] = Bs[kk][tx + 16 * c];
+            for (int c = 0; c < 4; ++c) bv[c] = Bs[kk][16 * c + li];
 #pragma unroll
-            for (int r = 0; r < 4; ++r)
+            for (int r = 0; r < 2; ++r)
 #pragma unroll
-                for (int c = 0; c < 4; ++c) {
-                    acc[r][c].x = fma(av[r].x, bv[c], acc[r][c].x);
-                    acc[r][c].y = fma(av[r].y, bv[c], acc[r][c].y);
-                }
+                for (int c = 0; c < 4; ++c) acc[r][c] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[r], bv[c], acc[r][c], 0, 0, 0);
         }
         __syncthreads();
     }
     const auto cp = gptr(C) + plane * N;
 #pragma unroll
-    for (int r = 0; r < 4; ++r) {
-        const int j = j0 + ty + 16 * r;
-        if (j >= nym) continue;
+    for (int r = 0; r < 2; ++r) {
+        // complex rows j0 + 16w + 8r + lk (v = 0, 1) and + 4 (v = 2, 3)
+        const int j = j0 + 16 * w + 8 * r + lk;
 #pragma unroll
         for (int c = 0; c < 4; ++c) {
-            const int n = n0 + tx + 16 * c;
-            if (n < N) gst<double2>(cp + (long)j * N + n, acc[r][c]);
+            const int n = n0 + 16 * c + li;
+            if (n >= N) continue;
+            if (j < nym) gst<double2>(cp + (long)j * N + n, make_double2(acc[r][c][0], acc[r][c][1]));
+            if (j + 4 < nym) gst<double2>(cp + (long)(j + 4) * N + n, make_double2(acc[r][c][2], acc[r][c][3]));
         }
     }
 }
@@ -196,7 +228,7 @@ static hipError_t launch_zeig(dfp_plan* p, const double2* A, const double* B, do
     const Geo& g = p->geo;
     const dim3 grid(ceil_div(g.nzp, kGemmTN), ceil_div(g.nym, kGemmTJ), g.nxh);
     const int slot = prof_begin(p, name);
-    hipLaunchKernelGGL(k_zeig, grid, dim3(kGemmThreads), 0, p->stream, A, B, C, g.nym, g.nzp, g.nzp);
+    hipLaunchKernelGGL(k_zeig_mfma, grid, dim3(kGemmThreads), 0, p->stream, A, B, C, g.nym, g.nzp, g.nzp);
     prof_end(p, slot);
     return hipGetLastError();
 }
