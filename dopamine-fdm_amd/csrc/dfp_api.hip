@@ -863,8 +863,8 @@ static int step_checks(dfp_plan* p, int flags) {
         return fail(p, DFP_E_ARG, "unknown flag bits 0x%x", flags);
     if (!(flags & DFP_FIELDS_ON_DEVICE))
         return fail(p, DFP_E_ARG, "the RK update and boundary conditions take device-resident fields only");
-    if (p->geo.xbc != 0 || p->geo.zbc != 0)
-        return fail(p, DFP_E_UNSUPPORTED, "the boundary conditions here are those of a periodic x and z");
+    if (p->geo.xbc != 0)
+        return fail(p, DFP_E_UNSUPPORTED, "the boundary conditions here are those of a periodic x");
     HIPCHK(p, hipSetDevice(p->device));
     return DFP_OK;
 }

@@ -1,5 +1,5 @@
-// The step either side of the projection, channel case (x periodic, y walls,
-// z periodic), one rank, on device-resident fields:
+// The step either side of the projection for a periodic x (the channel, the
+// periodic box, the duct), one rank, on device-resident fields:
 //   k_rk_update     RK3 stage update of the interior  time_integration.f90:256-258,
 //                                                     :394-396, :526-531
 //   k_wrap_x/z      periodic ghost planes             boundary_conditions.f90:263-289,
@@ -7,9 +7,10 @@
 //   k_robin_y       Robin (slip-length) wall rows     boundary_conditions.f90:784-804
 //   k_wrap_y        periodic y ghost rows (y_bc_type = 0, the periodic box)
 //                                                     boundary_conditions.f90:762-781
-// apply_boundary_conditions (:186-260) is the sequence x wrap, z wrap, y
-// walls (or y wrap, :229-233), x wrap, z wrap; on one rank its halo
-// exchanges (:170-183) exchange nothing.  Each launch covers the three fields (blockIdx.z); arithmetic is
+//   k_wall_z        z wall planes (the duct)          boundary_conditions.f90:737-758
+// apply_boundary_conditions (:186-260) is the sequence x wrap, z wrap (or
+// z walls, :215-223), y walls (or y wrap, :229-233), x wrap, z wrap (none
+// with z walls); on one rank its halo exchanges (:170-183) exchange nothing.  Each launch covers the three fields (blockIdx.z); arithmetic is
 // kept operation for operation (no contraction), so results equal the CPU
 // path bit for bit.
 #include <algorithm>
@@ -125,6 +126,24 @@ __global__ void __launch_bounds__(256) k_wrap_y(Field3 f) {
     }
 }
 
+// z wall planes over (i, j): W faces no-penetration (1 and nz zero), U and
+// V centres no-slip by mirroring (1 <- -2, nzg <- -(nzg-1))
+__global__ void __launch_bounds__(256) k_wall_z(Field3 f) {
+    const int fi = blockIdx.z;
+    const int n1 = f.n1[fi], n2 = f.n2[fi], n3 = f.n3[fi];
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    const long pl = (long)n1 * n2;
+    if (t >= pl) return;
+    double* F = f.F[fi] + t;  // F(i, j, :) with stride pl
+    if (fi == 2) {
+        F[0] = 0.0;
+        F[(n3 - 1) * pl] = 0.0;
+    } else {
+        F[0] = -F[pl];
+        F[(n3 - 1) * pl] = -F[(n3 - 2) * pl];
+    }
+}
+
 struct RobinArgs {
     const double* alpha[3];  // alpha_x (nx,2,nzg), alpha_y (nxg,2,nzg), alpha_z (nxg,2,nz)
     double dy1, dy2;         // y(2)-y(1), y(ny)-y(ny-1)
@@ -196,7 +215,10 @@ hipError_t launch_apply_bc(dfp_plan* p, double* U, double* V, double* W, const d
     const int slot = prof_begin(p, "apply_bc");
     for (int pass = 0; pass < 2; ++pass) {
         hipLaunchKernelGGL(k_wrap_x, dim3(ceil_div(px, 256), 1, 3), dim3(256), 0, p->stream, f);
-        hipLaunchKernelGGL(k_wrap_z, dim3(ceil_div(pz, 256), 1, 3), dim3(256), 0, p->stream, f);
+        if (p->geo.zbc == 0)
+            hipLaunchKernelGGL(k_wrap_z, dim3(ceil_div(pz, 256), 1, 3), dim3(256), 0, p->stream, f);
+        else if (pass == 0)
+            hipLaunchKernelGGL(k_wall_z, dim3(ceil_div(pz, 256), 1, 3), dim3(256), 0, p->stream, f);
         if (pass == 0 && p->geo.ybc == 0)
             hipLaunchKernelGGL(k_wrap_y, dim3(ceil_div(py, 256), 1, 3), dim3(256), 0, p->stream, f);
         else if (pass == 0)

@@ -44,18 +44,22 @@ Contains
   Subroutine ensure_plan
     Type(dfp_grid_t) :: g
     If ( c_associated(plan) ) Return
-    ! the channel (either x type), or the periodic box (y_bc_type = 0 with a periodic x, initialization.f90:96-97)
+    ! the channel (either x type), the periodic box (y_bc_type = 0 with a periodic x, initialization.f90:96-97)
+    ! or the 4-wall duct (z_bc_type = 1 with a periodic x and y walls, :98-100)
     If ( ( x_bc_type /= 0 .And. x_bc_type /= 1 ) .Or. ( y_bc_type /= 0 .And. y_bc_type /= 1 ) .Or. &
-         ( y_bc_type == 0 .And. x_bc_type /= 0 ) .Or. z_bc_type /= 0 ) Then
-       Write(error_unit,'(A)') 'ERROR: the dfp projection supports z periodic with y walls (x periodic or ' // &
-            'inflow/outflow) or with y and x periodic only'
+         ( y_bc_type == 0 .And. x_bc_type /= 0 ) .Or. ( z_bc_type /= 0 .And. z_bc_type /= 1 ) .Or. &
+         ( z_bc_type == 1 .And. ( x_bc_type /= 0 .Or. y_bc_type /= 1 ) ) ) Then
+       Write(error_unit,'(A)') 'ERROR: the dfp projection supports the channel (x periodic or inflow/outflow), ' // &
+            'the periodic box and the 4-wall duct only'
        Error Stop 1
     End If
     gx = x;  gy = y;  gz = z;  gyg = yg;  gzg = zg
     ! the periodic x drops its duplicate cell, the inflow/outflow x keeps all nxm (initialization.f90:331-338)
     g%nxp = nx - 2
     If ( x_bc_type == 1 ) g%nxp = nx - 1
+    ! z walls keep all nzm cells too (initialization.f90:339-345)
     g%nym = ny - 1;  g%nzp = nz - 2
+    If ( z_bc_type == 1 ) g%nzp = nz - 1
     g%dx = dx;  g%dz = dz
     g%x = c_loc(gx);  g%y = c_loc(gy);  g%z = c_loc(gz);  g%yg = c_loc(gyg);  g%zg = c_loc(gzg)
     g%x_bc_type = x_bc_type;  g%y_bc_type = y_bc_type;  g%z_bc_type = z_bc_type

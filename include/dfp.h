@@ -184,7 +184,7 @@ int dfp_max_divergence_noalloc(dfp_plan *plan, const double *U, const double *V,
                                double *out, int flags);
 
 /* The step either side of the projection, for a plan with a periodic x
- * (x_bc_type = 0: the channel or the periodic box) and device-resident fields (flags must hold DFP_FIELDS_ON_DEVICE;
+ * (x_bc_type = 0: the channel, the periodic box or the duct) and device-resident fields (flags must hold DFP_FIELDS_ON_DEVICE;
  * DFP_ASYNC allowed).  Fu, Fv, Fw are host arrays of `stage` device pointers
  * to the right-hand sides Fu_s(2:nx-1,2:nyg-1,2:nzg-1),
  * Fv_s(2:nxg-1,2:ny-1,2:nzg-1), Fw_s(2:nxg-1,2:nyg-1,2:nz-1)
@@ -196,7 +196,9 @@ int dfp_max_divergence_noalloc(dfp_plan *plan, const double *U, const double *V,
  * (boundary_conditions.f90:186-260): periodic x and z ghost planes and the
  * Robin (slip-length) wall rows with alpha_x(nx,2,nzg), alpha_y(nxg,2,nzg),
  * alpha_z(nxg,2,nz) (global.f90:328; 0 = no slip), or, for the periodic box,
- * the periodic y ghost rows (:229-233, :762-781; the alphas may be NULL).
+ * the periodic y ghost rows (:229-233, :762-781; the alphas may be NULL);
+ * the duct's z walls take W = 0 on the wall faces and mirrored U, V ghosts
+ * (:215-223, :737-758) in place of the z wraps.
  * dfp_substep: one RK3 substage around the projection as time_integration.f90
  * runs it without immersed boundaries, VOF or scalars (:256-304): update,
  * boundary conditions, dfp_project with c = rk_coef(stage,stage) (P saved

@@ -1068,6 +1068,21 @@ static void periodic_z(double *F, int n1, int n2, int n3, int faces) {
         }
 }
 
+/* boundary_conditions.f90:737-758: z walls (one rank): W faces no-penetration,
+   U and V centres no-slip by ghost mirroring */
+static void dirichlet_z(double *F, int n1, int n2, int n3, int faces) {
+    for (int j = 1; j <= n2; j++)
+        for (int i = 1; i <= n1; i++) {
+            if (faces) {
+                IDX3(F, n1, n2, i, j, 1) = 0.0;
+                IDX3(F, n1, n2, i, j, n3) = 0.0;
+            } else {
+                IDX3(F, n1, n2, i, j, 1) = -IDX3(F, n1, n2, i, j, 2);
+                IDX3(F, n1, n2, i, j, n3) = -IDX3(F, n1, n2, i, j, n3 - 1);
+            }
+        }
+}
+
 /* boundary_conditions.f90:762-781: periodic y (one rank); faces: V (id 1,
    1 <- ny-1, ny <- 2), else U and W centres (1 <- nyg-2, nyg-1 <- 2, nyg <- 3) */
 static void periodic_y(double *F, int n1, int n2, int n3, int faces) {
@@ -1105,17 +1120,24 @@ static void robin_y(const ora_plan *p, double *F, int n1, int n2, int n3, const 
 
 void ora_apply_bc(const ora_plan *p, double *U, double *V, double *W, const double *alpha_x, const double *alpha_y,
                   const double *alpha_z) {
-    /* boundary_conditions.f90:186-260 for x_bc_type = 0, z_bc_type = 0:
-       periodic x, periodic z, then Robin y (y_bc_type = 1) or periodic y
-       (y_bc_type = 0, :229-233; alphas unused), then the x and z wraps again
-       (:247-258) */
+    /* boundary_conditions.f90:186-260 for x_bc_type = 0: periodic x, periodic
+       z (or the z walls, :215-223), then Robin y (y_bc_type = 1) or periodic y
+       (y_bc_type = 0, :229-233; alphas unused), then the x wrap and, for a
+       periodic z, the z wrap again (:247-258) */
     for (int pass = 0; pass < 2; pass++) {
         periodic_x(U, p->nx, p->nyg, p->nzg, 1);
         periodic_x(V, p->nxg, p->ny, p->nzg, 0);
         periodic_x(W, p->nxg, p->nyg, p->nz, 0);
-        periodic_z(U, p->nx, p->nyg, p->nzg, 0);
-        periodic_z(V, p->nxg, p->ny, p->nzg, 0);
-        periodic_z(W, p->nxg, p->nyg, p->nz, 1);
+        if (!p->zbc) {
+            periodic_z(U, p->nx, p->nyg, p->nzg, 0);
+            periodic_z(V, p->nxg, p->ny, p->nzg, 0);
+            periodic_z(W, p->nxg, p->nyg, p->nz, 1);
+        } else if (pass == 0) {
+            /* :218-222 z walls (the second pass has no z part, :253-257) */
+            dirichlet_z(U, p->nx, p->nyg, p->nzg, 0);
+            dirichlet_z(V, p->nxg, p->ny, p->nzg, 0);
+            dirichlet_z(W, p->nxg, p->nyg, p->nz, 1);
+        }
         if (pass == 0 && !p->ybc) {
             periodic_y(U, p->nx, p->nyg, p->nzg, 0);
             periodic_y(V, p->nxg, p->ny, p->nzg, 1);

@@ -702,6 +702,18 @@ def _periodic_z(F, faces):
         F[:, :, n - 1] = F[:, :, 2]
 
 
+def _dirichlet_z(F, faces):
+    # boundary_conditions.f90:737-758: id 3 z faces (W) no-penetration, else
+    # z centres (U, V) no-slip by ghost mirroring
+    n = F.shape[2]
+    if faces:
+        F[:, :, 0] = 0.0
+        F[:, :, n - 1] = 0.0
+    else:
+        F[:, :, 0] = -F[:, :, 1]
+        F[:, :, n - 1] = -F[:, :, n - 2]
+
+
 def _periodic_y(F, faces):
     # boundary_conditions.f90:762-781: id 1 y faces (V), else y centres (U, W)
     n = F.shape[1]
@@ -736,10 +748,17 @@ def apply_boundary_conditions(g: ChannelGrid, U, V, W, alpha_x, alpha_y, alpha_z
     periodic z (U and V centres, W faces), Robin y walls (U with alpha_x, V
     faces with alpha_y, W with alpha_z) or, with y_bc_type = 0, the periodic
     y wrap (:229-233, V faces, U and W centres; the alphas are unused), then
-    the x and z wraps again (:247-258)."""
+    the x and z wraps again (:247-258).  With z walls (the duct) the z wraps
+    are the no-penetration / no-slip rows of :215-223 in the first pass and
+    nothing in the second."""
+    zwall = getattr(g, "z_bc_type", 0) == 1
     for first in (True, False):
         _periodic_x(U, True); _periodic_x(V, False); _periodic_x(W, False)
-        _periodic_z(U, False); _periodic_z(V, False); _periodic_z(W, True)
+        if not zwall:
+            _periodic_z(U, False); _periodic_z(V, False); _periodic_z(W, True)
+        elif first:
+            # :215-223 z walls; the second pass has no z part (:253-257)
+            _dirichlet_z(U, False); _dirichlet_z(V, False); _dirichlet_z(W, True)
         if first and getattr(g, "y_bc_type", 1) == 0:
             _periodic_y(U, False); _periodic_y(V, True); _periodic_y(W, False)
         elif first:

@@ -1,12 +1,12 @@
 !> Test driver: reads a case written by tests/test_fortran_dropin.py, fills the global arrays with the
 !  reference's lower bounds, runs the drop-in projection as time_integration does at RK stage 3, and
-!  writes the results back (raw little-endian float64 streams).  The case gives the x and y boundary types
+!  writes the results back (raw little-endian float64 streams).  The case gives the x, y and z boundary types
 !  and, for the inflow/outflow x, the outlet backflow state (beta, outflow_Uc, Pb).
 Program driver
   Use global
   Use projection
   Implicit None
-  Integer :: u_in, u_out, n(5), nxp, nym, nzp
+  Integer :: u_in, u_out, n(6), nxp, nym, nzp
   Real(Int64) :: hdr(5), maxdiv
   Character(256) :: fin, fout
 
@@ -14,11 +14,12 @@ Program driver
   Call get_command_argument(2, fout)
   Open(newunit=u_in, file=Trim(fin), access='stream', form='unformatted', status='old')
   Read(u_in) n
-  nxp = n(1); nym = n(2); nzp = n(3); x_bc_type = n(4); y_bc_type = n(5)
+  nxp = n(1); nym = n(2); nzp = n(3); x_bc_type = n(4); y_bc_type = n(5); z_bc_type = n(6)
   nx = nxp + 2
   If ( x_bc_type == 1 ) nx = nxp + 1
   ny = nym + 1;  nz = nzp + 2
-  nxg = nx + 1; nyg = nym + 2; nzg = nzp + 3
+  If ( z_bc_type == 1 ) nz = nzp + 1
+  nxg = nx + 1; nyg = nym + 2; nzg = nz + 1
   Read(u_in) hdr
   dx = hdr(1);  dz = hdr(2);  dt = hdr(3)
   outflow_backflow_beta = hdr(4);  outflow_Uc = hdr(5)

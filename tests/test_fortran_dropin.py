@@ -43,7 +43,8 @@ def _write_case(path, g, U, V, W, dt, beta=0.0, uc=0.0, Pb=None):
     if Pb is None:
         Pb = np.zeros((g.nyg - 2, g.nzg - 2), order="F")
     with open(path, "wb") as f:
-        np.array([g.nxp, g.nym, g.nzp, getattr(g, "x_bc_type", 0), getattr(g, "y_bc_type", 1)], dtype=np.int32).tofile(f)
+        np.array([g.nxp, g.nym, g.nzp, getattr(g, "x_bc_type", 0), getattr(g, "y_bc_type", 1),
+                  getattr(g, "z_bc_type", 0)], dtype=np.int32).tofile(f)
         np.array([g.dx, g.dz, dt, beta, uc]).tofile(f)
         for a in (g.x, g.y, g.z, g.yg, g.zg):
             np.ascontiguousarray(a, dtype=np.float64).tofile(f)
@@ -63,9 +64,10 @@ def _read_out(path, g):
     return out, maxdiv, phib
 
 
-# (grid dims, x_bc_type, outflow_backflow_beta); x_bc_type None: the periodic box (y_bc_type = 0)
+# (grid dims, x_bc_type, outflow_backflow_beta); x_bc_type "py": the periodic box (y_bc_type = 0),
+# "duct": the 4-wall duct (z_bc_type = 1, z faces stretched)
 DROPIN_CASES = [((16, 12, 8), 0, 0.0), ((24, 17, 18), 0, 0.0), ((16, 10, 12), 1, 0.0), ((12, 9, 8), 1, 0.8),
-                ((16, 13, 12), None, 0.0)]
+                ((16, 13, 12), "py", 0.0), ((12, 10, 14), "duct", 0.0)]
 
 
 @pytest.mark.gpu
@@ -73,7 +75,12 @@ DROPIN_CASES = [((16, 12, 8), 0, 0.0), ((24, 17, 18), 0, 0.0), ((16, 10, 12), 1,
 def test_dropin_projection_matches_oracle(dims, xbc, beta, tmp_path):
     if not os.path.exists(DRIVER):
         raise RuntimeError(f"Fortran driver not built: {DRIVER} (run __graft_entry__.build())")
-    g = dfp.periodic_y_grid(*dims) if xbc is None else make_grid(*dims, 1.8, x_bc_type=xbc)
+    if xbc == "py":
+        g = dfp.periodic_y_grid(*dims)
+    elif xbc == "duct":
+        g = dfp.duct_grid(*dims, 1.8, 1.5)
+    else:
+        g = make_grid(*dims, 1.8, x_bc_type=xbc)
     U, V, W, P, rhs = fields(g, 21, 0.1)
     Pb = np.asfortranarray(np.random.default_rng(2).standard_normal((g.nyg - 2, g.nzg - 2)))
     uc = 0.9

@@ -121,16 +121,50 @@ def test_duct_solve_and_device_resident():
     plan.close()
 
 
-def test_duct_refusals_on_the_device():
+def test_duct_refuses_a_z_split():
     g = dfp.duct_grid(16, 8, 12, 1.5)
     with pytest.raises(dfp.DfpError) as ei:
-        dfp.Plan(g, tuning=(4, 0, 0, 0))  # a z split: the duct has no z FFT
+        dfp.Plan(g, tuning=(4, 0, 0, 0))  # the duct has no z FFT to split
     assert ei.value.code == -2
+
+
+def test_duct_boundary_conditions_and_substep():
+    """apply_boundary_conditions with the z walls (boundary_conditions.f90:
+    215-223, :737-758) bit-identical, and one RK3 substage of the duct
+    (time_integration.f90:256-304) to 1e-12."""
+    g = dfp.duct_grid(16, 10, 12, 1.5, 1.2)
+    rng = np.random.default_rng(8)
+    F3 = [np.asfortranarray(rng.standard_normal(s)) for s in (g.shape_u, g.shape_v, g.shape_w)]
+    al = (np.asfortranarray(rng.uniform(0, 0.2, (g.nx, 2, g.nzg))),
+          np.asfortranarray(rng.uniform(0, 0.2, (g.nxg, 2, g.nzg))),
+          np.asfortranarray(rng.uniform(0, 0.2, (g.nxg, 2, g.nz))))
+    op = ref_c.CProjection(g)
+    o = [a.copy(order="F") for a in F3]
+    op.apply_boundary_conditions(*o, *al)
     plan = dfp.Plan(g)
-    U, V, W = (dfp.DeviceArray(s, zero=True) for s in (g.shape_u, g.shape_v, g.shape_w))
-    with pytest.raises(dfp.DfpError) as ei:
-        plan.apply_bc(U, V, W)
-    assert ei.value.code == -2
+    d = [dfp.DeviceArray.from_host(a) for a in F3]
+    dal = [dfp.DeviceArray.from_host(a) for a in al]
+    plan.apply_bc(*d, *dal)
+    for x, y in zip(d, o):
+        assert np.array_equal(x.download(), y)
+    stage = 3
+    coef = np.array([1 / 4, 0.0, 3 / 4])
+    Uo = [np.asfortranarray(rng.standard_normal(s)) for s in (g.shape_u, g.shape_v, g.shape_w)]
+    inner = ((g.nx - 2, g.nyg - 2, g.nzg - 2), (g.nxg - 2, g.ny - 2, g.nzg - 2), (g.nxg - 2, g.nyg - 2, g.nz - 2))
+    Fr = [[np.asfortranarray(rng.standard_normal(sh)) for _ in range(3)] for sh in inner]
+    P = np.zeros(g.shape_p, order="F")
+    rhs = np.zeros(g.shape_rhs, order="F")
+    o = [a.copy(order="F") for a in (*F3, P, rhs)]
+    op.rk_update(stage, o[0], o[1], o[2], *Uo, *Fr, coef, DT)
+    op.apply_boundary_conditions(o[0], o[1], o[2], *al)
+    op.compute_projection_step(o[0], o[1], o[2], o[4], o[3], DT, coef[stage - 1])
+    op.apply_boundary_conditions(o[0], o[1], o[2], *al)
+    op.close()
+    d = [dfp.DeviceArray.from_host(a) for a in (*F3, P, rhs)]
+    plan.substep(stage, d[0], d[1], d[2], *[dfp.DeviceArray.from_host(a) for a in Uo],
+                 *[[dfp.DeviceArray.from_host(x) for x in f] for f in Fr], coef, *dal, d[4], d[3], DT, save_p=True)
+    for lab, x, y in zip(("U", "V", "W", "P", "phi"), d, o):
+        assert rel_linf(x.download(), y) <= TOL, lab
     plan.close()
 
 

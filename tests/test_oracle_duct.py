@@ -130,3 +130,30 @@ def test_c_oracle_equals_numpy_duct(dims):
     op.close()
     for x, y in zip(a, b):
         assert np.max(np.abs(x - y)) <= 1e-12 * max(1.0, np.max(np.abs(y)))
+
+
+def test_duct_boundary_conditions():
+    # boundary_conditions.f90:186-260 with z walls (:215-223, :737-758): W is
+    # zero on the wall faces, U and V vanish at the wall (ghost mirrored), x
+    # wraps hold, y Robin rows as in the channel; C == numpy bit for bit
+    from oracle import ref_c
+
+    g = R.duct_grid(8, 6, 5, 1.5, 1.0)
+    rng = np.random.default_rng(2)
+    U, V, W = (np.asfortranarray(rng.standard_normal(s)) for s in (g.shape_u, g.shape_v, g.shape_w))
+    al = (np.asfortranarray(rng.uniform(0, 0.2, (g.nx, 2, g.nzg))),
+          np.asfortranarray(rng.uniform(0, 0.2, (g.nxg, 2, g.nzg))),
+          np.asfortranarray(rng.uniform(0, 0.2, (g.nxg, 2, g.nz))))
+    a = [x.copy(order="F") for x in (U, V, W)]
+    R.apply_boundary_conditions(g, *a, *al)
+    Ua, Va, Wa = a
+    assert np.all(Wa[:, :, 0] == 0.0) and np.all(Wa[:, :, -1] == 0.0)
+    for F in (Ua, Va):
+        assert np.array_equal(F[:, 1:-1, 0], -F[:, 1:-1, 1]) and np.array_equal(F[:, 1:-1, -1], -F[:, 1:-1, -2])
+    assert np.array_equal(Ua[0], Ua[g.nx - 2]) and np.array_equal(Ua[g.nx - 1], Ua[1])
+    b = [x.copy(order="F") for x in (U, V, W)]
+    op = ref_c.CProjection(g)
+    op.apply_boundary_conditions(*b, *al)
+    op.close()
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
