@@ -346,6 +346,7 @@ struct ora_plan {
     double *lam_z;    /* duct: eigenvalues of the z-wall operator, ascending (nzp) */
     double *Qz;       /* duct: orthonormal eigenvectors of S, column m at Qz + m*nzp */
     double *sqrt_w;   /* duct: square roots of the z cell widths */
+    double *dzz_l, *dzz_d, *dzz_u; /* z walls with a periodic y: Dzz(k+1,k), Dzz(k,k), Dzz(k,k+1) */
     cplx *work; /* poisson_y_c at np = 1: nxp*nym*nzp */
     int nthreads;
 };
@@ -541,9 +542,10 @@ ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
     if (g->y_bc_type != 0 && g->y_bc_type != 1) return NULL;
     /* periodic y needs periodic x (initialization.f90:96-97) and two rows */
     if (g->y_bc_type == 0 && (g->x_bc_type != 0 || g->nym < 2)) return NULL;
-    /* z walls: the 4-wall duct with a periodic x (initialization.f90:98-100) */
+    /* z walls with a periodic x (initialization.f90:98-100): the 4-wall duct
+       (y walls) or the spanwise walls with a periodic y */
     if (g->z_bc_type != 0 && g->z_bc_type != 1) return NULL;
-    if (g->z_bc_type == 1 && (g->x_bc_type != 0 || g->y_bc_type != 1)) return NULL;
+    if (g->z_bc_type == 1 && g->x_bc_type != 0) return NULL;
     ora_plan *p = (ora_plan *)calloc(1, sizeof(ora_plan));
     p->xbc = g->x_bc_type;
     p->ybc = g->y_bc_type;
@@ -602,7 +604,34 @@ ora_plan *ora_plan_create(const ora_grid *g, int nthreads) {
         wavenumbers(p->nym - 1, p->y[1] - p->y[0], p->kyy);
         fplan_init(&p->fy, p->nym - 1);
     }
-    if (p->zbc) duct_eigenbasis(p);
+    if (p->zbc && p->ybc) duct_eigenbasis(p);
+    if (p->zbc && !p->ybc) {
+        /* initialization.f90:544-577: Dzz (Dyy's construction on z, zg), row kk = k-2 */
+        const int n = p->nzp, nzg = p->nzg;
+        const double *z = p->z, *zg = p->zg;
+        p->dzz_l = (double *)calloc((size_t)(n > 1 ? n - 1 : 1), sizeof(double));
+        p->dzz_u = (double *)calloc((size_t)(n > 1 ? n - 1 : 1), sizeof(double));
+        p->dzz_d = (double *)calloc((size_t)n, sizeof(double));
+        for (int k = 2; k <= nzg - 1; k++) {
+            const double a = 1.0 / (z[k - 1] - z[k - 2]) / (zg[k] - zg[k - 1]);
+            const double b = 1.0 / (z[k - 1] - z[k - 2]) * (-1.0 / (zg[k] - zg[k - 1]) - 1.0 / (zg[k - 1] - zg[k - 2]));
+            const double c = 1.0 / (z[k - 1] - z[k - 2]) / (zg[k - 1] - zg[k - 2]);
+            const int kk = k - 2;
+            if (k == 2 && k == nzg - 1) {
+                p->dzz_d[kk] = a + b;
+            } else if (k == 2) {
+                p->dzz_d[kk] = b + c;
+                p->dzz_u[kk] = a;
+            } else if (k == nzg - 1) {
+                p->dzz_d[kk] = a + b;
+                p->dzz_l[kk - 1] = c;
+            } else {
+                p->dzz_u[kk] = a;
+                p->dzz_l[kk - 1] = c;
+                p->dzz_d[kk] = b;
+            }
+        }
+    }
     p->work = (cplx *)malloc(sizeof(cplx) * (size_t)p->nxp * p->nym * p->nzp);
 #ifdef _OPENMP
     p->nthreads = nthreads > 0 ? nthreads : omp_get_max_threads();
@@ -620,7 +649,8 @@ void ora_plan_destroy(ora_plan *p) {
     free(p->Pb); free(p->phib);
     fplan_free(&p->fx); fplan_free(&p->fz);
     if (!p->ybc) { free(p->kyy); fplan_free(&p->fy); }
-    if (p->zbc) { free(p->lam_z); free(p->Qz); free(p->sqrt_w); }
+    if (p->zbc && p->ybc) { free(p->lam_z); free(p->Qz); free(p->sqrt_w); }
+    if (p->zbc && !p->ybc) { free(p->dzz_l); free(p->dzz_d); free(p->dzz_u); }
     free(p->work);
     free(p);
 }
@@ -839,8 +869,52 @@ int ora_solve(ora_plan *p, double *rhs_p, double *P, double dt, double c) {
         fft_many(&p->fx, wk, (long)nym * nzp, 1, nxp, -1, p->nthreads);
     }
     int info_all = 0;
-    if (p->zbc) {
+    if (p->zbc && p->ybc) {
         info_all = duct_solve(p, wk);
+        goto inverse;
+    }
+    if (p->zbc) {
+        /* spanwise walls with a periodic y: :143-150 y FFT over the nyp rows of
+           each z slab; :219-245 one Zgtsv along z per (i, j) with Dzz + kxx +
+           kyy, D(2) *= 3/2 where i = j = 0; :311-315 inverse y FFT, /nyp */
+        const int nyp = nym - 1;
+        for (int k = 0; k < nzp; k++) fft_many(&p->fy, wk + (size_t)k * nym * nxp, nxp, nxp, 1, -1, p->nthreads);
+#pragma omp parallel num_threads(p->nthreads)
+        {
+            double *DL = (double *)malloc(sizeof(double) * (size_t)(nzp > 1 ? nzp - 1 : 1));
+            double *DU = (double *)malloc(sizeof(double) * (size_t)(nzp > 1 ? nzp - 1 : 1));
+            double *D = (double *)malloc(sizeof(double) * (size_t)nzp);
+            cplx *b = (cplx *)malloc(sizeof(cplx) * (size_t)nzp);
+#pragma omp for schedule(static)
+            for (int j = 0; j < nyp; j++)
+                for (int i = 0; i < nxp; i++) {
+                    const double ws = p->kxx[i] + p->kyy[j];
+                    for (int k = 0; k < nzp - 1; k++) {
+                        DL[k] = p->dzz_l[k];
+                        DU[k] = p->dzz_u[k];
+                    }
+                    for (int k = 0; k < nzp; k++) D[k] = p->dzz_d[k] + ws;
+                    if (i == 0 && j == 0) D[0] = 3.0 / 2.0 * D[0];
+                    cplx *line = wk + (size_t)j * nxp + i;
+                    for (int k = 0; k < nzp; k++) b[k] = line[(size_t)k * nym * nxp];
+                    const int info = zgtsv_real_coef(nzp, DL, D, DU, b);
+                    if (info != 0) {
+#pragma omp atomic write
+                        info_all = info;
+                    }
+                    for (int k = 0; k < nzp; k++) line[(size_t)k * nym * nxp] = b[k];
+                }
+            free(DL); free(DU); free(D); free(b);
+        }
+        for (int k = 0; k < nzp; k++) fft_many(&p->fy, wk + (size_t)k * nym * nxp, nxp, nxp, 1, +1, p->nthreads);
+#pragma omp parallel for schedule(static) num_threads(p->nthreads)
+        for (int k = 0; k < nzp; k++)
+            for (int j = 0; j < nyp; j++)
+                for (int i = 0; i < nxp; i++) {
+                    cplx *v = wk + ((size_t)k * nym + j) * nxp + i;
+                    v->re = v->re / (double)nyp;
+                    v->im = v->im / (double)nyp;
+                }
         goto inverse;
     }
     /* :134-136 C2C forward along z, stride nxp*nym, batch nxp*nym */

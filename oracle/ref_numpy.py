@@ -48,6 +48,14 @@ and the 4-wall duct (``x_bc_type=0, y_bc_type=1, z_bc_type=1``):
                                 inverse x with Real()/nxp (:326-331)
 * no z periodic fill .......... src/projection.f90:472-497; Neumann z ghosts of P (:423-436)
 
+and the spanwise walls with a periodic y (``x_bc_type=0, y_bc_type=0, z_bc_type=1``, uniform y):
+
+* solve ....................... src/projection.f90:143-150 (y FFT over the nyp = nym-1 rows,
+                                left in y-wavenumber space), :219-245 (one Zgtsv along z per
+                                (kx, ky) with Dzz + kxx + kyy, the pin at kx = ky = 0),
+                                :311-315 (inverse y FFT, /nyp), inverse x Real()/nxp, the
+                                duplicate row (:350-354)
+
 LAPACK ``zgtsv`` (unpinned LAPACK version, CMakeLists.txt:50-51; the routine's
 algorithm is unchanged since LAPACK 3.0) is restated from its published
 algorithm: Gaussian elimination with partial pivoting on the tridiagonal,
@@ -130,9 +138,10 @@ class ChannelGrid:
         if self.y_bc_type == 0 and (self.grid_type != 1 or self.x_bc_type != 0 or self.nym < 2):
             # initialization.f90:96-97, :128-129
             raise ValueError("periodic y needs a uniform y grid, a periodic x and nym >= 2")
-        if self.z_bc_type == 1 and (self.x_bc_type != 0 or self.y_bc_type != 1):
-            # the duct: initialization.f90:98-100 (GPU builds), y walls here
-            raise ValueError("z walls are restated for the duct: periodic x and y walls")
+        if self.z_bc_type == 1 and self.x_bc_type != 0:
+            # initialization.f90:98-100 (GPU builds): z walls with a periodic x,
+            # y walls (the duct) or a periodic y
+            raise ValueError("z walls are restated with a periodic x")
         nx, ny, nz = self.nx, self.ny, self.nz
         # genGridandIC.f90:23-25, :38-40  x(i) = Lx*(i-1)/(nx-1)
         self.x = np.array([self.Lx_i * float(i - 1) / float(nx - 1) for i in range(1, nx + 1)])
@@ -230,6 +239,13 @@ def periodic_y_grid(nxp, nym, nzp, Lx=2.0 * PI, Ly=2.0, Lz=PI):
     y cells duplicates the first, so the namelist length spans nym cells of
     Ly/(nym-1) and the period is Ly."""
     return ChannelGrid(nxp, nym, nzp, Lx * (nxp + 1) / nxp, Ly * nym / (nym - 1), Lz * (nzp + 1) / nzp, 0.0, 1, 0, 0)
+
+
+def zwall_periodic_y_grid(nxp, nym, nzm, alpha_z=0.0, Lx=2.0 * PI, Ly=2.0, Lz=2.0):
+    """Spanwise walls with a periodic x and y: uniform y of period Ly (the
+    last of the nym rows duplicates the first), z walls Lz apart with nzm
+    cells (uniform, or symmetric tanh with alpha_z > 0)."""
+    return ChannelGrid(nxp, nym, nzm, Lx * (nxp + 1) / nxp, Ly * nym / (nym - 1), Lz, 0.0, 1, 0, 0, 1, alpha_z)
 
 
 def duct_grid(nxp, nym, nzm, alpha, alpha_z=0.0, Lx=2.0 * PI, Ly=2.0, Lz=2.0):
@@ -411,9 +427,13 @@ class Projection:
         self.Dyy = dyy_matrix(g.y, g.yg)
         # periodic y: initialization.f90:481-489 (nyp = nym-1, dy = y(2)-y(1))
         self.kyy = wavenumbers(g.nym - 1, float(g.y[1] - g.y[0])) if getattr(g, "y_bc_type", 1) == 0 else None
-        self.duct = getattr(g, "z_bc_type", 0) == 1
-        if self.duct:
+        self.zwall = getattr(g, "z_bc_type", 0) == 1
+        self.duct = self.zwall and self.kyy is None
+        if self.zwall:
             self.kzz = None
+            # initialization.f90:544-577: Dzz, Dyy's construction on z, zg
+            self.Dzz = dyy_matrix(g.z, g.zg)
+        if self.duct:
             self.lambda_z, self.Qz, self.sqrt_w = z_eigenbasis(g.z, g.zg)
         # outlet backflow stabilisation (global.f90:188-197): off unless beta > 0
         self.beta = 0.0
@@ -481,6 +501,8 @@ class Projection:
             c = scipy.fft.dct(ry, type=4, axis=0).astype(np.complex128)
         if self.duct:
             return self._duct_solve(c, rhs_p, P, dt, rk_coef33)
+        if self.zwall:
+            return self._zwall_y_solve(c, rhs_p, P, dt, rk_coef33)
         # :134-136 complex FFT along z
         c = np.fft.fft(c, axis=2)
         if self.kyy is not None:
@@ -540,16 +562,42 @@ class Projection:
         c = np.einsum("ijm,km->ijk", zs, self.Qz) / self.sqrt_w[None, None, :]
         return self._inverse_xz(c, rhs_p, P, dt, rk_coef33)
 
+    def _zwall_y_solve(self, c, rhs_p, P, dt, rk_coef33):
+        # projection.f90:143-150: y FFT over the nyp = nym-1 rows; :219-245: one
+        # Zgtsv along z per (kx, ky) with Dzz + kxx + kyy, D(2) *= 3/2 where
+        # kx = ky = 0; :311-315: inverse y FFT, /nyp
+        g = self.g
+        nxp, nym, nzm = g.nxp, g.nym, g.nzp
+        nyp = nym - 1
+        cy = np.fft.fft(c[:, :nyp, :], axis=1)
+        dl1 = np.array([self.Dzz[k + 1, k] for k in range(nzm - 1)])
+        du1 = np.array([self.Dzz[k, k + 1] for k in range(nzm - 1)])
+        M = nxp * nyp
+        D = np.diag(self.Dzz)[None, :] + (self.kxx[:, None] + self.kyy[None, :]).reshape(M, order="F")[:, None]
+        D = D.copy()
+        D[0, 0] = 3.0 / 2.0 * D[0, 0]                       # mode i = 0, j = 0, row k = 2
+        B = cy.reshape(M, nzm, order="F")                   # B[i + nxp*j, k]
+        X, info = zgtsv(np.broadcast_to(dl1, (M, nzm - 1)), D, np.broadcast_to(du1, (M, nzm - 1)), B)
+        if np.any(info != 0):
+            raise FloatingPointError("zgtsv reported a zero pivot")
+        cy = X.reshape(nxp, nyp, nzm, order="F")
+        c = c.copy()
+        c[:, :nyp, :] = np.fft.ifft(cy, axis=1) * nyp / float(nyp)
+        return self._inverse_xz(c, rhs_p, P, dt, rk_coef33)
+
     def _inverse_xz(self, c, rhs_p, P, dt, rk_coef33):
         g = self.g
         nxp, nym, nzp = g.nxp, g.nym, g.nzp
         xdct = g.x_bc_type == 1
-        if self.duct:
+        if self.zwall:
             # :326-331 z solved in physical space: Real()/nxp
             c = np.fft.ifft(c, axis=0) * nxp
             phi = c.real / float(nxp)
             rhs_p[0:nxp, 0:nym, 0:nzp] = phi
             self.apply_periodic_xz_pressure(rhs_p)
+            if self.kyy is not None:
+                # :350-354 periodic y: the duplicate row
+                rhs_p[:, nym - 1, :] = rhs_p[:, 0, :]
             if P is not None:
                 P[1:g.nxg - 1, 1:g.nyg - 1, 1:g.nzg - 1] = rhs_p[0:g.nxg - 2, :, 0:g.nzg - 2] / (dt * rk_coef33)
                 self.finish_pressure_ghosts(P)

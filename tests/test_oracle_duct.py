@@ -157,3 +157,82 @@ def test_duct_boundary_conditions():
     op.close()
     for x, y in zip(a, b):
         assert np.array_equal(x, y)
+
+
+# spanwise walls with a periodic y (x periodic, y periodic, z walls)
+ZWPY = [(8, 5, 6, 0.0), (6, 9, 7, 1.5), (4, 4, 3, 0.0), (10, 7, 5, 2.0)]
+
+
+def _dense_zwall_py(g):
+    nxp, nyp, nzm = g.nxp, g.nym - 1, g.nzp
+    dy = float(g.y[1] - g.y[0])
+    Dzz = R.dyy_matrix(g.z, g.zg)
+    n = nxp * nyp * nzm
+    idx = lambda i, j, k: (i % nxp) + nxp * ((j % nyp) + nyp * k)
+    A = np.zeros((n, n))
+    for k in range(nzm):
+        for j in range(nyp):
+            for i in range(nxp):
+                r = idx(i, j, k)
+                for (a, b), h in (((1, 0), g.dx), ((0, 1), dy)):
+                    A[r, idx(i + a, j + b, k)] += 1.0 / h ** 2
+                    A[r, idx(i - a, j - b, k)] += 1.0 / h ** 2
+                    A[r, r] -= 2.0 / h ** 2
+                for kk in range(nzm):
+                    A[r, idx(i, j, kk)] += Dzz[k, kk]
+    # the pin of the kx = ky = 0 mode on its first z row: 0.5 Dzz(2,2) times
+    # the x-y mean of phi on the first z row
+    for j in range(nyp):
+        for i in range(nxp):
+            for jj in range(nyp):
+                for ii in range(nxp):
+                    A[idx(i, j, 0), idx(ii, jj, 0)] += 0.5 * Dzz[0, 0] / (nxp * nyp)
+    return A
+
+
+@pytest.mark.parametrize("dims", ZWPY)
+def test_zwall_periodic_y_solve_matches_dense_operator(dims):
+    g = R.zwall_periodic_y_grid(*dims)
+    rng = np.random.default_rng(sum(dims[:3]))
+    _, _, _, _, rhs = R.new_fields(g)
+    r = rng.standard_normal((g.nxp, g.nym, g.nzp))
+    rhs[:g.nxp, :, :g.nzp] = r
+    R.Projection(g).solve_poisson_equation(rhs)
+    nyp = g.nym - 1
+    phi = rhs[:g.nxp, :nyp, :g.nzp]
+    lhs = (_dense_zwall_py(g) @ phi.reshape(-1, order="F")).reshape(phi.shape, order="F")
+    assert np.max(np.abs(lhs - r[:, :nyp, :])) <= 1e-10 * np.max(np.abs(r))
+    # the duplicate row, the x duplicate, no z fill
+    assert np.array_equal(rhs[:, g.nym - 1, :], rhs[:, 0, :]) and np.array_equal(rhs[g.nxp], rhs[0])
+    assert np.all(rhs[:g.nxp, :, g.nzp] == 0.0)
+
+
+def _refresh_xy(g, U, V, W):
+    _refresh_x(g, U, V, W)
+    V[:, 0, :] = V[:, g.ny - 2, :]; V[:, g.ny - 1, :] = V[:, 1, :]
+    for F in (U, W):
+        F[:, 0, :] = F[:, g.nyg - 3, :]; F[:, g.nyg - 2, :] = F[:, 1, :]; F[:, g.nyg - 1, :] = F[:, 2, :]
+
+
+@pytest.mark.parametrize("dims", ZWPY)
+def test_zwall_periodic_y_projection_is_divergence_free_and_c_equals_numpy(dims):
+    from oracle import ref_c
+
+    g = R.zwall_periodic_y_grid(*dims)
+    rng = np.random.default_rng(40 + dims[0])
+    U, V, W, P, rhs = R.new_fields(g)
+    for F in (U, V, W):
+        F[...] = rng.standard_normal(F.shape)
+    W[:, :, 0] = 0.0; W[:, :, -1] = 0.0
+    _refresh_xy(g, U, V, W)
+    a = [x.copy(order="F") for x in (U, V, W, P, rhs)]
+    b = [x.copy(order="F") for x in (U, V, W, P, rhs)]
+    pr = R.Projection(g)
+    pr.compute_projection_step(a[0], a[1], a[2], a[4], a[3], 1e-3, 0.75)
+    op = ref_c.CProjection(g)
+    op.compute_projection_step(b[0], b[1], b[2], b[4], b[3], 1e-3, 0.75)
+    op.close()
+    for x, y in zip(a, b):
+        assert np.max(np.abs(x - y)) <= 1e-12 * max(1.0, np.max(np.abs(y)))
+    _refresh_xy(g, a[0], a[1], a[2])
+    assert pr.check_divergence(a[0], a[1], a[2]) <= 1e-10
