@@ -309,14 +309,16 @@ def duct_bytes(N: int, H: int) -> dict:
             "zeig_inv": 32 * H, "irfft_x": 16 * H + 8 * N, "correct": 56 * N}
 
 
-def bench_duct(K: int, Wm: int) -> dict:
+def bench_duct(K: int, Wm: int, periodic_y: bool = False) -> dict:
     """The 4-wall duct at 256x192x256 cells (y and z walls, tanh stretching
-    2.0 in both): same RK3 loop as the FP64 line, fields resident in HBM.
-    The z eigenbasis GEMMs are FP64 compute: 2 x 4 x nxh*nym*nzm^2 flops."""
+    2.0 in both), or with periodic_y the spanwise walls with a periodic y at
+    256x193x256 (192 independent y rows, z stretching 2.0): same RK3 loop as
+    the FP64 line, fields resident in HBM.  The z eigenbasis GEMMs are FP64
+    compute: 2 x 4 x nxh*nym*nzm^2 flops."""
     import dfp
     from dfp import hipmem
 
-    g = dfp.duct_grid(NXP, NYM, NZP, ALPHA, ALPHA)
+    g = dfp.zwall_periodic_y_grid(NXP, NYM + 1, NZP, ALPHA) if periodic_y else dfp.duct_grid(NXP, NYM, NZP, ALPHA, ALPHA)
     N, H = g.n_cells, g.n_modes
     plan = dfp.Plan(g)
     sets = make_sets(plan, g, 12, 0x5EED5000)
@@ -709,6 +711,7 @@ def main() -> int:
         out["substep"] = bench_substep(max(2, K), Wm)
     if not args.no_duct:
         out["duct"] = bench_duct(max(2, K // 2), Wm)
+        out["zwall_periodic_y"] = bench_duct(max(2, K // 2), Wm, periodic_y=True)
     if not args.no_batched:
         out["batched"] = bench_batched(save_tuning=args.save_tuning)
     if not args.no_serving:

@@ -184,11 +184,17 @@ int choose_shape(const dfp_grid* g, int f32, size_t lds_limit, const dfp_tuning*
     const int want_G = tune ? tune->z_rows : 0, want_f = tune ? tune->z_fused : 0;
     if (want_R < 0 || want_T < 0 || want_G < 0 || want_f < 0 || want_f > 2) return DFP_E_ARG;
     if (g->z_bc_type == 1) {
-        // the duct: no z transform (R = 1), the z eigenbasis kernels instead
+        // z walls: no z transform (R = 1), the z eigenbasis kernels instead
+        // (with a periodic y, the y transform of k_yfft_solve between them)
         if ((want_R > 1) || want_G > 0 || want_f > 0) return DFP_E_UNSUPPORTED;
         Shape sh;
         if (!factor(g->nxp / 2, sh.fx)) return DFP_E_UNSUPPORTED;
         factor(1, sh.fz);
+        if (g->y_bc_type == 0) {
+            if (!factor(g->nym - 1, sh.fy)) return DFP_E_UNSUPPORTED;
+            sh.gy = yfft_solve_columns(sh.fy, g->nym, lds_limit);
+            if (sh.gy == 0) return DFP_E_UNSUPPORTED;
+        }
         sh.T = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 0);
         sh.Ti = x_columns(sh, g->nxp, g->nzp, lds_limit, cs, want_T, 1);
         if (sh.T == 0 || sh.Ti == 0) return DFP_E_UNSUPPORTED;
@@ -280,6 +286,7 @@ void free_plan_memory(dfp_plan* p) {
     (void)hipFree(p->d_lu);
     (void)hipFree(p->d_spec2);
     (void)hipFree(p->d_duct);
+    (void)hipFree(p->d_z00);
     (void)hipFree(p->d_red);
     (void)hipFree(p->d_U);
     (void)hipFree(p->d_V);
@@ -499,8 +506,9 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     if (g->x_bc_type != 0 && g->x_bc_type != 1) return DFP_E_UNSUPPORTED;
     if ((g->y_bc_type != 0 && g->y_bc_type != 1) || (g->z_bc_type != 0 && g->z_bc_type != 1)) return DFP_E_UNSUPPORTED;
     // z walls: the 4-wall duct with a periodic x and y walls (initialization.f90:98-100), FP64
-    if (g->z_bc_type == 1 && (g->x_bc_type != 0 || g->y_bc_type != 1 || precision != DFP_FP64))
-        return DFP_E_UNSUPPORTED;
+    if (g->z_bc_type == 1 && (g->x_bc_type != 0 || precision != DFP_FP64)) return DFP_E_UNSUPPORTED;
+    // the pinned kx = ky = 0 line of the z walls with a periodic y is solved in one workgroup's LDS
+    if (g->z_bc_type == 1 && g->y_bc_type == 0 && g->nzp > 2048) return DFP_E_UNSUPPORTED;
     // periodic y (initialization.f90:96-97, :128-129): periodic x, a uniform
     // y grid and nym-1 = 2^a 3^b independent rows; solved in FP64
     if (g->y_bc_type == 0 && (g->x_bc_type != 0 || precision != DFP_FP64 || g->nym < 2))
@@ -639,6 +647,12 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     const size_t o_kxx = put(kxx_half), o_kzz = put(p->hkzz);
     const size_t o_dl = put(p->hdl), o_dd = put(p->hdd), o_du = put(p->hdu);
     const size_t o_kyy = hkyy.empty() ? (size_t)-1 : put(hkyy);
+    // z walls with a periodic y: Dzz (Dyy's construction on z, zg; initialization.f90:544-577)
+    std::vector<double> zl, zd, zu;
+    if (G.zbc == 1 && G.ybc == 0) dyy(p->hz, p->hzg, zl, zd, zu);
+    const size_t o_zl = zd.empty() ? (size_t)-1 : put(zl);
+    const size_t o_zd = zd.empty() ? (size_t)-1 : put(zd);
+    const size_t o_zu = zd.empty() ? (size_t)-1 : put(zu);
     const size_t o_x = p->hx.empty() ? (size_t)-1 : put(p->hx);
     const size_t o_y = put(p->hy), o_z = put(p->hz), o_yg = put(p->hyg);
     if (hipMalloc(&p->d_tables, sizeof(double) * tab.size()) != hipSuccess) return bail(DFP_E_NOMEM);
@@ -651,6 +665,11 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     p->ops.kxx = T + o_kxx; p->ops.kzz = T + o_kzz;
     p->ops.dl = T + o_dl; p->ops.dd = T + o_dd; p->ops.du = T + o_du;
     p->ops.kyy = o_kyy == (size_t)-1 ? nullptr : T + o_kyy;
+    if (o_zd != (size_t)-1) {
+        p->dzz_l = T + o_zl;
+        p->dzz_d = T + o_zd;
+        p->dzz_u = T + o_zu;
+    }
 
     // twiddles, in the order the x kernels stage them into LDS with one
     // copy: x stages, R2C/C2R packing (M+1) or the DCT-IV pre-twiddles
@@ -704,10 +723,13 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
 
     const size_t H = (size_t)G.nxh * G.nym * G.nzp;
     if (hipMalloc(&p->d_spec, sizeof(double2) * H) != hipSuccess) return bail(DFP_E_NOMEM);
-    if (G.zbc == 1) {  // the duct: eigenbasis tables, second spectrum, pivots per (kx, eigenmode)
+    if (G.zbc == 1) {  // z walls: eigenbasis tables, second spectrum; the duct's pivots per (kx, eigenmode)
         if (hipMalloc(&p->d_duct, sizeof(double) * duct_tab.size()) != hipSuccess ||
-            hipMalloc(&p->d_spec2, sizeof(double2) * H) != hipSuccess ||
-            hipMalloc(&p->d_lu, sizeof(double) * (size_t)G.nxh * G.nym * G.nzp) != hipSuccess)
+            hipMalloc(&p->d_spec2, sizeof(double2) * H) != hipSuccess)
+            return bail(DFP_E_NOMEM);
+        if (G.ybc == 1 && hipMalloc(&p->d_lu, sizeof(double) * (size_t)G.nxh * G.nym * G.nzp) != hipSuccess)
+            return bail(DFP_E_NOMEM);
+        if (G.ybc == 0 && hipMalloc(&p->d_z00, sizeof(double2) * (size_t)G.nzp) != hipSuccess)
             return bail(DFP_E_NOMEM);
         if (hipMemcpy(p->d_duct, duct_tab.data(), sizeof(double) * duct_tab.size(), hipMemcpyHostToDevice) != hipSuccess)
             return bail(DFP_E_HIP);
@@ -715,7 +737,8 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
         p->duct_qf = p->d_duct;
         p->duct_qi = p->d_duct + nn;
         p->duct_lam = p->d_duct + 2 * nn;
-        if (launch_duct_lu(p) != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) return bail(DFP_E_HIP);
+        if (G.ybc == 1 && (launch_duct_lu(p) != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess))
+            return bail(DFP_E_HIP);
     } else if (G.ybc == 1) {  // pivots of the wall solve
         const size_t nlu = (size_t)G.nxh * G.nym * (G.nzp / 2 + 1);
         if (hipMalloc(&p->d_lu, sizeof(double) * nlu) != hipSuccess) return bail(DFP_E_NOMEM);

@@ -211,6 +211,10 @@ struct dfp_plan {
     const double* duct_qi = nullptr;
     const double* duct_lam = nullptr;
     int pin_m = -1;
+    // z walls with a periodic y: Dzz's three diagonals (in d_tables) and the
+    // eigenbasis image of the pinned kx = ky = 0 line
+    const double *dzz_l = nullptr, *dzz_d = nullptr, *dzz_u = nullptr;
+    double2* d_z00 = nullptr;
     int zr = 1;                  // radix of the z stage applied by the x kernels (z split)
     const double2* tw_x = nullptr;
     const double2* tw_z = nullptr;
@@ -283,6 +287,8 @@ size_t zf_lds_bytes(int nym, int n, int ntw, int f32);
 // the duct: z eigenbasis transforms and y solves of the spectrum (k_duct.hip)
 hipError_t launch_duct_lu(dfp_plan* p);
 hipError_t launch_duct_solve(dfp_plan* p);
+size_t yfft_solve_lds_bytes(int nym, int G, int ntwy);
+int yfft_solve_columns(const FftDesc& fy, int nym, size_t lds_limit);
 // periodic y: z transform + y FFT solve of each (kx, u) sub-plane in LDS (FP64)
 hipError_t launch_zfused_py(dfp_plan* p);
 size_t zf_py_lds_bytes(int nym, int n, int ntwz, int ntwy);

@@ -14,6 +14,15 @@
 //                  (kx, m), in place
 // Spectrum layout: spec[(kx*nym + j)*nzp + k] (the x kernels' layout with no
 // z split), and the eigenmode planes the same with m in place of k.
+//
+// The spanwise walls with a periodic y (projection.f90:143-150, :219-245,
+// :311-315) use the same z eigenbasis: between the two GEMMs each x mode's
+// columns are transformed along y (the nyp = nym-1 independent rows) and
+// divided by kxx + kyy + lambda_z(m) (k_yfft_solve).  The reference solves
+// the kx = ky = 0 line along z with its pin D(2) *= 3/2, which fixes that
+// line's additive constant; k_zwall00 solves exactly that pinned system
+// (Thomas along z) and hands its eigenbasis image to k_yfft_solve, so phi
+// carries the reference's constant.
 #include "fft_lds.h"
 
 namespace dfp {
@@ -216,6 +225,129 @@ __global__ void __launch_bounds__(256) k_duct_ysolve(double2* __restrict__ S, co
     }
 }
 
+// The pinned kx = ky = 0 line, one workgroup: r(k) = sum over the nyp rows of
+// the kx = 0 plane (its ky = 0 y-mode); p = (Dzz with D(2) *= 3/2)^-1 r by
+// zgtsv's no-interchange elimination and back substitution (one lane);
+// z00(m) = sum_k p(k) Qf(k, m).  LDS: r and the inverse pivots, nzm each.
+constexpr int kZw00Threads = 256;
+__global__ void __launch_bounds__(kZw00Threads) k_zwall00(const double2* __restrict__ S, const double* __restrict__ qf,
+                                                          const double* __restrict__ dzl, const double* __restrict__ dzd,
+                                                          const double* __restrict__ dzu, double2* __restrict__ z00,
+                                                          int nym, int nzm) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* r = reinterpret_cast<double2*>(smem_raw);  // nzm
+    double* ip = reinterpret_cast<double*>(r + nzm);    // nzm
+    const int nyp = nym - 1;
+    for (int k = threadIdx.x; k < nzm; k += kZw00Threads) {
+        double sx = 0.0, sy = 0.0;
+        for (int j = 0; j < nyp; ++j) {
+            const double2 v = S[(long)j * nzm + k];
+            sx += v.x;
+            sy += v.y;
+        }
+        r[k] = make_double2(sx, sy);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double dp = 3.0 / 2.0 * dzd[0];
+        ip[0] = 1.0 / dp;
+        double yx = r[0].x, yy = r[0].y;
+        for (int k = 1; k < nzm; ++k) {
+            const double mult = dzl[k - 1] / dp;
+            dp = dzd[k] - mult * dzu[k - 1];
+            ip[k] = 1.0 / dp;
+            yx = r[k].x - mult * yx;
+            yy = r[k].y - mult * yy;
+            r[k] = make_double2(yx, yy);
+        }
+        double xx = 0.0, xy = 0.0;
+        for (int k = nzm - 1; k >= 0; --k) {
+            const double du = k < nzm - 1 ? dzu[k] : 0.0;
+            xx = (r[k].x - du * xx) * ip[k];
+            xy = (r[k].y - du * xy) * ip[k];
+            r[k] = make_double2(xx, xy);
+        }
+    }
+    __syncthreads();
+    for (int m = threadIdx.x; m < nzm; m += kZw00Threads) {
+        double sx = 0.0, sy = 0.0;
+        for (int k = 0; k < nzm; ++k) {
+            const double q = qf[(long)k * nzm + m];
+            sx = fma(r[k].x, q, sx);
+            sy = fma(r[k].y, q, sy);
+        }
+        z00[m] = make_double2(sx, sy);
+    }
+}
+
+// One workgroup per (kx, group of G eigenmode columns): the nym x G tile in
+// LDS, the y FFT down the columns over the nyp independent rows, each mode
+// divided by kxx + kyy + lambda_z(m) (the kx = ky = 0 row taken from z00),
+// the inverse FFT scaled by 1/nyp, row nym-1 set to row 0 (the duplicate,
+// projection.f90:350-354), stored back.  Columns past nzm are zero-filled
+// and not stored.
+constexpr int kYsThreads = 256;
+__global__ void __launch_bounds__(kYsThreads) k_yfft_solve(double2* __restrict__ S, const double2* __restrict__ twy_g,
+                                                           FftDesc fy, int ntwy, const double* __restrict__ kxx,
+                                                           const double* __restrict__ kyy,
+                                                           const double* __restrict__ lam_z,
+                                                           const double2* __restrict__ z00, int nym, int nzm, int G,
+                                                           FastDiv div_g) {
+    extern __shared__ __align__(16) unsigned char smem_raw[];
+    double2* A = reinterpret_cast<double2*>(smem_raw);  // nym x G
+    double2* twy = A + (long)nym * G;                   // ntwy
+    const int ngrp = (nzm + G - 1) / G;
+    const int kx = blockIdx.x / ngrp;
+    const int m0 = (blockIdx.x - kx * ngrp) * G;
+    const int nyp = nym - 1;
+    const auto plane = gptr(S) + (long)kx * nym * nzm;
+    for (int e = threadIdx.x; e < nym * G; e += kYsThreads) {
+        const int j = (int)div_g.div((uint32_t)e), c = e - j * G;
+        const int m = m0 + c;
+        A[e] = m < nzm ? gld<double2>(plane + (long)j * nzm + m) : make_double2(0.0, 0.0);
+    }
+    for (int e = threadIdx.x; e < ntwy; e += kYsThreads) twy[e] = twy_g[e];
+    __syncthreads();
+    lds_fft_cols<-1, 4>(A, 0, G, G, fy, twy, div_g);
+    const double lx = kxx[kx];
+    const double inv_ny = 1.0 / (double)nyp;
+    for (int e = threadIdx.x; e < nyp * G; e += kYsThreads) {
+        const int ky = (int)div_g.div((uint32_t)e), c = e - ky * G;
+        const int m = m0 + c;
+        if (m >= nzm) continue;
+        if (kx == 0 && ky == 0) {
+            // the pinned line's y-mode (the inverse FFT's 1/nyp applies to it too)
+            const double2 v = z00[m];
+            A[e] = make_double2(v.x * inv_ny, v.y * inv_ny);
+        } else {
+            const double2 v = A[e];
+            const double s = inv_ny / (lx + kyy[ky] + lam_z[m]);
+            A[e] = make_double2(v.x * s, v.y * s);
+        }
+    }
+    __syncthreads();
+    lds_fft_cols<1, 4>(A, 0, G, G, fy, twy, div_g);
+    for (int c = threadIdx.x; c < G; c += kYsThreads) A[(long)nyp * G + c] = A[c];
+    __syncthreads();
+    for (int e = threadIdx.x; e < nym * G; e += kYsThreads) {
+        const int j = (int)div_g.div((uint32_t)e), c = e - j * G;
+        const int m = m0 + c;
+        if (m < nzm) gst<double2>(plane + (long)j * nzm + m, A[e]);
+    }
+}
+
+size_t yfft_solve_lds_bytes(int nym, int G, int ntwy) { return sizeof(double2) * ((size_t)nym * G + ntwy); }
+
+// columns per k_yfft_solve workgroup: the largest power of two up to 16
+// whose columns one LDS transform call holds
+int yfft_solve_columns(const FftDesc& fy, int nym, size_t lds_limit) {
+    for (int G = 16; G >= 1; G /= 2)
+        if ((long)G * fy.n <= inplace_capacity(fy, kYsThreads, 4) &&
+            yfft_solve_lds_bytes(nym, G, twiddle_count(fy)) <= lds_limit)
+            return G;
+    return 0;
+}
+
 hipError_t launch_duct_lu(dfp_plan* p) {
     const Geo& g = p->geo;
     const long n = (long)g.nxh * g.nzp;
@@ -235,7 +367,27 @@ static hipError_t launch_zeig(dfp_plan* p, const double2* A, const double* B, do
 
 hipError_t launch_duct_solve(dfp_plan* p) {
     const Geo& g = p->geo;
-    hipError_t e = launch_zeig(p, p->d_spec, p->duct_qf, p->d_spec2, "zeig_fwd");
+    hipError_t e;
+    if (g.ybc == 0) {
+        // spanwise walls with a periodic y: the pinned kx = ky = 0 line, the
+        // forward z eigenbasis GEMM, the y transforms and divisions, the inverse GEMM
+        const size_t l00 = (sizeof(double2) + sizeof(double)) * (size_t)g.nzp;
+        int slot = prof_begin(p, "zwall00");
+        hipLaunchKernelGGL(k_zwall00, dim3(1), dim3(kZw00Threads), l00, p->stream, p->d_spec, p->duct_qf, p->dzz_l,
+                           p->dzz_d, p->dzz_u, p->d_z00, g.nym, g.nzp);
+        prof_end(p, slot);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        if ((e = launch_zeig(p, p->d_spec, p->duct_qf, p->d_spec2, "zeig_fwd")) != hipSuccess) return e;
+        const int G = p->gy, ngrp = ceil_div(g.nzp, G), ntwy = twiddle_count(p->fy);
+        slot = prof_begin(p, "yfft_solve");
+        hipLaunchKernelGGL(k_yfft_solve, dim3(g.nxh * ngrp), dim3(kYsThreads), yfft_solve_lds_bytes(g.nym, G, ntwy),
+                           p->stream, p->d_spec2, p->tw_y, p->fy, ntwy, p->ops.kxx, p->ops.kyy, p->duct_lam, p->d_z00,
+                           g.nym, g.nzp, G, FastDiv((uint32_t)G));
+        prof_end(p, slot);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+        return launch_zeig(p, p->d_spec2, p->duct_qi, p->d_spec, "zeig_inv");
+    }
+    e = launch_zeig(p, p->d_spec, p->duct_qf, p->d_spec2, "zeig_fwd");
     if (e != hipSuccess) return e;
     const long n = (long)g.nxh * g.nzp;
     const int slot = prof_begin(p, "duct_ysolve");

@@ -49,8 +49,8 @@ class ChannelGrid:
             raise ValueError("periodic y needs a uniform y grid (grid_type 1), a periodic x and nym >= 2")
         if self.z_bc_type not in (0, 1):
             raise ValueError("z_bc_type must be 0 (periodic) or 1 (walls)")
-        if self.z_bc_type == 1 and (self.x_bc_type != 0 or self.y_bc_type != 1):
-            raise ValueError("z walls are built as the 4-wall duct: periodic x and y walls")
+        if self.z_bc_type == 1 and self.x_bc_type != 0:
+            raise ValueError("z walls are built with a periodic x (the duct, or with a periodic y)")
         nx, ny, nz = self.nx, self.ny, self.nz
         self.x = np.array([self.Lx_i * float(i) / float(nx - 1) for i in range(nx)])
         if self.z_bc_type == 1 and self.alpha_z > 0.0:
@@ -127,6 +127,13 @@ def duct_grid(nxp, nym, nzm, alpha, alpha_z=0.0, Lx=2.0 * math.pi, Ly=2.0, Lz=2.
     stretching alpha, uniform if 0), z walls Lz apart with nzm cells (uniform,
     or symmetric tanh with alpha_z > 0)."""
     return ChannelGrid(nxp, nym, nzm, Lx * (nxp + 1) / nxp, Ly, Lz, alpha, 2 if alpha > 0 else 1, 0, 1, 1, alpha_z)
+
+
+def zwall_periodic_y_grid(nxp, nym, nzm, alpha_z=0.0, Lx=2.0 * math.pi, Ly=2.0, Lz=2.0) -> ChannelGrid:
+    """Spanwise walls with a periodic x and y: uniform y of period Ly (the
+    last of the nym rows duplicates the first), z walls Lz apart with nzm
+    cells (uniform, or symmetric tanh with alpha_z > 0)."""
+    return ChannelGrid(nxp, nym, nzm, Lx * (nxp + 1) / nxp, Ly * nym / (nym - 1), Lz, 0.0, 1, 0, 0, 1, alpha_z)
 
 
 def periodic_y_grid(nxp, nym, nzp, Lx=2.0 * math.pi, Ly=2.0, Lz=math.pi) -> ChannelGrid:

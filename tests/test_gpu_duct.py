@@ -174,3 +174,77 @@ def test_duct_full_size_parity():
     print("duct 256x192x256", flush=True)
     o, d = run_both(g, 23)
     _check(g, o, d, "duct256")
+
+
+# spanwise walls with a periodic y: (name, (nxp, nym, nzm, alpha_z))
+ZWPY_CASES = [
+    ("tiny_8x5x6", (8, 5, 6, 0.0)),
+    ("stretched_6x9x7", (6, 9, 7, 1.5)),
+    ("radix3_12x10x5", (12, 10, 5, 2.0)),
+    ("nym2_8x2x4", (8, 2, 4, 1.0)),
+    ("big_64x65x48", (64, 65, 48, 1.2)),
+    ("wide_96x33x130", (96, 33, 130, 2.0)),
+]
+
+
+def zwpy_fields(g, seed, sigma=0.3):
+    """As duct_fields with the periodic y: V, U and W periodic in y, W zero
+    on the z walls."""
+    from oracle import ref_numpy as R
+
+    rng = np.random.default_rng(seed)
+    U = np.asfortranarray(1.0 + sigma * rng.standard_normal(g.shape_u))
+    V = np.asfortranarray(sigma * rng.standard_normal(g.shape_v))
+    W = np.asfortranarray(sigma * rng.standard_normal(g.shape_w))
+    W[:, :, 0] = 0.0; W[:, :, -1] = 0.0
+    R.apply_boundary_conditions(g, U, V, W, None, None, None)
+    return U, V, W, np.zeros(g.shape_p, order="F"), np.zeros(g.shape_rhs, order="F")
+
+
+@pytest.mark.parametrize("name,dims", ZWPY_CASES, ids=[c[0] for c in ZWPY_CASES])
+def test_zwall_periodic_y_project(name, dims):
+    from oracle import ref_numpy as R
+
+    g = dfp.zwall_periodic_y_grid(*dims)
+    print(f"z walls, periodic y: {name}", flush=True)
+    U, V, W, P, rhs = zwpy_fields(g, 70 + dims[0])
+    o = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    op = ref_c.CProjection(g)
+    op.compute_projection_step(o[0], o[1], o[2], o[4], o[3], DT, C3)
+    plan = dfp.Plan(g)
+    d = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    plan.project(d[0], d[1], d[2], d[4], d[3], DT, C3, save_p=True)
+    plan.close()
+    for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+        err = rel_linf(a, b)
+        assert err <= TOL, f"{name} {lab}: rel-Linf {err:.3e}"
+    # the duplicate row carries row 0
+    assert np.max(np.abs(d[4][:, g.nym - 1, :] - d[4][:, 0, :])) <= 1e-14 * np.max(np.abs(d[4]))
+    U2, V2, W2 = (a.copy(order="F") for a in d[:3])
+    R.apply_boundary_conditions(g, U2, V2, W2, None, None, None)
+    md = op.check_divergence(U2, V2, W2)
+    op.close()
+    assert md <= div_bound(g, U2, V2, W2), (name, md)
+
+
+def test_zwall_periodic_y_full_size_parity():
+    """256 x 193 x 256 (192 independent y rows), z faces stretched 2.0."""
+    from oracle import ref_numpy as R
+
+    g = dfp.zwall_periodic_y_grid(256, 193, 256, 2.0)
+    print("z walls, periodic y: 256x193x256", flush=True)
+    U, V, W, P, rhs = zwpy_fields(g, 31)
+    o = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    op = ref_c.CProjection(g)
+    op.compute_projection_step(o[0], o[1], o[2], o[4], o[3], DT, C3)
+    plan = dfp.Plan(g)
+    d = [a.copy(order="F") for a in (U, V, W, P, rhs)]
+    plan.project(d[0], d[1], d[2], d[4], d[3], DT, C3, save_p=True)
+    plan.close()
+    for lab, a, b in zip(("U", "V", "W", "P", "phi"), d, o):
+        assert rel_linf(a, b) <= TOL, lab
+    U2, V2, W2 = (a.copy(order="F") for a in d[:3])
+    R.apply_boundary_conditions(g, U2, V2, W2, None, None, None)
+    md = op.check_divergence(U2, V2, W2)
+    op.close()
+    assert md <= div_bound(g, U2, V2, W2), md
