@@ -648,11 +648,27 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
     const size_t o_dl = put(p->hdl), o_dd = put(p->hdd), o_du = put(p->hdu);
     const size_t o_kyy = hkyy.empty() ? (size_t)-1 : put(hkyy);
     // z walls with a periodic y: Dzz (Dyy's construction on z, zg; initialization.f90:544-577)
-    std::vector<double> zl, zd, zu;
-    if (G.zbc == 1 && G.ybc == 0) dyy(p->hz, p->hzg, zl, zd, zu);
+    // and the elimination of its pinned kx = ky = 0 system (D(2) *= 3/2,
+    // projection.f90:241): multipliers DL(k-1)/D'(k-1) and inverse pivots 1/D'(k)
+    std::vector<double> zl, zd, zu, zmul, zip;
+    if (G.zbc == 1 && G.ybc == 0) {
+        dyy(p->hz, p->hzg, zl, zd, zu);
+        const int n = G.nzp;
+        zmul.assign(n, 0.0);
+        zip.assign(n, 0.0);
+        double dp = 3.0 / 2.0 * zd[0];
+        zip[0] = 1.0 / dp;
+        for (int k = 1; k < n; ++k) {
+            zmul[k] = zl[k - 1] / dp;
+            dp = zd[k] - zmul[k] * zu[k - 1];
+            zip[k] = 1.0 / dp;
+        }
+    }
     const size_t o_zl = zd.empty() ? (size_t)-1 : put(zl);
     const size_t o_zd = zd.empty() ? (size_t)-1 : put(zd);
     const size_t o_zu = zd.empty() ? (size_t)-1 : put(zu);
+    const size_t o_zm = zd.empty() ? (size_t)-1 : put(zmul);
+    const size_t o_zi = zd.empty() ? (size_t)-1 : put(zip);
     const size_t o_x = p->hx.empty() ? (size_t)-1 : put(p->hx);
     const size_t o_y = put(p->hy), o_z = put(p->hz), o_yg = put(p->hyg);
     if (hipMalloc(&p->d_tables, sizeof(double) * tab.size()) != hipSuccess) return bail(DFP_E_NOMEM);
@@ -669,6 +685,8 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
         p->dzz_l = T + o_zl;
         p->dzz_d = T + o_zd;
         p->dzz_u = T + o_zu;
+        p->zw_mult = T + o_zm;
+        p->zw_ip = T + o_zi;
     }
 
     // twiddles, in the order the x kernels stage them into LDS with one
@@ -762,6 +780,12 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
             hipMalloc(&p->d_part0, sizeof(double) * nparts_i) != hipSuccess ||
             hipMalloc(&p->d_pbar, sizeof(double) * G.nym) != hipSuccess)
             return bail(DFP_E_NOMEM);
+        if (hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming) != hipSuccess)
+            return bail(DFP_E_HIP);
+    } else if (G.zbc == 1 && G.ybc == 0) {
+        // the pinned kx = ky = 0 line of the z walls with a periodic y runs beside the forward GEMM
         if (hipStreamCreateWithFlags(&p->aux, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&p->fork_ev, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&p->join_ev, hipEventDisableTiming) != hipSuccess)

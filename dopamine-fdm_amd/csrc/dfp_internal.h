@@ -211,9 +211,11 @@ struct dfp_plan {
     const double* duct_qi = nullptr;
     const double* duct_lam = nullptr;
     int pin_m = -1;
-    // z walls with a periodic y: Dzz's three diagonals (in d_tables) and the
-    // eigenbasis image of the pinned kx = ky = 0 line
+    // z walls with a periodic y: Dzz's three diagonals, the multipliers DL/D'
+    // and inverse pivots of the pinned kx = ky = 0 z system (in d_tables),
+    // and that line's eigenbasis image
     const double *dzz_l = nullptr, *dzz_d = nullptr, *dzz_u = nullptr;
+    const double *zw_mult = nullptr, *zw_ip = nullptr;
     double2* d_z00 = nullptr;
     int zr = 1;                  // radix of the z stage applied by the x kernels (z split)
     const double2* tw_x = nullptr;
@@ -246,7 +248,8 @@ struct dfp_plan {
     double *d_U = nullptr, *d_V = nullptr, *d_W = nullptr, *d_rhs = nullptr, *d_P = nullptr;
     int64_t sz_u = 0, sz_v = 0, sz_w = 0, sz_p = 0, sz_rhs = 0;
     hipStream_t stream = nullptr;
-    // FP32 form: second stream for the (0,0)-mode solve beside the z solves
+    // second stream: the FP32 form's (0,0)-mode solve beside the z solves, or the
+    // pinned kx = ky = 0 line of the z walls with a periodic y beside the forward GEMM
     hipStream_t aux = nullptr;
     hipEvent_t fork_ev = nullptr, join_ev = nullptr;
     // profiling

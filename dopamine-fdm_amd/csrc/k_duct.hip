@@ -41,12 +41,14 @@ constexpr int kGemmKC = 32;  // k per LDS stage
 // are then the re and im parts of complex row l/16 and v = 2, 3 those of
 // row l/16 + 4, each pair stored as one 16-byte value.  A block's tile is 64
 // complex rows (128 real) x 64 columns; wave w computes real rows
-// 32w..32w+31 (two tile rows) x all 64 columns (four tile columns).
+// 32w..32w+31 (two tile rows) x all 64 columns (four tile columns).  Rows
+// j < nrow of each x mode's plane (nym rows apart) are transformed: a
+// periodic y leaves out its duplicate row nym-1.
 typedef double v4d __attribute__((ext_vector_type(4)));
 
 __global__ void __launch_bounds__(kGemmThreads) k_zeig_mfma(const double2* __restrict__ A,
                                                            const double* __restrict__ B, double2* __restrict__ C,
-                                                           int nym, int K, int N) {
+                                                           int nym, int nrow, int K, int N) {
     __shared__ double As[kGemmKC][2 * kGemmTJ + 2];  // real rows, padded
     __shared__ double Bs[kGemmKC][kGemmTN];
     const int t = threadIdx.x;
@@ -66,7 +68,7 @@ __global__ void __launch_bounds__(kGemmThreads) k_zeig_mfma(const double2* __res
             const int e = t + q * kGemmThreads;
             const int jj = e / kGemmKC, kk = e - jj * kGemmKC;
             const int j = j0 + jj, k = k0 + kk;
-            pa[q] = (j < nym && k < K) ? gld<double2>(a + (long)j * K + k) : make_double2(0.0, 0.0);
+            pa[q] = (j < nrow && k < K) ? gld<double2>(a + (long)j * K + k) : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int q = 0; q < QB; ++q) {
@@ -127,8 +129,8 @@ __global__ void __launch_bounds__(kGemmThreads) k_zeig_mfma(const double2* __res
         for (int c = 0; c < 4; ++c) {
             const int n = n0 + 16 * c + li;
             if (n >= N) continue;
-            if (j < nym) gst<double2>(cp + (long)j * N + n, make_double2(acc[r][c][0], acc[r][c][1]));
-            if (j + 4 < nym) gst<double2>(cp + (long)(j + 4) * N + n, make_double2(acc[r][c][2], acc[r][c][3]));
+            if (j < nrow) gst<double2>(cp + (long)j * N + n, make_double2(acc[r][c][0], acc[r][c][1]));
+            if (j + 4 < nrow) gst<double2>(cp + (long)(j + 4) * N + n, make_double2(acc[r][c][2], acc[r][c][3]));
         }
     }
 }
@@ -225,67 +227,122 @@ __global__ void __launch_bounds__(256) k_duct_ysolve(double2* __restrict__ S, co
     }
 }
 
-// The pinned kx = ky = 0 line, one workgroup: r(k) = sum over the nyp rows of
-// the kx = 0 plane (its ky = 0 y-mode); p = (Dzz with D(2) *= 3/2)^-1 r by
-// zgtsv's no-interchange elimination and back substitution (one lane);
-// z00(m) = sum_k p(k) Qf(k, m).  LDS: r and the inverse pivots, nzm each.
-constexpr int kZw00Threads = 256;
+// The pinned kx = ky = 0 line, one workgroup of 1024 lanes: r(k) = sum over
+// the nyp rows of the kx = 0 plane (its ky = 0 y-mode; four row groups per
+// column, eight independent partial sums each, combined in a fixed order);
+// p = (Dzz with D(2) *= 3/2)^-1 r by zgtsv's no-interchange elimination and
+// back substitution (one lane, with the plan's multipliers DL/D' and inverse
+// pivots of that fixed system staged in LDS first); z00(m) = sum_k p(k)
+// Qf(k, m) (eight partial sums per lane).  LDS: r, the multipliers, the
+// inverse pivots and the row-group partials.
+constexpr int kZw00Threads = 1024;
+constexpr int kZw00Groups = 4;
 __global__ void __launch_bounds__(kZw00Threads) k_zwall00(const double2* __restrict__ S, const double* __restrict__ qf,
-                                                          const double* __restrict__ dzl, const double* __restrict__ dzd,
+                                                          const double* __restrict__ zmul, const double* __restrict__ zip,
                                                           const double* __restrict__ dzu, double2* __restrict__ z00,
                                                           int nym, int nzm) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    double2* r = reinterpret_cast<double2*>(smem_raw);  // nzm
-    double* ip = reinterpret_cast<double*>(r + nzm);    // nzm
-    const int nyp = nym - 1;
+    double2* r = reinterpret_cast<double2*>(smem_raw);     // nzm
+    double* ip = reinterpret_cast<double*>(r + nzm);       // nzm: inverse pivots
+    double* mu = ip + nzm;                                 // nzm: multipliers (mu[0] unused)
+    double* uu = mu + nzm;                                 // nzm: DU(k), uu[nzm-1] = 0
+    double2* part = reinterpret_cast<double2*>(uu + nzm);  // kZw00Groups x (kZw00Threads / kZw00Groups)
     for (int k = threadIdx.x; k < nzm; k += kZw00Threads) {
-        double sx = 0.0, sy = 0.0;
-        for (int j = 0; j < nyp; ++j) {
-            const double2 v = S[(long)j * nzm + k];
-            sx += v.x;
-            sy += v.y;
-        }
-        r[k] = make_double2(sx, sy);
+        ip[k] = zip[k];
+        mu[k] = k > 0 ? zmul[k] : 0.0;
+        uu[k] = k < nzm - 1 ? dzu[k] : 0.0;
     }
-    __syncthreads();
+    constexpr int KB = kZw00Threads / kZw00Groups;       // columns per pass
+    const int nyp = nym - 1;
+    const int tk = threadIdx.x % KB, grp = threadIdx.x / KB;
+    for (int k0 = 0; k0 < nzm; k0 += KB) {
+        const int k = k0 + tk;
+        double2 acc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] = make_double2(0.0, 0.0);
+        if (k < nzm) {
+            int j = grp;
+            for (; j + 7 * kZw00Groups < nyp; j += 8 * kZw00Groups) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    const double2 v = S[(long)(j + u * kZw00Groups) * nzm + k];
+                    acc[u].x += v.x;
+                    acc[u].y += v.y;
+                }
+            }
+            for (; j < nyp; j += kZw00Groups) {
+                const double2 v = S[(long)j * nzm + k];
+                acc[0].x += v.x;
+                acc[0].y += v.y;
+            }
+        }
+        double2 sgrp = acc[0];
+#pragma unroll
+        for (int u = 1; u < 8; ++u) {
+            sgrp.x += acc[u].x;
+            sgrp.y += acc[u].y;
+        }
+        part[grp * KB + tk] = sgrp;
+        __syncthreads();
+        if (grp == 0 && k < nzm) {
+            double2 s = part[tk];
+            for (int q = 1; q < kZw00Groups; ++q) {
+                s.x += part[q * KB + tk].x;
+                s.y += part[q * KB + tk].y;
+            }
+            r[k] = s;
+        }
+        __syncthreads();
+    }
     if (threadIdx.x == 0) {
-        double dp = 3.0 / 2.0 * dzd[0];
-        ip[0] = 1.0 / dp;
         double yx = r[0].x, yy = r[0].y;
         for (int k = 1; k < nzm; ++k) {
-            const double mult = dzl[k - 1] / dp;
-            dp = dzd[k] - mult * dzu[k - 1];
-            ip[k] = 1.0 / dp;
-            yx = r[k].x - mult * yx;
-            yy = r[k].y - mult * yy;
+            yx = r[k].x - mu[k] * yx;
+            yy = r[k].y - mu[k] * yy;
             r[k] = make_double2(yx, yy);
         }
         double xx = 0.0, xy = 0.0;
         for (int k = nzm - 1; k >= 0; --k) {
-            const double du = k < nzm - 1 ? dzu[k] : 0.0;
-            xx = (r[k].x - du * xx) * ip[k];
-            xy = (r[k].y - du * xy) * ip[k];
+            xx = (r[k].x - uu[k] * xx) * ip[k];
+            xy = (r[k].y - uu[k] * xy) * ip[k];
             r[k] = make_double2(xx, xy);
         }
     }
     __syncthreads();
     for (int m = threadIdx.x; m < nzm; m += kZw00Threads) {
-        double sx = 0.0, sy = 0.0;
-        for (int k = 0; k < nzm; ++k) {
-            const double q = qf[(long)k * nzm + m];
-            sx = fma(r[k].x, q, sx);
-            sy = fma(r[k].y, q, sy);
+        double2 acc[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc[u] = make_double2(0.0, 0.0);
+        int k = 0;
+        for (; k + 7 < nzm; k += 8) {
+#pragma unroll
+            for (int u = 0; u < 8; ++u) {
+                const double q = qf[(long)(k + u) * nzm + m];
+                acc[u].x = fma(r[k + u].x, q, acc[u].x);
+                acc[u].y = fma(r[k + u].y, q, acc[u].y);
+            }
         }
-        z00[m] = make_double2(sx, sy);
+        for (; k < nzm; ++k) {
+            const double q = qf[(long)k * nzm + m];
+            acc[0].x = fma(r[k].x, q, acc[0].x);
+            acc[0].y = fma(r[k].y, q, acc[0].y);
+        }
+        double2 s = acc[0];
+#pragma unroll
+        for (int u = 1; u < 8; ++u) {
+            s.x += acc[u].x;
+            s.y += acc[u].y;
+        }
+        z00[m] = s;
     }
 }
 
-// One workgroup per (kx, group of G eigenmode columns): the nym x G tile in
-// LDS, the y FFT down the columns over the nyp independent rows, each mode
-// divided by kxx + kyy + lambda_z(m) (the kx = ky = 0 row taken from z00),
-// the inverse FFT scaled by 1/nyp, row nym-1 set to row 0 (the duplicate,
-// projection.f90:350-354), stored back.  Columns past nzm are zero-filled
-// and not stored.
+// One workgroup per (kx, group of G eigenmode columns): the nyp x G tile of
+// the independent rows in LDS, the y FFT down the columns, each mode divided
+// by kxx + kyy + lambda_z(m) (the kx = ky = 0 row taken from z00), the
+// inverse FFT scaled by 1/nyp, stored back.  The duplicate row nym-1
+// (projection.f90:350-354) is not carried: the inverse x kernel reads row 0
+// for it.  Columns past nzm are zero-filled and not stored.
 constexpr int kYsThreads = 256;
 __global__ void __launch_bounds__(kYsThreads) k_yfft_solve(double2* __restrict__ S, const double2* __restrict__ twy_g,
                                                            FftDesc fy, int ntwy, const double* __restrict__ kxx,
@@ -301,7 +358,7 @@ __global__ void __launch_bounds__(kYsThreads) k_yfft_solve(double2* __restrict__
     const int m0 = (blockIdx.x - kx * ngrp) * G;
     const int nyp = nym - 1;
     const auto plane = gptr(S) + (long)kx * nym * nzm;
-    for (int e = threadIdx.x; e < nym * G; e += kYsThreads) {
+    for (int e = threadIdx.x; e < nyp * G; e += kYsThreads) {
         const int j = (int)div_g.div((uint32_t)e), c = e - j * G;
         const int m = m0 + c;
         A[e] = m < nzm ? gld<double2>(plane + (long)j * nzm + m) : make_double2(0.0, 0.0);
@@ -327,9 +384,7 @@ __global__ void __launch_bounds__(kYsThreads) k_yfft_solve(double2* __restrict__
     }
     __syncthreads();
     lds_fft_cols<1, 4>(A, 0, G, G, fy, twy, div_g);
-    for (int c = threadIdx.x; c < G; c += kYsThreads) A[(long)nyp * G + c] = A[c];
-    __syncthreads();
-    for (int e = threadIdx.x; e < nym * G; e += kYsThreads) {
+    for (int e = threadIdx.x; e < nyp * G; e += kYsThreads) {
         const int j = (int)div_g.div((uint32_t)e), c = e - j * G;
         const int m = m0 + c;
         if (m < nzm) gst<double2>(plane + (long)j * nzm + m, A[e]);
@@ -358,9 +413,10 @@ hipError_t launch_duct_lu(dfp_plan* p) {
 
 static hipError_t launch_zeig(dfp_plan* p, const double2* A, const double* B, double2* C, const char* name) {
     const Geo& g = p->geo;
-    const dim3 grid(ceil_div(g.nzp, kGemmTN), ceil_div(g.nym, kGemmTJ), g.nxh);
+    const int nrow = g.ybc == 0 ? g.nym - 1 : g.nym;  // the duplicate row of a periodic y is not transformed
+    const dim3 grid(ceil_div(g.nzp, kGemmTN), ceil_div(nrow, kGemmTJ), g.nxh);
     const int slot = prof_begin(p, name);
-    hipLaunchKernelGGL(k_zeig_mfma, grid, dim3(kGemmThreads), 0, p->stream, A, B, C, g.nym, g.nzp, g.nzp);
+    hipLaunchKernelGGL(k_zeig_mfma, grid, dim3(kGemmThreads), 0, p->stream, A, B, C, g.nym, nrow, g.nzp, g.nzp);
     prof_end(p, slot);
     return hipGetLastError();
 }
@@ -371,13 +427,26 @@ hipError_t launch_duct_solve(dfp_plan* p) {
     if (g.ybc == 0) {
         // spanwise walls with a periodic y: the pinned kx = ky = 0 line, the
         // forward z eigenbasis GEMM, the y transforms and divisions, the inverse GEMM
-        const size_t l00 = (sizeof(double2) + sizeof(double)) * (size_t)g.nzp;
+        // the pinned line (one workgroup) runs on the plan's second stream
+        // beside the forward GEMM: both only read the spectrum, and the y
+        // solve joins them.  With per-kernel profiling on, both stay in order
+        // on the plan's stream.
+        const size_t l00 = (sizeof(double2) + 3 * sizeof(double)) * (size_t)g.nzp + sizeof(double2) * kZw00Threads;
+        const bool fork = !p->prof && p->aux;
+        hipStream_t s00 = p->stream;
+        if (fork) {
+            if ((e = hipEventRecord(p->fork_ev, p->stream)) != hipSuccess) return e;
+            if ((e = hipStreamWaitEvent(p->aux, p->fork_ev, 0)) != hipSuccess) return e;
+            s00 = p->aux;
+        }
         int slot = prof_begin(p, "zwall00");
-        hipLaunchKernelGGL(k_zwall00, dim3(1), dim3(kZw00Threads), l00, p->stream, p->d_spec, p->duct_qf, p->dzz_l,
-                           p->dzz_d, p->dzz_u, p->d_z00, g.nym, g.nzp);
+        hipLaunchKernelGGL(k_zwall00, dim3(1), dim3(kZw00Threads), l00, s00, p->d_spec, p->duct_qf, p->zw_mult,
+                           p->zw_ip, p->dzz_u, p->d_z00, g.nym, g.nzp);
         prof_end(p, slot);
         if ((e = hipGetLastError()) != hipSuccess) return e;
+        if (fork && (e = hipEventRecord(p->join_ev, p->aux)) != hipSuccess) return e;
         if ((e = launch_zeig(p, p->d_spec, p->duct_qf, p->d_spec2, "zeig_fwd")) != hipSuccess) return e;
+        if (fork && (e = hipStreamWaitEvent(p->stream, p->join_ev, 0)) != hipSuccess) return e;
         const int G = p->gy, ngrp = ceil_div(g.nzp, G), ntwy = twiddle_count(p->fy);
         slot = prof_begin(p, "yfft_solve");
         hipLaunchKernelGGL(k_yfft_solve, dim3(g.nxh * ngrp), dim3(kYsThreads), yfft_solve_lds_bytes(g.nym, G, ntwy),

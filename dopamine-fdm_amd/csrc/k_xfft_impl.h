@@ -293,7 +293,9 @@ __device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsig
     for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = gld<C>(gptr(q.tw) + e);
     __syncthreads();
 
-    // gather the R sub-line values of each (kx, column), undo the first z stage
+    // gather the R sub-line values of each (kx, column), undo the first z stage;
+    // a periodic y's duplicate row nym-1 reads row 0 (projection.f90:350-354)
+    const int js = (g.ybc == 0 && j == g.nym - 1) ? 0 : j;
     const int npairs = (M + 1) * nt;
     constexpr int HP = R <= 4 ? 2 : 1;  // pairs per thread in flight
     for (int base = 0; base < npairs; base += HP * kXThreads) {
@@ -306,7 +308,7 @@ __device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsig
             pk[h] = p0 < npairs ? pidx : -1;
             const int kx = (int)q.div_t.div((uint32_t)pidx);
             const int c = pidx - kx * nt;
-            const auto src = spec + ((long)kx * R * g.nym + j) * nz2 + k0 + c;
+            const auto src = spec + ((long)kx * R * g.nym + js) * nz2 + k0 + c;
 #pragma unroll
             for (int u = 0; u < R; ++u) av[h][u] = gld<C>(src + (long)u * g.nym * nz2);
         }
