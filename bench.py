@@ -409,7 +409,9 @@ def form_gbs(bytes_per_projection: int, projections_per_s: float) -> dict:
 def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
     """48 uneven cases (nxp, nzp in {32..128}, nym in {32,64,96}): one batched
     call (persistent launch per stage) against the same 48 projections issued
-    one plan at a time, in FP64 and in FP32+refine; tuned launch shapes."""
+    one plan at a time, in FP64 and in FP32+refine.  The batch runs the shape
+    family the autotuner picks inside the batched call; the one-at-a-time
+    baseline runs each size's own tuned shape."""
     import dfp
     from dfp import autotune, hipmem
     from dfp.batch import Batch, Case
@@ -421,8 +423,6 @@ def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
     t0 = time.perf_counter()
     autotune.tune_sizes(sizes, (dfp.DFP_FP64, dfp.DFP_FP32_REFINE), table)
     tune_s = time.perf_counter() - t0
-    if save_tuning:
-        autotune.save_table(table, save_tuning)
     out = {"cases": 48, "distinct_sizes": len(sizes), "tuning_s": round(tune_s, 1)}
     e0, e1 = hipmem.Event(), hipmem.Event()
 
@@ -437,40 +437,30 @@ def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
         return e0.elapsed_ms(e1) / reps
 
     for prec, name in ((dfp.DFP_FP64, "fp64"), (dfp.DFP_FP32_REFINE, "fp32_refine")):
-        # two shape sets for the batch: each size's own tuned shape (fastest
-        # one plan at a time) and the library defaults (one z-split radix
-        # family and the split z pair, which group into fewer, fuller
-        # launches); the batch keeps whichever runs the whole call faster
-        by_shape = {}
-        for shapes in ("tuned", "default"):
-            plans, cases = [], []
-            for sp in specs:
-                g = dfp.channel_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
-                plan = autotune.tuned_plan(g, prec, table if shapes == "tuned" else None)
-                f = [dfp.DeviceArray(sh) for sh in (g.shape_u, g.shape_v, g.shape_w)]
-                plan.synth(*f, seed=sp.seed, sigma=sp.sigma)
-                rhs = dfp.DeviceArray(g.shape_rhs, zero=True)
-                P = dfp.DeviceArray(g.shape_p, zero=True)
-                plans.append(plan)
-                cases.append(Case(plan, f[0], f[1], f[2], rhs, P, DT, RK_C[2], True))
-            b = Batch(max_cases=48, graphs=True)
-            occ = autotune.tune_batch(b, cases)
-            by_shape[shapes] = (time_batch(b, cases), occ, plans, cases, b)
-        shapes = min(by_shape, key=lambda k: by_shape[k][0])
-        batch_ms, occ = by_shape[shapes][0], by_shape[shapes][1]
+        # the batch's shapes: the family whose whole batched call runs fastest
+        # (each size's own tuned shape, the library defaults, one z-split
+        # radix for every size, or the fused z kernel), each family at its
+        # best workgroups per CU
+        pick = autotune.tune_batch_shapes(specs, prec, table, dt=DT, c=RK_C[2], reps=5)
+        b, cases = pick["batch"], pick["cases"]
+        batch_ms = time_batch(b, cases)
         # the baseline: the same projections one plan at a time with each
         # size's tuned shape
-        cases = by_shape["tuned"][3]
-        for cs in cases:
+        if pick["family"] == "per_plan":
+            seq_plans, seq_cases = [], cases
+        else:
+            seq_plans, seq_cases = autotune.build_cases(specs, "per_plan", prec, table, DT, RK_C[2])
+        for cs in seq_cases:
             cs.plan.project(cs.U, cs.V, cs.W, cs.rhs, cs.P, DT, RK_C[2], save_p=True, asynchronous=True)
         hipmem.synchronize()
         e0.record()
         for _ in range(reps):
-            for cs in cases:
+            for cs in seq_cases:
                 cs.plan.project(cs.U, cs.V, cs.W, cs.rhs, cs.P, DT, RK_C[2], save_p=True, asynchronous=True)
         e1.record()
         e1.synchronize()
         seq_ms = e0.elapsed_ms(e1) / reps
+        autotune.close_all(None, seq_plans)
         case_bytes = sum(b_model(cs.plan.g.n_cells, cs.plan.g.n_modes) if prec == dfp.DFP_FP64
                          else b_model_mixed(cs.plan.g.n_cells, cs.plan.g.n_modes, cs.plan.sweeps) for cs in cases)
         gbs = case_bytes / (batch_ms * 1e-3) / 1e9
@@ -478,12 +468,12 @@ def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
                      "B_model_bytes": int(case_bytes), "achieved_GBs": round(gbs, 1),
                      "frac_of_hbm_peak": round(gbs / HBM_PEAK_GBS, 4),
                      "sequential_ms": round(seq_ms, 4), "speedup_vs_sequential": round(seq_ms / batch_ms, 3),
-                     "occupancy": occ["occupancy"], "shapes": shapes,
-                     "batch_ms_by_shapes": {k: round(v[0], 4) for k, v in by_shape.items()}}
+                     "occupancy": pick["occupancy"], "shapes": pick["family"],
+                     "batch_ms_by_shapes": {k: round(v / 1e3, 4) for k, v in pick["by_family"].items()}}
         # repeated solves (rhs_p -> phi, as the reference's skip_p_save solves of
         # a preconditioner): one solve-only batched call against plan.solve
         # one case at a time, on the batch's shapes
-        bsolve, scases = by_shape[shapes][4], by_shape[shapes][3]
+        bsolve, scases = b, cases
         bsolve.project(scases, solve_only=True)
         hipmem.synchronize()
         e0.record()
@@ -502,10 +492,9 @@ def bench_batched(reps: int = 20, save_tuning: str | None = None) -> dict:
         out[name]["solve_only"] = {"batch_ms": round(solve_ms, 4), "solves_per_s": round(48 / (solve_ms / 1e3), 1),
                                    "sequential_ms": round(solve_seq_ms, 4),
                                    "speedup_vs_sequential": round(solve_seq_ms / solve_ms, 3)}
-        for v in by_shape.values():
-            v[4].close()
-            for pl in v[2]:
-                pl.close()
+        autotune.close_all(b, pick["plans"])
+    if save_tuning:
+        autotune.save_table(table, save_tuning)
     return out
 
 
@@ -676,9 +665,14 @@ def main() -> int:
         plan.close()
         del sets, rhs, P
         n_mix = max(3, min(n_sets, 24))
-        mixed = {"sweeps": args.sweeps}
+        # the byte-model bound on the speedup: the plain path's bytes over the
+        # FP32+refine form's, i.e. the speedup both would show at one HBM rate
+        ceiling = b_model(N, H) / b_model_mixed(N, H, args.sweeps)
+        mixed = {"sweeps": args.sweeps, "target_speedup_vs_fp64": 1.7,
+                 "byte_model_ceiling_vs_fp64": round(ceiling, 3)}
         v32, k32 = bench_mixed(g, K, Wm, args.sweeps, n_mix)
         mixed["256x192x256"] = {"projections_per_s": round(v32, 2), "speedup_vs_fp64": round(v32 / value, 3),
+                                "frac_of_byte_model_ceiling": round(v32 / value / ceiling, 3),
                                 **form_gbs(b_model_mixed(N, H, args.sweeps), v32),
                                 "kernels_us": k32, "rel_err_vs_fp64_by_sweeps": mixed_errors(g, (0, 1, 2, 3), 4242)}
         gl = dfp.channel_grid(2 * NXP, 2 * NYM, 2 * NZP, ALPHA)
@@ -693,6 +687,8 @@ def main() -> int:
         v32l, k32l = bench_mixed(gl, kl, 1, args.sweeps, 3)
         mixed["512x384x512"] = {"fp64_projections_per_s": round(v64l, 2), "projections_per_s": round(v32l, 2),
                                 "speedup_vs_fp64": round(v32l / v64l, 3),
+                                "byte_model_ceiling_vs_fp64": round(b_model(gl.n_cells, gl.n_modes) / b_model_mixed(
+                                    gl.n_cells, gl.n_modes, args.sweeps), 3),
                                 "fp64": form_gbs(b_model(gl.n_cells, gl.n_modes), v64l),
                                 **form_gbs(b_model_mixed(gl.n_cells, gl.n_modes, args.sweeps), v32l),
                                 "kernels_us": k32l,

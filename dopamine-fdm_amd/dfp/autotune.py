@@ -4,11 +4,15 @@ the batched launches, cached in a JSON table.
 For one grid it times, with HIP events on resident fields, every valid
 (z_split, x_columns, z_rows, z_fused) of ``dfp_plan_create_tuned`` in a
 bounded candidate set and keeps the fastest; for a batch it times the persistent
-launches at a few workgroups-per-CU settings.  The table maps
+launches at a few workgroups-per-CU settings under each shape family.  The table maps
 "nxp x nym x nzp:fp64|fp32" to the chosen shape and its time:
 
     {"device": "...", "plans": {"64x64x64:fp64": {"tuning": [4, 16, 8, 1], "us": 41.2}, ...},
-     "batch": {"occupancy": [4, 4, 8], "us": ...}}
+     "batch": {"fp64": {"family": "split_r2", "occupancy": [4, 4, 8], "us": ...}, ...}}
+
+A batch's shapes are chosen inside the batched call: the shape family
+(every size's own tuned shape, the library's, one z-split radix for all, or
+the fused z kernel for all) whose whole call runs fastest.
 
 The reference has no counterpart (its FFT and LAPACK libraries pick their
 own shapes).
@@ -192,3 +196,123 @@ def tune_batch(batch, cases, reps: int = 5, log=None) -> dict:
             best = (us, occ)
     batch.set_occupancy(*best[1])
     return {"occupancy": list(best[1]), "us": round(best[0], 2)}
+
+
+# Shape families a batch can run with.  The batch groups its persistent
+# launches by kernel variant (z split radix, fused or split z kernels), so a
+# family that gives many sizes the same variant makes fewer, fuller launches
+# even where each size alone would prefer another shape.
+BATCH_FAMILIES = ("default", "per_plan", "split_r2", "split_r4", "fused")
+
+
+def family_tuning(family: str, g: ChannelGrid, precision: int, table: dict | None):
+    """The dfp_plan_create_tuned shape of one case under a family (None: the
+    library's choice)."""
+    if family == "default":
+        return None
+    if family == "per_plan":
+        return lookup(table, g, precision) if table else None
+    if family.startswith("split_r"):
+        R = int(family[len("split_r"):])
+        return (R, 0, 0, 1) if g.nzp % R == 0 else None
+    if family == "fused":
+        return (0, 0, 0, 2)
+    raise ValueError(f"unknown shape family {family!r}")
+
+
+def family_plan(family: str, g: ChannelGrid, precision: int, table: dict | None, device: int = 0,
+                sweeps: int | None = None) -> Plan:
+    """A plan with the family's shape, or the library's where the device
+    cannot hold it."""
+    t = family_tuning(family, g, precision, table)
+    if t is not None:
+        try:
+            return Plan(g, device=device, precision=precision, sweeps=sweeps, tuning=t)
+        except DfpError as e:
+            if e.code not in (-1, -2):  # a shape out of range or beyond the device's LDS falls back
+                raise
+    return Plan(g, device=device, precision=precision, sweeps=sweeps)
+
+
+def build_cases(specs, family: str, precision: int, table: dict | None = None,
+                dt: float = 1e-3, c: float = 0.75):
+    """Plans of one shape family and resident seeded fields for the cases
+    `specs` (workloads.CaseSpec); returns (plans, [batch.Case])."""
+    from .batch import Case
+
+    plans, cases = [], []
+    for sp in specs:
+        g = channel_grid(sp.nxp, sp.nym, sp.nzp, sp.alpha)
+        plan = family_plan(family, g, precision, table)
+        f = [DeviceArray(sh) for sh in (g.shape_u, g.shape_v, g.shape_w)]
+        plan.synth(*f, seed=sp.seed, sigma=sp.sigma)
+        rhs = DeviceArray(g.shape_rhs, zero=True)
+        P = DeviceArray(g.shape_p, zero=True)
+        plans.append(plan)
+        cases.append(Case(plan, f[0], f[1], f[2], rhs, P, dt, c, True))
+    return plans, cases
+
+
+def time_batch(batch, cases, reps: int = 5, solve_only: bool = False) -> float:
+    """Microseconds per batched call (one untimed call first), HIP events."""
+    batch.project(cases, solve_only=solve_only)
+    synchronize()
+    e0, e1 = Event(), Event()
+    e0.record()
+    for _ in range(reps):
+        batch.project(cases, solve_only=solve_only, asynchronous=True)
+    e1.record()
+    e1.synchronize()
+    return 1e3 * e0.elapsed_ms(e1) / reps
+
+
+def tune_batch_shapes(specs, precision: int, table: dict | None = None, families=BATCH_FAMILIES,
+                      dt: float = 1e-3, c: float = 0.75, reps: int = 5, keep: bool = True, log=None) -> dict:
+    """Time whole batched calls of the cases `specs` under every shape family,
+    each at its best workgroups-per-CU setting (tune_batch), and pick the
+    fastest family.  Returns {"family", "us", "occupancy", "by_family"} and,
+    with keep, the winning family's "batch", "plans" and "cases" (the caller
+    closes them).  The choice is recorded in table["batch"] under the
+    precision's name."""
+    from .batch import Batch
+
+    best = None
+    by_family = {}
+    for fam in families:
+        plans, cases = build_cases(specs, fam, precision, table, dt, c)
+        b = Batch(max_cases=max(1, len(cases)), graphs=True)
+        occ = tune_batch(b, cases, reps=reps)
+        us = time_batch(b, cases, reps)
+        by_family[fam] = round(us, 2)
+        if log:
+            log(f"batch shapes {fam}: {us:.1f} us (occupancy {occ['occupancy']})")
+        if best is None or us < best[0]:
+            if best is not None:
+                close_all(best[3], best[4])
+            best = (us, fam, occ, b, plans, cases)
+        else:
+            close_all(b, plans)
+    us, fam, occ, b, plans, cases = best
+    if table is not None:
+        table.setdefault("batch", {})[_PREC_NAME[precision]] = {"family": fam, "us": round(us, 2),
+                                                                "occupancy": occ["occupancy"]}
+    out = {"family": fam, "us": round(us, 2), "occupancy": occ["occupancy"], "by_family": by_family}
+    if keep:
+        out.update(batch=b, plans=plans, cases=cases)
+    else:
+        close_all(b, plans)
+    return out
+
+
+def batch_family(table: dict | None, precision: int = DFP_FP64) -> str:
+    """The shape family a table records for batched calls ("per_plan" when
+    none is recorded)."""
+    e = (table or {}).get("batch", {}).get(_PREC_NAME[precision])
+    return e["family"] if e and "family" in e else "per_plan"
+
+
+def close_all(batch, plans):
+    if batch is not None:
+        batch.close()
+    for p in plans:
+        p.close()

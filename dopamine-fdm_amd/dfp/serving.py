@@ -31,7 +31,7 @@ from dataclasses import dataclass, field
 import numpy as np
 
 from ._lib import DFP_FP32_REFINE, DFP_FP64
-from .autotune import tuned_plan
+from .autotune import batch_family, family_plan
 from .batch import Batch, Case
 from .grid import channel_grid
 from .hipmem import DeviceArray, Event, Stream, mem_info
@@ -63,11 +63,14 @@ class Request:
 class _Resident:
     """Device state of one case id."""
 
-    def __init__(self, case_id, g, table, sweeps):
+    def __init__(self, case_id, g, table, sweeps, batched=True):
         self.case_id = case_id
         self.g = g
         self.table = table
         self.sweeps = sweeps
+        # a case served inside batches takes the shape family the table
+        # records for batched calls; one served alone, its own size's shape
+        self.batched = batched
         self.plans = {}
         self.U = DeviceArray(g.shape_u)
         self.V = DeviceArray(g.shape_v)
@@ -80,8 +83,9 @@ class _Resident:
     def plan(self, precision):
         p = self.plans.get(precision)
         if p is None:
-            p = tuned_plan(self.g, precision, self.table,
-                           sweeps=self.sweeps if precision == DFP_FP32_REFINE else None)
+            fam = batch_family(self.table, precision) if self.batched else "per_plan"
+            p = family_plan(fam, self.g, precision, self.table,
+                            sweeps=self.sweeps if precision == DFP_FP32_REFINE else None)
             self.plans[precision] = p
         return p
 
@@ -194,7 +198,7 @@ class Server:
         need = self._case_bytes(g, (req.precision,))
         if not self._evict_for(need):
             return None
-        r = _Resident(req.case_id, g, self.table, self.sweeps)
+        r = _Resident(req.case_id, g, self.table, self.sweeps, batched=not self._solo(req))
         try:
             r.plan(req.precision)
         except BaseException:

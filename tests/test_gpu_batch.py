@@ -355,3 +355,74 @@ def test_batch_rejects_a_non_finite_save_p_scale_as_dfp_project_does():
     finally:
         b.close()
         plan.close()
+
+
+def _upload_and_run(b, cases, specs, save_every=1):
+    hosts = []
+    for i, (cs, sp) in enumerate(zip(cases, specs)):
+        host = fields(cs.plan.g, sp.seed, sp.sigma)
+        host = (host[0], host[1], host[2], host[4], host[3])
+        for a, h in zip((cs.U, cs.V, cs.W, cs.rhs, cs.P), host):
+            a.upload(h)
+        hosts.append(host)
+    b.project(cases)
+    return hosts, [[a.download() for a in (cs.U, cs.V, cs.W, cs.rhs, cs.P)] for cs in cases]
+
+
+def test_batch_48_cases_fp32_refine_tuned_shapes_vs_oracle():
+    """The full 48-case FP32+refine batch on the shape family the autotuner
+    picks inside the batched call, a sample of cases against the oracle."""
+    from dfp import autotune
+
+    specs = batched_cases(48)
+    table = autotune.load_table()
+    print("batch of 48 uneven cases, FP32+refine, tuning shape families", flush=True)
+    pick = autotune.tune_batch_shapes(specs, dfp.DFP_FP32_REFINE, table, dt=DT, c=C3, reps=2)
+    print(f"picked {pick['family']} at occupancy {pick['occupancy']}: {pick['by_family']}", flush=True)
+    assert pick["family"] in autotune.BATCH_FAMILIES
+    assert set(pick["by_family"]) == set(autotune.BATCH_FAMILIES)
+    assert table["batch"]["fp32"]["family"] == pick["family"]
+    assert autotune.batch_family(table, dfp.DFP_FP32_REFINE) == pick["family"]
+    b, cases = pick["batch"], pick["cases"]
+    try:
+        hosts, outs = _upload_and_run(b, cases, specs)
+        sample = sorted({0, 5, 13, 26, 40, 47, max(range(48), key=lambda i: cases[i].plan.g.n_cells)})
+        for i in sample:
+            g = cases[i].plan.g
+            host = [a.copy(order="F") for a in hosts[i]]
+            oracle_project(g, host[0], host[1], host[2], host[4], host[3], DT, C3, save_p=True)
+            for lab, a, r in zip(("U", "V", "W", "phi", "P"), outs[i], host):
+                assert rel_linf(a, r) <= 1e-11, f"case {i} {(g.nxp, g.nym, g.nzp)} {lab}"
+    finally:
+        autotune.close_all(b, pick["plans"])
+
+
+@pytest.mark.parametrize("family", ["default", "per_plan", "split_r2", "split_r4", "fused"])
+@pytest.mark.parametrize("prec", [dfp.DFP_FP64, dfp.DFP_FP32_REFINE], ids=["fp64", "fp32"])
+def test_every_batch_shape_family_matches_oracle(family, prec):
+    """Each shape family the batch tuner may choose, batched over uneven
+    cases, against the oracle case by case."""
+    from dfp import autotune
+
+    specs = batched_cases(10, seed=13)
+    table = autotune.load_table()
+    print(f"batch family {family} prec {prec}: {[(s.nxp, s.nym, s.nzp) for s in specs]}", flush=True)
+    plans, cases = autotune.build_cases(specs, family, prec, table, DT, C3)
+    b = Batch(max_cases=len(cases), graphs=True)
+    try:
+        t = [p.tuning for p in plans]
+        if family == "fused":
+            assert all(x[3] == 2 for x in t)
+        if family.startswith("split_r"):
+            R = int(family[7:])
+            assert all(x[0] == R for x, sp in zip(t, specs) if sp.nzp % R == 0)
+        hosts, outs = _upload_and_run(b, cases, specs)
+        tol = 1e-12 if prec == dfp.DFP_FP64 else 1e-11
+        for i, cs in enumerate(cases):
+            g = cs.plan.g
+            host = [a.copy(order="F") for a in hosts[i]]
+            oracle_project(g, host[0], host[1], host[2], host[4], host[3], DT, C3, save_p=True)
+            for lab, a, r in zip(("U", "V", "W", "phi", "P"), outs[i], host):
+                assert rel_linf(a, r) <= tol, f"case {i} {(g.nxp, g.nym, g.nzp)} {t[i]} {lab}"
+    finally:
+        autotune.close_all(b, plans)

@@ -147,3 +147,53 @@ def test_a_rejected_size_leaves_the_other_requests_served():
         for a, b in zip(s.result(q.case_id), _alone(q)):
             assert np.array_equal(a, b), f"case {q.case_id}"
     s.close()
+
+
+def test_table_driven_shapes_serve_oracle_results():
+    """A server given a tuning table: batched cases take the table's shape
+    family for batched calls, cases served alone their size's own tuned
+    shape, and the results match the oracle."""
+    from dfp import autotune
+
+    # even case ids capped at 48 points a side (batched), odd ones at 96 (alone)
+    pool = {cid: (tuple(min(d, 48 if cid % 2 == 0 else 96) for d in shape), a)
+            for cid, (shape, a) in serving.make_case_pool(8, seed=4).items()}
+    trace = serving.poisson_requests(24, 1000.0, pool, fp32_fraction=0.5, seed=4)
+    solo = 200_000
+    table = {"plans": {}, "batch": {"fp64": {"family": "fused"}, "fp32": {"family": "split_r2"}}}
+    for shape, a in pool.values():
+        g = dfp.channel_grid(*shape, a)
+        for prec in (dfp.DFP_FP64, dfp.DFP_FP32_REFINE):
+            cs = [c for c in autotune.candidates(g) if c[2] == 0 and c[3] == 1]
+            table["plans"][autotune.size_key(*shape, prec)] = {"tuning": list(cs[len(cs) // 2]), "us": 0.0}
+    s = Server(max_batch=8, n_streams=2, table=table, solo_cells=solo)
+    reqs, _ = serving.run_trace(s, trace, realtime=False)
+    assert all(q.done is not None for q in reqs)
+    n_solo = n_batched = 0
+    for cid, r in s.resident.items():
+        assert r.batched == (r.g.nxp * r.g.nym * r.g.nzp < solo)
+        for prec, p in r.plans.items():
+            got = p.tuning
+            if r.batched:
+                n_batched += 1
+                fam = autotune.batch_family(table, prec)
+                if fam == "fused":
+                    assert got[3] == 2, (cid, got)
+                elif r.g.nzp % 2 == 0:
+                    assert got[0] == 2 and got[3] == 1, (cid, got)
+            else:
+                n_solo += 1
+                want = autotune.lookup(table, r.g, prec)
+                assert got[0] == want[0] and got[1] == want[1] and got[3] == want[3], (cid, got, want)
+    print(f"{n_batched} batched and {n_solo} solo plans", flush=True)
+    assert n_batched > 0 and n_solo > 0
+    for cid, q in _last_per_case(trace).items():
+        g = dfp.channel_grid(*q.shape, q.alpha)
+        U, V, W = dfp.synth_fields(g, q.seed, q.sigma)
+        P = np.zeros(g.shape_p, order="F")
+        rhs = np.zeros(g.shape_rhs, order="F")
+        oracle_project(g, U, V, W, P, rhs, serving.DT, serving.C3, save_p=True)
+        tol = 1e-12 if q.precision == dfp.DFP_FP64 else 1e-11
+        for lab, a, b in zip(("U", "V", "W", "phi", "P"), s.result(cid), (U, V, W, rhs, P)):
+            assert rel_linf(a, b) <= tol, f"case {cid} {lab}"
+    s.close()

@@ -68,7 +68,7 @@ def test_autotune_table_roundtrip(tmp_path):
 
 
 class _FakeResident:
-    def __init__(self, case_id, g, table, sweeps):
+    def __init__(self, case_id, g, table, sweeps, batched=True):
         self.case_id, self.g, self.plans, self.busy, self.freed = case_id, g, {}, False, False
 
     def plan(self, precision):
@@ -176,8 +176,8 @@ def test_serving_failed_launch_waits_for_queued_work_before_freeing_cases(monkey
             raise RuntimeError("launch failed")
 
     class _Res(_FakeResident):
-        def __init__(self, *a):
-            super().__init__(*a)
+        def __init__(self, *a, **kw):
+            super().__init__(*a, **kw)
             self.U = self.V = self.W = self.rhs = self.P = None
 
         def plan(self, precision):
@@ -201,3 +201,22 @@ def test_serving_failed_launch_waits_for_queued_work_before_freeing_cases(monkey
     assert log == ["synth", ("sync", [True])]
     assert not s.resident[7].busy
     assert [p.rid for p in s.pending] == [0]
+
+
+def test_batch_shape_families():
+    g = dfp.channel_grid(64, 32, 48, 2.0)
+    t = {"plans": {autotune.size_key(64, 32, 48, dfp.DFP_FP64): {"tuning": [4, 2, 8, 1], "us": 10.0}}, "batch": {}}
+    assert autotune.family_tuning("default", g, dfp.DFP_FP64, t) is None
+    assert autotune.family_tuning("per_plan", g, dfp.DFP_FP64, t) == (4, 2, 8, 1)
+    assert autotune.family_tuning("per_plan", g, dfp.DFP_FP32_REFINE, t) is None
+    assert autotune.family_tuning("split_r2", g, dfp.DFP_FP64, t) == (2, 0, 0, 1)
+    assert autotune.family_tuning("split_r4", g, dfp.DFP_FP64, t) == (4, 0, 0, 1)
+    # a radix that does not divide nzp leaves the size to the library
+    assert autotune.family_tuning("split_r4", dfp.channel_grid(64, 32, 18, 2.0), dfp.DFP_FP64, t) is None
+    assert autotune.family_tuning("fused", g, dfp.DFP_FP64, t) == (0, 0, 0, 2)
+    with pytest.raises(ValueError):
+        autotune.family_tuning("nope", g, dfp.DFP_FP64, t)
+    assert autotune.batch_family(t, dfp.DFP_FP64) == "per_plan"
+    t["batch"]["fp32"] = {"family": "split_r2", "occupancy": [4, 4, 8], "us": 1.0}
+    assert autotune.batch_family(t, dfp.DFP_FP32_REFINE) == "split_r2"
+    assert autotune.batch_family(None) == "per_plan"
