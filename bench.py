@@ -37,8 +37,8 @@ def algorithmic_bytes(N: int, H: int) -> dict:
     The whole projection is measured against B_model = 80 N + 128 H."""
     return {
         "div_rfft_x": 24 * N + 16 * H,   # read u*,v*,w*; write the x half-spectrum
-        "zsolve_fwd": 32 * H,            # read the spectrum; write the eliminated rows
-        "zsolve_bwd": 32 * H,            # read them; write the solved spectrum
+        "zsolve": 32 * H,                # read the spectrum, write the solved spectrum (the
+                                         # eliminated rows between the halves are not counted)
         "irfft_x": 16 * H + 8 * N,       # read the spectrum; write phi
         "correct": 56 * N,               # read phi, u*,v*,w*; write u,v,w
     }
@@ -222,7 +222,7 @@ def bench_mixed(g, K, Wm, sweeps, n_sets):
 def inflow_bytes(N: int) -> dict:
     """Per-launch bytes of the inflow/outflow x kernels (DCT-IV in x): the
     spectrum keeps all nxp DCT modes as complex values, 16 N bytes per pass."""
-    return {"div_dct_x": 24 * N + 16 * N, "zsolve_fwd": 32 * N, "zsolve_bwd": 32 * N,
+    return {"div_dct_x": 24 * N + 16 * N, "zsolve": 64 * N,
             "idct_x": 16 * N + 8 * N, "correct": 56 * N}
 
 

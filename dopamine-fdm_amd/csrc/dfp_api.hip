@@ -374,9 +374,8 @@ hipError_t z_solve(dfp_plan* p, int f32) {
         if (f32) return launch_zfused<float2>(p, make_zpar<float2>(p, 1));
         return launch_zfused<double2>(p, make_zpar<double2>(p, 0));
     }
-    hipError_t e = f32 ? launch_zsolve32(p, -1) : launch_zsolve(p, -1);
-    if (e != hipSuccess) return e;
-    return f32 ? launch_zsolve32(p, +1) : launch_zsolve(p, +1);
+    // elimination and back substitution of each sub-plane in one workgroup
+    return f32 ? launch_zsolve32(p) : launch_zsolve(p);
 }
 
 // The (0,0)-mode solve (row sums -> pbar) and the z solves (the spectrum,

@@ -156,7 +156,7 @@ int bfail(dfp_batch* b, int code, const char* fmt, ...) {
 
 // One launch of the batch: which kernel, on which records, with which case list.
 struct Launch {
-    enum Kind { XF, XI, ZF, ZB, ZFUSED, M00, SAVE, CORR } kind;
+    enum Kind { XF, XI, ZFB, ZFUSED, M00, SAVE, CORR } kind;
     int f32;      // spectrum precision
     int mode;     // x forward mode, or ACCUM for the inverse
     int R;        // z-split radix (x kernels)
@@ -472,16 +472,10 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
             sz[k].second.push_back(a);
             lsz[k] = std::max(lsz[k], z_lds_bytes(p, f32));
         }
-        // phase 1: the fused kernel and the forward halves of the split pairs;
-        // phase 2: the backward halves
+        // one phase: the fused kernel and the split pairs (each workgroup
+        // eliminates a sub-plane and substitutes back through it)
         if (!fz.empty()) add(Launch::ZFUSED, f32, 0, 0, tab, make_list(fz, w), lf);
-        std::vector<int> lists(sz.size());
-        for (size_t k = 0; k < sz.size(); ++k) {
-            lists[k] = make_list(sz[k].second, w);
-            add(Launch::ZF, f32, 0, sz[k].first, tab, lists[k], lsz[k]);
-        }
-        ++phase;
-        for (size_t k = 0; k < sz.size(); ++k) add(Launch::ZB, f32, 0, sz[k].first, tab, lists[k], lsz[k]);
+        for (size_t k = 0; k < sz.size(); ++k) add(Launch::ZFB, f32, 0, sz[k].first, tab, make_list(sz[k].second, w), lsz[k]);
         ++phase;
     };
     auto first = [](int k) { std::vector<int> v(k); for (int a = 0; a < k; ++a) v[a] = a; return v; };
@@ -543,15 +537,12 @@ int dfp_batch_project(dfp_batch* b, int n, dfp_plan* const* plans, double* const
                               : launch_xinv_batch(st, l.R, l.mode, reinterpret_cast<const XPar<double2>*>(D + l.tab),
                                                   o, ix, nc, gx, l.lds);
                     break;
-                case Launch::ZF:
-                case Launch::ZB: {
-                    const int dir = l.kind == Launch::ZF ? -1 : 1;
-                    e = l.f32 ? launch_zsolve_batch(st, dir, l.R, reinterpret_cast<const ZPar<float2>*>(D + l.tab), o,
+                case Launch::ZFB:
+                    e = l.f32 ? launch_zsolve_batch(st, l.R, reinterpret_cast<const ZPar<float2>*>(D + l.tab), o,
                                                     ix, nc, gz, l.lds)
-                              : launch_zsolve_batch(st, dir, l.R, reinterpret_cast<const ZPar<double2>*>(D + l.tab), o,
+                              : launch_zsolve_batch(st, l.R, reinterpret_cast<const ZPar<double2>*>(D + l.tab), o,
                                                     ix, nc, gz, l.lds);
                     break;
-                }
                 case Launch::ZFUSED:
                     e = l.f32 ? launch_zfused_batch(st, reinterpret_cast<const ZPar<float2>*>(D + l.tab), o, ix, nc,
                                                     gz, l.lds)

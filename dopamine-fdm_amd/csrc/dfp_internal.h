@@ -274,8 +274,9 @@ namespace dfp {
 hipError_t launch_div_rfft_x(dfp_plan* p, const double* U, const double* V, const double* W,
                              const double* rhs, int from_rhs, const double* phib = nullptr);
 hipError_t launch_lu_table(dfp_plan* p);  // pins the (0,0) mode of the periodic x only
-// dir -1: z FFT + forward elimination, +1: back substitution + inverse z FFT
-hipError_t launch_zsolve(dfp_plan* p, int dir);
+// z FFT + forward elimination, then back substitution + inverse z FFT, per
+// sub-plane in one workgroup
+hipError_t launch_zsolve(dfp_plan* p);
 int zsolve_rows_per_group(const FftDesc& fz, int nym, size_t lds_limit);
 bool z_two_pass(int n, int& n1, int& n2);
 // z kernel variant of a plan: -1 fused, else n1*32 + n2 of the split pair (0: staged transform)
@@ -301,7 +302,7 @@ size_t zf_lds_bytes(const dfp_plan* p, int f32);
 int zf_rows_per_call(const FftDesc& fz);
 // FP32 form
 hipError_t launch_rfft_x32(dfp_plan* p, int mode, const XArgs& a);
-hipError_t launch_zsolve32(dfp_plan* p, int dir);
+hipError_t launch_zsolve32(dfp_plan* p);
 hipError_t launch_irfft_x32(dfp_plan* p, double* rhs, int accumulate, const double* pbar, double* part0, double* P,
                             double dtc, int save_p);
 hipError_t launch_mode00(dfp_plan* p, hipStream_t s);  // rowpart -> pbar of the plan's FP32 buffers
@@ -345,7 +346,7 @@ template <typename C>
 hipError_t launch_xinv_batch(hipStream_t s, int R, int accum, const XPar<C>* cs, const int* off, const int* idx,
                              const int* nc, int grid, size_t lds);
 template <typename C>
-hipError_t launch_zsolve_batch(hipStream_t s, int dir, int variant, const ZPar<C>* cs, const int* off, const int* idx,
+hipError_t launch_zsolve_batch(hipStream_t s, int variant, const ZPar<C>* cs, const int* off, const int* idx,
                                const int* nc, int grid, size_t lds);
 hipError_t launch_correct_batch(hipStream_t s, const CPar* cs, const int* off, const int* idx, const int* nc,
                                 int grid);

@@ -13,17 +13,19 @@
 // (mult = DL/D', D' <- D - mult*DU); LAPACK interchanges rows only in a few
 // low modes, which moves the solution at the 1e-15 level.
 //
-// k_zsolve_fwd: rows j = 0..nym-1 in groups of G: FFT of the rows (two
+// k_zsolve, one workgroup per sub-plane (or P planes, below), two halves:
+//   elimination: rows j = 0..nym-1 in groups of G: FFT of the rows (two
 //   register passes with one LDS exchange for n = N1*N2, else the staged LDS
-//   transform), then per mode b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1).
-// k_zsolve_bwd: rows j = nym-1..0: x(j) = (b'(j) - DU(j) x(j+1)) / D'(j), then
-//   the inverse FFT of the rows.
-// One lane per mode runs the elimination of its column of the group.
-// Both read and write the spectrum once (64 H bytes for the pair in FP64,
+//   transform), then per mode b'(j) = b(j) - DL(j-1)/D'(j-1) * b'(j-1),
+//   the rows written back to the spectrum;
+//   back substitution: groups from the top, rows j = nym-1..0:
+//   x(j) = (b'(j) - DU(j) x(j+1)) / D'(j), then the inverse FFT of the rows.
+// One lane per mode runs the recurrence of its column of the group.  Each
+// half reads and writes the sub-plane once (64 H bytes for both in FP64,
 // 32 H in the FP32 form, whose z FFTs run in FP32 while the elimination
-// stays FP64 on values rounded to FP32 only where stored).  While a
-// group is transformed and swept, the next group's rows, pivots and
-// coefficients are already loading into registers.
+// stays FP64 on values rounded to FP32 only where stored).  While a group
+// is transformed and swept, the next group's rows, pivots and coefficients
+// are already loading into registers.
 #include <algorithm>
 
 #include "fft_lds.h"
@@ -532,19 +534,27 @@ bool z_two_pass(int n, int& n1, int& n2) {
     return false;
 }
 
-// N1 = N2 = 0: the staged transform
+// Both halves in one launch: the workgroup that eliminates a sub-plane
+// (writing its rows back) then substitutes back through the same sub-plane,
+// top group first, i.e. the rows it stored last.  Those reads follow their
+// writes by the duration of one workgroup rather than one launch, so they
+// are served from the caches in front of HBM, and the rows overwritten by
+// the back substitution are written to HBM once.  The barrier between the
+// halves orders the stores of every lane before the loads of every other
+// lane of the workgroup.  N1 = N2 = 0: the staged transform.
 template <typename C, int N1, int N2>
-__global__ void __launch_bounds__(kZsThreads) k_zsolve_fwd(ZPar<C> zp) {
+__global__ void __launch_bounds__(kZsThreads) k_zsolve(ZPar<C> zp) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
-    if constexpr (N1 == 0) zfwd_body<C>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
-    else zfwd2_body<C, N1, N2>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
-}
-
-template <typename C, int N1, int N2>
-__global__ void __launch_bounds__(kZsThreads) k_zsolve_bwd(ZPar<C> zp) {
-    extern __shared__ __align__(16) unsigned char smem_raw[];
-    if constexpr (N1 == 0) zbwd_body<C>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
-    else zbwd2_body<C, N1, N2>(zp, xcd_remap(blockIdx.x, gridDim.x), smem_raw);
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if constexpr (N1 == 0) {
+        zfwd_body<C>(zp, b, smem_raw);
+        __syncthreads();
+        zbwd_body<C>(zp, b, smem_raw);
+    } else {
+        zfwd2_body<C, N1, N2>(zp, b, smem_raw);
+        __syncthreads();
+        zbwd2_body<C, N1, N2>(zp, b, smem_raw);
+    }
 }
 
 __device__ __forceinline__ int zfind_case(const int* __restrict__ off, int nc, int w) {
@@ -557,7 +567,7 @@ __device__ __forceinline__ int zfind_case(const int* __restrict__ off, int nc, i
     return lo;
 }
 
-template <typename C, int DIR, int N1, int N2>
+template <typename C, int N1, int N2>
 __global__ void __launch_bounds__(kZsThreads) kb_zsolve(const ZPar<C>* __restrict__ cs, const int* __restrict__ off,
                                                         const int* __restrict__ idx, const int* __restrict__ pnc) {
     extern __shared__ __align__(16) unsigned char smem_raw[];
@@ -568,12 +578,15 @@ __global__ void __launch_bounds__(kZsThreads) kb_zsolve(const ZPar<C>* __restric
         const int w = base + me;
         if (w >= total) break;
         const int c = zfind_case(off, nc, w);
+        // both halves of the sub-plane, as k_zsolve
         if constexpr (N1 == 0) {
-            if (DIR < 0) zfwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
-            else zbwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
+            zfwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
+            __syncthreads();
+            zbwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
         } else {
-            if (DIR < 0) zfwd2_body<C, N1, N2>(cs[idx[c]], w - off[c], smem_raw);
-            else zbwd2_body<C, N1, N2>(cs[idx[c]], w - off[c], smem_raw);
+            zfwd2_body<C, N1, N2>(cs[idx[c]], w - off[c], smem_raw);
+            __syncthreads();
+            zbwd2_body<C, N1, N2>(cs[idx[c]], w - off[c], smem_raw);
         }
         __syncthreads();
     }
@@ -616,56 +629,52 @@ size_t z_lds_bytes(const dfp_plan* p, int f32) {
 int z_variant(const dfp_plan* p) { return p->zfused ? -1 : p->zn1 * 32 + p->zn2; }
 
 template <typename C>
-static hipError_t launch_zsolve_t(dfp_plan* p, int dir, const ZPar<C>& zp) {
+static hipError_t launch_zsolve_t(dfp_plan* p, const ZPar<C>& zp) {
     const bool f32 = sizeof(C) == 8;
     const size_t lds = z_lds_bytes(p, f32);
-    const int slot = prof_begin(p, dir < 0 ? (f32 ? "zsolve_fwd32" : "zsolve_fwd") : (f32 ? "zsolve_bwd32" : "zsolve_bwd"));
+    const int slot = prof_begin(p, f32 ? "zsolve32" : "zsolve");
     const dim3 grid(zp.nblocks), blk(kZsThreads);
     switch (z_variant(p)) {
-#define DFP_ZV_CASE(a, c)                                                                          \
-    case a * 32 + c:                                                                               \
-        if (dir < 0) hipLaunchKernelGGL((k_zsolve_fwd<C, a, c>), grid, blk, lds, p->stream, zp);  \
-        else hipLaunchKernelGGL((k_zsolve_bwd<C, a, c>), grid, blk, lds, p->stream, zp);          \
+#define DFP_ZV_CASE(a, c)                                                          \
+    case a * 32 + c:                                                               \
+        hipLaunchKernelGGL((k_zsolve<C, a, c>), grid, blk, lds, p->stream, zp);    \
         break;
         DFP_ZV_LIST(DFP_ZV_CASE)
 #undef DFP_ZV_CASE
         default:
-            if (dir < 0) hipLaunchKernelGGL((k_zsolve_fwd<C, 0, 0>), grid, blk, lds, p->stream, zp);
-            else hipLaunchKernelGGL((k_zsolve_bwd<C, 0, 0>), grid, blk, lds, p->stream, zp);
+            hipLaunchKernelGGL((k_zsolve<C, 0, 0>), grid, blk, lds, p->stream, zp);
     }
     prof_end(p, slot);
     return hipGetLastError();
 }
 
-hipError_t launch_zsolve(dfp_plan* p, int dir) { return launch_zsolve_t<double2>(p, dir, make_zpar<double2>(p, 0)); }
+hipError_t launch_zsolve(dfp_plan* p) { return launch_zsolve_t<double2>(p, make_zpar<double2>(p, 0)); }
 
-hipError_t launch_zsolve32(dfp_plan* p, int dir) { return launch_zsolve_t<float2>(p, dir, make_zpar<float2>(p, 1)); }
+hipError_t launch_zsolve32(dfp_plan* p) { return launch_zsolve_t<float2>(p, make_zpar<float2>(p, 1)); }
 
 // variant: z_variant of the cases of the list (all alike)
 template <typename C>
-hipError_t launch_zsolve_batch(hipStream_t s, int dir, int variant, const ZPar<C>* cs, const int* off, const int* idx,
+hipError_t launch_zsolve_batch(hipStream_t s, int variant, const ZPar<C>* cs, const int* off, const int* idx,
                                const int* nc, int grid, size_t lds) {
     const dim3 blk(kZsThreads);
     auto go = [&](auto k) {
         hipLaunchKernelGGL(k, dim3(resident_grid(k, grid, kZsThreads, lds)), blk, lds, s, cs, off, idx, nc);
     };
     switch (variant) {
-#define DFP_ZV_CASE(a, c)                              \
-    case a * 32 + c:                                   \
-        if (dir < 0) go(kb_zsolve<C, -1, a, c>);       \
-        else go(kb_zsolve<C, 1, a, c>);                \
+#define DFP_ZV_CASE(a, c)          \
+    case a * 32 + c:               \
+        go(kb_zsolve<C, a, c>);    \
         break;
         DFP_ZV_LIST(DFP_ZV_CASE)
 #undef DFP_ZV_CASE
         default:
-            if (dir < 0) go(kb_zsolve<C, -1, 0, 0>);
-            else go(kb_zsolve<C, 1, 0, 0>);
+            go(kb_zsolve<C, 0, 0>);
     }
     return hipGetLastError();
 }
-template hipError_t launch_zsolve_batch<double2>(hipStream_t, int, int, const ZPar<double2>*, const int*, const int*,
+template hipError_t launch_zsolve_batch<double2>(hipStream_t, int, const ZPar<double2>*, const int*, const int*,
                                                  const int*, int, size_t);
-template hipError_t launch_zsolve_batch<float2>(hipStream_t, int, int, const ZPar<float2>*, const int*, const int*,
+template hipError_t launch_zsolve_batch<float2>(hipStream_t, int, const ZPar<float2>*, const int*, const int*,
                                                 const int*, int, size_t);
 
 // rows per group: the group (its rows in each of the block's planes) fits
