@@ -574,7 +574,21 @@ def main() -> int:
     ap.add_argument("--save-tuning", default=None, help="write the autotuner table (with new entries) here")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="only run one device-to-device copy of this many MiB (calibrates FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--copy-probe-torch", action="store_true",
+                    help="with --copy-probe-mib: a torch elementwise copy (one read, one write) and add (two reads, "
+                         "one write) kernel in place of the runtime's copy")
     args = ap.parse_args()
+    if args.copy_probe_mib > 0 and args.copy_probe_torch:
+        import torch
+
+        n = args.copy_probe_mib * (1 << 20) // 8
+        a = torch.ones(n, dtype=torch.float64, device="cuda")
+        b, c = torch.empty_like(a), torch.empty_like(a)
+        b.copy_(a)
+        torch.add(a, b, out=c)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes": 8 * n, "add_probe_bytes": 24 * n}))
+        return 0
     if args.copy_probe_mib > 0:
         from dfp import hipmem
 
