@@ -220,3 +220,41 @@ def test_batch_shape_families():
     t["batch"]["fp32"] = {"family": "split_r2", "occupancy": [4, 4, 8], "us": 1.0}
     assert autotune.batch_family(t, dfp.DFP_FP32_REFINE) == "split_r2"
     assert autotune.batch_family(None) == "per_plan"
+
+
+def test_serving_plans_take_the_batch_family_or_their_own_shape(monkeypatch):
+    """Cases below solo_cells are served inside batches and take the shape
+    family the table records for batched calls; larger ones run alone and
+    take their own size's entry."""
+    from dfp import serving
+    from dfp.serving import Request
+
+    made = []
+    real = serving._Resident
+
+    class _Res(_FakeResident):
+        def __init__(self, case_id, g, table, sweeps, batched=True):
+            super().__init__(case_id, g, table, sweeps, batched)
+            made.append((case_id, batched))
+
+    monkeypatch.setattr(serving, "_Resident", _Res)
+    s = _bare_server(1e12)
+    s.solo_cells = 40 ** 3
+    s._admit(Request(0, 0, (32, 32, 32), 1.5, seed=0))
+    s._admit(Request(1, 1, (48, 48, 48), 1.5, seed=1))
+    assert made == [(0, True), (1, False)]
+
+    # the real resident's plan(): family per precision from the table
+    chosen = []
+    monkeypatch.setattr(serving, "family_plan",
+                        lambda fam, g, prec, table, sweeps=None: chosen.append((fam, prec, sweeps)) or object())
+    table = {"plans": {}, "batch": {"fp64": {"family": "split_r2"}, "fp32": {"family": "fused"}}}
+    g = dfp.channel_grid(32, 32, 32, 1.5)
+    for batched in (True, False):
+        r = real.__new__(real)
+        r.g, r.table, r.sweeps, r.plans, r.batched = g, table, 2, {}, batched
+        r.plan(dfp.DFP_FP64)
+        r.plan(dfp.DFP_FP32_REFINE)
+        r.plan(dfp.DFP_FP64)  # made once per precision
+    assert chosen == [("split_r2", dfp.DFP_FP64, None), ("fused", dfp.DFP_FP32_REFINE, 2),
+                      ("per_plan", dfp.DFP_FP64, None), ("per_plan", dfp.DFP_FP32_REFINE, 2)]
