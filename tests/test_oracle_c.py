@@ -1,4 +1,4 @@
-"""Pins the C oracle (oracle/dfp_oracle.c, the checker on the GPU box and the
+"""Pins the C oracle (oracle/dfp_oracle.c, the checker of the GPU tests and the
 host-CPU baseline) against the numpy oracle and the golden fixtures."""
 from __future__ import annotations
 
