@@ -316,3 +316,55 @@ def close_all(batch, plans):
         batch.close()
     for p in plans:
         p.close()
+
+
+def parse_size(text: str) -> tuple[int, int, int]:
+    """'256x192x256' -> (256, 192, 256)."""
+    parts = text.lower().split("x")
+    if len(parts) != 3 or not all(p.isdigit() and int(p) > 0 for p in parts):
+        raise ValueError(f"size {text!r} is not NXPxNYMxNZP")
+    return tuple(int(p) for p in parts)
+
+
+def main(argv=None) -> int:
+    """Tune this device and write the table:
+
+        python -m dfp.autotune --sizes 256x192x256,128x96x128 --precisions fp64,fp32 \\
+            [--batch 48] [--table PATH]
+
+    Sizes are channel grids (x, z periodic; y walls, stretching --alpha).
+    Entries already in the table are kept; --batch N also tunes the shape
+    family and workgroups per CU of an N-case batch of the uneven mix."""
+    import argparse
+
+    from .workloads import batched_cases
+
+    ap = argparse.ArgumentParser(prog="python -m dfp.autotune",
+                                 description="Tune this device's launch shapes and write the table.")
+    ap.add_argument("--sizes", default="", help="comma-separated NXPxNYMxNZP grids to tune one plan at a time")
+    ap.add_argument("--precisions", default="fp64", help="fp64, fp32 or both (comma-separated)")
+    ap.add_argument("--alpha", type=float, default=2.0, help="tanh stretching of the y grid")
+    ap.add_argument("--batch", type=int, default=0, help="also tune the batched call of this many uneven cases")
+    ap.add_argument("--table", default=DEFAULT_TABLE, help="table to read and update")
+    args = ap.parse_args(argv)
+    names = {"fp64": DFP_FP64, "fp32": DFP_FP32_REFINE}
+    precs = []
+    for n in filter(None, args.precisions.split(",")):
+        if n not in names:
+            ap.error(f"unknown precision {n!r} (fp64 or fp32)")
+        precs.append(names[n])
+    sizes = [parse_size(s) for s in filter(None, args.sizes.split(","))]
+    table = load_table(args.table)
+    tune_sizes(sizes, tuple(precs), table, alpha=args.alpha, log=print)
+    if args.batch > 0:
+        specs = batched_cases(args.batch)
+        tune_sizes(sorted({s.key for s in specs}), tuple(precs), table, log=print)
+        for prec in precs:
+            r = tune_batch_shapes(specs, prec, table, keep=False, log=print)
+            print(f"batch {_PREC_NAME[prec]}: {r['family']} at {r['occupancy']}, {r['us']} us")
+    save_table(table, args.table)
+    return 0
+
+
+if __name__ == "__main__":
+    raise SystemExit(main())

@@ -258,3 +258,15 @@ def test_serving_plans_take_the_batch_family_or_their_own_shape(monkeypatch):
         r.plan(dfp.DFP_FP64)  # made once per precision
     assert chosen == [("split_r2", dfp.DFP_FP64, None), ("fused", dfp.DFP_FP32_REFINE, 2),
                       ("per_plan", dfp.DFP_FP64, None), ("per_plan", dfp.DFP_FP32_REFINE, 2)]
+
+
+def test_autotune_command_line_parsing():
+    assert autotune.parse_size("256x192x256") == (256, 192, 256)
+    assert autotune.parse_size("64X32X48") == (64, 32, 48)
+    for bad in ("256x192", "ax2x3", "0x8x8", ""):
+        with pytest.raises(ValueError):
+            autotune.parse_size(bad)
+    with pytest.raises(SystemExit):
+        autotune.main(["--precisions", "fp16"])
+    with pytest.raises(SystemExit):
+        autotune.main(["--help"])
