@@ -3,8 +3,8 @@
 !  periodic (y_bc_type 0, the periodic box), on one rank: the
 !  same subroutine names and arguments, backed by the dfp HIP library through its C ABI.  The fields are the
 !  global-module arrays (global.f90:139-146, :188-197); they are copied to the device and back on each call.
-!  A caller that keeps them on the device passes device addresses with DFP_FIELDS_ON_DEVICE instead (see
-!  dfp_project in include/dfp.h).
+!  A caller that keeps them on the device calls compute_projection_step_device with their device addresses
+!  instead (no copies).
 Module projection
 
   Use iso_c_binding
@@ -18,7 +18,8 @@ Module projection
   Public :: compute_projection_step, compute_pseudo_pressure_rhs, set_outflow_backflow_pressure, &
             solve_poisson_equation, project_velocity, apply_periodic_xz_pressure, &
             update_ghost_interior_planes_pressure, update_ghost_interior_planes_pressure_x, &
-            finish_pressure_ghosts, check_divergence, projection_release
+            finish_pressure_ghosts, check_divergence, projection_release, &
+            compute_projection_step_device, check_divergence_device
 
   Type(c_ptr), Save :: plan = c_null_ptr
   ! the grid the plan was built from (c_loc needs TARGET arrays)
@@ -175,5 +176,48 @@ Contains
     Call dfp_check(dfp_max_divergence(plan, U, V, W, m, 0_c_int), 'dfp_max_divergence')
     max_divergence = m
   End Subroutine check_divergence
+
+  !> A device address as an array argument of the C ABI: only the address travels (the pointer is
+  !  contiguous, so no copy of the target is made on the host)
+  Subroutine as_array(addr, a)
+    Type(c_ptr), Intent(In) :: addr
+    Real(c_double), Pointer, Contiguous, Intent(Out) :: a(:)
+    Call c_f_pointer(addr, a, [1])
+  End Subroutine as_array
+
+  !> compute_projection_step (projection.f90:21-34) on fields the caller keeps on the device, as the
+  !  reference's OpenACC build keeps U, V, W, rhs_p and P present there: their device addresses (for
+  !  example from !$acc host_data use_device, or hipMalloc), with the global module's shapes; with the
+  !  outlet backflow term also those of Pb and phib, else c_null_ptr.  The grid and the time-step state
+  !  are the module's, as for compute_projection_step.  No copies over PCIe.
+  Subroutine compute_projection_step_device(U_d, V_d, W_d, rhs_p_d, P_d, Pb_d, phib_d)
+    Type(c_ptr), Intent(In) :: U_d, V_d, W_d, rhs_p_d, P_d, Pb_d, phib_d
+    Real(c_double), Pointer, Contiguous :: ud(:), vd(:), wd(:), rd(:), pd(:), pbd(:), phd(:)
+    Integer(c_int) :: flags
+    Call ensure_plan
+    Call as_array(U_d, ud);  Call as_array(V_d, vd);  Call as_array(W_d, wd)
+    Call as_array(rhs_p_d, rd);  Call as_array(P_d, pd)
+    If ( outflow_backflow_beta > 0d0 ) Then
+       Call as_array(Pb_d, pbd);  Call as_array(phib_d, phd)
+       Call dfp_check(dfp_set_outflow(plan, outflow_backflow_beta, outflow_Uc, pbd, phd), 'dfp_set_outflow')
+    Else
+       Call dfp_check(dfp_set_outflow(plan, 0d0, outflow_Uc, no_face, no_face), 'dfp_set_outflow')
+    End If
+    flags = DFP_FIELDS_ON_DEVICE
+    If ( rk_step == 3 ) flags = flags + DFP_SAVE_P
+    Call dfp_check(dfp_project(plan, ud, vd, wd, rd, pd, dt, rk_coef(rk_step,rk_step), flags), 'dfp_project')
+  End Subroutine compute_projection_step_device
+
+  !> check_divergence on device-resident U, V, W (their device addresses)
+  Subroutine check_divergence_device(U_d, V_d, W_d, max_divergence)
+    Type(c_ptr), Intent(In) :: U_d, V_d, W_d
+    Real(Int64), Intent(Out) :: max_divergence
+    Real(c_double), Pointer, Contiguous :: ud(:), vd(:), wd(:)
+    Real(c_double) :: m
+    Call ensure_plan
+    Call as_array(U_d, ud);  Call as_array(V_d, vd);  Call as_array(W_d, wd)
+    Call dfp_check(dfp_max_divergence(plan, ud, vd, wd, m, DFP_FIELDS_ON_DEVICE), 'dfp_max_divergence')
+    max_divergence = m
+  End Subroutine check_divergence_device
 
 End Module projection

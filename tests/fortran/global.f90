@@ -8,9 +8,10 @@ Module global
   Integer(Int32) :: rk_step = 1
   Real(Int64) :: dx, dz, dt
   Real(Int64), Allocatable, Dimension(:) :: x, y, z, yg, zg
-  Real(Int64), Allocatable, Dimension(:,:,:) :: U, V, W, P, rhs_p
+  ! TARGET: the test driver copies them to device memory it allocates itself
+  Real(Int64), Allocatable, Dimension(:,:,:), Target :: U, V, W, P, rhs_p
   Real(Int64), Allocatable, Dimension(:,:) :: rk_coef
   ! outlet backflow stabilisation of the inflow/outflow x (global.f90:188-197)
   Real(Int64) :: outflow_backflow_beta = 0d0, outflow_Uc = 0d0
-  Real(Int64), Allocatable, Dimension(:,:) :: Pb, phib
+  Real(Int64), Allocatable, Dimension(:,:), Target :: Pb, phib
 End Module global

@@ -2,7 +2,11 @@
 the reference's `projection` interface and reaches HIP through the C ABI.
 A Fortran driver fills the global arrays with the reference's bounds
 (rhs_p(2:nxg,2:nyg-1,2:nzg)), calls compute_projection_step at RK stage 3 and
-check_divergence, and the results are compared with the C oracle."""
+check_divergence, and the results are compared with the C oracle.  In its
+device mode the driver copies the fields to device memory it allocates and
+calls the device-resident entry points (compute_projection_step_device,
+check_divergence_device), whose results must equal the host entry points'
+bit for bit."""
 from __future__ import annotations
 
 import os
@@ -35,7 +39,8 @@ def test_dropin_compiles_and_links_against_the_c_abi():
             subprocess.run([FLANG, "-O1", f"-J{d}", "-c", s, "-o", o], check=True)
             objs.append(o)
         exe = os.path.join(d, "driver")
-        subprocess.run([FLANG, *objs, "-o", exe, "-L" + os.path.dirname(dfp.LIB_PATH), "-ldfp"], check=True)
+        subprocess.run([FLANG, *objs, "-o", exe, "-L" + os.path.dirname(dfp.LIB_PATH), "-ldfp",
+                        "-L/opt/rocm/lib", "-lamdhip64"], check=True)
         assert os.path.exists(exe)
 
 
@@ -101,3 +106,29 @@ def test_dropin_projection_matches_oracle(dims, xbc, beta, tmp_path):
     assert maxdiv == op.check_divergence(Uf, Vf, Wf)
     # and close to the oracle's value on its own fields (which differ at round-off)
     assert abs(maxdiv - op.check_divergence(U, V, W)) <= 1e-12 * max(1.0, abs(maxdiv))
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("dims,xbc,beta", [DROPIN_CASES[0], DROPIN_CASES[3], DROPIN_CASES[4], DROPIN_CASES[5]],
+                         ids=lambda v: str(v))
+def test_dropin_device_entry_points_match_host_entry_points(dims, xbc, beta, tmp_path):
+    if not os.path.exists(DRIVER):
+        raise RuntimeError(f"Fortran driver not built: {DRIVER} (run __graft_entry__.build())")
+    if xbc == "py":
+        g = dfp.periodic_y_grid(*dims)
+    elif xbc == "duct":
+        g = dfp.duct_grid(*dims, 1.8, 1.5)
+    else:
+        g = make_grid(*dims, 1.8, x_bc_type=xbc)
+    U, V, W, _, _ = fields(g, 23, 0.1)
+    Pb = np.asfortranarray(np.random.default_rng(3).standard_normal((g.nyg - 2, g.nzg - 2)))
+    fin = tmp_path / "in.bin"
+    _write_case(fin, g, U, V, W, 1e-3, beta, 0.9, Pb)
+    res = {}
+    for mode in ("host", "device"):
+        fout = tmp_path / f"out_{mode}.bin"
+        print(f"fortran driver {dims} x_bc_type {xbc} beta {beta} {mode}", flush=True)
+        subprocess.run([DRIVER, str(fin), str(fout), mode], check=True, timeout=120)
+        res[mode] = np.fromfile(fout, dtype=np.float64)
+    assert res["host"].shape == res["device"].shape
+    assert np.array_equal(res["host"], res["device"])
