@@ -132,6 +132,41 @@ def bench_host_staged(g, reps: int = 6) -> dict:
             "pcie_bytes_per_projection": int(moved), "reps": reps}
 
 
+def bench_fortran_dropin(g, reps_host: int = 6, reps_device: int = 60) -> dict:
+    """The Fortran drop-in module (Module projection, compute_projection_step
+    and compute_projection_step_device) timed through its test driver, a
+    Fortran program that fills the global-module arrays with the reference's
+    bounds: host arrays (copied over PCIe inside each call) and device copies
+    it allocates with hipMalloc.  Each call returns with its results in place;
+    RK stages cycle 1, 2, 3 with P saved at the third."""
+    import subprocess
+    import tempfile
+
+    import dfp
+
+    drv = os.path.join(ROOT, "tests", "fortran", "build", "driver")
+    if not os.path.exists(drv):
+        return {"skipped": "Fortran driver not built (tests/fortran/build/driver)"}
+    U, V, W = dfp.synth_fields(g, 98, SIGMA)
+    out = {"driver": "tests/fortran/driver.f90"}
+    with tempfile.TemporaryDirectory() as d:
+        fin = os.path.join(d, "case.bin")
+        with open(fin, "wb") as f:
+            np.array([g.nxp, g.nym, g.nzp, g.x_bc_type, g.y_bc_type, g.z_bc_type], dtype=np.int32).tofile(f)
+            np.array([g.dx, g.dz, DT, 0.0, 0.0]).tofile(f)
+            for a in (g.x, g.y, g.z, g.yg, g.zg):
+                np.ascontiguousarray(a, dtype=np.float64).tofile(f)
+            for a in (U, V, W, np.zeros((g.nyg - 2, g.nzg - 2), order="F")):
+                a.ravel(order="F").tofile(f)
+        del U, V, W
+        for mode, reps in (("host", reps_host), ("device", reps_device)):
+            r = subprocess.run([drv, fin, os.path.join(d, "unused.bin"), mode, str(reps)], capture_output=True,
+                               text=True, timeout=300, check=True)
+            us = float(r.stdout.split("us_per_call")[1].split()[0])
+            out[mode] = {"us_per_projection": round(us, 1), "projections_per_s": round(1e6 / us, 2), "reps": reps}
+    return out
+
+
 def make_sets(plan, g, n_sets, seed0):
     import dfp
 
@@ -710,6 +745,7 @@ def main() -> int:
         out["mixed"] = mixed
     if not args.no_host_staged:
         out["host_staged"] = bench_host_staged(g)
+        out["fortran_dropin"] = bench_fortran_dropin(g)
     if not args.no_inflow:
         if args.no_mixed:
             plan.close()

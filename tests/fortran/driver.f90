@@ -43,21 +43,31 @@ End Module hip_min
 !  writes the results back (raw little-endian float64 streams).  The case gives the x, y and z boundary types
 !  and, for the inflow/outflow x, the outlet backflow state (beta, outflow_Uc, Pb).  With a third argument
 !  "device" the fields are first copied to device memory the driver allocates, and the device-resident
-!  entry points run on them.
+!  entry points run on them.  A fourth argument reps > 0 times the drop-in instead: after one untimed
+!  call, reps calls at RK stages 1, 2, 3, 1, ... (P saved at stage 3), each returning with its results
+!  in place, and prints "us_per_call <microseconds>"; no output file is written.
 Program driver
   Use iso_c_binding
   Use global
   Use projection
   Use hip_min
   Implicit None
-  Integer :: u_in, u_out, n(6), nxp, nym, nzp
+  Integer :: u_in, u_out, n(6), nxp, nym, nzp, reps, r
+  Integer(Int64) :: c0, c1, crate
   Real(Int64) :: hdr(5), maxdiv
-  Character(256) :: fin, fout, mode
+  Character(256) :: fin, fout, mode, arg
   Type(c_ptr) :: Ud, Vd, Wd, rd, Pd, Pbd, phd
+  Logical :: dev
 
   Call get_command_argument(1, fin)
   Call get_command_argument(2, fout)
   Call get_command_argument(3, mode)
+  dev = Trim(mode) == 'device'
+  reps = 0
+  If ( command_argument_count() >= 4 ) Then
+     Call get_command_argument(4, arg)
+     Read(arg, *) reps
+  End If
   Open(newunit=u_in, file=Trim(fin), access='stream', form='unformatted', status='old')
   Read(u_in) n
   nxp = n(1); nym = n(2); nzp = n(3); x_bc_type = n(4); y_bc_type = n(5); z_bc_type = n(6)
@@ -84,7 +94,7 @@ Program driver
   rk_coef(1,1) = 8d0/15d0;  rk_coef(2,2) = 5d0/12d0;  rk_coef(3,3) = 3d0/4d0
 
   rk_step = 3
-  If ( Trim(mode) == 'device' ) Then
+  If ( dev ) Then
      Ud = to_device(c_loc(U), Size(U, kind=c_size_t));  Vd = to_device(c_loc(V), Size(V, kind=c_size_t))
      Wd = to_device(c_loc(W), Size(W, kind=c_size_t));  rd = to_device(c_loc(rhs_p), Size(rhs_p, kind=c_size_t))
      Pd = to_device(c_loc(P), Size(P, kind=c_size_t))
@@ -92,6 +102,20 @@ Program driver
      If ( outflow_backflow_beta > 0d0 ) Then
         Pbd = to_device(c_loc(Pb), Size(Pb, kind=c_size_t));  phd = to_device(c_loc(phib), Size(phib, kind=c_size_t))
      End If
+  End If
+  If ( reps > 0 ) Then
+     Call project_once
+     Call system_clock(c0, crate)
+     Do r = 1, reps
+        rk_step = Mod(r - 1, 3) + 1
+        Call project_once
+     End Do
+     Call system_clock(c1)
+     Write(*, '(A, F12.2)') 'us_per_call ', 1d6 * Real(c1 - c0, Int64) / Real(crate, Int64) / reps
+     Call projection_release
+     Stop
+  End If
+  If ( dev ) Then
      Call compute_projection_step_device(Ud, Vd, Wd, rd, Pd, Pbd, phd)
      Call check_divergence_device(Ud, Vd, Wd, maxdiv)
      Call from_device(c_loc(U), Ud, Size(U, kind=c_size_t));  Call from_device(c_loc(V), Vd, Size(V, kind=c_size_t))
@@ -110,4 +134,16 @@ Program driver
   Write(u_out) U, V, W, rhs_p, P, maxdiv, phib
   Close(u_out)
   Call projection_release
+
+Contains
+
+  !> one projection through the drop-in, host arrays or device copies
+  Subroutine project_once
+    If ( dev ) Then
+       Call compute_projection_step_device(Ud, Vd, Wd, rd, Pd, Pbd, phd)
+    Else
+       Call compute_projection_step
+    End If
+  End Subroutine project_once
+
 End Program driver
