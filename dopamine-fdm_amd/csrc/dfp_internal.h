@@ -387,6 +387,24 @@ __device__ __forceinline__ Glob<T>* gptr(T* p) {
     return (Glob<T>*)p;
 }
 
+// two consecutive doubles at an 8-byte aligned address
+typedef double dbl2u __attribute__((ext_vector_type(2), aligned(8)));
+
+__device__ __forceinline__ dbl2u ld2(const Glob<double>* p) { return *reinterpret_cast<const Glob<dbl2u>*>(p); }
+
+// store the elements e with ok_e; both with one 16-byte store
+__device__ __forceinline__ void st2(Glob<double>* p, double a, double b, bool ok0, bool ok1) {
+    if (ok0 && ok1) {
+        dbl2u v;
+        v.x = a;
+        v.y = b;
+        *reinterpret_cast<Glob<dbl2u>*>(p) = v;
+    } else {
+        if (ok0) p[0] = a;
+        if (ok1) p[1] = b;
+    }
+}
+
 // MI355X dispatches workgroup w of a launch to XCD w % 8, and each XCD has its
 // own L2.  xcd_remap(w, n) renumbers the n workgroups so that each XCD gets a
 // contiguous range of logical indices: neighbouring tiles (which read the

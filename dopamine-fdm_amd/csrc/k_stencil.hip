@@ -56,10 +56,11 @@ hipError_t launch_divergence(dfp_plan* p, const double* U, const double* V, cons
 //   without the backflow term)
 // V(i,j,k) -= (phi(j+1)-phi(j))/dyg(j)      i=2..nxg-1, j=2..ny-1,  k=2..nzg-1
 // W(i,j,k) -= (phi(k+1)-phi(k))/dzg(k)      i=2..nxg-1, j=2..nyg-1, k=2..nz-1
-// Each thread walks kCorrectK consecutive z planes of one (i,j) column, so
-// phi(k+1) read for W at one step is phi(k) of the next: phi is read from
-// memory once.
-constexpr int kCorrectK = 8;
+// Each thread takes two consecutive x points (i, i+1) of one row j, with
+// 16-byte loads and stores (8-byte aligned where the row length is odd), and
+// walks kCorrectK consecutive z planes, so phi(k+1) read for W at one step is
+// phi(k) of the next: phi is read from memory once.
+constexpr int kCorrectK = 4;
 
 __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
 #pragma clang fp contract(off)
@@ -72,19 +73,22 @@ __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
     const auto* __restrict__ rhs = gptr(c.rhs);
     const auto inv_dyg = gptr(g.inv_dyg);
     const auto inv_dzg = gptr(g.inv_dzg);
-    const int ni = g.nxg - 2;
+    const int np = (g.nxg - 1) / 2;  // pairs per row: ceil((nxg-2)/2)
     const int t = bx * 256 + threadIdx.x;
-    if (t >= ni * g.nym) return;
+    if (t >= np * g.nym) return;
     const int j = 2 + (int)c.div_ni.div((uint32_t)t);
-    const int i = 2 + t - (j - 2) * ni;
+    const int i = 2 + 2 * (t - (j - 2) * np);  // this lane's points: i, i+1
     const int k0 = 2 + bz * kCorrectK;
     const int k1 = min(k0 + kCorrectK - 1, g.nzg - 1);
     const long pr_k = (long)(g.nxg - 1) * g.nym;
     const long pu_k = (long)g.nx * g.nyg, pv_k = (long)g.nxg * g.ny, pw_k = (long)g.nxg * g.nyg;
-    const bool do_u = i <= g.nx - 1, do_v = j <= g.ny - 1;
+    // per point e = 0, 1 (i+e): U on i+e <= nx-1, V and W on i+e <= nxg-1
+    const bool u0 = i <= g.nx - 1, u1 = i + 1 <= g.nx - 1;
+    const bool in1 = i + 1 <= g.nxg - 1;
+    const bool do_v = j <= g.ny - 1;
     // inflow/outflow x: the outlet face U(nx) takes the face value of phi,
     // zero or phib (projection.f90:578-589)
-    const bool face_out = g.xbc == 1 && i == g.nx;
+    const bool f0 = g.xbc == 1 && i == g.nx, f1 = g.xbc == 1 && i + 1 == g.nx;
     const double idyg = do_v ? inv_dyg[j - 2] : 0.0;
     const long r = ridx(g, i, j, k0);
     const long iu = (i - 1) + (long)(j - 1) * g.nx + (long)(k0 - 1) * pu_k;
@@ -92,40 +96,60 @@ __device__ __forceinline__ void correct_body(const CPar& c, int bx, int bz) {
     const long iw = (i - 1) + (long)(j - 1) * g.nxg + (long)(k0 - 1) * pw_k;
     const int nk = k1 - k0 + 1;  // planes of this block, 1..kCorrectK
     // Every load of the block's planes is issued before any store (the
-    // arrays are distinct): phi on planes k0..k1+1 (rhs_p holds the planes up
-    // to nzg), phi(i+1), phi(j+1), u, v, w.  Plane indices are clamped to the
-    // block's last plane and phi(j+1) to row j on the last row, so every
-    // address is in range; values of unused planes are never stored.
-    double pc[kCorrectK + 1], px[kCorrectK], py[kCorrectK], uo[kCorrectK], vo[kCorrectK], wo[kCorrectK];
+    // arrays are distinct): phi(i..i+1) on planes k0..k1+1 (rhs_p holds the
+    // planes up to nzg and the points up to nxg, so i+1 <= nxg is in its
+    // row), phi(i+2) (clamped to nxg), phi(j+1) (row j on the last row), u,
+    // v, w.  A pair (i, i+1) of U, V or W reads one point past the row when
+    // i+1 is beyond it: the next row's first element, which exists because
+    // j and k stay below the arrays' last row and plane (U: j <= nyg-1 of
+    // 1..nyg, k <= nzg-1 of 1..nzg; V: k <= nzg-1, its last row j = ny only
+    // reached with do_v false and i+1 <= nxg; W: j <= nyg-1).  Plane
+    // indices are clamped to the block's last plane, so every address is in
+    // range; values of unused planes and points are never stored.
+    dbl2u pc[kCorrectK + 1], py[kCorrectK], uo[kCorrectK], vo[kCorrectK], wo[kCorrectK];
+    double px2[kCorrectK];
+    const int i2 = min(i + 2, g.nxg) - i;  // offset of phi(i+2) in the row
 #pragma unroll
-    for (int q = 0; q <= kCorrectK; ++q) pc[q] = rhs[r + min(q, nk) * pr_k];
+    for (int q = 0; q <= kCorrectK; ++q) pc[q] = ld2(rhs + r + min(q, nk) * pr_k);
     const long oy = do_v ? (long)(g.nxg - 1) : 0;
 #pragma unroll
     for (int q = 0; q < kCorrectK; ++q) {
         const int qq = min(q, nk - 1);
-        px[q] = rhs[r + qq * pr_k + 1];
-        py[q] = rhs[r + qq * pr_k + oy];
-        uo[q] = U[iu + qq * pu_k];
-        vo[q] = V[iv + qq * pv_k];
-        wo[q] = W[iw + qq * pw_k];
+        px2[q] = rhs[r + qq * pr_k + i2];
+        py[q] = ld2(rhs + r + qq * pr_k + oy);
+        uo[q] = ld2(U + iu + qq * pu_k);
+        vo[q] = ld2(V + iv + qq * pv_k);
+        wo[q] = ld2(W + iw + qq * pw_k);
     }
+    const auto phib = gptr(c.phib);  // null without the backflow term
 #pragma unroll
     for (int q = 0; q < kCorrectK; ++q) {
         if (q >= nk) break;
         const int k = k0 + q;
-        // phi(k+1) is used for W only, for k <= nz-1
-        if (do_u) {
-            U[iu + q * pu_k] = uo[q] - (px[q] - pc[q]) * g.inv_dx;
-        } else if (face_out) {
-            if (c.phib) {
-                const double fb = gptr(c.phib)[(j - 2) + (long)(k - 2) * g.nym];
-                U[iu + q * pu_k] = uo[q] + 2.0 * (pc[q] - fb) * g.inv_dx;
-            } else {
-                U[iu + q * pu_k] = uo[q] + 2.0 * pc[q] * g.inv_dx;
-            }
+        // U: interior points, or the outlet face
+        double ua = 0.0, ub = 0.0;
+        if (u0) {
+            ua = uo[q].x - (pc[q].y - pc[q].x) * g.inv_dx;
+        } else if (f0) {
+            ua = phib ? uo[q].x + 2.0 * (pc[q].x - phib[(j - 2) + (long)(k - 2) * g.nym]) * g.inv_dx
+                      : uo[q].x + 2.0 * pc[q].x * g.inv_dx;
         }
-        if (do_v) V[iv + q * pv_k] = vo[q] - (py[q] - pc[q]) * idyg;
-        if (k <= g.nz - 1) W[iw + q * pw_k] = wo[q] - (pc[q + 1] - pc[q]) * inv_dzg[k - 2];
+        if (u1) {
+            ub = uo[q].y - (px2[q] - pc[q].y) * g.inv_dx;
+        } else if (f1) {
+            ub = phib ? uo[q].y + 2.0 * (pc[q].y - phib[(j - 2) + (long)(k - 2) * g.nym]) * g.inv_dx
+                      : uo[q].y + 2.0 * pc[q].y * g.inv_dx;
+        }
+        st2(U + iu + q * pu_k, ua, ub, u0 || f0, u1 || f1);
+        if (do_v)
+            st2(V + iv + q * pv_k, vo[q].x - (py[q].x - pc[q].x) * idyg, vo[q].y - (py[q].y - pc[q].y) * idyg, true,
+                in1);
+        // phi(k+1) is used for W only, for k <= nz-1
+        if (k <= g.nz - 1) {
+            const double dz = inv_dzg[k - 2];
+            st2(W + iw + q * pw_k, wo[q].x - (pc[q + 1].x - pc[q].x) * dz, wo[q].y - (pc[q + 1].y - pc[q].y) * dz,
+                true, in1);
+        }
     }
 }
 
@@ -161,8 +185,8 @@ CPar make_cpar(const dfp_plan* p, double* U, double* V, double* W, const double*
     c.U = U; c.V = V; c.W = W; c.rhs = rhs;
     c.phib = nullptr;
     c.g = p->geo;
-    c.div_ni = FastDiv(p->geo.nxg - 2);
-    c.nbx = ceil_div((long)(p->geo.nxg - 2) * p->geo.nym, 256);
+    c.div_ni = FastDiv((p->geo.nxg - 1) / 2);  // pairs of x points per row
+    c.nbx = ceil_div((long)((p->geo.nxg - 1) / 2) * p->geo.nym, 256);
     return c;
 }
 

@@ -101,7 +101,6 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
     C* twl = smem + nl * ld;        // FFT stage twiddles
     C* rtl = twl + ntw;             // R2C unpack twiddles, M+1
     C* tzl = rtl + (M + 1);         // first z stage twiddles, nz2*(R-1)
-    Rl* Ar = reinterpret_cast<Rl*>(A);
     // the three tables are contiguous in the plan's twiddle array
     for (int e = threadIdx.x; e < ntw + (M + 1) + nz2 * (R - 1); e += kXThreads) twl[e] = gld<C>(gptr(q.tw) + e);
 
@@ -118,65 +117,73 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
         mean0 = mean0 / ((double)nxp * (double)g.nzp);
     }
     double rsum = 0.0;
-    // four elements per thread per batch; every load is issued before any is used
-    for (int base = 0; base < total; base += 4 * kXThreads) {
-        double v[4];
-        int dst[4];
-        long cell[4];
+    // two pairs of neighbouring points (i, i+1), i even, per thread per batch
+    // (16-byte field loads); every load is issued before any is used
+    const int total2 = total >> 1;  // nxp is even
+    for (int base = 0; base < total2; base += 2 * kXThreads) {
+        double v[2][2];
+        int dst[2];
+        long cell[2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
             const int e0 = base + (int)threadIdx.x + u * kXThreads;
-            const int e = min(e0, total - 1);
+            const int e = 2 * min(e0, total2 - 1);  // first point of the pair
             const int l = (int)q.div_nxp.div((uint32_t)e);
             const int i = e - l * nxp;
             const int t = (int)q.div_t.div((uint32_t)l);
             const int k = k0 + (l - t * nt) + t * nz2;
-            dst[u] = e0 < total ? l * 2 * ld + i : -1;
+            dst[u] = e0 < total2 ? l * ld + (i >> 1) : -1;  // complex slot of the pair
             cell[u] = i + j * pr_j + k * pr_k;
             if (MODE == 1) {
-                v[u] = arhs[cell[u]];
+                const dbl2u r2 = ld2(arhs + cell[u]);
+                v[u][0] = r2.x;
+                v[u][1] = r2.y;
             } else if (MODE == 2) {
                 // no contraction, so the single and batched kernels form the same value
 #pragma clang fp contract(off)
                 // rhs holds phi; index nxp (x) and nzp (z) hold the periodic duplicates of 0
-                const auto ph = arhs + cell[u];
-                const double p0 = ph[0];
-                const double pxm = ph[i == 0 ? nxp - 1 : -1];
-                const double pxp = ph[1];
-                const double pzm = ph[k == 0 ? (long)(g.nzp - 1) * pr_k : -pr_k];
-                const double pzp = ph[pr_k];
-                const double pym = j > 0 ? ph[-pr_j] : 0.0;
-                const double pyp = j < g.nym - 1 ? ph[pr_j] : 0.0;
-                const double lo = j > 0 ? odl[j - 1] : 0.0;
-                const double up = j < g.nym - 1 ? odu[j] : 0.0;
-                double lap = (pxp - 2.0 * p0 + pxm) * a.idx2 + (lo * pym + odd[j] * p0 + up * pyp) +
-                             (pzp - 2.0 * p0 + pzm) * a.idz2;
-                if (j == 0) lap += 0.5 * odd[0] * mean0;
-                v[u] = ab[cell[u]] - lap;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const int ih = i + h;
+                    const auto ph = arhs + cell[u] + h;
+                    const double p0 = ph[0];
+                    const double pxm = ph[ih == 0 ? nxp - 1 : -1];
+                    const double pxp = ph[1];
+                    const double pzm = ph[k == 0 ? (long)(g.nzp - 1) * pr_k : -pr_k];
+                    const double pzp = ph[pr_k];
+                    const double pym = j > 0 ? ph[-pr_j] : 0.0;
+                    const double pyp = j < g.nym - 1 ? ph[pr_j] : 0.0;
+                    const double lo = j > 0 ? odl[j - 1] : 0.0;
+                    const double up = j < g.nym - 1 ? odu[j] : 0.0;
+                    double lap = (pxp - 2.0 * p0 + pxm) * a.idx2 + (lo * pym + odd[j] * p0 + up * pyp) +
+                                 (pzp - 2.0 * p0 + pzm) * a.idz2;
+                    if (j == 0) lap += 0.5 * odd[0] * mean0;
+                    v[u][h] = ab[cell[u] + h] - lap;
+                }
             } else {
                 const long iu = (i + 1) + (j + 1) * pu_j + (k + 1) * pu_k;  // U(i+2,j+2,k+2)
                 const long iv = (i + 1) + (j + 1) * pv_j + (k + 1) * pv_k;  // V(i+2,j+2,k+2)
                 const long iw = (i + 1) + (j + 1) * pw_j + (k + 1) * pw_k;  // W(i+2,j+2,k+2)
-                const double u1 = aU[iu], u0 = aU[iu - 1];
-                const double v1 = aV[iv], v0 = aV[iv - pv_j];
-                const double w1 = aW[iw], w0 = aW[iw - pw_k];
+                const dbl2u ua = ld2(aU + iu - 1);  // U(i+1), U(i+2)
+                const double ub = aU[iu + 1];       // U(i+3)
+                const dbl2u v1 = ld2(aV + iv), v0 = ld2(aV + iv - pv_j);
+                const dbl2u w1 = ld2(aW + iw), w0 = ld2(aW + iw - pw_k);
                 const double idz = inv_dzf[k];
                 {
 #pragma clang fp contract(off)
                     // projection.f90:55-57
-                    const double du = (u1 - u0) * g.inv_dx;
-                    const double dv = (v1 - v0) * inv_dy;
-                    const double dw = (w1 - w0) * idz;
-                    v[u] = du + dv + dw;
+                    v[u][0] = (ua.y - ua.x) * g.inv_dx + (v1.x - v0.x) * inv_dy + (w1.x - w0.x) * idz;
+                    v[u][1] = (ub - ua.y) * g.inv_dx + (v1.y - v0.y) * inv_dy + (w1.y - w0.y) * idz;
                 }
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
             if (dst[u] >= 0) {
-                Ar[dst[u]] = (Rl)v[u];
-                if (MODE != 2 && abo) abo[cell[u]] = v[u];
-                rsum += v[u];
+                A[dst[u]] = CplxT<C>::make((Rl)v[u][0], (Rl)v[u][1]);
+                if (MODE != 2 && abo) st2(abo + cell[u], v[u][0], v[u][1], true, true);
+                rsum += v[u][0];
+                rsum += v[u][1];
             }
         }
     }
