@@ -279,7 +279,6 @@ kb_div_rfft_x(const XPar<C>* __restrict__ cs, const int* __restrict__ off, const
 // row (j = 0), part0[bx] (FP64, fixed order).
 template <typename C, int ACCUM, int R>
 __device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsigned char* smem_raw) {
-    using Rl = typename CplxT<C>::real;
     __shared__ double wsum[kXThreads / 64];
     const Geo& g = q.g;
     const auto spec = gptr(q.spec);
@@ -365,31 +364,43 @@ __device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsig
     const double scale = q.scale, dtc = q.dtc;
     const int save_p = q.save_p;
     double psum = 0.0;
-    for (int e = threadIdx.x; e < nl * nxp; e += kXThreads) {
+    // two neighbouring points (i, i+1), i even, per lane: one complex slot of
+    // the line, 16-byte stores of phi and of P's main copy
+    for (int e2 = threadIdx.x; e2 < (nl * nxp) >> 1; e2 += kXThreads) {
+        const int e = 2 * e2;
         const int l = (int)q.div_nxp.div((uint32_t)e);
         const int i = e - l * nxp;
         const int t = (int)q.div_t.div((uint32_t)l);
         const int k = k0 + (l - t * nt) + t * nz2;
-        double phi = (double)reinterpret_cast<const Rl*>(A + l * ld)[i] * scale;
-        if (pbar) phi += pbar[j];
-        if (ACCUM) phi += rr[i + k * pr_k];
-        psum += phi;
+        const C z = A[l * ld + (i >> 1)];
+        double phi0 = (double)z.x * scale, phi1 = (double)z.y * scale;
+        if (pbar) {
+            phi0 += pbar[j];
+            phi1 += pbar[j];
+        }
+        if (ACCUM) {
+            const dbl2u o = ld2(rr + i + k * pr_k);
+            phi0 += o.x;
+            phi1 += o.y;
+        }
+        psum += phi0;
+        psum += phi1;
         // rhs_p(i+2, j+2, k+2) and the duplicates (i = nxp -> nxg-1, k = nzp -> nzp+2;
         // z walls have no z duplicate, projection.f90:472-497)
-        rr[i + k * pr_k] = phi;
-        if (i == 0) rr[nxp + k * pr_k] = phi;
+        st2(rr + i + k * pr_k, phi0, phi1, true, true);
+        if (i == 0) rr[nxp + k * pr_k] = phi0;
         if (k == 0 && g.zbc == 0) {
-            rr[i + (long)g.nzp * pr_k] = phi;
-            if (i == 0) rr[nxp + (long)g.nzp * pr_k] = phi;
+            st2(rr + i + (long)g.nzp * pr_k, phi0, phi1, true, true);
+            if (i == 0) rr[nxp + (long)g.nzp * pr_k] = phi0;
         }
         if (save_p) {
-            // every copy of this cell in P (reference 1-based indices; 0 = none):
-            // x: i+2, 1 if i = nxp-1, nxg-1 if i = 0, nxg if i = 1
+            // every copy of these cells in P (reference 1-based indices; 0 = none):
+            // x: i+2, 1 if i = nxp-1, nxg-1 if i = 0, nxg if i = 1 (i even: the
+            //    first two for the second point, the third for the first)
             // y: j+2, 1 if j = 0, nyg if j = nym-1
             // z: k+2, 1 if k = nzp-1, nzg-1 if k = 0, nzg if k = 1 (mod nzp);
             //    z walls (Neumann, projection.f90:423-436): 1 if k = 0, nzg if k = nzp-1
-            const double pv = phi / dtc;
-            const int x1 = i == nxp - 1 ? 1 : 0, x2 = i == 0 ? g.nxg - 1 : 0, x3 = i == 1 ? g.nxg : 0;
+            const double pv0 = phi0 / dtc, pv1 = phi1 / dtc;
             const int y1 = j == 0 ? 1 : 0, y2 = j == g.nym - 1 ? g.nyg : 0;
             const bool zw = g.zbc == 1;
             const int z1 = zw ? (k == 0 ? 1 : 0) : (k == g.nzp - 1 ? 1 : 0);
@@ -403,12 +414,11 @@ __device__ __forceinline__ void xinv_body(const XPar<C>& q, int bx, int j, unsig
                 for (int b = 0; b < 3; ++b) {
                     const int yy = b == 0 ? j + 2 : (b == 1 ? y1 : y2);
                     if (yy == 0) continue;
-#pragma unroll
-                    for (int a = 0; a < 4; ++a) {
-                        const int xx = a == 0 ? i + 2 : (a == 1 ? x1 : (a == 2 ? x2 : x3));
-                        if (xx == 0) continue;
-                        P[(xx - 1) + (long)(yy - 1) * pp_j + (long)(zz - 1) * pp_k] = pv;
-                    }
+                    const auto Pr = P + (long)(yy - 1) * pp_j + (long)(zz - 1) * pp_k;
+                    st2(Pr + (i + 1), pv0, pv1, true, true);  // x = i+2, i+3
+                    if (i == 0) Pr[g.nxg - 2] = pv0;          // x = nxg-1
+                    if (i + 1 == nxp - 1) Pr[0] = pv1;        // x = 1
+                    if (i + 1 == 1) Pr[g.nxg - 1] = pv1;      // x = nxg
                 }
             }
         }
