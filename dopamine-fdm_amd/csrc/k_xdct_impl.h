@@ -62,45 +62,55 @@ __device__ __forceinline__ void xfwd_dct_body(const XPar<double2>& q, int bx, in
     const long pr_j = (long)(g.nxg - 1), pr_k = (long)(g.nxg - 1) * g.nym;
     const int total = nl * nxp;
     const double inv_dy = inv_dyf[j];
-    for (int base = 0; base < total; base += 4 * kXThreads) {
-        double v[4];
-        int dst[4];
+    // two pairs of neighbouring points (i, i+1), i even, per thread per batch
+    // (16-byte field loads); every load is issued before any is used
+    const int total2 = total >> 1;  // nxp is even
+    for (int base = 0; base < total2; base += 2 * kXThreads) {
+        double v[2][2];
+        int dst[2], di[2];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < 2; ++u) {
             const int e0 = base + (int)threadIdx.x + u * kXThreads;
-            const int e = min(e0, total - 1);
+            const int e = 2 * min(e0, total2 - 1);  // first point of the pair
             const int l = (int)q.div_nxp.div((uint32_t)e);
             const int i = e - l * nxp;
             const int t = (int)q.div_t.div((uint32_t)l);
             const int k = k0 + (l - t * nt) + t * nz2;
-            dst[u] = e0 < total ? l * nxp + dct_slot(i, nxp) : -1;
+            dst[u] = e0 < total2 ? l * nxp : -1;  // line l; the points go to slots dct_slot(i), dct_slot(i+1)
+            di[u] = i;
             if (MODE == 1) {
-                v[u] = arhs[i + j * pr_j + k * pr_k];
+                const dbl2u r2 = ld2(arhs + i + j * pr_j + k * pr_k);
+                v[u][0] = r2.x;
+                v[u][1] = r2.y;
             } else {
                 const long iu = (i + 1) + (j + 1) * pu_j + (k + 1) * pu_k;  // U(i+2,j+2,k+2)
                 const long iv = (i + 1) + (j + 1) * pv_j + (k + 1) * pv_k;  // V(i+2,j+2,k+2)
                 const long iw = (i + 1) + (j + 1) * pw_j + (k + 1) * pw_k;  // W(i+2,j+2,k+2)
-                const double u1 = aU[iu], u0 = aU[iu - 1];
-                const double v1 = aV[iv], v0 = aV[iv - pv_j];
-                const double w1 = aW[iw], w0 = aW[iw - pw_k];
+                const dbl2u ua = ld2(aU + iu - 1);  // U(i+1), U(i+2)
+                const double ub = aU[iu + 1];       // U(i+3)
+                const dbl2u v1 = ld2(aV + iv), v0 = ld2(aV + iv - pv_j);
+                const dbl2u w1 = ld2(aW + iw), w0 = ld2(aW + iw - pw_k);
                 const double idz = inv_dzf[k];
-                // the outlet face value of the backflow term (projection.f90:91)
-                const double fb = (a.phib && i == nxp - 1) ? aphib[j + (long)k * g.nym] : 0.0;
+                // the outlet face value of the backflow term (projection.f90:91), at i+1 = nxp-1
+                const bool last = a.phib && i + 1 == nxp - 1;
+                const double fb = last ? aphib[j + (long)k * g.nym] : 0.0;
                 {
 #pragma clang fp contract(off)
                     // projection.f90:55-57
-                    const double du = (u1 - u0) * g.inv_dx;
-                    const double dv = (v1 - v0) * inv_dy;
-                    const double dw = (w1 - w0) * idz;
-                    double d = du + dv + dw;
-                    if (a.phib && i == nxp - 1) d = d - 2.0 * fb * a.idx2;
-                    v[u] = d;
+                    v[u][0] = (ua.y - ua.x) * g.inv_dx + (v1.x - v0.x) * inv_dy + (w1.x - w0.x) * idz;
+                    double d = (ub - ua.y) * g.inv_dx + (v1.y - v0.y) * inv_dy + (w1.y - w0.y) * idz;
+                    if (last) d = d - 2.0 * fb * a.idx2;
+                    v[u][1] = d;
                 }
             }
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u)
-            if (dst[u] >= 0) Ar[dst[u]] = v[u];
+        for (int u = 0; u < 2; ++u) {
+            if (dst[u] >= 0) {
+                Ar[dst[u] + dct_slot(di[u], nxp)] = v[u][0];
+                Ar[dst[u] + dct_slot(di[u] + 1, nxp)] = v[u][1];
+            }
+        }
     }
     __syncthreads();
     for (int e = threadIdx.x; e < nl * M; e += kXThreads) {
@@ -244,22 +254,25 @@ __device__ __forceinline__ void xinv_dct_body(const XPar<double2>& q, int bx, in
     const auto P = gptr(q.P);
     const double scale = q.scale, dtc = q.dtc;
     const int save_p = q.save_p;
-    for (int e = threadIdx.x; e < nl * nxp; e += kXThreads) {
+    // two neighbouring points (i, i+1), i even, per lane: 16-byte LDS reads and
+    // stores of phi and of P's main copy
+    for (int e2 = threadIdx.x; e2 < (nl * nxp) >> 1; e2 += kXThreads) {
+        const int e = 2 * e2;
         const int l = (int)q.div_nxp.div((uint32_t)e);
         const int i = e - l * nxp;
         const int t = (int)q.div_t.div((uint32_t)l);
         const int k = k0 + (l - t * nt) + t * nz2;
-        const double phi = Ar[l * nxp + i] * scale;
+        const C z = A[(l * nxp + i) >> 1];
+        const double phi0 = z.x * scale, phi1 = z.y * scale;
         // rhs_p(i+2, j+2, k+2) and the z duplicate (k = nzp -> nzp+2)
-        rr[i + k * pr_k] = phi;
-        if (k == 0) rr[i + (long)g.nzp * pr_k] = phi;
+        st2(rr + i + k * pr_k, phi0, phi1, true, true);
+        if (k == 0) st2(rr + i + (long)g.nzp * pr_k, phi0, phi1, true, true);
         if (save_p) {
-            // every copy of this cell in P (reference 1-based indices; 0 = none):
-            // x: i+2, 1 if i = 0, nxg if i = nxp-1
+            // every copy of these cells in P (reference 1-based indices; 0 = none):
+            // x: i+2, 1 if i = 0 (the first point), nxg if i = nxp-1 (the second)
             // y: j+2, 1 if j = 0, nyg if j = nym-1
             // z: k+2, 1 if k = nzp-1, nzg-1 if k = 0, nzg if k = 1 (mod nzp)
-            const double pv = phi / dtc;
-            const int x1 = i == 0 ? 1 : 0, x2 = i == nxp - 1 ? g.nxg : 0;
+            const double pv0 = phi0 / dtc, pv1 = phi1 / dtc;
             const int y1 = j == 0 ? 1 : 0, y2 = j == g.nym - 1 ? g.nyg : 0;
             const int z1 = k == g.nzp - 1 ? 1 : 0, z2 = k == 0 ? g.nzg - 1 : 0, z3 = k == 1 % g.nzp ? g.nzg : 0;
 #pragma unroll
@@ -270,12 +283,10 @@ __device__ __forceinline__ void xinv_dct_body(const XPar<double2>& q, int bx, in
                 for (int b = 0; b < 3; ++b) {
                     const int yy = b == 0 ? j + 2 : (b == 1 ? y1 : y2);
                     if (yy == 0) continue;
-#pragma unroll
-                    for (int a = 0; a < 3; ++a) {
-                        const int xx = a == 0 ? i + 2 : (a == 1 ? x1 : x2);
-                        if (xx == 0) continue;
-                        P[(xx - 1) + (long)(yy - 1) * pp_j + (long)(zz - 1) * pp_k] = pv;
-                    }
+                    const auto Pr = P + (long)(yy - 1) * pp_j + (long)(zz - 1) * pp_k;
+                    st2(Pr + (i + 1), pv0, pv1, true, true);  // x = i+2, i+3
+                    if (i == 0) Pr[0] = pv0;                  // x = 1
+                    if (i + 1 == nxp - 1) Pr[g.nxg - 1] = pv1;  // x = nxg
                 }
             }
         }
