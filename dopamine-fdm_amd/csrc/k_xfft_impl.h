@@ -142,23 +142,27 @@ __device__ __forceinline__ void xfwd_body(const XPar<C>& q, int bx, int j, unsig
                 // no contraction, so the single and batched kernels form the same value
 #pragma clang fp contract(off)
                 // rhs holds phi; index nxp (x) and nzp (z) hold the periodic duplicates of 0
+                const auto ph = arhs + cell[u];
+                const dbl2u p0 = ld2(ph);                           // phi(i), phi(i+1)
+                const double pxm = ph[i == 0 ? nxp - 1 : -1];       // phi(i-1)
+                const double pxp = ph[2];                           // phi(i+2)
+                const dbl2u pzm = ld2(ph + (k == 0 ? (long)(g.nzp - 1) * pr_k : -pr_k));
+                const dbl2u pzp = ld2(ph + pr_k);
+                const dbl2u pym = j > 0 ? ld2(ph - pr_j) : dbl2u{0.0, 0.0};
+                const dbl2u pyp = j < g.nym - 1 ? ld2(ph + pr_j) : dbl2u{0.0, 0.0};
+                const dbl2u bb = ld2(ab + cell[u]);
+                const double lo = j > 0 ? odl[j - 1] : 0.0;
+                const double up = j < g.nym - 1 ? odu[j] : 0.0;
 #pragma unroll
                 for (int h = 0; h < 2; ++h) {
-                    const int ih = i + h;
-                    const auto ph = arhs + cell[u] + h;
-                    const double p0 = ph[0];
-                    const double pxm = ph[ih == 0 ? nxp - 1 : -1];
-                    const double pxp = ph[1];
-                    const double pzm = ph[k == 0 ? (long)(g.nzp - 1) * pr_k : -pr_k];
-                    const double pzp = ph[pr_k];
-                    const double pym = j > 0 ? ph[-pr_j] : 0.0;
-                    const double pyp = j < g.nym - 1 ? ph[pr_j] : 0.0;
-                    const double lo = j > 0 ? odl[j - 1] : 0.0;
-                    const double up = j < g.nym - 1 ? odu[j] : 0.0;
-                    double lap = (pxp - 2.0 * p0 + pxm) * a.idx2 + (lo * pym + odd[j] * p0 + up * pyp) +
-                                 (pzp - 2.0 * p0 + pzm) * a.idz2;
+                    const double c0 = h ? p0.y : p0.x;
+                    const double xm = h ? p0.x : pxm;
+                    const double xp = h ? pxp : p0.y;
+                    double lap = (xp - 2.0 * c0 + xm) * a.idx2 +
+                                 (lo * (h ? pym.y : pym.x) + odd[j] * c0 + up * (h ? pyp.y : pyp.x)) +
+                                 ((h ? pzp.y : pzp.x) - 2.0 * c0 + (h ? pzm.y : pzm.x)) * a.idz2;
                     if (j == 0) lap += 0.5 * odd[0] * mean0;
-                    v[u][h] = ab[cell[u] + h] - lap;
+                    v[u][h] = (h ? bb.y : bb.x) - lap;
                 }
             } else {
                 const long iu = (i + 1) + (j + 1) * pu_j + (k + 1) * pu_k;  // U(i+2,j+2,k+2)
