@@ -284,6 +284,7 @@ void free_plan_memory(dfp_plan* p) {
     (void)hipFree(p->d_pbar);
     (void)hipFree(p->d_spec);
     (void)hipFree(p->d_lu);
+    (void)hipFree(p->d_lucol);
     (void)hipFree(p->d_spec2);
     (void)hipFree(p->d_duct);
     (void)hipFree(p->d_z00);
@@ -760,6 +761,14 @@ int dfp_plan_create_tuned(const dfp_grid* g, int precision, int device, const df
         const size_t nlu = (size_t)G.nxh * G.nym * (G.nzp / 2 + 1);
         if (hipMalloc(&p->d_lu, sizeof(double) * nlu) != hipSuccess) return bail(DFP_E_NOMEM);
         if (launch_lu_table(p) != hipSuccess || hipStreamSynchronize(p->stream) != hipSuccess) return bail(DFP_E_HIP);
+        // the table's column of each (u, m) mode of the z split
+        const int nz2c = G.nzp / p->zr;
+        std::vector<int> cols((size_t)G.nzp);
+        for (int u = 0; u < p->zr; ++u)
+            for (int m = 0; m < nz2c; ++m) cols[(size_t)u * nz2c + m] = lu_pos_mu(m, u, p->zr, nz2c);
+        if (hipMalloc(&p->d_lucol, sizeof(int) * cols.size()) != hipSuccess) return bail(DFP_E_NOMEM);
+        if (hipMemcpy(p->d_lucol, cols.data(), sizeof(int) * cols.size(), hipMemcpyHostToDevice) != hipSuccess)
+            return bail(DFP_E_HIP);
     }
     p->red_len = (int)((long)ceil_div((long)(G.nxg - 2) * G.nym, 256) * (G.nzg - 2));
     if (hipMalloc(&p->d_red, sizeof(double) * (p->red_len + 8)) != hipSuccess) return bail(DFP_E_NOMEM);

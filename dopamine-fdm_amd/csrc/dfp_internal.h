@@ -124,6 +124,7 @@ template <typename C>
 struct ZPar {
     C* spec;
     const double* lu;
+    const int* lucol;    // pivot column of sub-plane u, mode m: lucol[u * nzp/R + m]
     const double* dl;
     const double* du;
     FftDesc fz;          // z FFT of length nzp/R
@@ -229,7 +230,8 @@ struct dfp_plan {
     const float2* tw_z2_32 = nullptr;
     // device work arrays
     double2* d_spec = nullptr;   // (nxp/2+1) x nym x nzp complex, layout [kx][u][j][m], kz = zr*m + u
-    double* d_lu = nullptr;      // inverse pivots of the y solve, [kx][j][min(kz,nzp-kz)]
+    double* d_lu = nullptr;      // inverse pivots of the y solve, [kx][j][lu_pos(kz)]
+    int* d_lucol = nullptr;      // lu_pos of mode (u, m) of the z split: [u * nzp/R + m]
     double* d_red = nullptr;     // reduction partials
     int red_len = 0;
     // FP32 form: the FP64 right-hand side, partial sums and the (0,0) mode
@@ -403,6 +405,27 @@ __device__ __forceinline__ void st2(Glob<double>* p, double a, double b, bool ok
         if (ok0) p[0] = a;
         if (ok1) p[1] = b;
     }
+}
+
+// Column of the pivot table for z mode kz of a plan split into R z sub-planes
+// of nz2 = nzp/R modes (kz = R m + u).  The pivots depend on kz only through
+// min(kz, nzp - kz), and mode nzp - kz lies in sub-plane R - u (mod R) at
+// position nz2 - 1 - m (u > 0) or nz2 - m (u = 0).  Columns are laid out so
+// that each sub-plane's modes read one contiguous run: u = 0 first (m and
+// nz2 - m share a column), then per pair {u, R - u} the nz2 columns of u in
+// m order (R - u reads them in reverse), and u = R/2 pairs m with
+// nz2 - 1 - m.  A bijection of the nzp/2 + 1 folded modes onto
+// [0, nzp/2 + 1); R = 1 gives column min(kz, nzp - kz).
+__host__ __device__ __forceinline__ int lu_pos_mu(int m, int u, int R, int nz2) {
+    if (u == 0) return m > 0 ? min(m, nz2 - m) : 0;
+    const int c = min(u, R - u);
+    const int base = nz2 / 2 + 1 + (c - 1) * nz2;
+    if (2 * u == R) return base + min(m, nz2 - 1 - m);
+    return base + (u < R - u ? m : nz2 - 1 - m);
+}
+__host__ __device__ __forceinline__ int lu_pos(int kz, int R, int nz2) {
+    const int m = kz / R;
+    return lu_pos_mu(m, kz - m * R, R, nz2);
 }
 
 // MI355X dispatches workgroup w of a launch to XCD w % 8, and each XCD has its

@@ -89,8 +89,7 @@ __device__ __forceinline__ void zfused_body(const ZPar<C>& zp, int b, unsigned c
     const int c = t / n;
     const bool act = c < P && t < n * P;
     const int s = min(c * L, nym), e = min(s + L, nym);
-    const int kz = R * m + u;
-    const auto lcol = gptr(zp.lu) + (long)kx * nym * nzs + min(kz, nzp - kz);
+    const auto lcol = gptr(zp.lu) + (long)kx * nym * nzs + lu_pos_mu(m, u, R, n);
     // phase 1, elimination: local yh(j) from yh(s-1) = 0, gain g(j) = prod(-mu)
     if (act && s < e) {
         double yx = 0.0, yy = 0.0, gn = 1.0;

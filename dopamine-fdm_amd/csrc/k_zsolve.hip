@@ -8,8 +8,9 @@
 //   D(2) *= 3/2 for the (0,0) mode (:292).
 // Its LU factors depend only on kxx(kx) + kzz(kz), and kzz(kz) = kzz(nzp-kz)
 // exactly, so the plan holds the inverse pivots 1/D'(j) of the elimination
-// once per (kx, min(kz, nzp-kz)):  lu[(kx*nym + j)*nzs + kzs],
-// nzs = nzp/2 + 1.  The elimination follows zgtsv's no-interchange branch
+// once per (kx, min(kz, nzp-kz)):  lu[(kx*nym + j)*nzs + lu_pos(kz, R, nz2)],
+// nzs = nzp/2 + 1, the columns ordered so that each (kx, u) sub-plane reads
+// one contiguous run per row (lu_pos in dfp_internal.h).  The elimination follows zgtsv's no-interchange branch
 // (mult = DL/D', D' <- D - mult*DU); LAPACK interchanges rows only in a few
 // low modes, which moves the solution at the 1e-15 level.
 //
@@ -35,7 +36,8 @@ namespace dfp {
 // pin: the periodic x has the singular (0,0) mode, pinned by D(2) *= 3/2
 // (projection.f90:292); the DCT-IV modes of the inflow/outflow x are all
 // nonzero and nothing is pinned
-__global__ void __launch_bounds__(256) k_lu_table(double* __restrict__ lu, Ops o, int nxh, int nym, int nzp, int pin) {
+__global__ void __launch_bounds__(256) k_lu_table(double* __restrict__ lu, Ops o, int nxh, int nym, int nzp, int R,
+                                                  int pin) {
     const int nzs = nzp / 2 + 1;
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= (long)nxh * nzs) return;
@@ -44,7 +46,7 @@ __global__ void __launch_bounds__(256) k_lu_table(double* __restrict__ lu, Ops o
     const double lam = o.kxx[kx] + o.kzz[kzs];
     double dp = o.dd[0] + lam;
     if (pin && kx == 0 && kzs == 0) dp = 3.0 / 2.0 * dp;
-    double* col = lu + (long)kx * nym * nzs + kzs;
+    double* col = lu + (long)kx * nym * nzs + lu_pos(kzs, R, nzp / R);
     col[0] = 1.0 / dp;
     for (int j = 1; j < nym; ++j) {
         const double mult = o.dl[j - 1] / dp;
@@ -57,7 +59,7 @@ hipError_t launch_lu_table(dfp_plan* p) {
     const Geo& g = p->geo;
     const long n = (long)g.nxh * (g.nzp / 2 + 1);
     hipLaunchKernelGGL(k_lu_table, dim3(ceil_div(n, 256)), dim3(256), 0, p->stream, p->d_lu, p->ops, g.nxh, g.nym,
-                       g.nzp, g.xbc == 0 ? 1 : 0);
+                       g.nzp, p->zr, g.xbc == 0 ? 1 : 0);
     return hipGetLastError();
 }
 
@@ -79,8 +81,7 @@ constexpr int kZsAhead = 2;  // rows read ahead in the elimination sweeps
         pr##K = gld<C>(src_ + e);                                                 \
         const int r = (int)div_n.div((uint32_t)e);                                \
         const int mm = e - r * n;                                                 \
-        const int kz = R * mm + u;                                                \
-        pl##K = lsrc_[(long)r * nzs + min(kz, nzp - kz)];                         \
+        pl##K = lsrc_[(long)r * nzs + lucol[u * n + mm]];                         \
     }
 #define ZS_PREFETCH(J0, GQ, COEF, COFF)                                           \
     do {                                                                          \
@@ -216,6 +217,7 @@ __device__ __forceinline__ void zfwd_body(const ZPar<C>& zp, int b, unsigned cha
     double* dus = lus + G * n;
     const auto plane = gptr(zp.spec) + (long)b * nym * n;
     const auto lup = gptr(zp.lu) + (long)kx * nym * nzs;
+    const auto lucol = gptr(zp.lucol);
     for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = gld<C>(gptr(zp.tw) + e);
     ZS_REGS;
     ZS_PREFETCH(0, min(G, nym), dl, -1);
@@ -254,6 +256,7 @@ __device__ __forceinline__ void zbwd_body(const ZPar<C>& zp, int b, unsigned cha
     double* dus = lus + G * n;
     const auto plane = gptr(zp.spec) + (long)b * nym * n;
     const auto lup = gptr(zp.lu) + (long)kx * nym * nzs;
+    const auto lucol = gptr(zp.lucol);
     for (int e = threadIdx.x; e < ntw; e += kZsThreads) twl[e] = gld<C>(gptr(zp.tw) + e);
     // groups from the top: [j0, j0+gr), j0 = max(0, nym - (gi+1)*G)
     const int ngroups = (nym + G - 1) / G;
@@ -327,7 +330,10 @@ __device__ __forceinline__ void zpoint(int k, int gq, int& r, int& p, int& m) {
     p = v - r * P;
 }
 
-template <typename C, int N1, int N2>
+// COLS: form the pivot column of a one-plane row that divides the threads in
+// registers (the single-case kernels) instead of reading the plan's column
+// table (the batched kernels, where the extra registers cost a wave)
+template <typename C, int N1, int N2, bool COLS>
 __device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int blk, unsigned char* smem_raw) {
     constexpr int N = N1 * N2;
     constexpr int P = zplanes(N);
@@ -349,9 +355,15 @@ __device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int blk, unsigned 
     double pl[kZsPf];
     double pd;
     // offset of row 0 of the pivots of position mm of plane p
+    // one plane whose rows divide the threads: every item of a thread has the
+    // mode threadIdx.x % N, whose column is formed in registers (clamped items
+    // of a short last group read that column of their row, unused); else the
+    // plan's column table
     auto lu_col = [&](int p, int mm) {
-        const int bp = plane_of(p), kx = bp / R, kz = R * mm + (bp - kx * R);
-        return (long)kx * nym * nzs + min(kz, nzp - kz);
+        const int bp = plane_of(p), kx = bp / R;
+        if constexpr (COLS && P == 1 && kZsThreads % N == 0)  // every item has mode threadIdx.x % N
+            return (long)kx * nym * nzs + lu_pos_mu((int)threadIdx.x % N, bp - kx * R, R, N);
+        return (long)kx * nym * nzs + gptr(zp.lucol)[(bp - kx * R) * N + mm];
     };
     auto prefetch = [&](int j0, int gq) {
 #pragma unroll
@@ -424,7 +436,10 @@ __device__ __forceinline__ void zfwd2_body(const ZPar<C>& zp, int blk, unsigned 
     }
 }
 
-template <typename C, int N1, int N2>
+// COLS: form the pivot column of a one-plane row that divides the threads in
+// registers (the single-case kernels) instead of reading the plan's column
+// table (the batched kernels, where the extra registers cost a wave)
+template <typename C, int N1, int N2, bool COLS>
 __device__ __forceinline__ void zbwd2_body(const ZPar<C>& zp, int blk, unsigned char* smem_raw) {
     constexpr int N = N1 * N2;
     constexpr int P = zplanes(N);
@@ -445,9 +460,15 @@ __device__ __forceinline__ void zbwd2_body(const ZPar<C>& zp, int blk, unsigned 
     C pr[kZsPf];
     double pl[kZsPf];
     double pd;
+    // one plane whose rows divide the threads: every item of a thread has the
+    // mode threadIdx.x % N, whose column is formed in registers (clamped items
+    // of a short last group read that column of their row, unused); else the
+    // plan's column table
     auto lu_col = [&](int p, int mm) {
-        const int bp = plane_of(p), kx = bp / R, kz = R * mm + (bp - kx * R);
-        return (long)kx * nym * nzs + min(kz, nzp - kz);
+        const int bp = plane_of(p), kx = bp / R;
+        if constexpr (COLS && P == 1 && kZsThreads % N == 0)  // every item has mode threadIdx.x % N
+            return (long)kx * nym * nzs + lu_pos_mu((int)threadIdx.x % N, bp - kx * R, R, N);
+        return (long)kx * nym * nzs + gptr(zp.lucol)[(bp - kx * R) * N + mm];
     };
     // rows j0..j0+gq-1 of the block's planes
     auto prefetch = [&](int j0, int gq) {
@@ -551,9 +572,9 @@ __global__ void __launch_bounds__(kZsThreads) k_zsolve(ZPar<C> zp) {
         __syncthreads();
         zbwd_body<C>(zp, b, smem_raw);
     } else {
-        zfwd2_body<C, N1, N2>(zp, b, smem_raw);
+        zfwd2_body<C, N1, N2, true>(zp, b, smem_raw);
         __syncthreads();
-        zbwd2_body<C, N1, N2>(zp, b, smem_raw);
+        zbwd2_body<C, N1, N2, true>(zp, b, smem_raw);
     }
 }
 
@@ -584,9 +605,9 @@ __global__ void __launch_bounds__(kZsThreads) kb_zsolve(const ZPar<C>* __restric
             __syncthreads();
             zbwd_body<C>(cs[idx[c]], w - off[c], smem_raw);
         } else {
-            zfwd2_body<C, N1, N2>(cs[idx[c]], w - off[c], smem_raw);
+            zfwd2_body<C, N1, N2, false>(cs[idx[c]], w - off[c], smem_raw);
             __syncthreads();
-            zbwd2_body<C, N1, N2>(cs[idx[c]], w - off[c], smem_raw);
+            zbwd2_body<C, N1, N2, false>(cs[idx[c]], w - off[c], smem_raw);
         }
         __syncthreads();
     }
@@ -598,6 +619,7 @@ ZPar<C> make_zpar(const dfp_plan* p, int f32_zero00) {
     const Geo& g = p->geo;
     z.spec = reinterpret_cast<C*>(p->d_spec);
     z.lu = p->d_lu;
+    z.lucol = p->d_lucol;
     z.dl = p->ops.dl;
     z.du = p->ops.du;
     z.fz = p->fz;
